@@ -1,0 +1,433 @@
+// api.hip — the C ABI declared in include/rtx.h.
+//
+// Host-side validation mirrors the valid-usage rules the reference relies on (main.cpp:726-812, 848-945,
+// 1349-1355): vertex format R32G32B32_SFLOAT, uint32 indices, caller-owned storage at least as large as the
+// size query, 256-B aligned addresses.  Errors come back as rtx_result codes with a message in
+// rtx_last_error_string; nothing throws across the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "build.h"
+#include "rtx_internal.h"
+#include "trace.h"
+
+struct rtx_context_t {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+    rtx::DeviceState* state = nullptr;
+    uint2* gstack = nullptr;
+    uint64_t gstack_entries = 0;
+    int cu_count = 256;
+    rtx_trace_config cfg{256, 3, 12, 0};
+    uint32_t gdepth = 64;
+    bool stats = false;
+};
+
+namespace {
+
+constexpr uint32_t kMaxPrims = (1u << 30) - 1u;
+
+int fail(rtx_context ctx, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+int fail(rtx_context ctx, int code, const char* fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        ctx->last_error = buf;
+    }
+    return code;
+}
+
+int hip_fail(rtx_context ctx, hipError_t e, const char* where) {
+    return fail(ctx, RTX_ERROR_DEVICE_LOST, "%s: %s", where, hipGetErrorString(e));
+}
+
+bool aligned(const void* p, uint64_t a) { return ((uintptr_t)p % a) == 0; }
+
+int check_geometries(rtx_context ctx, const rtx_blas_geometry* g, uint32_t count, uint64_t* total) {
+    if (count > 0 && !g) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometries is NULL");
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const rtx_blas_geometry& x = g[i];
+        if (x.primitive_count == 0) continue;
+        if (!x.vertex_data_device || !x.index_data_device)
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: vertex or index data is NULL", i);
+        if (x.index_type != RTX_INDEX_TYPE_UINT32)
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: only RTX_INDEX_TYPE_UINT32 is supported", i);
+        if (x.vertex_stride < 12 || (x.vertex_stride & 3))
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: vertex_stride must be >= 12 and a multiple of 4",
+                        i);
+        if (x.primitive_offset & 3)
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: primitive_offset must be a multiple of 4", i);
+        if (x.transform_data_device && (x.transform_offset & 15))
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: transform_offset must be a multiple of 16", i);
+        n += x.primitive_count;
+    }
+    if (n > kMaxPrims) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "too many primitives (%llu)", (unsigned long long)n);
+    *total = n;
+    return RTX_SUCCESS;
+}
+
+int ensure_gstack(rtx_context ctx, uint32_t grid) {
+    uint64_t need = (uint64_t)grid * ctx->cfg.block_size * ctx->gdepth;
+    if (need <= ctx->gstack_entries) return RTX_SUCCESS;
+    if (ctx->gstack) (void)hipFree(ctx->gstack);
+    ctx->gstack = nullptr;
+    ctx->gstack_entries = 0;
+    hipError_t e = hipMalloc(&ctx->gstack, need * sizeof(uint2));
+    if (e != hipSuccess) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "stack spill area: %s", hipGetErrorString(e));
+    ctx->gstack_entries = need;
+    return RTX_SUCCESS;
+}
+
+rtx::TraceLaunch launch_shape(rtx_context ctx) {
+    rtx::TraceLaunch L;
+    L.block_size = ctx->cfg.block_size;
+    L.grid = (uint32_t)ctx->cu_count * ctx->cfg.blocks_per_cu;
+    L.lds_depth = ctx->cfg.lds_stack_depth;
+    L.gdepth = ctx->gdepth;
+    return L;
+}
+
+int check_tlas(rtx_context ctx, uint64_t tlas) {
+    if (tlas == 0 || (tlas & 255u)) return fail(ctx, RTX_ERROR_INVALID_AS, "TLAS address 0x%llx is not a TLAS",
+                                               (unsigned long long)tlas);
+    return RTX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtx_create_context(int device, void* hip_stream, rtx_context* out_ctx) {
+    if (!out_ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    *out_ctx = nullptr;
+    rtx_context ctx = new (std::nothrow) rtx_context_t();
+    if (!ctx) return RTX_ERROR_OUT_OF_HOST_MEMORY;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return RTX_ERROR_INITIALIZATION_FAILED;
+    }
+    ctx->device = device;
+    if (hip_stream) {
+        ctx->stream = (hipStream_t)hip_stream;
+    } else {
+        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete ctx;
+            return RTX_ERROR_INITIALIZATION_FAILED;
+        }
+        ctx->own_stream = true;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->cu_count = cus;
+    e = hipMalloc(&ctx->state, sizeof(rtx::DeviceState));
+    if (e == hipSuccess) e = hipMemset(ctx->state, 0, sizeof(rtx::DeviceState));
+    if (e != hipSuccess) {
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return RTX_ERROR_INITIALIZATION_FAILED;
+    }
+    *out_ctx = ctx;
+    return RTX_SUCCESS;
+}
+
+int rtx_destroy_context(rtx_context ctx) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->gstack) (void)hipFree(ctx->gstack);
+    if (ctx->state) (void)hipFree(ctx->state);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RTX_SUCCESS;
+}
+
+const char* rtx_last_error_string(rtx_context ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+void* rtx_get_stream(rtx_context ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int rtx_sync(rtx_context ctx) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_sync");
+    uint32_t flags = 0;
+    e = hipMemcpy(&flags, &ctx->state->error_flags, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_sync flags");
+    if (flags) {
+        uint32_t zero = 0;
+        (void)hipMemcpy(&ctx->state->error_flags, &zero, 4, hipMemcpyHostToDevice);
+        if (flags & rtx::kErrStackOverflow)
+            return fail(ctx, RTX_ERROR_STACK_OVERFLOW, "traversal stack overflow (depth > %u)",
+                        ctx->cfg.lds_stack_depth + ctx->gdepth);
+        if (flags & rtx::kErrSortTimeout) return fail(ctx, RTX_ERROR_DEVICE_LOST, "radix sort look-back timed out");
+        return fail(ctx, RTX_ERROR_INVALID_AS, "device reported an invalid acceleration structure");
+    }
+    return RTX_SUCCESS;
+}
+
+int rtx_get_blas_build_sizes(rtx_context ctx, const rtx_blas_geometry* geometries, uint32_t geometry_count,
+                             uint32_t build_flags, rtx_as_build_sizes* out_sizes) {
+    if (!ctx || !out_sizes) return RTX_ERROR_INVALID_ARGUMENT;
+    uint64_t n = 0;
+    int r = check_geometries(ctx, geometries, geometry_count, &n);
+    if (r) return r;
+    const bool k64 = (build_flags & RTX_BUILD_MORTON63_BIT) != 0;
+    out_sizes->acceleration_structure_size = rtx::blas_as_bytes((uint32_t)n);
+    out_sizes->build_scratch_size = rtx::blas_scratch_bytes((uint32_t)n, k64, geometry_count);
+    out_sizes->update_scratch_size = 0;
+    return RTX_SUCCESS;
+}
+
+int rtx_get_tlas_build_sizes(rtx_context ctx, uint32_t instance_count, uint32_t build_flags,
+                             rtx_as_build_sizes* out_sizes) {
+    if (!ctx || !out_sizes) return RTX_ERROR_INVALID_ARGUMENT;
+    if (instance_count > (1u << 24)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "instance_count above 2^24");
+    const bool k64 = (build_flags & RTX_BUILD_MORTON63_BIT) != 0;
+    out_sizes->acceleration_structure_size = rtx::tlas_as_bytes(instance_count);
+    out_sizes->build_scratch_size = rtx::tlas_scratch_bytes(instance_count, k64);
+    out_sizes->update_scratch_size = 0;
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uint32_t geometry_count,
+                       uint32_t build_flags, void* dst_device, uint64_t dst_size, void* scratch_device,
+                       uint64_t scratch_size, uint64_t* out_address) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    uint64_t n = 0;
+    int r = check_geometries(ctx, geometries, geometry_count, &n);
+    if (r) return r;
+    const bool k64 = (build_flags & RTX_BUILD_MORTON63_BIT) != 0;
+    if (!dst_device || !aligned(dst_device, 256) || !scratch_device || !aligned(scratch_device, 256))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "dst and scratch must be non-NULL and 256-B aligned");
+    if (dst_size < rtx::blas_as_bytes((uint32_t)n))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "dst_size %llu below the size query",
+                    (unsigned long long)dst_size);
+    if (scratch_size < rtx::blas_scratch_bytes((uint32_t)n, k64, geometry_count))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "scratch_size %llu below the size query",
+                    (unsigned long long)scratch_size);
+    std::vector<rtx::GeomDesc> gd(geometry_count ? geometry_count : 1);
+    uint32_t base = 0;
+    for (uint32_t i = 0; i < geometry_count; ++i) {
+        const rtx_blas_geometry& x = geometries[i];
+        rtx::GeomDesc& d = gd[i];
+        d.vtx = (const uint8_t*)x.vertex_data_device;
+        d.idx = (const uint32_t*)((const uint8_t*)x.index_data_device + x.primitive_offset);
+        d.xform = x.transform_data_device ? (const float*)((const uint8_t*)x.transform_data_device + x.transform_offset)
+                                          : nullptr;
+        d.stride = x.vertex_stride;
+        d.prim_count = x.primitive_count;
+        d.first_vertex = x.first_vertex;
+        d.prim_base = base;
+        d.max_vertex = x.max_vertex;
+        base += x.primitive_count;
+    }
+    hipError_t e = rtx::build_blas(gd.data(), geometry_count, (uint32_t)n, build_flags, dst_device, scratch_device,
+                                   &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_build_blas");
+    if (out_address) *out_address = (uint64_t)(uintptr_t)dst_device;
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_build_tlas(rtx_context ctx, const rtx_instance* instances_device, uint32_t instance_count,
+                       uint32_t build_flags, void* dst_device, uint64_t dst_size, void* scratch_device,
+                       uint64_t scratch_size, uint64_t* out_address) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    const bool k64 = (build_flags & RTX_BUILD_MORTON63_BIT) != 0;
+    if (instance_count > 0 && !instances_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "instances is NULL");
+    if (instance_count > (1u << 24)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "instance_count above 2^24");
+    if (!dst_device || !aligned(dst_device, 256) || !scratch_device || !aligned(scratch_device, 256))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "dst and scratch must be non-NULL and 256-B aligned");
+    if (dst_size < rtx::tlas_as_bytes(instance_count))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "dst_size below the size query");
+    if (scratch_size < rtx::tlas_scratch_bytes(instance_count, k64))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "scratch_size below the size query");
+    hipError_t e = rtx::build_tlas(instances_device, instance_count, build_flags, dst_device, scratch_device,
+                                   &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_build_tlas");
+    if (out_address) *out_address = (uint64_t)(uintptr_t)dst_device;
+    return RTX_SUCCESS;
+}
+
+int rtx_get_as_info(rtx_context ctx, uint64_t as_address, rtx_as_info* out_info) {
+    if (!ctx || !out_info) return RTX_ERROR_INVALID_ARGUMENT;
+    if (as_address == 0 || (as_address & 255u)) return fail(ctx, RTX_ERROR_INVALID_AS, "bad AS address");
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_as_info");
+    rtx::AsHeader h;
+    e = hipMemcpy(&h, (const void*)(uintptr_t)as_address, sizeof(h), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_as_info copy");
+    if (h.magic != rtx::kMagicBlas && h.magic != rtx::kMagicTlas)
+        return fail(ctx, RTX_ERROR_INVALID_AS, "no acceleration structure at 0x%llx", (unsigned long long)as_address);
+    out_info->type = h.type;
+    out_info->primitive_count = h.prim_count;
+    out_info->node_count = h.node_count;
+    out_info->slot_count = h.slot_count;
+    out_info->max_depth = h.type ? h.max_total_depth : h.max_depth;
+    out_info->geometry_count = h.geometry_count;
+    for (int a = 0; a < 3; ++a) {
+        out_info->bounds[a] = h.lo[a];
+        out_info->bounds[3 + a] = h.hi[a];
+    }
+    out_info->sah_cost = h.sah_cost;
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_trace_rays(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                       const rtx_trace_params* params, uint32_t width, uint32_t height, float* film_rgba32f_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    int r = check_tlas(ctx, tlas_address);
+    if (r) return r;
+    if (!sbt || !camera || !film_rgba32f_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt, camera and film must be non-NULL");
+    if (sbt->hit_record_count > 0 && !sbt->hit_records_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt hit records are NULL");
+    if ((uint64_t)width * height > 0xFFFFFFFFull)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "launch size above 2^32 rays");
+    rtx_trace_params p = params ? *params : rtx_trace_params{RTX_RAY_FLAG_OPAQUE, 0xFFu, 0.0f, 100.0f};
+    rtx::TraceRequest q{};
+    q.tlas = (const void*)(uintptr_t)tlas_address;
+    q.mode = 2;
+    q.n = width * height;
+    q.cull_mask = p.cull_mask & 0xFFu;
+    // raygen aspect terms (main.cpp:1038-1039) in the GLSL operation order, tan evaluated in FP64
+    const double rad = (double)camera->y_fov_degree * (3.14159265358979323846 / 180.0);
+    const float ay = (float)std::tan(rad * 0.5);
+    const float t = ay * (float)width;
+    q.ay = ay;
+    q.ax = t / (float)height;
+    for (int k = 0; k < 3; ++k) {
+        q.cam[k] = camera->position[k];
+        q.miss[k] = sbt->miss_color[k];
+    }
+    q.width = width;
+    q.height = height;
+    q.tmin = p.tmin;
+    q.tmax = p.tmax;
+    q.records = sbt->hit_records_device;
+    q.nrec = sbt->hit_record_count;
+    q.sbt_offset = sbt->sbt_record_offset;
+    q.sbt_stride = sbt->sbt_record_stride;
+    q.film = film_rgba32f_device;
+    q.state = ctx->state;
+    q.stats = ctx->stats;
+    rtx::TraceLaunch L = launch_shape(ctx);
+    if ((r = ensure_gstack(ctx, L.grid))) return r;
+    q.gstack = ctx->gstack;
+    hipError_t e = rtx::trace_rays_launch(q, L, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_trace_rays");
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device, uint32_t ray_count,
+                             uint32_t mode, uint32_t cull_mask, rtx_hit* hits_device, uint32_t* occluded_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    int r = check_tlas(ctx, tlas_address);
+    if (r) return r;
+    if (mode != RTX_QUERY_CLOSEST_HIT && mode != RTX_QUERY_ANY_HIT)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "mode must be RTX_QUERY_CLOSEST_HIT or RTX_QUERY_ANY_HIT");
+    if (ray_count > 0 && !rays_device) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "rays is NULL");
+    if (ray_count > 0 && mode == RTX_QUERY_CLOSEST_HIT && !hits_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "hits is NULL");
+    if (ray_count > 0 && mode == RTX_QUERY_ANY_HIT && !occluded_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "occluded is NULL");
+    if (!aligned(rays_device, 16)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "rays must be 16-B aligned");
+    rtx::TraceRequest q{};
+    q.tlas = (const void*)(uintptr_t)tlas_address;
+    q.mode = mode;
+    q.n = ray_count;
+    q.cull_mask = cull_mask & 0xFFu;
+    q.rays = rays_device;
+    q.hits = hits_device;
+    q.occluded = occluded_device;
+    q.state = ctx->state;
+    q.stats = ctx->stats;
+    rtx::TraceLaunch L = launch_shape(ctx);
+    if ((r = ensure_gstack(ctx, L.grid))) return r;
+    q.gstack = ctx->gstack;
+    hipError_t e = rtx::trace_rays_launch(q, L, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_trace_ray_buffer");
+    return RTX_SUCCESS;
+}
+
+int rtx_set_trace_stats(rtx_context ctx, int enable) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    ctx->stats = enable != 0;
+    return RTX_SUCCESS;
+}
+
+int rtx_get_trace_stats(rtx_context ctx, rtx_trace_stats* out_stats) {
+    if (!ctx || !out_stats) return RTX_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_trace_stats");
+    rtx::DeviceState st;
+    e = hipMemcpy(&st, ctx->state, sizeof(st), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_trace_stats copy");
+    out_stats->rays = st.stat_rays;
+    out_stats->node_fetches = st.stat_nodes;
+    out_stats->triangle_tests = st.stat_tris;
+    out_stats->instance_entries = st.stat_insts;
+    return RTX_SUCCESS;
+}
+
+int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg) {
+    if (!ctx || !cfg) return RTX_ERROR_INVALID_ARGUMENT;
+    rtx_trace_config c = ctx->cfg;
+    if (cfg->block_size) c.block_size = cfg->block_size;
+    if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
+    if (cfg->lds_stack_depth) c.lds_stack_depth = cfg->lds_stack_depth;
+    if (c.block_size < 64 || c.block_size > 256 || (c.block_size & 63))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
+    if (c.blocks_per_cu < 1 || c.blocks_per_cu > 32)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 32]");
+    if (c.lds_stack_depth < 1 || (uint64_t)c.lds_stack_depth * c.block_size * 8 > 64 * 1024)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "LDS stack above 64 KiB per block");
+    ctx->cfg = c;
+    return RTX_SUCCESS;
+}
+
+int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg) {
+    if (!ctx || !out_cfg) return RTX_ERROR_INVALID_ARGUMENT;
+    *out_cfg = ctx->cfg;
+    return RTX_SUCCESS;
+}
+
+int rtx_debug_lbvh(rtx_context ctx, const float* boxes_device, uint32_t n, uint32_t morton_bits, void* codes_device,
+                   uint32_t* order_device, int32_t* left_device, int32_t* right_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "morton_bits must be 30 or 63");
+    if (n > kMaxPrims) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "n too large");
+    if (n > 0 && (!boxes_device || !codes_device || !order_device))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
+    if (n > 1 && (!left_device || !right_device)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
+    hipError_t e = rtx::debug_lbvh(boxes_device, n, morton_bits == 63, codes_device, order_device, left_device,
+                                   right_device, &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_debug_lbvh");
+    return rtx_sync(ctx);
+}
+
+int rtx_debug_sort(rtx_context ctx, void* keys_device, uint32_t* values_device, uint32_t n, uint32_t key_bytes) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (key_bytes != 4 && key_bytes != 8) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "key_bytes must be 4 or 8");
+    if (n > kMaxPrims) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "n too large");
+    if (n > 0 && (!keys_device || !values_device)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
+    hipError_t e = rtx::debug_sort(keys_device, values_device, n, (int)key_bytes, &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_debug_sort");
+    return rtx_sync(ctx);
+}
+
+}  // extern "C"

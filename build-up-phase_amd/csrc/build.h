@@ -1,0 +1,27 @@
+// build.h — acceleration-structure builders (BLAS over triangles, TLAS over instances).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtx_internal.h"
+
+namespace rtx {
+
+uint64_t blas_as_bytes(uint32_t prim_count);
+uint64_t tlas_as_bytes(uint32_t instance_count);
+uint64_t blas_scratch_bytes(uint32_t prim_count, bool morton63, uint32_t geometry_count);
+uint64_t tlas_scratch_bytes(uint32_t instance_count, bool morton63);
+
+// Enqueue a BLAS build of prim_count triangles described by geometry_count host descriptors.
+hipError_t build_blas(const GeomDesc* geoms_host, uint32_t geometry_count, uint32_t prim_count, uint32_t flags,
+                      void* dst, void* scratch, uint32_t* err_flags, hipStream_t stream);
+// Enqueue a TLAS build over instance_count device-resident rtx_instance records.
+hipError_t build_tlas(const rtx_instance* instances_device, uint32_t instance_count, uint32_t flags, void* dst,
+                      void* scratch, uint32_t* err_flags, hipStream_t stream);
+
+// Verification hooks: Morton -> sort -> Karras on caller boxes; the bare radix sort.  Both block.
+hipError_t debug_lbvh(const float* boxes, uint32_t n, bool morton63, void* codes_out, uint32_t* order_out,
+                      int32_t* left_out, int32_t* right_out, uint32_t* err_flags, hipStream_t stream);
+hipError_t debug_sort(void* keys, uint32_t* vals, uint32_t n, int key_bytes, uint32_t* err_flags, hipStream_t stream);
+
+}  // namespace rtx

@@ -1,0 +1,845 @@
+// build.hip — LBVH builder for BLAS (triangles) and TLAS (instances) on gfx950.
+//
+// Replaces the driver build behind vkCmdBuildAccelerationStructuresKHR (main.cpp:808 for the BLAS,
+// main.cpp:932 for the TLAS).  Pipeline, all on the context stream:
+//   prep      one thread per primitive: fetch + transform triangle vertices (BLAS) or read the 64-B instance
+//             record, invert its transform in FP64 and bound the BLAS root box in world space (TLAS);
+//             writes a 64-B leaf record, the primitive box, and the centroid bounds (ordered-int atomics)
+//   morton    30-bit (uint32) or 63-bit (uint64) Morton code of each box centroid
+//   sort      stable onesweep radix sort of (code, primitive index)                         [sort.hip]
+//   karras    binary radix tree over the sorted codes (Karras 2012), one thread per internal node
+//   refit     bottom-up boxes: the second thread to reach a node merges its children (atomic arrival count)
+//   collapse  level-synchronous top-down conversion to 8-wide nodes: each node adopts up to 8 children by
+//             repeatedly opening the child with the largest surface area, places them by octant, quantises
+//             their boxes to 8 bits per plane and allocates their slots with one atomic add.
+// Karras trees over 30-bit codes with 32-bit index tie-breaks are at most 64 levels deep (63-bit codes: 96),
+// so the host enqueues that many collapse levels; a level whose frontier is empty returns at once.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "build.h"
+#include "rtx_internal.h"
+#include "sort.h"
+
+namespace rtx {
+
+namespace {
+
+constexpr int kCollapseBlocks = 1024;
+constexpr int kMaxLevels32 = 66;
+constexpr int kMaxLevels64 = 98;
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ float box_area(const float* b) {
+    float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    if (!(dx >= 0.0f) || !(dy >= 0.0f) || !(dz >= 0.0f)) return 0.0f;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Geometry transform at build time: x' = ((m00*x + m01*y) + m02*z) + m03 with separate roundings
+// (this file is compiled with -ffp-contract=off).
+__device__ __forceinline__ void xform_vertex(const float* m, const float* v, float* o) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        float a = m[4 * r + 0] * v[0];
+        float b = m[4 * r + 1] * v[1];
+        float c = m[4 * r + 2] * v[2];
+        float s = a + b;
+        s = s + c;
+        s = s + m[4 * r + 3];
+        o[r] = s;
+    }
+}
+
+__device__ void reduce_bounds(const float* c, bool valid, uint32_t* cb) {
+    // block-wide min/max of centroids through LDS, then one atomic per block and axis
+    __shared__ uint32_t s[6];
+    if (threadIdx.x < 6) s[threadIdx.x] = (threadIdx.x < 3) ? 0xFFFFFFFFu : 0u;
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint32_t o = float_to_ordered(c[a]);
+            atomicMin(&s[a], o);
+            atomicMax(&s[3 + a], o);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        if (s[threadIdx.x] != 0xFFFFFFFFu) atomicMin(&cb[threadIdx.x], s[threadIdx.x]);
+    } else if (threadIdx.x < 6) {
+        if (s[threadIdx.x] != 0u) atomicMax(&cb[threadIdx.x], s[threadIdx.x]);
+    }
+}
+
+__global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restrict__ geoms, int ngeom, uint32_t n,
+                                                        TriSlot* __restrict__ leaves, float* __restrict__ boxes,
+                                                        uint32_t* __restrict__ cbounds) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = false;
+    float cen[3] = {0, 0, 0};
+    if (i < n) {
+        int lo = 0, hi = ngeom - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (geoms[mid].prim_base <= i) lo = mid; else hi = mid - 1;
+        }
+        const GeomDesc g = geoms[lo];
+        uint32_t p = i - g.prim_base;
+        TriSlot t;
+        valid = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uint32_t vi = g.idx[3 * (uint64_t)p + k];
+            float v[3];
+            if (vi > g.max_vertex) {
+                valid = false;
+                v[0] = v[1] = v[2] = 0.0f;
+            } else {
+                const float* src = (const float*)(g.vtx + (uint64_t)(vi + g.first_vertex) * g.stride);
+                v[0] = src[0];
+                v[1] = src[1];
+                v[2] = src[2];
+            }
+            if (g.xform) {
+                float o[3];
+                xform_vertex(g.xform, v, o);
+                t.v[3 * k] = o[0];
+                t.v[3 * k + 1] = o[1];
+                t.v[3 * k + 2] = o[2];
+            } else {
+                t.v[3 * k] = v[0];
+                t.v[3 * k + 1] = v[1];
+                t.v[3 * k + 2] = v[2];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) valid = valid && isfinite(t.v[k]);
+        t.prim = p;
+        t.geom = (uint32_t)lo;
+        t.pad[0] = valid ? 1u : 0u;
+        t.pad[1] = t.pad[2] = t.pad[3] = t.pad[4] = 0u;
+        leaves[i] = t;
+        float* b = boxes + 6 * (uint64_t)i;
+        if (valid) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float l = fminf(t.v[a], fminf(t.v[3 + a], t.v[6 + a]));
+                float h = fmaxf(t.v[a], fmaxf(t.v[3 + a], t.v[6 + a]));
+                b[a] = l;
+                b[3 + a] = h;
+                float s = l + h;
+                cen[a] = s * 0.5f;
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                b[a] = INFINITY;
+                b[3 + a] = -INFINITY;
+            }
+        }
+    }
+    reduce_bounds(cen, valid, cbounds);
+}
+
+__device__ void invert3x4_d(const double* m, double* inv) {
+    double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    double r = 1.0 / det;
+    inv[0] = A * r;
+    inv[1] = -(b * i - c * h) * r;
+    inv[2] = (b * f - c * e) * r;
+    inv[4] = B * r;
+    inv[5] = (a * i - c * g) * r;
+    inv[6] = -(a * f - c * d) * r;
+    inv[8] = C * r;
+    inv[9] = -(a * h - b * g) * r;
+    inv[10] = (a * e - b * d) * r;
+#pragma unroll
+    for (int row = 0; row < 3; ++row)
+        inv[4 * row + 3] = -(inv[4 * row + 0] * m[3] + inv[4 * row + 1] * m[7] + inv[4 * row + 2] * m[11]);
+}
+
+__device__ __forceinline__ float round_down_f(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+__device__ __forceinline__ float round_up_f(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+__global__ __launch_bounds__(256) void prep_instances_kernel(const rtx_instance* __restrict__ inst, uint32_t n,
+                                                             InstSlot* __restrict__ leaves,
+                                                             InstanceInfo* __restrict__ table,
+                                                             float* __restrict__ boxes,
+                                                             uint32_t* __restrict__ cbounds,
+                                                             uint32_t* __restrict__ max_blas_depth) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = false;
+    float cen[3] = {0, 0, 0};
+    if (i < n) {
+        rtx_instance r = inst[i];
+        double m[12];
+        for (int k = 0; k < 12; ++k) m[k] = (double)(&r.transform[0][0])[k];
+        double inv[12];
+        invert3x4_d(m, inv);
+        InstanceInfo info;
+        for (int k = 0; k < 12; ++k) info.inv[k] = inv[k];
+        info.custom = r.custom_index_and_mask & 0xFFFFFFu;
+        info.mask = r.custom_index_and_mask >> 24;
+        info.sbt_offset = r.sbt_offset_and_flags & 0xFFFFFFu;
+        info.flags = r.sbt_offset_and_flags >> 24;
+        info.blas = r.blas_address;
+        info.pad = 0;
+        bool ok = r.blas_address != 0 && (r.blas_address & 255u) == 0 && isfinite(inv[0]) && isfinite(inv[5]) &&
+                  isfinite(inv[10]);
+        InstSlot s;
+        for (int k = 0; k < 12; ++k) s.inv[k] = (float)inv[k];
+        s.blas = r.blas_address;
+        s.instance_id = i;
+        s.mask = info.mask;
+        float* b = boxes + 6 * (uint64_t)i;
+        if (ok) {
+            const AsHeader* h = (const AsHeader*)r.blas_address;
+            ok = h->magic == kMagicBlas && h->prim_count > 0 && h->slot_count > 0;
+            if (ok) {
+                double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+                for (int c = 0; c < 8; ++c) {
+                    double p[3] = {(double)((c & 1) ? h->hi[0] : h->lo[0]), (double)((c & 2) ? h->hi[1] : h->lo[1]),
+                                   (double)((c & 4) ? h->hi[2] : h->lo[2])};
+                    for (int a = 0; a < 3; ++a) {
+                        double w = m[4 * a] * p[0] + m[4 * a + 1] * p[1] + m[4 * a + 2] * p[2] + m[4 * a + 3];
+                        lo[a] = fmin(lo[a], w);
+                        hi[a] = fmax(hi[a], w);
+                    }
+                }
+                double ext = 0;
+                for (int a = 0; a < 3; ++a) ext = fmax(ext, fmax(fabs(lo[a]), fabs(hi[a])) + (hi[a] - lo[a]));
+                double pad = ext * 0x1p-20;
+                for (int a = 0; a < 3; ++a) {
+                    b[a] = round_down_f(lo[a] - pad);
+                    b[3 + a] = round_up_f(hi[a] + pad);
+                    float s2 = b[a] + b[3 + a];
+                    cen[a] = s2 * 0.5f;
+                }
+                atomicMax(max_blas_depth, h->max_depth);
+                valid = true;
+            }
+        }
+        if (!valid) {
+            for (int a = 0; a < 3; ++a) {
+                b[a] = INFINITY;
+                b[3 + a] = -INFINITY;
+            }
+        }
+        leaves[i] = s;
+        table[i] = info;
+    }
+    reduce_bounds(cen, valid, cbounds);
+}
+
+__global__ __launch_bounds__(256) void box_bounds_kernel(const float* __restrict__ boxes, uint32_t n,
+                                                         uint32_t* __restrict__ cbounds) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float cen[3] = {0, 0, 0};
+    bool valid = false;
+    if (i < n) {
+        const float* b = boxes + 6 * (uint64_t)i;
+        valid = b[0] <= b[3] && b[1] <= b[4] && b[2] <= b[5];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float s = b[a] + b[3 + a];
+            cen[a] = s * 0.5f;
+        }
+    }
+    reduce_bounds(cen, valid, cbounds);
+}
+
+__device__ __forceinline__ uint32_t quant_axis(float c, float cmin, float scale, uint32_t maxq) {
+    float f = (c - cmin) * scale;
+    if (!(f > 0.0f)) return 0u;
+    if (f >= (float)maxq) return maxq;
+    return (uint32_t)f;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void morton_kernel(const float* __restrict__ boxes, uint32_t n,
+                                                     const uint32_t* __restrict__ cbounds, K* __restrict__ keys) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    constexpr bool k64 = sizeof(K) == 8;
+    const float cells = k64 ? 2097152.0f : 1024.0f;
+    const uint32_t maxq = k64 ? 2097151u : 1023u;
+    const float* b = boxes + 6 * (uint64_t)i;
+    uint32_t q[3];
+    bool valid = b[0] <= b[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float cmin = ordered_to_float(cbounds[a]);
+        float cmax = ordered_to_float(cbounds[3 + a]);
+        float ext = cmax - cmin;
+        float scale = ext > 0.0f ? cells / ext : 0.0f;
+        float s = b[a] + b[3 + a];
+        float c = s * 0.5f;
+        q[a] = quant_axis(c, cmin, scale, maxq);
+    }
+    K code;
+    if (k64)
+        code = (K)((spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]));
+    else
+        code = (K)((spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]));
+    keys[i] = valid ? code : (K)~(K)0;
+}
+
+template <typename K>
+__device__ __forceinline__ int karras_delta(const K* keys, int64_t n, int64_t i, int64_t j) {
+    if (j < 0 || j >= n) return -1;
+    K a = keys[i], b = keys[j];
+    constexpr int kb = (int)sizeof(K) * 8;
+    if (a == b) return kb + __clz((uint32_t)(i ^ j));
+    if (sizeof(K) == 8) return __clzll((unsigned long long)(a ^ b));
+    return __clz((uint32_t)(a ^ b));
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void karras_kernel(const K* __restrict__ keys, uint32_t n, int32_t* __restrict__ left,
+                                                     int32_t* __restrict__ right, int32_t* __restrict__ parent) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i == 0) parent[0] = -1;
+    if (i >= (int64_t)n - 1) return;
+    int64_t N = n;
+    int d = (karras_delta(keys, N, i, i + 1) - karras_delta(keys, N, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = karras_delta(keys, N, i, i - d);
+    int64_t lmax = 2;
+    while (karras_delta(keys, N, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+        if (karras_delta(keys, N, i, i + (l + t) * d) > dmin) l += t;
+    int64_t j = i + l * d;
+    int dnode = karras_delta(keys, N, i, j);
+    int64_t s = 0;
+    int64_t t = l;
+    do {
+        t = (t + 1) / 2;
+        if (s + t <= l && karras_delta(keys, N, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    int32_t lc = (lo == gamma) ? (int32_t)(-1 - gamma) : (int32_t)gamma;
+    int32_t rc = (hi == gamma + 1) ? (int32_t)(-1 - (gamma + 1)) : (int32_t)(gamma + 1);
+    left[i] = lc;
+    right[i] = rc;
+    parent[lc >= 0 ? lc : (N - 1) + (-1 - lc)] = (int32_t)i;
+    parent[rc >= 0 ? rc : (N - 1) + (-1 - rc)] = (int32_t)i;
+}
+
+__device__ __forceinline__ float load_coherent(const float* p) {
+    return __uint_as_float(__hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// node boxes: index 0..n-2 internal, n-1+pos leaves (sorted position)
+__global__ __launch_bounds__(256) void refit_kernel(uint32_t n, const int32_t* __restrict__ left,
+                                                    const int32_t* __restrict__ right,
+                                                    const int32_t* __restrict__ parent,
+                                                    const uint32_t* __restrict__ order,
+                                                    const float* __restrict__ prim_boxes, float* nboxes,
+                                                    uint32_t* arrivals) {
+    uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= n) return;
+    const float* pb = prim_boxes + 6 * (uint64_t)order[pos];
+    float* lb = nboxes + 6 * ((uint64_t)n - 1 + pos);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) lb[k] = pb[k];
+    int32_t node = parent[n - 1 + pos];
+    while (node >= 0) {
+        __threadfence();
+        uint32_t old = atomicAdd(&arrivals[node], 1u);
+        if (old == 0u) return;
+        __threadfence();
+        int32_t l = left[node], r = right[node];
+        uint64_t li = l >= 0 ? (uint64_t)l : (uint64_t)n - 1 + (uint64_t)(-1 - l);
+        uint64_t ri = r >= 0 ? (uint64_t)r : (uint64_t)n - 1 + (uint64_t)(-1 - r);
+        float out[6];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            out[a] = fminf(load_coherent(nboxes + 6 * li + a), load_coherent(nboxes + 6 * ri + a));
+            out[3 + a] = fmaxf(load_coherent(nboxes + 6 * li + 3 + a), load_coherent(nboxes + 6 * ri + 3 + a));
+        }
+        float* nb = nboxes + 6 * (uint64_t)node;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) nb[k] = out[k];
+        node = parent[node];
+    }
+}
+
+// Header setup after refit: root box and the 16-bit origin quantisation step.
+__global__ void header_kernel(AsHeader* h, uint32_t type, uint32_t flags, uint32_t n, uint32_t ngeom,
+                              const float* nboxes, uint64_t capacity_slots, uint64_t table_offset,
+                              uint32_t* level_counts, uint2* tasks0) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    h->magic = type ? kMagicTlas : kMagicBlas;
+    h->version = kVersion;
+    h->type = type;
+    h->flags = flags;
+    h->prim_count = n;
+    h->geometry_count = ngeom;
+    h->node_count = 0;
+    h->slot_count = 0;
+    h->max_depth = 0;
+    h->max_total_depth = 0;
+    h->build_error = 0;
+    h->slots_offset = kHeaderBytes;
+    h->inst_table_offset = table_offset;
+    h->capacity_slots = capacity_slots;
+    h->sah_cost = 0.0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (n > 0) {
+        const float* rb = (n == 1) ? nboxes + 0 : nboxes;  // root: internal 0, or leaf 0 when n == 1
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = rb[a];
+            hi[a] = rb[3 + a];
+        }
+    }
+    bool empty = !(lo[0] <= hi[0]) || n == 0;
+    for (int a = 0; a < 3; ++a) {
+        if (empty) {
+            lo[a] = 0.0f;
+            hi[a] = 0.0f;
+        }
+        h->lo[a] = lo[a];
+        h->hi[a] = hi[a];
+        float ext = hi[a] - lo[a];
+        h->qscale[a] = ext > 0.0f ? ext / 65535.0f : 0.0f;
+    }
+    h->lo[3] = h->hi[3] = h->qscale[3] = 0.0f;
+    if (empty) {
+        level_counts[0] = 0;
+        return;
+    }
+    h->slot_count = 1;  // slot 0 = root
+    level_counts[0] = 1;
+    // root task: BVH2 root is internal node 0, or leaf 0 (encoded -1) when n == 1
+    tasks0[0] = make_uint2(n == 1 ? 0xFFFFFFFFu : 0u, 0u);
+}
+
+__device__ __forceinline__ float exp_scale(uint32_t e) { return __uint_as_float(e << 23); }
+
+// Quantisation frame of one axis: origin p (decoded from pq) and exponent byte e with fma(255, 2^e, p) >= hi.
+__device__ void make_frame(float lo, float hi, float root_lo, float qscale, uint16_t& pq_out, float& p_out,
+                           uint32_t& e_out) {
+    uint32_t pq = 0;
+    if (qscale > 0.0f) {
+        float f = floorf((lo - root_lo) / qscale);
+        pq = f <= 0.0f ? 0u : (f >= 65535.0f ? 65535u : (uint32_t)f);
+        while (pq > 0 && fmaf((float)pq, qscale, root_lo) > lo) --pq;
+    }
+    float p = fmaf((float)pq, qscale, root_lo);
+    uint32_t e = 1;  // 2^-126
+    float x = (hi - p) / 255.0f;
+    if (x > 0.0f) {
+        int k;
+        frexpf(x, &k);  // x = m * 2^k, m in [0.5, 1)
+        int ee = k - 1 + 127;
+        if (ee < 1) ee = 1;
+        if (ee > 254) ee = 254;
+        e = (uint32_t)ee;
+        while (e < 254 && fmaf(255.0f, exp_scale(e), p) < hi) ++e;
+    }
+    pq_out = (uint16_t)pq;
+    p_out = p;
+    e_out = e;
+}
+
+__device__ __forceinline__ void quant_child(float clo, float chi, float p, uint32_t e, uint8_t& qlo, uint8_t& qhi) {
+    float s = exp_scale(e);
+    float fl = floorf((clo - p) / s);
+    int ql = fl <= 0.0f ? 0 : (fl >= 255.0f ? 255 : (int)fl);
+    while (ql > 0 && fmaf((float)ql, s, p) > clo) --ql;
+    float fh = ceilf((chi - p) / s);
+    int qh = fh <= 0.0f ? 0 : (fh >= 255.0f ? 255 : (int)fh);
+    while (qh < 255 && fmaf((float)qh, s, p) < chi) ++qh;
+    if (qh < ql) qh = ql;
+    qlo = (uint8_t)ql;
+    qhi = (uint8_t)qh;
+}
+
+// One collapse level.  Task = (BVH2 node code, destination slot); node code 0xFFFFFFFF = leaf 0 (n == 1 root).
+__global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const int32_t* __restrict__ left,
+                                                             const int32_t* __restrict__ right,
+                                                             const uint32_t* __restrict__ order,
+                                                             const float* __restrict__ nboxes,
+                                                             const uint4* __restrict__ leaf_src,
+                                                             AsHeader* __restrict__ h, uint4* __restrict__ slots,
+                                                             uint32_t* __restrict__ level_counts, int level,
+                                                             const uint2* __restrict__ tasks_in,
+                                                             uint2* __restrict__ tasks_out) {
+    const uint32_t count = level_counts[level];
+    if (count == 0) return;
+    const float rlo[3] = {h->lo[0], h->lo[1], h->lo[2]};
+    const float qs[3] = {h->qscale[0], h->qscale[1], h->qscale[2]};
+    for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < count; ti += gridDim.x * blockDim.x) {
+        uint2 task = tasks_in[ti];
+        int32_t ch[8];
+        int cnt;
+        if (task.x == 0xFFFFFFFFu) {
+            ch[0] = -1;
+            cnt = 1;
+        } else {
+            ch[0] = left[task.x];
+            ch[1] = right[task.x];
+            cnt = 2;
+        }
+        // open the internal child with the largest surface area until 8 children
+        while (cnt < 8) {
+            int best = -1;
+            float best_a = -1.0f;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j < cnt && ch[j] >= 0) {
+                    float a = box_area(nboxes + 6 * (uint64_t)ch[j]);
+                    if (a > best_a) {
+                        best_a = a;
+                        best = j;
+                    }
+                }
+            }
+            if (best < 0) break;
+            int32_t c = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j == best) c = ch[j];
+            int32_t l = left[c], r = right[c];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (j == best) ch[j] = l;
+                if (j == cnt) ch[j] = r;
+            }
+            ++cnt;
+        }
+        // child boxes; drop empty ones
+        float cb[8][6];
+        bool cv[8];
+        float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            cv[j] = false;
+            if (j < cnt) {
+                int32_t c = ch[j];
+                uint64_t bi = c >= 0 ? (uint64_t)c : (uint64_t)n - 1 + (uint64_t)(-1 - c);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) cb[j][k] = nboxes[6 * bi + k];
+                cv[j] = cb[j][0] <= cb[j][3] && cb[j][1] <= cb[j][4] && cb[j][2] <= cb[j][5];
+                if (cv[j]) {
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        nlo[a] = fminf(nlo[a], cb[j][a]);
+                        nhi[a] = fmaxf(nhi[a], cb[j][3 + a]);
+                    }
+                }
+            }
+        }
+        // octant placement: greedy assignment of children to positions by centroid direction
+        int pos_of[8];
+        uint32_t used = 0;
+        float nc[3] = {0.5f * (nlo[0] + nhi[0]), 0.5f * (nlo[1] + nhi[1]), 0.5f * (nlo[2] + nhi[2])};
+#pragma unroll
+        for (int j = 0; j < 8; ++j) pos_of[j] = -1;
+        for (int it = 0; it < 8; ++it) {
+            float best = -INFINITY;
+            int bj = -1, bp = -1;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (!cv[j] || pos_of[j] >= 0) continue;
+                float dx = 0.5f * (cb[j][0] + cb[j][3]) - nc[0];
+                float dy = 0.5f * (cb[j][1] + cb[j][4]) - nc[1];
+                float dz = 0.5f * (cb[j][2] + cb[j][5]) - nc[2];
+#pragma unroll
+                for (int p = 0; p < 8; ++p) {
+                    if (used & (1u << p)) continue;
+                    float sc = ((p & 1) ? dx : -dx) + ((p & 2) ? dy : -dy) + ((p & 4) ? dz : -dz);
+                    if (sc > best) {
+                        best = sc;
+                        bj = j;
+                        bp = p;
+                    }
+                }
+            }
+            if (bj < 0) break;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j == bj) pos_of[j] = bp;
+            used |= 1u << bp;
+        }
+        // node frame
+        Node8 node;
+        float p[3];
+        uint32_t e[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            uint16_t pq;
+            make_frame(nlo[a], nhi[a], rlo[a], qs[a], pq, p[a], e[a]);
+            node.pq[a] = pq;
+            node.e[a] = (uint8_t)e[a];
+        }
+        node.imask = 0;
+        node.vmask = (uint8_t)used;
+        node.pad = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                node.qlo[a][k] = 0;
+                node.qhi[a][k] = 0;
+            }
+        const uint32_t nvalid = (uint32_t)__popc(used);
+        const uint32_t base = atomicAdd(&h->slot_count, nvalid);
+        node.base = base;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (!cv[j]) continue;
+            int k = pos_of[j];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) quant_child(cb[j][a], cb[j][3 + a], p[a], e[a], node.qlo[a][k], node.qhi[a][k]);
+            uint32_t slot = base + (uint32_t)__popc(used & ((1u << k) - 1u));
+            int32_t c = ch[j];
+            if (c >= 0) {
+                node.imask |= (uint8_t)(1u << k);
+                uint32_t o = atomicAdd(&level_counts[level + 1], 1u);
+                tasks_out[o] = make_uint2((uint32_t)c, slot);
+            } else {
+                uint32_t prim = order[-1 - c];
+                const uint4* src = leaf_src + 4 * (uint64_t)prim;
+                uint4* dst = slots + 4 * (uint64_t)slot;
+                dst[0] = src[0];
+                dst[1] = src[1];
+                dst[2] = src[2];
+                dst[3] = src[3];
+            }
+        }
+        const uint4* nsrc = (const uint4*)&node;
+        uint4* ndst = slots + 4 * (uint64_t)task.y;
+        ndst[0] = nsrc[0];
+        ndst[1] = nsrc[1];
+        ndst[2] = nsrc[2];
+        ndst[3] = nsrc[3];
+        atomicAdd(&h->node_count, 1u);
+        atomicMax(&h->max_depth, (uint32_t)(level + 1));
+    }
+}
+
+__global__ void finish_tlas_kernel(AsHeader* h, const uint32_t* max_blas_depth) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) h->max_total_depth = h->max_depth + *max_blas_depth;
+}
+__global__ void finish_blas_kernel(AsHeader* h) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) h->max_total_depth = h->max_depth;
+}
+
+__global__ void fill_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---- scratch layout --------------------------------------------------------------------------------------------
+struct Layout {
+    uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, tasks0, tasks1, levels, cbounds,
+        misc, geoms, total;
+};
+
+Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom) {
+    Layout L;
+    uint64_t o = 0;
+    auto take = [&](uint64_t bytes) {
+        uint64_t at = o;
+        o += align_up(bytes ? bytes : 1, 256);
+        return at;
+    };
+    uint64_t nn = n ? n : 1;
+    L.leaves = take(nn * 64);
+    L.boxes = take(nn * 24);
+    L.keys = take(nn * key_bytes);
+    L.vals = take(nn * 4);
+    L.sort = take(sort_scratch_bytes(n, key_bytes));
+    L.left = take(nn * 4);
+    L.right = take(nn * 4);
+    L.parent = take(2 * nn * 4);
+    L.nboxes = take(2 * nn * 24);
+    L.arrivals = take(nn * 4);
+    L.tasks0 = take(nn * 8);
+    L.tasks1 = take(nn * 8);
+    L.levels = take(128 * 4);
+    L.cbounds = take(8 * 4);
+    L.misc = take(64);
+    L.geoms = take((uint64_t)(ngeom ? ngeom : 1) * sizeof(GeomDesc));
+    L.total = o;
+    return L;
+}
+
+uint64_t as_bytes(uint32_t n, bool tlas) {
+    uint64_t slots = 2ull * (n ? n : 1) + 1;
+    uint64_t b = kHeaderBytes + slots * kSlotBytes;
+    if (tlas) b = align_up(b, 256) + (uint64_t)(n ? n : 1) * sizeof(InstanceInfo);
+    return align_up(b, 256);
+}
+
+// Common LBVH tail: morton -> sort -> karras -> refit -> header -> collapse.
+hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint32_t ngeom, uint8_t* scratch,
+                       const Layout& L, uint8_t* dst, uint64_t table_offset, uint32_t* err_flags, hipStream_t s) {
+    AsHeader* h = (AsHeader*)dst;
+    uint4* slots = (uint4*)(dst + kHeaderBytes);
+    float* boxes = (float*)(scratch + L.boxes);
+    uint32_t* cb = (uint32_t*)(scratch + L.cbounds);
+    uint32_t* vals = (uint32_t*)(scratch + L.vals);
+    int32_t* left = (int32_t*)(scratch + L.left);
+    int32_t* right = (int32_t*)(scratch + L.right);
+    int32_t* parent = (int32_t*)(scratch + L.parent);
+    float* nboxes = (float*)(scratch + L.nboxes);
+    uint32_t* arrivals = (uint32_t*)(scratch + L.arrivals);
+    uint32_t* levels = (uint32_t*)(scratch + L.levels);
+    uint2* t0 = (uint2*)(scratch + L.tasks0);
+    uint2* t1 = (uint2*)(scratch + L.tasks1);
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    hipError_t err;
+    if (n > 0) {
+        if (k64) {
+            morton_kernel<uint64_t><<<gb, 256, 0, s>>>(boxes, n, cb, (uint64_t*)(scratch + L.keys));
+            if ((err = sort_pairs_u64((uint64_t*)(scratch + L.keys), vals, n, scratch + L.sort, err_flags, s)))
+                return err;
+        } else {
+            morton_kernel<uint32_t><<<gb, 256, 0, s>>>(boxes, n, cb, (uint32_t*)(scratch + L.keys));
+            if ((err = sort_pairs_u32((uint32_t*)(scratch + L.keys), vals, n, scratch + L.sort, err_flags, s)))
+                return err;
+        }
+        if (n > 1) {
+            if (k64)
+                karras_kernel<uint64_t><<<gb, 256, 0, s>>>((uint64_t*)(scratch + L.keys), n, left, right, parent);
+            else
+                karras_kernel<uint32_t><<<gb, 256, 0, s>>>((uint32_t*)(scratch + L.keys), n, left, right, parent);
+            fill_kernel<<<256, 256, 0, s>>>(arrivals, n, 0u);
+        } else {
+            fill_kernel<<<1, 64, 0, s>>>((uint32_t*)parent, 1, 0xFFFFFFFFu);
+        }
+        refit_kernel<<<gb, 256, 0, s>>>(n, left, right, parent, vals, boxes, nboxes, arrivals);
+    }
+    fill_kernel<<<1, 128, 0, s>>>(levels, 128, 0u);
+    // root box: internal node 0, or the single leaf (stored at index n-1 == 0)
+    header_kernel<<<1, 64, 0, s>>>(h, type, flags, n, ngeom, nboxes, 2ull * (n ? n : 1) + 1, table_offset, levels, t0);
+    if (n > 0) {
+        const int max_levels = k64 ? kMaxLevels64 : kMaxLevels32;
+        for (int lv = 0; lv < max_levels; ++lv) {
+            uint2* tin = (lv & 1) ? t1 : t0;
+            uint2* tout = (lv & 1) ? t0 : t1;
+            collapse_level_kernel<<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
+                                                                  (const uint4*)(scratch + L.leaves), h, slots, levels,
+                                                                  lv, tin, tout);
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t debug_lbvh(const float* boxes, uint32_t n, bool k64, void* codes_out, uint32_t* order_out,
+                      int32_t* left_out, int32_t* right_out, uint32_t* err_flags, hipStream_t s) {
+    Layout L = make_layout(n, k64 ? 8 : 4, 0);
+    uint8_t* sc = nullptr;
+    hipError_t e = hipMalloc(&sc, L.total);
+    if (e) return e;
+    uint32_t* cb = (uint32_t*)(sc + L.cbounds);
+    fill_kernel<<<1, 64, 0, s>>>(cb, 3, 0xFFFFFFFFu);
+    fill_kernel<<<1, 64, 0, s>>>(cb + 3, 3, 0u);
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    uint32_t* vals = (uint32_t*)(sc + L.vals);
+    if (n > 0) {
+        box_bounds_kernel<<<gb, 256, 0, s>>>(boxes, n, cb);
+        if (k64) {
+            morton_kernel<uint64_t><<<gb, 256, 0, s>>>(boxes, n, cb, (uint64_t*)(sc + L.keys));
+            e = sort_pairs_u64((uint64_t*)(sc + L.keys), vals, n, sc + L.sort, err_flags, s);
+            if (!e && n > 1)
+                karras_kernel<uint64_t><<<gb, 256, 0, s>>>((uint64_t*)(sc + L.keys), n, (int32_t*)(sc + L.left),
+                                                           (int32_t*)(sc + L.right), (int32_t*)(sc + L.parent));
+        } else {
+            morton_kernel<uint32_t><<<gb, 256, 0, s>>>(boxes, n, cb, (uint32_t*)(sc + L.keys));
+            e = sort_pairs_u32((uint32_t*)(sc + L.keys), vals, n, sc + L.sort, err_flags, s);
+            if (!e && n > 1)
+                karras_kernel<uint32_t><<<gb, 256, 0, s>>>((uint32_t*)(sc + L.keys), n, (int32_t*)(sc + L.left),
+                                                           (int32_t*)(sc + L.right), (int32_t*)(sc + L.parent));
+        }
+    }
+    if (!e && n > 0) e = hipMemcpyAsync(codes_out, sc + L.keys, (uint64_t)n * (k64 ? 8 : 4), hipMemcpyDeviceToDevice, s);
+    if (!e && n > 0) e = hipMemcpyAsync(order_out, vals, (uint64_t)n * 4, hipMemcpyDeviceToDevice, s);
+    if (!e && n > 1) e = hipMemcpyAsync(left_out, sc + L.left, (uint64_t)(n - 1) * 4, hipMemcpyDeviceToDevice, s);
+    if (!e && n > 1) e = hipMemcpyAsync(right_out, sc + L.right, (uint64_t)(n - 1) * 4, hipMemcpyDeviceToDevice, s);
+    hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(sc);
+    return e ? e : e2;
+}
+
+hipError_t debug_sort(void* keys, uint32_t* vals, uint32_t n, int key_bytes, uint32_t* err_flags, hipStream_t s) {
+    uint8_t* sc = nullptr;
+    hipError_t e = hipMalloc(&sc, sort_scratch_bytes(n, key_bytes) + 256);
+    if (e) return e;
+    if (key_bytes == 8)
+        e = sort_pairs_u64((uint64_t*)keys, vals, n, sc, err_flags, s);
+    else
+        e = sort_pairs_u32((uint32_t*)keys, vals, n, sc, err_flags, s);
+    hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(sc);
+    return e ? e : e2;
+}
+
+uint64_t blas_as_bytes(uint32_t n) { return as_bytes(n, false); }
+uint64_t tlas_as_bytes(uint32_t n) { return as_bytes(n, true); }
+uint64_t blas_scratch_bytes(uint32_t n, bool k64, uint32_t ngeom) { return make_layout(n, k64 ? 8 : 4, ngeom).total; }
+uint64_t tlas_scratch_bytes(uint32_t n, bool k64) { return make_layout(n, k64 ? 8 : 4, 0).total; }
+
+hipError_t build_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, uint32_t flags, void* dst,
+                      void* scratch, uint32_t* err_flags, hipStream_t s) {
+    const bool k64 = (flags & RTX_BUILD_MORTON63_BIT) != 0;
+    Layout L = make_layout(n, k64 ? 8 : 4, ngeom);
+    uint8_t* sc = (uint8_t*)scratch;
+    hipError_t err;
+    GeomDesc* gd = (GeomDesc*)(sc + L.geoms);
+    if (ngeom) {
+        if ((err = hipMemcpyAsync(gd, geoms_host, ngeom * sizeof(GeomDesc), hipMemcpyHostToDevice, s))) return err;
+    }
+    uint32_t* cb = (uint32_t*)(sc + L.cbounds);
+    fill_kernel<<<1, 64, 0, s>>>(cb, 3, 0xFFFFFFFFu);
+    fill_kernel<<<1, 64, 0, s>>>(cb + 3, 3, 0u);
+    if (n > 0)
+        prep_tris_kernel<<<(n + 255) / 256, 256, 0, s>>>(gd, (int)ngeom, n, (TriSlot*)(sc + L.leaves),
+                                                         (float*)(sc + L.boxes), cb);
+    if ((err = lbvh_common(n, k64, 0, flags, ngeom, sc, L, (uint8_t*)dst, 0, err_flags, s))) return err;
+    finish_blas_kernel<<<1, 64, 0, s>>>((AsHeader*)dst);
+    return hipGetLastError();
+}
+
+hipError_t build_tlas(const rtx_instance* inst, uint32_t n, uint32_t flags, void* dst, void* scratch,
+                      uint32_t* err_flags, hipStream_t s) {
+    const bool k64 = (flags & RTX_BUILD_MORTON63_BIT) != 0;
+    Layout L = make_layout(n, k64 ? 8 : 4, 0);
+    uint8_t* sc = (uint8_t*)scratch;
+    uint64_t table_offset = align_up(kHeaderBytes + (2ull * (n ? n : 1) + 1) * kSlotBytes, 256);
+    uint32_t* cb = (uint32_t*)(sc + L.cbounds);
+    uint32_t* maxd = (uint32_t*)(sc + L.misc);
+    fill_kernel<<<1, 64, 0, s>>>(cb, 3, 0xFFFFFFFFu);
+    fill_kernel<<<1, 64, 0, s>>>(cb + 3, 3, 0u);
+    fill_kernel<<<1, 64, 0, s>>>(maxd, 1, 0u);
+    if (n > 0)
+        prep_instances_kernel<<<(n + 255) / 256, 256, 0, s>>>(inst, n, (InstSlot*)(sc + L.leaves),
+                                                              (InstanceInfo*)((uint8_t*)dst + table_offset),
+                                                              (float*)(sc + L.boxes), cb, maxd);
+    hipError_t err;
+    if ((err = lbvh_common(n, k64, 1, flags, 0, sc, L, (uint8_t*)dst, table_offset, err_flags, s))) return err;
+    finish_tlas_kernel<<<1, 64, 0, s>>>((AsHeader*)dst, maxd);
+    return hipGetLastError();
+}
+
+}  // namespace rtx

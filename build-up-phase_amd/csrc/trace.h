@@ -1,0 +1,42 @@
+// trace.h — launch interface of the persistent traversal kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rtx_internal.h"
+
+namespace rtx {
+
+struct TraceRequest {
+    const void* tlas;
+    uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film
+    uint32_t n;
+    uint32_t cull_mask;
+    const rtx_ray* rays;
+    rtx_hit* hits;
+    uint32_t* occluded;
+    // raygen
+    float cam[3];
+    float ax, ay;
+    uint32_t width, height;
+    float tmin, tmax;
+    const rtx_hit_record* records;
+    uint32_t nrec, sbt_offset, sbt_stride;
+    float miss[3];
+    float* film;
+    // runtime
+    DeviceState* state;
+    uint2* gstack;
+    bool stats;
+};
+
+struct TraceLaunch {
+    uint32_t grid;
+    uint32_t block_size;
+    uint32_t lds_depth;  // stack entries per lane in LDS
+    uint32_t gdepth;     // further entries per lane in the global spill area
+};
+
+hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);
+
+}  // namespace rtx

@@ -1,0 +1,659 @@
+// trace.hip — persistent BVH8 traversal for gfx950: ray buffers (closest / any hit) and the reference raygen.
+//
+// Replaces the driver traversal behind traceRayEXT (call site main.cpp:1047-1052) and the raygen / miss /
+// closest-hit programs (main.cpp:1033-1091).
+//
+// Per lane (wave64; one persistent grid, rays fetched 64 at a time from an atomic counter, lanes refilled
+// when fewer than 40 of 64 are still busy):
+//   * world ray (FP32) -> TLAS: nodes are tested with a conservative quantised slab test, instance leaves
+//     enter their BLAS;
+//   * BLAS entry (FP64): the ray is transformed by the FP64 inverse instance transform and its origin moved
+//     onto the BLAS root box (t_e = entry distance), then rounded to FP32.  All FP32 work inside the BLAS
+//     happens on coordinates of the size of the BLAS box, not of the distance to the camera;
+//   * triangles: watertight test (Woop, Benthin & Wald 2013) with the shear written as three row vectors, so
+//     no per-lane component selection is needed per triangle; exact-zero edge functions are recomputed in FP64;
+//   * traversal order: children are stored by octant; the group mask is permuted by the ray's sign octant so
+//     that lowest-bit-first visits near children first (Ylitie, Karras & Laine 2017 node groups);
+//   * stack: per-lane entries (slot base, child mask, valid mask, flags) in LDS, spilling to a global
+//     per-thread area past the LDS depth; overflow beyond that raises kErrStackOverflow;
+//   * closest hit: t, u, v are recomputed in FP64 (Moller-Trumbore) for the winning triangle.
+#include <cmath>
+
+#include "rtx_internal.h"
+#include "trace.h"
+
+namespace rtx {
+
+namespace {
+
+constexpr float kErrScale = 0x1p-20f;
+constexpr float kTinyDir = 0x1p-80f;
+constexpr uint32_t kEntryLeaf = 1u << 16;
+constexpr uint32_t kEntryTlas = 1u << 17;
+constexpr uint32_t kMaxSteps = 1u << 22;  // per ray; a valid BVH never needs more than ~2 steps per slot
+
+__device__ __forceinline__ uint32_t oct_permute(uint32_t m, uint32_t oct) {
+    if (oct & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
+    if (oct & 2u) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
+    if (oct & 4u) m = ((m & 0x0Fu) << 4) | ((m >> 4) & 0x0Fu);
+    return m;
+}
+
+__device__ __forceinline__ float safe_rcp(float d) {
+    float a = fabsf(d) < kTinyDir ? copysignf(kTinyDir, d) : d;
+    return 1.0f / a;
+}
+
+__device__ __forceinline__ float byte_f(uint32_t w, int b) {
+    return (float)((w >> (8 * b)) & 255u);
+}
+
+struct Space {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float ix, iy, iz;
+    uint32_t oct;
+    float toff;
+};
+
+__device__ __forceinline__ void setup_space(Space& s) {
+    s.ix = safe_rcp(s.dx);
+    s.iy = safe_rcp(s.dy);
+    s.iz = safe_rcp(s.dz);
+    s.oct = (s.dx < 0.0f ? 1u : 0u) | (s.dy < 0.0f ? 2u : 0u) | (s.dz < 0.0f ? 4u : 0u);
+}
+
+// Quantised 8-child slab test.  Returns the child-position hit mask.
+__device__ __forceinline__ uint32_t test_node(const uint4 w0, const uint4 w1, const uint4 w2, const uint4 w3,
+                                              const Space& s, const float* rlo, const float* qs, float smin,
+                                              float smax) {
+    const float pqx = (float)(w0.x & 0xFFFFu), pqy = (float)(w0.x >> 16), pqz = (float)(w0.y & 0xFFFFu);
+    const uint32_t ex = (w0.y >> 16) & 255u, ey = w0.y >> 24, ez = w0.z & 255u;
+    const uint32_t vmask = (w0.z >> 16) & 255u;
+    const float px = fmaf(pqx, qs[0], rlo[0]);
+    const float py = fmaf(pqy, qs[1], rlo[1]);
+    const float pz = fmaf(pqz, qs[2], rlo[2]);
+    const float sx = __uint_as_float(ex << 23), sy = __uint_as_float(ey << 23), sz = __uint_as_float(ez << 23);
+    const float Ax = (px - s.ox) * s.ix, Ay = (py - s.oy) * s.iy, Az = (pz - s.oz) * s.iz;
+    const float Sx = sx * s.ix, Sy = sy * s.iy, Sz = sz * s.iz;
+    const float ex_ = (fabsf(Ax) + 255.0f * fabsf(Sx) + fabsf(px * s.ix)) * kErrScale;
+    const float ey_ = (fabsf(Ay) + 255.0f * fabsf(Sy) + fabsf(py * s.iy)) * kErrScale;
+    const float ez_ = (fabsf(Az) + 255.0f * fabsf(Sz) + fabsf(pz * s.iz)) * kErrScale;
+    const float Anx = Ax - ex_, Afx = Ax + ex_;
+    const float Any = Ay - ey_, Afy = Ay + ey_;
+    const float Anz = Az - ez_, Afz = Az + ez_;
+    // byte arrays: qlo x = (w1.x, w1.y), qlo y = (w1.z, w1.w), qlo z = (w2.x, w2.y),
+    //              qhi x = (w2.z, w2.w), qhi y = (w3.x, w3.y), qhi z = (w3.z, w3.w)
+    const bool nx = s.ix < 0.0f, ny = s.iy < 0.0f, nz = s.iz < 0.0f;
+    const uint32_t nxa = nx ? w2.z : w1.x, nxb = nx ? w2.w : w1.y, fxa = nx ? w1.x : w2.z, fxb = nx ? w1.y : w2.w;
+    const uint32_t nya = ny ? w3.x : w1.z, nyb = ny ? w3.y : w1.w, fya = ny ? w1.z : w3.x, fyb = ny ? w1.w : w3.y;
+    const uint32_t nza = nz ? w3.z : w2.x, nzb = nz ? w3.w : w2.y, fza = nz ? w2.x : w3.z, fzb = nz ? w2.y : w3.w;
+    uint32_t hit = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int b = k & 3;
+        const uint32_t wnx = k < 4 ? nxa : nxb, wfx = k < 4 ? fxa : fxb;
+        const uint32_t wny = k < 4 ? nya : nyb, wfy = k < 4 ? fya : fyb;
+        const uint32_t wnz = k < 4 ? nza : nzb, wfz = k < 4 ? fza : fzb;
+        const float tnx = fmaf(byte_f(wnx, b), Sx, Anx);
+        const float tny = fmaf(byte_f(wny, b), Sy, Any);
+        const float tnz = fmaf(byte_f(wnz, b), Sz, Anz);
+        const float tfx = fmaf(byte_f(wfx, b), Sx, Afx);
+        const float tfy = fmaf(byte_f(wfy, b), Sy, Afy);
+        const float tfz = fmaf(byte_f(wfz, b), Sz, Afz);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, smin));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, smax));
+        hit |= (tn <= tf) ? (1u << k) : 0u;
+    }
+    return hit & vmask;
+}
+
+struct Shear {
+    float rx0, rx1, rx2, ry0, ry1, ry2, rz0, rz1, rz2;
+};
+
+__device__ __forceinline__ void setup_shear(const Space& s, Shear& sh) {
+    const float ax = fabsf(s.dx), ay = fabsf(s.dy), az = fabsf(s.dz);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    const float dkz = kz == 0 ? s.dx : (kz == 1 ? s.dy : s.dz);
+    if (dkz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    const float dkx = kx == 0 ? s.dx : (kx == 1 ? s.dy : s.dz);
+    const float dky = ky == 0 ? s.dx : (ky == 1 ? s.dy : s.dz);
+    const float Sx = dkx / dkz, Sy = dky / dkz, Sz = 1.0f / dkz;
+    sh.rx0 = kx == 0 ? 1.0f : (kz == 0 ? -Sx : 0.0f);
+    sh.rx1 = kx == 1 ? 1.0f : (kz == 1 ? -Sx : 0.0f);
+    sh.rx2 = kx == 2 ? 1.0f : (kz == 2 ? -Sx : 0.0f);
+    sh.ry0 = ky == 0 ? 1.0f : (kz == 0 ? -Sy : 0.0f);
+    sh.ry1 = ky == 1 ? 1.0f : (kz == 1 ? -Sy : 0.0f);
+    sh.ry2 = ky == 2 ? 1.0f : (kz == 2 ? -Sy : 0.0f);
+    sh.rz0 = kz == 0 ? Sz : 0.0f;
+    sh.rz1 = kz == 1 ? Sz : 0.0f;
+    sh.rz2 = kz == 2 ? Sz : 0.0f;
+}
+
+// Watertight test in the current (rebased object) space.  Returns true and the parametric distance s when
+// the triangle is hit with s in [smin, smax].
+__device__ __forceinline__ bool test_triangle(const uint4 a, const uint4 b, const uint4 c, const Space& s,
+                                              const Shear& sh, float smin, float smax, float& t_out) {
+    const float v0x = __uint_as_float(a.x) - s.ox, v0y = __uint_as_float(a.y) - s.oy, v0z = __uint_as_float(a.z) - s.oz;
+    const float v1x = __uint_as_float(a.w) - s.ox, v1y = __uint_as_float(b.x) - s.oy, v1z = __uint_as_float(b.y) - s.oz;
+    const float v2x = __uint_as_float(b.z) - s.ox, v2y = __uint_as_float(b.w) - s.oy, v2z = __uint_as_float(c.x) - s.oz;
+    const float Ax = fmaf(v0z, sh.rx2, fmaf(v0y, sh.rx1, v0x * sh.rx0));
+    const float Ay = fmaf(v0z, sh.ry2, fmaf(v0y, sh.ry1, v0x * sh.ry0));
+    const float Az = fmaf(v0z, sh.rz2, fmaf(v0y, sh.rz1, v0x * sh.rz0));
+    const float Bx = fmaf(v1z, sh.rx2, fmaf(v1y, sh.rx1, v1x * sh.rx0));
+    const float By = fmaf(v1z, sh.ry2, fmaf(v1y, sh.ry1, v1x * sh.ry0));
+    const float Bz = fmaf(v1z, sh.rz2, fmaf(v1y, sh.rz1, v1x * sh.rz0));
+    const float Cx = fmaf(v2z, sh.rx2, fmaf(v2y, sh.rx1, v2x * sh.rx0));
+    const float Cy = fmaf(v2z, sh.ry2, fmaf(v2y, sh.ry1, v2x * sh.ry0));
+    const float Cz = fmaf(v2z, sh.rz2, fmaf(v2y, sh.rz1, v2x * sh.rz0));
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * Az + V * Bz + W * Cz;
+    const float adet = fabsf(det);
+    const float Ts = det < 0.0f ? -T : T;
+    if (Ts < smin * adet || Ts > smax * adet) return false;
+    t_out = Ts / adet;
+    return true;
+}
+
+struct TraceArgs {
+    const uint8_t* tlas;
+    const rtx_ray* rays;
+    uint32_t n;
+    uint32_t cull_mask;
+    rtx_hit* hits;
+    uint32_t* occluded;
+    // raygen
+    float cam[3];
+    float ax, ay;
+    uint32_t width, height;
+    float tmin, tmax;
+    const rtx_hit_record* records;
+    uint32_t nrec, sbt_offset, sbt_stride;
+    float miss[3];
+    float* film;
+    // runtime
+    DeviceState* state;
+    uint2* gstack;
+    uint32_t lds_depth;
+    uint32_t gdepth;
+};
+
+enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2 };
+
+// FP64 recomputation of (t, u, v) for the winning triangle (Moller & Trumbore 1997) in object space.
+__device__ void refine_hit(const double* inv, const float* wo, const float* wd, const TriSlot* tri, float& t,
+                           float& u, float& v) {
+    double o[3], d[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        o[r] = inv[4 * r] * wo[0] + inv[4 * r + 1] * wo[1] + inv[4 * r + 2] * wo[2] + inv[4 * r + 3];
+        d[r] = inv[4 * r] * wd[0] + inv[4 * r + 1] * wd[1] + inv[4 * r + 2] * wd[2];
+    }
+    double v0[3], e1[3], e2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        v0[a] = tri->v[a];
+        e1[a] = (double)tri->v[3 + a] - v0[a];
+        e2[a] = (double)tri->v[6 + a] - v0[a];
+    }
+    double p[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    double inv_det = 1.0 / det;
+    double sv[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+    double uu = (sv[0] * p[0] + sv[1] * p[1] + sv[2] * p[2]) * inv_det;
+    double q[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2], sv[0] * e1[1] - sv[1] * e1[0]};
+    double vv = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv_det;
+    double tt = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv_det;
+    t = (float)tt;
+    u = (float)uu;
+    v = (float)vv;
+}
+
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
+    extern __shared__ uint2 lds_stack[];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t gtid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const AsHeader* th = (const AsHeader*)args.tlas;
+    const uint8_t* tslots = args.tlas + kHeaderBytes;
+    const InstanceInfo* table = (const InstanceInfo*)(args.tlas + th->inst_table_offset);
+    const float trlo[3] = {th->lo[0], th->lo[1], th->lo[2]};
+    const float tqs[3] = {th->qscale[0], th->qscale[1], th->qscale[2]};
+    const bool tlas_empty = th->slot_count == 0;
+    const uint32_t tlas_nslots = th->slot_count;
+    const uint32_t tlas_ninst = th->prim_count;
+    const uint32_t D = args.lds_depth;
+    const uint32_t GD = args.gdepth;
+
+    uint64_t st_nodes = 0, st_tris = 0, st_insts = 0, st_rays = 0;
+
+    bool active = false, exhausted = false;
+    uint32_t ray_id = 0;
+    float wo[3], wd[3], wtmin = 0.0f, tc = 0.0f;
+    Space s;
+    Shear sh;
+    const uint8_t* cslots = tslots;
+    float crlo[3], cqs[3];
+    bool in_blas = false;
+    uint32_t cur_inst = 0;
+    uint32_t cur_nslots = tlas_nslots;
+    uint32_t steps = 0;
+    uint32_t ng_base = 0, ng_bits = 0;  // bits: ordered mask | vmask << 8
+    uint32_t lg_base = 0, lg_bits = 0;  // bits: child-position mask | vmask << 8
+    uint32_t sp = 0;
+    uint32_t hit_inst = kNoHit, hit_slot = 0;
+    bool occl = false;
+
+    for (;;) {
+        const uint64_t need = __ballot(!active && !exhausted);
+        if (need) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&args.state->ray_counter, (uint32_t)__popcll(need));
+            base = __shfl(base, leader);
+            if (!active && !exhausted) {
+                const uint32_t id = base + (uint32_t)__popcll(need & lt_mask);
+                if (id < args.n) {
+                    ray_id = id;
+                    active = true;
+                    if (MODE == kModeRaygen) {
+                        const uint32_t x = id % args.width, y = id / args.width;
+                        const float sx = ((float)x + 0.5f) / (float)args.width;
+                        const float sy = ((float)y + 0.5f) / (float)args.height;
+                        const float nx = sx * 2.0f - 1.0f;
+                        const float ny = sy * 2.0f - 1.0f;
+                        wo[0] = args.cam[0];
+                        wo[1] = args.cam[1];
+                        wo[2] = args.cam[2];
+                        wd[0] = nx * args.ax;
+                        wd[1] = -(ny * args.ay);
+                        wd[2] = -1.0f;
+                        wtmin = args.tmin;
+                        tc = args.tmax;
+                    } else {
+                        const float4* rp = (const float4*)(args.rays + id);
+                        const float4 r0 = rp[0], r1 = rp[1];
+                        wo[0] = r0.x;
+                        wo[1] = r0.y;
+                        wo[2] = r0.z;
+                        wtmin = r0.w;
+                        wd[0] = r1.x;
+                        wd[1] = r1.y;
+                        wd[2] = r1.z;
+                        tc = r1.w;
+                    }
+                    s.ox = wo[0];
+                    s.oy = wo[1];
+                    s.oz = wo[2];
+                    s.dx = wd[0];
+                    s.dy = wd[1];
+                    s.dz = wd[2];
+                    s.toff = 0.0f;
+                    setup_space(s);
+                    cslots = tslots;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        crlo[a] = trlo[a];
+                        cqs[a] = tqs[a];
+                    }
+                    in_blas = false;
+                    cur_nslots = tlas_nslots;
+                    steps = 0;
+                    ng_base = 0;
+                    ng_bits = tlas_empty ? 0u : (1u | (1u << 8));  // the root: slot 0, one valid child
+                    lg_bits = 0;
+                    sp = 0;
+                    hit_inst = kNoHit;
+                    occl = false;
+                    if (STATS) ++st_rays;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (!__any(active)) break;
+        const bool wave_exhausted = __any(exhausted);
+
+        int iter = 0;
+        while (active) {
+            bool done = false;
+            if (++steps > kMaxSteps) {
+                atomicOr(&args.state->error_flags, kErrBadAs);
+                lg_bits = 0;
+                ng_bits = 0;
+                sp = 0;
+            }
+            if (lg_bits & 255u) {
+                if (!in_blas) {
+                    // instance leaf: enter its BLAS
+                    const uint32_t m = lg_bits & 255u;
+                    const uint32_t k = (uint32_t)(__ffs(m) - 1);
+                    const uint32_t vm = lg_bits >> 8;
+                    lg_bits &= ~(1u << k);
+                    const uint32_t slot = lg_base + (uint32_t)__popc(vm & ((1u << k) - 1u));
+                    uint4 q = make_uint4(0u, 0u, 0u, 0u);
+                    if (slot < tlas_nslots) {
+                        q = *(const uint4*)(tslots + (uint64_t)slot * kSlotBytes + 48);
+                        if (q.z >= tlas_ninst) q.w = 0u;
+                    } else {
+                        atomicOr(&args.state->error_flags, kErrBadAs);
+                    }
+                    const uint64_t blas = (uint64_t)q.x | ((uint64_t)q.y << 32);
+                    const uint32_t iid = q.z, imask = q.w;
+                    if ((imask & args.cull_mask) != 0u) {
+                        if (STATS) ++st_insts;
+                        // push the remaining TLAS work
+                        if (lg_bits & 255u) {
+                            const uint2 e = make_uint2(lg_base, (lg_bits & 0xFFFFu) | kEntryLeaf | kEntryTlas);
+                            if (sp < D) lds_stack[sp * blockDim.x + threadIdx.x] = e;
+                            else if (sp < D + GD) args.gstack[gtid * GD + (sp - D)] = e;
+                            ++sp;
+                        }
+                        if (ng_bits & 255u) {
+                            const uint2 e = make_uint2(ng_base, (ng_bits & 0xFFFFu) | kEntryTlas);
+                            if (sp < D) lds_stack[sp * blockDim.x + threadIdx.x] = e;
+                            else if (sp < D + GD) args.gstack[gtid * GD + (sp - D)] = e;
+                            ++sp;
+                        }
+                        lg_bits = 0;
+                        ng_bits = 0;
+                        if (sp > D + GD) {
+                            atomicOr(&args.state->error_flags, kErrStackOverflow);
+                            done = true;
+                        } else {
+                            const InstanceInfo* info = table + iid;
+                            const AsHeader* bh = (const AsHeader*)blas;
+                            double M[12];
+#pragma unroll
+                            for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
+                            double od[3], dd[3];
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) {
+                                od[r] = M[4 * r] * wo[0] + M[4 * r + 1] * wo[1] + M[4 * r + 2] * wo[2] + M[4 * r + 3];
+                                dd[r] = M[4 * r] * wd[0] + M[4 * r + 1] * wd[1] + M[4 * r + 2] * wd[2];
+                            }
+                            const float4 blo = *(const float4*)bh->lo;
+                            const float4 bhi = *(const float4*)bh->hi;
+                            const float4 bqs = *(const float4*)bh->qscale;
+                            const double lo3[3] = {blo.x, blo.y, blo.z}, hi3[3] = {bhi.x, bhi.y, bhi.z};
+                            double tn = -INFINITY;
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                const double pad = (hi3[a] - lo3[a]) * (1.0 / 64.0) + 1e-30;
+                                if (dd[a] != 0.0) {
+                                    const double r = 1.0 / dd[a];
+                                    const double t1 = (lo3[a] - pad - od[a]) * r, t2 = (hi3[a] + pad - od[a]) * r;
+                                    tn = fmax(tn, fmin(t1, t2));
+                                }
+                            }
+                            const double te = fmax((double)wtmin, tn);
+                            s.ox = (float)(od[0] + te * dd[0]);
+                            s.oy = (float)(od[1] + te * dd[1]);
+                            s.oz = (float)(od[2] + te * dd[2]);
+                            s.dx = (float)dd[0];
+                            s.dy = (float)dd[1];
+                            s.dz = (float)dd[2];
+                            s.toff = (float)te;
+                            setup_space(s);
+                            setup_shear(s, sh);
+                            cslots = (const uint8_t*)blas + kHeaderBytes;
+                            crlo[0] = blo.x;
+                            crlo[1] = blo.y;
+                            crlo[2] = blo.z;
+                            cqs[0] = bqs.x;
+                            cqs[1] = bqs.y;
+                            cqs[2] = bqs.z;
+                            in_blas = true;
+                            cur_inst = iid;
+                            cur_nslots = bh->slot_count;
+                            ng_base = 0;
+                            ng_bits = cur_nslots ? (1u | (1u << 8)) : 0u;
+                        }
+                    }
+                } else {
+                    // triangles of the leaf group
+                    const uint32_t vm = lg_bits >> 8;
+                    uint32_t m = lg_bits & 255u;
+                    lg_bits = 0;
+                    const float smin = wtmin - s.toff;
+                    while (m) {
+                        const uint32_t k = (uint32_t)(__ffs(m) - 1);
+                        m &= m - 1u;
+                        const uint32_t slot = lg_base + (uint32_t)__popc(vm & ((1u << k) - 1u));
+                        if (slot >= cur_nslots) {
+                            atomicOr(&args.state->error_flags, kErrBadAs);
+                            break;
+                        }
+                        const uint4* tp = (const uint4*)(cslots + (uint64_t)slot * kSlotBytes);
+                        const uint4 a = tp[0], b = tp[1], c = tp[2];
+                        if (STATS) ++st_tris;
+                        float th;
+                        const float smax = tc - s.toff;
+                        if (test_triangle(a, b, c, s, sh, smin, smax, th)) {
+                            if (MODE == kModeAny) {
+                                occl = true;
+                                done = true;
+                                break;
+                            }
+                            tc = s.toff + th;
+                            hit_inst = cur_inst;
+                            hit_slot = slot;
+                        }
+                    }
+                }
+            } else if (ng_bits & 255u) {
+                const uint32_t om = ng_bits & 255u;
+                const uint32_t j = (uint32_t)(__ffs(om) - 1);
+                const uint32_t k = j ^ s.oct;
+                const uint32_t vm = (ng_bits >> 8) & 255u;
+                const uint32_t slot = ng_base + (uint32_t)__popc(vm & ((1u << k) - 1u));
+                ng_bits &= ~(1u << j);
+                if (ng_bits & 255u) {
+                    const uint2 e = make_uint2(ng_base, (ng_bits & 0xFFFFu) | (in_blas ? 0u : kEntryTlas));
+                    if (sp < D) lds_stack[sp * blockDim.x + threadIdx.x] = e;
+                    else if (sp < D + GD) args.gstack[gtid * GD + (sp - D)] = e;
+                    ++sp;
+                    if (sp > D + GD) {
+                        atomicOr(&args.state->error_flags, kErrStackOverflow);
+                        done = true;
+                    }
+                }
+                if (!done && slot >= cur_nslots) {
+                    atomicOr(&args.state->error_flags, kErrBadAs);
+                    ng_bits = 0;
+                    lg_bits = 0;
+                    done = sp == 0;
+                } else if (!done) {
+                    const uint4* np = (const uint4*)(cslots + (uint64_t)slot * kSlotBytes);
+                    const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+                    if (STATS) ++st_nodes;
+                    const float smin = wtmin - s.toff;
+                    const float smax = tc - s.toff;
+                    const uint32_t hm = test_node(w0, w1, w2, w3, s, crlo, cqs, smin, smax);
+                    const uint32_t imask = (w0.z >> 8) & 255u;
+                    const uint32_t vmask = (w0.z >> 16) & 255u;
+                    ng_base = w0.w;
+                    ng_bits = oct_permute(hm & imask, s.oct) | (vmask << 8);
+                    lg_base = w0.w;
+                    lg_bits = (hm & ~imask) | (vmask << 8);
+                }
+            } else if (sp > 0) {
+                --sp;
+                const uint2 e = sp < D ? lds_stack[sp * blockDim.x + threadIdx.x] : args.gstack[gtid * GD + (sp - D)];
+                if ((e.y & kEntryTlas) && in_blas) {
+                    in_blas = false;
+                    s.ox = wo[0];
+                    s.oy = wo[1];
+                    s.oz = wo[2];
+                    s.dx = wd[0];
+                    s.dy = wd[1];
+                    s.dz = wd[2];
+                    s.toff = 0.0f;
+                    setup_space(s);
+                    cslots = tslots;
+                    cur_nslots = tlas_nslots;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        crlo[a] = trlo[a];
+                        cqs[a] = tqs[a];
+                    }
+                }
+                if (e.y & kEntryLeaf) {
+                    lg_base = e.x;
+                    lg_bits = e.y & 0xFFFFu;
+                } else {
+                    ng_base = e.x;
+                    ng_bits = e.y & 0xFFFFu;
+                }
+            } else {
+                done = true;
+            }
+
+            if (done) {
+                // write the result of ray_id
+                if (MODE == kModeAny) {
+                    args.occluded[ray_id] = occl ? 1u : 0u;
+                } else {
+                    float t = 0.0f, u = 0.0f, v = 0.0f;
+                    uint32_t prim = 0, geom = 0;
+                    const InstanceInfo* info = nullptr;
+                    if (hit_inst != kNoHit) {
+                        info = table + hit_inst;
+                        double M[12];
+#pragma unroll
+                        for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
+                        const TriSlot* tri =
+                            (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes + (uint64_t)hit_slot * kSlotBytes);
+                        refine_hit(M, wo, wd, tri, t, u, v);
+                        prim = tri->prim;
+                        geom = tri->geom;
+                    }
+                    if (MODE == kModeClosest) {
+                        uint32_t* hp = (uint32_t*)(args.hits + ray_id);
+                        hp[0] = __float_as_uint(t);
+                        hp[1] = __float_as_uint(u);
+                        hp[2] = __float_as_uint(v);
+                        hp[3] = prim;
+                        hp[4] = geom;
+                        hp[5] = hit_inst;
+                    } else {
+                        float r = args.miss[0], g = args.miss[1], bl = args.miss[2];
+                        if (hit_inst != kNoHit) {
+                            const uint32_t rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
+                            if (rec < args.nrec) {
+                                const rtx_hit_record hr = args.records[rec];
+                                bool bary = hr.program == RTX_PROGRAM_BARYCENTRIC;
+                                if (hr.program == RTX_PROGRAM_MATCH_BARYCENTRIC)
+                                    bary = (int32_t)prim == hr.match_primitive && (int32_t)hit_inst == hr.match_instance &&
+                                           (int32_t)info->custom == hr.match_custom_index &&
+                                           (int32_t)geom == hr.match_geometry;
+                                if (bary) {
+                                    r = 1.0f - u - v;
+                                    g = u;
+                                    bl = v;
+                                } else {
+                                    r = hr.color[0];
+                                    g = hr.color[1];
+                                    bl = hr.color[2];
+                                }
+                            } else {
+                                r = g = bl = 0.0f;
+                            }
+                        }
+                        *(float4*)(args.film + 4 * (uint64_t)ray_id) = make_float4(r, g, bl, 0.0f);
+                    }
+                }
+                active = false;
+                break;
+            }
+            if (++iter >= 16) {
+                iter = 0;
+                if (!wave_exhausted && __popcll(__ballot(true)) < 40) break;
+            }
+        }
+    }
+
+    if (STATS) {
+        atomicAdd(&args.state->stat_nodes, (unsigned long long)st_nodes);
+        atomicAdd(&args.state->stat_tris, (unsigned long long)st_tris);
+        atomicAdd(&args.state->stat_insts, (unsigned long long)st_insts);
+        atomicAdd(&args.state->stat_rays, (unsigned long long)st_rays);
+    }
+}
+
+__global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->ray_counter = 0;
+        if (clear_stats) {
+            st->stat_nodes = 0;
+            st->stat_tris = 0;
+            st->stat_insts = 0;
+            st->stat_rays = 0;
+        }
+    }
+}
+
+template <int MODE, bool STATS>
+hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
+    const size_t lds = (size_t)L.lds_depth * L.block_size * sizeof(uint2);
+    trace_kernel<MODE, STATS><<<L.grid, L.block_size, lds, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStream_t s) {
+    TraceArgs a{};
+    a.tlas = (const uint8_t*)r.tlas;
+    a.rays = r.rays;
+    a.n = r.n;
+    a.cull_mask = r.cull_mask;
+    a.hits = r.hits;
+    a.occluded = r.occluded;
+    for (int k = 0; k < 3; ++k) {
+        a.cam[k] = r.cam[k];
+        a.miss[k] = r.miss[k];
+    }
+    a.ax = r.ax;
+    a.ay = r.ay;
+    a.width = r.width;
+    a.height = r.height;
+    a.tmin = r.tmin;
+    a.tmax = r.tmax;
+    a.records = r.records;
+    a.nrec = r.nrec;
+    a.sbt_offset = r.sbt_offset;
+    a.sbt_stride = r.sbt_stride;
+    a.film = r.film;
+    a.state = r.state;
+    a.gstack = r.gstack;
+    a.lds_depth = L.lds_depth;
+    a.gdepth = L.gdepth;
+    reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
+    if (r.n == 0) return hipGetLastError();
+    switch (r.mode) {
+        case 0: return r.stats ? launch<kModeClosest, true>(a, L, s) : launch<kModeClosest, false>(a, L, s);
+        case 1: return r.stats ? launch<kModeAny, true>(a, L, s) : launch<kModeAny, false>(a, L, s);
+        default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
+    }
+}
+
+}  // namespace rtx

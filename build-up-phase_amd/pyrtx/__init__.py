@@ -1,0 +1,383 @@
+"""pyrtx — Python binding of librtx.so (the C ABI in include/rtx.h) for tests and benchmarks.
+
+The calls follow the Vulkan KHR ray-tracing flow of vulkan-raytraced-triangle/main.cpp:
+    ctx.build_blas(geometries)          ~ createBLAS        (main.cpp:674-831)
+    ctx.build_tlas(instance records)    ~ createTLAS        (main.cpp:833-949)
+    ctx.trace_rays(tlas, sbt, camera)   ~ vkCmdTraceRaysKHR (main.cpp:1349-1355)
+Device memory comes from torch (HIP) tensors; work is recorded on torch's current stream.  There is no CPU
+fallback: if librtx.so is missing or no GPU is present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "librtx.so")
+
+RTX_SUCCESS = 0
+RTX_GEOMETRY_OPAQUE_BIT = 0x1
+RTX_BUILD_PREFER_FAST_TRACE_BIT = 0x4
+RTX_BUILD_PREFER_FAST_BUILD_BIT = 0x8
+RTX_BUILD_MORTON63_BIT = 0x10000
+RTX_BUILD_NO_TREELET_BIT = 0x20000
+RTX_INDEX_TYPE_UINT32 = 1
+RTX_QUERY_CLOSEST_HIT = 0
+RTX_QUERY_ANY_HIT = 1
+RTX_RAY_FLAG_OPAQUE = 0x1
+MISS = 0xFFFFFFFF
+
+ERRORS = {-1: "OUT_OF_HOST_MEMORY", -2: "OUT_OF_DEVICE_MEMORY", -3: "INITIALIZATION_FAILED", -4: "DEVICE_LOST",
+          -1000: "INVALID_ARGUMENT", -1001: "STACK_OVERFLOW", -1002: "INVALID_AS"}
+
+
+class RtxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rtx error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+# ---- C structures (include/rtx.h) -------------------------------------------------------------------------------
+class BlasGeometry(ctypes.Structure):
+    _fields_ = [("vertex_data_device", ctypes.c_void_p), ("vertex_stride", ctypes.c_uint64),
+                ("max_vertex", ctypes.c_uint32), ("index_type", ctypes.c_uint32),
+                ("index_data_device", ctypes.c_void_p), ("transform_data_device", ctypes.c_void_p),
+                ("primitive_count", ctypes.c_uint32), ("primitive_offset", ctypes.c_uint32),
+                ("first_vertex", ctypes.c_uint32), ("transform_offset", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class BuildSizes(ctypes.Structure):
+    _fields_ = [("acceleration_structure_size", ctypes.c_uint64), ("build_scratch_size", ctypes.c_uint64),
+                ("update_scratch_size", ctypes.c_uint64)]
+
+
+class AsInfo(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("primitive_count", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
+                ("slot_count", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("geometry_count", ctypes.c_uint32),
+                ("bounds", ctypes.c_float * 6), ("sah_cost", ctypes.c_double)]
+
+
+class Sbt(ctypes.Structure):
+    _fields_ = [("hit_records_device", ctypes.c_void_p), ("hit_record_count", ctypes.c_uint32),
+                ("sbt_record_offset", ctypes.c_uint32), ("sbt_record_stride", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("miss_color", ctypes.c_float * 3),
+                ("miss_alpha_reserved", ctypes.c_float)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_float * 3), ("y_fov_degree", ctypes.c_float)]
+
+
+class TraceParams(ctypes.Structure):
+    _fields_ = [("ray_flags", ctypes.c_uint32), ("cull_mask", ctypes.c_uint32), ("tmin", ctypes.c_float),
+                ("tmax", ctypes.c_float)]
+
+
+class TraceStats(ctypes.Structure):
+    _fields_ = [("rays", ctypes.c_uint64), ("node_fetches", ctypes.c_uint64), ("triangle_tests", ctypes.c_uint64),
+                ("instance_entries", ctypes.c_uint64)]
+
+
+class TraceConfig(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
+                ("lds_stack_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+# symbol -> (restype, argtypes); the exported surface of include/rtx.h
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+SIGNATURES = {
+    "rtx_create_context": (ctypes.c_int, [ctypes.c_int, _P, ctypes.POINTER(_P)]),
+    "rtx_destroy_context": (ctypes.c_int, [_P]),
+    "rtx_last_error_string": (ctypes.c_char_p, [_P]),
+    "rtx_sync": (ctypes.c_int, [_P]),
+    "rtx_get_stream": (_P, [_P]),
+    "rtx_get_blas_build_sizes": (ctypes.c_int, [_P, _P, _U32, _U32, ctypes.POINTER(BuildSizes)]),
+    "rtx_get_tlas_build_sizes": (ctypes.c_int, [_P, _U32, _U32, ctypes.POINTER(BuildSizes)]),
+    "rtx_cmd_build_blas": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _U64, _P, _U64, ctypes.POINTER(_U64)]),
+    "rtx_cmd_build_tlas": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _U64, _P, _U64, ctypes.POINTER(_U64)]),
+    "rtx_get_as_info": (ctypes.c_int, [_P, _U64, ctypes.POINTER(AsInfo)]),
+    "rtx_cmd_trace_rays": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), ctypes.POINTER(Camera),
+                                          ctypes.POINTER(TraceParams), _U32, _U32, _P]),
+    "rtx_cmd_trace_ray_buffer": (ctypes.c_int, [_P, _U64, _P, _U32, _U32, _U32, _P, _P]),
+    "rtx_set_trace_stats": (ctypes.c_int, [_P, ctypes.c_int]),
+    "rtx_get_trace_stats": (ctypes.c_int, [_P, ctypes.POINTER(TraceStats)]),
+    "rtx_set_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
+    "rtx_get_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
+    "rtx_debug_lbvh": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _P]),
+    "rtx_debug_sort": (ctypes.c_int, [_P, _P, _P, _U32, _U32]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load librtx.so and bind every C-ABI symbol.  Raises if the library is missing."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RtxError(-3, f"{path} not built: run __graft_entry__.build() or make -C {PKG_DIR}")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if path == LIB_PATH:
+        _lib = L
+    return L
+
+
+# ---- host mirror of the reference flow ---------------------------------------------------------------------------
+@dataclass
+class AccelerationStructure:
+    storage: object     # torch uint8 tensor holding the AS
+    address: int
+    count: int          # triangles or instances
+    keepalive: tuple = ()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dev_tensor(arr, device):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(arr)).to(device)
+
+
+class Context:
+    """One rtx_context on a HIP device, recording on torch's current stream."""
+
+    def __init__(self, device: int = 0, stream=None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RtxError(-3, "no GPU: librtx has no CPU path")
+        self.lib = load_library()
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
+        h = ctypes.c_void_p()
+        self._check(self.lib.rtx_create_context(device, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h)))
+        self.handle = h
+
+    def _check(self, code):
+        if code != RTX_SUCCESS:
+            msg = self.lib.rtx_last_error_string(self.handle).decode() if getattr(self, "handle", None) else ""
+            raise RtxError(code, msg)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.rtx_destroy_context(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self._check(self.lib.rtx_sync(self.handle))
+
+    # -- acceleration structures ---------------------------------------------------------------------------------
+    def _geometry_structs(self, geoms):
+        """geoms: list of dicts with device tensors: vertices (float32 [V,3]), indices (uint32/int32 [T,3]),
+        transform (float32 [3,4]) or None."""
+        arr = (BlasGeometry * max(1, len(geoms)))()
+        for i, g in enumerate(geoms):
+            v, ix, x = g["vertices"], g["indices"], g.get("transform")
+            arr[i].vertex_data_device = v.data_ptr()
+            arr[i].vertex_stride = 12
+            arr[i].max_vertex = max(0, v.shape[0] - 1)
+            arr[i].index_type = RTX_INDEX_TYPE_UINT32
+            arr[i].index_data_device = ix.data_ptr()
+            arr[i].transform_data_device = x.data_ptr() if x is not None else None
+            arr[i].primitive_count = ix.numel() // 3
+            arr[i].primitive_offset = 0
+            arr[i].first_vertex = 0
+            arr[i].transform_offset = 0
+            arr[i].flags = RTX_GEOMETRY_OPAQUE_BIT
+        return arr
+
+    def blas_build_sizes(self, geoms, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
+        arr = self._geometry_structs(geoms)
+        sz = BuildSizes()
+        self._check(self.lib.rtx_get_blas_build_sizes(self.handle, ctypes.cast(arr, _P), len(geoms), flags,
+                                                      ctypes.byref(sz)))
+        return sz.acceleration_structure_size, sz.build_scratch_size
+
+    def build_blas(self, geoms, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT, scratch=None) -> AccelerationStructure:
+        torch = _torch()
+        arr = self._geometry_structs(geoms)
+        sz = BuildSizes()
+        self._check(self.lib.rtx_get_blas_build_sizes(self.handle, ctypes.cast(arr, _P), len(geoms), flags,
+                                                      ctypes.byref(sz)))
+        storage = torch.empty(int(sz.acceleration_structure_size), dtype=torch.uint8, device=self.device)
+        if scratch is None or scratch.numel() < sz.build_scratch_size:
+            scratch = torch.empty(int(sz.build_scratch_size), dtype=torch.uint8, device=self.device)
+        addr = ctypes.c_uint64()
+        self._check(self.lib.rtx_cmd_build_blas(self.handle, ctypes.cast(arr, _P), len(geoms), flags,
+                                                storage.data_ptr(), storage.numel(), scratch.data_ptr(),
+                                                scratch.numel(), ctypes.byref(addr)))
+        n = sum(g["indices"].numel() // 3 for g in geoms)
+        return AccelerationStructure(storage, addr.value, n, (scratch, tuple(geoms)))
+
+    def build_tlas(self, records, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT, scratch=None) -> AccelerationStructure:
+        """records: numpy structured array of 64-B instance records (oracle-compatible layout) or a device uint8
+        tensor of n*64 bytes."""
+        torch = _torch()
+        if isinstance(records, np.ndarray):
+            raw = np.ascontiguousarray(records).view(np.uint8).reshape(-1)
+            inst = _dev_tensor(raw, self.device)
+        else:
+            inst = records
+        n = inst.numel() // 64
+        sz = BuildSizes()
+        self._check(self.lib.rtx_get_tlas_build_sizes(self.handle, n, flags, ctypes.byref(sz)))
+        storage = torch.empty(int(sz.acceleration_structure_size), dtype=torch.uint8, device=self.device)
+        if scratch is None or scratch.numel() < sz.build_scratch_size:
+            scratch = torch.empty(int(sz.build_scratch_size), dtype=torch.uint8, device=self.device)
+        addr = ctypes.c_uint64()
+        self._check(self.lib.rtx_cmd_build_tlas(self.handle, inst.data_ptr(), n, flags, storage.data_ptr(),
+                                                storage.numel(), scratch.data_ptr(), scratch.numel(),
+                                                ctypes.byref(addr)))
+        return AccelerationStructure(storage, addr.value, n, (scratch, inst))
+
+    def as_info(self, a: AccelerationStructure) -> AsInfo:
+        info = AsInfo()
+        self._check(self.lib.rtx_get_as_info(self.handle, a.address, ctypes.byref(info)))
+        return info
+
+    # -- tracing --------------------------------------------------------------------------------------------------
+    def trace_rays(self, tlas: AccelerationStructure, records, camera, width, height, miss_color=(0.0, 0.0, 0.2),
+                   tmin=0.0, tmax=100.0, cull_mask=0xFF, sbt_offset=0, sbt_stride=1, film=None):
+        """vkCmdTraceRaysKHR with the reference raygen (main.cpp:1033-1055) into an RGBA32F film [H, W, 4]."""
+        torch = _torch()
+        if isinstance(records, np.ndarray):
+            rec = _dev_tensor(np.ascontiguousarray(records).view(np.uint8).reshape(-1), self.device)
+        else:
+            rec = records
+        nrec = rec.numel() // 48
+        if film is None:
+            film = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
+        sbt = Sbt(rec.data_ptr() if nrec else None, nrec, sbt_offset, sbt_stride, 0,
+                  (ctypes.c_float * 3)(*miss_color), 0.0)
+        cam = Camera((ctypes.c_float * 3)(*camera[:3]), float(camera[3]))
+        params = TraceParams(RTX_RAY_FLAG_OPAQUE, cull_mask, tmin, tmax)
+        self._check(self.lib.rtx_cmd_trace_rays(self.handle, tlas.address, ctypes.byref(sbt), ctypes.byref(cam),
+                                                ctypes.byref(params), width, height, film.data_ptr()))
+        self._keep = (rec,)
+        return film
+
+    def trace_ray_buffer(self, tlas: AccelerationStructure, rays, mode=RTX_QUERY_CLOSEST_HIT, cull_mask=0xFF,
+                         out=None):
+        """rays: device float32 tensor [N, 8].  Returns hits (uint32 view [N, 6]: t, u, v as float bits, prim,
+        geom, inst) for closest-hit, or occluded uint32 [N] for any-hit."""
+        torch = _torch()
+        n = rays.shape[0]
+        if mode == RTX_QUERY_CLOSEST_HIT:
+            if out is None:
+                out = torch.empty((n, 6), dtype=torch.int32, device=self.device)
+            self._check(self.lib.rtx_cmd_trace_ray_buffer(self.handle, tlas.address, rays.data_ptr(), n, mode,
+                                                          cull_mask, out.data_ptr(), None))
+        else:
+            if out is None:
+                out = torch.empty((n,), dtype=torch.int32, device=self.device)
+            self._check(self.lib.rtx_cmd_trace_ray_buffer(self.handle, tlas.address, rays.data_ptr(), n, mode,
+                                                          cull_mask, None, out.data_ptr()))
+        return out
+
+    def set_stats(self, enable: bool):
+        self._check(self.lib.rtx_set_trace_stats(self.handle, int(enable)))
+
+    def stats(self) -> dict:
+        s = TraceStats()
+        self._check(self.lib.rtx_get_trace_stats(self.handle, ctypes.byref(s)))
+        return {"rays": s.rays, "node_fetches": s.node_fetches, "triangle_tests": s.triangle_tests,
+                "instance_entries": s.instance_entries}
+
+    def set_config(self, block_size=0, blocks_per_cu=0, lds_stack_depth=0):
+        c = TraceConfig(block_size, blocks_per_cu, lds_stack_depth, 0)
+        self._check(self.lib.rtx_set_trace_config(self.handle, ctypes.byref(c)))
+
+    def config(self) -> dict:
+        c = TraceConfig()
+        self._check(self.lib.rtx_get_trace_config(self.handle, ctypes.byref(c)))
+        return {"block_size": c.block_size, "blocks_per_cu": c.blocks_per_cu, "lds_stack_depth": c.lds_stack_depth}
+
+    # -- verification hooks ----------------------------------------------------------------------------------------
+    def debug_lbvh(self, boxes, bits=30):
+        """boxes: device float32 [n, 6].  Returns (sorted codes, order, left, right) as numpy arrays."""
+        torch = _torch()
+        n = boxes.shape[0]
+        codes = torch.empty((max(n, 1),), dtype=torch.int64 if bits == 63 else torch.int32, device=self.device)
+        order = torch.empty((max(n, 1),), dtype=torch.int32, device=self.device)
+        left = torch.empty((max(n - 1, 1),), dtype=torch.int32, device=self.device)
+        right = torch.empty((max(n - 1, 1),), dtype=torch.int32, device=self.device)
+        self._check(self.lib.rtx_debug_lbvh(self.handle, boxes.data_ptr(), n, bits, codes.data_ptr(),
+                                            order.data_ptr(), left.data_ptr(), right.data_ptr()))
+        c = codes.cpu().numpy()[:n].view(np.uint64 if bits == 63 else np.uint32)
+        return c, order.cpu().numpy()[:n].view(np.uint32), left.cpu().numpy()[:max(n - 1, 0)], \
+            right.cpu().numpy()[:max(n - 1, 0)]
+
+    def debug_sort(self, keys: np.ndarray):
+        torch = _torch()
+        kb = keys.dtype.itemsize
+        n = keys.size
+        k = torch.from_numpy(keys.view(np.int64 if kb == 8 else np.int32).copy()).to(self.device)
+        v = torch.empty((max(n, 1),), dtype=torch.int32, device=self.device)
+        self._check(self.lib.rtx_debug_sort(self.handle, k.data_ptr(), v.data_ptr(), n, kb))
+        return k.cpu().numpy().view(keys.dtype), v.cpu().numpy()[:n].view(np.uint32)
+
+    # -- scenes ---------------------------------------------------------------------------------------------------
+    def upload_scene(self, scene, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
+        """Build one BLAS per scene.blas entry and the TLAS over scene instances.  Returns (blases, tlas)."""
+        torch = _torch()
+        blases = []
+        for geoms in scene.blas:
+            dg = []
+            for g in geoms:
+                d = {"vertices": _dev_tensor(g.vertices.astype(np.float32), self.device),
+                     "indices": _dev_tensor(g.indices.astype(np.uint32).view(np.int32), self.device),
+                     "transform": None if g.transform is None else
+                     _dev_tensor(np.asarray(g.transform, np.float32).reshape(3, 4), self.device)}
+                dg.append(d)
+            blases.append(self.build_blas(dg, flags))
+        rec = instance_records(scene, [b.address for b in blases])
+        tlas = self.build_tlas(rec, flags)
+        torch.cuda.current_stream(self.device)
+        return blases, tlas
+
+
+INSTANCE_DTYPE = np.dtype([("xf", np.float32, (12,)), ("cm", np.uint32), ("sf", np.uint32), ("addr", np.uint64)])
+assert INSTANCE_DTYPE.itemsize == 64
+
+
+def instance_records(scene, blas_addresses) -> np.ndarray:
+    """64-B VkAccelerationStructureInstanceKHR records (main.cpp:848-858) for scene's instances."""
+    n = len(scene.blas_of_instance)
+    rec = np.zeros(n, INSTANCE_DTYPE)
+    rec["xf"] = np.asarray(scene.transforms, np.float32).reshape(n, 12)
+    rec["cm"] = (np.asarray(scene.custom_index, np.uint32) & 0xFFFFFF) | (np.asarray(scene.mask, np.uint32) << 24)
+    rec["sf"] = (np.asarray(scene.sbt_offset, np.uint32) & 0xFFFFFF) | (np.asarray(scene.flags, np.uint32) << 24)
+    rec["addr"] = np.asarray([blas_addresses[b] for b in scene.blas_of_instance], np.uint64)
+    return rec
+
+
+def hits_to_numpy(h):
+    """Split a [N, 6] int32 hit tensor into (t, u, v float32 arrays, prim, geom, inst uint32 arrays)."""
+    a = h.cpu().numpy()
+    f = a[:, :3].view(np.float32)
+    u = a[:, 3:].view(np.uint32)
+    return f[:, 0], f[:, 1], f[:, 2], u[:, 0], u[:, 1], u[:, 2]

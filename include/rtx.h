@@ -1,0 +1,279 @@
+/*
+ * rtx.h — C ABI of the MI355X (gfx950) ray-tracing core.
+ *
+ * The entry points follow the Vulkan KHR ray-tracing flow of
+ * vulkan-raytraced-triangle/main.cpp one for one:
+ *
+ *   vkGetAccelerationStructureBuildSizesKHR (main.cpp:757, 893) -> rtx_get_blas_build_sizes / rtx_get_tlas_build_sizes
+ *   vkCreateAccelerationStructureKHR + ...DeviceAddressKHR (main.cpp:776-785, 912-919)
+ *                                                             -> folded into rtx_cmd_build_* (caller owns the storage,
+ *                                                                the call returns the 64-bit AS address)
+ *   vkCmdBuildAccelerationStructuresKHR (main.cpp:808, 932)   -> rtx_cmd_build_blas / rtx_cmd_build_tlas
+ *   vkCreateRayTracingPipelinesKHR + SBT (main.cpp:1093-1172, 1264-1320)
+ *                                                             -> compiled-in programs + rtx_hit_record table (rtx_sbt)
+ *   vkCmdTraceRaysKHR (main.cpp:1349-1355)                    -> rtx_cmd_trace_rays
+ *   vkQueueWaitIdle (main.cpp:820, 942)                       -> rtx_sync
+ *
+ * plus ray-buffer queries (closest hit / any hit) and the path-tracing integrator.
+ *
+ * Conventions
+ *  - Every pointer named *_device is a HIP device pointer; everything else is host memory.
+ *  - Return values follow VkResult: 0 = success, negative = error.  No C++ exception crosses the ABI.
+ *  - An rtx_context is externally synchronised (like a VkQueue).  All rtx_cmd_* calls are ordered on the
+ *    context's HIP stream and return before the GPU work finishes; rtx_sync waits for it.
+ *  - Only gfx950 code objects are built.
+ */
+#ifndef RTX_H_
+#define RTX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTX_API_VERSION 1u
+
+/* ---- result codes (VkResult values where one exists) ---------------------------------------- */
+typedef enum rtx_result {
+    RTX_SUCCESS = 0,
+    RTX_ERROR_OUT_OF_HOST_MEMORY = -1,      /* VK_ERROR_OUT_OF_HOST_MEMORY */
+    RTX_ERROR_OUT_OF_DEVICE_MEMORY = -2,    /* VK_ERROR_OUT_OF_DEVICE_MEMORY: storage/scratch below the size query */
+    RTX_ERROR_INITIALIZATION_FAILED = -3,   /* VK_ERROR_INITIALIZATION_FAILED */
+    RTX_ERROR_DEVICE_LOST = -4,             /* VK_ERROR_DEVICE_LOST: a HIP call failed */
+    RTX_ERROR_INVALID_ARGUMENT = -1000,     /* bad pointer, size, count or flag */
+    RTX_ERROR_STACK_OVERFLOW = -1001,       /* traversal stack exceeded the built depth bound */
+    RTX_ERROR_INVALID_AS = -1002            /* address is not an acceleration structure of this library */
+} rtx_result;
+
+typedef struct rtx_context_t* rtx_context;
+
+/* Create a context on HIP device `device`.  `hip_stream` is a hipStream_t to record on; NULL makes the
+ * context create its own (default-priority, non-blocking) stream. */
+int rtx_create_context(int device, void* hip_stream, rtx_context* out_ctx);
+int rtx_destroy_context(rtx_context ctx);
+/* Last error message of this context (never NULL; "" when none). */
+const char* rtx_last_error_string(rtx_context ctx);
+/* vkQueueWaitIdle (main.cpp:820, 942): wait for all work recorded on the context's stream, then report
+ * device-side error flags raised by that work (e.g. RTX_ERROR_STACK_OVERFLOW). */
+int rtx_sync(rtx_context ctx);
+/* The HIP stream the context records on. */
+void* rtx_get_stream(rtx_context ctx);
+
+/* ---- acceleration-structure inputs ------------------------------------------------------------ */
+
+/* VkGeometryFlagBitsKHR / VkBuildAccelerationStructureFlagBitsKHR subset */
+#define RTX_GEOMETRY_OPAQUE_BIT 0x1u                       /* main.cpp:741 */
+#define RTX_BUILD_PREFER_FAST_TRACE_BIT 0x4u               /* main.cpp:751, 887 */
+#define RTX_BUILD_PREFER_FAST_BUILD_BIT 0x8u
+#define RTX_BUILD_MORTON63_BIT 0x10000u                    /* 63-bit Morton keys (default: 30-bit) */
+#define RTX_BUILD_NO_TREELET_BIT 0x20000u                  /* skip SAH treelet refinement */
+
+#define RTX_INDEX_TYPE_UINT32 1u                           /* VK_INDEX_TYPE_UINT32, main.cpp:736 */
+
+/* One triangle geometry: VkAccelerationStructureGeometryTrianglesDataKHR (main.cpp:726-742) merged with its
+ * VkAccelerationStructureBuildRangeInfoKHR (main.cpp:791-804).  Vertex format is R32G32B32_SFLOAT
+ * (main.cpp:732).  Triangle p of the geometry uses vertices
+ *   index_data[primitive_offset/4 + 3p + k] + first_vertex,  k = 0,1,2
+ * each fetched at vertex_data + vertex_index * vertex_stride bytes.  When transform_data_device is not NULL
+ * the row-major 3x4 matrix at transform_data_device + transform_offset bytes is applied to the vertices at
+ * build time (VkTransformMatrixKHR, main.cpp:684-695, 795-804). */
+typedef struct rtx_blas_geometry {
+    const float* vertex_data_device;
+    uint64_t vertex_stride;           /* bytes, >= 12, multiple of 4 */
+    uint32_t max_vertex;              /* highest vertex index the indices may reference */
+    uint32_t index_type;              /* RTX_INDEX_TYPE_UINT32 */
+    const uint32_t* index_data_device;
+    const float* transform_data_device; /* nullable */
+    uint32_t primitive_count;
+    uint32_t primitive_offset;        /* bytes into index data, multiple of 4 */
+    uint32_t first_vertex;
+    uint32_t transform_offset;        /* bytes into transform data, multiple of 16 */
+    uint32_t flags;                   /* RTX_GEOMETRY_*_BIT */
+    uint32_t reserved;
+} rtx_blas_geometry;
+
+/* VkAccelerationStructureInstanceKHR, byte-for-byte (64 B; main.cpp:848-858).  transform is the
+ * object-to-world row-major 3x4; blas_address is a value returned by rtx_cmd_build_blas. */
+typedef struct rtx_instance {
+    float transform[3][4];
+    uint32_t custom_index_and_mask;          /* instanceCustomIndex:24 | mask:8 (mask in the high byte) */
+    uint32_t sbt_offset_and_flags;           /* instanceShaderBindingTableRecordOffset:24 | flags:8 */
+    uint64_t blas_address;
+} rtx_instance;
+
+#define RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT 0x1u /* VK_GEOMETRY_INSTANCE_..., main.cpp:852 */
+#define RTX_INSTANCE_FORCE_OPAQUE_BIT 0x4u
+
+/* VkAccelerationStructureBuildSizesInfoKHR */
+typedef struct rtx_as_build_sizes {
+    uint64_t acceleration_structure_size;
+    uint64_t build_scratch_size;
+    uint64_t update_scratch_size;
+} rtx_as_build_sizes;
+
+/* vkGetAccelerationStructureBuildSizesKHR for a BLAS (main.cpp:757-762).  Pure query. */
+int rtx_get_blas_build_sizes(rtx_context ctx, const rtx_blas_geometry* geometries, uint32_t geometry_count,
+                             uint32_t build_flags, rtx_as_build_sizes* out_sizes);
+/* vkGetAccelerationStructureBuildSizesKHR for a TLAS (main.cpp:893-898).  Pure query. */
+int rtx_get_tlas_build_sizes(rtx_context ctx, uint32_t instance_count, uint32_t build_flags,
+                             rtx_as_build_sizes* out_sizes);
+
+/* vkCreateAccelerationStructureKHR + vkCmdBuildAccelerationStructuresKHR + vkGetAccelerationStructureDevice-
+ * AddressKHR for a BLAS (main.cpp:776-812).  dst_device (>= acceleration_structure_size bytes, 256-B aligned)
+ * and scratch_device (>= build_scratch_size bytes, 256-B aligned) are caller-owned.  Inputs and scratch may
+ * be released once the stream has passed this call (main.cpp:823-830).  *out_address receives the BLAS
+ * address to put into rtx_instance.blas_address. */
+int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uint32_t geometry_count,
+                       uint32_t build_flags, void* dst_device, uint64_t dst_size, void* scratch_device,
+                       uint64_t scratch_size, uint64_t* out_address);
+
+/* Same for a TLAS over instance_count rtx_instance records in device memory (main.cpp:912-945). */
+int rtx_cmd_build_tlas(rtx_context ctx, const rtx_instance* instances_device, uint32_t instance_count,
+                       uint32_t build_flags, void* dst_device, uint64_t dst_size, void* scratch_device,
+                       uint64_t scratch_size, uint64_t* out_address);
+
+/* Statistics written by a build (read from device memory by the host; blocks until the build is done). */
+typedef struct rtx_as_info {
+    uint32_t type;                 /* 0 = BLAS, 1 = TLAS */
+    uint32_t primitive_count;      /* triangles or instances */
+    uint32_t node_count;           /* 64-B BVH8 nodes */
+    uint32_t slot_count;           /* 64-B slots (nodes + leaves) */
+    uint32_t max_depth;            /* BVH8 levels */
+    uint32_t geometry_count;
+    float bounds[6];               /* lo xyz, hi xyz */
+    double sah_cost;               /* SAH cost of the binary hierarchy before collapse */
+} rtx_as_info;
+int rtx_get_as_info(rtx_context ctx, uint64_t as_address, rtx_as_info* out_info);
+
+/* ---- shader binding table ----------------------------------------------------------------------- */
+
+/* Compiled-in hit programs (the reference compiles GLSL at run time, shader_module.h:11-74). */
+#define RTX_PROGRAM_RECORD_COLOR 0u      /* hitValue = record color */
+#define RTX_PROGRAM_BARYCENTRIC 1u       /* hitValue = (1-u-v, u, v) */
+#define RTX_PROGRAM_MATCH_BARYCENTRIC 2u /* main.cpp:1080-1091: barycentric colour when gl_PrimitiveID,
+                                            gl_InstanceID, gl_InstanceCustomIndexEXT and gl_GeometryIndexEXT
+                                            all equal the record's match_* fields, else the record color */
+#define RTX_PROGRAM_LAMBERTIAN 3u        /* path tracer: albedo = color, emission = emission */
+
+/* One hit-group record (main.cpp:1307-1317: handle + float color[3]); 48 B. */
+typedef struct rtx_hit_record {
+    uint32_t program;
+    float color[3];
+    float emission[3];
+    int32_t match_primitive;
+    int32_t match_instance;
+    int32_t match_custom_index;
+    int32_t match_geometry;
+    uint32_t reserved;
+} rtx_hit_record;
+
+/* Hit-group table + miss program.  Record index for a hit (Vulkan spec 40.3.1, main.cpp:1252-1263):
+ *   instance sbt offset + geometry index * sbt_record_stride + sbt_record_offset */
+typedef struct rtx_sbt {
+    const rtx_hit_record* hit_records_device;
+    uint32_t hit_record_count;
+    uint32_t sbt_record_offset;    /* traceRayEXT sbtRecordOffset (main.cpp:1049: 0) */
+    uint32_t sbt_record_stride;    /* traceRayEXT sbtRecordStride (main.cpp:1049: 1) */
+    uint32_t reserved;
+    float miss_color[3];           /* miss program (main.cpp:1063-1066: (0, 0, 0.2)) */
+    float miss_alpha_reserved;
+} rtx_sbt;
+
+/* Camera uniform block of the raygen shader (std140, 16 B; main.cpp:1003-1006, value {0,0,10,60} :1015). */
+typedef struct rtx_camera {
+    float position[3];
+    float y_fov_degree;
+} rtx_camera;
+
+/* traceRayEXT arguments used by the raygen program (main.cpp:1047-1052). */
+#define RTX_RAY_FLAG_OPAQUE 0x1u                 /* gl_RayFlagsOpaqueEXT */
+#define RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT 0x4u /* gl_RayFlagsTerminateOnFirstHitEXT */
+typedef struct rtx_trace_params {
+    uint32_t ray_flags;    /* RTX_RAY_FLAG_* */
+    uint32_t cull_mask;    /* 8-bit, ANDed with the instance mask */
+    float tmin;
+    float tmax;
+} rtx_trace_params;
+
+/* vkCmdTraceRaysKHR(width, height, 1) with the reference raygen program (main.cpp:1033-1055):
+ *   aspect_y = tan(radians(fov) / 2), aspect_x = aspect_y * width / height
+ *   ndc = (launchID + 0.5) / launchSize * 2 - 1
+ *   dir = (ndc.x * aspect_x, -ndc.y * aspect_y, -1)   (not normalised)
+ * and stores vec4(hitValue, 0) into film_rgba32f_device[(y * width + x) * 4 + c].
+ * params may be NULL: then tmin 0, tmax 100, opaque, cull mask 0xFF (main.cpp:1047-1052). */
+int rtx_cmd_trace_rays(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                       const rtx_trace_params* params, uint32_t width, uint32_t height,
+                       float* film_rgba32f_device);
+
+/* ---- ray-buffer queries (ray-query form of traceRayEXT) ------------------------------------------- */
+
+/* 32 B */
+typedef struct rtx_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} rtx_ray;
+
+/* 24 B.  instance_id == 0xFFFFFFFF marks a miss.  (u, v) are the barycentrics of the Vulkan
+ * hitAttributeEXT: P = (1-u-v) v0 + u v1 + v v2. */
+typedef struct rtx_hit {
+    float t;
+    float u;
+    float v;
+    uint32_t primitive_id;    /* gl_PrimitiveID: index within the geometry */
+    uint32_t geometry_index;  /* gl_GeometryIndexEXT */
+    uint32_t instance_id;     /* gl_InstanceID: index in the TLAS instance array */
+} rtx_hit;
+
+#define RTX_QUERY_CLOSEST_HIT 0u
+#define RTX_QUERY_ANY_HIT 1u
+
+/* Trace ray_count rays.  mode RTX_QUERY_CLOSEST_HIT writes hits_device[i] (rtx_hit); RTX_QUERY_ANY_HIT writes
+ * occluded_device[i] = 1 if any opaque triangle lies in [tmin, tmax], else 0.  The unused output may be NULL.
+ * cull_mask is ANDed with each instance mask (0xFF in the reference). */
+int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device,
+                             uint32_t ray_count, uint32_t mode, uint32_t cull_mask, rtx_hit* hits_device,
+                             uint32_t* occluded_device);
+
+/* Per-ray traversal counters of the last rtx_cmd_trace_ray_buffer call issued with stats enabled:
+ * total 64-B node fetches, triangle tests and instance entries summed over all rays. */
+typedef struct rtx_trace_stats {
+    uint64_t rays;
+    uint64_t node_fetches;
+    uint64_t triangle_tests;
+    uint64_t instance_entries;
+} rtx_trace_stats;
+int rtx_set_trace_stats(rtx_context ctx, int enable);
+int rtx_get_trace_stats(rtx_context ctx, rtx_trace_stats* out_stats);
+
+/* Traversal kernel launch shape (the autotuner's knobs).  0 fields keep the current value. */
+typedef struct rtx_trace_config {
+    uint32_t block_size;      /* threads per block: 64..1024, multiple of 64 */
+    uint32_t blocks_per_cu;   /* persistent blocks per compute unit */
+    uint32_t lds_stack_depth; /* stack entries held in LDS per lane before spilling to global memory */
+    uint32_t reserved;
+} rtx_trace_config;
+int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg);
+int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg);
+
+/* ---- verification hooks (no counterpart in the reference: they expose intermediate LBVH build steps so that
+ * parity tests can compare them with the CPU oracle) ---------------------------------------------------------- */
+
+/* Run the builder's Morton / sort / hierarchy steps on n primitive boxes (device, 6 floats each: lo xyz, hi xyz)
+ * and copy out (device buffers): sorted codes (uint32 for morton_bits 30, uint64 for 63), the sorted primitive
+ * order (uint32), and for the n-1 internal nodes of the binary radix tree the left/right child codes
+ * (c >= 0 internal node, c < 0 leaf at sorted position -1-c).  Blocks until done; allocates its own scratch. */
+int rtx_debug_lbvh(rtx_context ctx, const float* boxes_device, uint32_t n, uint32_t morton_bits, void* codes_device,
+                   uint32_t* order_device, int32_t* left_device, int32_t* right_device);
+
+/* Sort n (key, value) pairs with the builder's radix sort: keys uint32 (key_bytes 4) or uint64 (8); values are
+ * replaced by the stable sorted permutation.  Blocks until done; allocates its own scratch. */
+int rtx_debug_sort(rtx_context ctx, void* keys_device, uint32_t* values_device, uint32_t n, uint32_t key_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTX_H_ */

@@ -1,0 +1,225 @@
+"""FP64 CPU oracle for the ray-tracing hot path (ctypes wrapper over liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker.  The product library never imports it.  What each function restates is documented in rtx_oracle.cpp
+with citations of vulkan-raytraced-triangle/main.cpp.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+MISS = 0xFFFFFFFF
+STATUS_MISS, STATUS_HIT, STATUS_TIE = 0, 1, 2
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        L.oracle_blas_create.restype = ctypes.c_int
+        L.oracle_blas_create.argtypes = [ctypes.c_int, P, P, P, P]
+        L.oracle_tlas_create.restype = ctypes.c_int
+        L.oracle_tlas_create.argtypes = [ctypes.c_int, P]
+        L.oracle_reset.restype = None
+        L.oracle_blas_delta.restype = ctypes.c_double
+        L.oracle_blas_delta.argtypes = [ctypes.c_int]
+        L.oracle_trace.restype = None
+        L.oracle_trace.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, P, P, P, ctypes.c_int]
+        L.oracle_trace_brute.restype = None
+        L.oracle_trace_brute.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_uint32, P, P, P]
+        L.oracle_aspect.restype = None
+        L.oracle_aspect.argtypes = [ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, P, P]
+        L.oracle_raygen.restype = None
+        L.oracle_raygen.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float, P]
+        L.oracle_shade.restype = None
+        L.oracle_shade.argtypes = [ctypes.c_int, ctypes.c_int64, P, P, P, ctypes.c_uint32, ctypes.c_uint32,
+                                   ctypes.c_uint32, P, P]
+        L.oracle_centroids.restype = None
+        L.oracle_centroids.argtypes = [ctypes.c_int, P, P, P, P, P]
+        L.oracle_morton.restype = None
+        L.oracle_morton.argtypes = [P, ctypes.c_int64, ctypes.c_int, P]
+        L.oracle_karras.restype = None
+        L.oracle_karras.argtypes = [P, ctypes.c_int64, ctypes.c_int, P, P]
+        L.oracle_pcg32.restype = None
+        L.oracle_pcg32.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
+        L.oracle_pcg4d.restype = None
+        L.oracle_pcg4d.argtypes = [P, ctypes.c_int64, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _geom_arrays(geoms):
+    """geoms: list of (vertices float32 [V,3], indices uint32 [T,3], transform float32 [3,4] or None)."""
+    keep = []
+    vp, ip, xp, pc = [], [], [], []
+    for v, i, x in geoms:
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        i = np.ascontiguousarray(i, dtype=np.uint32)
+        keep += [v, i]
+        vp.append(_ptr(v))
+        ip.append(_ptr(i))
+        pc.append(i.shape[0] if i.ndim == 2 else i.size // 3)
+        if x is None:
+            xp.append(0)
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float32).reshape(3, 4)
+            keep.append(x)
+            xp.append(_ptr(x))
+    arrs = (np.array(vp, dtype=np.uint64), np.array(ip, dtype=np.uint64), np.array(pc, dtype=np.uint32),
+            np.array(xp, dtype=np.uint64))
+    return keep, arrs
+
+
+def reset() -> None:
+    lib().oracle_reset()
+
+
+def blas_create(geoms) -> int:
+    keep, (vp, ip, pc, xp) = _geom_arrays(geoms)
+    return lib().oracle_blas_create(len(geoms), _ptr(vp), _ptr(ip), _ptr(pc), _ptr(xp))
+
+
+def blas_delta(h: int) -> float:
+    return lib().oracle_blas_delta(h)
+
+
+def instance_records(transforms, blas_handles, custom_index=None, mask=None, sbt_offset=None, flags=None):
+    """Pack 64-B instance records (main.cpp:848-858 layout).  blas_handles fill the address field."""
+    n = len(blas_handles)
+    rec = np.zeros(n, dtype=np.dtype([("xf", np.float32, (12,)), ("cm", np.uint32), ("sf", np.uint32),
+                                      ("addr", np.uint64)]))
+    rec["xf"] = np.asarray(transforms, dtype=np.float32).reshape(n, 12)
+    ci = np.zeros(n, np.uint32) if custom_index is None else np.asarray(custom_index, np.uint32)
+    mk = np.full(n, 0xFF, np.uint32) if mask is None else np.asarray(mask, np.uint32)
+    so = np.zeros(n, np.uint32) if sbt_offset is None else np.asarray(sbt_offset, np.uint32)
+    fl = np.zeros(n, np.uint32) if flags is None else np.asarray(flags, np.uint32)
+    rec["cm"] = (ci & 0xFFFFFF) | (mk << 24)
+    rec["sf"] = (so & 0xFFFFFF) | (fl << 24)
+    rec["addr"] = np.asarray(blas_handles, dtype=np.uint64)
+    return rec
+
+
+def tlas_create(records: np.ndarray) -> int:
+    records = np.ascontiguousarray(records)
+    assert records.dtype.itemsize == 64
+    return lib().oracle_tlas_create(len(records), _ptr(records))
+
+
+def trace(tlas: int, rays: np.ndarray, any_hit: bool = False, cull_mask: int = 0xFF, threads: int = 0):
+    """rays: float32 [N,8].  Returns (t_uv float64 [N,3], ids uint32 [N,4] (prim, geom, inst, custom),
+    status int32 [N])."""
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    t_uv = np.zeros((n, 3), np.float64)
+    ids = np.zeros((n, 4), np.uint32)
+    st = np.zeros(n, np.int32)
+    lib().oracle_trace(tlas, _ptr(rays), n, int(any_hit), cull_mask, _ptr(t_uv), _ptr(ids), _ptr(st), threads)
+    return t_uv, ids, st
+
+
+def trace_brute(tlas: int, rays: np.ndarray, cull_mask: int = 0xFF):
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    t_uv = np.zeros((n, 3), np.float64)
+    ids = np.zeros((n, 4), np.uint32)
+    st = np.zeros(n, np.int32)
+    lib().oracle_trace_brute(tlas, _ptr(rays), n, cull_mask, _ptr(t_uv), _ptr(ids), _ptr(st))
+    return t_uv, ids, st
+
+
+def aspect(fov_deg: float, w: int, h: int):
+    ax = ctypes.c_float()
+    ay = ctypes.c_float()
+    lib().oracle_aspect(fov_deg, w, h, ctypes.addressof(ax), ctypes.addressof(ay))
+    return ax.value, ay.value
+
+
+def raygen(camera, w: int, h: int, tmin: float = 0.0, tmax: float = 100.0) -> np.ndarray:
+    cam = np.ascontiguousarray(camera, dtype=np.float32).reshape(4)
+    rays = np.zeros((h * w, 8), np.float32)
+    lib().oracle_raygen(_ptr(cam), w, h, tmin, tmax, _ptr(rays))
+    return rays
+
+
+def shade(tlas: int, t_uv, ids, records: np.ndarray, sbt_offset: int, sbt_stride: int, miss_color) -> np.ndarray:
+    n = ids.shape[0]
+    film = np.zeros((n, 4), np.float32)
+    rec = np.ascontiguousarray(records)
+    assert rec.dtype.itemsize == 48
+    miss = np.ascontiguousarray(miss_color, dtype=np.float32)
+    t_uv = np.ascontiguousarray(t_uv, dtype=np.float64)
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    lib().oracle_shade(tlas, n, _ptr(t_uv), _ptr(ids), _ptr(rec), len(rec), sbt_offset, sbt_stride, _ptr(miss),
+                       _ptr(film))
+    return film
+
+
+def trace_rays(tlas: int, records, camera, w, h, miss_color=(0.0, 0.0, 0.2), tmin=0.0, tmax=100.0, cull_mask=0xFF,
+               sbt_offset=0, sbt_stride=1, threads=0):
+    """Oracle of vkCmdTraceRaysKHR with the reference raygen/miss/chit: returns (film [H,W,4], status [H,W])."""
+    rays = raygen(camera, w, h, tmin, tmax)
+    t_uv, ids, st = trace(tlas, rays, False, cull_mask, threads)
+    film = shade(tlas, t_uv, ids, records, sbt_offset, sbt_stride, miss_color)
+    return film.reshape(h, w, 4), st.reshape(h, w)
+
+
+def centroids(geoms) -> np.ndarray:
+    keep, (vp, ip, pc, xp) = _geom_arrays(geoms)
+    n = int(pc.sum())
+    cen = np.zeros((n, 3), np.float32)
+    lib().oracle_centroids(len(geoms), _ptr(vp), _ptr(ip), _ptr(pc), _ptr(xp), _ptr(cen))
+    return cen
+
+
+def morton(cen: np.ndarray, bits: int = 30) -> np.ndarray:
+    cen = np.ascontiguousarray(cen, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(cen.shape[0], np.uint32 if bits == 30 else np.uint64)
+    lib().oracle_morton(_ptr(cen), cen.shape[0], bits, _ptr(out))
+    return out
+
+
+def sort_pairs(keys: np.ndarray):
+    """Stable key sort: returns (sorted keys, original indices)."""
+    order = np.argsort(keys, kind="stable").astype(np.uint32)
+    return keys[order], order
+
+
+def karras(sorted_keys: np.ndarray, key_bits: int):
+    keys = np.ascontiguousarray(sorted_keys.astype(np.uint64))
+    n = keys.size
+    left = np.zeros(max(n - 1, 0), np.int32)
+    right = np.zeros(max(n - 1, 0), np.int32)
+    lib().oracle_karras(_ptr(keys), n, key_bits, _ptr(left), _ptr(right))
+    return left, right
+
+
+def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().oracle_pcg32(initstate, initseq, n, _ptr(out))
+    return out
+
+
+def pcg4d(v: np.ndarray) -> np.ndarray:
+    v = np.ascontiguousarray(v, dtype=np.uint32).reshape(-1, 4)
+    out = np.zeros_like(v)
+    lib().oracle_pcg4d(_ptr(v), v.shape[0], _ptr(out))
+    return out

@@ -1,0 +1,826 @@
+// rtx_oracle.cpp — FP64 CPU oracle for the ray-tracing hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
+// library, as the checker; the product (build-up-phase_amd/librtx.so) never links or calls it.
+//
+// What it restates (reference = vulkan-raytraced-triangle/main.cpp):
+//   * raygen (main.cpp:1033-1055): pinhole camera, unnormalised direction, tmin 0 / tmax 100, film = vec4(hit, 0)
+//   * miss (main.cpp:1063-1066) and closest-hit (main.cpp:1080-1091) programs, SBT record indexing
+//     instanceSBTOffset + geometryIndex * stride + offset (main.cpp:1252-1263, 1307-1317)
+//   * the driver's traversal semantics (traceRayEXT call site main.cpp:1047-1052; the driver itself is closed):
+//     closest opaque triangle with t in [tmin, tmax], per-geometry 3x4 transforms applied at build time
+//     (main.cpp:684-695, 795-804), per-instance object-to-world 3x4 transforms (main.cpp:835-858), instance
+//     mask & cull mask, barycentrics (u, v) with P = (1-u-v) v0 + u v1 + v v2, IDs gl_PrimitiveID /
+//     gl_GeometryIndexEXT / gl_InstanceID / gl_InstanceCustomIndexEXT.
+//     Intersection is evaluated in FP64 (Woop/Benthin/Wald 2013 edge functions in a sheared frame).
+//   * LBVH build steps: 30/63-bit Morton codes of triangle-box centroids, stable key sort, Karras 2012
+//     hierarchy emission.
+//   * PCG32 (O'Neill 2014) and the pcg4d hash (Jarzynski & Olano 2020).
+//
+// Epsilon-ties.  The GPU decides hits in FP32; this oracle decides in FP64 and, for every ray, reports whether
+// the FP64 answer is robust against FP32 rounding.  In object space the GPU works on a ray whose origin has
+// been moved to the BLAS bounding box, so all FP32 position errors are bounded by
+//     delta = 2^-19 * (R + D)       R = max |coordinate| of the BLAS box, D = its diagonal
+// (32 ulps of the largest magnitude involved).  Positions are compared in the sheared 2D frame of the
+// watertight test, where a 3D error of size e moves a point by at most 3e.  A ray is an epsilon-tie when
+//   - a candidate triangle's 2D point lies within 3*delta of one of its edges, or
+//   - two candidates' hit distances differ by less than the sum of their t-uncertainties, or
+//   - the closest hit lies within its t-uncertainty of tmin or tmax.
+// Everything else must match the GPU bit-exactly in hit IDs.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct d3 {
+    double x, y, z;
+};
+inline d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline d3 cross(d3 a, d3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double comp(d3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+inline double len(d3 a) { return std::sqrt(dot(a, a)); }
+
+constexpr double kEps32 = 5.9604644775390625e-08;  // 2^-24
+constexpr double kDeltaScale = 1.0 / 524288.0;     // 2^-19
+
+// ---------------------------------------------------------------------------------------------------------
+// Geometry transform at build time.  Defined FP32 operation order (no contraction):
+//   x' = ((m00*x + m01*y) + m02*z) + m03
+// The GPU builder evaluates the same expression, so transformed vertices are bit-identical.
+// ---------------------------------------------------------------------------------------------------------
+inline void xform_f32(const float* m, const float* v, float* o) {
+    for (int r = 0; r < 3; ++r) {
+        float a = m[4 * r + 0] * v[0];
+        float b = m[4 * r + 1] * v[1];
+        float c = m[4 * r + 2] * v[2];
+        float s = a + b;
+        s = s + c;
+        s = s + m[4 * r + 3];
+        o[r] = s;
+    }
+}
+
+struct Tri {
+    float v[9];
+    uint32_t prim, geom;
+};
+
+struct BNode {
+    double lo[3], hi[3];
+    int32_t left;   // internal: left child index; leaf: -1
+    int32_t right;  // internal: right child; leaf: first triangle index
+    int32_t count;  // leaf: triangle count; internal: 0
+};
+
+struct Blas {
+    std::vector<Tri> tris;
+    std::vector<uint32_t> order;  // BVH leaf order -> tris index
+    std::vector<BNode> nodes;
+    double lo[3], hi[3];
+    double delta;  // FP32 position uncertainty (object space)
+};
+
+struct Inst {
+    double m[12];     // object -> world
+    double minv[12];  // world -> object
+    uint32_t custom, mask, sbt_offset, flags;
+    int blas;
+    bool valid;
+};
+
+struct Tlas {
+    std::vector<Inst> inst;
+};
+
+std::vector<Blas*> g_blas;
+std::vector<Tlas*> g_tlas;
+
+void tri_box(const Tri& t, double* lo, double* hi) {
+    for (int a = 0; a < 3; ++a) {
+        double x0 = t.v[a], x1 = t.v[3 + a], x2 = t.v[6 + a];
+        lo[a] = std::min(x0, std::min(x1, x2));
+        hi[a] = std::max(x0, std::max(x1, x2));
+    }
+}
+
+// Median-split binary BVH, leaf size <= 4.  Boxes are exact FP64 bounds; the traversal inflates them.
+int build_rec(Blas& b, std::vector<double>& cen, uint32_t* idx, int n, int depth) {
+    int me = (int)b.nodes.size();
+    b.nodes.push_back(BNode{});
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; ++i) {
+        double tl[3], th[3];
+        tri_box(b.tris[idx[i]], tl, th);
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], tl[a]);
+            hi[a] = std::max(hi[a], th[a]);
+            clo[a] = std::min(clo[a], cen[3 * idx[i] + a]);
+            chi[a] = std::max(chi[a], cen[3 * idx[i] + a]);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        b.nodes[me].lo[a] = lo[a];
+        b.nodes[me].hi[a] = hi[a];
+    }
+    if (n <= 4 || depth > 60) {
+        b.nodes[me].left = -1;
+        b.nodes[me].right = (int)(idx - b.order.data());
+        b.nodes[me].count = n;
+        return me;
+    }
+    int ax = 0;
+    double ext = chi[0] - clo[0];
+    for (int a = 1; a < 3; ++a)
+        if (chi[a] - clo[a] > ext) {
+            ext = chi[a] - clo[a];
+            ax = a;
+        }
+    int mid = n / 2;
+    std::nth_element(idx, idx + mid, idx + n, [&](uint32_t p, uint32_t q) {
+        double cp = cen[3 * p + ax], cq = cen[3 * q + ax];
+        return cp < cq || (cp == cq && p < q);
+    });
+    int l = build_rec(b, cen, idx, mid, depth + 1);
+    int r = build_rec(b, cen, idx + mid, n - mid, depth + 1);
+    b.nodes[me].left = l;
+    b.nodes[me].right = r;
+    b.nodes[me].count = 0;
+    return me;
+}
+
+void invert3x4(const double* m, double* inv) {
+    // inverse of [A | t]: [A^-1 | -A^-1 t], A^-1 by cofactors (FP64)
+    double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    double r = 1.0 / det;
+    inv[0] = A * r;
+    inv[1] = -(b * i - c * h) * r;
+    inv[2] = (b * f - c * e) * r;
+    inv[4] = B * r;
+    inv[5] = (a * i - c * g) * r;
+    inv[6] = -(a * f - c * d) * r;
+    inv[8] = C * r;
+    inv[9] = -(a * h - b * g) * r;
+    inv[10] = (a * e - b * d) * r;
+    for (int row = 0; row < 3; ++row)
+        inv[4 * row + 3] = -(inv[4 * row + 0] * m[3] + inv[4 * row + 1] * m[7] + inv[4 * row + 2] * m[11]);
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// FP64 ray/triangle evaluation in the sheared frame of the watertight test.
+// ---------------------------------------------------------------------------------------------------------
+struct RayFrame {
+    d3 o, d;
+    int kx, ky, kz;
+    double Sx, Sy, Sz;
+    double dlen;
+};
+
+RayFrame make_frame(d3 o, d3 d) {
+    RayFrame f;
+    f.o = o;
+    f.d = d;
+    double ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (comp(d, kz) < 0) std::swap(kx, ky);
+    f.kx = kx;
+    f.ky = ky;
+    f.kz = kz;
+    f.Sx = comp(d, kx) / comp(d, kz);
+    f.Sy = comp(d, ky) / comp(d, kz);
+    f.Sz = 1.0 / comp(d, kz);
+    f.dlen = len(d);
+    return f;
+}
+
+struct Cand {
+    bool inside;    // FP64 says hit (all margins >= 0, det != 0)
+    bool near;      // within 3*delta of an edge
+    double margin;  // signed min distance to the edges in the 2D frame (>0 inside)
+    double t, u, v;
+    double tunc;    // t uncertainty
+};
+
+// Returns false when the triangle is far from the ray (margin < -3 delta) or degenerate/parallel.
+bool eval_tri(const RayFrame& f, const Tri& tr, double delta, Cand& c) {
+    d3 v0{tr.v[0], tr.v[1], tr.v[2]}, v1{tr.v[3], tr.v[4], tr.v[5]}, v2{tr.v[6], tr.v[7], tr.v[8]};
+    d3 A = v0 - f.o, B = v1 - f.o, C = v2 - f.o;
+    double Ax = comp(A, f.kx) - f.Sx * comp(A, f.kz), Ay = comp(A, f.ky) - f.Sy * comp(A, f.kz);
+    double Bx = comp(B, f.kx) - f.Sx * comp(B, f.kz), By = comp(B, f.ky) - f.Sy * comp(B, f.kz);
+    double Cx = comp(C, f.kx) - f.Sx * comp(C, f.kz), Cy = comp(C, f.ky) - f.Sy * comp(C, f.kz);
+    double U = Cx * By - Cy * Bx;
+    double V = Ax * Cy - Ay * Cx;
+    double W = Bx * Ay - By * Ax;
+    double det = U + V + W;
+    if (det == 0.0) return false;
+    double s = det > 0 ? 1.0 : -1.0;
+    double lbc = std::hypot(Bx - Cx, By - Cy), lca = std::hypot(Cx - Ax, Cy - Ay), lab = std::hypot(Ax - Bx, Ay - By);
+    if (lbc == 0 || lca == 0 || lab == 0) return false;
+    double mU = s * U / lbc, mV = s * V / lca, mW = s * W / lab;
+    double m = std::min(mU, std::min(mV, mW));
+    double band = 3.0 * delta;
+    if (m < -band) return false;
+    double Az = f.Sz * comp(A, f.kz), Bz = f.Sz * comp(B, f.kz), Cz = f.Sz * comp(C, f.kz);
+    double T = U * Az + V * Bz + W * Cz;
+    c.t = T / det;
+    c.u = V / det;
+    c.v = W / det;
+    c.margin = m;
+    c.inside = m >= 0;
+    c.near = std::fabs(m) < band;
+    // t uncertainty: a position error delta becomes delta / sin(theta) along the ray
+    d3 n = cross(v1 - v0, v2 - v0);
+    double sn = std::fabs(dot(n, f.d)) / (len(n) * f.dlen);
+    sn = std::max(sn, 1e-12);
+    c.tunc = (3.0 * delta / sn) / f.dlen + 64.0 * kEps32 * std::fabs(c.t);
+    return true;
+}
+
+struct HitOut {
+    double t, u, v;
+    uint32_t prim, geom, inst, custom;
+    int32_t status;  // 0 robust miss, 1 robust hit, 2 epsilon-tie
+    uint32_t n_cand;
+};
+
+struct Collector {
+    // best inside candidate + bookkeeping for tie detection
+    bool have = false;
+    double t = 0, u = 0, v = 0, tunc = 0;
+    uint32_t prim = 0, geom = 0, inst = 0;
+    bool tie = false;
+    // list of "possibly hit" candidates (inside or near) for t-tie checks
+    struct C {
+        double t, tunc;
+        bool inside, near;
+    };
+    std::vector<C> cands;
+};
+
+void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint32_t inst_id, Collector& col) {
+    if (b.nodes.empty()) return;
+    // boxes inflated by 4*delta so that every near-edge candidate is visited
+    double infl = 4.0 * b.delta;
+    double inv[3] = {1.0 / f.d.x, 1.0 / f.d.y, 1.0 / f.d.z};
+    int stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const BNode& n = b.nodes[stack[--sp]];
+        double t0 = -std::numeric_limits<double>::infinity(), t1 = std::numeric_limits<double>::infinity();
+        bool miss = false;
+        for (int a = 0; a < 3; ++a) {
+            double o = comp(f.o, a), lo = n.lo[a] - infl, hi = n.hi[a] + infl;
+            double da = comp(f.d, a);
+            if (da == 0.0) {
+                if (o < lo || o > hi) miss = true;
+                continue;
+            }
+            double ta = (lo - o) * inv[a], tb = (hi - o) * inv[a];
+            if (ta > tb) std::swap(ta, tb);
+            t0 = std::max(t0, ta);
+            t1 = std::min(t1, tb);
+        }
+        if (miss) continue;
+        double slack = 1e-9 * (std::fabs(t0) + std::fabs(t1)) + 1e-300;
+        if (t0 > t1 + slack) continue;
+        // t window with a generous margin for tie detection
+        double tw = (tmax < std::numeric_limits<double>::infinity()) ? tmax * (1 + 1e-4) + 1e-6 : tmax;
+        if (t1 < tmin - std::fabs(tmin) * 1e-4 - 1e-6 || t0 > tw) continue;
+        if (n.left >= 0) {
+            stack[sp++] = n.left;
+            stack[sp++] = n.right;
+            continue;
+        }
+        for (int k = 0; k < n.count; ++k) {
+            const Tri& tr = b.tris[b.order[n.right + k]];
+            Cand c;
+            if (!eval_tri(f, tr, b.delta, c)) continue;
+            // candidates relevant to [tmin, tmax] (with uncertainty)
+            if (c.t + c.tunc < tmin || c.t - c.tunc > tmax) continue;
+            if (!c.inside && !c.near) continue;
+            col.cands.push_back({c.t, c.tunc, c.inside, c.near});
+            if (c.inside && c.t >= tmin && c.t <= tmax) {
+                bool better = !col.have || c.t < col.t ||
+                              (c.t == col.t && (inst_id < col.inst || (inst_id == col.inst &&
+                                                                       (tr.geom < col.geom || (tr.geom == col.geom && tr.prim < col.prim)))));
+                if (better) {
+                    col.have = true;
+                    col.t = c.t;
+                    col.u = c.u;
+                    col.v = c.v;
+                    col.tunc = c.tunc;
+                    col.prim = tr.prim;
+                    col.geom = tr.geom;
+                    col.inst = inst_id;
+                }
+            }
+        }
+    }
+}
+
+void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, HitOut& out) {
+    d3 o{ray[0], ray[1], ray[2]}, d{ray[4], ray[5], ray[6]};
+    double tmin = ray[3], tmax = ray[7];
+    Collector col;
+    for (size_t i = 0; i < tl.inst.size(); ++i) {
+        const Inst& in = tl.inst[i];
+        if (!in.valid || ((in.mask & cull) == 0)) continue;
+        const Blas& b = *g_blas[in.blas];
+        const double* M = in.minv;
+        d3 oo{M[0] * o.x + M[1] * o.y + M[2] * o.z + M[3], M[4] * o.x + M[5] * o.y + M[6] * o.z + M[7],
+              M[8] * o.x + M[9] * o.y + M[10] * o.z + M[11]};
+        d3 dd{M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z,
+              M[8] * d.x + M[9] * d.y + M[10] * d.z};
+        if (dd.x == 0 && dd.y == 0 && dd.z == 0) continue;
+        RayFrame f = make_frame(oo, dd);
+        trace_blas(b, f, tmin, tmax, (uint32_t)i, col);
+    }
+    out.n_cand = (uint32_t)col.cands.size();
+    out.custom = 0;
+    // Every entry of col.cands is either inside the triangle or within 3*delta of an edge, and its t lies in
+    // [tmin, tmax] up to its t-uncertainty.
+    if (any_hit) {
+        bool robust = false;
+        for (auto& c : col.cands)
+            if (c.inside && !c.near && c.t - c.tunc >= tmin && c.t + c.tunc <= tmax) robust = true;
+        out.status = robust ? 1 : (col.cands.empty() ? 0 : 2);
+    }
+    if (!col.have) {
+        out.t = 0;
+        out.u = out.v = 0;
+        out.prim = out.geom = 0;
+        out.inst = 0xFFFFFFFFu;
+        if (!any_hit) out.status = col.cands.empty() ? 0 : 2;
+        return;
+    }
+    out.t = col.t;
+    out.u = col.u;
+    out.v = col.v;
+    out.prim = col.prim;
+    out.geom = col.geom;
+    out.inst = col.inst;
+    out.custom = tl.inst[col.inst].custom;
+    if (any_hit) return;
+    bool tie = col.t - col.tunc < tmin || col.t + col.tunc > tmax;
+    int close = 0;
+    for (auto& c : col.cands) {
+        // a near-edge candidate that could be in front of (or level with) the winner
+        if (c.near && c.t - c.tunc <= col.t + col.tunc) tie = true;
+        // another hit whose distance is indistinguishable from the winner's
+        if (c.inside && std::fabs(c.t - col.t) <= c.tunc + col.tunc) ++close;
+    }
+    if (close > 1) tie = true;
+    out.status = tie ? 2 : 1;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Morton codes (LBVH step 1).  Defined FP32 arithmetic, restated bit-for-bit by the GPU builder.
+// ---------------------------------------------------------------------------------------------------------
+inline uint32_t quant(float c, float cmin, float scale, uint32_t maxq) {
+    float f = (c - cmin) * scale;
+    if (!(f > 0.0f)) return 0;  // also catches NaN
+    if (f >= (float)maxq) return maxq;
+    return (uint32_t)f;
+}
+inline uint32_t spread10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+inline uint64_t spread21(uint64_t v) {
+    v &= 0x1FFFFFull;
+    v = (v | (v << 32)) & 0x1F00000000FFFFull;
+    v = (v | (v << 16)) & 0x1F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+inline int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+inline int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+
+// ---------------------------------------------------------------------------------------------------------
+// PCG (O'Neill 2014 pcg32; Jarzynski & Olano 2020 pcg4d)
+// ---------------------------------------------------------------------------------------------------------
+struct Pcg32 {
+    uint64_t state, inc;
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ull + inc;
+        uint32_t xorshifted = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xorshifted >> rot) | (xorshifted << ((32u - rot) & 31u));
+    }
+    void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0u;
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// ---- scene ------------------------------------------------------------------------------------------------
+// Create a BLAS from ngeom triangle geometries.  vtx[g]: FP32 xyz vertices (tightly packed), idx[g]: uint32
+// triangles (3 per primitive), xform[g]: row-major 3x4 or NULL.
+int oracle_blas_create(int ngeom, const float* const* vtx, const uint32_t* const* idx, const uint32_t* prim_count,
+                       const float* const* xform) {
+    Blas* b = new Blas();
+    for (int g = 0; g < ngeom; ++g) {
+        for (uint32_t p = 0; p < prim_count[g]; ++p) {
+            Tri t;
+            for (int k = 0; k < 3; ++k) {
+                const float* v = vtx[g] + 3 * (size_t)idx[g][3 * (size_t)p + k];
+                if (xform[g])
+                    xform_f32(xform[g], v, &t.v[3 * k]);
+                else
+                    std::memcpy(&t.v[3 * k], v, 12);
+            }
+            t.prim = p;
+            t.geom = (uint32_t)g;
+            b->tris.push_back(t);
+        }
+    }
+    size_t n = b->tris.size();
+    for (int a = 0; a < 3; ++a) {
+        b->lo[a] = 1e300;
+        b->hi[a] = -1e300;
+    }
+    std::vector<double> cen(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        double lo[3], hi[3];
+        tri_box(b->tris[i], lo, hi);
+        for (int a = 0; a < 3; ++a) {
+            cen[3 * i + a] = 0.5 * (lo[a] + hi[a]);
+            b->lo[a] = std::min(b->lo[a], lo[a]);
+            b->hi[a] = std::max(b->hi[a], hi[a]);
+        }
+    }
+    if (n) {
+        double R = 0, D2 = 0;
+        for (int a = 0; a < 3; ++a) {
+            R = std::max(R, std::max(std::fabs(b->lo[a]), std::fabs(b->hi[a])));
+            D2 += (b->hi[a] - b->lo[a]) * (b->hi[a] - b->lo[a]);
+        }
+        b->delta = kDeltaScale * (R + std::sqrt(D2));
+        b->order.resize(n);
+        for (size_t i = 0; i < n; ++i) b->order[i] = (uint32_t)i;
+        b->nodes.reserve(n);
+        build_rec(*b, cen, b->order.data(), (int)n, 0);
+    } else {
+        b->delta = 0;
+    }
+    g_blas.push_back(b);
+    return (int)g_blas.size() - 1;
+}
+
+// Instances: 64-B records in the VkAccelerationStructureInstanceKHR layout (main.cpp:848-858), except that
+// the last 8 bytes hold an oracle BLAS handle instead of a device address.
+int oracle_tlas_create(int ninst, const void* records) {
+    Tlas* t = new Tlas();
+    const uint8_t* p = (const uint8_t*)records;
+    for (int i = 0; i < ninst; ++i, p += 64) {
+        float m[12];
+        std::memcpy(m, p, 48);
+        uint32_t cm, sf;
+        uint64_t h;
+        std::memcpy(&cm, p + 48, 4);
+        std::memcpy(&sf, p + 52, 4);
+        std::memcpy(&h, p + 56, 8);
+        Inst in;
+        for (int k = 0; k < 12; ++k) in.m[k] = m[k];
+        invert3x4(in.m, in.minv);
+        in.custom = cm & 0xFFFFFFu;
+        in.mask = cm >> 24;
+        in.sbt_offset = sf & 0xFFFFFFu;
+        in.flags = sf >> 24;
+        in.blas = (int)h;
+        in.valid = (h < g_blas.size()) && std::isfinite(in.minv[0]);
+        t->inst.push_back(in);
+    }
+    g_tlas.push_back(t);
+    return (int)g_tlas.size() - 1;
+}
+
+void oracle_reset() {
+    for (auto* b : g_blas) delete b;
+    for (auto* t : g_tlas) delete t;
+    g_blas.clear();
+    g_tlas.clear();
+}
+
+double oracle_blas_delta(int blas) { return g_blas[blas]->delta; }
+
+// Trace n rays (32-B records: ox oy oz tmin dx dy dz tmax).  Outputs (any may be NULL):
+//   t_uv[3n] doubles, ids[4n] = prim, geom, inst (0xFFFFFFFF miss), custom index, status[n] (0 miss, 1 hit,
+//   2 epsilon-tie).  threads <= 0 uses hardware concurrency.
+void oracle_trace(int tlas, const float* rays, int64_t n, int any_hit, uint32_t cull_mask, double* t_uv,
+                  uint32_t* ids, int32_t* status, int threads) {
+    const Tlas& tl = *g_tlas[tlas];
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            int64_t s = next.fetch_add(256);
+            if (s >= n) break;
+            int64_t e = std::min<int64_t>(n, s + 256);
+            for (int64_t i = s; i < e; ++i) {
+                HitOut h;
+                trace_one(tl, rays + 8 * i, cull_mask, any_hit, h);
+                if (t_uv) {
+                    t_uv[3 * i] = h.t;
+                    t_uv[3 * i + 1] = h.u;
+                    t_uv[3 * i + 2] = h.v;
+                }
+                if (ids) {
+                    ids[4 * i] = h.prim;
+                    ids[4 * i + 1] = h.geom;
+                    ids[4 * i + 2] = h.inst;
+                    ids[4 * i + 3] = h.custom;
+                }
+                if (status) status[i] = h.status;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
+
+// Brute-force variant of oracle_trace for small scenes: every triangle of every instance, no BVH.
+void oracle_trace_brute(int tlas, const float* rays, int64_t n, uint32_t cull_mask, double* t_uv, uint32_t* ids,
+                        int32_t* status) {
+    const Tlas& tl = *g_tlas[tlas];
+    for (int64_t r = 0; r < n; ++r) {
+        const float* ray = rays + 8 * r;
+        d3 o{ray[0], ray[1], ray[2]}, d{ray[4], ray[5], ray[6]};
+        double tmin = ray[3], tmax = ray[7];
+        bool have = false;
+        double bt = 0, bu = 0, bv = 0;
+        uint32_t bp = 0, bg = 0, bi = 0xFFFFFFFFu;
+        bool tie = false;
+        for (size_t i = 0; i < tl.inst.size(); ++i) {
+            const Inst& in = tl.inst[i];
+            if (!in.valid || ((in.mask & cull_mask) == 0)) continue;
+            const Blas& b = *g_blas[in.blas];
+            const double* M = in.minv;
+            d3 oo{M[0] * o.x + M[1] * o.y + M[2] * o.z + M[3], M[4] * o.x + M[5] * o.y + M[6] * o.z + M[7],
+                  M[8] * o.x + M[9] * o.y + M[10] * o.z + M[11]};
+            d3 dd{M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z,
+                  M[8] * d.x + M[9] * d.y + M[10] * d.z};
+            RayFrame f = make_frame(oo, dd);
+            for (const Tri& tr : b.tris) {
+                Cand c;
+                if (!eval_tri(f, tr, b.delta, c)) continue;
+                if (c.near && c.t + c.tunc >= tmin && c.t - c.tunc <= tmax) tie = true;
+                if (c.inside && c.t >= tmin && c.t <= tmax) {
+                    bool better = !have || c.t < bt;
+                    if (better) {
+                        have = true;
+                        bt = c.t;
+                        bu = c.u;
+                        bv = c.v;
+                        bp = tr.prim;
+                        bg = tr.geom;
+                        bi = (uint32_t)i;
+                    }
+                }
+            }
+        }
+        if (t_uv) {
+            t_uv[3 * r] = bt;
+            t_uv[3 * r + 1] = bu;
+            t_uv[3 * r + 2] = bv;
+        }
+        if (ids) {
+            ids[4 * r] = bp;
+            ids[4 * r + 1] = bg;
+            ids[4 * r + 2] = bi;
+            ids[4 * r + 3] = have ? tl.inst[bi].custom : 0;
+        }
+        if (status) status[r] = tie ? 2 : (have ? 1 : 0);
+    }
+}
+
+// ---- raygen (main.cpp:1033-1055) ------------------------------------------------------------------------
+// aspect terms are computed once per launch in FP32, following the GLSL expression order:
+//   aspect_y = tan(radians(fov) * 0.5)   (evaluated in FP64 and rounded: GLSL leaves tan's precision open)
+//   aspect_x = aspect_y * float(W) / float(H)
+void oracle_aspect(float fov_deg, uint32_t w, uint32_t h, float* ax, float* ay) {
+    double rad = (double)fov_deg * (3.14159265358979323846 / 180.0);
+    float a_y = (float)std::tan(rad * 0.5);
+    float t = a_y * (float)w;
+    *ay = a_y;
+    *ax = t / (float)h;
+}
+
+// Primary rays of a width x height launch, row-major, 32-B records.
+void oracle_raygen(const float* cam, uint32_t w, uint32_t h, float tmin, float tmax, float* rays) {
+    float ax, ay;
+    oracle_aspect(cam[3], w, h, &ax, &ay);
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+            float sx = ((float)x + 0.5f) / (float)w;
+            float sy = ((float)y + 0.5f) / (float)h;
+            float nx = sx * 2.0f - 1.0f;
+            float ny = sy * 2.0f - 1.0f;
+            float* r = rays + 8 * ((size_t)y * w + x);
+            r[0] = cam[0];
+            r[1] = cam[1];
+            r[2] = cam[2];
+            r[3] = tmin;
+            r[4] = nx * ax;
+            r[5] = -(ny * ay);
+            r[6] = -1.0f;
+            r[7] = tmax;
+        }
+}
+
+// closest-hit / miss programs (main.cpp:1063-1066, 1080-1091) applied to oracle hits.
+// records: 12 floats each (48 B rtx_hit_record layout: program, color[3], emission[3], match[4], reserved).
+void oracle_shade(int tlas, int64_t n, const double* t_uv, const uint32_t* ids, const void* records,
+                  uint32_t nrec, uint32_t sbt_offset, uint32_t sbt_stride, const float* miss, float* film) {
+    const Tlas& tl = *g_tlas[tlas];
+    const uint8_t* rec = (const uint8_t*)records;
+    for (int64_t i = 0; i < n; ++i) {
+        float* px = film + 4 * i;
+        px[3] = 0.0f;
+        uint32_t inst = ids[4 * i + 2];
+        if (inst == 0xFFFFFFFFu) {
+            px[0] = miss[0];
+            px[1] = miss[1];
+            px[2] = miss[2];
+            continue;
+        }
+        uint32_t prim = ids[4 * i], geom = ids[4 * i + 1];
+        uint32_t r = tl.inst[inst].sbt_offset + geom * sbt_stride + sbt_offset;
+        if (r >= nrec) {
+            px[0] = px[1] = px[2] = 0.0f;
+            continue;
+        }
+        const uint8_t* R = rec + 48 * (size_t)r;
+        uint32_t prog;
+        float col[3];
+        int32_t match[4];
+        std::memcpy(&prog, R, 4);
+        std::memcpy(col, R + 4, 12);
+        std::memcpy(match, R + 28, 16);
+        float u = (float)t_uv[3 * i + 1], v = (float)t_uv[3 * i + 2];
+        bool bary = prog == 1u;
+        if (prog == 2u)
+            bary = (int32_t)prim == match[0] && (int32_t)inst == match[1] &&
+                   (int32_t)tl.inst[inst].custom == match[2] && (int32_t)geom == match[3];
+        if (bary) {
+            px[0] = 1.0f - u - v;
+            px[1] = u;
+            px[2] = v;
+        } else {
+            px[0] = col[0];
+            px[1] = col[1];
+            px[2] = col[2];
+        }
+    }
+}
+
+// ---- LBVH steps -------------------------------------------------------------------------------------------
+// Triangle boxes and centroids after the build-time transform; centroid = (lo + hi) * 0.5 in FP32.
+void oracle_centroids(int ngeom, const float* const* vtx, const uint32_t* const* idx, const uint32_t* prim_count,
+                      const float* const* xform, float* cen) {
+    size_t o = 0;
+    for (int g = 0; g < ngeom; ++g)
+        for (uint32_t p = 0; p < prim_count[g]; ++p, ++o) {
+            float v[9];
+            for (int k = 0; k < 3; ++k) {
+                const float* s = vtx[g] + 3 * (size_t)idx[g][3 * (size_t)p + k];
+                if (xform[g])
+                    xform_f32(xform[g], s, &v[3 * k]);
+                else
+                    std::memcpy(&v[3 * k], s, 12);
+            }
+            for (int a = 0; a < 3; ++a) {
+                float lo = std::fmin(v[a], std::fmin(v[3 + a], v[6 + a]));
+                float hi = std::fmax(v[a], std::fmax(v[3 + a], v[6 + a]));
+                float s = lo + hi;
+                cen[3 * o + a] = s * 0.5f;
+            }
+        }
+}
+
+// Morton codes over the centroid bounds.  bits = 30 (uint32 out) or 63 (uint64 out).
+// scale_a = float(2^k) / (cmax_a - cmin_a) (0 for a flat axis); q = trunc((c - cmin) * scale), clamped.
+void oracle_morton(const float* cen, int64_t n, int bits, void* out) {
+    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            cmin[a] = std::fmin(cmin[a], cen[3 * i + a]);
+            cmax[a] = std::fmax(cmax[a], cen[3 * i + a]);
+        }
+    float cells = bits == 30 ? 1024.0f : 2097152.0f;
+    uint32_t maxq = bits == 30 ? 1023u : 2097151u;
+    float scale[3];
+    for (int a = 0; a < 3; ++a) {
+        float ext = cmax[a] - cmin[a];
+        scale[a] = ext > 0.0f ? cells / ext : 0.0f;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; ++a) q[a] = quant(cen[3 * i + a], cmin[a], scale[a], maxq);
+        if (bits == 30)
+            ((uint32_t*)out)[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+        else
+            ((uint64_t*)out)[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+    }
+}
+
+// Karras 2012 hierarchy over n sorted keys (key_bits 32 or 64; keys passed as uint64).  Internal node i
+// (0..n-2) gets children left[i], right[i]; a child value c >= 0 is an internal node, c < 0 encodes leaf
+// (-1 - sorted position).  Equal keys are disambiguated by position: delta = key_bits + clz32(i ^ j).
+void oracle_karras(const uint64_t* keys, int64_t n, int key_bits, int32_t* left, int32_t* right) {
+    auto delta = [&](int64_t i, int64_t j) -> int {
+        if (j < 0 || j >= n) return -1;
+        uint64_t a = keys[i], b = keys[j];
+        if (a == b) return key_bits + clz32((uint32_t)(i ^ j));
+        return key_bits == 32 ? clz32((uint32_t)(a ^ b)) : clz64(a ^ b);
+    };
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        int d = (delta(i, i + 1) - delta(i, i - 1)) >= 0 ? 1 : -1;
+        int dmin = delta(i, i - d);
+        int64_t lmax = 2;
+        while (delta(i, i + lmax * d) > dmin) lmax *= 2;
+        int64_t l = 0;
+        for (int64_t t = lmax / 2; t >= 1; t /= 2)
+            if (delta(i, i + (l + t) * d) > dmin) l += t;
+        int64_t j = i + l * d;
+        int dnode = delta(i, j);
+        int64_t s = 0;
+        int64_t span = l;
+        // binary search for the split: largest s with delta(i, i + s*d) > dnode
+        int64_t t = span;
+        do {
+            t = (t + 1) / 2;
+            if (s + t < span + 1 && delta(i, i + (s + t) * d) > dnode) s += t;
+        } while (t > 1);
+        int64_t gamma = i + s * d + std::min(d, 0);
+        int64_t lo = std::min(i, j), hi = std::max(i, j);
+        left[i] = (lo == gamma) ? (int32_t)(-1 - gamma) : (int32_t)gamma;
+        right[i] = (hi == gamma + 1) ? (int32_t)(-1 - (gamma + 1)) : (int32_t)(gamma + 1);
+    }
+}
+
+// ---- PCG ----------------------------------------------------------------------------------------------------
+void oracle_pcg32(uint64_t initstate, uint64_t initseq, int n, uint32_t* out) {
+    Pcg32 r;
+    r.seed(initstate, initseq);
+    for (int i = 0; i < n; ++i) out[i] = r.next();
+}
+
+// pcg4d (Jarzynski & Olano 2020, "Hash Functions for GPU Rendering", JCGT 9(3)).
+void oracle_pcg4d(const uint32_t* in, int64_t n, uint32_t* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t x = in[4 * i], y = in[4 * i + 1], z = in[4 * i + 2], w = in[4 * i + 3];
+        x = x * 1664525u + 1013904223u;
+        y = y * 1664525u + 1013904223u;
+        z = z * 1664525u + 1013904223u;
+        w = w * 1664525u + 1013904223u;
+        x += y * w;
+        y += z * x;
+        z += x * y;
+        w += y * z;
+        x ^= x >> 16;
+        y ^= y >> 16;
+        z ^= z >> 16;
+        w ^= w >> 16;
+        x += y * w;
+        y += z * x;
+        z += x * y;
+        w += y * z;
+        out[4 * i] = x;
+        out[4 * i + 1] = y;
+        out[4 * i + 2] = z;
+        out[4 * i + 3] = w;
+    }
+}
+
+}  // extern "C"

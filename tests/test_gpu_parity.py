@@ -1,0 +1,236 @@
+"""GPU parity: librtx (HIP, gfx950) against the FP64 oracle on the same seeded inputs.
+
+Bar: hit instance / geometry / primitive IDs, miss/occluded decisions, Morton codes, sort order and the
+binary-radix-tree topology are bit-exact on every ray the oracle does not flag as an epsilon-tie; t is within
+1e-5 relative and the barycentrics within 1e-5 absolute of the FP64 values."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import compare_closest, oracle_scene, small_mixed_scene
+from pyrtx import scenes as S
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import pyrtx
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a GPU")
+    c = pyrtx.Context(0)
+    yield c
+    c.sync()
+    c.close()
+
+
+@pytest.fixture(autouse=True)
+def _reset_oracle():
+    O.reset()
+    yield
+    O.reset()
+
+
+def _trace_closest(ctx, tlas, rays):
+    import pyrtx
+    r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).cuda()
+    h = ctx.trace_ray_buffer(tlas, r, pyrtx.RTX_QUERY_CLOSEST_HIT)
+    ctx.sync()
+    return pyrtx.hits_to_numpy(h)
+
+
+def _trace_any(ctx, tlas, rays):
+    import pyrtx
+    r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).cuda()
+    o = ctx.trace_ray_buffer(tlas, r, pyrtx.RTX_QUERY_ANY_HIT)
+    ctx.sync()
+    return o.cpu().numpy().astype(np.uint32)
+
+
+# ---- build steps ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 5, 4095, 4096, 4097, 100003])
+@pytest.mark.parametrize("kb", [4, 8])
+def test_radix_sort_matches_stable_sort(ctx, n, kb):
+    _say(f"sort n={n} key_bytes={kb}")
+    rng = np.random.default_rng(n + kb)
+    if kb == 4:
+        keys = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+        keys[: n // 3] = keys[: n // 3] & 0xFF00  # many duplicates
+    else:
+        keys = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+        keys[: n // 3] = keys[: n // 3] & 0xFFFF000000000000
+    sk, order = ctx.debug_sort(keys)
+    ok, oo = O.sort_pairs(keys)
+    assert np.array_equal(sk, ok)
+    assert np.array_equal(order, oo)
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("n", [1, 2, 3, 1000, 50000])
+def test_morton_sort_karras_bit_exact(ctx, bits, n):
+    _say(f"lbvh bits={bits} n={n}")
+    rng = np.random.default_rng(bits * 7 + n)
+    lo = rng.uniform(-5, 5, (n, 3)).astype(np.float32)
+    ext = rng.uniform(0, 0.3, (n, 3)).astype(np.float32)
+    if n > 100:
+        lo[10:60] = lo[10]  # identical boxes -> equal codes -> index tie-breaks
+        ext[10:60] = ext[10]
+    boxes = np.concatenate([lo, lo + ext], 1).astype(np.float32)
+    codes, order, left, right = ctx.debug_lbvh(torch.from_numpy(boxes).cuda(), bits)
+    cen = ((boxes[:, :3] + boxes[:, 3:]) * np.float32(0.5)).astype(np.float32)
+    oc = O.morton(cen, bits)
+    ok, oo = O.sort_pairs(oc)
+    assert np.array_equal(codes, ok)
+    assert np.array_equal(order, oo)
+    if n > 1:
+        ol, orr = O.karras(ok, 32 if bits == 30 else 64)
+        assert np.array_equal(left, ol)
+        assert np.array_equal(right, orr)
+
+
+# ---- reference scenes -------------------------------------------------------------------------------------------
+def test_r0_film_matches_oracle(ctx):
+    _say("R0 film 1200x800")
+    sc = S.scene_r0()
+    blases, tlas = ctx.upload_scene(sc)
+    film = ctx.trace_rays(tlas, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    ctx.sync()
+    f = film.cpu().numpy()
+    t = oracle_scene(sc)
+    of, st = O.trace_rays(t, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    robust = st != 2
+    assert np.all(f[..., 3] == 0.0)
+    diff = np.abs(f - of)[robust]
+    assert diff.max() <= 1e-5, diff.max()
+    # record colours and the miss colour are bit-exact
+    rec_or_miss = robust & ~((of[..., 0] + of[..., 1] + of[..., 2] > 0.999) & (of[..., 0] + of[..., 1] + of[..., 2] < 1.001))
+    assert np.array_equal(f[rec_or_miss], of[rec_or_miss])
+    assert (st == 2).sum() < 500
+
+
+@pytest.mark.parametrize("z", [-2.5, 2.5])
+def test_c1_film_matches_oracle(ctx, z):
+    _say(f"C1 camera z={z}")
+    sc = S.scene_c1(z)
+    blases, tlas = ctx.upload_scene(sc)
+    film = ctx.trace_rays(tlas, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    ctx.sync()
+    f = film.cpu().numpy()
+    t = oracle_scene(sc)
+    of, st = O.trace_rays(t, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    robust = st != 2
+    assert np.abs(f - of)[robust].max() <= 1e-5
+    if z < 0:
+        assert np.all(st == 0)
+
+
+def test_raygen_rays_match_oracle_transcription():
+    # the numpy generator used for ray buffers and the oracle's raygen agree bit for bit
+    cam = (0.0, 0.0, 150.0, 60.0)
+    a = S.camera_rays(cam, 640, 360, 0.0, 100.0)
+    b = O.raygen(cam, 640, 360, 0.0, 100.0)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+# ---- ray buffers ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("seed", [3, 11])
+def test_mixed_scene_closest_and_any_hit(ctx, seed):
+    _say(f"mixed scene seed={seed}")
+    sc = small_mixed_scene(seed=seed, instances=24, freq=14, cells=20)
+    blases, tlas = ctx.upload_scene(sc)
+    t = oracle_scene(sc)
+    coh = S.camera_rays(sc.camera, 160, 120)
+    inc = S.incoherent_rays(sc, 20000, seed=seed)
+    for name, rays in (("coherent", coh), ("incoherent", inc)):
+        g = _trace_closest(ctx, tlas, rays)
+        tuv, ids, st = O.trace(t, rays)
+        rep = compare_closest(g, tuv, ids, st)
+        _say(f"  {name}: {rep}")
+        assert rep["robust_hits"] > 100
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
+        assert rep["t_bad"] == 0 and rep["uv_bad"] == 0
+        assert rep["ties"] < 0.01 * len(rays)
+        occ = _trace_any(ctx, tlas, rays)
+        _, _, sta = O.trace(t, rays, any_hit=True)
+        assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
+
+
+def test_query_scene_reduced_parity(ctx):
+    _say("query scene 1/200 scale")
+    sc = S.query_scene(scale_tris=0.005)
+    blases, tlas = ctx.upload_scene(sc)
+    t = oracle_scene(sc)
+    coh = S.camera_rays(sc.camera, 256, 256)
+    inc = S.incoherent_rays(sc, 30000)
+    for name, rays in (("coherent", coh), ("incoherent", inc)):
+        g = _trace_closest(ctx, tlas, rays)
+        tuv, ids, st = O.trace(t, rays)
+        rep = compare_closest(g, tuv, ids, st)
+        _say(f"  {name}: {rep}")
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
+        assert rep["t_bad"] == 0 and rep["uv_bad"] == 0
+
+
+def test_cull_mask_and_instance_mask(ctx):
+    _say("cull mask")
+    sc = small_mixed_scene(seed=5, instances=12, freq=8, cells=8)
+    sc.mask = np.where(np.arange(12) % 2 == 0, 0x01, 0x02).astype(np.uint32)
+    blases, tlas = ctx.upload_scene(sc)
+    t = oracle_scene(sc)
+    rays = S.incoherent_rays(sc, 5000, seed=2)
+    import pyrtx
+    r = torch.from_numpy(rays).cuda()
+    for cull in (0x01, 0x02, 0x03, 0x00):
+        h = pyrtx.hits_to_numpy(ctx.trace_ray_buffer(tlas, r, pyrtx.RTX_QUERY_CLOSEST_HIT, cull_mask=cull))
+        ctx.sync()
+        tuv, ids, st = O.trace(t, rays, cull_mask=cull)
+        rep = compare_closest(h, tuv, ids, st)
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
+        if cull == 0:
+            assert np.all(h[5] == 0xFFFFFFFF)
+
+
+def test_empty_and_single_triangle(ctx):
+    _say("degenerate sizes")
+    import pyrtx
+    # one triangle, one instance
+    sc = S.scene_c1(2.5)
+    blases, tlas = ctx.upload_scene(sc)
+    info = ctx.as_info(blases[0])
+    assert info.primitive_count == 1 and info.node_count == 1 and info.slot_count == 2
+    # TLAS with zero instances: every ray misses
+    empty = ctx.build_tlas(np.zeros(0, pyrtx.INSTANCE_DTYPE))
+    rays = torch.from_numpy(S.camera_rays(sc.camera, 16, 16)).cuda()
+    h = pyrtx.hits_to_numpy(ctx.trace_ray_buffer(empty, rays))
+    ctx.sync()
+    assert np.all(h[5] == 0xFFFFFFFF)
+
+
+def test_error_paths(ctx):
+    _say("error paths")
+    import pyrtx
+    v = torch.zeros((3, 3), dtype=torch.float32, device="cuda")
+    i = torch.zeros((1, 3), dtype=torch.int32, device="cuda")
+    geoms = [{"vertices": v, "indices": i, "transform": None}]
+    as_size, scratch_size = ctx.blas_build_sizes(geoms)
+    arr = ctx._geometry_structs(geoms)
+    import ctypes
+    dst = torch.empty(as_size, dtype=torch.uint8, device="cuda")
+    small = torch.empty(max(256, scratch_size // 2), dtype=torch.uint8, device="cuda")
+    addr = ctypes.c_uint64()
+    rc = ctx.lib.rtx_cmd_build_blas(ctx.handle, ctypes.cast(arr, ctypes.c_void_p), 1, 0, dst.data_ptr(), as_size,
+                                    small.data_ptr(), small.numel(), ctypes.byref(addr))
+    assert rc == -2  # RTX_ERROR_OUT_OF_DEVICE_MEMORY
+    rc = ctx.lib.rtx_cmd_trace_ray_buffer(ctx.handle, 0, None, 0, 0, 0xFF, None, None)
+    assert rc == -1002
+    with pytest.raises(pyrtx.RtxError):
+        ctx.set_config(block_size=100)

@@ -1,0 +1,292 @@
+"""CPU tests that pin the FP64 oracle before it is trusted as the checker.
+
+Each test checks the oracle against something it does not share code with: the reference scene's closed-form
+image (vulkan-raytraced-triangle/main.cpp constants), textbook definitions (bit interleaving, binary radix
+trees), brute force, published known answers (PCG32), and invariants."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import oracle_scene, small_mixed_scene
+from pyrtx import scenes as S
+
+
+@pytest.fixture(autouse=True)
+def _reset():
+    O.reset()
+    yield
+    O.reset()
+
+
+def r0_closed_form(W=1200, H=800):
+    """The R0 picture derived by hand: every hit has t = 10 (origin z = 10, dir.z = -1); hit point
+    (X, Y) = (10 ax ndc.x, -10 ay ndc.y); quads and colours from main.cpp:676-695, 835-858, 1080-1091, 1307-1317."""
+    ay = math.tan(math.radians(60.0) / 2)
+    ax = ay * W / H
+    x = np.arange(W)
+    y = np.arange(H)
+    nx = (x + 0.5) / W * 2 - 1
+    ny = (y + 0.5) / H * 2 - 1
+    X = np.broadcast_to(10 * ax * nx[None, :], (H, W))
+    Y = np.broadcast_to(-10 * ay * ny[:, None], (H, W))
+    img = np.zeros((H, W, 4))
+    img[..., 2] = 0.2
+    cols = {(0, 0): (0.6, 0.1, 0.2), (0, 1): (0.1, 0.8, 0.4), (1, 0): (0.9, 0.7, 0.1), (1, 1): (0.3, 0.6, 0.9)}
+    for inst, (y0, y1) in enumerate([(1, 3), (-3, -1)]):
+        for g, (x0, x1) in enumerate([(-3, -1), (1, 3)]):
+            m = (X >= x0) & (X <= x1) & (Y >= y0) & (Y <= y1)
+            img[m, :3] = cols[(inst, g)]
+            if (inst, g) == (1, 1):
+                px, py = X - 2, Y + 2
+                m2 = m & (px + py > 0)
+                img[m2, 0] = ((1 - py) / 2)[m2]
+                img[m2, 1] = ((px + py) / 2)[m2]
+                img[m2, 2] = ((1 - px) / 2)[m2]
+    # distance (in world units) to the nearest quad edge or tri-1 diagonal, to exclude edge pixels
+    d = np.full((H, W), np.inf)
+    for cx in (-2, 2):
+        for cy in (2, -2):
+            lx, ly = X - cx, Y - cy
+            inside = (np.abs(lx) <= 1) & (np.abs(ly) <= 1)
+            edge = np.minimum(np.abs(np.abs(lx) - 1), np.abs(np.abs(ly) - 1))
+            diag = np.abs(lx + ly) / math.sqrt(2)
+            d = np.where(inside | (edge < 1e-3), np.minimum(d, np.minimum(edge, diag)), d)
+    return img, d
+
+
+def test_r0_matches_closed_form():
+    sc = S.scene_r0()
+    t = oracle_scene(sc)
+    film, st = O.trace_rays(t, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    img, d = r0_closed_form()
+    assert film.shape == (800, 1200, 4)
+    assert np.all(film[..., 3] == 0.0)  # imageStore(vec4(hit, 0.0)), main.cpp:1054
+    clear = d > 1e-4
+    assert np.abs(film[clear] - img[clear]).max() < 2e-6
+    # every hit is at t = 10 exactly
+    rays = O.raygen(sc.camera, sc.width, sc.height)
+    tuv, ids, st2 = O.trace(t, rays)
+    assert np.all(tuv[st2 == 1, 0] == 10.0)
+    # ties only near edges/diagonals
+    assert np.all(d.ravel()[st2 == 2] < 1e-3)
+    # hit counts: 4 quads of 2x2 world units
+    assert (st2 == 1).sum() > 70000
+
+
+def test_c1_literal_all_miss_and_mirror_closed_form():
+    sc = S.scene_c1(-2.5)
+    t = oracle_scene(sc)
+    film, st = O.trace_rays(t, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    assert np.all(st == 0)
+    assert np.all(film[..., :3] == np.array([0, 0, 0.2], np.float32))
+    O.reset()
+    sc = S.scene_c1(2.5)
+    t = oracle_scene(sc)
+    film, st = O.trace_rays(t, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    W, H = sc.width, sc.height
+    ay = math.tan(math.radians(30.0))
+    ax = ay * W / H
+    x = np.arange(W)
+    y = np.arange(H)
+    X = np.broadcast_to(2.5 * ax * ((x + 0.5) / W * 2 - 1)[None, :], (H, W))
+    Y = np.broadcast_to(-2.5 * ay * ((y + 0.5) / H * 2 - 1)[:, None], (H, W))
+    v0, v1, v2 = np.array([1, 1.0]), np.array([-1, 1.0]), np.array([0, -1.0])
+    den = (v1[1] - v2[1]) * (v0[0] - v2[0]) + (v2[0] - v1[0]) * (v0[1] - v2[1])
+    b0 = ((v1[1] - v2[1]) * (X - v2[0]) + (v2[0] - v1[0]) * (Y - v2[1])) / den
+    b1 = ((v2[1] - v0[1]) * (X - v2[0]) + (v0[0] - v2[0]) * (Y - v2[1])) / den
+    b2 = 1 - b0 - b1
+    inside = (b0 > 1e-4) & (b1 > 1e-4) & (b2 > 1e-4)
+    outside = (b0 < -1e-4) | (b1 < -1e-4) | (b2 < -1e-4)
+    assert np.all(st[inside] == 1) and np.all(st[outside] == 0)
+    exp = np.stack([b0, b1, b2], -1)
+    assert np.abs(film[inside][:, :3] - exp[inside]).max() < 2e-6
+    rays = O.raygen(sc.camera, W, H)
+    tuv, ids, st2 = O.trace(t, rays)
+    assert np.allclose(tuv[st2 == 1, 0], 2.5, rtol=0, atol=1e-12)
+
+
+def test_bvh_equals_brute_force():
+    sc = small_mixed_scene(seed=3, instances=10, freq=6, cells=8)
+    t = oracle_scene(sc)
+    rng = np.random.default_rng(0)
+    rays = S.camera_rays(sc.camera, 64, 48)
+    inc = S.incoherent_rays(sc, 3000, seed=5)
+    for rr in (rays, inc):
+        a = O.trace(t, rr)
+        b = O.trace_brute(t, rr)
+        hit = b[2] == 1
+        assert np.array_equal(a[1][hit], b[1][hit])
+        assert np.array_equal(a[2] == 0, b[2] == 0)
+        assert np.abs(a[0][hit] - b[0][hit]).max() == 0.0
+    del rng
+
+
+def test_known_answers_axis_aligned():
+    verts = np.array([[0, 0, 0], [2, 0, 0], [0, 2, 0]], np.float32)
+    idx = np.array([[0, 1, 2]], np.uint32)
+    b = O.blas_create([(verts, idx, None)])
+    t = O.tlas_create(O.instance_records(S.translation(z=5.0)[None], [b]))
+    # ray from (0.5, 0.25, 10) straight down -z hits at z=5 -> t = 5, P local (0.5, 0.25): u = 0.25, v = 0.125
+    rays = np.array([[0.5, 0.25, 10, 0, 0, 0, -1, 100], [0.5, 0.25, 10, 0, 0, 0, -2, 100],
+                     [3, 3, 10, 0, 0, 0, -1, 100], [0.5, 0.25, 10, 0, 0, 0, -1, 4.9]], np.float32)
+    tuv, ids, st = O.trace(t, rays)
+    assert list(st) == [1, 1, 0, 0]
+    assert tuv[0, 0] == 5.0 and abs(tuv[0, 1] - 0.25) < 1e-15 and abs(tuv[0, 2] - 0.125) < 1e-15
+    assert tuv[1, 0] == 2.5  # unnormalised direction: t is parametric
+    assert ids[0, 2] == 0 and ids[2, 2] == O.MISS
+
+
+def test_watertight_fan_and_ties():
+    # 8-triangle fan around the origin: interior rays hit exactly one triangle, rays through the shared
+    # vertex or along shared edges are epsilon-ties, never misses.
+    k = 8
+    ang = np.linspace(0, 2 * np.pi, k, endpoint=False)
+    verts = np.concatenate([[[0, 0, 0]], np.stack([np.cos(ang), np.sin(ang), np.zeros(k)], 1)]).astype(np.float32)
+    idx = np.array([[0, 1 + i, 1 + (i + 1) % k] for i in range(k)], np.uint32)
+    b = O.blas_create([(verts, idx, None)])
+    t = O.tlas_create(O.instance_records(S.translation()[None], [b]))
+    rng = np.random.default_rng(1)
+    r = rng.uniform(0.05, 0.85, 2000)
+    a = rng.uniform(0, 2 * np.pi, 2000)
+    rays = np.zeros((2000, 8), np.float32)
+    rays[:, 0] = r * np.cos(a)
+    rays[:, 1] = r * np.sin(a)
+    rays[:, 2] = 1
+    rays[:, 6] = -1
+    rays[:, 7] = 10
+    tuv, ids, st = O.trace(t, rays)
+    assert np.all(st != 0)
+    sector = (np.floor(np.mod(a, 2 * np.pi) / (2 * np.pi / k)).astype(int)) % k
+    ok = st == 1
+    assert np.all(ids[ok, 0] == sector[ok])
+    # through the shared vertex and exactly along edges
+    edge = np.zeros((k + 1, 8), np.float32)
+    edge[:k, 0] = 0.5 * np.cos(ang)
+    edge[:k, 1] = 0.5 * np.sin(ang)
+    edge[:, 2] = 1
+    edge[:, 6] = -1
+    edge[:, 7] = 10
+    _, _, st = O.trace(t, edge)
+    assert np.all(st == 2)
+
+
+def _interleave(x, y, z, bits):
+    code = 0
+    for i in range(bits):
+        code |= ((x >> i) & 1) << (3 * i + 2)
+        code |= ((y >> i) & 1) << (3 * i + 1)
+        code |= ((z >> i) & 1) << (3 * i)
+    return code
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_morton_textbook(bits):
+    rng = np.random.default_rng(bits)
+    n = 4000
+    cen = rng.uniform(-3, 5, (n, 3)).astype(np.float32)
+    cen[:50] = cen[50]  # duplicates
+    codes = O.morton(cen, bits)
+    k = bits // 3
+    cells = np.float32(2 ** k)
+    cmin = cen.min(0)
+    cmax = cen.max(0)
+    scale = (cells / (cmax - cmin)).astype(np.float32)
+    f = ((cen - cmin).astype(np.float32) * scale).astype(np.float32)
+    q = np.where(f > 0, np.minimum(f, np.float32(2 ** k - 1)), 0).astype(np.uint64)
+    for i in range(0, n, 7):
+        assert int(codes[i]) == _interleave(int(q[i, 0]), int(q[i, 1]), int(q[i, 2]), k)
+    # exhaustive per-axis spread of 10-bit values
+    if bits == 30:
+        for axis in range(3):
+            c = np.zeros((1024, 3), np.float32)
+            c[:, axis] = np.arange(1024, dtype=np.float32) + 0.5
+            c[0, :] = 0.0
+            c[1023, :] = 1024.0
+            codes = O.morton(c, 30)
+            vals = np.arange(1024)
+            for v in vals[1:-1]:
+                qq = [0, 0, 0]
+                qq[axis] = int(v)
+                assert int(codes[v]) == _interleave(qq[0], qq[1], qq[2], 10)
+
+
+def _radix_tree(keys, bits):
+    """Textbook binary radix tree (top-down split at the first differing bit, index tie-break), labelled as in
+    Karras 2012: a range [a, b] is internal node a if it is a right child, b if it is a left child; root = 0."""
+    n = len(keys)
+    ext = [(int(k) << 32) | i for i, k in enumerate(keys)]
+    left = np.zeros(n - 1, np.int64)
+    right = np.zeros(n - 1, np.int64)
+    stack = [(0, n - 1, 0)]
+    while stack:
+        a, b, label = stack.pop()
+        x = ext[a] ^ ext[b]
+        hb = x.bit_length() - 1
+        # split: last position whose bit hb equals ext[a]'s
+        lo, hi = a, b
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if (ext[mid] >> hb) == (ext[a] >> hb):
+                lo = mid
+            else:
+                hi = mid
+        g = lo
+        left[label] = g if g > a else -1 - g
+        right[label] = g + 1 if g + 1 < b else -1 - (g + 1)
+        if g > a:
+            stack.append((a, g, g))
+        if g + 1 < b:
+            stack.append((g + 1, b, g + 1))
+    return left, right
+
+
+@pytest.mark.parametrize("bits,n", [(30, 1000), (30, 2), (63, 777)])
+def test_karras_equals_textbook_radix_tree(bits, n):
+    rng = np.random.default_rng(n)
+    kb = 32 if bits == 30 else 64
+    hi = 2 ** 30 if bits == 30 else 2 ** 62
+    keys = np.sort(rng.integers(0, hi, n, dtype=np.uint64)) if n > 2 else np.array([5, 5], np.uint64)
+    if n > 10:
+        keys[10:30] = keys[10]  # duplicate keys exercise the index tie-break
+        keys = np.sort(keys)
+    left, right = O.karras(keys, kb)
+    el, er = _radix_tree(keys, kb)
+    assert np.array_equal(left, el) and np.array_equal(right, er)
+    # invariants: every leaf once, every internal node except the root once
+    kids = np.concatenate([left, right])
+    leaves = -1 - kids[kids < 0]
+    internal = kids[kids >= 0]
+    assert np.array_equal(np.sort(leaves), np.arange(n))
+    assert np.array_equal(np.sort(internal), np.arange(1, n - 1))
+
+
+def test_pcg32_known_answers():
+    # pcg32-demo (O'Neill, pcg-c-basic): seed 42, sequence 54
+    got = O.pcg32(42, 54, 6)
+    assert [hex(int(x)) for x in got] == ["0xa15c02b7", "0x7b47f409", "0xba1d3330", "0x83d2f293", "0xbfa4784b",
+                                          "0xcbed606e"]
+
+
+def test_pcg4d_uniformity():
+    v = np.stack(np.meshgrid(np.arange(64), np.arange(64), np.arange(8), np.arange(2), indexing="ij"), -1)
+    out = O.pcg4d(v.reshape(-1, 4).astype(np.uint32))
+    assert out.shape == (64 * 64 * 8 * 2, 4)
+    for c in range(4):
+        h = np.bincount((out[:, c] >> 28).astype(np.int64), minlength=16)
+        e = out.shape[0] / 16
+        chi2 = ((h - e) ** 2 / e).sum()
+        assert chi2 < 50  # 15 dof
+    assert len(np.unique(out[:, 0])) > 0.99 * out.shape[0]
+
+
+def test_tie_rule_is_rare_on_query_scene_sample():
+    sc = S.query_scene(scale_tris=0.002, instances=64)
+    t = oracle_scene(sc)
+    rays = S.camera_rays(sc.camera, 96, 96)
+    tuv, ids, st = O.trace(t, rays)
+    assert (st == 2).mean() < 0.02
+    inc = S.incoherent_rays(sc, 4000)
+    tuv, ids, st = O.trace(t, inc)
+    assert (st == 2).mean() < 0.02 and (st == 1).sum() > 20
