@@ -21,7 +21,6 @@
 struct rtx_context_t {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool own_stream = false;
     std::string last_error;
     rtx::DeviceState* state = nullptr;
     uint2* gstack = nullptr;
@@ -121,23 +120,13 @@ int rtx_create_context(int device, void* hip_stream, rtx_context* out_ctx) {
         return RTX_ERROR_INITIALIZATION_FAILED;
     }
     ctx->device = device;
-    if (hip_stream) {
-        ctx->stream = (hipStream_t)hip_stream;
-    } else {
-        e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-        if (e != hipSuccess) {
-            delete ctx;
-            return RTX_ERROR_INITIALIZATION_FAILED;
-        }
-        ctx->own_stream = true;
-    }
+    ctx->stream = (hipStream_t)hip_stream;  // NULL: the default stream
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         ctx->cu_count = cus;
     e = hipMalloc(&ctx->state, sizeof(rtx::DeviceState));
     if (e == hipSuccess) e = hipMemset(ctx->state, 0, sizeof(rtx::DeviceState));
     if (e != hipSuccess) {
-        if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return RTX_ERROR_INITIALIZATION_FAILED;
     }
@@ -150,7 +139,6 @@ int rtx_destroy_context(rtx_context ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->gstack) (void)hipFree(ctx->gstack);
     if (ctx->state) (void)hipFree(ctx->state);
-    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RTX_SUCCESS;
 }

@@ -318,7 +318,8 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
                     cur_nslots = tlas_nslots;
                     steps = 0;
                     ng_base = 0;
-                    ng_bits = tlas_empty ? 0u : (1u | (1u << 8));  // the root: slot 0, one valid child
+                    // the root as a one-child group: child position 0 sits at ordered bit (0 ^ oct)
+                    ng_bits = tlas_empty ? 0u : ((1u << s.oct) | (1u << 8));
                     lg_bits = 0;
                     sp = 0;
                     hit_inst = kNoHit;
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
                             cur_inst = iid;
                             cur_nslots = bh->slot_count;
                             ng_base = 0;
-                            ng_bits = cur_nslots ? (1u | (1u << 8)) : 0u;
+                            ng_bits = cur_nslots ? ((1u << s.oct) | (1u << 8)) : 0u;
                         }
                     }
                 } else {

@@ -49,8 +49,8 @@ typedef enum rtx_result {
 
 typedef struct rtx_context_t* rtx_context;
 
-/* Create a context on HIP device `device`.  `hip_stream` is a hipStream_t to record on; NULL makes the
- * context create its own (default-priority, non-blocking) stream. */
+/* Create a context on HIP device `device`.  `hip_stream` is the hipStream_t to record on; NULL means the
+ * device's default (null) stream, as in HIP itself. */
 int rtx_create_context(int device, void* hip_stream, rtx_context* out_ctx);
 int rtx_destroy_context(rtx_context ctx);
 /* Last error message of this context (never NULL; "" when none). */

@@ -194,6 +194,7 @@ def test_cull_mask_and_instance_mask(ctx):
         ctx.sync()
         tuv, ids, st = O.trace(t, rays, cull_mask=cull)
         rep = compare_closest(h, tuv, ids, st)
+        _say(f"  cull {cull:#x}: {rep}, gpu hits {int((h[5] != 0xFFFFFFFF).sum())}")
         assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
         if cull == 0:
             assert np.all(h[5] == 0xFFFFFFFF)
