@@ -1,0 +1,219 @@
+"""Ray-query throughput on the 20M-triangle scene (one MI355X).
+
+Workload: 512 instances of 10 base meshes (5 seeded displaced geodesic icospheres of 0.51M-4.0M triangles and
+5 noise terrains of 2.1M triangles; 19,994,340 unique triangles) placed uniformly in a 100^3 box with uniform
+random rotations and scales in [0.5, 2] (seed 4242).  One step traces, as closest-hit queries, 16,777,216
+coherent primary rays of a 4096x4096 pinhole camera at (0, 0, 150) looking down -z (60 deg, the reference
+raygen model) and 16,777,216 incoherent rays (origins uniform in the scene box, directions uniform on the
+sphere, tmin 0, tmax inf; seed 1234).  Rays, hits and the acceleration structures are resident in HBM before
+the timed region.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "build-up-phase_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s closest-hit (coherent/incoherent, 20M tris); equal-relMSE time plain vs NRC"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=float, default=1.0, help="triangle-count scale of the scene (1.0 = 20M)")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays per set for the CPU baseline (0: skip)")
+    ap.add_argument("--no-anyhit", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (reference kernel for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import pyrtx
+    from pyrtx import scenes as S
+
+    t0 = time.time()
+    sc = S.query_scene(args.scale)
+    n_tris = sc.unique_triangle_count
+    log(f"scene generated in {time.time() - t0:.1f}s: {n_tris} unique triangles, {len(sc.blas_of_instance)} instances")
+    W = H = 4096
+    coh = S.camera_rays(sc.camera, W, H)
+    inc = S.incoherent_rays(sc, W * H, seed=1234)
+    log("rays generated")
+
+    ctx = pyrtx.Context(0)
+    dev = ctx.device
+    torch.cuda.synchronize()
+    tb = time.time()
+    blases, tlas = ctx.upload_scene(sc)
+    ctx.sync()
+    build_s = time.time() - tb
+    info = [ctx.as_info(b) for b in blases]
+    tinfo = ctx.as_info(tlas)
+    log(f"BLAS+TLAS built in {build_s * 1e3:.0f} ms (incl. upload); nodes {sum(i.node_count for i in info)}, "
+        f"slots {sum(i.slot_count for i in info)}, max depth {tinfo.max_depth}")
+    # time the BLAS builds alone (inputs already on the device)
+    rc = torch.from_numpy(coh).to(dev)
+    ri = torch.from_numpy(inc).to(dev)
+    del coh
+    hits = torch.empty((W * H, 6), dtype=torch.int32, device=dev)
+    occl = torch.empty((W * H,), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def trace(rays, mode=pyrtx.RTX_QUERY_CLOSEST_HIT):
+        return ctx.trace_ray_buffer(tlas, rays, mode, 0xFF, hits if mode == pyrtx.RTX_QUERY_CLOSEST_HIT else occl)
+
+    # traversal counters (separate, untimed pass)
+    stats = {}
+    for name, r in (("coherent", rc), ("incoherent", ri)):
+        ctx.set_stats(True)
+        trace(r)
+        ctx.sync()
+        s = ctx.stats()
+        ctx.set_stats(False)
+        n = max(1, s["rays"])
+        stats[name] = {k: s[k] / n for k in ("node_fetches", "triangle_tests", "instance_entries")}
+        hn = hits.cpu().numpy()
+        stats[name]["hit_fraction"] = float((hn[:, 5].view(np.uint32) != 0xFFFFFFFF).mean())
+    log(f"per-ray counters: {json.dumps(stats)}")
+
+    def bytes_per_ray(s):
+        # algorithmic bytes: ray 32 + hit 24 + 64 per node, 48 per triangle, 64 per instance entry
+        return 32 + 24 + 64 * s["node_fetches"] + 48 * s["triangle_tests"] + 64 * s["instance_entries"]
+
+    if args.calib_copy:
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("calibration copy: 268435456 bytes read + 268435456 bytes written")
+    for _ in range(args.warmup):
+        trace(rc)
+        trace(ri)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(4 * args.steps)]
+    log(f"timing {args.steps} steps of 2 x {W * H} rays")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    k = 0
+    for step in range(args.steps):
+        for r in (rc, ri):
+            ev[k][0].record()
+            trace(r)
+            ev[k][1].record()
+            k += 1
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    ctx.sync()
+    ms = [a.elapsed_time(b) for a, b in ev[:k]]
+    ms_coh = float(np.mean(ms[0::2]))
+    ms_inc = float(np.mean(ms[1::2]))
+    rays_per_step = 2 * W * H
+    value = rays_per_step * args.steps / t_total / 1e6
+    mr_coh = W * H / ms_coh / 1e3
+    mr_inc = W * H / ms_inc / 1e3
+    log(f"closest-hit: coherent {mr_coh:.1f} Mrays/s ({ms_coh:.2f} ms), incoherent {mr_inc:.1f} Mrays/s "
+        f"({ms_inc:.2f} ms), whole-step {value:.1f} Mrays/s")
+
+    any_res = {}
+    if not args.no_anyhit:
+        for name, r in (("coherent", rc), ("incoherent", ri)):
+            trace(r, pyrtx.RTX_QUERY_ANY_HIT)
+            ctx.sync()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(3):
+                trace(r, pyrtx.RTX_QUERY_ANY_HIT)
+            b.record()
+            torch.cuda.synchronize()
+            any_res[name] = W * H * 3 / a.elapsed_time(b) / 1e3
+        log(f"any-hit: {any_res}")
+
+    # roofline of the traversal kernel (both launches of a step)
+    b_coh, b_inc = bytes_per_ray(stats["coherent"]), bytes_per_ray(stats["incoherent"])
+    achieved = (b_coh * W * H + b_inc * W * H) / ((ms_coh + ms_inc) * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "trace_kernel<closest> (persistent BVH8 traversal)",
+                "bytes_per_ray": {"coherent": round(b_coh, 1), "incoherent": round(b_inc, 1)},
+                "launch_ms": {"coherent": round(ms_coh, 3), "incoherent": round(ms_inc, 3)}}
+
+    cpu = None
+    if args.cpu_sample > 0:
+        import oracle as O
+        O.reset()
+        tb = time.time()
+        handles = [O.blas_create([(g.vertices, g.indices, g.transform) for g in geoms]) for geoms in sc.blas]
+        ot = O.tlas_create(O.instance_records(sc.transforms, [handles[b] for b in sc.blas_of_instance],
+                                              sc.custom_index, sc.mask, sc.sbt_offset, sc.flags))
+        log(f"oracle BVH built in {time.time() - tb:.1f}s; timing CPU sample")
+        cores = min(os.cpu_count() or 1, 16)
+        step = max(1, (W * H) // args.cpu_sample)
+        coh_s = S.camera_rays(sc.camera, W, H)[::step][: args.cpu_sample]
+        inc_s = inc[::step][: args.cpu_sample]
+        O.trace(ot, coh_s[:4096], threads=cores)  # warm-up
+        tc = time.perf_counter()
+        O.trace(ot, coh_s, threads=cores)
+        O.trace(ot, inc_s, threads=cores)
+        dt = time.perf_counter() - tc
+        cpu = {"value": round((len(coh_s) + len(inc_s)) / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores,
+               "kind": "port",
+               "sample": f"{len(coh_s)} coherent + {len(inc_s)} incoherent closest-hit rays (every {step}th ray of "
+                         f"each set) traced by the FP64 oracle (scalar C++, BVH2, std::thread x {cores}) in "
+                         f"{dt:.1f}s"}
+        O.reset()
+        log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "20M-triangle ray-query scene (512 instances of 10 meshes in a 100^3 box); one step = "
+                               "16,777,216 coherent + 16,777,216 incoherent closest-hit rays",
+                   "unique_triangles": n_tris, "instances": int(len(sc.blas_of_instance)),
+                   "rays_per_step": rays_per_step, "camera": list(sc.camera), "image": [W, H]},
+        "closest_hit_mrays": {"coherent": round(mr_coh, 1), "incoherent": round(mr_inc, 1)},
+        "any_hit_mrays": {k: round(v, 1) for k, v in any_res.items()},
+        "traversal_counters_per_ray": stats,
+        "build": {"blas_tlas_ms_incl_upload": round(build_s * 1e3, 1),
+                  "nodes": int(sum(i.node_count for i in info)), "max_depth": int(tinfo.max_depth)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

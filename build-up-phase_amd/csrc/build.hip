@@ -344,8 +344,16 @@ __global__ __launch_bounds__(256) void karras_kernel(const K* __restrict__ keys,
 __device__ __forceinline__ float load_coherent(const float* p) {
     return __uint_as_float(__hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
+__device__ __forceinline__ void store_coherent(float* p, float v) {
+    __hip_atomic_store((uint32_t*)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // node boxes: index 0..n-2 internal, n-1+pos leaves (sorted position)
+//
+// Cross-thread hand-off without agent-scope fences (on gfx950 those write back and invalidate the whole L2 of
+// the XCD): boxes are published with agent-scope atomic stores, which are coherent across XCDs, the wave waits
+// for its stores to complete (s_waitcnt) before it bumps the parent's arrival count, and the second arriver
+// reads its sibling's box with agent-scope atomic loads, issued only after its counter atomic returned.
 __global__ __launch_bounds__(256) void refit_kernel(uint32_t n, const int32_t* __restrict__ left,
                                                     const int32_t* __restrict__ right,
                                                     const int32_t* __restrict__ parent,
@@ -356,26 +364,32 @@ __global__ __launch_bounds__(256) void refit_kernel(uint32_t n, const int32_t* _
     if (pos >= n) return;
     const float* pb = prim_boxes + 6 * (uint64_t)order[pos];
     float* lb = nboxes + 6 * ((uint64_t)n - 1 + pos);
+    float cur[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) lb[k] = pb[k];
+    for (int k = 0; k < 6; ++k) cur[k] = pb[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) store_coherent(lb + k, cur[k]);
+    uint64_t self = (uint64_t)n - 1 + pos;
     int32_t node = parent[n - 1 + pos];
     while (node >= 0) {
-        __threadfence();
-        uint32_t old = atomicAdd(&arrivals[node], 1u);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        uint32_t old = __hip_atomic_fetch_add(&arrivals[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (old == 0u) return;
-        __threadfence();
         int32_t l = left[node], r = right[node];
         uint64_t li = l >= 0 ? (uint64_t)l : (uint64_t)n - 1 + (uint64_t)(-1 - l);
         uint64_t ri = r >= 0 ? (uint64_t)r : (uint64_t)n - 1 + (uint64_t)(-1 - r);
-        float out[6];
+        const uint64_t sib = (li == self) ? ri : li;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            out[a] = fminf(load_coherent(nboxes + 6 * li + a), load_coherent(nboxes + 6 * ri + a));
-            out[3 + a] = fmaxf(load_coherent(nboxes + 6 * li + 3 + a), load_coherent(nboxes + 6 * ri + 3 + a));
+            cur[a] = fminf(cur[a], load_coherent(nboxes + 6 * sib + a));
+            cur[3 + a] = fmaxf(cur[3 + a], load_coherent(nboxes + 6 * sib + 3 + a));
         }
         float* nb = nboxes + 6 * (uint64_t)node;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) nb[k] = out[k];
+        for (int k = 0; k < 6; ++k) store_coherent(nb + k, cur[k]);
+        self = (uint64_t)node;
         node = parent[node];
     }
 }

@@ -235,3 +235,25 @@ def test_error_paths(ctx):
     assert rc == -1002
     with pytest.raises(pyrtx.RtxError):
         ctx.set_config(block_size=100)
+
+
+def test_cpp_host_sample_matches_oracle(tmp_path):
+    """The C++ transcription of main.cpp's flow on the C ABI renders the reference picture."""
+    import os
+    import subprocess
+    from conftest import PKG
+    exe = os.path.join(PKG, "host", "raytraced_triangle")
+    assert os.path.exists(exe), "build the sample first (make -C build-up-phase_amd)"
+    out = tmp_path / "r0.ppm"
+    _say("C++ host sample")
+    subprocess.run([exe, str(out)], check=True, timeout=120)
+    data = out.read_bytes()
+    assert data.startswith(b"P6\n1200 800\n255\n")
+    img = np.frombuffer(data[len(b"P6\n1200 800\n255\n"):], np.uint8).reshape(800, 1200, 3)
+    sc = S.scene_r0()
+    t = oracle_scene(sc)
+    of, st = O.trace_rays(t, sc.records, sc.camera, sc.width, sc.height, sc.miss_color)
+    ref = np.clip(of[..., :3], 0, 1) * 255.0
+    robust = st != 2
+    # 8-bit rounding of identical values: at most one code apart (half-way cases)
+    assert np.abs(img.astype(np.float64) - ref)[robust].max() <= 0.5 + 1e-3
