@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "build.h"
+#include "pathtrace.h"
 #include "rtx_internal.h"
 #include "trace.h"
 
@@ -29,6 +30,8 @@ struct rtx_context_t {
     rtx_trace_config cfg{256, 3, 12, 0};
     uint32_t gdepth = 64;
     bool stats = false;
+    void* pt_scratch = nullptr;
+    uint64_t pt_scratch_bytes = 0;
 };
 
 namespace {
@@ -138,6 +141,7 @@ int rtx_destroy_context(rtx_context ctx) {
     if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->gstack) (void)hipFree(ctx->gstack);
+    if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
     if (ctx->state) (void)hipFree(ctx->state);
     delete ctx;
     return RTX_SUCCESS;
@@ -348,6 +352,75 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
     q.gstack = ctx->gstack;
     hipError_t e = rtx::trace_rays_launch(q, L, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_trace_ray_buffer");
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                   const rtx_render_params* params, float* film_rgba32f_device, double* accum_rgb_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    int r = check_tlas(ctx, tlas_address);
+    if (r) return r;
+    if (!sbt || !camera || !params) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt, camera, params must be set");
+    const rtx_render_params& p = *params;
+    if (p.width == 0 || p.height == 0 || (uint64_t)p.width * p.height > (1ull << 28))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "film size must be 1..2^28 pixels");
+    if (p.max_depth == 0 || p.max_depth > 64) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "max_depth in 1..64");
+    if (p.light_triangle_count > 0 && !p.light_triangles_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "light triangles are NULL");
+    if (p.light_triangle_count > (1u << 20)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "too many lights");
+    if (!film_rgba32f_device && !accum_rgb_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "film and accumulator are both NULL");
+    if (sbt->hit_record_count > 0 && !sbt->hit_records_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt hit records are NULL");
+    const uint32_t npix = p.width * p.height;
+    const uint64_t need = rtx::pt_scratch_bytes(npix, p.max_depth, p.light_triangle_count);
+    if (need > ctx->pt_scratch_bytes) {
+        if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
+        ctx->pt_scratch = nullptr;
+        ctx->pt_scratch_bytes = 0;
+        hipError_t e = hipMalloc(&ctx->pt_scratch, need);
+        if (e != hipSuccess) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "path tracer queues: %s",
+                                         hipGetErrorString(e));
+        ctx->pt_scratch_bytes = need;
+    }
+    rtx::PtArgs a{};
+    rtx::pt_layout(a, ctx->pt_scratch, npix, p.max_depth, p.light_triangle_count);
+    a.tlas = (const void*)(uintptr_t)tlas_address;
+    a.width = p.width;
+    a.height = p.height;
+    a.npix = npix;
+    a.max_depth = p.max_depth;
+    a.rr_depth = p.rr_depth;
+    a.seed = p.seed;
+    for (int k = 0; k < 3; ++k) a.cam[k] = camera->position[k];
+    const double rad = (double)camera->y_fov_degree * (3.14159265358979323846 / 180.0);
+    a.ay = (float)std::tan(rad * 0.5);
+    const float t = a.ay * (float)p.width;
+    a.ax = t / (float)p.height;
+    a.ray_offset = p.ray_offset;
+    a.records = sbt->hit_records_device;
+    a.nrec = sbt->hit_record_count;
+    a.sbt_offset = sbt->sbt_record_offset;
+    a.sbt_stride = sbt->sbt_record_stride;
+    a.lights = p.light_triangles_device;
+    a.nlights = p.light_triangle_count;
+    a.state = ctx->state;
+    rtx::TraceLaunch L = launch_shape(ctx);
+    if ((r = ensure_gstack(ctx, L.grid))) return r;
+    a.gstack = ctx->gstack;
+    hipError_t e = rtx::pt_render(a, L, p.spp, p.sample_offset, film_rgba32f_device, accum_rgb_device, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render");
+    return RTX_SUCCESS;
+}
+
+int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays) {
+    if (!ctx || !out_rays) return RTX_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_render_ray_count");
+    rtx::DeviceState st;
+    e = hipMemcpy(&st, ctx->state, sizeof(st), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_render_ray_count copy");
+    *out_rays = st.pt_rays;
     return RTX_SUCCESS;
 }
 

@@ -1,0 +1,334 @@
+// pathtrace.hip — wavefront path tracer (Laine, Karras & Aila 2013, "Megakernels Considered Harmful") for gfx950.
+//
+// One pass renders one sample for every pixel.  Per path vertex (depth d):
+//   extend  persistent traversal over the extension queue (ray count read from device memory)  [trace.hip]
+//   shade   emission seen at d = 0, next-event estimation toward one light point (shadow queue), Russian
+//           roulette, cosine-weighted continuation (extension queue for d + 1); queues are compacted with one
+//           64-lane ballot + one atomic per wave
+//   shadow  any-hit traversal over the shadow queue                                               [trace.hip]
+//   gather  unoccluded shadow contributions are added to their path's radiance
+// After the last vertex the pass's radiance is added to the FP32 (and optionally FP64) accumulators.  Every path
+// owns one pixel in a pass, so no accumulation needs atomics and results do not depend on queue order.
+#include <cmath>
+
+#include "pathtrace.h"
+#include "rtx_internal.h"
+#include "sampling.h"
+
+namespace rtx {
+
+namespace {
+
+__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred, uint32_t* total) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) {
+        *total = 0;
+        return 0xFFFFFFFFu;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    *total = (uint32_t)__popcll(m);
+    return pred ? base + (uint32_t)__popcll(m & lt) : 0xFFFFFFFFu;
+}
+
+__global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, float* __restrict__ cdf) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* L = lights + 12 * (uint64_t)i;
+        const float e1x = L[3] - L[0], e1y = L[4] - L[1], e1z = L[5] - L[2];
+        const float e2x = L[6] - L[0], e2y = L[7] - L[1], e2z = L[8] - L[2];
+        const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+        acc += 0.5f * sqrtf(cx * cx + cy * cy + cz * cz);
+        cdf[i] = acc;
+    }
+    cdf[n] = acc;
+}
+
+__global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) a.counts[0] = a.npix;
+    if (i >= a.npix) return;
+    const uint32_t x = i % a.width, y = i / a.width;
+    const uint4 j = pcg4d(x, y, sample, a.seed ^ kJitterKey);
+    const float sx = ((float)x + unit_float(j.x)) / (float)a.width;
+    const float sy = ((float)y + unit_float(j.y)) / (float)a.height;
+    const float nx = sx * 2.0f - 1.0f;
+    const float ny = sy * 2.0f - 1.0f;
+    rtx_ray r;
+    r.origin[0] = a.cam[0];
+    r.origin[1] = a.cam[1];
+    r.origin[2] = a.cam[2];
+    r.tmin = 0.0f;
+    r.direction[0] = nx * a.ax;
+    r.direction[1] = -(ny * a.ay);
+    r.direction[2] = -1.0f;
+    r.tmax = INFINITY;
+    a.ext_rays[0][i] = r;
+    a.ext_pid[0][i] = i;
+    a.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    a.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, uint32_t sample, int cur) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t count = a.counts[3 * depth];
+    bool push_ext = false, push_sh = false;
+    rtx_ray ext{}, sh{};
+    float4 contrib = make_float4(0, 0, 0, 0);
+    float4 thr_new = make_float4(0, 0, 0, 0);
+    uint32_t pid = 0;
+    if (q < count) {
+        pid = a.ext_pid[cur][q];
+        const rtx_ray r = a.ext_rays[cur][q];
+        const PathHit h = a.hits[q];
+        if (h.rec != 0xFFFFFFFFu && h.rec < a.nrec) {
+            const rtx_hit_record rec = a.records[h.rec];
+            const float alb[3] = {rec.color[0], rec.color[1], rec.color[2]};
+            const bool lambert = rec.program == RTX_PROGRAM_LAMBERTIAN;
+            const float d[3] = {r.direction[0], r.direction[1], r.direction[2]};
+            const float dn = h.n[0] * d[0] + h.n[1] * d[1] + h.n[2] * d[2];
+            const bool front = dn < 0.0f;
+            float4 thr = a.thr[pid];
+            if (depth == 0 && front && lambert) {
+                float4 rd = a.rad[pid];
+                rd.x += thr.x * rec.emission[0];
+                rd.y += thr.y * rec.emission[1];
+                rd.z += thr.z * rec.emission[2];
+                a.rad[pid] = rd;
+            }
+            const float nf[3] = {front ? h.n[0] : -h.n[0], front ? h.n[1] : -h.n[1], front ? h.n[2] : -h.n[2]};
+            float P[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) P[k] = fmaf(h.t, d[k], r.origin[k]);
+            float org[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) org[k] = fmaf(a.ray_offset, nf[k], P[k]);
+            const bool reflective = alb[0] > 0.0f || alb[1] > 0.0f || alb[2] > 0.0f;
+            const uint32_t x = pid % a.width, y = pid / a.width;
+            const uint4 ra = pcg4d(x, y, sample, a.seed + 2u * depth);
+            const uint4 rb = pcg4d(x, y, sample, a.seed + 2u * depth + 1u);
+            if (reflective && a.nlights > 0) {
+                // next-event estimation: light triangle by area, point uniform on it
+                const float total = a.cdf[a.nlights];
+                const float ut = unit_float(ra.x) * total;
+                uint32_t lo = 0, hi = a.nlights - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (a.cdf[mid] > ut) hi = mid; else lo = mid + 1;
+                }
+                const float* L = a.lights + 12 * (uint64_t)lo;
+                const float su = sqrtf(unit_float(ra.y));
+                const float b0 = 1.0f - su, b1 = unit_float(ra.z) * su, b2 = 1.0f - b0 - b1;
+                float lp[3], e1[3], e2[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lp[k] = b0 * L[k] + b1 * L[3 + k] + b2 * L[6 + k];
+                    e1[k] = L[3 + k] - L[k];
+                    e2[k] = L[6 + k] - L[k];
+                }
+                float nl[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+                const float nll = sqrtf(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
+                float wi[3] = {lp[0] - org[0], lp[1] - org[1], lp[2] - org[2]};
+                const float d2 = wi[0] * wi[0] + wi[1] * wi[1] + wi[2] * wi[2];
+                if (nll > 0.0f && d2 > 0.0f) {
+                    const float inv_d = 1.0f / sqrtf(d2);
+                    const float cx = (nf[0] * wi[0] + nf[1] * wi[1] + nf[2] * wi[2]) * inv_d;
+                    const float cl = -(nl[0] * wi[0] + nl[1] * wi[1] + nl[2] * wi[2]) * inv_d / nll;
+                    if (cx > 0.0f && cl > 0.0f) {
+                        const float g = cx * cl / d2 * total * 0.31830988618379067154f;
+                        contrib.x = thr.x * alb[0] * L[9] * g;
+                        contrib.y = thr.y * alb[1] * L[10] * g;
+                        contrib.z = thr.z * alb[2] * L[11] * g;
+                        push_sh = contrib.x > 0.0f || contrib.y > 0.0f || contrib.z > 0.0f;
+                        sh.origin[0] = org[0];
+                        sh.origin[1] = org[1];
+                        sh.origin[2] = org[2];
+                        sh.tmin = 0.0f;
+                        sh.direction[0] = wi[0];
+                        sh.direction[1] = wi[1];
+                        sh.direction[2] = wi[2];
+                        sh.tmax = 1.0f - 1e-4f;
+                    }
+                }
+            }
+            if (reflective && depth + 1 < a.max_depth) {
+                thr_new = make_float4(thr.x * alb[0], thr.y * alb[1], thr.z * alb[2], 0.0f);
+                bool alive = true;
+                if (depth >= a.rr_depth) {
+                    const float p = fminf(0.95f, fmaxf(thr_new.x, fmaxf(thr_new.y, thr_new.z)));
+                    if (unit_float(ra.w) >= p) {
+                        alive = false;
+                    } else {
+                        thr_new.x /= p;
+                        thr_new.y /= p;
+                        thr_new.z /= p;
+                    }
+                }
+                if (alive) {
+                    float dir[3];
+                    cosine_dir(nf, unit_float(rb.x), unit_float(rb.y), dir);
+                    ext.origin[0] = org[0];
+                    ext.origin[1] = org[1];
+                    ext.origin[2] = org[2];
+                    ext.tmin = 0.0f;
+                    ext.direction[0] = dir[0];
+                    ext.direction[1] = dir[1];
+                    ext.direction[2] = dir[2];
+                    ext.tmax = INFINITY;
+                    push_ext = true;
+                }
+            }
+        }
+    }
+    uint32_t n_sh = 0, n_ext = 0;
+    const uint32_t so = wave_push(&a.counts[3 * depth + 1], push_sh, &n_sh);
+    const uint32_t eo = wave_push(&a.counts[3 * (depth + 1)], push_ext, &n_ext);
+    if (push_sh) {
+        a.sh_rays[so] = sh;
+        a.sh_pid[so] = pid;
+        a.sh_contrib[so] = contrib;
+    }
+    if (push_ext) {
+        a.ext_rays[cur ^ 1][eo] = ext;
+        a.ext_pid[cur ^ 1][eo] = pid;
+        a.thr[pid] = thr_new;
+    }
+    if ((threadIdx.x & 63u) == 0 && (n_sh + n_ext) > 0) atomicAdd(&a.state->pt_rays, (unsigned long long)(n_sh + n_ext));
+}
+
+__global__ __launch_bounds__(256) void shadow_gather_kernel(PtArgs a, uint32_t depth) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.counts[3 * depth + 1]) return;
+    if (a.occl[q]) return;
+    const uint32_t pid = a.sh_pid[q];
+    const float4 c = a.sh_contrib[q];
+    float4 r = a.rad[pid];
+    r.x += c.x;
+    r.y += c.y;
+    r.z += c.z;
+    a.rad[pid] = r;
+}
+
+__global__ __launch_bounds__(256) void accumulate_kernel(PtArgs a, double* __restrict__ acc64) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.npix) return;
+    const float4 r = a.rad[i];
+    float4 s = a.accum[i];
+    s.x += r.x;
+    s.y += r.y;
+    s.z += r.z;
+    a.accum[i] = s;
+    if (acc64) {
+        acc64[3 * (uint64_t)i] += (double)r.x;
+        acc64[3 * (uint64_t)i + 1] += (double)r.y;
+        acc64[3 * (uint64_t)i + 2] += (double)r.z;
+    }
+}
+
+__global__ __launch_bounds__(256) void resolve_kernel(PtArgs a, float inv_spp, float* __restrict__ film) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.npix) return;
+    const float4 s = a.accum[i];
+    *(float4*)(film + 4 * (uint64_t)i) = make_float4(s.x * inv_spp, s.y * inv_spp, s.z * inv_spp, 1.0f);
+}
+
+__global__ void fill_words_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void count_camera_rays_kernel(DeviceState* st, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->pt_rays += n;
+}
+
+}  // namespace
+
+uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
+    uint64_t n = npix ? npix : 1;
+    uint64_t b = 0;
+    b += 2 * align_up(n * sizeof(rtx_ray), 256) + 2 * align_up(n * 4, 256);  // extension queues
+    b += align_up(n * sizeof(PathHit), 256);                                  // hits
+    b += align_up(n * sizeof(rtx_ray), 256) + align_up(n * 4, 256) + align_up(n * 16, 256);  // shadow queue
+    b += align_up(n * 4, 256);                                                // occluded
+    b += 3 * align_up(n * 16, 256);                                           // thr, rad, accum
+    b += align_up((3ull * (max_depth + 2)) * 4, 256);                          // counts
+    b += align_up((nlights + 1ull) * 4, 256);                                 // light cdf
+    return b;
+}
+
+void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights) {
+    uint8_t* p = (uint8_t*)scratch;
+    uint64_t n = npix ? npix : 1;
+    auto take = [&](uint64_t bytes) {
+        uint8_t* r = p;
+        p += align_up(bytes, 256);
+        return r;
+    };
+    a.ext_rays[0] = (rtx_ray*)take(n * sizeof(rtx_ray));
+    a.ext_rays[1] = (rtx_ray*)take(n * sizeof(rtx_ray));
+    a.ext_pid[0] = (uint32_t*)take(n * 4);
+    a.ext_pid[1] = (uint32_t*)take(n * 4);
+    a.hits = (PathHit*)take(n * sizeof(PathHit));
+    a.sh_rays = (rtx_ray*)take(n * sizeof(rtx_ray));
+    a.sh_pid = (uint32_t*)take(n * 4);
+    a.sh_contrib = (float4*)take(n * 16);
+    a.occl = (uint32_t*)take(n * 4);
+    a.thr = (float4*)take(n * 16);
+    a.rad = (float4*)take(n * 16);
+    a.accum = (float4*)take(n * 16);
+    a.counts = (uint32_t*)take(3ull * (max_depth + 2) * 4);
+    a.cdf = (float*)take((nlights + 1ull) * 4);
+}
+
+hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t sample_offset, float* film,
+                     double* acc64, hipStream_t s) {
+    const unsigned gb = (a.npix + 255) / 256;
+    const uint64_t ncounts = 3ull * (a.max_depth + 2);
+    if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
+    fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
+    hipError_t e;
+    for (uint32_t si = 0; si < spp; ++si) {
+        const uint32_t sample = sample_offset + si;
+        fill_words_kernel<<<1, 64, 0, s>>>(a.counts, ncounts, 0u);
+        camera_kernel<<<gb, 256, 0, s>>>(a, sample);
+        count_camera_rays_kernel<<<1, 64, 0, s>>>(a.state, a.npix);
+        int cur = 0;
+        for (uint32_t d = 0; d < a.max_depth; ++d) {
+            TraceRequest ext{};
+            ext.tlas = a.tlas;
+            ext.mode = 3;
+            ext.n_ptr = a.counts + 3 * d;
+            ext.cull_mask = 0xFFu;
+            ext.rays = a.ext_rays[cur];
+            ext.path_hits = a.hits;
+            ext.sbt_offset = a.sbt_offset;
+            ext.sbt_stride = a.sbt_stride;
+            ext.state = a.state;
+            ext.gstack = a.gstack;
+            if ((e = trace_rays_launch(ext, L, s))) return e;
+            shade_kernel<<<gb, 256, 0, s>>>(a, d, sample, cur);
+            TraceRequest sh{};
+            sh.tlas = a.tlas;
+            sh.mode = 1;
+            sh.n_ptr = a.counts + 3 * d + 1;
+            sh.cull_mask = 0xFFu;
+            sh.rays = a.sh_rays;
+            sh.occluded = a.occl;
+            sh.state = a.state;
+            sh.gstack = a.gstack;
+            if ((e = trace_rays_launch(sh, L, s))) return e;
+            shadow_gather_kernel<<<gb, 256, 0, s>>>(a, d);
+            cur ^= 1;
+        }
+        accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
+    }
+    if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, film);
+    return hipGetLastError();
+}
+
+}  // namespace rtx

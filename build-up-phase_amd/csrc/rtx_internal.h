@@ -116,6 +116,16 @@ struct BuildTri {
 };
 static_assert(sizeof(BuildTri) == 48, "BuildTri must be 48 B");
 
+// Closest hit as the path tracer consumes it (48 B): world-space unit geometric normal and SBT record index.
+struct PathHit {
+    float t, u, v;
+    uint32_t inst, prim, geom;
+    float n[3];
+    uint32_t rec;  // 0xFFFFFFFF on a miss
+    uint32_t pad[2];
+};
+static_assert(sizeof(PathHit) == 48, "PathHit must be 48 B");
+
 // Context-owned device state.
 struct DeviceState {
     uint32_t ray_counter;
@@ -125,6 +135,7 @@ struct DeviceState {
     unsigned long long stat_tris;
     unsigned long long stat_insts;
     unsigned long long stat_rays;
+    unsigned long long pt_rays;  // rays traced by the path tracer (extension + shadow)
 };
 
 constexpr uint32_t kErrStackOverflow = 1u;

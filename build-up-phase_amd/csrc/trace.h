@@ -9,8 +9,10 @@ namespace rtx {
 
 struct TraceRequest {
     const void* tlas;
-    uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film
+    uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film, 3 path hits
     uint32_t n;
+    const uint32_t* n_ptr;  // device ray count (overrides n when set)
+    PathHit* path_hits;
     uint32_t cull_mask;
     const rtx_ray* rays;
     rtx_hit* hits;

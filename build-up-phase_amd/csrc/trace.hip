@@ -175,6 +175,8 @@ __device__ __forceinline__ bool test_triangle(const uint4 a, const uint4 b, cons
 struct TraceArgs {
     const uint8_t* tlas;
     const rtx_ray* rays;
+    const uint32_t* n_ptr;  // when set, the ray count is read from device memory (queues)
+    PathHit* path_hits;
     uint32_t n;
     uint32_t cull_mask;
     rtx_hit* hits;
@@ -195,7 +197,7 @@ struct TraceArgs {
     uint32_t gdepth;
 };
 
-enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2 };
+enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3 };
 
 // FP64 recomputation of (t, u, v) for the winning triangle (Moller & Trumbore 1997) in object space.
 __device__ void refine_hit(const double* inv, const float* wo, const float* wd, const TriSlot* tri, float& t,
@@ -226,6 +228,26 @@ __device__ void refine_hit(const double* inv, const float* wo, const float* wd, 
     v = (float)vv;
 }
 
+// World-space geometric normal (v1 - v0) x (v2 - v0) of a triangle under the instance transform whose inverse is
+// inv: n_world = inv^T n_object (FP64, normalised, rounded to FP32).
+__device__ void world_normal(const double* inv, const TriSlot* tri, float* n) {
+    double e1[3], e2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        e1[a] = (double)tri->v[3 + a] - (double)tri->v[a];
+        e2[a] = (double)tri->v[6 + a] - (double)tri->v[a];
+    }
+    double no[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    double nw[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nw[c] = inv[c] * no[0] + inv[4 + c] * no[1] + inv[8 + c] * no[2];
+    double l = sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+    double r = l > 0.0 ? 1.0 / l : 0.0;
+    n[0] = (float)(nw[0] * r);
+    n[1] = (float)(nw[1] * r);
+    n[2] = (float)(nw[2] * r);
+}
+
 template <int MODE, bool STATS>
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
     extern __shared__ uint2 lds_stack[];
@@ -242,6 +264,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
     const uint32_t tlas_ninst = th->prim_count;
     const uint32_t D = args.lds_depth;
     const uint32_t GD = args.gdepth;
+    const uint32_t nrays = args.n_ptr ? *args.n_ptr : args.n;
 
     uint64_t st_nodes = 0, st_tris = 0, st_insts = 0, st_rays = 0;
 
@@ -271,7 +294,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
             base = __shfl(base, leader);
             if (!active && !exhausted) {
                 const uint32_t id = base + (uint32_t)__popcll(need & lt_mask);
-                if (id < args.n) {
+                if (id < nrays) {
                     ray_id = id;
                     active = true;
                     if (MODE == kModeRaygen) {
@@ -547,7 +570,27 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
                         prim = tri->prim;
                         geom = tri->geom;
                     }
-                    if (MODE == kModeClosest) {
+                    if (MODE == kModePath) {
+                        PathHit ph;
+                        ph.t = t;
+                        ph.u = u;
+                        ph.v = v;
+                        ph.inst = hit_inst;
+                        ph.prim = prim;
+                        ph.geom = geom;
+                        ph.n[0] = ph.n[1] = ph.n[2] = 0.0f;
+                        ph.rec = 0xFFFFFFFFu;
+                        if (hit_inst != kNoHit) {
+                            double M[12];
+#pragma unroll
+                            for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
+                            const TriSlot* tri = (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes +
+                                                                  (uint64_t)hit_slot * kSlotBytes);
+                            world_normal(M, tri, ph.n);
+                            ph.rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
+                        }
+                        args.path_hits[ray_id] = ph;
+                    } else if (MODE == kModeClosest) {
                         uint32_t* hp = (uint32_t*)(args.hits + ray_id);
                         hp[0] = __float_as_uint(t);
                         hp[1] = __float_as_uint(u);
@@ -625,6 +668,8 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     TraceArgs a{};
     a.tlas = (const uint8_t*)r.tlas;
     a.rays = r.rays;
+    a.n_ptr = r.n_ptr;
+    a.path_hits = r.path_hits;
     a.n = r.n;
     a.cull_mask = r.cull_mask;
     a.hits = r.hits;
@@ -649,10 +694,11 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.lds_depth = L.lds_depth;
     a.gdepth = L.gdepth;
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
-    if (r.n == 0) return hipGetLastError();
+    if (r.n == 0 && !r.n_ptr) return hipGetLastError();
     switch (r.mode) {
         case 0: return r.stats ? launch<kModeClosest, true>(a, L, s) : launch<kModeClosest, false>(a, L, s);
         case 1: return r.stats ? launch<kModeAny, true>(a, L, s) : launch<kModeAny, false>(a, L, s);
+        case 3: return launch<kModePath, false>(a, L, s);
         default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
     }
 }

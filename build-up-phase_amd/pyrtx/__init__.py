@@ -84,6 +84,14 @@ class TraceStats(ctypes.Structure):
                 ("instance_entries", ctypes.c_uint64)]
 
 
+class RenderParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("rr_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32),
+                ("sample_offset", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("light_triangles_device", ctypes.c_void_p), ("light_triangle_count", ctypes.c_uint32),
+                ("ray_offset", ctypes.c_float)]
+
+
 class TraceConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("lds_stack_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -111,6 +119,9 @@ SIGNATURES = {
     "rtx_get_trace_stats": (ctypes.c_int, [_P, ctypes.POINTER(TraceStats)]),
     "rtx_set_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
     "rtx_get_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
+    "rtx_cmd_render": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), ctypes.POINTER(Camera),
+                                      ctypes.POINTER(RenderParams), _P, _P]),
+    "rtx_get_render_ray_count": (ctypes.c_int, [_P, ctypes.POINTER(_U64)]),
     "rtx_debug_lbvh": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _P]),
     "rtx_debug_sort": (ctypes.c_int, [_P, _P, _P, _U32, _U32]),
 }
@@ -297,6 +308,38 @@ class Context:
             self._check(self.lib.rtx_cmd_trace_ray_buffer(self.handle, tlas.address, rays.data_ptr(), n, mode,
                                                           cull_mask, None, out.data_ptr()))
         return out
+
+    def render(self, tlas: AccelerationStructure, records, camera, lights, width, height, spp=1, max_depth=8,
+               rr_depth=3, seed=1234, sample_offset=0, ray_offset=1e-4, film=None, accum64=None):
+        """Wavefront path tracer (rtx_cmd_render).  lights: numpy float32 [L, 12] (v0 v1 v2 emission) or a device
+        tensor.  Returns the film [H, W, 4] (mean radiance of these samples)."""
+        torch = _torch()
+        if isinstance(records, np.ndarray):
+            rec = _dev_tensor(np.ascontiguousarray(records).view(np.uint8).reshape(-1), self.device)
+        else:
+            rec = records
+        if isinstance(lights, np.ndarray):
+            lt = _dev_tensor(np.ascontiguousarray(lights, np.float32).reshape(-1, 12), self.device)
+        else:
+            lt = lights
+        nl = lt.shape[0] if lt is not None else 0
+        if film is None:
+            film = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
+        nrec = rec.numel() // 48
+        sbt = Sbt(rec.data_ptr() if nrec else None, nrec, 0, 1, 0, (ctypes.c_float * 3)(0, 0, 0), 0.0)
+        cam = Camera((ctypes.c_float * 3)(*camera[:3]), float(camera[3]))
+        p = RenderParams(width, height, spp, max_depth, rr_depth, seed, sample_offset, 0,
+                         lt.data_ptr() if nl else None, nl, ray_offset)
+        self._check(self.lib.rtx_cmd_render(self.handle, tlas.address, ctypes.byref(sbt), ctypes.byref(cam),
+                                            ctypes.byref(p), film.data_ptr(),
+                                            accum64.data_ptr() if accum64 is not None else None))
+        self._keep = (rec, lt)
+        return film
+
+    def render_ray_count(self) -> int:
+        n = ctypes.c_uint64()
+        self._check(self.lib.rtx_get_render_ray_count(self.handle, ctypes.byref(n)))
+        return n.value
 
     def set_stats(self, enable: bool):
         self._check(self.lib.rtx_set_trace_stats(self.handle, int(enable)))

@@ -339,3 +339,92 @@ def incoherent_rays(scene: Scene, n: int, seed: int = 1234, tmin: float = 0.0, t
     rays[:, 4:7] = d
     rays[:, 7] = tmax
     return rays
+
+
+# ---------------------------------------------------------------------------------------------------------------
+# path-tracing scenes
+# ---------------------------------------------------------------------------------------------------------------
+
+def _quad(a, b, c, d):
+    """Two triangles (a, b, c), (a, c, d); their geometric normal follows (b - a) x (c - a)."""
+    v = np.array([a, b, c, d], np.float32)
+    return v, np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+
+def _merge(parts):
+    vs, ix, off = [], [], 0
+    for v, i in parts:
+        vs.append(v)
+        ix.append(i + off)
+        off += len(v)
+    return np.concatenate(vs).astype(np.float32), np.concatenate(ix).astype(np.uint32)
+
+
+CORNELL_WHITE = (0.75, 0.75, 0.75)
+CORNELL_RED = (0.75, 0.15, 0.15)
+CORNELL_GREEN = (0.15, 0.75, 0.15)
+CORNELL_LIGHT_RADIANCE = 15.0
+
+
+def cornell_scene(sphere_level: int = 8, seed: int = 1234, width: int = 1920, height: int = 1080):
+    """Cornell-style unit box with a displaced icosphere.
+
+    24 box triangles: 6 walls of the unit cube (12; the front wall z = 1 closes the box behind the camera), an
+    open-bottom block (5 faces, 10) and a 0.25 x 0.25 light quad (2) just under the ceiling facing down with
+    radiance 15.  Albedos: walls and block 0.75 white, left wall (x = 0) red, right wall (x = 1) green.
+    Icosphere: subdivision level `sphere_level` (20 * 4^level triangles; level 8 = 1,310,720), radius 0.3,
+    centre (0.6, 0.32, 0.45), displaced along the normal by 3-octave value noise of amplitude 0.03 (seed).
+    Camera: the reference raygen model at (0.5, 0.5, 0.95) looking down -z, 60 deg.
+    Returns (Scene, lights [2, 12])."""
+    white = [_quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)),      # floor
+             _quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)),      # ceiling
+             _quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0)),      # back
+             _quad((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1))]      # front
+    x0, x1, z0, z1, h = 0.08, 0.30, 0.62, 0.84, 0.30
+    block = [_quad((x0, h, z0), (x0, h, z1), (x1, h, z1), (x1, h, z0)),
+             _quad((x0, 0, z0), (x1, 0, z0), (x1, h, z0), (x0, h, z0)),
+             _quad((x0, 0, z1), (x0, h, z1), (x1, h, z1), (x1, 0, z1)),
+             _quad((x0, 0, z0), (x0, h, z0), (x0, h, z1), (x0, 0, z1)),
+             _quad((x1, 0, z0), (x1, 0, z1), (x1, h, z1), (x1, h, z0))]
+    red = [_quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0))]
+    green = [_quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1))]
+    ly = 1.0 - 1e-3
+    lv, li = _quad((0.375, ly, 0.375), (0.625, ly, 0.375), (0.625, ly, 0.625), (0.375, ly, 0.625))
+    geoms = [Geometry(*_merge(white + block)), Geometry(*_merge(red)), Geometry(*_merge(green)), Geometry(lv, li)]
+    assert sum(g.indices.shape[0] for g in geoms) == 24
+    sph = icosphere(2 ** sphere_level, radius=0.3, seed=seed, amplitude=0.1, noise_freq=4.0,
+                    center=(0.6, 0.32, 0.45))
+    recs = hit_records([PROGRAM_LAMBERTIAN] * 5,
+                       [CORNELL_WHITE, CORNELL_RED, CORNELL_GREEN, (0.0, 0.0, 0.0), CORNELL_WHITE],
+                       emissions=[(0, 0, 0), (0, 0, 0), (0, 0, 0), (CORNELL_LIGHT_RADIANCE,) * 3, (0, 0, 0)])
+    lights = np.zeros((2, 12), np.float32)
+    for k in range(2):
+        lights[k, :9] = lv[li[k]].reshape(-1)
+        lights[k, 9:] = CORNELL_LIGHT_RADIANCE
+    sc = Scene(blas=[geoms, [sph]], transforms=np.stack([translation(), translation()]),
+               blas_of_instance=np.array([0, 1]), custom_index=np.zeros(2, np.uint32),
+               mask=np.full(2, 0xFF, np.uint32), sbt_offset=np.array([0, 4], np.uint32), flags=np.zeros(2, np.uint32),
+               records=recs, camera=(0.5, 0.5, 0.95, 60.0), width=width, height=height,
+               meta={"name": "cornell", "sphere_level": sphere_level})
+    return sc, lights
+
+
+def furnace_scene(albedo: float = 0.5, emission: float = 1.0, width: int = 64, height: int = 64):
+    """Closed unit cube whose 12 inward-facing triangles all emit `emission` and reflect `albedo`.  With NEE over
+    the same triangles the expected radiance is emission * (1 + a + ... + a^max_depth) (white furnace)."""
+    faces = [_quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0)),   # floor, normal +y
+             _quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)),   # ceiling, normal -y
+             _quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)),   # back z=0, normal +z
+             _quad((0, 0, 1), (0, 1, 1), (1, 1, 1), (1, 0, 1)),   # front z=1, normal -z
+             _quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)),   # left x=0, normal +x
+             _quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0))]   # right x=1, normal -x
+    v, i = _merge(faces)
+    recs = hit_records([PROGRAM_LAMBERTIAN], [(albedo,) * 3], emissions=[(emission,) * 3])
+    lights = np.zeros((12, 12), np.float32)
+    lights[:, :9] = v[i].reshape(12, 9)
+    lights[:, 9:] = emission
+    sc = Scene(blas=[[Geometry(v, i)]], transforms=translation()[None], blas_of_instance=np.array([0]),
+               custom_index=np.zeros(1, np.uint32), mask=np.full(1, 0xFF, np.uint32),
+               sbt_offset=np.zeros(1, np.uint32), flags=np.zeros(1, np.uint32), records=recs,
+               camera=(0.5, 0.5, 0.9, 60.0), width=width, height=height, meta={"name": "furnace"})
+    return sc, lights

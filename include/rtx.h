@@ -259,6 +259,39 @@ typedef struct rtx_trace_config {
 int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg);
 int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg);
 
+/* ---- path tracing: wavefront integrator (north-star extension of the raygen / closest-hit flow) ---------------
+ *
+ * Camera rays follow the reference raygen model (main.cpp:1033-1052) with the pixel position jittered inside the
+ * pixel.  Surfaces are Lambertian (hit records with RTX_PROGRAM_LAMBERTIAN: albedo = color, emitted radiance =
+ * emission on the side the geometric normal (v1-v0)x(v2-v0) points to).  Estimator per pixel sample:
+ *   vertex 0 adds the emission it sees; every vertex d < max_depth adds next-event estimation toward one point
+ *   chosen on the light triangles (triangle by area, point uniform); paths continue by cosine-weighted sampling,
+ *   with Russian roulette p = min(0.95, max(throughput)) when continuing from vertex d >= rr_depth.
+ * Random numbers are counter-based: pcg4d(x, y, sample, seed + 2d) and pcg4d(x, y, sample, seed + 2d + 1) per
+ * vertex, pcg4d(x, y, sample, seed ^ 0xA511E9B3) for the pixel jitter. */
+typedef struct rtx_render_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t spp;                 /* samples per pixel rendered by this call */
+    uint32_t max_depth;           /* path vertices, camera hit included (8) */
+    uint32_t rr_depth;            /* Russian roulette when continuing from vertex index >= rr_depth (3) */
+    uint32_t seed;                /* PCG seed (1234) */
+    uint32_t sample_offset;       /* index of this call's first sample (progressive rendering) */
+    uint32_t flags;               /* reserved, 0 */
+    const float* light_triangles_device; /* 12 floats per light: v0, v1, v2 (world), emitted radiance rgb */
+    uint32_t light_triangle_count;
+    float ray_offset;             /* secondary-ray origin offset along the normal, scene units (1e-4) */
+} rtx_render_params;
+
+/* Render params->spp samples per pixel.  film_rgba32f_device receives the mean radiance of these samples
+ * (alpha 1).  accum_rgb_device (nullable) receives += the per-pixel radiance sums in FP64 (3 doubles per pixel),
+ * for progressive FP64-accumulated references. */
+int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                   const rtx_render_params* params, float* film_rgba32f_device, double* accum_rgb_device);
+
+/* Rays traced by rtx_cmd_render since the context was created (camera, extension and shadow rays). */
+int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays);
+
 /* ---- verification hooks (no counterpart in the reference: they expose intermediate LBVH build steps so that
  * parity tests can compare them with the CPU oracle) ---------------------------------------------------------- */
 

@@ -59,6 +59,10 @@ def lib():
         L.oracle_pcg32.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
         L.oracle_pcg4d.restype = None
         L.oracle_pcg4d.argtypes = [P, ctypes.c_int64, P]
+        L.oracle_render.restype = None
+        L.oracle_render.argtypes = [ctypes.c_int, P, ctypes.c_uint32, P, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                    ctypes.c_uint32, P, ctypes.c_uint32, ctypes.c_float, P, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -223,3 +227,18 @@ def pcg4d(v: np.ndarray) -> np.ndarray:
     out = np.zeros_like(v)
     lib().oracle_pcg4d(_ptr(v), v.shape[0], _ptr(out))
     return out
+
+
+def render(tlas: int, records: np.ndarray, camera, lights: np.ndarray, w: int, h: int, spp: int,
+           sample_offset: int = 0, max_depth: int = 8, rr_depth: int = 3, seed: int = 1234,
+           ray_offset: float = 1e-4, threads: int = 0, accum: np.ndarray | None = None) -> np.ndarray:
+    """FP64 restatement of rtx_cmd_render: returns the per-pixel radiance sums [h, w, 3] (float64)."""
+    rec = np.ascontiguousarray(records)
+    assert rec.dtype.itemsize == 48
+    cam = np.ascontiguousarray(camera, dtype=np.float32).reshape(4)
+    lt = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 12)
+    if accum is None:
+        accum = np.zeros((h, w, 3), np.float64)
+    lib().oracle_render(tlas, _ptr(rec), len(rec), _ptr(cam), w, h, spp, sample_offset, max_depth, rr_depth, seed,
+                        _ptr(lt), lt.shape[0], ray_offset, _ptr(accum), threads)
+    return accum

@@ -260,6 +260,7 @@ struct Collector {
     bool have = false;
     double t = 0, u = 0, v = 0, tunc = 0;
     uint32_t prim = 0, geom = 0, inst = 0;
+    const Tri* tri = nullptr;
     bool tie = false;
     // list of "possibly hit" candidates (inside or near) for t-tie checks
     struct C {
@@ -325,6 +326,7 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
                     col.prim = tr.prim;
                     col.geom = tr.geom;
                     col.inst = inst_id;
+                    col.tri = &tr;
                 }
             }
         }
@@ -821,6 +823,194 @@ void oracle_pcg4d(const uint32_t* in, int64_t n, uint32_t* out) {
         out[4 * i + 2] = z;
         out[4 * i + 3] = w;
     }
+}
+
+// ---- path tracer (the estimator of rtx_cmd_render, evaluated in FP64) --------------------------------------
+}  // extern "C"
+
+namespace {
+
+struct PHit {
+    bool hit = false;
+    double t = 0;
+    uint32_t inst = 0, geom = 0, prim = 0;
+    double n[3] = {0, 0, 0};
+};
+
+// Closest opaque hit of the world ray (o, d) in [tmin, tmax] (FP64, no tie bookkeeping needed here).
+PHit pt_closest(const Tlas& tl, const double* o, const double* d, double tmin, double tmax) {
+    Collector col;
+    for (size_t i = 0; i < tl.inst.size(); ++i) {
+        const Inst& in = tl.inst[i];
+        if (!in.valid) continue;
+        const double* M = in.minv;
+        d3 oo{M[0] * o[0] + M[1] * o[1] + M[2] * o[2] + M[3], M[4] * o[0] + M[5] * o[1] + M[6] * o[2] + M[7],
+              M[8] * o[0] + M[9] * o[1] + M[10] * o[2] + M[11]};
+        d3 dd{M[0] * d[0] + M[1] * d[1] + M[2] * d[2], M[4] * d[0] + M[5] * d[1] + M[6] * d[2],
+              M[8] * d[0] + M[9] * d[1] + M[10] * d[2]};
+        if (dd.x == 0 && dd.y == 0 && dd.z == 0) continue;
+        RayFrame f = make_frame(oo, dd);
+        trace_blas(*g_blas[in.blas], f, tmin, tmax, (uint32_t)i, col);
+    }
+    PHit h;
+    if (!col.have) return h;
+    h.hit = true;
+    h.t = col.t;
+    h.inst = col.inst;
+    h.geom = col.geom;
+    h.prim = col.prim;
+    const Tri& tr = *col.tri;
+    double e1[3], e2[3];
+    for (int a = 0; a < 3; ++a) {
+        e1[a] = (double)tr.v[3 + a] - tr.v[a];
+        e2[a] = (double)tr.v[6 + a] - tr.v[a];
+    }
+    double no[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double* M = tl.inst[col.inst].minv;
+    double nw[3];
+    for (int c = 0; c < 3; ++c) nw[c] = M[c] * no[0] + M[4 + c] * no[1] + M[8 + c] * no[2];
+    double l = std::sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+    for (int c = 0; c < 3; ++c) h.n[c] = l > 0 ? nw[c] / l : 0.0;
+    return h;
+}
+
+inline double ufl(uint32_t v) { return (double)(v >> 8) * (1.0 / 16777216.0); }
+
+void pcg4d_one(uint32_t x, uint32_t y, uint32_t z, uint32_t w, uint32_t* o) {
+    uint32_t in[4] = {x, y, z, w};
+    oracle_pcg4d(in, 1, o);
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP64 restatement of rtx_cmd_render for pixels [0, W*H): adds each sample's radiance to accum (3 doubles per
+// pixel).  records: rtx_hit_record array (48 B each); lights: 12 floats per triangle.
+void oracle_render(int tlas, const void* records, uint32_t nrec, const float* cam, uint32_t W, uint32_t H,
+                   uint32_t spp, uint32_t sample_offset, uint32_t max_depth, uint32_t rr_depth, uint32_t seed,
+                   const float* lights, uint32_t nlights, float ray_offset, double* accum, int threads) {
+    const Tlas& tl = *g_tlas[tlas];
+    float axf, ayf;
+    oracle_aspect(cam[3], W, H, &axf, &ayf);
+    const double ax = axf, ay = ayf;
+    std::vector<double> cdf(nlights + 1, 0.0);
+    double acc = 0;
+    for (uint32_t i = 0; i < nlights; ++i) {
+        const float* L = lights + 12 * i;
+        double e1[3] = {(double)L[3] - L[0], (double)L[4] - L[1], (double)L[5] - L[2]};
+        double e2[3] = {(double)L[6] - L[0], (double)L[7] - L[1], (double)L[8] - L[2]};
+        double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        acc += 0.5 * std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        cdf[i] = acc;
+    }
+    const double total = acc;
+    const uint8_t* rec = (const uint8_t*)records;
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int64_t> next{0};
+    const int64_t npix = (int64_t)W * H;
+    auto work = [&]() {
+        for (;;) {
+            int64_t p = next.fetch_add(1);
+            if (p >= npix) break;
+            uint32_t x = (uint32_t)(p % W), y = (uint32_t)(p / W);
+            for (uint32_t si = 0; si < spp; ++si) {
+                uint32_t sample = sample_offset + si;
+                uint32_t j[4];
+                pcg4d_one(x, y, sample, seed ^ 0xA511E9B3u, j);
+                double sx = ((double)x + ufl(j[0])) / W, sy = ((double)y + ufl(j[1])) / H;
+                double o[3] = {cam[0], cam[1], cam[2]};
+                double d[3] = {(sx * 2 - 1) * ax, -((sy * 2 - 1) * ay), -1.0};
+                double thr[3] = {1, 1, 1}, rad[3] = {0, 0, 0};
+                for (uint32_t depth = 0; depth < max_depth; ++depth) {
+                    PHit h = pt_closest(tl, o, d, 0.0, INFINITY);
+                    if (!h.hit) break;
+                    uint32_t ri = tl.inst[h.inst].sbt_offset + h.geom;
+                    if (ri >= nrec) break;
+                    const uint8_t* R = rec + 48 * (size_t)ri;
+                    uint32_t prog;
+                    float colf[3], emf[3];
+                    std::memcpy(&prog, R, 4);
+                    std::memcpy(colf, R + 4, 12);
+                    std::memcpy(emf, R + 16, 12);
+                    const bool lambert = prog == 3u;
+                    double dn = h.n[0] * d[0] + h.n[1] * d[1] + h.n[2] * d[2];
+                    bool front = dn < 0;
+                    if (depth == 0 && front && lambert)
+                        for (int c = 0; c < 3; ++c) rad[c] += thr[c] * emf[c];
+                    double nf[3];
+                    for (int c = 0; c < 3; ++c) nf[c] = front ? h.n[c] : -h.n[c];
+                    double P[3], org[3];
+                    for (int c = 0; c < 3; ++c) {
+                        P[c] = o[c] + h.t * d[c];
+                        org[c] = P[c] + (double)ray_offset * nf[c];
+                    }
+                    bool reflective = colf[0] > 0 || colf[1] > 0 || colf[2] > 0;
+                    if (!reflective) break;
+                    uint32_t ra[4], rb[4];
+                    pcg4d_one(x, y, sample, seed + 2u * depth, ra);
+                    pcg4d_one(x, y, sample, seed + 2u * depth + 1u, rb);
+                    if (nlights > 0) {
+                        double ut = ufl(ra[0]) * total;
+                        uint32_t li = 0;
+                        while (li + 1 < nlights && cdf[li] <= ut) ++li;
+                        const float* L = lights + 12 * li;
+                        double su = std::sqrt(ufl(ra[1]));
+                        double b0 = 1 - su, b1 = ufl(ra[2]) * su, b2 = 1 - b0 - b1;
+                        double lp[3], e1[3], e2[3];
+                        for (int c = 0; c < 3; ++c) {
+                            lp[c] = b0 * L[c] + b1 * L[3 + c] + b2 * L[6 + c];
+                            e1[c] = (double)L[3 + c] - L[c];
+                            e2[c] = (double)L[6 + c] - L[c];
+                        }
+                        double nl[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                        e1[0] * e2[1] - e1[1] * e2[0]};
+                        double nll = std::sqrt(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
+                        double wi[3] = {lp[0] - org[0], lp[1] - org[1], lp[2] - org[2]};
+                        double d2 = wi[0] * wi[0] + wi[1] * wi[1] + wi[2] * wi[2];
+                        if (nll > 0 && d2 > 0) {
+                            double id = 1.0 / std::sqrt(d2);
+                            double cx = (nf[0] * wi[0] + nf[1] * wi[1] + nf[2] * wi[2]) * id;
+                            double cl = -(nl[0] * wi[0] + nl[1] * wi[1] + nl[2] * wi[2]) * id / nll;
+                            if (cx > 0 && cl > 0) {
+                                PHit sh = pt_closest(tl, org, wi, 0.0, 1.0 - 1e-4);
+                                if (!sh.hit) {
+                                    double g = cx * cl / d2 * total / 3.14159265358979323846;
+                                    for (int c = 0; c < 3; ++c) rad[c] += thr[c] * colf[c] * L[9 + c] * g;
+                                }
+                            }
+                        }
+                    }
+                    if (depth + 1 >= max_depth) break;
+                    double tn[3] = {thr[0] * colf[0], thr[1] * colf[1], thr[2] * colf[2]};
+                    if (depth >= rr_depth) {
+                        double pr = std::min(0.95, std::max(tn[0], std::max(tn[1], tn[2])));
+                        if (ufl(ra[3]) >= pr) break;
+                        for (int c = 0; c < 3; ++c) tn[c] /= pr;
+                    }
+                    for (int c = 0; c < 3; ++c) thr[c] = tn[c];
+                    // cosine-weighted direction around nf, basis of Duff et al. 2017
+                    double sign = std::copysign(1.0, nf[2]);
+                    double a = -1.0 / (sign + nf[2]);
+                    double bb = nf[0] * nf[1] * a;
+                    double t[3] = {1.0 + sign * nf[0] * nf[0] * a, sign * bb, -sign * nf[0]};
+                    double b[3] = {bb, sign + nf[1] * nf[1] * a, -nf[1]};
+                    double u1 = ufl(rb[0]), u2 = ufl(rb[1]);
+                    double r = std::sqrt(u1), phi = 2 * 3.14159265358979323846 * u2;
+                    double lx = r * std::cos(phi), ly = r * std::sin(phi), lz = std::sqrt(std::max(0.0, 1 - u1));
+                    for (int c = 0; c < 3; ++c) {
+                        d[c] = lx * t[c] + ly * b[c] + lz * nf[c];
+                        o[c] = org[c];
+                    }
+                }
+                for (int c = 0; c < 3; ++c) accum[3 * p + c] += rad[c];
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
 }
 
 }  // extern "C"

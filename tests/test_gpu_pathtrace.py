@@ -1,0 +1,92 @@
+"""GPU tests of the wavefront path tracer (rtx_cmd_render).
+
+The furnace tests check the GPU integrator against a closed form; the Cornell tests compare it with the FP64
+oracle's restatement of the same estimator fed by the same pcg4d random numbers, so most paths are identical
+and the images agree pixel by pixel up to FP32 rounding and rare path divergences."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import oracle_scene
+from pyrtx import scenes as S
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import pyrtx
+    c = pyrtx.Context(0)
+    yield c
+    c.sync()
+    c.close()
+
+
+@pytest.fixture(autouse=True)
+def _reset_oracle():
+    O.reset()
+    yield
+    O.reset()
+
+
+def relmse(x, ref):
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+@pytest.mark.parametrize("rr_depth", [8, 3])
+def test_furnace_closed_form(ctx, rr_depth):
+    _say(f"furnace rr_depth={rr_depth}")
+    a = 0.5
+    sc, lights = S.furnace_scene(a, 1.0, 64, 64)
+    blases, tlas = ctx.upload_scene(sc)
+    film = ctx.render(tlas, sc.records, sc.camera, lights, 64, 64, spp=256, max_depth=8, rr_depth=rr_depth)
+    ctx.sync()
+    f = film.cpu().numpy()
+    expected = (1 - a ** 9) / (1 - a)
+    mean = f[..., :3].mean()
+    assert abs(mean - expected) / expected < 0.005, (mean, expected)
+    assert np.all(f[..., 3] == 1.0)
+
+
+def test_cornell_matches_fp64_oracle(ctx):
+    _say("cornell vs oracle")
+    W, H, spp = 96, 54, 16
+    sc, lights = S.cornell_scene(4, width=W, height=H)
+    blases, tlas = ctx.upload_scene(sc)
+    film = ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=spp)
+    ctx.sync()
+    g = film.cpu().numpy()[..., :3].astype(np.float64)
+    t = oracle_scene(sc)
+    o = O.render(t, sc.records, sc.camera, lights, W, H, spp) / spp
+    err = relmse(g, o)
+    close = np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o))
+    _say(f"  relMSE(gpu, oracle) = {err:.3g}, pixels within 1e-3: {close.all(-1).mean():.4f}")
+    assert err < 1e-3
+    assert close.all(-1).mean() > 0.95
+
+
+def test_render_is_deterministic_and_accumulates_fp64(ctx):
+    _say("determinism")
+    W, H = 64, 36
+    sc, lights = S.cornell_scene(3, width=W, height=H)
+    blases, tlas = ctx.upload_scene(sc)
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+    f1 = ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=4, accum64=acc).cpu().numpy()
+    f2 = ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=4, accum64=acc).cpu().numpy()
+    ctx.sync()
+    assert np.array_equal(f1, f2)
+    a = acc.cpu().numpy()
+    assert np.allclose(a / 8.0, f1[..., :3].astype(np.float64), rtol=1e-5, atol=1e-6)
+    # progressive rendering: samples [4, 8) differ from [0, 4)
+    f3 = ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=4, sample_offset=4).cpu().numpy()
+    assert not np.array_equal(f1, f3)
+    n = ctx.render_ray_count()
+    assert n > 3 * W * H * 4
