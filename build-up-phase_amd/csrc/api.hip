@@ -10,11 +10,13 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "build.h"
+#include "nrc.h"
 #include "pathtrace.h"
 #include "rtx_internal.h"
 #include "trace.h"
@@ -34,9 +36,42 @@ struct rtx_context_t {
     uint64_t pt_scratch_bytes = 0;
 };
 
+struct rtx_nrc_t {
+    rtx_context ctx = nullptr;
+    rtx::NrcState st;
+    void* train_buf = nullptr;  // training samples of rtx_cmd_render_nrc
+    uint64_t train_buf_bytes = 0;
+};
+
 namespace {
 
 constexpr uint32_t kMaxPrims = (1u << 30) - 1u;
+
+// splitmix64 (Steele, Lea & Flood 2014): deterministic weight initialisation
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+void nrc_init_params(uint32_t seed, std::vector<float>& p) {
+    p.assign(RTX_NRC_PARAM_COUNT, 0.0f);
+    uint64_t st = seed;
+    const int in_dim[6] = {96, 64, 64, 64, 64, 64}, out_dim[6] = {64, 64, 64, 64, 64, 16};
+    const int fan_in[6] = {81, 64, 64, 64, 64, 64};
+    size_t off = 0;
+    for (int l = 0; l < 6; ++l) {
+        const double bound = std::sqrt(6.0 / fan_in[l]);
+        for (int r = 0; r < out_dim[l]; ++r)
+            for (int c = 0; c < in_dim[l]; ++c) {
+                const double u = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+                const bool used = c < fan_in[l] && (l < 5 || r < 3);
+                p[off + (size_t)r * in_dim[l] + c] = used ? (float)((2.0 * u - 1.0) * bound) : 0.0f;
+            }
+        off += (size_t)out_dim[l] * in_dim[l] + out_dim[l];  // biases stay 0
+    }
+}
 
 int fail(rtx_context ctx, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
 int fail(rtx_context ctx, int code, const char* fmt, ...) {
@@ -355,9 +390,8 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
     return RTX_SUCCESS;
 }
 
-int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
-                   const rtx_render_params* params, float* film_rgba32f_device, double* accum_rgb_device) {
-    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                        const rtx_render_params* params, bool have_out, rtx::PtArgs& a, rtx::TraceLaunch& L) {
     int r = check_tlas(ctx, tlas_address);
     if (r) return r;
     if (!sbt || !camera || !params) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt, camera, params must be set");
@@ -368,8 +402,7 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
     if (p.light_triangle_count > 0 && !p.light_triangles_device)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "light triangles are NULL");
     if (p.light_triangle_count > (1u << 20)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "too many lights");
-    if (!film_rgba32f_device && !accum_rgb_device)
-        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "film and accumulator are both NULL");
+    if (!have_out) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "film and accumulator are both NULL");
     if (sbt->hit_record_count > 0 && !sbt->hit_records_device)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt hit records are NULL");
     const uint32_t npix = p.width * p.height;
@@ -383,15 +416,17 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
                                          hipGetErrorString(e));
         ctx->pt_scratch_bytes = need;
     }
-    rtx::PtArgs a{};
+    a = rtx::PtArgs{};
     rtx::pt_layout(a, ctx->pt_scratch, npix, p.max_depth, p.light_triangle_count);
     a.tlas = (const void*)(uintptr_t)tlas_address;
     a.width = p.width;
     a.height = p.height;
     a.npix = npix;
+    a.npaths = npix;
     a.max_depth = p.max_depth;
     a.rr_depth = p.rr_depth;
     a.seed = p.seed;
+    a.cache_depth = 2;
     for (int k = 0; k < 3; ++k) a.cam[k] = camera->position[k];
     const double rad = (double)camera->y_fov_degree * (3.14159265358979323846 / 180.0);
     a.ay = (float)std::tan(rad * 0.5);
@@ -405,11 +440,48 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
     a.lights = p.light_triangles_device;
     a.nlights = p.light_triangle_count;
     a.state = ctx->state;
-    rtx::TraceLaunch L = launch_shape(ctx);
+    L = launch_shape(ctx);
     if ((r = ensure_gstack(ctx, L.grid))) return r;
     a.gstack = ctx->gstack;
-    hipError_t e = rtx::pt_render(a, L, p.spp, p.sample_offset, film_rgba32f_device, accum_rgb_device, ctx->stream);
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                   const rtx_render_params* params, float* film_rgba32f_device, double* accum_rgb_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    rtx::PtArgs a;
+    rtx::TraceLaunch L;
+    int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L);
+    if (r) return r;
+    hipError_t e = rtx::pt_render(a, L, params->spp, params->sample_offset, film_rgba32f_device, accum_rgb_device,
+                                  ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render");
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                       const rtx_render_params* params, rtx_nrc nrc, uint32_t train_paths,
+                       float* film_rgba32f_device, double* accum_rgb_device) {
+    if (!ctx || !nrc) return RTX_ERROR_INVALID_ARGUMENT;
+    rtx::PtArgs a;
+    rtx::TraceLaunch L;
+    int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L);
+    if (r) return r;
+    if (params->max_depth < 3) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "max_depth must be >= 3");
+    if (train_paths > a.npix) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "train_paths above the pixel count");
+    const uint64_t need = rtx::pt_train_bytes(train_paths, params->max_depth);
+    if (need > nrc->train_buf_bytes) {
+        if (nrc->train_buf) (void)hipFree(nrc->train_buf);
+        nrc->train_buf = nullptr;
+        nrc->train_buf_bytes = 0;
+        hipError_t e = hipMalloc(&nrc->train_buf, need);
+        if (e) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc training buffers: %s", hipGetErrorString(e));
+        nrc->train_buf_bytes = need;
+    }
+    uint32_t count = 0;
+    hipError_t e = rtx::pt_render_nrc(a, L, nrc->st, train_paths, nrc->train_buf, params->spp, params->sample_offset,
+                                      film_rgba32f_device, accum_rgb_device, &count, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render_nrc");
     return RTX_SUCCESS;
 }
 
@@ -421,6 +493,130 @@ int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays) {
     e = hipMemcpy(&st, ctx->state, sizeof(st), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_render_ray_count copy");
     *out_rays = st.pt_rays;
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc) {
+    if (!ctx || !cfg || !out_nrc) return RTX_ERROR_INVALID_ARGUMENT;
+    *out_nrc = nullptr;
+    if (cfg->batch_size == 0 || (cfg->batch_size % 64) || cfg->batch_size > (1u << 24))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "batch_size must be a positive multiple of 64 up to 2^24");
+    rtx_nrc n = new (std::nothrow) rtx_nrc_t();
+    if (!n) return RTX_ERROR_OUT_OF_HOST_MEMORY;
+    n->ctx = ctx;
+    rtx::NrcState& st = n->st;
+    st.batch_size = cfg->batch_size;
+    st.max_blocks = cfg->batch_size / 64;
+    st.lr = cfg->learning_rate;
+    st.beta1 = cfg->beta1;
+    st.beta2 = cfg->beta2;
+    st.eps = cfg->epsilon;
+    for (int a = 0; a < 3; ++a) {
+        st.lo[a] = cfg->scene_min[a];
+        st.hi[a] = cfg->scene_max[a];
+    }
+    const uint64_t pbytes = RTX_NRC_PARAM_COUNT * sizeof(float);
+    hipError_t e = hipMalloc(&st.params, pbytes);
+    if (!e) e = hipMalloc(&st.m, pbytes);
+    if (!e) e = hipMalloc(&st.v, pbytes);
+    if (!e) e = hipMalloc(&st.fwd, rtx::nrc_fragment_bytes());
+    st.bwd = nullptr;
+    if (!e) e = hipMalloc(&st.partials, (uint64_t)st.max_blocks * pbytes);
+    if (!e) e = hipMalloc(&st.loss, sizeof(float) * 4);
+    if (e) {
+        rtx_nrc_destroy(n);
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc buffers: %s", hipGetErrorString(e));
+    }
+    st.bwd = (uint8_t*)st.fwd + 46 * 64 * 16;
+    std::vector<float> p;
+    nrc_init_params(cfg->seed, p);
+    std::vector<float> zeros(RTX_NRC_PARAM_COUNT, 0.0f);
+    e = hipMemcpy(st.params, p.data(), pbytes, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(st.m, zeros.data(), pbytes, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(st.v, zeros.data(), pbytes, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(st.loss, zeros.data(), sizeof(float) * 4, hipMemcpyHostToDevice);
+    if (!e) e = rtx::nrc_pack(st, ctx->stream);
+    if (!e) e = hipStreamSynchronize(ctx->stream);
+    if (e) {
+        rtx_nrc_destroy(n);
+        return hip_fail(ctx, e, "rtx_nrc_create");
+    }
+    *out_nrc = n;
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_destroy(rtx_nrc n) {
+    if (!n) return RTX_ERROR_INVALID_ARGUMENT;
+    if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
+    rtx::NrcState& st = n->st;
+    if (st.params) (void)hipFree(st.params);
+    if (st.m) (void)hipFree(st.m);
+    if (st.v) (void)hipFree(st.v);
+    if (st.fwd) (void)hipFree(st.fwd);
+    if (st.partials) (void)hipFree(st.partials);
+    if (st.loss) (void)hipFree(st.loss);
+    if (n->train_buf) (void)hipFree(n->train_buf);
+    delete n;
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_cmd_infer(rtx_nrc n, const rtx_nrc_query* queries_device, uint32_t count, float* out_rgb_device) {
+    if (!n) return RTX_ERROR_INVALID_ARGUMENT;
+    if (count > 0 && (!queries_device || !out_rgb_device))
+        return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
+    if (!aligned(queries_device, 16)) return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "queries must be 16-B aligned");
+    hipError_t e = rtx::nrc_infer(n->st, queries_device, count, out_rgb_device, n->ctx->stream);
+    if (e) return hip_fail(n->ctx, e, "rtx_nrc_cmd_infer");
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_cmd_train(rtx_nrc n, const rtx_nrc_sample* samples_device, uint32_t count) {
+    if (!n) return RTX_ERROR_INVALID_ARGUMENT;
+    if (count > 0 && !samples_device) return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
+    if (!aligned(samples_device, 16)) return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "samples must be 16-B aligned");
+    for (uint64_t off = 0; off < count; off += n->st.batch_size) {
+        const uint32_t m = (uint32_t)std::min<uint64_t>(n->st.batch_size, count - off);
+        hipError_t e = rtx::nrc_step(n->st, samples_device + off, m, nullptr, nullptr, n->ctx->stream);
+        if (e) return hip_fail(n->ctx, e, "rtx_nrc_cmd_train");
+    }
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_debug_gradient(rtx_nrc n, const rtx_nrc_sample* samples_device, uint32_t count, float* grad_device,
+                           float* loss_device) {
+    if (!n || !grad_device || !loss_device) return RTX_ERROR_INVALID_ARGUMENT;
+    if (count == 0 || count > n->st.batch_size)
+        return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "count must be in 1..batch_size");
+    hipError_t e = rtx::nrc_step(n->st, samples_device, count, grad_device, loss_device, n->ctx->stream);
+    if (e) return hip_fail(n->ctx, e, "rtx_nrc_debug_gradient");
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_get_params(rtx_nrc n, float* params_host) {
+    if (!n || !params_host) return RTX_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(n->ctx->stream);
+    if (!e) e = hipMemcpy(params_host, n->st.params, RTX_NRC_PARAM_COUNT * sizeof(float), hipMemcpyDeviceToHost);
+    if (e) return hip_fail(n->ctx, e, "rtx_nrc_get_params");
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_set_params(rtx_nrc n, const float* params_host) {
+    if (!n || !params_host) return RTX_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(n->ctx->stream);
+    if (!e) e = hipMemcpy(n->st.params, params_host, RTX_NRC_PARAM_COUNT * sizeof(float), hipMemcpyHostToDevice);
+    if (!e) e = rtx::nrc_pack(n->st, n->ctx->stream);
+    if (e) return hip_fail(n->ctx, e, "rtx_nrc_set_params");
+    return RTX_SUCCESS;
+}
+
+int rtx_nrc_get_status(rtx_nrc n, uint32_t* steps, float* last_loss) {
+    if (!n) return RTX_ERROR_INVALID_ARGUMENT;
+    hipError_t e = hipStreamSynchronize(n->ctx->stream);
+    float l = 0.0f;
+    if (!e) e = hipMemcpy(&l, n->st.loss, sizeof(float), hipMemcpyDeviceToHost);
+    if (e) return hip_fail(n->ctx, e, "rtx_nrc_get_status");
+    if (steps) *steps = n->st.steps;
+    if (last_loss) *last_loss = l;
     return RTX_SUCCESS;
 }
 

@@ -1,0 +1,455 @@
+// nrc.hip — neural radiance cache (Muller, Rousselle, Novak & Keller 2021) on gfx950 MFMA.
+//
+// Network 96 -> 64 -> 64 -> 64 -> 64 -> 64 -> 16, ReLU on hidden layers, BF16 operands, FP32 accumulation
+// (v_mfma_f32_16x16x32_bf16), FP32 master weights, Adam.
+//
+// Layout trick: a layer is computed transposed, H^T (out x 16 samples) = W (out x in) * A^T (in x 16 samples),
+// with the weights as the MFMA A operand and 16 samples as the N dimension.  The accumulator of output tile t
+// holds, in lane l, rows 16t + 4(l/16) + i (i < 4) of sample l % 16.  For the next layer's B operand, lane l
+// must hold 8 k-values of sample l % 16 at k-group l / 16; we take the 4 rows of tile 2s and the 4 rows of
+// tile 2s + 1 that the lane already holds, i.e. the k order of step s is
+//     kperm(s, g, j) = 32s + 4g + j (j < 4),   32s + 16 + 4g + (j - 4) (j >= 4)
+// and pre-permute the next layer's weight columns the same way (fragment packing below).  Activations never
+// leave registers between layers.
+//
+// Training: one 256-thread block = 4 waves x 16 samples.  Forward activations are also written to LDS as
+// A^T[unit][64 samples] (bf16); the backward data gradient uses W^T as the A operand (same trick); the weight
+// gradient dW = G^T (out x 64) * A (64 x in) is an MFMA GEMM over the 64 samples of the block whose A operand
+// (G^T rows) and B operand (A^T rows) are 16-byte LDS reads.  Each block writes its partial gradient; a second
+// kernel sums the partials in block order (deterministic) and applies Adam.
+#include <cmath>
+
+#include "nrc.h"
+#include "rtx_internal.h"
+
+namespace rtx {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__host__ __device__ constexpr int in_dim(int l) { return l == 0 ? 96 : 64; }
+__host__ __device__ constexpr int out_dim(int l) { return l == 5 ? 16 : 64; }
+__host__ __device__ constexpr int w_off(int l) { return l == 0 ? 0 : 6208 + (l - 1) * 4160; }
+__host__ __device__ constexpr int b_off(int l) { return w_off(l) + out_dim(l) * in_dim(l); }
+// forward fragments (tiles over out, steps over in) and backward fragments (tiles over in, steps over out)
+__host__ __device__ constexpr int fwd_base(int l) { return l == 0 ? 0 : 12 + (l - 1) * 8; }
+__host__ __device__ constexpr int bwd_base(int l) { return (l - 1) * 8; }  // l = 1..5
+constexpr int kFwdFrags = 46;
+constexpr int kBwdFrags = 36;
+static_assert(b_off(5) + 16 == (int)RTX_NRC_PARAM_COUNT, "parameter layout");
+
+__device__ __forceinline__ int kperm(int s, int g, int j) { return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+struct Norm {
+    float lo[3];
+    float inv[3];
+};
+
+// Input feature f of a query (f < 96): f < 72 -> axis f/24, frequency k = (f%24)/2, sin (even) / cos (odd) of
+// 2^k pi p; 72..74 normal, 75..77 direction, 78..80 albedo, 81..95 zero.
+__device__ __forceinline__ float feature(int f, const float* pn, const rtx_nrc_query& q) {
+    if (f < 72) {
+        const int a = f / 24, r = f % 24, k = r >> 1;
+        const float p = a == 0 ? pn[0] : (a == 1 ? pn[1] : pn[2]);
+        const float x = ldexpf(p, k);
+        return (r & 1) ? cospif(x) : sinpif(x);
+    }
+    float v = 0.0f;
+    v = f == 72 ? q.normal[0] : v;
+    v = f == 73 ? q.normal[1] : v;
+    v = f == 74 ? q.normal[2] : v;
+    v = f == 75 ? q.direction[0] : v;
+    v = f == 76 ? q.direction[1] : v;
+    v = f == 77 ? q.direction[2] : v;
+    v = f == 78 ? q.albedo[0] : v;
+    v = f == 79 ? q.albedo[1] : v;
+    v = f == 80 ? q.albedo[2] : v;
+    return v;
+}
+
+__device__ __forceinline__ void load_query(const rtx_nrc_query* src, rtx_nrc_query& q) {
+    const float4* p = (const float4*)src;
+    const float4 a = p[0], b = p[1], c = p[2];
+    q.position[0] = a.x; q.position[1] = a.y; q.position[2] = a.z;
+    q.normal[0] = a.w; q.normal[1] = b.x; q.normal[2] = b.y;
+    q.direction[0] = b.z; q.direction[1] = b.w; q.direction[2] = c.x;
+    q.albedo[0] = c.y; q.albedo[1] = c.z; q.albedo[2] = c.w;
+}
+
+// B fragments of the input layer for the lane's sample: k = 32s + 8g + j.
+__device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, int g, bool valid, bf16x8* in0,
+                                       __bf16* at0, int col) {
+    float pn[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) pn[a] = (q.position[a] - nm.lo[a]) * nm.inv[a];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int f = 32 * s + 8 * g + j;
+            const float v = valid ? feature(f, pn, q) : 0.0f;
+            in0[s][j] = (__bf16)v;
+            if (at0) at0[f * 64 + col] = in0[s][j];
+        }
+}
+
+// One forward layer: acc[t] = W_l (tile t) * in + bias.
+template <int L>
+__device__ __forceinline__ void layer_forward(const bf16x8* __restrict__ fwd, const float* __restrict__ params,
+                                              const bf16x8* in, f32x4* acc, int lane) {
+    constexpr int T = out_dim(L) / 16, S = in_dim(L) / 32;
+    const int g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < S; ++s) c = mfma(fwd[(fwd_base(L) + t * S + s) * 64 + lane], in[s], c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] += params[b_off(L) + 16 * t + 4 * g + i];
+        acc[t] = c;
+    }
+}
+
+// ReLU and repack 4 output tiles into the 2 B fragments of the next layer (k order kperm).
+__device__ __forceinline__ void relu_pack(const f32x4* acc, bf16x8* out) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[s][j] = (__bf16)fmaxf(0.0f, acc[2 * s + (j >> 2)][j & 3]);
+}
+
+// Store the activations held in B fragments (kperm order) into LDS A^T[unit][64 samples].
+__device__ __forceinline__ void store_act(const bf16x8* in, __bf16* at, int g, int col) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) at[kperm(s, g, j) * 64 + col] = in[s][j];
+}
+
+template <int L>
+__device__ __forceinline__ void forward_all(const bf16x8* fwd, const float* params, const bf16x8* in0,
+                                            bf16x8 (*acts)[2], f32x4& out, int lane) {
+    f32x4 acc[4];
+    layer_forward<0>(fwd, params, in0, acc, lane);
+    relu_pack(acc, acts[0]);
+    layer_forward<1>(fwd, params, acts[0], acc, lane);
+    relu_pack(acc, acts[1]);
+    layer_forward<2>(fwd, params, acts[1], acc, lane);
+    relu_pack(acc, acts[2]);
+    layer_forward<3>(fwd, params, acts[2], acc, lane);
+    relu_pack(acc, acts[3]);
+    layer_forward<4>(fwd, params, acts[3], acc, lane);
+    relu_pack(acc, acts[4]);
+    f32x4 o[1];
+    layer_forward<5>(fwd, params, acts[4], o, lane);
+    out = o[0];
+}
+
+__global__ __launch_bounds__(256) void nrc_infer_kernel(const bf16x8* __restrict__ fwd,
+                                                        const float* __restrict__ params,
+                                                        const rtx_nrc_query* __restrict__ queries, uint32_t n,
+                                                        const uint32_t* __restrict__ n_dev, Norm nm,
+                                                        float* __restrict__ out_rgb) {
+    if (n_dev) n = min(n, *n_dev);
+    const int lane = threadIdx.x & 63;
+    const int c = lane & 15, g = lane >> 4;
+    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+    const uint64_t idx = wave * 16 + c;
+    const bool valid = idx < n;
+    rtx_nrc_query q{};
+    if (valid) load_query(queries + idx, q);
+    bf16x8 in0[3];
+    encode(q, nm, g, valid, in0, nullptr, 0);
+    bf16x8 acts[5][2];
+    f32x4 out;
+    forward_all<0>(fwd, params, in0, acts, out, lane);
+    if (valid && g == 0) {
+        out_rgb[3 * idx] = fmaxf(0.0f, out[0]);
+        out_rgb[3 * idx + 1] = fmaxf(0.0f, out[1]);
+        out_rgb[3 * idx + 2] = fmaxf(0.0f, out[2]);
+    }
+}
+
+// LDS of the training kernel (64 samples): A^T of the layer inputs and G^T of the current layer.
+struct TrainLds {
+    __bf16 at0[96 * 64];
+    __bf16 at[5][64 * 64];
+    __bf16 gt[64 * 64];
+    float red[8];
+};
+
+// Weight gradient of layer L for the block's 64 samples: dW tile (t, u) = G^T rows * A columns.
+template <int L>
+__device__ __forceinline__ void weight_grad(TrainLds& sm, const __bf16* at, float* __restrict__ part, int wave,
+                                            int lane) {
+    constexpr int T = out_dim(L) / 16, U = in_dim(L) / 16, N = T * U;
+    const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int k = wave; k < N; k += 4) {
+        const int t = k / U, u = k % U;
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const bf16x8 a = *(const bf16x8*)(sm.gt + (16 * t + r) * 64 + 32 * s + 8 * g);
+            const bf16x8 b = *(const bf16x8*)(at + (16 * u + r) * 64 + 32 * s + 8 * g);
+            c = mfma(a, b, c);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) part[w_off(L) + (16 * t + 4 * g + i) * in_dim(L) + 16 * u + r] = c[i];
+    }
+    // bias gradient: sum over the 64 samples of G^T rows
+    const int tid = threadIdx.x;
+    if (tid < out_dim(L)) {
+        float sum = 0.0f;
+        for (int b = 0; b < 64; ++b) sum += (float)sm.gt[tid * 64 + b];
+        part[b_off(L) + tid] = sum;
+    }
+}
+
+// Backward data gradient through layer L (1..5): G_{L-1} = (W_L^T G_L) masked by the ReLU of layer L-1.
+template <int L>
+__device__ __forceinline__ void back_data(const bf16x8* __restrict__ bwd, const bf16x8* gin, const bf16x8* act,
+                                          f32x4* gout, int lane) {
+    constexpr int S = (L == 5) ? 1 : 2;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < S; ++s) c = mfma(bwd[(bwd_base(L) + t * S + s) * 64 + lane], gin[s], c);
+        gout[t] = c;
+    }
+    // ReLU mask: element (t, i) sits at fragment s = t / 2, j = (t % 2) * 4 + i of the layer input
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (!((float)act[t >> 1][(t & 1) * 4 + i] > 0.0f)) gout[t][i] = 0.0f;
+}
+
+__device__ __forceinline__ void store_grad(const f32x4* gv, int tiles, __bf16* gt, int g, int col, bf16x8* frag) {
+    // C-layout gradient -> LDS G^T and -> B fragments (kperm order) for the next backward step
+    for (int t = 0; t < tiles; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) gt[(16 * t + 4 * g + i) * 64 + col] = (__bf16)gv[t][i];
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int t = 2 * s + (j >> 2);
+            frag[s][j] = t < tiles ? (__bf16)gv[t][j & 3] : (__bf16)0.0f;
+        }
+}
+
+__global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict__ fwd, const bf16x8* __restrict__ bwd,
+                                                        const float* __restrict__ params,
+                                                        const rtx_nrc_sample* __restrict__ samples, uint32_t n,
+                                                        uint32_t stride, uint32_t offset, float inv_n, Norm nm,
+                                                        float* __restrict__ partials, float* __restrict__ loss_sum) {
+    __shared__ TrainLds sm;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int col = wave * 16 + c;  // sample column within the block
+    const uint64_t idx = (uint64_t)blockIdx.x * 64 + col;
+    const bool valid = idx < n;
+    float* part = partials + (uint64_t)blockIdx.x * RTX_NRC_PARAM_COUNT;
+
+    rtx_nrc_sample smp{};
+    if (valid) {
+        const uint64_t si = idx * stride + offset;
+        load_query(&samples[si].query, smp.query);
+        smp.target[0] = samples[si].target[0];
+        smp.target[1] = samples[si].target[1];
+        smp.target[2] = samples[si].target[2];
+    }
+    // forward, keeping every layer input in registers (B fragments) and in LDS (A^T)
+    bf16x8 in0[3];
+    encode(smp.query, nm, g, valid, in0, sm.at0, col);
+    bf16x8 acts[5][2];
+    f32x4 acc[4];
+    layer_forward<0>(fwd, params, in0, acc, lane);
+    relu_pack(acc, acts[0]);
+    store_act(acts[0], sm.at[0], g, col);
+    layer_forward<1>(fwd, params, acts[0], acc, lane);
+    relu_pack(acc, acts[1]);
+    store_act(acts[1], sm.at[1], g, col);
+    layer_forward<2>(fwd, params, acts[1], acc, lane);
+    relu_pack(acc, acts[2]);
+    store_act(acts[2], sm.at[2], g, col);
+    layer_forward<3>(fwd, params, acts[2], acc, lane);
+    relu_pack(acc, acts[3]);
+    store_act(acts[3], sm.at[3], g, col);
+    layer_forward<4>(fwd, params, acts[3], acc, lane);
+    relu_pack(acc, acts[4]);
+    store_act(acts[4], sm.at[4], g, col);
+    f32x4 out[1];
+    layer_forward<5>(fwd, params, acts[4], out, lane);
+
+    // relative L2 loss on RGB (rows 0..2 of the output tile, held by k-group 0)
+    f32x4 gy[1] = {{0.0f, 0.0f, 0.0f, 0.0f}};
+    float loss = 0.0f;
+    if (valid && g == 0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float y = out[0][i], t = smp.target[i];
+            const float den = y * y + 0.01f;
+            gy[0][i] = 2.0f * (y - t) / den * inv_n;
+            loss += (y - t) * (y - t) / den;
+        }
+    }
+    bf16x8 gfrag[2];
+    store_grad(gy, 1, sm.gt, g, col, gfrag);
+    __syncthreads();
+    weight_grad<5>(sm, sm.at[4], part, wave, lane);
+    f32x4 gv[4];
+    back_data<5>(bwd, gfrag, acts[4], gv, lane);
+    __syncthreads();
+    store_grad(gv, 4, sm.gt, g, col, gfrag);
+    __syncthreads();
+    weight_grad<4>(sm, sm.at[3], part, wave, lane);
+    back_data<4>(bwd, gfrag, acts[3], gv, lane);
+    __syncthreads();
+    store_grad(gv, 4, sm.gt, g, col, gfrag);
+    __syncthreads();
+    weight_grad<3>(sm, sm.at[2], part, wave, lane);
+    back_data<3>(bwd, gfrag, acts[2], gv, lane);
+    __syncthreads();
+    store_grad(gv, 4, sm.gt, g, col, gfrag);
+    __syncthreads();
+    weight_grad<2>(sm, sm.at[1], part, wave, lane);
+    back_data<2>(bwd, gfrag, acts[1], gv, lane);
+    __syncthreads();
+    store_grad(gv, 4, sm.gt, g, col, gfrag);
+    __syncthreads();
+    weight_grad<1>(sm, sm.at[0], part, wave, lane);
+    back_data<1>(bwd, gfrag, acts[0], gv, lane);
+    __syncthreads();
+    store_grad(gv, 4, sm.gt, g, col, gfrag);
+    __syncthreads();
+    weight_grad<0>(sm, sm.at0, part, wave, lane);
+
+    // block loss
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) sm.red[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss_sum, (sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3]) * inv_n);
+}
+
+// Sum the block partials (block order: deterministic) and take one Adam step, or only write the gradient.
+__global__ __launch_bounds__(256) void nrc_adam_kernel(const float* __restrict__ partials, uint32_t nblk,
+                                                       float* __restrict__ params, float* __restrict__ m,
+                                                       float* __restrict__ v, float lr, float b1, float b2, float eps,
+                                                       float c1, float c2, float* __restrict__ grad_out) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= RTX_NRC_PARAM_COUNT) return;
+    float gsum = 0.0f;
+    for (uint32_t b = 0; b < nblk; ++b) gsum += partials[(uint64_t)b * RTX_NRC_PARAM_COUNT + p];
+    if (grad_out) {
+        grad_out[p] = gsum;
+        return;
+    }
+    const float mm = b1 * m[p] + (1.0f - b1) * gsum;
+    const float vv = b2 * v[p] + (1.0f - b2) * gsum * gsum;
+    m[p] = mm;
+    v[p] = vv;
+    params[p] -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+}
+
+// Rebuild the BF16 MFMA fragments from the FP32 master parameters.
+__global__ __launch_bounds__(256) void nrc_pack_kernel(const float* __restrict__ params, bf16x8* __restrict__ fwd,
+                                                       bf16x8* __restrict__ bwd) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = (kFwdFrags + kBwdFrags) * 64;
+    if (id >= total) return;
+    const int lane = id & 63, frag = id >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    bf16x8 out;
+    if (frag < kFwdFrags) {
+        int l = 0;
+        while (l < 5 && frag >= fwd_base(l + 1)) ++l;
+        const int S = in_dim(l) / 32;
+        const int local = frag - fwd_base(l), t = local / S, s = local % S;
+        const int row = 16 * t + r;
+        for (int j = 0; j < 8; ++j) {
+            const int colk = l == 0 ? 32 * s + 8 * g + j : kperm(s, g, j);
+            out[j] = (__bf16)params[w_off(l) + row * in_dim(l) + colk];
+        }
+        fwd[frag * 64 + lane] = out;
+    } else {
+        const int bf = frag - kFwdFrags;
+        int l = 1;
+        while (l < 5 && bf >= bwd_base(l + 1)) ++l;
+        const int S = l == 5 ? 1 : 2;
+        const int local = bf - bwd_base(l), t = local / S, s = local % S;
+        const int colin = 16 * t + r;
+        for (int j = 0; j < 8; ++j) {
+            const int rowout = kperm(s, g, j);
+            out[j] = rowout < out_dim(l) ? (__bf16)params[w_off(l) + rowout * in_dim(l) + colin] : (__bf16)0.0f;
+        }
+        bwd[bf * 64 + lane] = out;
+    }
+}
+
+__global__ void nrc_zero_kernel(float* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+}  // namespace
+
+uint64_t nrc_fragment_bytes() { return (uint64_t)(kFwdFrags + kBwdFrags) * 64 * sizeof(bf16x8); }
+
+hipError_t nrc_pack(NrcState& st, hipStream_t s) {
+    nrc_pack_kernel<<<((kFwdFrags + kBwdFrags) * 64 + 255) / 256, 256, 0, s>>>(st.params, (bf16x8*)st.fwd,
+                                                                               (bf16x8*)st.bwd);
+    return hipGetLastError();
+}
+
+static Norm make_norm(const NrcState& st) {
+    Norm nm;
+    for (int a = 0; a < 3; ++a) {
+        nm.lo[a] = st.lo[a];
+        const float e = st.hi[a] - st.lo[a];
+        nm.inv[a] = e > 0.0f ? 1.0f / e : 0.0f;
+    }
+    return nm;
+}
+
+hipError_t nrc_infer(const NrcState& st, const rtx_nrc_query* q, uint32_t n, float* out, hipStream_t s,
+                     const uint32_t* n_dev) {
+    if (n == 0) return hipSuccess;
+    const uint64_t waves = (n + 15) / 16;
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    nrc_infer_kernel<<<blocks, 256, 0, s>>>((const bf16x8*)st.fwd, st.params, q, n, n_dev, make_norm(st), out);
+    return hipGetLastError();
+}
+
+hipError_t nrc_step(NrcState& st, const rtx_nrc_sample* samples, uint32_t n, float* grad_out, float* loss_out,
+                    hipStream_t s, uint32_t stride, uint32_t offset) {
+    if (n == 0) return hipSuccess;
+    const uint32_t nblk = (n + 63) / 64;
+    if (nblk > st.max_blocks) return hipErrorInvalidValue;
+    float* loss = loss_out ? loss_out : st.loss;
+    nrc_zero_kernel<<<1, 64, 0, s>>>(loss, 1);
+    nrc_train_kernel<<<nblk, 256, 0, s>>>((const bf16x8*)st.fwd, (const bf16x8*)st.bwd, st.params, samples, n,
+                                          stride, offset, 1.0f / (float)n, make_norm(st), st.partials, loss);
+    const unsigned pb = (RTX_NRC_PARAM_COUNT + 255) / 256;
+    if (grad_out) {
+        nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, 0, 0, 0, 0, 0, 0, grad_out);
+        return hipGetLastError();
+    }
+    st.steps += 1;
+    const double c1 = 1.0 / (1.0 - std::pow((double)st.beta1, (double)st.steps));
+    const double c2 = 1.0 / (1.0 - std::pow((double)st.beta2, (double)st.steps));
+    nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, st.lr, st.beta1, st.beta2, st.eps,
+                                       (float)c1, (float)c2, nullptr);
+    return nrc_pack(st, s);
+}
+
+}  // namespace rtx

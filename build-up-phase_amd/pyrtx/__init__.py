@@ -92,6 +92,15 @@ class RenderParams(ctypes.Structure):
                 ("ray_offset", ctypes.c_float)]
 
 
+class NrcConfig(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint32), ("batch_size", ctypes.c_uint32), ("learning_rate", ctypes.c_float),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
+                ("scene_min", ctypes.c_float * 3), ("scene_max", ctypes.c_float * 3)]
+
+
+NRC_PARAM_COUNT = 23888
+
+
 class TraceConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("lds_stack_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
@@ -122,6 +131,16 @@ SIGNATURES = {
     "rtx_cmd_render": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), ctypes.POINTER(Camera),
                                       ctypes.POINTER(RenderParams), _P, _P]),
     "rtx_get_render_ray_count": (ctypes.c_int, [_P, ctypes.POINTER(_U64)]),
+    "rtx_nrc_create": (ctypes.c_int, [_P, ctypes.POINTER(NrcConfig), ctypes.POINTER(_P)]),
+    "rtx_nrc_destroy": (ctypes.c_int, [_P]),
+    "rtx_nrc_cmd_infer": (ctypes.c_int, [_P, _P, _U32, _P]),
+    "rtx_nrc_cmd_train": (ctypes.c_int, [_P, _P, _U32]),
+    "rtx_nrc_get_params": (ctypes.c_int, [_P, _P]),
+    "rtx_nrc_set_params": (ctypes.c_int, [_P, _P]),
+    "rtx_nrc_get_status": (ctypes.c_int, [_P, ctypes.POINTER(_U32), ctypes.POINTER(ctypes.c_float)]),
+    "rtx_cmd_render_nrc": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), ctypes.POINTER(Camera),
+                                          ctypes.POINTER(RenderParams), _P, _U32, _P, _P]),
+    "rtx_nrc_debug_gradient": (ctypes.c_int, [_P, _P, _U32, _P, _P]),
     "rtx_debug_lbvh": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _P]),
     "rtx_debug_sort": (ctypes.c_int, [_P, _P, _P, _U32, _U32]),
 }
@@ -336,6 +355,28 @@ class Context:
         self._keep = (rec, lt)
         return film
 
+    def render_nrc(self, tlas: AccelerationStructure, records, camera, lights, nrc: "Nrc", width, height, spp=1,
+                   train_paths=65536, max_depth=8, rr_depth=3, seed=1234, sample_offset=0, ray_offset=1e-4,
+                   film=None, accum64=None):
+        """rtx_cmd_render_nrc: one training pass of train_paths paths + spp cache-terminated samples per pixel."""
+        torch = _torch()
+        rec = _dev_tensor(np.ascontiguousarray(records).view(np.uint8).reshape(-1), self.device) \
+            if isinstance(records, np.ndarray) else records
+        lt = _dev_tensor(np.ascontiguousarray(lights, np.float32).reshape(-1, 12), self.device) \
+            if isinstance(lights, np.ndarray) else lights
+        if film is None:
+            film = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
+        nrec = rec.numel() // 48
+        sbt = Sbt(rec.data_ptr(), nrec, 0, 1, 0, (ctypes.c_float * 3)(0, 0, 0), 0.0)
+        cam = Camera((ctypes.c_float * 3)(*camera[:3]), float(camera[3]))
+        p = RenderParams(width, height, spp, max_depth, rr_depth, seed, sample_offset, 0, lt.data_ptr(),
+                         lt.shape[0], ray_offset)
+        self._check(self.lib.rtx_cmd_render_nrc(self.handle, tlas.address, ctypes.byref(sbt), ctypes.byref(cam),
+                                                ctypes.byref(p), nrc.handle, train_paths, film.data_ptr(),
+                                                accum64.data_ptr() if accum64 is not None else None))
+        self._keep = (rec, lt)
+        return film
+
     def render_ray_count(self) -> int:
         n = ctypes.c_uint64()
         self._check(self.lib.rtx_get_render_ray_count(self.handle, ctypes.byref(n)))
@@ -401,6 +442,66 @@ class Context:
         tlas = self.build_tlas(rec, flags)
         torch.cuda.current_stream(self.device)
         return blases, tlas
+
+
+class Nrc:
+    """Neural radiance cache bound to a Context (rtx_nrc_* functions)."""
+
+    def __init__(self, ctx: "Context", scene_min, scene_max, seed=1, batch_size=16384, lr=1e-3, beta1=0.9,
+                 beta2=0.99, eps=1e-8):
+        self.ctx = ctx
+        cfg = NrcConfig(seed, batch_size, lr, beta1, beta2, eps, (ctypes.c_float * 3)(*scene_min),
+                        (ctypes.c_float * 3)(*scene_max))
+        h = ctypes.c_void_p()
+        ctx._check(ctx.lib.rtx_nrc_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.batch_size = batch_size
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.rtx_nrc_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def infer(self, queries):
+        """queries: device float32 [n, 12] -> device float32 [n, 3]."""
+        torch = _torch()
+        n = queries.shape[0]
+        out = torch.empty((max(n, 1), 3), dtype=torch.float32, device=self.ctx.device)
+        self.ctx._check(self.ctx.lib.rtx_nrc_cmd_infer(self.handle, queries.data_ptr(), n, out.data_ptr()))
+        return out[:n]
+
+    def train(self, samples):
+        """samples: device float32 [n, 16] (query[12], target[3], pad)."""
+        self.ctx._check(self.ctx.lib.rtx_nrc_cmd_train(self.handle, samples.data_ptr(), samples.shape[0]))
+
+    def gradient(self, samples):
+        torch = _torch()
+        g = torch.empty(NRC_PARAM_COUNT, dtype=torch.float32, device=self.ctx.device)
+        loss = torch.empty(4, dtype=torch.float32, device=self.ctx.device)
+        self.ctx._check(self.ctx.lib.rtx_nrc_debug_gradient(self.handle, samples.data_ptr(), samples.shape[0],
+                                                            g.data_ptr(), loss.data_ptr()))
+        return g.cpu().numpy(), float(loss[0].item())
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(NRC_PARAM_COUNT, np.float32)
+        self.ctx._check(self.ctx.lib.rtx_nrc_get_params(self.handle, p.ctypes.data))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        self.ctx._check(self.ctx.lib.rtx_nrc_set_params(self.handle, p.ctypes.data))
+
+    def status(self):
+        st = ctypes.c_uint32()
+        loss = ctypes.c_float()
+        self.ctx._check(self.ctx.lib.rtx_nrc_get_status(self.handle, ctypes.byref(st), ctypes.byref(loss)))
+        return st.value, loss.value
 
 
 INSTANCE_DTYPE = np.dtype([("xf", np.float32, (12,)), ("cm", np.uint32), ("sf", np.uint32), ("addr", np.uint64)])

@@ -292,6 +292,62 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
 /* Rays traced by rtx_cmd_render since the context was created (camera, extension and shadow rays). */
 int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays);
 
+/* ---- neural radiance cache (Muller et al. 2021) --------------------------------------------------------------
+ *
+ * MLP: 96 inputs (81 used) -> 5 hidden layers of 64 (ReLU) -> 16 outputs (RGB in the first 3), BF16 weights and
+ * activations on MFMA with FP32 accumulation, FP32 master weights, Adam.  Input encoding of a query: position
+ * normalised to the config box, 12 frequencies per axis sin(2^k pi p), cos(2^k pi p) (72 features), normal (3),
+ * direction (3), albedo (3).  Loss: relative L2, sum over RGB of (y - t)^2 / (y^2 + 0.01), mean over the batch.
+ * Parameter vector (RTX_NRC_PARAM_COUNT floats): W1[64][96] b1[64] W2[64][64] b2 ... W5 b5 W6[16][64] b6[16]. */
+#define RTX_NRC_PARAM_COUNT 23888u
+
+typedef struct rtx_nrc_t* rtx_nrc;
+
+typedef struct rtx_nrc_config {
+    uint32_t seed;            /* weight initialisation seed (He-uniform weights, zero biases) */
+    uint32_t batch_size;      /* samples per Adam step, multiple of 64 */
+    float learning_rate;      /* 1e-3 */
+    float beta1, beta2, epsilon;
+    float scene_min[3];       /* position normalisation box */
+    float scene_max[3];
+} rtx_nrc_config;
+
+/* 48 B: a path vertex as the cache sees it; direction points from the vertex toward the previous vertex */
+typedef struct rtx_nrc_query {
+    float position[3];
+    float normal[3];
+    float direction[3];
+    float albedo[3];
+} rtx_nrc_query;
+
+/* 64 B: a training sample, query + target radiance */
+typedef struct rtx_nrc_sample {
+    rtx_nrc_query query;
+    float target[3];
+    float reserved;
+} rtx_nrc_sample;
+
+int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc);
+int rtx_nrc_destroy(rtx_nrc nrc);
+/* radiance predictions (3 floats per query, clamped at 0) */
+int rtx_nrc_cmd_infer(rtx_nrc nrc, const rtx_nrc_query* queries_device, uint32_t count, float* out_rgb_device);
+/* ceil(count / batch_size) Adam steps over consecutive batches of samples_device */
+int rtx_nrc_cmd_train(rtx_nrc nrc, const rtx_nrc_sample* samples_device, uint32_t count);
+/* FP32 master parameters (checkpoint / restore); host buffers of RTX_NRC_PARAM_COUNT floats */
+int rtx_nrc_get_params(rtx_nrc nrc, float* params_host);
+int rtx_nrc_set_params(rtx_nrc nrc, const float* params_host);
+/* Adam steps taken so far and the mean loss of the last step */
+int rtx_nrc_get_status(rtx_nrc nrc, uint32_t* steps, float* last_loss);
+
+/* One frame of the cache-accelerated integrator: train_paths full-length training paths from random pixels
+ * (sample index params->sample_offset) give one training sample per vertex after the first (target: the path's
+ * radiance from that vertex on, from next-event estimation); one Adam step per batch_size samples; then every
+ * pixel traces params->spp paths that stop at the 2nd bounce (vertex index 2) and add throughput x the cache's
+ * prediction there.  film / accum as in rtx_cmd_render. */
+int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                       const rtx_render_params* params, rtx_nrc nrc, uint32_t train_paths,
+                       float* film_rgba32f_device, double* accum_rgb_device);
+
 /* ---- verification hooks (no counterpart in the reference: they expose intermediate LBVH build steps so that
  * parity tests can compare them with the CPU oracle) ---------------------------------------------------------- */
 
@@ -305,6 +361,12 @@ int rtx_debug_lbvh(rtx_context ctx, const float* boxes_device, uint32_t n, uint3
 /* Sort n (key, value) pairs with the builder's radix sort: keys uint32 (key_bytes 4) or uint64 (8); values are
  * replaced by the stable sorted permutation.  Blocks until done; allocates its own scratch. */
 int rtx_debug_sort(rtx_context ctx, void* keys_device, uint32_t* values_device, uint32_t n, uint32_t key_bytes);
+
+/* Gradient of the batch loss with respect to the FP32 parameters (RTX_NRC_PARAM_COUNT floats, device) and the
+ * loss itself (1 float, device), without updating the cache.  No counterpart in the reference: verification of
+ * the backward kernels. */
+int rtx_nrc_debug_gradient(rtx_nrc nrc, const rtx_nrc_sample* samples_device, uint32_t count, float* grad_device,
+                           float* loss_device);
 
 #ifdef __cplusplus
 }

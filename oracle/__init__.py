@@ -59,6 +59,11 @@ def lib():
         L.oracle_pcg32.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
         L.oracle_pcg4d.restype = None
         L.oracle_pcg4d.argtypes = [P, ctypes.c_int64, P]
+        L.oracle_nrc.restype = None
+        L.oracle_nrc.argtypes = [P, P, ctypes.c_int, P, P, ctypes.c_int, P, P, P]
+        L.oracle_adam.restype = None
+        L.oracle_adam.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_int]
         L.oracle_render.restype = None
         L.oracle_render.argtypes = [ctypes.c_int, P, ctypes.c_uint32, P, ctypes.c_uint32, ctypes.c_uint32,
                                     ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -242,3 +247,28 @@ def render(tlas: int, records: np.ndarray, camera, lights: np.ndarray, w: int, h
     lib().oracle_render(tlas, _ptr(rec), len(rec), _ptr(cam), w, h, spp, sample_offset, max_depth, rr_depth, seed,
                         _ptr(lt), lt.shape[0], ray_offset, _ptr(accum), threads)
     return accum
+
+
+NRC_PARAM_COUNT = 23888
+
+
+def nrc(params: np.ndarray, samples: np.ndarray, lo, hi, bf16: bool = True, want_grad: bool = True):
+    """FP64 cache MLP over samples (float32 [n, 16]: query[12], target[3], pad).  Returns (pred [n,3], grad
+    [23888] or None, mean loss)."""
+    p = np.ascontiguousarray(params, np.float32)
+    smp = np.ascontiguousarray(samples, np.float32).reshape(-1, 16)
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    n = smp.shape[0]
+    out = np.zeros((n, 3), np.float64)
+    grad = np.zeros(NRC_PARAM_COUNT, np.float64) if want_grad else None
+    loss = np.zeros(1, np.float64)
+    lib().oracle_nrc(_ptr(p), _ptr(smp), n, _ptr(lo), _ptr(hi), int(bf16), _ptr(out),
+                     _ptr(grad) if want_grad else None, _ptr(loss))
+    return out, grad, float(loss[0])
+
+
+def adam(params, m, v, grad, lr, b1, b2, eps, step):
+    """In-place Adam step on float32 arrays."""
+    lib().oracle_adam(_ptr(params), _ptr(m), _ptr(v), _ptr(np.ascontiguousarray(grad, np.float64)), params.size,
+                      lr, b1, b2, eps, step)

@@ -1014,3 +1014,132 @@ void oracle_render(int tlas, const void* records, uint32_t nrec, const float* ca
 }
 
 }  // extern "C"
+
+// ---- neural radiance cache MLP (restatement of the GPU network, FP64) -----------------------------------------
+namespace {
+
+constexpr int kNrcParams = 23888;
+const int kIn[6] = {96, 64, 64, 64, 64, 64};
+const int kOut[6] = {64, 64, 64, 64, 64, 16};
+int nrc_woff(int l) { return l == 0 ? 0 : 6208 + (l - 1) * 4160; }
+int nrc_boff(int l) { return nrc_woff(l) + kOut[l] * kIn[l]; }
+
+// IEEE binary32 -> bfloat16 -> binary32, round to nearest even
+float bf16r(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) return x;
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+double sinpi_d(double x) {
+    double r = std::fmod(x, 2.0);
+    return std::sin(3.14159265358979323846 * r);
+}
+double cospi_d(double x) {
+    double r = std::fmod(x, 2.0);
+    return std::cos(3.14159265358979323846 * r);
+}
+
+// features of one query (12 floats: position, normal, direction, albedo)
+void nrc_features(const float* q, const float* lo, const float* hi, double* f) {
+    float pn[3];
+    for (int a = 0; a < 3; ++a) {
+        float e = hi[a] - lo[a];
+        float inv = e > 0.0f ? 1.0f / e : 0.0f;
+        float d = q[a] - lo[a];
+        pn[a] = d * inv;
+    }
+    for (int i = 0; i < 96; ++i) {
+        double v = 0.0;
+        if (i < 72) {
+            int a = i / 24, r = i % 24, k = r >> 1;
+            double x = std::ldexp((double)pn[a], k);
+            v = (r & 1) ? cospi_d(x) : sinpi_d(x);
+        } else if (i < 81) {
+            v = q[3 + (i - 72)];
+        }
+        f[i] = v;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Forward + backward of the cache MLP over n samples (16 floats each: query[12], target[3], pad).  With bf16 != 0
+// the GPU's rounding points are emulated (weights, features, activations and layer gradients rounded to bf16,
+// FP64 accumulation).  Outputs (nullable): out[n*3] predictions (before clamping), grad[23888] = d(mean loss)/dp,
+// loss[1] = mean relative L2 loss.
+void oracle_nrc(const float* params, const float* samples, int n, const float* lo, const float* hi, int bf16,
+                double* out, double* grad, double* loss) {
+    auto R = [&](double x) -> double { return bf16 ? (double)bf16r((float)x) : x; };
+    std::vector<double> W(kNrcParams);
+    for (int i = 0; i < kNrcParams; ++i) W[i] = bf16 ? (double)bf16r(params[i]) : (double)params[i];
+    // biases are used in FP32 on the GPU
+    for (int l = 0; l < 6; ++l)
+        for (int r = 0; r < kOut[l]; ++r) W[nrc_boff(l) + r] = params[nrc_boff(l) + r];
+    if (grad)
+        for (int i = 0; i < kNrcParams; ++i) grad[i] = 0.0;
+    double total = 0.0;
+    std::vector<double> act[7];
+    for (int b = 0; b < n; ++b) {
+        const float* sm = samples + 16 * b;
+        act[0].assign(96, 0.0);
+        nrc_features(sm, lo, hi, act[0].data());
+        for (int i = 0; i < 96; ++i) act[0][i] = R(act[0][i]);
+        for (int l = 0; l < 6; ++l) {
+            act[l + 1].assign(kOut[l], 0.0);
+            for (int r = 0; r < kOut[l]; ++r) {
+                double acc = W[nrc_boff(l) + r];
+                for (int c = 0; c < kIn[l]; ++c) acc += W[nrc_woff(l) + r * kIn[l] + c] * act[l][c];
+                act[l + 1][r] = l < 5 ? R(std::max(0.0, acc)) : acc;
+            }
+        }
+        std::vector<double> g(16, 0.0);
+        for (int i = 0; i < 3; ++i) {
+            double y = act[6][i], t = sm[12 + i];
+            double den = y * y + 0.01;
+            total += (y - t) * (y - t) / den;
+            g[i] = 2.0 * (y - t) / den / n;
+            if (out) out[3 * b + i] = y;
+        }
+        if (!grad) continue;
+        for (int l = 5; l >= 0; --l) {
+            for (int r = 0; r < kOut[l]; ++r) g[r] = R(g[r]);
+            for (int r = 0; r < kOut[l]; ++r) {
+                grad[nrc_boff(l) + r] += g[r];
+                for (int c = 0; c < kIn[l]; ++c) grad[nrc_woff(l) + r * kIn[l] + c] += g[r] * act[l][c];
+            }
+            if (l == 0) break;
+            std::vector<double> gp(kIn[l], 0.0);
+            for (int c = 0; c < kIn[l]; ++c) {
+                double acc = 0.0;
+                for (int r = 0; r < kOut[l]; ++r) acc += W[nrc_woff(l) + r * kIn[l] + c] * g[r];
+                gp[c] = act[l][c] > 0.0 ? acc : 0.0;
+            }
+            g = gp;
+        }
+    }
+    if (loss) *loss = total / n;
+}
+
+// One Adam step on FP32 parameters (restatement of the update: bias-corrected moments).
+void oracle_adam(float* params, float* m, float* v, const double* grad, int n, float lr, float b1, float b2,
+                 float eps, int step) {
+    double c1 = 1.0 / (1.0 - std::pow((double)b1, step)), c2 = 1.0 / (1.0 - std::pow((double)b2, step));
+    for (int i = 0; i < n; ++i) {
+        double g = grad[i];
+        double mm = b1 * (double)m[i] + (1.0 - b1) * g;
+        double vv = b2 * (double)v[i] + (1.0 - b2) * g * g;
+        m[i] = (float)mm;
+        v[i] = (float)vv;
+        params[i] = (float)((double)params[i] - lr * (mm * c1) / (std::sqrt(vv * c2) + eps));
+    }
+}
+
+}  // extern "C"

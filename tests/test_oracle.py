@@ -290,3 +290,32 @@ def test_tie_rule_is_rare_on_query_scene_sample():
     inc = S.incoherent_rays(sc, 4000)
     tuv, ids, st = O.trace(t, inc)
     assert (st == 2).mean() < 0.02 and (st == 1).sum() > 20
+
+
+def test_nrc_oracle_backprop_matches_finite_differences():
+    # The relative L2 loss treats its denominator (y^2 + 0.01) as a constant (stop-gradient), so the FP64
+    # backprop is checked against central differences of the loss with denominators frozen at the current
+    # prediction.
+    rng = np.random.default_rng(0)
+    p = (rng.standard_normal(O.NRC_PARAM_COUNT) * 0.1).astype(np.float32)
+    smp = np.zeros((8, 16), np.float32)
+    smp[:, :12] = rng.uniform(0, 1, (8, 12))
+    smp[:, 12:15] = rng.uniform(0, 2, (8, 3))
+    lo, hi = np.zeros(3, np.float32), np.ones(3, np.float32)
+    y0, g, _ = O.nrc(p, smp, lo, hi, bf16=False)
+    den = y0 ** 2 + 0.01
+    t = smp[:, 12:15].astype(np.float64)
+
+    def surrogate(q):
+        y = O.nrc(q, smp, lo, hi, bf16=False, want_grad=False)[0]
+        return ((y - t) ** 2 / den).sum() / len(smp)
+
+    idx = rng.choice(O.NRC_PARAM_COUNT, 60, replace=False)
+    for i in idx:
+        q1 = p.copy()
+        q1[i] += np.float32(1e-3)
+        q2 = p.copy()
+        q2[i] -= np.float32(1e-3)
+        fd = (surrogate(q1) - surrogate(q2)) / (float(q1[i]) - float(q2[i]))
+        assert abs(fd - g[i]) <= 1e-7 + 1e-6 * abs(g[i])
+
