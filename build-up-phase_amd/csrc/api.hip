@@ -467,7 +467,9 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     rtx::TraceLaunch L;
     int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L);
     if (r) return r;
-    if (params->max_depth < 3) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "max_depth must be >= 3");
+    if (params->max_depth <= nrc->st.cache_vertex)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "max_depth must exceed the cache vertex");
+    a.cache_depth = nrc->st.cache_vertex;
     if (train_paths > a.npix) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "train_paths above the pixel count");
     const uint64_t need = rtx::pt_train_bytes(train_paths, params->max_depth);
     if (need > nrc->train_buf_bytes) {
@@ -511,6 +513,11 @@ int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc)
     st.beta1 = cfg->beta1;
     st.beta2 = cfg->beta2;
     st.eps = cfg->epsilon;
+    st.cache_vertex = cfg->cache_vertex ? cfg->cache_vertex : 1u;
+    if (st.cache_vertex > 62) {
+        delete n;
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "cache_vertex above 62");
+    }
     for (int a = 0; a < 3; ++a) {
         st.lo[a] = cfg->scene_min[a];
         st.hi[a] = cfg->scene_max[a];

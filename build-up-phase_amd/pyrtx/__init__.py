@@ -95,7 +95,8 @@ class RenderParams(ctypes.Structure):
 class NrcConfig(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint32), ("batch_size", ctypes.c_uint32), ("learning_rate", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
-                ("scene_min", ctypes.c_float * 3), ("scene_max", ctypes.c_float * 3)]
+                ("scene_min", ctypes.c_float * 3), ("scene_max", ctypes.c_float * 3),
+                ("cache_vertex", ctypes.c_uint32)]
 
 
 NRC_PARAM_COUNT = 23888
@@ -448,10 +449,10 @@ class Nrc:
     """Neural radiance cache bound to a Context (rtx_nrc_* functions)."""
 
     def __init__(self, ctx: "Context", scene_min, scene_max, seed=1, batch_size=16384, lr=1e-3, beta1=0.9,
-                 beta2=0.99, eps=1e-8):
+                 beta2=0.99, eps=1e-8, cache_vertex=1):
         self.ctx = ctx
         cfg = NrcConfig(seed, batch_size, lr, beta1, beta2, eps, (ctypes.c_float * 3)(*scene_min),
-                        (ctypes.c_float * 3)(*scene_max))
+                        (ctypes.c_float * 3)(*scene_max), cache_vertex)
         h = ctypes.c_void_p()
         ctx._check(ctx.lib.rtx_nrc_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

@@ -366,22 +366,28 @@ CORNELL_GREEN = (0.15, 0.75, 0.15)
 CORNELL_LIGHT_RADIANCE = 15.0
 
 
-def cornell_scene(sphere_level: int = 8, seed: int = 1234, width: int = 1920, height: int = 1080):
-    """Cornell-style unit box with a displaced icosphere.
+def cornell_scene(sphere_level: int = 8, seed: int = 1234, width: int = 1920, height: int = 1080,
+                  closed: bool = True):
+    """Cornell-style unit cube with a displaced icosphere.
 
-    24 box triangles: 5 walls of the unit cube (10; open at z = 1, facing the camera), a closed block (6 faces,
-    12) and a 0.25 x 0.25 light quad (2) just under the ceiling, facing down, radiance 15.  Albedos: walls and
-    block 0.75 white, left wall (x = 0) red (0.75, 0.15, 0.15), right wall (x = 1) green (0.15, 0.75, 0.15).
-    Icosphere: subdivision level `sphere_level` (20 * 4^level triangles; level 8 = 1,310,720), radius 0.3,
-    centre (0.6, 0.32, 0.45), displaced along the normal by 3-octave value noise of amplitude 0.03 (seed).
-    Camera: the reference raygen model at (0.5, 0.5, 1.9) looking down -z, 40 deg vertical field of view.
+    closed (default): the 6 walls of the unit cube (12 triangles; the camera is inside), an open-bottom block
+    (5 faces, 10) and a 0.25 x 0.25 light quad (2) just under the ceiling facing down with radiance 15: 24
+    triangles.  Camera: the reference raygen model at (0.5, 0.5, 0.99) looking down -z, 75 deg vertical field of
+    view.  Icosphere centre (0.58, 0.31, 0.36).
+    closed=False: the classic open-front box: 5 walls (10), a closed block (12), the light (2); camera outside at
+    (0.5, 0.5, 1.9), 40 deg; icosphere centre (0.6, 0.32, 0.45).
+    Albedos: walls and block 0.75 white, left wall (x = 0) red (0.75, 0.15, 0.15), right wall (x = 1) green
+    (0.15, 0.75, 0.15).  Icosphere: subdivision level `sphere_level` (20 * 4^level triangles; level 8 =
+    1,310,720), radius 0.3, displaced along the normal by 3-octave value noise of amplitude 0.03 (seed).
     Returns (Scene, lights [2, 12])."""
     white = [_quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)),      # floor
              _quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)),      # ceiling
              _quad((0, 0, 0), (0, 1, 0), (1, 1, 0), (1, 0, 0))]      # back
+    if closed:
+        white.append(_quad((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)))  # front wall behind the camera
     x0, x1, z0, z1, h = 0.08, 0.30, 0.62, 0.84, 0.30
-    block = [_quad((x0, h, z0), (x0, h, z1), (x1, h, z1), (x1, h, z0)),
-             _quad((x0, 0, z0), (x1, 0, z0), (x1, 0, z1), (x0, 0, z1)),
+    block = [_quad((x0, h, z0), (x0, h, z1), (x1, h, z1), (x1, h, z0))] + \
+        ([] if closed else [_quad((x0, 0, z0), (x1, 0, z0), (x1, 0, z1), (x0, 0, z1))]) + [
              _quad((x0, 0, z0), (x1, 0, z0), (x1, h, z0), (x0, h, z0)),
              _quad((x0, 0, z1), (x0, h, z1), (x1, h, z1), (x1, 0, z1)),
              _quad((x0, 0, z0), (x0, h, z0), (x0, h, z1), (x0, 0, z1)),
@@ -392,8 +398,8 @@ def cornell_scene(sphere_level: int = 8, seed: int = 1234, width: int = 1920, he
     lv, li = _quad((0.375, ly, 0.375), (0.625, ly, 0.375), (0.625, ly, 0.625), (0.375, ly, 0.625))
     geoms = [Geometry(*_merge(white + block)), Geometry(*_merge(red)), Geometry(*_merge(green)), Geometry(lv, li)]
     assert sum(g.indices.shape[0] for g in geoms) == 24
-    sph = icosphere(2 ** sphere_level, radius=0.3, seed=seed, amplitude=0.1, noise_freq=4.0,
-                    center=(0.6, 0.32, 0.45))
+    centre = (0.58, 0.31, 0.36) if closed else (0.6, 0.32, 0.45)
+    sph = icosphere(2 ** sphere_level, radius=0.3, seed=seed, amplitude=0.1, noise_freq=4.0, center=centre)
     recs = hit_records([PROGRAM_LAMBERTIAN] * 5,
                        [CORNELL_WHITE, CORNELL_RED, CORNELL_GREEN, (0.0, 0.0, 0.0), CORNELL_WHITE],
                        emissions=[(0, 0, 0), (0, 0, 0), (0, 0, 0), (CORNELL_LIGHT_RADIANCE,) * 3, (0, 0, 0)])
@@ -404,8 +410,8 @@ def cornell_scene(sphere_level: int = 8, seed: int = 1234, width: int = 1920, he
     sc = Scene(blas=[geoms, [sph]], transforms=np.stack([translation(), translation()]),
                blas_of_instance=np.array([0, 1]), custom_index=np.zeros(2, np.uint32),
                mask=np.full(2, 0xFF, np.uint32), sbt_offset=np.array([0, 4], np.uint32), flags=np.zeros(2, np.uint32),
-               records=recs, camera=(0.5, 0.5, 1.9, 40.0), width=width, height=height,
-               meta={"name": "cornell", "sphere_level": sphere_level})
+               records=recs, camera=(0.5, 0.5, 0.99, 75.0) if closed else (0.5, 0.5, 1.9, 40.0), width=width,
+               height=height, meta={"name": "cornell", "sphere_level": sphere_level, "closed": closed})
     return sc, lights
 
 

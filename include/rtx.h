@@ -310,6 +310,8 @@ typedef struct rtx_nrc_config {
     float beta1, beta2, epsilon;
     float scene_min[3];       /* position normalisation box */
     float scene_max[3];
+    uint32_t cache_vertex;    /* rtx_cmd_render_nrc: path vertex (camera hit = 0) replaced by the cache query;
+                                 0 selects 1, i.e. paths scatter at the primary hit and end at their 2nd bounce */
 } rtx_nrc_config;
 
 /* 48 B: a path vertex as the cache sees it; direction points from the vertex toward the previous vertex */
@@ -342,8 +344,8 @@ int rtx_nrc_get_status(rtx_nrc nrc, uint32_t* steps, float* last_loss);
 /* One frame of the cache-accelerated integrator: train_paths full-length training paths from random pixels
  * (sample index params->sample_offset) give one training sample per vertex after the first (target: the path's
  * radiance from that vertex on, from next-event estimation); one Adam step per batch_size samples; then every
- * pixel traces params->spp paths that stop at the 2nd bounce (vertex index 2) and add throughput x the cache's
- * prediction there.  film / accum as in rtx_cmd_render. */
+ * pixel traces params->spp paths that stop at vertex cfg->cache_vertex (instead of scattering there) and add
+ * throughput x the cache's prediction.  film / accum as in rtx_cmd_render. */
 int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
                        const rtx_render_params* params, rtx_nrc nrc, uint32_t train_paths,
                        float* film_rgba32f_device, double* accum_rgb_device);
