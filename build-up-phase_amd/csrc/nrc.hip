@@ -341,15 +341,26 @@ __global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict
     if (threadIdx.x == 0) atomicAdd(loss_sum, (sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3]) * inv_n);
 }
 
-// Sum the block partials (block order: deterministic) and take one Adam step, or only write the gradient.
+// Sum the block partials and take one Adam step, or only write the gradient.  A block covers 16 parameters with
+// 16 threads each; thread k sums partials k, k + 16, ... and a fixed LDS tree adds the 16 slices (deterministic).
 __global__ __launch_bounds__(256) void nrc_adam_kernel(const float* __restrict__ partials, uint32_t nblk,
                                                        float* __restrict__ params, float* __restrict__ m,
                                                        float* __restrict__ v, float lr, float b1, float b2, float eps,
                                                        float c1, float c2, float* __restrict__ grad_out) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= RTX_NRC_PARAM_COUNT) return;
-    float gsum = 0.0f;
-    for (uint32_t b = 0; b < nblk; ++b) gsum += partials[(uint64_t)b * RTX_NRC_PARAM_COUNT + p];
+    __shared__ float s[16][17];
+    const uint32_t pi = threadIdx.x & 15u, k = threadIdx.x >> 4;
+    const uint32_t p = blockIdx.x * 16u + pi;
+    float acc = 0.0f;
+    if (p < RTX_NRC_PARAM_COUNT)
+        for (uint32_t b = k; b < nblk; b += 16) acc += partials[(uint64_t)b * RTX_NRC_PARAM_COUNT + p];
+    s[k][pi] = acc;
+    __syncthreads();
+    for (uint32_t h = 8; h > 0; h >>= 1) {
+        if (k < h) s[k][pi] += s[k + h][pi];
+        __syncthreads();
+    }
+    if (k != 0 || p >= RTX_NRC_PARAM_COUNT) return;
+    const float gsum = s[0][pi];
     if (grad_out) {
         grad_out[p] = gsum;
         return;
@@ -439,7 +450,7 @@ hipError_t nrc_step(NrcState& st, const rtx_nrc_sample* samples, uint32_t n, flo
     nrc_zero_kernel<<<1, 64, 0, s>>>(loss, 1);
     nrc_train_kernel<<<nblk, 256, 0, s>>>((const bf16x8*)st.fwd, (const bf16x8*)st.bwd, st.params, samples, n,
                                           stride, offset, 1.0f / (float)n, make_norm(st), st.partials, loss);
-    const unsigned pb = (RTX_NRC_PARAM_COUNT + 255) / 256;
+    const unsigned pb = (RTX_NRC_PARAM_COUNT + 15) / 16;
     if (grad_out) {
         nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, 0, 0, 0, 0, 0, 0, grad_out);
         return hipGetLastError();

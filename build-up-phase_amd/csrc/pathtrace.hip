@@ -28,20 +28,35 @@ namespace {
 
 constexpr uint32_t kTrainKey = 0x5EEDu;
 
-__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred, uint32_t* total) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) {
-        *total = 0;
-        return 0xFFFFFFFFu;
-    }
-    const uint32_t lane = threadIdx.x & 63u;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
+// Block-wide compaction of up to 3 queues: one atomic per queue per 256-thread block.  Every thread of the block
+// must call it.  Returns each thread's slot (or 0xFFFFFFFF) in o[q] and the block totals in tot[q].
+__device__ __forceinline__ void block_push3(uint32_t* c0, uint32_t* c1, uint32_t* c2, const bool p[3], uint32_t o[3],
+                                            uint32_t tot[3]) {
+    __shared__ uint32_t s_cnt[3][4];
+    __shared__ uint32_t s_base[3];
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    *total = (uint32_t)__popcll(m);
-    return pred ? base + (uint32_t)__popcll(m & lt) : 0xFFFFFFFFu;
+    uint64_t m[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        m[q] = __ballot(p[q]);
+        if (lane == 0) s_cnt[q][w] = (uint32_t)__popcll(m[q]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int q = threadIdx.x;
+        const uint32_t t = s_cnt[q][0] + s_cnt[q][1] + s_cnt[q][2] + s_cnt[q][3];
+        uint32_t* c = q == 0 ? c0 : (q == 1 ? c1 : c2);
+        s_base[q] = (t && c) ? atomicAdd(c, t) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        uint32_t off = s_base[q];
+        for (uint32_t k = 0; k < w; ++k) off += s_cnt[q][k];
+        o[q] = p[q] ? off + (uint32_t)__popcll(m[q] & lt) : 0xFFFFFFFFu;
+        tot[q] = s_cnt[q][0] + s_cnt[q][1] + s_cnt[q][2] + s_cnt[q][3];
+    }
 }
 
 __global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, float* __restrict__ cdf) {
@@ -246,27 +261,25 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
             }
         }
     }
-    uint32_t n_sh = 0, n_ext = 0, n_cq = 0;
-    const uint32_t so = wave_push(&a.counts[3 * depth + 1], push_sh, &n_sh);
-    const uint32_t eo = wave_push(&a.counts[3 * (depth + 1)], push_ext, &n_ext);
+    const bool pr[3] = {push_sh, push_ext, push_cq};
+    uint32_t o[3], tot[3];
+    block_push3(&a.counts[3 * depth + 1], &a.counts[3 * (depth + 1)],
+                MODE == kPtCache ? &a.counts[3 * (depth + 1) + 2] : nullptr, pr, o, tot);
     if (push_sh) {
-        a.sh_rays[so] = sh;
-        a.sh_pid[so] = pid;
-        a.sh_contrib[so] = contrib;
+        a.sh_rays[o[0]] = sh;
+        a.sh_pid[o[0]] = pid;
+        a.sh_contrib[o[0]] = contrib;
     }
     if (push_ext) {
-        a.ext_rays[cur ^ 1][eo] = ext;
-        a.ext_pid[cur ^ 1][eo] = pid;
+        a.ext_rays[cur ^ 1][o[1]] = ext;
+        a.ext_pid[cur ^ 1][o[1]] = pid;
         a.thr[pid] = thr_new;
     }
-    if (MODE == kPtCache) {
-        const uint32_t co = wave_push(&a.counts[3 * (depth + 1) + 2], push_cq, &n_cq);
-        if (push_cq) {
-            a.cq[co] = cq;
-            a.cq_pid[co] = pid;
-        }
+    if (MODE == kPtCache && push_cq) {
+        a.cq[o[2]] = cq;
+        a.cq_pid[o[2]] = pid;
     }
-    if ((threadIdx.x & 63u) == 0 && (n_sh + n_ext) > 0) atomicAdd(&a.state->pt_rays, (unsigned long long)(n_sh + n_ext));
+    if (threadIdx.x == 0 && (tot[0] + tot[1]) > 0) atomicAdd(&a.state->pt_rays, (unsigned long long)(tot[0] + tot[1]));
 }
 
 __global__ __launch_bounds__(256) void shadow_gather_kernel(PtArgs a, uint32_t depth) {

@@ -285,15 +285,31 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
     uint32_t hit_inst = kNoHit, hit_slot = 0;
     bool occl = false;
 
+    // Wave-uniform pool of ray ids [pool_next, pool_end): one atomic fetches `chunk` consecutive ids (256 when the
+    // launch has at least 8 rounds of work for every wave, else 64 so that small launches still spread over all
+    // waves).
+    const uint32_t waves_total = gridDim.x * (blockDim.x >> 6);
+    const uint32_t chunk = (uint64_t)nrays >= (uint64_t)waves_total * 64u * 8u ? 256u : 64u;
+    uint32_t pool_next = 0, pool_end = 0;
     for (;;) {
         const uint64_t need = __ballot(!active && !exhausted);
         if (need) {
-            const int leader = __ffsll((unsigned long long)need) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(&args.state->ray_counter, (uint32_t)__popcll(need));
-            base = __shfl(base, leader);
+            const uint32_t want = (uint32_t)__popcll(need);
+            const uint32_t rank = (uint32_t)__popcll(need & lt_mask);
+            const uint32_t avail = pool_end - pool_next;
+            const uint32_t served = avail < want ? avail : want;
+            uint32_t id = rank < served ? pool_next + rank : 0xFFFFFFFFu;
+            pool_next += served;
+            if (served < want && pool_end < nrays) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&args.state->ray_counter, chunk);
+                base = __shfl(base, 0);
+                pool_next = base;
+                pool_end = base + chunk;
+                if (rank >= served) id = pool_next + (rank - served);
+                pool_next += want - served;
+            }
             if (!active && !exhausted) {
-                const uint32_t id = base + (uint32_t)__popcll(need & lt_mask);
                 if (id < nrays) {
                     ray_id = id;
                     active = true;
