@@ -8,6 +8,12 @@ raygen model) and 16,777,216 incoherent rays (origins uniform in the scene box, 
 sphere, tmin 0, tmax inf; seed 1234).  Rays, hits and the acceleration structures are resident in HBM before
 the timed region.
 
+The second half of the metric runs the path tracer on the closed unit-cube Cornell box with a displaced
+level-8 icosphere (1,310,720 triangles + 24 box/light triangles): 1920x1080, depth 8, Russian roulette after
+depth 3, seed 1234.  It times a plain 64-spp render, builds a 4096-spp FP64-accumulated reference, and measures
+the time the neural-radiance-cache renderer (one training pass + one cache-terminated sample per frame) needs to
+reach the plain render's relMSE = mean((x - ref)^2 / (ref^2 + 0.01)).
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -37,6 +43,9 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="triangle-count scale of the scene (1.0 = 20M)")
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays per set for the CPU baseline (0: skip)")
     ap.add_argument("--no-anyhit", action="store_true")
+    ap.add_argument("--no-render", action="store_true", help="skip the path-tracing / NRC equal-relMSE part")
+    ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the FP64 reference render")
+    ap.add_argument("--nrc-frames", type=int, default=160, help="NRC frames rendered (one sample each)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -186,6 +195,8 @@ def main():
         O.reset()
         log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads")
 
+    render = None if args.no_render else render_bench(ctx, args, np, torch, pyrtx, S)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -210,9 +221,96 @@ def main():
                   "nodes": int(sum(i.node_count for i in info)), "max_depth": int(tinfo.max_depth)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "path_tracing": render,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def render_bench(ctx, args, np, torch, pyrtx, S):
+    """Plain 64-spp path tracing vs the neural radiance cache at equal relMSE (1920x1080 Cornell box)."""
+    W, H, SPP = 1920, 1080, 64
+    sc, lights = S.cornell_scene(8, width=W, height=H)
+    dev = ctx.device
+    blases, tlas = ctx.upload_scene(sc)
+    rec = torch.from_numpy(np.ascontiguousarray(sc.records).view(np.uint8).reshape(-1).copy()).to(dev)
+    lt = torch.from_numpy(np.ascontiguousarray(lights, np.float32).reshape(-1, 12).copy()).to(dev)
+    ctx.sync()
+    log(f"path tracing scene: {sc.unique_triangle_count} triangles, {W}x{H}")
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    def relmse(x, ref):
+        return torch.mean((x - ref) ** 2 / (ref ** 2 + 0.01))
+
+    ctx.render(tlas, rec, sc.camera, lt, W, H, spp=2, sample_offset=1 << 24)  # warm-up
+    ctx.sync()
+    r0 = ctx.render_ray_count()
+    a, b = ev(), ev()
+    a.record()
+    film = ctx.render(tlas, rec, sc.camera, lt, W, H, spp=SPP)
+    b.record()
+    torch.cuda.synchronize()
+    ms_plain = a.elapsed_time(b)
+    n_rays = ctx.render_ray_count() - r0
+    log(f"plain {SPP} spp: {ms_plain:.1f} ms, {n_rays} rays, {n_rays / ms_plain / 1e3:.1f} Mrays/s")
+
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+    chunks = max(1, args.ref_spp // 64)
+    tr = time.time()
+    for k in range(chunks):
+        ctx.render(tlas, rec, sc.camera, lt, W, H, spp=64, sample_offset=(1 << 20) + 64 * k, accum64=acc)
+        if (k + 1) % 16 == 0:
+            ctx.sync()
+            log(f"reference {64 * (k + 1)}/{64 * chunks} spp ({time.time() - tr:.0f}s)")
+    ctx.sync()
+    ref = (acc / (64 * chunks)).float()
+    err_plain = float(relmse(film[..., :3], ref))
+    log(f"plain relMSE {err_plain:.5f} vs {64 * chunks}-spp reference")
+
+    lo, hi = S.scene_bounds(sc)
+    nrc = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=16384, cache_vertex=1)
+    accn = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+    errs = torch.empty(args.nrc_frames, dtype=torch.float32, device=dev)
+    evs = [(ev(), ev()) for _ in range(args.nrc_frames)]
+    torch.cuda.synchronize()
+    tw = time.perf_counter()
+    for fr in range(args.nrc_frames):
+        evs[fr][0].record()
+        ctx.render_nrc(tlas, rec, sc.camera, lt, nrc, W, H, spp=1, train_paths=65536, sample_offset=fr,
+                       accum64=accn)
+        evs[fr][1].record()
+        errs[fr] = relmse((accn / (fr + 1)).float(), ref)
+    torch.cuda.synchronize()
+    wall_nrc = time.perf_counter() - tw
+    fms = np.array([x.elapsed_time(y) for x, y in evs])
+    e = errs.cpu().numpy()
+    cum = np.cumsum(fms)
+    hit = np.nonzero(e <= err_plain)[0]
+    if len(hit):
+        f = int(hit[0])
+        ms_nrc = float(cum[f])
+    else:
+        f, ms_nrc = None, None
+    steps, loss = nrc.status()
+    nrc.close()
+    log(f"NRC: frame {fms.mean():.2f} ms, relMSE after {args.nrc_frames} frames {e[-1]:.5f}; reaches plain relMSE "
+        f"at frame {None if f is None else f + 1} ({ms_nrc} ms)")
+    return {
+        "workload": f"closed Cornell box + displaced level-8 icosphere ({sc.unique_triangle_count} triangles), "
+                    f"{W}x{H}, depth 8, RR after 3, seed 1234",
+        "plain": {"spp": SPP, "ms": round(ms_plain, 2), "rays": int(n_rays),
+                  "mrays_per_s": round(n_rays / ms_plain / 1e3, 1), "relmse": round(err_plain, 6)},
+        "reference_spp": 64 * chunks,
+        "nrc": {"frames": args.nrc_frames, "train_paths_per_frame": 65536, "batch_size": 16384,
+                "cache_vertex": 1, "frame_ms": round(float(fms.mean()), 3),
+                "relmse_final": round(float(e[-1]), 6), "adam_steps": int(steps),
+                "frames_to_plain_relmse": None if f is None else f + 1,
+                "ms_to_plain_relmse": None if ms_nrc is None else round(ms_nrc, 2),
+                "wall_s": round(wall_nrc, 2)},
+        "equal_relmse_speedup": None if ms_nrc is None else round(ms_plain / ms_nrc, 2),
+    }
 
 
 if __name__ == "__main__":

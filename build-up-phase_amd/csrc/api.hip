@@ -200,6 +200,8 @@ int rtx_sync(rtx_context ctx) {
             return fail(ctx, RTX_ERROR_STACK_OVERFLOW, "traversal stack overflow (depth > %u)",
                         ctx->cfg.lds_stack_depth + ctx->gdepth);
         if (flags & rtx::kErrSortTimeout) return fail(ctx, RTX_ERROR_DEVICE_LOST, "radix sort look-back timed out");
+        if (flags & rtx::kErrBuildDepth)
+            return fail(ctx, RTX_ERROR_INVALID_AS, "BVH deeper than the builder's collapse level bound");
         return fail(ctx, RTX_ERROR_INVALID_AS, "device reported an invalid acceleration structure");
     }
     return RTX_SUCCESS;
@@ -681,6 +683,22 @@ int rtx_debug_lbvh(rtx_context ctx, const float* boxes_device, uint32_t n, uint3
     hipError_t e = rtx::debug_lbvh(boxes_device, n, morton_bits == 63, codes_device, order_device, left_device,
                                    right_device, &ctx->state->error_flags, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_debug_lbvh");
+    return rtx_sync(ctx);
+}
+
+int rtx_debug_treelet(rtx_context ctx, const float* boxes_device, uint32_t n, uint32_t morton_bits, uint32_t rounds,
+                      uint32_t* order_device, int32_t* left_device, int32_t* right_device, float* cost_device,
+                      uint32_t* height_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (morton_bits != 30 && morton_bits != 63)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "morton_bits must be 30 or 63");
+    if (rounds > 3) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "rounds must be 0..3");
+    if (n < 2 || n > kMaxPrims) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "n must be in [2, 2^30]");
+    if (!boxes_device || !order_device || !left_device || !right_device || !cost_device || !height_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
+    hipError_t e = rtx::debug_treelet(boxes_device, n, morton_bits == 63, rounds, order_device, left_device,
+                                      right_device, cost_device, height_device, &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_debug_treelet");
     return rtx_sync(ctx);
 }
 

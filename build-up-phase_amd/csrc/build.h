@@ -22,6 +22,9 @@ hipError_t build_tlas(const rtx_instance* instances_device, uint32_t instance_co
 // Verification hooks: Morton -> sort -> Karras on caller boxes; the bare radix sort.  Both block.
 hipError_t debug_lbvh(const float* boxes, uint32_t n, bool morton63, void* codes_out, uint32_t* order_out,
                       int32_t* left_out, int32_t* right_out, uint32_t* err_flags, hipStream_t stream);
+hipError_t debug_treelet(const float* boxes, uint32_t n, bool morton63, uint32_t rounds, uint32_t* order_out,
+                         int32_t* left_out, int32_t* right_out, float* cost_out, uint32_t* hgt_out,
+                         uint32_t* err_flags, hipStream_t s);
 hipError_t debug_sort(void* keys, uint32_t* vals, uint32_t n, int key_bytes, uint32_t* err_flags, hipStream_t stream);
 
 }  // namespace rtx

@@ -348,55 +348,371 @@ __device__ __forceinline__ void store_coherent(float* p, float v) {
     __hip_atomic_store((uint32_t*)p, __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// node boxes: index 0..n-2 internal, n-1+pos leaves (sorted position)
+__device__ __forceinline__ uint32_t load_coherent_u32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_coherent_u32(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int32_t load_coherent_i32(const int32_t* p) {
+    return (int32_t)load_coherent_u32((const uint32_t*)p);
+}
+__device__ __forceinline__ void store_coherent_i32(int32_t* p, int32_t v) { store_coherent_u32((uint32_t*)p, (uint32_t)v); }
+
+// ---- bottom-up refit with SAH treelet refinement ------------------------------------------------------------
 //
-// Cross-thread hand-off without agent-scope fences (on gfx950 those write back and invalidate the whole L2 of
-// the XCD): boxes are published with agent-scope atomic stores, which are coherent across XCDs, the wave waits
-// for its stores to complete (s_waitcnt) before it bumps the parent's arrival count, and the second arriver
-// reads its sibling's box with agent-scope atomic loads, issued only after its counter atomic returned.
-__global__ __launch_bounds__(256) void refit_kernel(uint32_t n, const int32_t* __restrict__ left,
-                                                    const int32_t* __restrict__ right,
-                                                    const int32_t* __restrict__ parent,
-                                                    const uint32_t* __restrict__ order,
-                                                    const float* __restrict__ prim_boxes, float* nboxes,
-                                                    uint32_t* arrivals) {
-    uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos >= n) return;
-    const float* pb = prim_boxes + 6 * (uint64_t)order[pos];
-    float* lb = nboxes + 6 * ((uint64_t)n - 1 + pos);
-    float cur[6];
+// Node numbering: internal nodes 0..n-2, leaves n-1+pos (sorted position); a child code c >= 0 is internal node
+// c, c < 0 the leaf at position -1-c.  Per internal node the pass keeps the box, the SAH cost of its subtree
+// (sum of internal-node surface areas, the only term a restructuring can change), the primitive count and the
+// height (leaves: cost 0, count 1, height 0).  The node cost is area + (cost(left) + cost(right)).
+//
+// The second thread to reach a node (atomic arrival count) merges its children.  When a node with at least
+// min_prims primitives is complete, the whole wave restructures the treelet below it (Karras & Aila 2013,
+// "Fast Parallel Construction of High-Quality Bounding Volume Hierarchies"):
+//   formation  start from the node's two children and open the internal treelet leaf with the largest surface
+//              area (lowest slot on ties) until there are 7 leaves; the node and the 5 opened nodes are the
+//              treelet's 6 internal nodes
+//   DP         over the 127 leaf subsets S in order of size: cost(S) = area(S) + min over splits {P, S\P} of
+//              (cost(P) + cost(S\P)), P containing the lowest leaf of S; ties go to the smallest P
+//   rebuild    only on a strict cost decrease and when the treelet height does not grow (so the whole tree
+//              keeps the depth bound the collapse relies on): breadth-first from S = all leaves, internal
+//              nodes reused in formation order.
+// A treelet covers only nodes of its root's finished subtree and is processed after that subtree is final, so
+// the result does not depend on the order in which waves finish.  Cross-wave hand-off uses agent-scope atomic
+// loads and stores (coherent across XCDs without L2 write-back fences): a wave waits for its stores to
+// complete (s_waitcnt) before it bumps a parent's arrival count.
+constexpr int kTreeletLeaves = 7;
+constexpr int kTreeletSubsets = 127;
+
+// Subsets of the 7 treelet leaves ordered by size, then value; kSizeOffset[k] = first subset of size k.
+__constant__ uint8_t kSubsetsBySize[127] = {
+    1, 2, 4, 8, 16, 32, 64, 3, 5, 6, 9, 10, 12, 17, 18, 20, 24, 33, 34, 36, 40, 48, 65, 66, 68, 72, 80, 96, 7, 11,
+    13, 14, 19, 21, 22, 25, 26, 28, 35, 37, 38, 41, 42, 44, 49, 50, 52, 56, 67, 69, 70, 73, 74, 76, 81, 82, 84, 88,
+    97, 98, 100, 104, 112, 15, 23, 27, 29, 30, 39, 43, 45, 46, 51, 53, 54, 57, 58, 60, 71, 75, 77, 78, 83, 85, 86,
+    89, 90, 92, 99, 101, 102, 105, 106, 108, 113, 114, 116, 120, 31, 47, 55, 59, 61, 62, 79, 87, 91, 93, 94, 103,
+    107, 109, 110, 115, 117, 118, 121, 122, 124, 63, 95, 111, 119, 123, 125, 126, 127};
+__constant__ uint8_t kSizeOffset[9] = {0, 0, 7, 28, 63, 98, 119, 126, 127};
+
+struct TreeletLds {
+    float area[128];
+    float cost[128];
+    uint32_t part[128];
+    float box[kTreeletLeaves][6];
+    float lcost[kTreeletLeaves];
+    uint32_t lcnt[kTreeletLeaves], lhgt[kTreeletLeaves];
+    int32_t leaf[kTreeletLeaves];
+    int32_t pool[kTreeletLeaves - 1];
+    // rebuild list: subset, node, child codes, child list index (-1: treelet leaf), height
+    uint32_t ls[kTreeletLeaves - 1];
+    int32_t lx[kTreeletLeaves - 1], lc[kTreeletLeaves - 1][2], li[kTreeletLeaves - 1][2];
+    uint32_t lh[kTreeletLeaves - 1];
+    uint32_t commit;
+};
+
+__device__ __forceinline__ uint64_t node_index(uint32_t n, int32_t c) {
+    return c >= 0 ? (uint64_t)c : (uint64_t)n - 1 + (uint64_t)(-1 - c);
+}
+
+__device__ __forceinline__ float coherent_area(const float* nboxes, uint64_t i) {
+    float b[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cur[k] = pb[k];
+    for (int k = 0; k < 6; ++k) b[k] = load_coherent(nboxes + 6 * i + k);
+    return box_area(b);
+}
+
+// Whole-wave treelet restructuring below internal node R (uniform arguments).  Returns R's cost and height
+// afterwards through new_cost / new_h.
+__device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old_cost, uint32_t old_h,
+                                 int32_t* left, int32_t* right, int32_t* parent, float* nboxes, float* cost,
+                                 uint32_t* cnt, uint32_t* hgt, float& new_cost, uint32_t& new_h) {
+    const uint32_t lane = threadIdx.x;
+    __syncthreads();
+    // formation (every lane computes the same thing; the loads are wave-uniform)
+    int32_t lf[kTreeletLeaves];
+    float la[kTreeletLeaves];
+    int32_t pool[kTreeletLeaves - 1];
+    lf[0] = load_coherent_i32(left + R);
+    lf[1] = load_coherent_i32(right + R);
+    la[0] = lf[0] >= 0 ? coherent_area(nboxes, (uint64_t)lf[0]) : 0.0f;
+    la[1] = lf[1] >= 0 ? coherent_area(nboxes, (uint64_t)lf[1]) : 0.0f;
+    pool[0] = R;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) store_coherent(lb + k, cur[k]);
-    uint64_t self = (uint64_t)n - 1 + pos;
-    int32_t node = parent[n - 1 + pos];
-    while (node >= 0) {
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        uint32_t old = __hip_atomic_fetch_add(&arrivals[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 0u) return;
-        int32_t l = left[node], r = right[node];
-        uint64_t li = l >= 0 ? (uint64_t)l : (uint64_t)n - 1 + (uint64_t)(-1 - l);
-        uint64_t ri = r >= 0 ? (uint64_t)r : (uint64_t)n - 1 + (uint64_t)(-1 - r);
-        const uint64_t sib = (li == self) ? ri : li;
+    for (int k = 2; k < kTreeletLeaves; ++k) {
+        int best = -1;
+        float ba = -1.0f;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            cur[a] = fminf(cur[a], load_coherent(nboxes + 6 * sib + a));
-            cur[3 + a] = fmaxf(cur[3 + a], load_coherent(nboxes + 6 * sib + 3 + a));
+        for (int j = 0; j < k; ++j)
+            if (lf[j] >= 0 && la[j] > ba) {
+                ba = la[j];
+                best = j;
+            }
+        if (best < 0) {  // fewer than 7 primitives below R: nothing to restructure
+            new_cost = old_cost;
+            new_h = old_h;
+            return;
         }
-        float* nb = nboxes + 6 * (uint64_t)node;
+        int32_t c = 0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) store_coherent(nb + k, cur[k]);
-        self = (uint64_t)node;
-        node = parent[node];
+        for (int j = 0; j < k; ++j)
+            if (j == best) c = lf[j];
+        pool[k - 1] = c;
+        const int32_t l = load_coherent_i32(left + c), r = load_coherent_i32(right + c);
+        const float al = l >= 0 ? coherent_area(nboxes, (uint64_t)l) : 0.0f;
+        const float ar = r >= 0 ? coherent_area(nboxes, (uint64_t)r) : 0.0f;
+#pragma unroll
+        for (int j = 0; j < k; ++j)
+            if (j == best) {
+                lf[j] = l;
+                la[j] = al;
+            }
+        lf[k] = r;
+        la[k] = ar;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < kTreeletLeaves; ++j) T.leaf[j] = lf[j];
+#pragma unroll
+        for (int j = 0; j < kTreeletLeaves - 1; ++j) T.pool[j] = pool[j];
+    }
+    __syncthreads();
+    if (lane < (uint32_t)kTreeletLeaves) {
+        const int32_t c = T.leaf[lane];
+        const uint64_t bi = node_index(n, c);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) T.box[lane][k] = load_coherent(nboxes + 6 * bi + k);
+        T.lcost[lane] = c >= 0 ? load_coherent(cost + c) : 0.0f;
+        T.lcnt[lane] = c >= 0 ? load_coherent_u32(cnt + c) : 1u;
+        T.lhgt[lane] = c >= 0 ? load_coherent_u32(hgt + c) : 0u;
+    }
+    __syncthreads();
+    // surface area of every subset's union box; singleton costs
+    for (uint32_t S = lane + 1; S <= (uint32_t)kTreeletSubsets; S += 64) {
+        float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+        for (int j = 0; j < kTreeletLeaves; ++j)
+            if (S & (1u << j)) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    b[a] = fminf(b[a], T.box[j][a]);
+                    b[3 + a] = fmaxf(b[3 + a], T.box[j][3 + a]);
+                }
+            }
+        T.area[S] = box_area(b);
+        if (__popc(S) == 1) {
+            T.cost[S] = T.lcost[__ffs(S) - 1];
+            T.part[S] = 0;
+        }
+    }
+    __syncthreads();
+    // subsets of size 2..6: one lane per subset
+    for (int k = 2; k < kTreeletLeaves; ++k) {
+        const uint32_t o = kSizeOffset[k], m = kSizeOffset[k + 1] - o;
+        if (lane < m) {
+            const uint32_t S = kSubsetsBySize[o + lane];
+            const uint32_t low = S & (0u - S);
+            float best = INFINITY;
+            uint32_t bp = 0;
+            for (uint32_t P = (S - 1) & S; P; P = (P - 1) & S) {
+                if (!(P & low)) continue;
+                const float c = T.cost[P] + T.cost[S ^ P];
+                if (c < best || (c == best && P < bp)) {
+                    best = c;
+                    bp = P;
+                }
+            }
+            T.cost[S] = T.area[S] + best;
+            T.part[S] = bp;
+        }
+        __syncthreads();
+    }
+    // all 7 leaves: the 63 splits containing leaf 0, one per lane, then a (cost, split) minimum over the wave
+    float c7 = INFINITY;
+    uint32_t p7 = 0xFFFFFFFFu;
+    if (lane < 63u) {
+        p7 = 1u | (lane << 1);
+        c7 = T.cost[p7] + T.cost[127u ^ p7];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float oc = __shfl_xor(c7, off);
+        const uint32_t op = (uint32_t)__shfl_xor((int)p7, off);
+        if (oc < c7 || (oc == c7 && op < p7)) {
+            c7 = oc;
+            p7 = op;
+        }
+    }
+    const float total = T.area[127] + c7;
+    new_cost = old_cost;
+    new_h = old_h;
+    if (!(total < old_cost)) return;
+    if (lane == 0) {
+        T.part[127] = p7;
+        T.ls[0] = 127u;
+        T.lx[0] = R;
+        uint32_t cntl = 1, next = 1;
+        bool ok = true;
+        for (uint32_t i = 0; i < cntl && ok; ++i) {
+            const uint32_t S = T.ls[i];
+            const uint32_t P = T.part[S];
+            const uint32_t C[2] = {P, S ^ P};
+            if (P == 0u || (P & S) != P || P == S) {  // no finite split was found
+                ok = false;
+                break;
+            }
+            for (int h = 0; h < 2; ++h) {
+                if (__popc(C[h]) == 1) {
+                    T.lc[i][h] = T.leaf[__ffs(C[h]) - 1];
+                    T.li[i][h] = -1 - (__ffs(C[h]) - 1);
+                } else {
+                    const int32_t x = T.pool[next++];
+                    T.ls[cntl] = C[h];
+                    T.lx[cntl] = x;
+                    T.lc[i][h] = x;
+                    T.li[i][h] = (int32_t)cntl;
+                    ++cntl;
+                }
+            }
+        }
+        for (int i = kTreeletLeaves - 2; ok && i >= 0; --i) {
+            uint32_t hc[2];
+            for (int h = 0; h < 2; ++h) {
+                const int32_t q = T.li[i][h];
+                hc[h] = q >= 0 ? T.lh[q] : T.lhgt[-1 - q];
+            }
+            T.lh[i] = 1u + (hc[0] > hc[1] ? hc[0] : hc[1]);
+        }
+        T.commit = ok && T.lh[0] <= old_h ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!T.commit) return;
+    if (lane < (uint32_t)(kTreeletLeaves - 1)) {
+        const int32_t x = T.lx[lane];
+        const uint32_t S = T.ls[lane];
+        store_coherent_i32(left + x, T.lc[lane][0]);
+        store_coherent_i32(right + x, T.lc[lane][1]);
+        store_coherent_i32(parent + node_index(n, T.lc[lane][0]), x);
+        store_coherent_i32(parent + node_index(n, T.lc[lane][1]), x);
+        if (lane > 0) {  // the root's box is unchanged; its cost and height are published by its owner
+            float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            uint32_t np = 0;
+            for (int j = 0; j < kTreeletLeaves; ++j)
+                if (S & (1u << j)) {
+                    for (int a = 0; a < 3; ++a) {
+                        b[a] = fminf(b[a], T.box[j][a]);
+                        b[3 + a] = fmaxf(b[3 + a], T.box[j][3 + a]);
+                    }
+                    np += T.lcnt[j];
+                }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) store_coherent(nboxes + 6 * (uint64_t)x + k, b[k]);
+            store_coherent(cost + x, T.cost[S]);
+            store_coherent_u32(cnt + x, np);
+            store_coherent_u32(hgt + x, T.lh[lane]);
+        }
+    }
+    new_cost = total;
+    new_h = T.lh[0];
+}
+
+// Treelet refinement rounds: default kTreeletRounds, RTX_BUILD_TREELET_ROUNDS_* overrides, NO_TREELET disables;
+// round r restructures nodes with at least kTreeletMinPrims[r] primitives.
+constexpr int kTreeletRounds = 2;
+constexpr uint32_t kTreeletMinPrims[3] = {7, 14, 28};
+int treelet_rounds(uint32_t flags, uint32_t n) {
+    if ((flags & RTX_BUILD_NO_TREELET_BIT) || n < (uint32_t)kTreeletLeaves) return 0;
+    const uint32_t r = (flags >> RTX_BUILD_TREELET_ROUNDS_SHIFT) & 3u;
+    return r ? (int)r : kTreeletRounds;
+}
+
+// One wave per block.  min_prims = 0xFFFFFFFF: plain refit.
+__global__ __launch_bounds__(64) void refit_refine_kernel(uint32_t n, int32_t* left, int32_t* right, int32_t* parent,
+                                                          const uint32_t* __restrict__ order,
+                                                          const float* __restrict__ prim_boxes, float* nboxes,
+                                                          float* cost, uint32_t* cnt, uint32_t* hgt,
+                                                          uint32_t* arrivals, uint32_t min_prims) {
+    __shared__ TreeletLds T;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t pos = blockIdx.x * 64u + lane;
+    bool active = pos < n;
+    float cur[6] = {0, 0, 0, 0, 0, 0};
+    float ccost = 0.0f;
+    uint32_t ccnt = 1, chgt = 0;
+    uint64_t self = 0;
+    int32_t node = -1;
+    if (active) {
+        const float* pb = prim_boxes + 6 * (uint64_t)order[pos];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cur[k] = pb[k];
+        self = (uint64_t)n - 1 + pos;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) store_coherent(nboxes + 6 * self + k, cur[k]);
+        node = load_coherent_i32(parent + self);
+    }
+    while (__ballot(active)) {
+        bool own = false;
+        if (active) {
+            if (node < 0) {
+                active = false;
+            } else {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                const uint32_t old = __hip_atomic_fetch_add(&arrivals[node], 1u, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+                if (old == 0u) {
+                    active = false;
+                } else {
+                    own = true;
+                    const int32_t l = load_coherent_i32(left + node), r = load_coherent_i32(right + node);
+                    const uint64_t li = node_index(n, l), ri = node_index(n, r);
+                    const bool self_left = li == self;
+                    const int32_t sc = self_left ? r : l;
+                    const uint64_t sib = self_left ? ri : li;
+                    const float scost = sc >= 0 ? load_coherent(cost + sc) : 0.0f;
+                    const uint32_t scnt = sc >= 0 ? load_coherent_u32(cnt + sc) : 1u;
+                    const uint32_t shgt = sc >= 0 ? load_coherent_u32(hgt + sc) : 0u;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        cur[a] = fminf(cur[a], load_coherent(nboxes + 6 * sib + a));
+                        cur[3 + a] = fmaxf(cur[3 + a], load_coherent(nboxes + 6 * sib + 3 + a));
+                    }
+                    const float cl = self_left ? ccost : scost, cr = self_left ? scost : ccost;
+                    ccost = box_area(cur) + (cl + cr);
+                    ccnt += scnt;
+                    chgt = 1u + (chgt > shgt ? chgt : shgt);
+                }
+            }
+        }
+        uint64_t need = __ballot(own && ccnt >= min_prims);
+        while (need) {
+            const int L = __ffsll((unsigned long long)need) - 1;
+            need &= need - 1;
+            const int32_t R = __shfl(node, L);
+            const float rc = __shfl(ccost, L);
+            const uint32_t rh = (uint32_t)__shfl((int)chgt, L);
+            float nc;
+            uint32_t nh;
+            treelet_optimize(T, n, R, rc, rh, left, right, parent, nboxes, cost, cnt, hgt, nc, nh);
+            if ((int)lane == L) {
+                ccost = nc;
+                chgt = nh;
+            }
+        }
+        if (own) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) store_coherent(nboxes + 6 * (uint64_t)node + k, cur[k]);
+            store_coherent(cost + node, ccost);
+            store_coherent_u32(cnt + node, ccnt);
+            store_coherent_u32(hgt + node, chgt);
+            self = (uint64_t)node;
+            node = load_coherent_i32(parent + node);
+        }
     }
 }
 
 // Header setup after refit: root box and the 16-bit origin quantisation step.
 __global__ void header_kernel(AsHeader* h, uint32_t type, uint32_t flags, uint32_t n, uint32_t ngeom,
-                              const float* nboxes, uint64_t capacity_slots, uint64_t table_offset,
+                              const float* nboxes, const float* cost, uint64_t capacity_slots, uint64_t table_offset,
                               uint32_t* level_counts, uint2* tasks0) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     h->magic = type ? kMagicTlas : kMagicBlas;
@@ -423,6 +739,10 @@ __global__ void header_kernel(AsHeader* h, uint32_t type, uint32_t flags, uint32
         }
     }
     bool empty = !(lo[0] <= hi[0]) || n == 0;
+    if (n > 1) {
+        const float ra = box_area(nboxes);
+        h->sah_cost = ra > 0.0f ? (double)cost[0] / (double)ra : 0.0;
+    }
     for (int a = 0; a < 3; ++a) {
         if (empty) {
             lo[a] = 0.0f;
@@ -649,6 +969,14 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
     }
 }
 
+// Tasks left after the last enqueued level mean the binary tree was deeper than the level bound.
+__global__ void collapse_check_kernel(AsHeader* h, const uint32_t* level_counts, int max_levels, uint32_t* err) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && level_counts[max_levels] != 0u) {
+        h->build_error = 1u;
+        atomicOr(err, kErrBuildDepth);
+    }
+}
+
 __global__ void finish_tlas_kernel(AsHeader* h, const uint32_t* max_blas_depth) {
     if (threadIdx.x == 0 && blockIdx.x == 0) h->max_total_depth = h->max_depth + *max_blas_depth;
 }
@@ -663,8 +991,8 @@ __global__ void fill_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
 
 // ---- scratch layout --------------------------------------------------------------------------------------------
 struct Layout {
-    uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, tasks0, tasks1, levels, cbounds,
-        misc, geoms, total;
+    uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, cost, cnt, hgt, tasks0, tasks1,
+        levels, cbounds, misc, geoms, total;
 };
 
 Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom) {
@@ -686,6 +1014,9 @@ Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom) {
     L.parent = take(2 * nn * 4);
     L.nboxes = take(2 * nn * 24);
     L.arrivals = take(nn * 4);
+    L.cost = take(nn * 4);
+    L.cnt = take(nn * 4);
+    L.hgt = take(nn * 4);
     L.tasks0 = take(nn * 8);
     L.tasks1 = take(nn * 8);
     L.levels = take(128 * 4);
@@ -703,6 +1034,21 @@ uint64_t as_bytes(uint32_t n, bool tlas) {
     return align_up(b, 256);
 }
 
+// Refit with treelet refinement; each further round restructures the tree the previous one left.
+void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L, const float* prim_boxes,
+                  hipStream_t s) {
+    const int rounds = treelet_rounds(flags, n);
+    uint32_t* arrivals = (uint32_t*)(scratch + L.arrivals);
+    for (int round = 0; round < (rounds ? rounds : 1); ++round) {
+        if (n > 1) fill_kernel<<<256, 256, 0, s>>>(arrivals, n, 0u);
+        refit_refine_kernel<<<(n + 63) / 64, 64, 0, s>>>(
+            n, (int32_t*)(scratch + L.left), (int32_t*)(scratch + L.right), (int32_t*)(scratch + L.parent),
+            (const uint32_t*)(scratch + L.vals), prim_boxes, (float*)(scratch + L.nboxes), (float*)(scratch + L.cost),
+            (uint32_t*)(scratch + L.cnt), (uint32_t*)(scratch + L.hgt), arrivals,
+            rounds ? kTreeletMinPrims[round] : 0xFFFFFFFFu);
+    }
+}
+
 // Common LBVH tail: morton -> sort -> karras -> refit -> header -> collapse.
 hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint32_t ngeom, uint8_t* scratch,
                        const Layout& L, uint8_t* dst, uint64_t table_offset, uint32_t* err_flags, hipStream_t s) {
@@ -715,7 +1061,7 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
     int32_t* right = (int32_t*)(scratch + L.right);
     int32_t* parent = (int32_t*)(scratch + L.parent);
     float* nboxes = (float*)(scratch + L.nboxes);
-    uint32_t* arrivals = (uint32_t*)(scratch + L.arrivals);
+    float* cost = (float*)(scratch + L.cost);
     uint32_t* levels = (uint32_t*)(scratch + L.levels);
     uint2* t0 = (uint2*)(scratch + L.tasks0);
     uint2* t1 = (uint2*)(scratch + L.tasks1);
@@ -736,15 +1082,14 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
                 karras_kernel<uint64_t><<<gb, 256, 0, s>>>((uint64_t*)(scratch + L.keys), n, left, right, parent);
             else
                 karras_kernel<uint32_t><<<gb, 256, 0, s>>>((uint32_t*)(scratch + L.keys), n, left, right, parent);
-            fill_kernel<<<256, 256, 0, s>>>(arrivals, n, 0u);
         } else {
             fill_kernel<<<1, 64, 0, s>>>((uint32_t*)parent, 1, 0xFFFFFFFFu);
         }
-        refit_kernel<<<gb, 256, 0, s>>>(n, left, right, parent, vals, boxes, nboxes, arrivals);
+        refit_passes(n, flags, scratch, L, boxes, s);
     }
     fill_kernel<<<1, 128, 0, s>>>(levels, 128, 0u);
     // root box: internal node 0, or the single leaf (stored at index n-1 == 0)
-    header_kernel<<<1, 64, 0, s>>>(h, type, flags, n, ngeom, nboxes, 2ull * (n ? n : 1) + 1, table_offset, levels, t0);
+    header_kernel<<<1, 64, 0, s>>>(h, type, flags, n, ngeom, nboxes, cost, 2ull * (n ? n : 1) + 1, table_offset, levels, t0);
     if (n > 0) {
         const int max_levels = k64 ? kMaxLevels64 : kMaxLevels32;
         for (int lv = 0; lv < max_levels; ++lv) {
@@ -754,14 +1099,18 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
                                                                   (const uint4*)(scratch + L.leaves), h, slots, levels,
                                                                   lv, tin, tout);
         }
+        collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, max_levels, err_flags);
     }
     return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t debug_lbvh(const float* boxes, uint32_t n, bool k64, void* codes_out, uint32_t* order_out,
-                      int32_t* left_out, int32_t* right_out, uint32_t* err_flags, hipStream_t s) {
+namespace {
+// Morton / sort / Karras on caller boxes, optionally followed by the refit + refinement passes (rounds >= 0).
+hipError_t debug_tree(const float* boxes, uint32_t n, bool k64, int rounds, void* codes_out, uint32_t* order_out,
+                      int32_t* left_out, int32_t* right_out, float* cost_out, uint32_t* hgt_out, uint32_t* err_flags,
+                      hipStream_t s) {
     Layout L = make_layout(n, k64 ? 8 : 4, 0);
     uint8_t* sc = nullptr;
     hipError_t e = hipMalloc(&sc, L.total);
@@ -787,13 +1136,32 @@ hipError_t debug_lbvh(const float* boxes, uint32_t n, bool k64, void* codes_out,
                                                            (int32_t*)(sc + L.right), (int32_t*)(sc + L.parent));
         }
     }
-    if (!e && n > 0) e = hipMemcpyAsync(codes_out, sc + L.keys, (uint64_t)n * (k64 ? 8 : 4), hipMemcpyDeviceToDevice, s);
+    if (!e && rounds >= 0 && n > 1)
+        refit_passes(n, rounds ? RTX_BUILD_TREELET_ROUNDS(rounds) : RTX_BUILD_NO_TREELET_BIT, sc, L, boxes, s);
+    const uint64_t m = n > 1 ? (uint64_t)(n - 1) * 4 : 0;
+    if (!e && n > 0 && codes_out)
+        e = hipMemcpyAsync(codes_out, sc + L.keys, (uint64_t)n * (k64 ? 8 : 4), hipMemcpyDeviceToDevice, s);
     if (!e && n > 0) e = hipMemcpyAsync(order_out, vals, (uint64_t)n * 4, hipMemcpyDeviceToDevice, s);
-    if (!e && n > 1) e = hipMemcpyAsync(left_out, sc + L.left, (uint64_t)(n - 1) * 4, hipMemcpyDeviceToDevice, s);
-    if (!e && n > 1) e = hipMemcpyAsync(right_out, sc + L.right, (uint64_t)(n - 1) * 4, hipMemcpyDeviceToDevice, s);
+    if (!e && m) e = hipMemcpyAsync(left_out, sc + L.left, m, hipMemcpyDeviceToDevice, s);
+    if (!e && m) e = hipMemcpyAsync(right_out, sc + L.right, m, hipMemcpyDeviceToDevice, s);
+    if (!e && m && cost_out) e = hipMemcpyAsync(cost_out, sc + L.cost, m, hipMemcpyDeviceToDevice, s);
+    if (!e && m && hgt_out) e = hipMemcpyAsync(hgt_out, sc + L.hgt, m, hipMemcpyDeviceToDevice, s);
     hipError_t e2 = hipStreamSynchronize(s);
     (void)hipFree(sc);
     return e ? e : e2;
+}
+}  // namespace
+
+hipError_t debug_lbvh(const float* boxes, uint32_t n, bool k64, void* codes_out, uint32_t* order_out,
+                      int32_t* left_out, int32_t* right_out, uint32_t* err_flags, hipStream_t s) {
+    return debug_tree(boxes, n, k64, -1, codes_out, order_out, left_out, right_out, nullptr, nullptr, err_flags, s);
+}
+
+hipError_t debug_treelet(const float* boxes, uint32_t n, bool k64, uint32_t rounds, uint32_t* order_out,
+                         int32_t* left_out, int32_t* right_out, float* cost_out, uint32_t* hgt_out,
+                         uint32_t* err_flags, hipStream_t s) {
+    return debug_tree(boxes, n, k64, (int)rounds, nullptr, order_out, left_out, right_out, cost_out, hgt_out,
+                      err_flags, s);
 }
 
 hipError_t debug_sort(void* keys, uint32_t* vals, uint32_t n, int key_bytes, uint32_t* err_flags, hipStream_t s) {

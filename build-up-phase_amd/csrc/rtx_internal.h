@@ -141,6 +141,7 @@ struct DeviceState {
 constexpr uint32_t kErrStackOverflow = 1u;
 constexpr uint32_t kErrSortTimeout = 2u;
 constexpr uint32_t kErrBadAs = 4u;
+constexpr uint32_t kErrBuildDepth = 8u;
 
 // ---- small device helpers ---------------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t float_to_ordered(float f) {

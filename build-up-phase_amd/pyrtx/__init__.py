@@ -26,6 +26,12 @@ RTX_BUILD_PREFER_FAST_TRACE_BIT = 0x4
 RTX_BUILD_PREFER_FAST_BUILD_BIT = 0x8
 RTX_BUILD_MORTON63_BIT = 0x10000
 RTX_BUILD_NO_TREELET_BIT = 0x20000
+RTX_BUILD_TREELET_ROUNDS_SHIFT = 18
+
+
+def RTX_BUILD_TREELET_ROUNDS(r: int) -> int:
+    return (r & 3) << RTX_BUILD_TREELET_ROUNDS_SHIFT
+
 RTX_INDEX_TYPE_UINT32 = 1
 RTX_QUERY_CLOSEST_HIT = 0
 RTX_QUERY_ANY_HIT = 1
@@ -144,6 +150,7 @@ SIGNATURES = {
     "rtx_nrc_debug_gradient": (ctypes.c_int, [_P, _P, _U32, _P, _P]),
     "rtx_debug_lbvh": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _P]),
     "rtx_debug_sort": (ctypes.c_int, [_P, _P, _P, _U32, _U32]),
+    "rtx_debug_treelet": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -415,6 +422,21 @@ class Context:
         c = codes.cpu().numpy()[:n].view(np.uint64 if bits == 63 else np.uint32)
         return c, order.cpu().numpy()[:n].view(np.uint32), left.cpu().numpy()[:max(n - 1, 0)], \
             right.cpu().numpy()[:max(n - 1, 0)]
+
+    def debug_treelet(self, boxes, rounds, bits=30):
+        """boxes: device float32 [n, 6], n >= 2.  Returns (order, left, right, cost, height) after the build's
+        refit + treelet refinement passes."""
+        torch = _torch()
+        n = boxes.shape[0]
+        order = torch.empty((n,), dtype=torch.int32, device=self.device)
+        left = torch.empty((n - 1,), dtype=torch.int32, device=self.device)
+        right = torch.empty((n - 1,), dtype=torch.int32, device=self.device)
+        cost = torch.empty((n - 1,), dtype=torch.float32, device=self.device)
+        hgt = torch.empty((n - 1,), dtype=torch.int32, device=self.device)
+        self._check(self.lib.rtx_debug_treelet(self.handle, boxes.data_ptr(), n, bits, rounds, order.data_ptr(),
+                                               left.data_ptr(), right.data_ptr(), cost.data_ptr(), hgt.data_ptr()))
+        return (order.cpu().numpy().view(np.uint32), left.cpu().numpy(), right.cpu().numpy(), cost.cpu().numpy(),
+                hgt.cpu().numpy().view(np.uint32))
 
     def debug_sort(self, keys: np.ndarray):
         torch = _torch()

@@ -69,6 +69,10 @@ void* rtx_get_stream(rtx_context ctx);
 #define RTX_BUILD_PREFER_FAST_BUILD_BIT 0x8u
 #define RTX_BUILD_MORTON63_BIT 0x10000u                    /* 63-bit Morton keys (default: 30-bit) */
 #define RTX_BUILD_NO_TREELET_BIT 0x20000u                  /* skip SAH treelet refinement */
+/* Treelet refinement rounds in bits 18-19 (0 = default 2, 1..3 rounds); round r restructures nodes with at
+ * least 7 * 2^r primitives. */
+#define RTX_BUILD_TREELET_ROUNDS_SHIFT 18u
+#define RTX_BUILD_TREELET_ROUNDS(r) (((uint32_t)(r) & 3u) << RTX_BUILD_TREELET_ROUNDS_SHIFT)
 
 #define RTX_INDEX_TYPE_UINT32 1u                           /* VK_INDEX_TYPE_UINT32, main.cpp:736 */
 
@@ -143,7 +147,8 @@ typedef struct rtx_as_info {
     uint32_t max_depth;            /* BVH8 levels */
     uint32_t geometry_count;
     float bounds[6];               /* lo xyz, hi xyz */
-    double sah_cost;               /* SAH cost of the binary hierarchy before collapse */
+    double sah_cost;               /* SAH cost of the binary hierarchy before collapse: sum of internal-node
+                                      surface areas / root surface area (0 for fewer than 2 primitives) */
 } rtx_as_info;
 int rtx_get_as_info(rtx_context ctx, uint64_t as_address, rtx_as_info* out_info);
 
@@ -359,6 +364,13 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
  * (c >= 0 internal node, c < 0 leaf at sorted position -1-c).  Blocks until done; allocates its own scratch. */
 int rtx_debug_lbvh(rtx_context ctx, const float* boxes_device, uint32_t n, uint32_t morton_bits, void* codes_device,
                    uint32_t* order_device, int32_t* left_device, int32_t* right_device);
+
+/* rtx_debug_lbvh followed by the build's refit and SAH treelet refinement (rounds 0..3; 0 = refit only), on
+ * n >= 2 boxes: copies out the sorted primitive order, the refined left/right child codes and, per internal node,
+ * the subtree cost (float: sum of internal-node surface areas) and height (uint32).  Blocks until done. */
+int rtx_debug_treelet(rtx_context ctx, const float* boxes_device, uint32_t n, uint32_t morton_bits, uint32_t rounds,
+                      uint32_t* order_device, int32_t* left_device, int32_t* right_device, float* cost_device,
+                      uint32_t* height_device);
 
 /* Sort n (key, value) pairs with the builder's radix sort: keys uint32 (key_bytes 4) or uint64 (8); values are
  * replaced by the stable sorted permutation.  Blocks until done; allocates its own scratch. */

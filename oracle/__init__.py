@@ -55,6 +55,10 @@ def lib():
         L.oracle_morton.argtypes = [P, ctypes.c_int64, ctypes.c_int, P]
         L.oracle_karras.restype = None
         L.oracle_karras.argtypes = [P, ctypes.c_int64, ctypes.c_int, P, P]
+        L.oracle_treelet_dp.restype = None
+        L.oracle_treelet_dp.argtypes = [P, P, P, P, P]
+        L.oracle_treelet_refine.restype = None
+        L.oracle_treelet_refine.argtypes = [ctypes.c_uint32, P, P, P, ctypes.c_uint32, P, P, P]
         L.oracle_pcg32.restype = None
         L.oracle_pcg32.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, P]
         L.oracle_pcg4d.restype = None
@@ -219,6 +223,32 @@ def karras(sorted_keys: np.ndarray, key_bits: int):
     right = np.zeros(max(n - 1, 0), np.int32)
     lib().oracle_karras(_ptr(keys), n, key_bits, _ptr(left), _ptr(right))
     return left, right
+
+
+def treelet_dp(boxes: np.ndarray, leaf_cost: np.ndarray):
+    """Subset DP of the treelet refinement over 7 leaves: boxes float32 [7, 6], leaf subtree costs [7].
+    Returns (area[128], cost[128], part[128]) indexed by leaf subset bitmask."""
+    b = np.ascontiguousarray(boxes, np.float32).reshape(7, 6)
+    c = np.ascontiguousarray(leaf_cost, np.float32).reshape(7)
+    area = np.zeros(128, np.float32)
+    cost = np.zeros(128, np.float32)
+    part = np.zeros(128, np.uint32)
+    lib().oracle_treelet_dp(_ptr(b), _ptr(c), _ptr(area), _ptr(cost), _ptr(part))
+    return area, cost, part
+
+
+def treelet_refine(left: np.ndarray, right: np.ndarray, leaf_boxes: np.ndarray, rounds: int):
+    """Refit + SAH treelet refinement of a binary radix tree (Karras child codes), leaf boxes in sorted order.
+    Returns (left, right, cost, height, sah) with cost/height per internal node."""
+    L = np.ascontiguousarray(left, np.int32).copy()
+    R = np.ascontiguousarray(right, np.int32).copy()
+    n = L.size + 1
+    b = np.ascontiguousarray(leaf_boxes, np.float32).reshape(n, 6)
+    cost = np.zeros(n - 1, np.float32)
+    hgt = np.zeros(n - 1, np.uint32)
+    sah = ctypes.c_double()
+    lib().oracle_treelet_refine(n, _ptr(L), _ptr(R), _ptr(b), rounds, _ptr(cost), _ptr(hgt), ctypes.byref(sah))
+    return L, R, cost, hgt, sah.value
 
 
 def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:

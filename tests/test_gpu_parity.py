@@ -10,6 +10,7 @@ import pytest
 
 import oracle as O
 from helpers import compare_closest, oracle_scene, small_mixed_scene
+import pyrtx
 from pyrtx import scenes as S
 
 pytestmark = pytest.mark.gpu
@@ -95,6 +96,58 @@ def test_morton_sort_karras_bit_exact(ctx, bits, n):
         ol, orr = O.karras(ok, 32 if bits == 30 else 64)
         assert np.array_equal(left, ol)
         assert np.array_equal(right, orr)
+
+
+def _random_boxes(n, seed):
+    rng = np.random.default_rng(seed)
+    lo = rng.uniform(-5, 5, (n, 3)).astype(np.float32)
+    ext = (rng.uniform(0, 0.3, (n, 3)) ** 2).astype(np.float32)
+    if n > 100:
+        lo[10:60] = lo[10]  # identical boxes: equal codes, equal areas, ties in the treelet DP
+        ext[10:60] = ext[10]
+    return np.concatenate([lo, lo + ext], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("n,rounds", [(2, 1), (7, 1), (1000, 0), (1000, 1), (50000, 2), (200000, 3)])
+def test_treelet_refinement_bit_exact(ctx, bits, n, rounds):
+    _say(f"treelet bits={bits} n={n} rounds={rounds}")
+    boxes = _random_boxes(n, bits + n + rounds)
+    db = torch.from_numpy(boxes).cuda()
+    _, order, kl, kr = ctx.debug_lbvh(db, bits)  # the Karras tree, bit-exact with the oracle's (test above)
+    o2, L, R, cost, hgt = ctx.debug_treelet(db, rounds, bits)
+    assert np.array_equal(o2, order)
+    el, er, ec, eh, _ = O.treelet_refine(kl, kr, boxes[order], rounds)
+    assert np.array_equal(L, el) and np.array_equal(R, er)
+    assert np.array_equal(cost.view(np.uint32), ec.view(np.uint32))
+    assert np.array_equal(hgt, eh)
+    if rounds == 0:
+        assert np.array_equal(L, kl) and np.array_equal(R, kr)
+
+
+def test_treelet_refinement_on_mesh_and_reported_cost(ctx):
+    _say("treelet on terrain + icosphere BLAS")
+    for g in (S.terrain(96, seed=5), S.icosphere(30, radius=1.0, seed=2, amplitude=0.05)):
+        tri = g.vertices[g.indices]
+        boxes = np.concatenate([tri.min(1), tri.max(1)], 1).astype(np.float32)
+        db = torch.from_numpy(boxes).cuda()
+        _, order, kl, kr = ctx.debug_lbvh(db, 30)
+        sah = {}
+        for rounds in (0, 2):
+            o2, L, R, cost, hgt = ctx.debug_treelet(db, rounds, 30)
+            el, er, ec, eh, esah = O.treelet_refine(kl, kr, boxes[order], rounds)
+            assert np.array_equal(L, el) and np.array_equal(R, er)
+            assert np.array_equal(cost.view(np.uint32), ec.view(np.uint32))
+            sah[rounds] = esah
+        assert sah[2] < 0.95 * sah[0]
+        # the BLAS build runs the same passes: its reported SAH cost is the oracle's
+        geom = [{"vertices": torch.from_numpy(g.vertices.astype(np.float32)).cuda(),
+                 "indices": torch.from_numpy(g.indices.astype(np.uint32).view(np.int32)).cuda(), "transform": None}]
+        for flags, rounds in ((pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT, 2),
+                              (pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT | pyrtx.RTX_BUILD_NO_TREELET_BIT, 0)):
+            b = ctx.build_blas(geom, flags)
+            info = ctx.as_info(b)
+            assert abs(info.sah_cost - sah[rounds]) <= 1e-12 * sah[rounds], (info.sah_cost, sah[rounds])
 
 
 # ---- reference scenes -------------------------------------------------------------------------------------------

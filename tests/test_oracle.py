@@ -319,3 +319,113 @@ def test_nrc_oracle_backprop_matches_finite_differences():
         fd = (surrogate(q1) - surrogate(q2)) / (float(q1[i]) - float(q2[i]))
         assert abs(fd - g[i]) <= 1e-7 + 1e-6 * abs(g[i])
 
+
+
+# ---- SAH treelet refinement -------------------------------------------------------------------------------------
+def _all_trees(k):
+    """Every full binary tree over leaves 0..k-1 (leaf k inserted above every subtree of every smaller tree):
+    (2k-3)!! trees."""
+    def insert(t, leaf):
+        yield (t, leaf)
+        if isinstance(t, tuple):
+            a, b = t
+            for x in insert(a, leaf):
+                yield (x, b)
+            for x in insert(b, leaf):
+                yield (a, x)
+    trees = [0]
+    for leaf in range(1, k):
+        trees = [x for t in trees for x in insert(t, leaf)]
+    return trees
+
+
+def _union_area64(boxes, mask):
+    sel = boxes[[j for j in range(7) if mask >> j & 1]].astype(np.float64)
+    lo, hi = sel[:, :3].min(0), sel[:, 3:].max(0)
+    d = hi - lo
+    return d[0] * d[1] + d[1] * d[2] + d[2] * d[0]
+
+
+def _tree_cost64(t, boxes, leaf_cost):
+    if not isinstance(t, tuple):
+        return 1 << t, float(leaf_cost[t])
+    ma, ca = _tree_cost64(t[0], boxes, leaf_cost)
+    mb, cb = _tree_cost64(t[1], boxes, leaf_cost)
+    m = ma | mb
+    return m, _union_area64(boxes, m) + ca + cb
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_treelet_dp_equals_brute_force_over_all_10395_trees(seed):
+    rng = np.random.default_rng(seed)
+    lo = rng.uniform(-1, 1, (7, 3)).astype(np.float32)
+    ext = rng.uniform(0, 1, (7, 3)) ** (3 if seed % 2 else 1)
+    boxes = np.concatenate([lo, (lo + ext).astype(np.float32)], 1)
+    leaf_cost = np.where(rng.random(7) < 0.5, 0.0, rng.uniform(0, 20, 7)).astype(np.float32)
+    area, cost, part = O.treelet_dp(boxes, leaf_cost)
+    trees = _all_trees(7)
+    assert len(trees) == 10395
+    brute = min(_tree_cost64(t, boxes, leaf_cost)[1] for t in trees)
+    assert abs(cost[127] - brute) <= 1e-5 * brute
+    # the tree the split table encodes has that cost
+    def rebuild(S):
+        if bin(S).count("1") == 1:
+            return S.bit_length() - 1
+        return (rebuild(int(part[S])), rebuild(S ^ int(part[S])))
+    assert abs(_tree_cost64(rebuild(127), boxes, leaf_cost)[1] - brute) <= 1e-5 * brute
+    for S in range(1, 128):
+        assert abs(area[S] - _union_area64(boxes, S)) <= 1e-6 * max(1.0, area[S])
+        if bin(S).count("1") > 1:
+            P = int(part[S])
+            assert P & S == P and 0 < P < S and P & (S & -S)
+
+
+def _tree_check(left, right, leaf_boxes):
+    """Validate a binary tree (Karras codes) and return (float64 SAH from scratch, height)."""
+    n = left.size + 1
+    kids = np.concatenate([left, right])
+    assert np.array_equal(np.sort(-1 - kids[kids < 0]), np.arange(n))
+    assert np.array_equal(np.sort(kids[kids >= 0]), np.arange(1, n - 1))
+    b64 = leaf_boxes.astype(np.float64)
+    def walk(c):
+        if c < 0:
+            return b64[-1 - c], 0.0, 0
+        bl, cl, hl = walk(int(left[c]))
+        br, cr, hr = walk(int(right[c]))
+        b = np.concatenate([np.minimum(bl[:3], br[:3]), np.maximum(bl[3:], br[3:])])
+        d = b[3:] - b[:3]
+        return b, d[0] * d[1] + d[1] * d[2] + d[2] * d[0] + cl + cr, 1 + max(hl, hr)
+    import sys
+    sys.setrecursionlimit(10000)
+    b, c, h = walk(0)
+    d = b[3:] - b[:3]
+    return c / (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]), h
+
+
+@pytest.mark.parametrize("mesh", ["sphere", "random"])
+def test_treelet_refinement_invariants(mesh):
+    if mesh == "sphere":
+        g = S.icosphere(12, radius=1.0, seed=3, amplitude=0.05)
+        tri = g.vertices[g.indices]
+    else:
+        rng = np.random.default_rng(11)
+        c = rng.uniform(-5, 5, (3000, 1, 3))
+        tri = (c + rng.normal(0, 0.2, (3000, 3, 3))).astype(np.float32)
+    boxes = np.concatenate([tri.min(1), tri.max(1)], 1).astype(np.float32)
+    cen = ((boxes[:, :3] + boxes[:, 3:]) * np.float32(0.5)).astype(np.float32)
+    keys, order = O.sort_pairs(O.morton(cen, 30))
+    kl, kr = O.karras(keys, 32)
+    lb = boxes[order]
+    res = {r: O.treelet_refine(kl, kr, lb, r) for r in (0, 1, 2, 3)}
+    # rounds = 0 is a plain refit: the Karras tree is unchanged
+    assert np.array_equal(res[0][0], kl) and np.array_equal(res[0][1], kr)
+    prev = None
+    for r in (0, 1, 2, 3):
+        L, R, cost, hgt, sah = res[r]
+        sah64, h = _tree_check(L, R, lb)
+        assert abs(sah - sah64) <= 1e-4 * sah64          # reported cost matches the tree
+        assert hgt[0] == h and h <= res[0][3][0]          # height never grows
+        if prev is not None:
+            assert sah <= prev * (1 + 1e-6)              # cost never increases
+        prev = sah
+    assert res[1][4] < 0.97 * res[0][4]                  # refinement does reduce the SAH cost
