@@ -29,7 +29,7 @@ struct rtx_context_t {
     uint2* gstack = nullptr;
     uint64_t gstack_entries = 0;
     int cu_count = 256;
-    rtx_trace_config cfg{256, 3, 12, 0};
+    rtx_trace_config cfg{256, 4, 12, RTX_TRACE_RAY_CHUNK_AUTO, 48, 4};
     uint32_t gdepth = 64;
     bool stats = false;
     void* pt_scratch = nullptr;
@@ -132,8 +132,12 @@ rtx::TraceLaunch launch_shape(rtx_context ctx) {
     rtx::TraceLaunch L;
     L.block_size = ctx->cfg.block_size;
     L.grid = (uint32_t)ctx->cu_count * ctx->cfg.blocks_per_cu;
+    L.blocks_per_cu = ctx->cfg.blocks_per_cu;
     L.lds_depth = ctx->cfg.lds_stack_depth;
     L.gdepth = ctx->gdepth;
+    L.ray_chunk = ctx->cfg.ray_chunk;
+    L.refill_lanes = ctx->cfg.refill_lanes;
+    L.refill_interval = ctx->cfg.refill_interval;
     return L;
 }
 
@@ -655,6 +659,15 @@ int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg) {
     if (cfg->block_size) c.block_size = cfg->block_size;
     if (cfg->blocks_per_cu) c.blocks_per_cu = cfg->blocks_per_cu;
     if (cfg->lds_stack_depth) c.lds_stack_depth = cfg->lds_stack_depth;
+    if (cfg->ray_chunk) c.ray_chunk = cfg->ray_chunk;
+    if (cfg->refill_lanes) c.refill_lanes = cfg->refill_lanes;
+    if (cfg->refill_interval) c.refill_interval = cfg->refill_interval;
+    if (c.ray_chunk != RTX_TRACE_RAY_CHUNK_AUTO && (c.ray_chunk < 64 || c.ray_chunk > 4096 || (c.ray_chunk & 63)))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "ray_chunk must be a multiple of 64 up to 4096, or AUTO");
+    if (c.refill_lanes < 1 || c.refill_lanes > 64)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "refill_lanes must be in [1, 64]");
+    if (c.refill_interval < 1 || c.refill_interval > 64)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "refill_interval must be in [1, 64]");
     if (c.block_size < 64 || c.block_size > 256 || (c.block_size & 63))
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
     if (c.blocks_per_cu < 1 || c.blocks_per_cu > 32)
@@ -668,6 +681,103 @@ int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg) {
 int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg) {
     if (!ctx || !out_cfg) return RTX_ERROR_INVALID_ARGUMENT;
     *out_cfg = ctx->cfg;
+    return RTX_SUCCESS;
+}
+
+// Coordinate descent over the traversal knobs, each candidate timed with HIP events on the context stream; a
+// candidate replaces the incumbent only when it is faster by more than timing noise (1.5%).
+constexpr float kTuneGain = 0.985f;
+int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device, uint32_t ray_count,
+                       uint32_t mode, void* results_device, uint32_t reps, rtx_trace_config* out_cfg,
+                       float* out_ms_per_launch) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (ray_count == 0 || !rays_device || !results_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "autotuning needs a non-empty ray buffer and a result buffer");
+    if (reps == 0 || reps > 64) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "reps must be in [1, 64]");
+    rtx_hit* hits = mode == RTX_QUERY_CLOSEST_HIT ? (rtx_hit*)results_device : nullptr;
+    uint32_t* occl = mode == RTX_QUERY_ANY_HIT ? (uint32_t*)results_device : nullptr;
+    hipEvent_t ev[2];
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) {
+        e = hipEventCreate(&ev[1]);
+        if (e != hipSuccess) (void)hipEventDestroy(ev[0]);
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_autotune_trace events");
+    const rtx_trace_config start = ctx->cfg;
+    int r = RTX_SUCCESS;
+    // time one candidate; +inf when the configuration is rejected
+    auto time_cfg = [&](const rtx_trace_config& c, float& ms) -> int {
+        ms = INFINITY;
+        const rtx_trace_config keep = ctx->cfg;
+        if (rtx_set_trace_config(ctx, &c) != RTX_SUCCESS) {
+            ctx->cfg = keep;
+            ctx->last_error.clear();
+            return RTX_SUCCESS;
+        }
+        int rr = rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0xFFu, hits, occl);
+        if (rr) return rr;
+        hipError_t he = hipEventRecord(ev[0], ctx->stream);
+        for (uint32_t i = 0; he == hipSuccess && i < reps; ++i) {
+            rr = rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0xFFu, hits, occl);
+            if (rr) return rr;
+        }
+        if (he == hipSuccess) he = hipEventRecord(ev[1], ctx->stream);
+        if (he == hipSuccess) he = hipEventSynchronize(ev[1]);
+        float el = 0.0f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&el, ev[0], ev[1]);
+        if (he != hipSuccess) return hip_fail(ctx, he, "rtx_autotune_trace timing");
+        ms = el / (float)reps;
+        return rtx_sync(ctx);
+    };
+    rtx_trace_config best = start;
+    float best_ms = INFINITY;
+    r = time_cfg(best, best_ms);
+    // knob groups, tried in order; every group keeps the best value found so far for the other knobs
+    static const uint32_t shapes[][2] = {{256, 2}, {256, 3}, {256, 4}, {128, 6}, {128, 8}, {64, 12}, {64, 16}};
+    static const uint32_t refills[][2] = {{40, 16}, {32, 8}, {40, 4}, {48, 4}, {48, 2}, {56, 2}, {56, 4}, {60, 1}};
+    static const uint32_t chunks[] = {RTX_TRACE_RAY_CHUNK_AUTO, 64, 128, 256, 512};
+    static const uint32_t depths[] = {4, 6, 8, 12, 16};
+    for (int pass = 0; pass < 2 && r == RTX_SUCCESS; ++pass) {
+        for (auto& sh : shapes) {
+            rtx_trace_config c = best;
+            c.block_size = sh[0];
+            c.blocks_per_cu = sh[1];
+            float ms;
+            if ((r = time_cfg(c, ms))) break;
+            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+        }
+        for (size_t i = 0; r == RTX_SUCCESS && i < sizeof(refills) / sizeof(refills[0]); ++i) {
+            rtx_trace_config c = best;
+            c.refill_lanes = refills[i][0];
+            c.refill_interval = refills[i][1];
+            float ms;
+            if ((r = time_cfg(c, ms))) break;
+            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+        }
+        for (size_t i = 0; r == RTX_SUCCESS && i < sizeof(chunks) / sizeof(chunks[0]); ++i) {
+            rtx_trace_config c = best;
+            c.ray_chunk = chunks[i];
+            float ms;
+            if ((r = time_cfg(c, ms))) break;
+            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+        }
+        for (size_t i = 0; r == RTX_SUCCESS && i < sizeof(depths) / sizeof(depths[0]); ++i) {
+            rtx_trace_config c = best;
+            c.lds_stack_depth = depths[i];
+            float ms;
+            if ((r = time_cfg(c, ms))) break;
+            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+        }
+    }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (r != RTX_SUCCESS) {
+        ctx->cfg = start;
+        return r;
+    }
+    ctx->cfg = best;
+    if (out_cfg) *out_cfg = best;
+    if (out_ms_per_launch) *out_ms_per_launch = best_ms;
     return RTX_SUCCESS;
 }
 

@@ -35,8 +35,12 @@ struct TraceRequest {
 struct TraceLaunch {
     uint32_t grid;
     uint32_t block_size;
+    uint32_t blocks_per_cu;
     uint32_t lds_depth;  // stack entries per lane in LDS
     uint32_t gdepth;     // further entries per lane in the global spill area
+    uint32_t ray_chunk;        // ray ids per queue fetch (< 64: automatic)
+    uint32_t refill_lanes;     // refill a wave once fewer lanes than this are busy (1..64)
+    uint32_t refill_interval;  // traversal steps between refill checks (1..64)
 };
 
 hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);

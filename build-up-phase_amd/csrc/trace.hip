@@ -3,8 +3,8 @@
 // Replaces the driver traversal behind traceRayEXT (call site main.cpp:1047-1052) and the raygen / miss /
 // closest-hit programs (main.cpp:1033-1091).
 //
-// Per lane (wave64; one persistent grid, rays fetched 64 at a time from an atomic counter, lanes refilled
-// when fewer than 40 of 64 are still busy):
+// Per lane (wave64; one persistent grid; a wave takes ray ids from an atomic counter in chunks and refills its
+// idle lanes once fewer than refill_lanes of 64 are busy, checked every refill_interval steps):
 //   * world ray (FP32) -> TLAS: nodes are tested with a conservative quantised slab test, instance leaves
 //     enter their BLAS;
 //   * BLAS entry (FP64): the ray is transformed by the FP64 inverse instance transform and its origin moved
@@ -195,6 +195,9 @@ struct TraceArgs {
     uint2* gstack;
     uint32_t lds_depth;
     uint32_t gdepth;
+    uint32_t ray_chunk;        // ray ids per queue fetch (< 64: automatic)
+    uint32_t refill_lanes;     // refill a wave once fewer lanes than this are busy
+    uint32_t refill_interval;  // steps between refill checks
 };
 
 enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3 };
@@ -249,7 +252,7 @@ __device__ void world_normal(const double* inv, const TriSlot* tri, float* n) {
 }
 
 template <int MODE, bool STATS>
-__global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
+__device__ __forceinline__ void trace_body(const TraceArgs& args) {
     extern __shared__ uint2 lds_stack[];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -289,7 +292,8 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
     // launch has at least 8 rounds of work for every wave, else 64 so that small launches still spread over all
     // waves).
     const uint32_t waves_total = gridDim.x * (blockDim.x >> 6);
-    const uint32_t chunk = (uint64_t)nrays >= (uint64_t)waves_total * 64u * 8u ? 256u : 64u;
+    const uint32_t chunk = args.ray_chunk >= 64u ? args.ray_chunk
+                           : ((uint64_t)nrays >= (uint64_t)waves_total * 64u * 8u ? 256u : 64u);
     uint32_t pool_next = 0, pool_end = 0;
     for (;;) {
         const uint64_t need = __ballot(!active && !exhausted);
@@ -381,7 +385,43 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
                 ng_bits = 0;
                 sp = 0;
             }
-            if (lg_bits & 255u) {
+            // a lane without pending work pops its next group and processes it in this same step
+            if (!(lg_bits & 255u) && !(ng_bits & 255u)) {
+                if (sp > 0) {
+                    --sp;
+                    const uint2 e =
+                        sp < D ? lds_stack[sp * blockDim.x + threadIdx.x] : args.gstack[gtid * GD + (sp - D)];
+                    if ((e.y & kEntryTlas) && in_blas) {
+                        in_blas = false;
+                        s.ox = wo[0];
+                        s.oy = wo[1];
+                        s.oz = wo[2];
+                        s.dx = wd[0];
+                        s.dy = wd[1];
+                        s.dz = wd[2];
+                        s.toff = 0.0f;
+                        setup_space(s);
+                        cslots = tslots;
+                        cur_nslots = tlas_nslots;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            crlo[a] = trlo[a];
+                            cqs[a] = tqs[a];
+                        }
+                    }
+                    if (e.y & kEntryLeaf) {
+                        lg_base = e.x;
+                        lg_bits = e.y & 0xFFFFu;
+                    } else {
+                        ng_base = e.x;
+                        ng_bits = e.y & 0xFFFFu;
+                    }
+                } else {
+                    done = true;
+                }
+            }
+            if (done) {
+            } else if (lg_bits & 255u) {
                 if (!in_blas) {
                     // instance leaf: enter its BLAS
                     const uint32_t m = lg_bits & 255u;
@@ -535,36 +575,6 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
                     lg_base = w0.w;
                     lg_bits = (hm & ~imask) | (vmask << 8);
                 }
-            } else if (sp > 0) {
-                --sp;
-                const uint2 e = sp < D ? lds_stack[sp * blockDim.x + threadIdx.x] : args.gstack[gtid * GD + (sp - D)];
-                if ((e.y & kEntryTlas) && in_blas) {
-                    in_blas = false;
-                    s.ox = wo[0];
-                    s.oy = wo[1];
-                    s.oz = wo[2];
-                    s.dx = wd[0];
-                    s.dy = wd[1];
-                    s.dz = wd[2];
-                    s.toff = 0.0f;
-                    setup_space(s);
-                    cslots = tslots;
-                    cur_nslots = tlas_nslots;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        crlo[a] = trlo[a];
-                        cqs[a] = tqs[a];
-                    }
-                }
-                if (e.y & kEntryLeaf) {
-                    lg_base = e.x;
-                    lg_bits = e.y & 0xFFFFu;
-                } else {
-                    ng_base = e.x;
-                    ng_bits = e.y & 0xFFFFu;
-                }
-            } else {
-                done = true;
             }
 
             if (done) {
@@ -644,9 +654,9 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
                 active = false;
                 break;
             }
-            if (++iter >= 16) {
+            if (++iter >= (int)args.refill_interval) {
                 iter = 0;
-                if (!wave_exhausted && __popcll(__ballot(true)) < 40) break;
+                if (!wave_exhausted && __popcll(__ballot(true)) < args.refill_lanes) break;
             }
         }
     }
@@ -657,6 +667,17 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
         atomicAdd(&args.state->stat_insts, (unsigned long long)st_insts);
         atomicAdd(&args.state->stat_rays, (unsigned long long)st_rays);
     }
+}
+
+// Two occupancy variants of the same traversal: unconstrained registers, and at most 128 VGPRs so that four
+// waves fit on a SIMD (chosen when the launch shape asks for four or more waves per SIMD).
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
+    trace_body<MODE, STATS>(args);
+}
+template <int MODE, bool STATS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel_w4(TraceArgs args) {
+    trace_body<MODE, STATS>(args);
 }
 
 __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
@@ -674,7 +695,11 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
 template <int MODE, bool STATS>
 hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
     const size_t lds = (size_t)L.lds_depth * L.block_size * sizeof(uint2);
-    trace_kernel<MODE, STATS><<<L.grid, L.block_size, lds, s>>>(a);
+    const uint32_t waves_per_simd = (L.block_size / 64u) * L.blocks_per_cu / 4u;
+    if (waves_per_simd >= 4u)
+        trace_kernel_w4<MODE, STATS><<<L.grid, L.block_size, lds, s>>>(a);
+    else
+        trace_kernel<MODE, STATS><<<L.grid, L.block_size, lds, s>>>(a);
     return hipGetLastError();
 }
 
@@ -709,6 +734,9 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.gstack = r.gstack;
     a.lds_depth = L.lds_depth;
     a.gdepth = L.gdepth;
+    a.ray_chunk = L.ray_chunk;
+    a.refill_lanes = L.refill_lanes;
+    a.refill_interval = L.refill_interval;
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
     if (r.n == 0 && !r.n_ptr) return hipGetLastError();
     switch (r.mode) {

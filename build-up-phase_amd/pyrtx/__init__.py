@@ -110,7 +110,11 @@ NRC_PARAM_COUNT = 23888
 
 class TraceConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
-                ("lds_stack_depth", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("lds_stack_depth", ctypes.c_uint32), ("ray_chunk", ctypes.c_uint32),
+                ("refill_lanes", ctypes.c_uint32), ("refill_interval", ctypes.c_uint32)]
+
+
+RTX_TRACE_RAY_CHUNK_AUTO = 1
 
 
 # symbol -> (restype, argtypes); the exported surface of include/rtx.h
@@ -150,6 +154,8 @@ SIGNATURES = {
     "rtx_nrc_debug_gradient": (ctypes.c_int, [_P, _P, _U32, _P, _P]),
     "rtx_debug_lbvh": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _P]),
     "rtx_debug_sort": (ctypes.c_int, [_P, _P, _P, _U32, _U32]),
+    "rtx_autotune_trace": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _U32, _U32, _P, _U32, ctypes.POINTER(TraceConfig),
+                                          ctypes.POINTER(ctypes.c_float)]),
     "rtx_debug_treelet": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
 }
 
@@ -399,14 +405,28 @@ class Context:
         return {"rays": s.rays, "node_fetches": s.node_fetches, "triangle_tests": s.triangle_tests,
                 "instance_entries": s.instance_entries}
 
-    def set_config(self, block_size=0, blocks_per_cu=0, lds_stack_depth=0):
-        c = TraceConfig(block_size, blocks_per_cu, lds_stack_depth, 0)
+    def set_config(self, block_size=0, blocks_per_cu=0, lds_stack_depth=0, ray_chunk=0, refill_lanes=0,
+                   refill_interval=0):
+        c = TraceConfig(block_size, blocks_per_cu, lds_stack_depth, ray_chunk, refill_lanes, refill_interval)
         self._check(self.lib.rtx_set_trace_config(self.handle, ctypes.byref(c)))
 
     def config(self) -> dict:
         c = TraceConfig()
         self._check(self.lib.rtx_get_trace_config(self.handle, ctypes.byref(c)))
-        return {"block_size": c.block_size, "blocks_per_cu": c.blocks_per_cu, "lds_stack_depth": c.lds_stack_depth}
+        return {"block_size": c.block_size, "blocks_per_cu": c.blocks_per_cu, "lds_stack_depth": c.lds_stack_depth,
+                "ray_chunk": c.ray_chunk, "refill_lanes": c.refill_lanes, "refill_interval": c.refill_interval}
+
+    def autotune(self, tlas: AccelerationStructure, rays, mode=RTX_QUERY_CLOSEST_HIT, reps=2):
+        """rtx_autotune_trace on a device ray tensor [N, 8] float32; the best configuration stays active.
+        Returns (config dict, ms per launch)."""
+        torch = _torch()
+        n = rays.shape[0]
+        res = torch.empty((n, 6) if mode == RTX_QUERY_CLOSEST_HIT else (n,), dtype=torch.int32, device=self.device)
+        c = TraceConfig()
+        ms = ctypes.c_float()
+        self._check(self.lib.rtx_autotune_trace(self.handle, tlas.address, rays.data_ptr(), n, mode, res.data_ptr(),
+                                                reps, ctypes.byref(c), ctypes.byref(ms)))
+        return self.config(), ms.value
 
     # -- verification hooks ----------------------------------------------------------------------------------------
     def debug_lbvh(self, boxes, bits=30):

@@ -254,15 +254,29 @@ typedef struct rtx_trace_stats {
 int rtx_set_trace_stats(rtx_context ctx, int enable);
 int rtx_get_trace_stats(rtx_context ctx, rtx_trace_stats* out_stats);
 
-/* Traversal kernel launch shape (the autotuner's knobs).  0 fields keep the current value. */
+/* Traversal kernel launch shape and scheduling (the autotuner's knobs).  0 fields keep the current value. */
 typedef struct rtx_trace_config {
-    uint32_t block_size;      /* threads per block: 64..1024, multiple of 64 */
-    uint32_t blocks_per_cu;   /* persistent blocks per compute unit */
+    uint32_t block_size;      /* threads per block: 64, 128, 192 or 256 */
+    uint32_t blocks_per_cu;   /* persistent blocks per compute unit (1..32) */
     uint32_t lds_stack_depth; /* stack entries held in LDS per lane before spilling to global memory */
-    uint32_t reserved;
+    uint32_t ray_chunk;       /* ray ids a wave takes from the global queue per fetch: a multiple of 64 up to
+                                 4096, or RTX_TRACE_RAY_CHUNK_AUTO (64, or 256 when every wave gets >= 8 chunks) */
+    uint32_t refill_lanes;    /* a wave fetches new rays once fewer than this many of its 64 lanes are busy */
+    uint32_t refill_interval; /* traversal steps between two refill checks (1..64) */
 } rtx_trace_config;
+#define RTX_TRACE_RAY_CHUNK_AUTO 1u
 int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg);
 int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg);
+
+/* Project autotuner for the traversal kernel (no counterpart in the reference, whose traversal is the driver's).
+ * Times candidate configurations on the caller's ray buffer with HIP events (coordinate descent, two passes:
+ * block size x blocks per CU, refill policy, ray chunk, LDS stack depth; `reps` timed launches each; a change
+ * must gain more than 1.5%), keeps the fastest on the context and returns it with its time per launch.  Tune on
+ * the workload itself: a few milliseconds of rays per launch at least.  results_device receives the query results
+ * (rtx_hit[] for closest hit, uint32[] for any hit).  Blocks until done. */
+int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device, uint32_t ray_count,
+                       uint32_t query_mode, void* results_device, uint32_t reps, rtx_trace_config* out_cfg,
+                       float* out_ms_per_launch);
 
 /* ---- path tracing: wavefront integrator (north-star extension of the raygen / closest-hit flow) ---------------
  *

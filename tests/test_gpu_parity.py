@@ -217,6 +217,29 @@ def test_mixed_scene_closest_and_any_hit(ctx, seed):
         assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
 
 
+def test_trace_results_independent_of_launch_config(ctx):
+    _say("trace configs")
+    sc = small_mixed_scene(seed=5, instances=24, freq=14, cells=20)
+    blases, tlas = ctx.upload_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=5)])
+    base_cfg = ctx.config()
+    ref = _trace_closest(ctx, tlas, rays)
+    ref_any = _trace_any(ctx, tlas, rays)
+    try:
+        for cfg in ({"blocks_per_cu": 4}, {"blocks_per_cu": 4, "lds_stack_depth": 4},
+                    {"block_size": 64, "blocks_per_cu": 16, "ray_chunk": 1024},
+                    {"block_size": 128, "blocks_per_cu": 6, "refill_lanes": 64, "refill_interval": 1},
+                    {"block_size": 256, "blocks_per_cu": 1, "ray_chunk": 64, "refill_lanes": 8,
+                     "refill_interval": 64}):
+            ctx.set_config(**cfg)
+            got = _trace_closest(ctx, tlas, rays)
+            assert all(np.array_equal(a, b) for a, b in zip(got, ref)), cfg
+            assert np.array_equal(_trace_any(ctx, tlas, rays), ref_any), cfg
+            ctx.set_config(**base_cfg)
+    finally:
+        ctx.set_config(**base_cfg)
+
+
 def test_query_scene_reduced_parity(ctx):
     _say("query scene 1/200 scale")
     sc = S.query_scene(scale_tris=0.005)
