@@ -44,6 +44,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays per set for the CPU baseline (0: skip)")
     ap.add_argument("--no-anyhit", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing / NRC equal-relMSE part")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default traversal configuration")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the FP64 reference render")
     ap.add_argument("--nrc-frames", type=int, default=160, help="NRC frames rendered (one sample each)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -91,9 +92,28 @@ def main():
     def trace(rays, mode=pyrtx.RTX_QUERY_CLOSEST_HIT):
         return ctx.trace_ray_buffer(tlas, rays, mode, 0xFF, hits if mode == pyrtx.RTX_QUERY_CLOSEST_HIT else occl)
 
-    # traversal counters (separate, untimed pass)
+    default_cfg = ctx.config()
+    # project autotuner: one traversal configuration per ray set and query type (untimed setup step)
+    tuned = {}
+    if not args.no_autotune:
+        for name, r in (("coherent", rc), ("incoherent", ri)):
+            for qn, mode in (("closest", pyrtx.RTX_QUERY_CLOSEST_HIT), ("any", pyrtx.RTX_QUERY_ANY_HIT)):
+                if qn == "any" and args.no_anyhit:
+                    continue
+                ctx.set_config(**default_cfg)
+                cfg, tms = ctx.autotune(tlas, r, mode, reps=2)
+                tuned[f"{qn}/{name}"] = cfg
+                log(f"autotuned {qn}/{name}: {cfg} ({tms:.3f} ms)")
+    cfg_of = lambda key: tuned.get(key, default_cfg)
+
+    def trace_cfg(key, rays, mode=pyrtx.RTX_QUERY_CLOSEST_HIT):
+        ctx.set_config(**cfg_of(key))
+        return trace(rays, mode)
+
+    # traversal counters (separate, untimed pass, same configurations as the timed launches)
     stats = {}
     for name, r in (("coherent", rc), ("incoherent", ri)):
+        ctx.set_config(**cfg_of(f"closest/{name}"))
         ctx.set_stats(True)
         trace(r)
         ctx.sync()
@@ -118,8 +138,8 @@ def main():
         del a, b
         log("calibration copy: 268435456 bytes read + 268435456 bytes written")
     for _ in range(args.warmup):
-        trace(rc)
-        trace(ri)
+        trace_cfg("closest/coherent", rc)
+        trace_cfg("closest/incoherent", ri)
     ctx.sync()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(4 * args.steps)]
@@ -128,7 +148,8 @@ def main():
     t_start = time.perf_counter()
     k = 0
     for step in range(args.steps):
-        for r in (rc, ri):
+        for key, r in (("closest/coherent", rc), ("closest/incoherent", ri)):
+            ctx.set_config(**cfg_of(key))
             ev[k][0].record()
             trace(r)
             ev[k][1].record()
@@ -149,6 +170,7 @@ def main():
     any_res = {}
     if not args.no_anyhit:
         for name, r in (("coherent", rc), ("incoherent", ri)):
+            ctx.set_config(**cfg_of(f"any/{name}"))
             trace(r, pyrtx.RTX_QUERY_ANY_HIT)
             ctx.sync()
             a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -195,6 +217,7 @@ def main():
         O.reset()
         log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads")
 
+    ctx.set_config(**default_cfg)
     render = None if args.no_render else render_bench(ctx, args, np, torch, pyrtx, S)
 
     out = {
@@ -217,6 +240,7 @@ def main():
         "closest_hit_mrays": {"coherent": round(mr_coh, 1), "incoherent": round(mr_inc, 1)},
         "any_hit_mrays": {k: round(v, 1) for k, v in any_res.items()},
         "traversal_counters_per_ray": stats,
+        "autotuned_trace_config": tuned or None,
         "build": {"blas_tlas_ms_incl_upload": round(build_s * 1e3, 1),
                   "nodes": int(sum(i.node_count for i in info)), "max_depth": int(tinfo.max_depth)},
         "roofline": roofline,
@@ -244,6 +268,10 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
     def relmse(x, ref):
         return torch.mean((x - ref) ** 2 / (ref ** 2 + 0.01))
 
+    tuned = None
+    if not args.no_autotune:
+        tuned, tms = ctx.autotune_render(tlas, rec, sc.camera, lt, W, H, spp=1)
+        log(f"autotuned path tracing: {tuned} ({tms:.2f} ms per 1-spp render)")
     ctx.render(tlas, rec, sc.camera, lt, W, H, spp=2, sample_offset=1 << 24)  # warm-up
     ctx.sync()
     r0 = ctx.render_ray_count()
@@ -310,6 +338,7 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
                 "ms_to_plain_relmse": None if ms_nrc is None else round(ms_nrc, 2),
                 "wall_s": round(wall_nrc, 2)},
         "equal_relmse_speedup": None if ms_nrc is None else round(ms_plain / ms_nrc, 2),
+        "autotuned_trace_config": tuned,
     }
 
 

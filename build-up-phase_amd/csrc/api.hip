@@ -29,11 +29,12 @@ struct rtx_context_t {
     uint2* gstack = nullptr;
     uint64_t gstack_entries = 0;
     int cu_count = 256;
-    rtx_trace_config cfg{256, 4, 12, RTX_TRACE_RAY_CHUNK_AUTO, 48, 4};
+    rtx_trace_config cfg{256, 4, 12, RTX_TRACE_RAY_CHUNK_AUTO, 48, 4, 8};
     uint32_t gdepth = 64;
     bool stats = false;
     void* pt_scratch = nullptr;
     uint64_t pt_scratch_bytes = 0;
+    std::vector<rtx_nrc_t*> caches;  // live caches; detached when the context is destroyed first
 };
 
 struct rtx_nrc_t {
@@ -138,6 +139,7 @@ rtx::TraceLaunch launch_shape(rtx_context ctx) {
     L.ray_chunk = ctx->cfg.ray_chunk;
     L.refill_lanes = ctx->cfg.refill_lanes;
     L.refill_interval = ctx->cfg.refill_interval;
+    L.leaf_lanes = ctx->cfg.leaf_lanes;
     return L;
 }
 
@@ -179,6 +181,7 @@ int rtx_create_context(int device, void* hip_stream, rtx_context* out_ctx) {
 int rtx_destroy_context(rtx_context ctx) {
     if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
     (void)hipStreamSynchronize(ctx->stream);
+    for (rtx_nrc n : ctx->caches) n->ctx = nullptr;  // they can still be destroyed, nothing else
     if (ctx->gstack) (void)hipFree(ctx->gstack);
     if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
     if (ctx->state) (void)hipFree(ctx->state);
@@ -397,7 +400,8 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
 }
 
 static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
-                        const rtx_render_params* params, bool have_out, rtx::PtArgs& a, rtx::TraceLaunch& L) {
+                        const rtx_render_params* params, bool have_out, rtx::PtArgs& a, rtx::TraceLaunch& L,
+                        uint32_t extra_paths = 0) {
     int r = check_tlas(ctx, tlas_address);
     if (r) return r;
     if (!sbt || !camera || !params) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt, camera, params must be set");
@@ -412,7 +416,7 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
     if (sbt->hit_record_count > 0 && !sbt->hit_records_device)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt hit records are NULL");
     const uint32_t npix = p.width * p.height;
-    const uint64_t need = rtx::pt_scratch_bytes(npix, p.max_depth, p.light_triangle_count);
+    const uint64_t need = rtx::pt_scratch_bytes(npix + extra_paths, p.max_depth, p.light_triangle_count);
     if (need > ctx->pt_scratch_bytes) {
         if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
         ctx->pt_scratch = nullptr;
@@ -423,7 +427,7 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
         ctx->pt_scratch_bytes = need;
     }
     a = rtx::PtArgs{};
-    rtx::pt_layout(a, ctx->pt_scratch, npix, p.max_depth, p.light_triangle_count);
+    rtx::pt_layout(a, ctx->pt_scratch, npix + extra_paths, p.max_depth, p.light_triangle_count);
     a.tlas = (const void*)(uintptr_t)tlas_address;
     a.width = p.width;
     a.height = p.height;
@@ -469,15 +473,19 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
                        const rtx_render_params* params, rtx_nrc nrc, uint32_t train_paths,
                        float* film_rgba32f_device, double* accum_rgb_device) {
     if (!ctx || !nrc) return RTX_ERROR_INVALID_ARGUMENT;
+    if (nrc->ctx != ctx) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "the cache belongs to another context");
     rtx::PtArgs a;
     rtx::TraceLaunch L;
-    int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L);
+    if (params && (uint64_t)params->width * params->height + train_paths > (1ull << 28))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "pixels + train_paths above 2^28");
+    int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L,
+                         train_paths);
     if (r) return r;
     if (params->max_depth <= nrc->st.cache_vertex)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "max_depth must exceed the cache vertex");
     a.cache_depth = nrc->st.cache_vertex;
-    if (train_paths > a.npix) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "train_paths above the pixel count");
-    const uint64_t need = rtx::pt_train_bytes(train_paths, params->max_depth);
+    const uint32_t rec_depths = rtx::nrc_record_depths(params->max_depth, a.cache_depth, nrc->st.train_suffix);
+    const uint64_t need = rtx::pt_train_bytes(train_paths, rec_depths);
     if (need > nrc->train_buf_bytes) {
         if (nrc->train_buf) (void)hipFree(nrc->train_buf);
         nrc->train_buf = nullptr;
@@ -487,8 +495,9 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
         nrc->train_buf_bytes = need;
     }
     uint32_t count = 0;
-    hipError_t e = rtx::pt_render_nrc(a, L, nrc->st, train_paths, nrc->train_buf, params->spp, params->sample_offset,
-                                      film_rgba32f_device, accum_rgb_device, &count, ctx->stream);
+    hipError_t e = rtx::pt_render_nrc(a, L, nrc->st, train_paths, nrc->st.train_suffix, nrc->train_buf, params->spp,
+                                      params->sample_offset, film_rgba32f_device, accum_rgb_device, &count,
+                                      ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render_nrc");
     return RTX_SUCCESS;
 }
@@ -520,6 +529,11 @@ int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc)
     st.beta2 = cfg->beta2;
     st.eps = cfg->epsilon;
     st.cache_vertex = cfg->cache_vertex ? cfg->cache_vertex : 1u;
+    st.train_suffix = cfg->train_suffix;
+    if (st.train_suffix > 62) {
+        delete n;
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "train_suffix above 62");
+    }
     if (st.cache_vertex > 62) {
         delete n;
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "cache_vertex above 62");
@@ -554,13 +568,18 @@ int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc)
         rtx_nrc_destroy(n);
         return hip_fail(ctx, e, "rtx_nrc_create");
     }
+    ctx->caches.push_back(n);
     *out_nrc = n;
     return RTX_SUCCESS;
 }
 
 int rtx_nrc_destroy(rtx_nrc n) {
     if (!n) return RTX_ERROR_INVALID_ARGUMENT;
-    if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
+    if (n->ctx) {
+        (void)hipStreamSynchronize(n->ctx->stream);
+        auto& v = n->ctx->caches;
+        v.erase(std::remove(v.begin(), v.end(), n), v.end());
+    }
     rtx::NrcState& st = n->st;
     if (st.params) (void)hipFree(st.params);
     if (st.m) (void)hipFree(st.m);
@@ -574,7 +593,7 @@ int rtx_nrc_destroy(rtx_nrc n) {
 }
 
 int rtx_nrc_cmd_infer(rtx_nrc n, const rtx_nrc_query* queries_device, uint32_t count, float* out_rgb_device) {
-    if (!n) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!n || !n->ctx) return RTX_ERROR_INVALID_ARGUMENT;  // NULL, or its context was destroyed
     if (count > 0 && (!queries_device || !out_rgb_device))
         return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
     if (!aligned(queries_device, 16)) return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "queries must be 16-B aligned");
@@ -584,7 +603,7 @@ int rtx_nrc_cmd_infer(rtx_nrc n, const rtx_nrc_query* queries_device, uint32_t c
 }
 
 int rtx_nrc_cmd_train(rtx_nrc n, const rtx_nrc_sample* samples_device, uint32_t count) {
-    if (!n) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!n || !n->ctx) return RTX_ERROR_INVALID_ARGUMENT;  // NULL, or its context was destroyed
     if (count > 0 && !samples_device) return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "NULL buffer");
     if (!aligned(samples_device, 16)) return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "samples must be 16-B aligned");
     for (uint64_t off = 0; off < count; off += n->st.batch_size) {
@@ -597,7 +616,8 @@ int rtx_nrc_cmd_train(rtx_nrc n, const rtx_nrc_sample* samples_device, uint32_t 
 
 int rtx_nrc_debug_gradient(rtx_nrc n, const rtx_nrc_sample* samples_device, uint32_t count, float* grad_device,
                            float* loss_device) {
-    if (!n || !grad_device || !loss_device) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!n || !n->ctx) return RTX_ERROR_INVALID_ARGUMENT;  // NULL, or its context was destroyed
+    if (!grad_device || !loss_device) return RTX_ERROR_INVALID_ARGUMENT;
     if (count == 0 || count > n->st.batch_size)
         return fail(n->ctx, RTX_ERROR_INVALID_ARGUMENT, "count must be in 1..batch_size");
     hipError_t e = rtx::nrc_step(n->st, samples_device, count, grad_device, loss_device, n->ctx->stream);
@@ -606,7 +626,8 @@ int rtx_nrc_debug_gradient(rtx_nrc n, const rtx_nrc_sample* samples_device, uint
 }
 
 int rtx_nrc_get_params(rtx_nrc n, float* params_host) {
-    if (!n || !params_host) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!n || !n->ctx) return RTX_ERROR_INVALID_ARGUMENT;  // NULL, or its context was destroyed
+    if (!params_host) return RTX_ERROR_INVALID_ARGUMENT;
     hipError_t e = hipStreamSynchronize(n->ctx->stream);
     if (!e) e = hipMemcpy(params_host, n->st.params, RTX_NRC_PARAM_COUNT * sizeof(float), hipMemcpyDeviceToHost);
     if (e) return hip_fail(n->ctx, e, "rtx_nrc_get_params");
@@ -614,7 +635,8 @@ int rtx_nrc_get_params(rtx_nrc n, float* params_host) {
 }
 
 int rtx_nrc_set_params(rtx_nrc n, const float* params_host) {
-    if (!n || !params_host) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!n || !n->ctx) return RTX_ERROR_INVALID_ARGUMENT;  // NULL, or its context was destroyed
+    if (!params_host) return RTX_ERROR_INVALID_ARGUMENT;
     hipError_t e = hipStreamSynchronize(n->ctx->stream);
     if (!e) e = hipMemcpy(n->st.params, params_host, RTX_NRC_PARAM_COUNT * sizeof(float), hipMemcpyHostToDevice);
     if (!e) e = rtx::nrc_pack(n->st, n->ctx->stream);
@@ -623,7 +645,7 @@ int rtx_nrc_set_params(rtx_nrc n, const float* params_host) {
 }
 
 int rtx_nrc_get_status(rtx_nrc n, uint32_t* steps, float* last_loss) {
-    if (!n) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!n || !n->ctx) return RTX_ERROR_INVALID_ARGUMENT;  // NULL, or its context was destroyed
     hipError_t e = hipStreamSynchronize(n->ctx->stream);
     float l = 0.0f;
     if (!e) e = hipMemcpy(&l, n->st.loss, sizeof(float), hipMemcpyDeviceToHost);
@@ -662,6 +684,9 @@ int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg) {
     if (cfg->ray_chunk) c.ray_chunk = cfg->ray_chunk;
     if (cfg->refill_lanes) c.refill_lanes = cfg->refill_lanes;
     if (cfg->refill_interval) c.refill_interval = cfg->refill_interval;
+    if (cfg->leaf_lanes) c.leaf_lanes = cfg->leaf_lanes;
+    if (c.leaf_lanes < 1 || c.leaf_lanes > 64)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "leaf_lanes must be in [1, 64]");
     if (c.ray_chunk != RTX_TRACE_RAY_CHUNK_AUTO && (c.ray_chunk < 64 || c.ray_chunk > 4096 || (c.ray_chunk & 63)))
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "ray_chunk must be a multiple of 64 up to 4096, or AUTO");
     if (c.refill_lanes < 1 || c.refill_lanes > 64)
@@ -687,24 +712,19 @@ int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg) {
 // Coordinate descent over the traversal knobs, each candidate timed with HIP events on the context stream; a
 // candidate replaces the incumbent only when it is faster by more than timing noise (1.5%).
 constexpr float kTuneGain = 0.985f;
-int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device, uint32_t ray_count,
-                       uint32_t mode, void* results_device, uint32_t reps, rtx_trace_config* out_cfg,
-                       float* out_ms_per_launch) {
-    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
-    if (ray_count == 0 || !rays_device || !results_device)
-        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "autotuning needs a non-empty ray buffer and a result buffer");
-    if (reps == 0 || reps > 64) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "reps must be in [1, 64]");
-    rtx_hit* hits = mode == RTX_QUERY_CLOSEST_HIT ? (rtx_hit*)results_device : nullptr;
-    uint32_t* occl = mode == RTX_QUERY_ANY_HIT ? (uint32_t*)results_device : nullptr;
+
+extern "C++" template <typename Launch>
+static int autotune_core(rtx_context ctx, const char* who, uint32_t reps, Launch launch, rtx_trace_config* out_cfg,
+                         float* out_ms) {
+    if (reps == 0 || reps > 64) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "%s: reps must be in [1, 64]", who);
     hipEvent_t ev[2];
     hipError_t e = hipEventCreate(&ev[0]);
     if (e == hipSuccess) {
         e = hipEventCreate(&ev[1]);
         if (e != hipSuccess) (void)hipEventDestroy(ev[0]);
     }
-    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_autotune_trace events");
+    if (e != hipSuccess) return hip_fail(ctx, e, who);
     const rtx_trace_config start = ctx->cfg;
-    int r = RTX_SUCCESS;
     // time one candidate; +inf when the configuration is rejected
     auto time_cfg = [&](const rtx_trace_config& c, float& ms) -> int {
         ms = INFINITY;
@@ -714,59 +734,62 @@ int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* ra
             ctx->last_error.clear();
             return RTX_SUCCESS;
         }
-        int rr = rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0xFFu, hits, occl);
+        int rr = launch();
         if (rr) return rr;
         hipError_t he = hipEventRecord(ev[0], ctx->stream);
-        for (uint32_t i = 0; he == hipSuccess && i < reps; ++i) {
-            rr = rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0xFFu, hits, occl);
-            if (rr) return rr;
-        }
+        for (uint32_t i = 0; he == hipSuccess && i < reps; ++i)
+            if ((rr = launch())) return rr;
         if (he == hipSuccess) he = hipEventRecord(ev[1], ctx->stream);
         if (he == hipSuccess) he = hipEventSynchronize(ev[1]);
         float el = 0.0f;
         if (he == hipSuccess) he = hipEventElapsedTime(&el, ev[0], ev[1]);
-        if (he != hipSuccess) return hip_fail(ctx, he, "rtx_autotune_trace timing");
+        if (he != hipSuccess) return hip_fail(ctx, he, who);
         ms = el / (float)reps;
         return rtx_sync(ctx);
     };
-    rtx_trace_config best = start;
-    float best_ms = INFINITY;
-    r = time_cfg(best, best_ms);
-    // knob groups, tried in order; every group keeps the best value found so far for the other knobs
     static const uint32_t shapes[][2] = {{256, 2}, {256, 3}, {256, 4}, {128, 6}, {128, 8}, {64, 12}, {64, 16}};
     static const uint32_t refills[][2] = {{40, 16}, {32, 8}, {40, 4}, {48, 4}, {48, 2}, {56, 2}, {56, 4}, {60, 1}};
+    static const uint32_t leafs[] = {1, 4, 8, 16, 24, 32, 48};
     static const uint32_t chunks[] = {RTX_TRACE_RAY_CHUNK_AUTO, 64, 128, 256, 512};
     static const uint32_t depths[] = {4, 6, 8, 12, 16};
+    rtx_trace_config best = start;
+    float best_ms = INFINITY;
+    int r = time_cfg(best, best_ms);
+    auto consider = [&](const rtx_trace_config& c) {
+        float ms;
+        if (r == RTX_SUCCESS && (r = time_cfg(c, ms)) == RTX_SUCCESS && ms < kTuneGain * best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    };
+    // knob groups in order; each keeps the best values found so far for the other knobs
     for (int pass = 0; pass < 2 && r == RTX_SUCCESS; ++pass) {
         for (auto& sh : shapes) {
             rtx_trace_config c = best;
             c.block_size = sh[0];
             c.blocks_per_cu = sh[1];
-            float ms;
-            if ((r = time_cfg(c, ms))) break;
-            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+            consider(c);
         }
-        for (size_t i = 0; r == RTX_SUCCESS && i < sizeof(refills) / sizeof(refills[0]); ++i) {
+        for (auto& rf : refills) {
             rtx_trace_config c = best;
-            c.refill_lanes = refills[i][0];
-            c.refill_interval = refills[i][1];
-            float ms;
-            if ((r = time_cfg(c, ms))) break;
-            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+            c.refill_lanes = rf[0];
+            c.refill_interval = rf[1];
+            consider(c);
         }
-        for (size_t i = 0; r == RTX_SUCCESS && i < sizeof(chunks) / sizeof(chunks[0]); ++i) {
+        for (uint32_t v : leafs) {
             rtx_trace_config c = best;
-            c.ray_chunk = chunks[i];
-            float ms;
-            if ((r = time_cfg(c, ms))) break;
-            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+            c.leaf_lanes = v;
+            consider(c);
         }
-        for (size_t i = 0; r == RTX_SUCCESS && i < sizeof(depths) / sizeof(depths[0]); ++i) {
+        for (uint32_t v : chunks) {
             rtx_trace_config c = best;
-            c.lds_stack_depth = depths[i];
-            float ms;
-            if ((r = time_cfg(c, ms))) break;
-            if (ms < kTuneGain * best_ms) best_ms = ms, best = c;
+            c.ray_chunk = v;
+            consider(c);
+        }
+        for (uint32_t v : depths) {
+            rtx_trace_config c = best;
+            c.lds_stack_depth = v;
+            consider(c);
         }
     }
     (void)hipEventDestroy(ev[0]);
@@ -777,8 +800,33 @@ int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* ra
     }
     ctx->cfg = best;
     if (out_cfg) *out_cfg = best;
-    if (out_ms_per_launch) *out_ms_per_launch = best_ms;
+    if (out_ms) *out_ms = best_ms;
     return RTX_SUCCESS;
+}
+
+int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device, uint32_t ray_count,
+                       uint32_t mode, void* results_device, uint32_t reps, rtx_trace_config* out_cfg,
+                       float* out_ms_per_launch) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (ray_count == 0 || !rays_device || !results_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "autotuning needs a non-empty ray buffer and a result buffer");
+    rtx_hit* hits = mode == RTX_QUERY_CLOSEST_HIT ? (rtx_hit*)results_device : nullptr;
+    uint32_t* occl = mode == RTX_QUERY_ANY_HIT ? (uint32_t*)results_device : nullptr;
+    return autotune_core(
+        ctx, "rtx_autotune_trace", reps,
+        [&]() { return rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0xFFu, hits, occl); },
+        out_cfg, out_ms_per_launch);
+}
+
+int rtx_autotune_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                        const rtx_render_params* params, float* film_rgba32f_device, uint32_t reps,
+                        rtx_trace_config* out_cfg, float* out_ms_per_launch) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!film_rgba32f_device) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "autotuning needs a film");
+    return autotune_core(
+        ctx, "rtx_autotune_render", reps,
+        [&]() { return rtx_cmd_render(ctx, tlas_address, sbt, camera, params, film_rgba32f_device, nullptr); },
+        out_cfg, out_ms_per_launch);
 }
 
 int rtx_debug_lbvh(rtx_context ctx, const float* boxes_device, uint32_t n, uint32_t morton_bits, void* codes_device,

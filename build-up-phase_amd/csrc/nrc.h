@@ -19,6 +19,7 @@ struct NrcState {
     uint32_t batch_size = 0;
     uint32_t steps = 0;
     uint32_t cache_vertex = 1;
+    uint32_t train_suffix = 0;  // training-path vertices past the cache vertex before the self-training query
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
     float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
 };

@@ -9,14 +9,17 @@
 
 namespace rtx {
 
-enum PtMode : uint32_t { kPtPlain = 0, kPtTrain = 1, kPtCache = 2 };
+enum PtMode : uint32_t { kPtPlain = 0, kPtCache = 2 };
 
 struct PtArgs {
     const void* tlas;
     uint32_t width, height, npix;
-    uint32_t npaths;       // path slots of this pass (npix, or the training path count)
+    uint32_t npaths;       // path slots of this pass: npix render paths + ntrain training paths
+    uint32_t ntrain;       // training paths (slots npix .. npix + ntrain - 1; cache mode)
     uint32_t mode;         // PtMode
-    uint32_t cache_depth;  // vertex at which cache-mode paths query the cache
+    uint32_t cache_depth;  // vertex at which cache-mode render paths query the cache
+    uint32_t train_end;    // last vertex of training paths
+    uint32_t tail;         // 1: training paths query the cache at train_end (self-training)
     uint32_t max_depth, rr_depth, seed;
     float cam[3];
     float ax, ay;
@@ -38,22 +41,25 @@ struct PtArgs {
     float4* thr;
     float4* rad;
     float4* accum;
-    uint32_t* counts;  // per depth d: [3d] extension rays, [3d+1] shadow rays, [3(d+1)+2] cache queries
+    uint32_t* counts;  // per depth d: [3d] extension rays, [3d+1] shadow rays; [2] cache queries of the pass
     float* cdf;
     rtx_nrc_query* cq;  // cache queries
     uint32_t* cq_pid;
     float* cq_pred;
-    float4* rec;        // training vertex records, [(d-1) * npaths + pid] x 5 float4
+    float4* rec;        // training vertex records, [(d-1) * ntrain + (pid - npix)] x 5 float4
 };
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights);
 void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights);
 hipError_t pt_render(PtArgs a, const TraceLaunch& launch, uint32_t spp, uint32_t sample_offset, float* film,
                      double* accum64, hipStream_t stream);
-uint64_t pt_train_bytes(uint32_t train_paths, uint32_t max_depth);
-// training pass + Adam steps (synchronises once to read the sample count), then spp cache-terminated passes
-hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& launch, NrcState& nrc, uint32_t train_paths, void* train_buf,
-                         uint32_t spp, uint32_t sample_offset, float* film, double* accum64, uint32_t* host_count,
-                         hipStream_t stream);
+uint64_t pt_train_bytes(uint32_t train_paths, uint32_t record_depths);
+// vertices per training path that become samples
+uint32_t nrc_record_depths(uint32_t max_depth, uint32_t cache_depth, uint32_t train_suffix);
+// one frame: spp cache-terminated passes, the first carrying train_paths training paths (self-training when
+// train_suffix > 0), then the Adam steps (synchronises once to read the sample count)
+hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& launch, NrcState& nrc, uint32_t train_paths,
+                         uint32_t train_suffix, void* train_buf, uint32_t spp, uint32_t sample_offset, float* film,
+                         double* accum64, uint32_t* host_count, hipStream_t stream);
 
 }  // namespace rtx

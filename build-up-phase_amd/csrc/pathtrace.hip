@@ -10,11 +10,15 @@
 //   gather  unoccluded shadow contributions are added to their path's radiance
 // Every path owns its slot in a pass, so no accumulation needs atomics and results do not depend on queue order.
 //
-// Cache mode: render paths stop at vertex `cache_depth`, emit a cache query there, and add throughput x the
-// cache's prediction.  Training mode: paths start at random pixels and record, at every vertex d >= 1, the cache
-// query, the throughput and the radiance gathered so far; after the pass the record becomes a training sample
-// whose target is (final radiance - radiance before the vertex) / throughput, i.e. the radiance the path carried
-// from that vertex toward the previous one.  Samples are compacted with a deterministic block scan.
+// Cache mode (one pass per frame, Muller et al. 2021): path slots [0, npix) are render paths that stop at vertex
+// `cache_depth` and query the cache there; slots [npix, npix + ntrain) are training paths from random pixels that
+// share the same queues, record at every vertex d >= 1 the cache query, the throughput and the radiance gathered
+// so far, and either run to full length or, with a training suffix, stop at vertex train_end and query the cache
+// themselves (self-training).  All queries of the pass are inferred in one launch; every path adds throughput x
+// prediction; then each record becomes a training sample whose target is (final radiance - radiance before the
+// vertex) / throughput, i.e. the radiance the path carried from that vertex toward the previous one.  Samples are
+// compacted with a deterministic block scan.
+#include <algorithm>
 #include <cmath>
 
 #include "nrc.h"
@@ -59,6 +63,9 @@ __device__ __forceinline__ void block_push3(uint32_t* c0, uint32_t* c1, uint32_t
     }
 }
 
+// counts[] slot of the training-tail query queue (past the per-depth extension / shadow counts)
+__host__ __device__ __forceinline__ uint32_t tail_counter(const PtArgs& a) { return 3u * a.max_depth + 2u; }
+
 __global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, float* __restrict__ cdf) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     float acc = 0.0f;
@@ -74,8 +81,8 @@ __global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, f
 }
 
 __device__ __forceinline__ void path_xy(const PtArgs& a, uint32_t pid, uint32_t& x, uint32_t& y) {
-    if (a.mode == kPtTrain) {
-        x = pid;
+    if (pid >= a.npix) {  // training path
+        x = pid - a.npix;
         y = 0x7FFFFFFFu;
     } else {
         x = pid % a.width;
@@ -89,8 +96,8 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
     if (i >= a.npaths) return;
     uint32_t px, py;
     uint4 j;
-    if (a.mode == kPtTrain) {
-        const uint4 h = pcg4d(i, sample, a.seed, kTrainKey);
+    if (i >= a.npix) {  // training path: a random pixel
+        const uint4 h = pcg4d(i - a.npix, sample, a.seed, kTrainKey);
         const uint32_t pix = h.x % a.npix;
         px = pix % a.width;
         py = pix / a.width;
@@ -125,7 +132,6 @@ template <int MODE>
 __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, uint32_t sample, int cur) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = a.counts[3 * depth];
-    const bool cache_vertex = MODE == kPtCache && depth == a.cache_depth;
     bool push_ext = false, push_sh = false, push_cq = false;
     rtx_ray ext{}, sh{};
     rtx_nrc_query cq{};
@@ -134,6 +140,11 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
     uint32_t pid = 0;
     if (q < count) {
         pid = a.ext_pid[cur][q];
+        const bool train = MODE == kPtCache && pid >= a.npix;
+        // the vertex where this path queries the cache instead of scattering
+        const bool query_here = MODE == kPtCache && ((!train && depth == a.cache_depth) ||
+                                                     (train && a.tail && depth == a.train_end));
+        const bool record = train && depth >= 1 && !query_here;
         const rtx_ray r = a.ext_rays[cur][q];
         const PathHit h = a.hits[q];
         if (h.rec != 0xFFFFFFFFu && h.rec < a.nrec) {
@@ -159,7 +170,7 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
 #pragma unroll
             for (int k = 0; k < 3; ++k) org[k] = fmaf(a.ray_offset, nf[k], P[k]);
             const bool reflective = alb[0] > 0.0f || alb[1] > 0.0f || alb[2] > 0.0f;
-            if (MODE != kPtPlain && (cache_vertex || (MODE == kPtTrain && depth >= 1))) {
+            if (query_here || record) {
                 const float il = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -169,17 +180,17 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
                     cq.albedo[k] = alb[k];
                 }
             }
-            if (MODE == kPtTrain && depth >= 1) {
+            if (record) {
                 // record the vertex before its own next-event contribution is gathered
                 const float4 rd = a.rad[pid];
-                float4* rp = a.rec + ((uint64_t)(depth - 1) * a.npaths + pid) * 5;
+                float4* rp = a.rec + ((uint64_t)(depth - 1) * a.ntrain + (pid - a.npix)) * 5;
                 rp[0] = make_float4(cq.position[0], cq.position[1], cq.position[2], cq.normal[0]);
                 rp[1] = make_float4(cq.normal[1], cq.normal[2], cq.direction[0], cq.direction[1]);
                 rp[2] = make_float4(cq.direction[2], cq.albedo[0], cq.albedo[1], cq.albedo[2]);
                 rp[3] = make_float4(thr.x, thr.y, thr.z, rd.x);
                 rp[4] = make_float4(rd.y, rd.z, 1.0f, 0.0f);
             }
-            if (cache_vertex) {
+            if (query_here) {
                 push_cq = reflective;
             } else {
                 uint32_t x, y;
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
                         }
                     }
                 }
-                if (reflective && depth + 1 < a.max_depth) {
+                if (reflective && depth + 1 < a.max_depth && (!train || !a.tail || depth < a.train_end)) {
                     thr_new = make_float4(thr.x * alb[0], thr.y * alb[1], thr.z * alb[2], 0.0f);
                     bool alive = true;
                     if (depth >= a.rr_depth) {
@@ -263,8 +274,12 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
     }
     const bool pr[3] = {push_sh, push_ext, push_cq};
     uint32_t o[3], tot[3];
+    // render paths queue their cache queries at cache_depth (slots from 0), training tails at train_end (slots from
+    // npix); the two never share a depth
+    const bool tail_depth = MODE == kPtCache && depth != a.cache_depth;
     block_push3(&a.counts[3 * depth + 1], &a.counts[3 * (depth + 1)],
-                MODE == kPtCache ? &a.counts[3 * (depth + 1) + 2] : nullptr, pr, o, tot);
+                MODE == kPtCache ? &a.counts[tail_depth ? tail_counter(a) : 2] : nullptr, pr, o, tot);
+    if (tail_depth && push_cq) o[2] += a.npix;
     if (push_sh) {
         a.sh_rays[o[0]] = sh;
         a.sh_pid[o[0]] = pid;
@@ -295,9 +310,10 @@ __global__ __launch_bounds__(256) void shadow_gather_kernel(PtArgs a, uint32_t d
     a.rad[pid] = r;
 }
 
-__global__ __launch_bounds__(256) void cache_gather_kernel(PtArgs a, uint32_t depth) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.counts[3 * (depth + 1) + 2]) return;
+// Adds throughput x prediction for the queries [base, base + *count) of the query buffer.
+__global__ __launch_bounds__(256) void cache_gather_kernel(PtArgs a, const uint32_t* count, uint32_t base) {
+    const uint32_t q = base + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= base + *count) return;
     const uint32_t pid = a.cq_pid[q];
     const float4 t = a.thr[pid];
     float4 r = a.rad[pid];
@@ -389,7 +405,7 @@ __global__ __launch_bounds__(256) void sample_scatter_kernel(PtArgs a, uint64_t 
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     if (!v) return;
     off += (uint32_t)__popcll(m & lt);
-    const uint32_t pid = (uint32_t)(r % a.npaths);
+    const uint32_t pid = a.npix + (uint32_t)(r % a.ntrain);
     const float4* rp = a.rec + r * 5;
     const float4 q0 = rp[0], q1 = rp[1], q2 = rp[2], q3 = rp[3], q4 = rp[4];
     const float4 fin = a.rad[pid];
@@ -458,15 +474,14 @@ void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint
     a.cq_pred = (float*)take(n * 12);
 }
 
-uint64_t pt_train_bytes(uint32_t train_paths, uint32_t max_depth) {
-    const uint64_t nrec = (uint64_t)train_paths * (max_depth > 1 ? max_depth - 1 : 1);
+uint64_t pt_train_bytes(uint32_t train_paths, uint32_t record_depths) {
+    const uint64_t nrec = (uint64_t)train_paths * (record_depths ? record_depths : 1);
     const uint64_t nblk = (nrec + 255) / 256;
     return align_up(nrec * 80, 256) + align_up(nrec * sizeof(rtx_nrc_sample), 256) + align_up(nblk * 4, 256) + 256;
 }
 
-// One pass: vertices 0 .. last_depth.  In cache mode the cache is queried at last_depth == a.cache_depth.
-static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uint32_t last_depth,
-                           const NrcState* nrc, hipStream_t s) {
+// One pass: vertices 0 .. last_depth (cache queries stay queued for the caller).
+static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uint32_t last_depth, hipStream_t s) {
     const unsigned gb = (a.npaths + 255) / 256;
     const uint64_t ncounts = 3ull * (a.max_depth + 2);
     fill_words_kernel<<<1, 64, 0, s>>>(a.counts, ncounts, 0u);
@@ -487,18 +502,10 @@ static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uin
         ext.state = a.state;
         ext.gstack = a.gstack;
         if ((e = trace_rays_launch(ext, L, s))) return e;
-        const bool cache_here = a.mode == kPtCache && d == a.cache_depth;
-        if (a.mode == kPtTrain)
-            shade_kernel<kPtTrain><<<gb, 256, 0, s>>>(a, d, sample, cur);
-        else if (a.mode == kPtCache)
+        if (a.mode == kPtCache)
             shade_kernel<kPtCache><<<gb, 256, 0, s>>>(a, d, sample, cur);
         else
             shade_kernel<kPtPlain><<<gb, 256, 0, s>>>(a, d, sample, cur);
-        if (cache_here) {
-            if ((e = nrc_infer(*nrc, a.cq, a.npaths, a.cq_pred, s, a.counts + 3 * (d + 1) + 2))) return e;
-            cache_gather_kernel<<<gb, 256, 0, s>>>(a, d);
-            break;
-        }
         TraceRequest sh{};
         sh.tlas = a.tlas;
         sh.mode = 1;
@@ -515,48 +522,72 @@ static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uin
     return hipGetLastError();
 }
 
+// Cache predictions for one query queue, added to the paths' radiance.
+static hipError_t infer_gather(PtArgs& a, const NrcState& nrc, const uint32_t* count, uint32_t base, uint32_t cap,
+                               hipStream_t s) {
+    hipError_t e = nrc_infer(nrc, a.cq + base, cap, a.cq_pred + 3ull * base, s, count);
+    if (e) return e;
+    cache_gather_kernel<<<(cap + 255) / 256, 256, 0, s>>>(a, count, base);
+    return hipGetLastError();
+}
+
 hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t sample_offset, float* film,
                      double* acc64, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
     a.mode = kPtPlain;
     a.npaths = a.npix;
+    a.ntrain = 0;
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
     fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
     hipError_t e;
     for (uint32_t si = 0; si < spp; ++si) {
-        if ((e = run_pass(a, L, sample_offset + si, a.max_depth - 1, nullptr, s))) return e;
+        if ((e = run_pass(a, L, sample_offset + si, a.max_depth - 1, s))) return e;
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
     if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, film);
     return hipGetLastError();
 }
 
-hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, NrcState& nrc, uint32_t train_paths, void* train_buf,
-                         uint32_t spp, uint32_t sample_offset, float* film, double* acc64, uint32_t* host_count,
-                         hipStream_t s) {
+hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, NrcState& nrc, uint32_t train_paths, uint32_t train_suffix,
+                         void* train_buf, uint32_t spp, uint32_t sample_offset, float* film, double* acc64,
+                         uint32_t* host_count, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
     hipError_t e;
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
-    if (train_paths > 0) {
-        // training paths: full length, vertex records
-        PtArgs t = a;
-        t.mode = kPtTrain;
-        t.npaths = train_paths;
-        const uint64_t nrec = (uint64_t)train_paths * (a.max_depth - 1);
+    fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
+    a.mode = kPtCache;
+    a.tail = train_suffix > 0 ? 1u : 0u;
+    a.train_end = a.tail ? std::min(a.max_depth - 1, a.cache_depth + train_suffix) : a.max_depth - 1;
+    const uint32_t rec_depths = nrc_record_depths(a.max_depth, a.cache_depth, train_suffix);
+    for (uint32_t si = 0; si < spp; ++si) {
+        // the first sample carries the frame's training paths
+        const uint32_t ntrain = si == 0 ? train_paths : 0u;
+        a.ntrain = ntrain;
+        a.npaths = a.npix + ntrain;
+        const uint64_t nrec = (uint64_t)ntrain * rec_depths;
         const uint64_t nblk = (nrec + 255) / 256;
         uint8_t* p = (uint8_t*)train_buf;
-        t.rec = (float4*)p;
+        a.rec = (float4*)p;
         p += align_up(nrec * 80, 256);
         rtx_nrc_sample* samples = (rtx_nrc_sample*)p;
         p += align_up(nrec * sizeof(rtx_nrc_sample), 256);
         uint32_t* block_counts = (uint32_t*)p;
         p += align_up(nblk * 4, 256);
         uint32_t* total = (uint32_t*)p;
-        fill_words_kernel<<<1024, 256, 0, s>>>((uint32_t*)t.rec, nrec * 20, 0u);
-        if ((e = run_pass(t, L, sample_offset, a.max_depth - 1, nullptr, s))) return e;
-        sample_count_kernel<<<(unsigned)nblk, 256, 0, s>>>(t, nrec, block_counts);
+        if (ntrain) fill_words_kernel<<<1024, 256, 0, s>>>((uint32_t*)a.rec, nrec * 20, 0u);
+        const uint32_t last = ntrain ? std::max(a.cache_depth, a.train_end) : a.cache_depth;
+        if ((e = run_pass(a, L, sample_offset + si, last, s))) return e;
+        if (ntrain == 0) {
+            if ((e = infer_gather(a, nrc, a.counts + 2, 0, a.npix, s))) return e;
+            accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
+            continue;
+        }
+        // training tails first (self-training), then the samples and the Adam steps, then the render queries with
+        // the updated cache
+        if (a.tail && (e = infer_gather(a, nrc, a.counts + tail_counter(a), a.npix, ntrain, s))) return e;
+        sample_count_kernel<<<(unsigned)nblk, 256, 0, s>>>(a, nrec, block_counts);
         sample_scan_kernel<<<1, 1024, 0, s>>>(block_counts, (uint32_t)nblk, total);
-        sample_scatter_kernel<<<(unsigned)nblk, 256, 0, s>>>(t, nrec, block_counts, samples);
+        sample_scatter_kernel<<<(unsigned)nblk, 256, 0, s>>>(a, nrec, block_counts, samples);
         if ((e = hipMemcpyAsync(host_count, total, 4, hipMemcpyDeviceToHost, s))) return e;
         if ((e = hipStreamSynchronize(s))) return e;
         const uint32_t count = *host_count;
@@ -568,17 +599,20 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, NrcState& nrc, uint32_t
                 if ((e = nrc_step(nrc, samples, nbatch, nullptr, nullptr, s, nb, b))) return e;
             }
         }
-    }
-    // render paths terminated into the cache
-    a.mode = kPtCache;
-    a.npaths = a.npix;
-    fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
-    for (uint32_t si = 0; si < spp; ++si) {
-        if ((e = run_pass(a, L, sample_offset + si, a.cache_depth, &nrc, s))) return e;
+        if ((e = infer_gather(a, nrc, a.counts + 2, 0, a.npix, s))) return e;
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
     if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, film);
     return hipGetLastError();
+}
+
+uint32_t nrc_record_depths(uint32_t max_depth, uint32_t cache_depth, uint32_t train_suffix) {
+    // records at vertices 1 .. train_end - 1 (suffix) or 1 .. max_depth - 1 (full length)
+    if (train_suffix > 0) {
+        const uint32_t end = std::min(max_depth - 1, cache_depth + train_suffix);
+        return end > 1 ? end - 1 : 1;
+    }
+    return max_depth > 1 ? max_depth - 1 : 1;
 }
 
 }  // namespace rtx

@@ -198,6 +198,7 @@ struct TraceArgs {
     uint32_t ray_chunk;        // ray ids per queue fetch (< 64: automatic)
     uint32_t refill_lanes;     // refill a wave once fewer lanes than this are busy
     uint32_t refill_interval;  // steps between refill checks
+    uint32_t leaf_lanes;       // pending leaf groups needed before a leaf step runs
 };
 
 enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3 };
@@ -420,8 +421,15 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                     done = true;
                 }
             }
+            // Postponed leaves: the leaf step (instance entry in FP64, or triangle tests) runs only once enough lanes
+            // of the wave hold a pending leaf group, or when no lane can traverse; until then lanes with a pending
+            // leaf keep traversing their node group, or wait.
+            const bool has_leaf = !done && (lg_bits & 255u) != 0u;
+            const uint64_t leaf_lanes = __ballot(has_leaf);
+            const uint64_t node_lanes = __ballot(!done && (ng_bits & 255u) != 0u);
+            const bool leaf_phase = (uint32_t)__popcll(leaf_lanes) >= args.leaf_lanes || node_lanes == 0ull;
             if (done) {
-            } else if (lg_bits & 255u) {
+            } else if (has_leaf && leaf_phase) {
                 if (!in_blas) {
                     // instance leaf: enter its BLAS
                     const uint32_t m = lg_bits & 255u;
@@ -572,8 +580,22 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                     const uint32_t vmask = (w0.z >> 16) & 255u;
                     ng_base = w0.w;
                     ng_bits = oct_permute(hm & imask, s.oct) | (vmask << 8);
-                    lg_base = w0.w;
-                    lg_bits = (hm & ~imask) | (vmask << 8);
+                    const uint32_t lg_new = (hm & ~imask) | (vmask << 8);
+                    if (!(lg_bits & 255u)) {
+                        lg_base = w0.w;
+                        lg_bits = lg_new;
+                    } else if (lg_new & 255u) {
+                        // a leaf group is already pending: stack the new one
+                        const uint2 e =
+                            make_uint2(w0.w, (lg_new & 0xFFFFu) | kEntryLeaf | (in_blas ? 0u : kEntryTlas));
+                        if (sp < D) lds_stack[sp * blockDim.x + threadIdx.x] = e;
+                        else if (sp < D + GD) args.gstack[gtid * GD + (sp - D)] = e;
+                        ++sp;
+                        if (sp > D + GD) {
+                            atomicOr(&args.state->error_flags, kErrStackOverflow);
+                            done = true;
+                        }
+                    }
                 }
             }
 
@@ -737,6 +759,7 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.ray_chunk = L.ray_chunk;
     a.refill_lanes = L.refill_lanes;
     a.refill_interval = L.refill_interval;
+    a.leaf_lanes = L.leaf_lanes;
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
     if (r.n == 0 && !r.n_ptr) return hipGetLastError();
     switch (r.mode) {

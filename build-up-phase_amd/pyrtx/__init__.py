@@ -102,7 +102,7 @@ class NrcConfig(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint32), ("batch_size", ctypes.c_uint32), ("learning_rate", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
                 ("scene_min", ctypes.c_float * 3), ("scene_max", ctypes.c_float * 3),
-                ("cache_vertex", ctypes.c_uint32)]
+                ("cache_vertex", ctypes.c_uint32), ("train_suffix", ctypes.c_uint32)]
 
 
 NRC_PARAM_COUNT = 23888
@@ -111,7 +111,8 @@ NRC_PARAM_COUNT = 23888
 class TraceConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("lds_stack_depth", ctypes.c_uint32), ("ray_chunk", ctypes.c_uint32),
-                ("refill_lanes", ctypes.c_uint32), ("refill_interval", ctypes.c_uint32)]
+                ("refill_lanes", ctypes.c_uint32), ("refill_interval", ctypes.c_uint32),
+                ("leaf_lanes", ctypes.c_uint32)]
 
 
 RTX_TRACE_RAY_CHUNK_AUTO = 1
@@ -156,6 +157,8 @@ SIGNATURES = {
     "rtx_debug_sort": (ctypes.c_int, [_P, _P, _P, _U32, _U32]),
     "rtx_autotune_trace": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _U32, _U32, _P, _U32, ctypes.POINTER(TraceConfig),
                                           ctypes.POINTER(ctypes.c_float)]),
+    "rtx_autotune_render": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, _P, _P, _U32, ctypes.POINTER(TraceConfig),
+                                           ctypes.POINTER(ctypes.c_float)]),
     "rtx_debug_treelet": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
 }
 
@@ -342,11 +345,8 @@ class Context:
                                                           cull_mask, None, out.data_ptr()))
         return out
 
-    def render(self, tlas: AccelerationStructure, records, camera, lights, width, height, spp=1, max_depth=8,
-               rr_depth=3, seed=1234, sample_offset=0, ray_offset=1e-4, film=None, accum64=None):
-        """Wavefront path tracer (rtx_cmd_render).  lights: numpy float32 [L, 12] (v0 v1 v2 emission) or a device
-        tensor.  Returns the film [H, W, 4] (mean radiance of these samples)."""
-        torch = _torch()
+    def _render_args(self, records, camera, lights, width, height, spp, max_depth, rr_depth, seed, sample_offset,
+                     ray_offset):
         if isinstance(records, np.ndarray):
             rec = _dev_tensor(np.ascontiguousarray(records).view(np.uint8).reshape(-1), self.device)
         else:
@@ -356,18 +356,42 @@ class Context:
         else:
             lt = lights
         nl = lt.shape[0] if lt is not None else 0
-        if film is None:
-            film = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
         nrec = rec.numel() // 48
         sbt = Sbt(rec.data_ptr() if nrec else None, nrec, 0, 1, 0, (ctypes.c_float * 3)(0, 0, 0), 0.0)
         cam = Camera((ctypes.c_float * 3)(*camera[:3]), float(camera[3]))
         p = RenderParams(width, height, spp, max_depth, rr_depth, seed, sample_offset, 0,
                          lt.data_ptr() if nl else None, nl, ray_offset)
+        self._keep = (rec, lt)
+        return sbt, cam, p
+
+    def render(self, tlas: AccelerationStructure, records, camera, lights, width, height, spp=1, max_depth=8,
+               rr_depth=3, seed=1234, sample_offset=0, ray_offset=1e-4, film=None, accum64=None):
+        """Wavefront path tracer (rtx_cmd_render).  lights: numpy float32 [L, 12] (v0 v1 v2 emission) or a device
+        tensor.  Returns the film [H, W, 4] (mean radiance of these samples)."""
+        torch = _torch()
+        sbt, cam, p = self._render_args(records, camera, lights, width, height, spp, max_depth, rr_depth, seed,
+                                        sample_offset, ray_offset)
+        if film is None:
+            film = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
         self._check(self.lib.rtx_cmd_render(self.handle, tlas.address, ctypes.byref(sbt), ctypes.byref(cam),
                                             ctypes.byref(p), film.data_ptr(),
                                             accum64.data_ptr() if accum64 is not None else None))
-        self._keep = (rec, lt)
         return film
+
+    def autotune_render(self, tlas: AccelerationStructure, records, camera, lights, width, height, spp=1,
+                        max_depth=8, rr_depth=3, seed=1234, ray_offset=1e-4, reps=2):
+        """rtx_autotune_render: tune the traversal configuration on spp-sample renders of this workload; the best
+        configuration stays active.  Returns (config dict, ms per render)."""
+        torch = _torch()
+        sbt, cam, p = self._render_args(records, camera, lights, width, height, spp, max_depth, rr_depth, seed,
+                                        1 << 28, ray_offset)
+        film = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
+        c = TraceConfig()
+        ms = ctypes.c_float()
+        self._check(self.lib.rtx_autotune_render(self.handle, tlas.address, ctypes.byref(sbt), ctypes.byref(cam),
+                                                 ctypes.byref(p), film.data_ptr(), reps, ctypes.byref(c),
+                                                 ctypes.byref(ms)))
+        return self.config(), ms.value
 
     def render_nrc(self, tlas: AccelerationStructure, records, camera, lights, nrc: "Nrc", width, height, spp=1,
                    train_paths=65536, max_depth=8, rr_depth=3, seed=1234, sample_offset=0, ray_offset=1e-4,
@@ -406,15 +430,17 @@ class Context:
                 "instance_entries": s.instance_entries}
 
     def set_config(self, block_size=0, blocks_per_cu=0, lds_stack_depth=0, ray_chunk=0, refill_lanes=0,
-                   refill_interval=0):
-        c = TraceConfig(block_size, blocks_per_cu, lds_stack_depth, ray_chunk, refill_lanes, refill_interval)
+                   refill_interval=0, leaf_lanes=0):
+        c = TraceConfig(block_size, blocks_per_cu, lds_stack_depth, ray_chunk, refill_lanes, refill_interval,
+                        leaf_lanes)
         self._check(self.lib.rtx_set_trace_config(self.handle, ctypes.byref(c)))
 
     def config(self) -> dict:
         c = TraceConfig()
         self._check(self.lib.rtx_get_trace_config(self.handle, ctypes.byref(c)))
         return {"block_size": c.block_size, "blocks_per_cu": c.blocks_per_cu, "lds_stack_depth": c.lds_stack_depth,
-                "ray_chunk": c.ray_chunk, "refill_lanes": c.refill_lanes, "refill_interval": c.refill_interval}
+                "ray_chunk": c.ray_chunk, "refill_lanes": c.refill_lanes, "refill_interval": c.refill_interval,
+                "leaf_lanes": c.leaf_lanes}
 
     def autotune(self, tlas: AccelerationStructure, rays, mode=RTX_QUERY_CLOSEST_HIT, reps=2):
         """rtx_autotune_trace on a device ray tensor [N, 8] float32; the best configuration stays active.
@@ -491,10 +517,10 @@ class Nrc:
     """Neural radiance cache bound to a Context (rtx_nrc_* functions)."""
 
     def __init__(self, ctx: "Context", scene_min, scene_max, seed=1, batch_size=16384, lr=1e-3, beta1=0.9,
-                 beta2=0.99, eps=1e-8, cache_vertex=1):
+                 beta2=0.99, eps=1e-8, cache_vertex=1, train_suffix=0):
         self.ctx = ctx
         cfg = NrcConfig(seed, batch_size, lr, beta1, beta2, eps, (ctypes.c_float * 3)(*scene_min),
-                        (ctypes.c_float * 3)(*scene_max), cache_vertex)
+                        (ctypes.c_float * 3)(*scene_max), cache_vertex, train_suffix)
         h = ctypes.c_void_p()
         ctx._check(ctx.lib.rtx_nrc_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

@@ -263,6 +263,8 @@ typedef struct rtx_trace_config {
                                  4096, or RTX_TRACE_RAY_CHUNK_AUTO (64, or 256 when every wave gets >= 8 chunks) */
     uint32_t refill_lanes;    /* a wave fetches new rays once fewer than this many of its 64 lanes are busy */
     uint32_t refill_interval; /* traversal steps between two refill checks (1..64) */
+    uint32_t leaf_lanes;      /* postponed leaves: a wave runs its leaf step (BLAS entry, triangle tests) once this
+                                 many lanes hold a pending leaf group or no lane can traverse (1..64; 1 = at once) */
 } rtx_trace_config;
 #define RTX_TRACE_RAY_CHUNK_AUTO 1u
 int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg);
@@ -270,7 +272,8 @@ int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg);
 
 /* Project autotuner for the traversal kernel (no counterpart in the reference, whose traversal is the driver's).
  * Times candidate configurations on the caller's ray buffer with HIP events (coordinate descent, two passes:
- * block size x blocks per CU, refill policy, ray chunk, LDS stack depth; `reps` timed launches each; a change
+ * block size x blocks per CU, refill policy, postponed-leaf threshold, ray chunk, LDS stack depth; `reps` timed
+ * launches each; a change
  * must gain more than 1.5%), keeps the fastest on the context and returns it with its time per launch.  Tune on
  * the workload itself: a few milliseconds of rays per launch at least.  results_device receives the query results
  * (rtx_hit[] for closest hit, uint32[] for any hit).  Blocks until done. */
@@ -309,6 +312,12 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
                    const rtx_render_params* params, float* film_rgba32f_device, double* accum_rgb_device);
 
 /* Rays traced by rtx_cmd_render since the context was created (camera, extension and shadow rays). */
+/* rtx_autotune_trace for the path tracer: times `reps` rtx_cmd_render calls with `params` per candidate
+ * configuration (film written, no FP64 accumulation), keeps the fastest.  Blocks until done. */
+int rtx_autotune_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
+                        const rtx_render_params* params, float* film_rgba32f_device, uint32_t reps,
+                        rtx_trace_config* out_cfg, float* out_ms_per_launch);
+
 int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays);
 
 /* ---- neural radiance cache (Muller et al. 2021) --------------------------------------------------------------
@@ -331,6 +340,9 @@ typedef struct rtx_nrc_config {
     float scene_max[3];
     uint32_t cache_vertex;    /* rtx_cmd_render_nrc: path vertex (camera hit = 0) replaced by the cache query;
                                  0 selects 1, i.e. paths scatter at the primary hit and end at their 2nd bounce */
+    uint32_t train_suffix;    /* training paths: 0 = full length (unbiased targets); k > 0 = self-training: they
+                                 stop at vertex cache_vertex + k and take the cache's prediction there as the rest
+                                 of their radiance */
 } rtx_nrc_config;
 
 /* 48 B: a path vertex as the cache sees it; direction points from the vertex toward the previous vertex */
@@ -348,6 +360,8 @@ typedef struct rtx_nrc_sample {
     float reserved;
 } rtx_nrc_sample;
 
+/* A cache belongs to the context it was created on.  Destroying that context first leaves the cache usable only
+ * for rtx_nrc_destroy (every other call then returns RTX_ERROR_INVALID_ARGUMENT). */
 int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc);
 int rtx_nrc_destroy(rtx_nrc nrc);
 /* radiance predictions (3 floats per query, clamped at 0) */
@@ -360,11 +374,13 @@ int rtx_nrc_set_params(rtx_nrc nrc, const float* params_host);
 /* Adam steps taken so far and the mean loss of the last step */
 int rtx_nrc_get_status(rtx_nrc nrc, uint32_t* steps, float* last_loss);
 
-/* One frame of the cache-accelerated integrator: train_paths full-length training paths from random pixels
- * (sample index params->sample_offset) give one training sample per vertex after the first (target: the path's
- * radiance from that vertex on, from next-event estimation); one Adam step per batch_size samples; then every
- * pixel traces params->spp paths that stop at vertex cfg->cache_vertex (instead of scattering there) and add
- * throughput x the cache's prediction.  film / accum as in rtx_cmd_render. */
+/* One frame of the cache-accelerated integrator.  Every pixel traces params->spp paths that stop at vertex
+ * cfg->cache_vertex (instead of scattering there) and add throughput x the cache's prediction.  The first of these
+ * passes also carries train_paths training paths from random pixels (sample index params->sample_offset) through
+ * the same queues: each gives one training sample per vertex after the first (target: the path's radiance from
+ * that vertex on), either at full length or, with cfg->train_suffix > 0, ending in a cache query of their own
+ * (self-training).  The predictions come from the cache before this frame's update; then one Adam step per
+ * batch_size samples.  film / accum as in rtx_cmd_render. */
 int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
                        const rtx_render_params* params, rtx_nrc nrc, uint32_t train_paths,
                        float* film_rgba32f_device, double* accum_rgb_device);

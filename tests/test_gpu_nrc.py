@@ -123,3 +123,82 @@ def test_training_learns_a_smooth_field(ctx):
     assert steps == 240
     assert l1 < 0.05 * l0
     nrc.close()
+
+
+def _cornell_small(ctx, W, H):
+    from pyrtx import scenes as S
+    sc, lights = S.cornell_scene(5, width=W, height=H)
+    blases, tlas = ctx.upload_scene(sc)
+    rec = torch.from_numpy(np.ascontiguousarray(sc.records).view(np.uint8).reshape(-1).copy()).cuda()
+    lt = torch.from_numpy(np.ascontiguousarray(lights, np.float32).reshape(-1, 12).copy()).cuda()
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+    for k in range(16):
+        ctx.render(tlas, rec, sc.camera, lt, W, H, spp=64, sample_offset=(1 << 20) + 64 * k, accum64=acc)
+    ref = (acc / 1024).cpu().numpy()
+    return sc, (blases, tlas), rec, lt, ref
+
+
+def _relmse(x, ref):
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def test_render_nrc_beats_plain_at_equal_samples(ctx):
+    """The trained cache is what the NRC renderer gains: at the same number of samples per pixel its image is
+    closer to the converged reference than plain path tracing, and its bias (mean radiance) stays small."""
+    import pyrtx
+    _say("render_nrc vs plain")
+    W, H, frames = 320, 180, 48
+    sc, (blases, tlas), rec, lt, ref = _cornell_small(ctx, W, H)
+    plain = ctx.render(tlas, rec, sc.camera, lt, W, H, spp=frames)[..., :3].cpu().numpy().astype(np.float64)
+    nrc = pyrtx.Nrc(ctx, (0, 0, 0), (1, 1, 1), seed=1, batch_size=4096, cache_vertex=1)
+    try:
+        accn = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+        for fr in range(frames):
+            ctx.render_nrc(tlas, rec, sc.camera, lt, nrc, W, H, spp=1, train_paths=16384, sample_offset=fr,
+                           accum64=accn)
+        ctx.sync()
+        img = (accn / frames).cpu().numpy()
+        steps, loss = nrc.status()
+    finally:
+        nrc.close()
+    e_plain, e_nrc = _relmse(plain, ref), _relmse(img, ref)
+    bias = img.mean() / ref.mean() - 1.0
+    _say(f"  relMSE plain {e_plain:.5f}, nrc {e_nrc:.5f}; mean bias {bias:+.4f}; {steps} Adam steps, loss {loss:.4f}")
+    assert steps >= frames
+    assert np.isfinite(img).all()
+    assert e_nrc < 0.7 * e_plain
+    assert abs(bias) < 0.05
+
+
+def test_self_training_converges(ctx):
+    """Training paths that end in a cache query of their own (train_suffix > 0): the frame images approach the
+    reference as the cache trains (the early, untrained frames are biased)."""
+    import pyrtx
+    _say("render_nrc self-training")
+    W, H = 320, 180
+    sc, (blases, tlas), rec, lt, ref = _cornell_small(ctx, W, H)
+    nrc = pyrtx.Nrc(ctx, (0, 0, 0), (1, 1, 1), seed=1, batch_size=4096, cache_vertex=1, train_suffix=3)
+    errs = []
+    try:
+        for block in range(4):
+            accn = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+            for fr in range(24):
+                ctx.render_nrc(tlas, rec, sc.camera, lt, nrc, W, H, spp=1, train_paths=16384,
+                               sample_offset=24 * block + fr, accum64=accn)
+            errs.append(_relmse((accn / 24).cpu().numpy(), ref))
+    finally:
+        nrc.close()
+    _say(f"  relMSE of 24-frame blocks: {[round(e, 5) for e in errs]}")
+    assert all(np.isfinite(errs))
+    assert errs[-1] < 0.5 * errs[0]
+
+
+def test_cache_outliving_its_context(ctx):
+    """Destroying a context before its cache leaves the cache destroyable (and every other call an error)."""
+    import pyrtx
+    c2 = pyrtx.Context(0)
+    n2 = pyrtx.Nrc(c2, (0, 0, 0), (1, 1, 1), seed=2, batch_size=64)
+    c2.close()
+    with pytest.raises(pyrtx.RtxError):
+        n2.status()
+    n2.close()

@@ -230,7 +230,7 @@ def test_trace_results_independent_of_launch_config(ctx):
                     {"block_size": 64, "blocks_per_cu": 16, "ray_chunk": 1024},
                     {"block_size": 128, "blocks_per_cu": 6, "refill_lanes": 64, "refill_interval": 1},
                     {"block_size": 256, "blocks_per_cu": 1, "ray_chunk": 64, "refill_lanes": 8,
-                     "refill_interval": 64}):
+                     "refill_interval": 64}, {"leaf_lanes": 16}, {"leaf_lanes": 64, "block_size": 64}):
             ctx.set_config(**cfg)
             got = _trace_closest(ctx, tlas, rays)
             assert all(np.array_equal(a, b) for a, b in zip(got, ref)), cfg
