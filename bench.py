@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--no-anyhit", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing / NRC equal-relMSE part")
     ap.add_argument("--no-autotune", action="store_true", help="use the default traversal configuration")
+    ap.add_argument("--no-query", action="store_true",
+                    help="skip the 20M-triangle ray-query part (profiling the path-tracing part alone)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the FP64 reference render")
     ap.add_argument("--nrc-frames", type=int, default=160, help="NRC frames rendered (one sample each)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -60,6 +62,14 @@ def main():
 
     import pyrtx
     from pyrtx import scenes as S
+
+    if args.no_query:
+        ctx = pyrtx.Context(0)
+        render = render_bench(ctx, args, np, torch, pyrtx, S)
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "path_tracing": render}),
+              flush=True)
+        ctx.close()
+        return
 
     t0 = time.time()
     sc = S.query_scene(args.scale)
@@ -185,8 +195,11 @@ def main():
     # roofline of the traversal kernel (both launches of a step)
     b_coh, b_inc = bytes_per_ray(stats["coherent"]), bytes_per_ray(stats["incoherent"])
     achieved = (b_coh * W * H + b_inc * W * H) / ((ms_coh + ms_inc) * 1e-3) / 1e9
+    traffic = measured_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else round((traffic["coherent"] + traffic["incoherent"]) / 2),
+                "traffic_bytes_per_launch": traffic,
                 "kernel": "trace_kernel<closest> (persistent BVH8 traversal)",
                 "bytes_per_ray": {"coherent": round(b_coh, 1), "incoherent": round(b_inc, 1)},
                 "launch_ms": {"coherent": round(ms_coh, 3), "incoherent": round(ms_inc, 3)}}
@@ -249,6 +262,19 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def measured_traffic():
+    """HBM bytes per traversal launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated) as recorded in the
+    committed profile of this kernel and workload; None when the profile is absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "trace_kernel_hbm_traffic_20M_query_scene.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"bench-traffic-bytes-per-launch: coherent (\d+) incoherent (\d+)", txt)
+    return {"coherent": int(m.group(1)), "incoherent": int(m.group(2))} if m else None
 
 
 def render_bench(ctx, args, np, torch, pyrtx, S):
