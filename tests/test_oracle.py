@@ -429,3 +429,28 @@ def test_treelet_refinement_invariants(mesh):
             assert sah <= prev * (1 + 1e-6)              # cost never increases
         prev = sah
     assert res[1][4] < 0.97 * res[0][4]                  # refinement does reduce the SAH cost
+
+
+# ---- path tracer restatement -------------------------------------------------------------------------------------
+@pytest.mark.parametrize("rr_depth", [8, 3])
+def test_oracle_render_white_furnace(rr_depth):
+    """Closed cube whose walls all emit 1 and reflect a: with NEE over the same walls the estimator's expectation
+    is 1 + a + ... + a^8 at max depth 8 (the white-furnace closed form), with or without Russian roulette."""
+    a = 0.5
+    sc, lights = S.furnace_scene(a, 1.0, 16, 16)
+    t = oracle_scene(sc)
+    img = O.render(t, sc.records, sc.camera, lights, 16, 16, 128, max_depth=8, rr_depth=rr_depth) / 128
+    expected = (1 - a ** 9) / (1 - a)
+    assert abs(img.mean() - expected) / expected < 0.01, (img.mean(), expected)
+
+
+def test_oracle_render_error_falls_as_one_over_spp():
+    """Monte Carlo: the squared error against a high-sample estimate falls ~1/spp (independent sample sets)."""
+    sc, lights = S.cornell_scene(2, width=12, height=8)
+    t = oracle_scene(sc)
+    ref = O.render(t, sc.records, sc.camera, lights, 12, 8, 2048, sample_offset=1 << 20) / 2048
+    e = []
+    for spp in (8, 32):
+        x = O.render(t, sc.records, sc.camera, lights, 12, 8, spp) / spp
+        e.append(float(np.mean((x - ref) ** 2)))
+    assert 2.0 < e[0] / e[1] < 8.0, e
