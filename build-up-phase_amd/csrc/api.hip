@@ -268,12 +268,106 @@ int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uin
         d.first_vertex = x.first_vertex;
         d.prim_base = base;
         d.max_vertex = x.max_vertex;
+        d.geom_local = i;
+        d.blas = 0;
         base += x.primitive_count;
     }
     hipError_t e = rtx::build_blas(gd.data(), geometry_count, (uint32_t)n, build_flags, dst_device, scratch_device,
                                    &ctx->state->error_flags, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_build_blas");
     if (out_address) *out_address = (uint64_t)(uintptr_t)dst_device;
+    return RTX_SUCCESS;
+}
+
+static int check_batch(rtx_context ctx, const rtx_blas_build_info* infos, uint32_t count, uint32_t flags,
+                       uint64_t* total, uint32_t* ngeom) {
+    if (count == 0 || !infos) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "no BLAS infos");
+    if (count > (1u << 20)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "more than 2^20 BLASes in one batch");
+    if (flags & RTX_BUILD_MORTON63_BIT)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "batched builds use 30-bit Morton codes");
+    uint64_t n = 0, g = 0;
+    for (uint32_t b = 0; b < count; ++b) {
+        uint64_t nb = 0;
+        int r = check_geometries(ctx, infos[b].geometries, infos[b].geometry_count, &nb);
+        if (r) return r;
+        n += nb;
+        g += infos[b].geometry_count;
+    }
+    if (n > kMaxPrims) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "too many primitives (%llu)", (unsigned long long)n);
+    if (g > (1u << 24)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "too many geometries");
+    *total = n;
+    *ngeom = (uint32_t)g;
+    return RTX_SUCCESS;
+}
+
+int rtx_get_blas_batch_build_sizes(rtx_context ctx, const rtx_blas_build_info* infos, uint32_t count,
+                                   uint32_t build_flags, uint64_t* as_sizes, uint64_t* scratch_size) {
+    if (!ctx || !scratch_size) return RTX_ERROR_INVALID_ARGUMENT;
+    uint64_t n = 0;
+    uint32_t ng = 0;
+    int r = check_batch(ctx, infos, count, build_flags, &n, &ng);
+    if (r) return r;
+    if (as_sizes) {
+        for (uint32_t b = 0; b < count; ++b) {
+            uint64_t nb = 0;
+            (void)check_geometries(ctx, infos[b].geometries, infos[b].geometry_count, &nb);
+            as_sizes[b] = rtx::blas_as_bytes((uint32_t)nb);
+        }
+    }
+    *scratch_size = rtx::blas_batch_scratch_bytes((uint32_t)n, ng, count);
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, uint32_t count, uint32_t build_flags,
+                             void* scratch_device, uint64_t scratch_size, uint64_t* out_addresses) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    uint64_t n = 0;
+    uint32_t ng = 0;
+    int r = check_batch(ctx, infos, count, build_flags, &n, &ng);
+    if (r) return r;
+    if (!scratch_device || !aligned(scratch_device, 256))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "scratch must be non-NULL and 256-B aligned");
+    if (scratch_size < rtx::blas_batch_scratch_bytes((uint32_t)n, ng, count))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "scratch_size %llu below the size query",
+                    (unsigned long long)scratch_size);
+    std::vector<rtx::GeomDesc> gd(ng ? ng : 1);
+    std::vector<uint2> binfo(count);
+    std::vector<rtx::AsHeader*> dst(count);
+    uint32_t base = 0, gi = 0;
+    for (uint32_t b = 0; b < count; ++b) {
+        const rtx_blas_build_info& in = infos[b];
+        uint64_t nb = 0;
+        (void)check_geometries(ctx, in.geometries, in.geometry_count, &nb);
+        if (!in.dst_device || !aligned(in.dst_device, 256))
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "BLAS %u: dst must be non-NULL and 256-B aligned", b);
+        if (in.dst_size < rtx::blas_as_bytes((uint32_t)nb))
+            return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "BLAS %u: dst_size below the size query", b);
+        for (uint32_t i = 0; i < in.geometry_count; ++i, ++gi) {
+            const rtx_blas_geometry& x = in.geometries[i];
+            rtx::GeomDesc& d = gd[gi];
+            d.vtx = (const uint8_t*)x.vertex_data_device;
+            d.idx = (const uint32_t*)((const uint8_t*)x.index_data_device + x.primitive_offset);
+            d.xform = x.transform_data_device
+                          ? (const float*)((const uint8_t*)x.transform_data_device + x.transform_offset)
+                          : nullptr;
+            d.stride = x.vertex_stride;
+            d.prim_count = x.primitive_count;
+            d.first_vertex = x.first_vertex;
+            d.prim_base = base;
+            d.max_vertex = x.max_vertex;
+            d.geom_local = i;
+            d.blas = b;
+            base += x.primitive_count;
+        }
+        binfo[b] = make_uint2((uint32_t)nb, in.geometry_count);
+        dst[b] = (rtx::AsHeader*)in.dst_device;
+    }
+    hipError_t e = rtx::build_blas_batch(gd.data(), ng, binfo.data(), dst.data(), count, (uint32_t)n, build_flags,
+                                         scratch_device, &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_build_blas_batch");
+    // the host arrays above are read by asynchronous copies from pageable memory, which complete before return
+    if (out_addresses)
+        for (uint32_t b = 0; b < count; ++b) out_addresses[b] = (uint64_t)(uintptr_t)infos[b].dst_device;
     return RTX_SUCCESS;
 }
 

@@ -15,6 +15,13 @@ uint64_t tlas_scratch_bytes(uint32_t instance_count, bool morton63);
 // Enqueue a BLAS build of prim_count triangles described by geometry_count host descriptors.
 hipError_t build_blas(const GeomDesc* geoms_host, uint32_t geometry_count, uint32_t prim_count, uint32_t flags,
                       void* dst, void* scratch, uint32_t* err_flags, hipStream_t stream);
+// Enqueue the builds of nbatch BLASes in one pass over all their triangles.  geoms_host: all geometries in BLAS
+// order (GeomDesc.blas / geom_local / prim_base over the concatenation); binfo_host[b] = (triangles, geometries);
+// dst_host[b]: the BLAS storage (>= blas_as_bytes of its triangle count).  30-bit Morton codes only.
+uint64_t blas_batch_scratch_bytes(uint32_t prim_count, uint32_t geometry_count, uint32_t nbatch);
+hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t geometry_count, const uint2* binfo_host,
+                            AsHeader* const* dst_host, uint32_t nbatch, uint32_t prim_count, uint32_t flags,
+                            void* scratch, uint32_t* err_flags, hipStream_t stream);
 // Enqueue a TLAS build over instance_count device-resident rtx_instance records.
 hipError_t build_tlas(const rtx_instance* instances_device, uint32_t instance_count, uint32_t flags, void* dst,
                       void* scratch, uint32_t* err_flags, hipStream_t stream);

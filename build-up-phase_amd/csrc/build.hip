@@ -76,9 +76,42 @@ __device__ void reduce_bounds(const float* c, bool valid, uint32_t* cb) {
     }
 }
 
+// Batched builds: per-BLAS centroid bounds (6 words per BLAS).  Threads of the block's first BLAS reduce through
+// LDS; the few threads of a block that straddles a BLAS boundary use global atomics directly.
+__device__ void reduce_bounds_seg(const float* c, bool valid, uint32_t blas, uint32_t* cb) {
+    __shared__ uint32_t s[6];
+    __shared__ uint32_t sb;
+    if (threadIdx.x == 0) sb = blas;
+    if (threadIdx.x < 6) s[threadIdx.x] = (threadIdx.x < 3) ? 0xFFFFFFFFu : 0u;
+    __syncthreads();
+    const uint32_t b0 = sb;
+    if (valid) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t o = float_to_ordered(c[a]);
+            if (blas == b0) {
+                atomicMin(&s[a], o);
+                atomicMax(&s[3 + a], o);
+            } else {
+                atomicMin(&cb[6 * (uint64_t)blas + a], o);
+                atomicMax(&cb[6 * (uint64_t)blas + 3 + a], o);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        if (s[threadIdx.x] != 0xFFFFFFFFu) atomicMin(&cb[6 * (uint64_t)b0 + threadIdx.x], s[threadIdx.x]);
+    } else if (threadIdx.x < 6) {
+        if (s[threadIdx.x] != 0u) atomicMax(&cb[6 * (uint64_t)b0 + threadIdx.x], s[threadIdx.x]);
+    }
+}
+
+// prim_blas != nullptr: batched build (per-primitive BLAS id out, per-BLAS centroid bounds).
 __global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restrict__ geoms, int ngeom, uint32_t n,
                                                         TriSlot* __restrict__ leaves, float* __restrict__ boxes,
-                                                        uint32_t* __restrict__ cbounds) {
+                                                        uint32_t* __restrict__ cbounds,
+                                                        uint32_t* __restrict__ prim_blas) {
+    uint32_t blas = 0;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = false;
     float cen[3] = {0, 0, 0};
@@ -90,6 +123,8 @@ __global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restri
         }
         const GeomDesc g = geoms[lo];
         uint32_t p = i - g.prim_base;
+        blas = g.blas;
+        if (prim_blas) prim_blas[i] = blas;
         TriSlot t;
         valid = true;
 #pragma unroll
@@ -120,7 +155,7 @@ __global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restri
 #pragma unroll
         for (int k = 0; k < 9; ++k) valid = valid && isfinite(t.v[k]);
         t.prim = p;
-        t.geom = (uint32_t)lo;
+        t.geom = g.geom_local;
         t.pad[0] = valid ? 1u : 0u;
         t.pad[1] = t.pad[2] = t.pad[3] = t.pad[4] = 0u;
         leaves[i] = t;
@@ -143,7 +178,10 @@ __global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restri
             }
         }
     }
-    reduce_bounds(cen, valid, cbounds);
+    if (prim_blas)
+        reduce_bounds_seg(cen, valid, blas, cbounds);
+    else
+        reduce_bounds(cen, valid, cbounds);
 }
 
 __device__ void invert3x4_d(const double* m, double* inv) {
@@ -299,6 +337,35 @@ __global__ __launch_bounds__(256) void morton_kernel(const float* __restrict__ b
     keys[i] = valid ? code : (K)~(K)0;
 }
 
+// Batched builds: key = BLAS id << 30 | 30-bit Morton code over that BLAS's centroid bounds, so that a stable
+// sort groups the BLASes and every BLAS becomes one subtree of the binary radix tree.
+__global__ __launch_bounds__(256) void morton_batch_kernel(const float* __restrict__ boxes, uint32_t n,
+                                                           const uint32_t* __restrict__ cbounds,
+                                                           const uint32_t* __restrict__ prim_blas,
+                                                           uint64_t* __restrict__ keys) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t blas = prim_blas[i];
+    const uint32_t* cb = cbounds + 6 * (uint64_t)blas;
+    const float* b = boxes + 6 * (uint64_t)i;
+    uint32_t q[3];
+    const bool valid = b[0] <= b[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float cmin = ordered_to_float(cb[a]);
+        float cmax = ordered_to_float(cb[3 + a]);
+        float ext = cmax - cmin;
+        float scale = ext > 0.0f ? 1024.0f / ext : 0.0f;
+        float s = b[a] + b[3 + a];
+        float c = s * 0.5f;
+        q[a] = quant_axis(c, cmin, scale, 1023u);
+    }
+    const uint32_t code = valid ? ((spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2])) : 0x3FFFFFFFu;
+    keys[i] = ((uint64_t)blas << 30) | code;
+}
+
+constexpr uint32_t kMixedBlas = 0xFFFFFFFFu;
+
 template <typename K>
 __device__ __forceinline__ int karras_delta(const K* keys, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
@@ -309,9 +376,11 @@ __device__ __forceinline__ int karras_delta(const K* keys, int64_t n, int64_t i,
     return __clz((uint32_t)(a ^ b));
 }
 
+// node_blas != nullptr (batched builds): the BLAS that owns all primitives of internal node i, or kMixedBlas.
 template <typename K>
 __global__ __launch_bounds__(256) void karras_kernel(const K* __restrict__ keys, uint32_t n, int32_t* __restrict__ left,
-                                                     int32_t* __restrict__ right, int32_t* __restrict__ parent) {
+                                                     int32_t* __restrict__ right, int32_t* __restrict__ parent,
+                                                     uint32_t* __restrict__ node_blas = nullptr) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i == 0) parent[0] = -1;
     if (i >= (int64_t)n - 1) return;
@@ -339,6 +408,33 @@ __global__ __launch_bounds__(256) void karras_kernel(const K* __restrict__ keys,
     right[i] = rc;
     parent[lc >= 0 ? lc : (N - 1) + (-1 - lc)] = (int32_t)i;
     parent[rc >= 0 ? rc : (N - 1) + (-1 - rc)] = (int32_t)i;
+    if (node_blas) {
+        const uint64_t blo = (uint64_t)keys[lo] >> 30, bhi = (uint64_t)keys[hi] >> 30;
+        node_blas[i] = blo == bhi ? (uint32_t)blo : kMixedBlas;
+    }
+}
+
+// Batched builds: the root of BLAS b is its topmost node whose primitives all belong to b (a leaf for a
+// one-triangle BLAS).  Runs over the 2n-1 nodes before the refit; the refinement never moves these roots.
+__global__ __launch_bounds__(256) void blas_roots_kernel(uint32_t n, const uint64_t* __restrict__ keys,
+                                                         const int32_t* __restrict__ parent,
+                                                         const uint32_t* __restrict__ node_blas,
+                                                         int32_t* __restrict__ roots) {
+    const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= 2ull * n - 1) return;
+    uint32_t b;
+    int32_t code;
+    if (k < (uint64_t)n - 1) {
+        b = node_blas[k];
+        code = (int32_t)k;
+    } else {
+        const uint64_t pos = k - (n - 1);
+        b = (uint32_t)(keys[pos] >> 30);
+        code = -1 - (int32_t)pos;
+    }
+    if (b == kMixedBlas) return;
+    const int32_t p = parent[k];
+    if (p < 0 || node_blas[p] == kMixedBlas) roots[b] = code;
 }
 
 __device__ __forceinline__ float load_coherent(const float* p) {
@@ -629,7 +725,8 @@ __global__ __launch_bounds__(64) void refit_refine_kernel(uint32_t n, int32_t* l
                                                           const uint32_t* __restrict__ order,
                                                           const float* __restrict__ prim_boxes, float* nboxes,
                                                           float* cost, uint32_t* cnt, uint32_t* hgt,
-                                                          uint32_t* arrivals, uint32_t min_prims) {
+                                                          uint32_t* arrivals, uint32_t min_prims,
+                                                          const uint32_t* __restrict__ node_blas) {
     __shared__ TreeletLds T;
     const uint32_t lane = threadIdx.x;
     const uint32_t pos = blockIdx.x * 64u + lane;
@@ -683,7 +780,8 @@ __global__ __launch_bounds__(64) void refit_refine_kernel(uint32_t n, int32_t* l
                 }
             }
         }
-        uint64_t need = __ballot(own && ccnt >= min_prims);
+        // batched builds never restructure across BLASes
+        uint64_t need = __ballot(own && ccnt >= min_prims && (!node_blas || node_blas[node] != kMixedBlas));
         while (need) {
             const int L = __ffsll((unsigned long long)need) - 1;
             need &= need - 1;
@@ -805,26 +903,34 @@ __device__ __forceinline__ void quant_child(float clo, float chi, float p, uint3
     qhi = (uint8_t)qh;
 }
 
-// One collapse level.  Task = (BVH2 node code, destination slot); node code 0xFFFFFFFF = leaf 0 (n == 1 root).
+// One collapse level.  Task = (BVH2 node code, destination slot); a negative node code is a leaf that is the
+// root itself (one-primitive structure).  Batched builds (as_ptr != nullptr) carry each task's BLAS in a parallel
+// array and write into that BLAS's structure.
 __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const int32_t* __restrict__ left,
                                                              const int32_t* __restrict__ right,
                                                              const uint32_t* __restrict__ order,
                                                              const float* __restrict__ nboxes,
                                                              const uint4* __restrict__ leaf_src,
-                                                             AsHeader* __restrict__ h, uint4* __restrict__ slots,
+                                                             AsHeader* __restrict__ h1, uint4* __restrict__ slots1,
                                                              uint32_t* __restrict__ level_counts, int level,
                                                              const uint2* __restrict__ tasks_in,
-                                                             uint2* __restrict__ tasks_out) {
+                                                             uint2* __restrict__ tasks_out,
+                                                             AsHeader* const* __restrict__ as_ptr,
+                                                             const uint32_t* __restrict__ tblas_in,
+                                                             uint32_t* __restrict__ tblas_out) {
     const uint32_t count = level_counts[level];
     if (count == 0) return;
-    const float rlo[3] = {h->lo[0], h->lo[1], h->lo[2]};
-    const float qs[3] = {h->qscale[0], h->qscale[1], h->qscale[2]};
     for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < count; ti += gridDim.x * blockDim.x) {
         uint2 task = tasks_in[ti];
+        const uint32_t tb = as_ptr ? tblas_in[ti] : 0u;
+        AsHeader* h = as_ptr ? as_ptr[tb] : h1;
+        uint4* slots = as_ptr ? (uint4*)((uint8_t*)h + kHeaderBytes) : slots1;
+        const float rlo[3] = {h->lo[0], h->lo[1], h->lo[2]};
+        const float qs[3] = {h->qscale[0], h->qscale[1], h->qscale[2]};
         int32_t ch[8];
         int cnt;
-        if (task.x == 0xFFFFFFFFu) {
-            ch[0] = -1;
+        if ((int32_t)task.x < 0) {
+            ch[0] = (int32_t)task.x;
             cnt = 1;
         } else {
             ch[0] = left[task.x];
@@ -948,6 +1054,7 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                 node.imask |= (uint8_t)(1u << k);
                 uint32_t o = atomicAdd(&level_counts[level + 1], 1u);
                 tasks_out[o] = make_uint2((uint32_t)c, slot);
+                if (as_ptr) tblas_out[o] = tb;
             } else {
                 uint32_t prim = order[-1 - c];
                 const uint4* src = leaf_src + 4 * (uint64_t)prim;
@@ -969,10 +1076,80 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
     }
 }
 
+// Batched builds: one thread per BLAS writes its header (as header_kernel) and queues its root task.
+__global__ __launch_bounds__(64) void batch_header_kernel(AsHeader* const* __restrict__ as_ptr,
+                                                          const uint2* __restrict__ binfo, uint32_t nbatch,
+                                                          const int32_t* __restrict__ roots, uint32_t n,
+                                                          uint32_t flags, const float* __restrict__ nboxes,
+                                                          const float* __restrict__ cost,
+                                                          uint32_t* __restrict__ level_counts,
+                                                          uint2* __restrict__ tasks0, uint32_t* __restrict__ tblas0) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbatch) return;
+    AsHeader* h = as_ptr[b];
+    const uint32_t np = binfo[b].x;
+    h->magic = kMagicBlas;
+    h->version = kVersion;
+    h->type = 0;
+    h->flags = flags;
+    h->prim_count = np;
+    h->geometry_count = binfo[b].y;
+    h->node_count = 0;
+    h->slot_count = 0;
+    h->max_depth = 0;
+    h->max_total_depth = 0;
+    h->build_error = 0;
+    h->slots_offset = kHeaderBytes;
+    h->inst_table_offset = 0;
+    h->capacity_slots = 2ull * (np ? np : 1) + 1;
+    h->sah_cost = 0.0;
+    const int32_t root = np ? roots[b] : 0x7FFFFFFF;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool empty = root == 0x7FFFFFFF;
+    if (!empty) {
+        const float* rb = nboxes + 6 * (root >= 0 ? (uint64_t)root : (uint64_t)n - 1 + (uint64_t)(-1 - root));
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = rb[a];
+            hi[a] = rb[3 + a];
+        }
+        empty = !(lo[0] <= hi[0]);
+        if (root >= 0) {
+            const float ra = box_area(rb);
+            h->sah_cost = ra > 0.0f ? (double)cost[root] / (double)ra : 0.0;
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        if (empty) {
+            lo[a] = 0.0f;
+            hi[a] = 0.0f;
+        }
+        h->lo[a] = lo[a];
+        h->hi[a] = hi[a];
+        const float ext = hi[a] - lo[a];
+        h->qscale[a] = ext > 0.0f ? ext / 65535.0f : 0.0f;
+    }
+    h->lo[3] = h->hi[3] = h->qscale[3] = 0.0f;
+    if (empty) return;
+    h->slot_count = 1;
+    const uint32_t o = atomicAdd(&level_counts[0], 1u);
+    tasks0[o] = make_uint2((uint32_t)root, 0u);
+    tblas0[o] = b;
+}
+
+__global__ void batch_finish_kernel(AsHeader* const* __restrict__ as_ptr, uint32_t nbatch) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nbatch) as_ptr[b]->max_total_depth = as_ptr[b]->max_depth;
+}
+
+__global__ void fill_bounds_kernel(uint32_t* __restrict__ cb, uint32_t nbatch) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 6 * nbatch) cb[i] = (i % 6) < 3 ? 0xFFFFFFFFu : 0u;
+}
+
 // Tasks left after the last enqueued level mean the binary tree was deeper than the level bound.
 __global__ void collapse_check_kernel(AsHeader* h, const uint32_t* level_counts, int max_levels, uint32_t* err) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && level_counts[max_levels] != 0u) {
-        h->build_error = 1u;
+        if (h) h->build_error = 1u;
         atomicOr(err, kErrBuildDepth);
     }
 }
@@ -993,9 +1170,11 @@ __global__ void fill_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
 struct Layout {
     uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, cost, cnt, hgt, tasks0, tasks1,
         levels, cbounds, misc, geoms, total;
+    // batched builds only
+    uint64_t pblas = 0, nblas = 0, roots = 0, asptr = 0, binfo = 0, tblas0 = 0, tblas1 = 0, bcb = 0;
 };
 
-Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom) {
+Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom, uint32_t nbatch = 0) {
     Layout L;
     uint64_t o = 0;
     auto take = [&](uint64_t bytes) {
@@ -1023,6 +1202,16 @@ Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom) {
     L.cbounds = take(8 * 4);
     L.misc = take(64);
     L.geoms = take((uint64_t)(ngeom ? ngeom : 1) * sizeof(GeomDesc));
+    if (nbatch) {
+        L.pblas = take(nn * 4);
+        L.nblas = take(nn * 4);
+        L.roots = take((uint64_t)nbatch * 4);
+        L.asptr = take((uint64_t)nbatch * 8);
+        L.binfo = take((uint64_t)nbatch * 8);
+        L.tblas0 = take(nn * 4);
+        L.tblas1 = take(nn * 4);
+        L.bcb = take((uint64_t)nbatch * 24);
+    }
     L.total = o;
     return L;
 }
@@ -1036,7 +1225,7 @@ uint64_t as_bytes(uint32_t n, bool tlas) {
 
 // Refit with treelet refinement; each further round restructures the tree the previous one left.
 void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L, const float* prim_boxes,
-                  hipStream_t s) {
+                  hipStream_t s, const uint32_t* node_blas = nullptr) {
     const int rounds = treelet_rounds(flags, n);
     uint32_t* arrivals = (uint32_t*)(scratch + L.arrivals);
     for (int round = 0; round < (rounds ? rounds : 1); ++round) {
@@ -1045,7 +1234,7 @@ void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L,
             n, (int32_t*)(scratch + L.left), (int32_t*)(scratch + L.right), (int32_t*)(scratch + L.parent),
             (const uint32_t*)(scratch + L.vals), prim_boxes, (float*)(scratch + L.nboxes), (float*)(scratch + L.cost),
             (uint32_t*)(scratch + L.cnt), (uint32_t*)(scratch + L.hgt), arrivals,
-            rounds ? kTreeletMinPrims[round] : 0xFFFFFFFFu);
+            rounds ? kTreeletMinPrims[round] : 0xFFFFFFFFu, node_blas);
     }
 }
 
@@ -1097,7 +1286,7 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
             uint2* tout = (lv & 1) ? t0 : t1;
             collapse_level_kernel<<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
                                                                   (const uint4*)(scratch + L.leaves), h, slots, levels,
-                                                                  lv, tin, tout);
+                                                                  lv, tin, tout, nullptr, nullptr, nullptr);
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, max_levels, err_flags);
     }
@@ -1181,6 +1370,9 @@ uint64_t blas_as_bytes(uint32_t n) { return as_bytes(n, false); }
 uint64_t tlas_as_bytes(uint32_t n) { return as_bytes(n, true); }
 uint64_t blas_scratch_bytes(uint32_t n, bool k64, uint32_t ngeom) { return make_layout(n, k64 ? 8 : 4, ngeom).total; }
 uint64_t tlas_scratch_bytes(uint32_t n, bool k64) { return make_layout(n, k64 ? 8 : 4, 0).total; }
+uint64_t blas_batch_scratch_bytes(uint32_t n, uint32_t ngeom, uint32_t nbatch) {
+    return make_layout(n, 8, ngeom, nbatch ? nbatch : 1).total;
+}
 
 hipError_t build_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, uint32_t flags, void* dst,
                       void* scratch, uint32_t* err_flags, hipStream_t s) {
@@ -1197,7 +1389,7 @@ hipError_t build_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, ui
     fill_kernel<<<1, 64, 0, s>>>(cb + 3, 3, 0u);
     if (n > 0)
         prep_tris_kernel<<<(n + 255) / 256, 256, 0, s>>>(gd, (int)ngeom, n, (TriSlot*)(sc + L.leaves),
-                                                         (float*)(sc + L.boxes), cb);
+                                                         (float*)(sc + L.boxes), cb, nullptr);
     if ((err = lbvh_common(n, k64, 0, flags, ngeom, sc, L, (uint8_t*)dst, 0, err_flags, s))) return err;
     finish_blas_kernel<<<1, 64, 0, s>>>((AsHeader*)dst);
     return hipGetLastError();
@@ -1221,6 +1413,69 @@ hipError_t build_tlas(const rtx_instance* inst, uint32_t n, uint32_t flags, void
     hipError_t err;
     if ((err = lbvh_common(n, k64, 1, flags, 0, sc, L, (uint8_t*)dst, table_offset, err_flags, s))) return err;
     finish_tlas_kernel<<<1, 64, 0, s>>>((AsHeader*)dst, maxd);
+    return hipGetLastError();
+}
+
+// Batched BLAS build: every step runs once over the primitives of all BLASes.  Keys are BLAS id << 30 | Morton
+// code, so after the stable sort each BLAS is one contiguous range and one subtree of the single Karras tree; the
+// part of the tree above the BLAS roots is never collapsed, refined or stored.
+hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t ngeom, const uint2* binfo_host,
+                            AsHeader* const* dst_host, uint32_t nbatch, uint32_t n, uint32_t flags, void* scratch,
+                            uint32_t* err_flags, hipStream_t s) {
+    Layout L = make_layout(n, 8, ngeom, nbatch);
+    uint8_t* sc = (uint8_t*)scratch;
+    hipError_t err;
+    GeomDesc* gd = (GeomDesc*)(sc + L.geoms);
+    AsHeader** as_ptr = (AsHeader**)(sc + L.asptr);
+    uint2* binfo = (uint2*)(sc + L.binfo);
+    if (ngeom && (err = hipMemcpyAsync(gd, geoms_host, ngeom * sizeof(GeomDesc), hipMemcpyHostToDevice, s)))
+        return err;
+    if ((err = hipMemcpyAsync(as_ptr, dst_host, nbatch * sizeof(AsHeader*), hipMemcpyHostToDevice, s))) return err;
+    if ((err = hipMemcpyAsync(binfo, binfo_host, nbatch * sizeof(uint2), hipMemcpyHostToDevice, s))) return err;
+    uint32_t* cb = (uint32_t*)(sc + L.bcb);
+    uint32_t* pblas = (uint32_t*)(sc + L.pblas);
+    uint32_t* nblas = (uint32_t*)(sc + L.nblas);
+    int32_t* roots = (int32_t*)(sc + L.roots);
+    uint64_t* keys = (uint64_t*)(sc + L.keys);
+    uint32_t* vals = (uint32_t*)(sc + L.vals);
+    int32_t* left = (int32_t*)(sc + L.left);
+    int32_t* right = (int32_t*)(sc + L.right);
+    int32_t* parent = (int32_t*)(sc + L.parent);
+    float* nboxes = (float*)(sc + L.nboxes);
+    float* boxes = (float*)(sc + L.boxes);
+    uint32_t* levels = (uint32_t*)(sc + L.levels);
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    fill_bounds_kernel<<<(6 * nbatch + 255) / 256, 256, 0, s>>>(cb, nbatch);
+    fill_kernel<<<(nbatch + 255) / 256, 256, 0, s>>>((uint32_t*)roots, nbatch, 0x7FFFFFFFu);
+    if (n > 0) {
+        prep_tris_kernel<<<gb, 256, 0, s>>>(gd, (int)ngeom, n, (TriSlot*)(sc + L.leaves), boxes, cb, pblas);
+        morton_batch_kernel<<<gb, 256, 0, s>>>(boxes, n, cb, pblas, keys);
+        if ((err = sort_pairs_u64(keys, vals, n, sc + L.sort, err_flags, s))) return err;
+        if (n > 1) {
+            karras_kernel<uint64_t><<<gb, 256, 0, s>>>(keys, n, left, right, parent, nblas);
+        } else {
+            fill_kernel<<<1, 64, 0, s>>>((uint32_t*)parent, 1, 0xFFFFFFFFu);
+        }
+        blas_roots_kernel<<<(unsigned)((2ull * n - 1 + 255) / 256), 256, 0, s>>>(n, keys, parent, nblas, roots);
+        refit_passes(n, flags, sc, L, boxes, s, n > 1 ? nblas : nullptr);
+    }
+    fill_kernel<<<1, 128, 0, s>>>(levels, 128, 0u);
+    batch_header_kernel<<<(nbatch + 63) / 64, 64, 0, s>>>(as_ptr, binfo, nbatch, roots, n, flags, nboxes,
+                                                          (const float*)(sc + L.cost), levels,
+                                                          (uint2*)(sc + L.tasks0), (uint32_t*)(sc + L.tblas0));
+    if (n > 0) {
+        for (int lv = 0; lv < kMaxLevels32; ++lv) {
+            uint2* tin = (uint2*)(sc + ((lv & 1) ? L.tasks1 : L.tasks0));
+            uint2* tout = (uint2*)(sc + ((lv & 1) ? L.tasks0 : L.tasks1));
+            uint32_t* bin = (uint32_t*)(sc + ((lv & 1) ? L.tblas1 : L.tblas0));
+            uint32_t* bout = (uint32_t*)(sc + ((lv & 1) ? L.tblas0 : L.tblas1));
+            collapse_level_kernel<<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
+                                                                  (const uint4*)(sc + L.leaves), nullptr, nullptr,
+                                                                  levels, lv, tin, tout, as_ptr, bin, bout);
+        }
+        collapse_check_kernel<<<1, 64, 0, s>>>(nullptr, levels, kMaxLevels32, err_flags);
+    }
+    batch_finish_kernel<<<(nbatch + 63) / 64, 64, 0, s>>>(as_ptr, nbatch);
     return hipGetLastError();
 }
 

@@ -105,6 +105,8 @@ struct GeomDesc {
     uint32_t first_vertex;
     uint32_t prim_base;     // prefix sum of primitive counts
     uint32_t max_vertex;
+    uint32_t geom_local;    // geometry index within its BLAS (gl_GeometryIndexEXT)
+    uint32_t blas;          // BLAS of a batched build (0 otherwise)
 };
 
 // Triangle record in build scratch (48 B): transformed vertices + IDs.

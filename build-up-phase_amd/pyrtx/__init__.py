@@ -98,6 +98,11 @@ class RenderParams(ctypes.Structure):
                 ("ray_offset", ctypes.c_float)]
 
 
+class BlasBuildInfo(ctypes.Structure):
+    _fields_ = [("geometries", ctypes.c_void_p), ("geometry_count", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("dst_device", ctypes.c_void_p), ("dst_size", ctypes.c_uint64)]
+
+
 class NrcConfig(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint32), ("batch_size", ctypes.c_uint32), ("learning_rate", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float),
@@ -159,6 +164,8 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_float)]),
     "rtx_autotune_render": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P, _P, _P, _U32, ctypes.POINTER(TraceConfig),
                                            ctypes.POINTER(ctypes.c_float)]),
+    "rtx_get_blas_batch_build_sizes": (ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.POINTER(_U64)]),
+    "rtx_cmd_build_blas_batch": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _U64, _P]),
     "rtx_debug_treelet": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
 }
 
@@ -279,6 +286,34 @@ class Context:
                                                 scratch.numel(), ctypes.byref(addr)))
         n = sum(g["indices"].numel() // 3 for g in geoms)
         return AccelerationStructure(storage, addr.value, n, (scratch, tuple(geoms)))
+
+    def build_blas_batch(self, geom_lists, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
+        """rtx_cmd_build_blas_batch: one build pass over several BLASes (list of geometry lists as for
+        build_blas).  Returns one AccelerationStructure per list."""
+        torch = _torch()
+        k = len(geom_lists)
+        arrs = [self._geometry_structs(g) for g in geom_lists]
+        infos = (BlasBuildInfo * k)()
+        for b, (g, arr) in enumerate(zip(geom_lists, arrs)):
+            infos[b].geometries = ctypes.cast(arr, _P)
+            infos[b].geometry_count = len(g)
+        as_sizes = (ctypes.c_uint64 * k)()
+        scratch_size = ctypes.c_uint64()
+        self._check(self.lib.rtx_get_blas_batch_build_sizes(self.handle, ctypes.cast(infos, _P), k, flags,
+                                                            ctypes.cast(as_sizes, _P), ctypes.byref(scratch_size)))
+        storages = [torch.empty(int(as_sizes[b]), dtype=torch.uint8, device=self.device) for b in range(k)]
+        for b in range(k):
+            infos[b].dst_device = storages[b].data_ptr()
+            infos[b].dst_size = storages[b].numel()
+        scratch = torch.empty(int(scratch_size.value), dtype=torch.uint8, device=self.device)
+        addrs = (ctypes.c_uint64 * k)()
+        self._check(self.lib.rtx_cmd_build_blas_batch(self.handle, ctypes.cast(infos, _P), k, flags,
+                                                      scratch.data_ptr(), scratch.numel(), ctypes.cast(addrs, _P)))
+        out = []
+        for b, g in enumerate(geom_lists):
+            n = sum(x["indices"].numel() // 3 for x in g)
+            out.append(AccelerationStructure(storages[b], addrs[b], n, (scratch, tuple(g))))
+        return out
 
     def build_tlas(self, records, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT, scratch=None) -> AccelerationStructure:
         """records: numpy structured array of 64-B instance records (oracle-compatible layout) or a device uint8
@@ -494,10 +529,11 @@ class Context:
         return k.cpu().numpy().view(keys.dtype), v.cpu().numpy()[:n].view(np.uint32)
 
     # -- scenes ---------------------------------------------------------------------------------------------------
-    def upload_scene(self, scene, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
-        """Build one BLAS per scene.blas entry and the TLAS over scene instances.  Returns (blases, tlas)."""
+    def upload_scene(self, scene, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT, batched=False):
+        """Build one BLAS per scene.blas entry (one batched build when batched=True) and the TLAS over scene
+        instances.  Returns (blases, tlas)."""
         torch = _torch()
-        blases = []
+        lists = []
         for geoms in scene.blas:
             dg = []
             for g in geoms:
@@ -506,7 +542,11 @@ class Context:
                      "transform": None if g.transform is None else
                      _dev_tensor(np.asarray(g.transform, np.float32).reshape(3, 4), self.device)}
                 dg.append(d)
-            blases.append(self.build_blas(dg, flags))
+            lists.append(dg)
+        if batched:
+            blases = self.build_blas_batch(lists, flags)
+        else:
+            blases = [self.build_blas(dg, flags) for dg in lists]
         rec = instance_records(scene, [b.address for b in blases])
         tlas = self.build_tlas(rec, flags)
         torch.cuda.current_stream(self.device)

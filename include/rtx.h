@@ -133,6 +133,25 @@ int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uin
                        uint32_t build_flags, void* dst_device, uint64_t dst_size, void* scratch_device,
                        uint64_t scratch_size, uint64_t* out_address);
 
+/* One BLAS of a batched build: its geometries (host array) and its caller-owned storage. */
+typedef struct rtx_blas_build_info {
+    const rtx_blas_geometry* geometries;
+    uint32_t geometry_count;
+    uint32_t reserved;
+    void* dst_device;                 /* >= its acceleration_structure_size, 256-B aligned */
+    uint64_t dst_size;
+} rtx_blas_build_info;
+
+/* vkCmdBuildAccelerationStructuresKHR with infoCount > 1 (main.cpp:808 passes one): every build step runs once over
+ * the triangles of all `count` BLASes, so small BLASes no longer leave the GPU idle.  Sizes: as_sizes[b] (may be
+ * NULL) per BLAS, *scratch_size for the whole batch.  30-bit Morton codes only (RTX_BUILD_MORTON63_BIT is
+ * rejected).  out_addresses[b] receives BLAS b's address.  The structures are the same as single builds would
+ * give, except that equal Morton codes are ordered by the batch-wide triangle index. */
+int rtx_get_blas_batch_build_sizes(rtx_context ctx, const rtx_blas_build_info* infos, uint32_t count,
+                                   uint32_t build_flags, uint64_t* as_sizes, uint64_t* scratch_size);
+int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, uint32_t count, uint32_t build_flags,
+                             void* scratch_device, uint64_t scratch_size, uint64_t* out_addresses);
+
 /* Same for a TLAS over instance_count rtx_instance records in device memory (main.cpp:912-945). */
 int rtx_cmd_build_tlas(rtx_context ctx, const rtx_instance* instances_device, uint32_t instance_count,
                        uint32_t build_flags, void* dst_device, uint64_t dst_size, void* scratch_device,

@@ -240,6 +240,63 @@ def test_trace_results_independent_of_launch_config(ctx):
         ctx.set_config(**base_cfg)
 
 
+def _batch_scene(seed=21, instances=30):
+    """Uneven BLASes for a batched build: 1 and 2 triangles, two icospheres, a terrain, a transformed two-geometry
+    BLAS, plus (built, never instanced) an empty one."""
+    from pyrtx import scenes as S
+    rng = np.random.default_rng(seed)
+    one = S.Geometry(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    two = S.Geometry(np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32),
+                     np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
+    s1 = S.icosphere(6, radius=1.0, seed=seed, amplitude=0.05)
+    s2 = S.icosphere(15, radius=1.0, seed=seed + 1, amplitude=0.08)
+    ter = S.terrain(20, size=2.0, height=0.3, seed=seed + 2)
+    xf = np.zeros((3, 4), np.float32)
+    xf[:, :3] = np.eye(3)
+    xf[:, 3] = (0.5, 0.0, 0.0)
+    pair = [S.Geometry(s1.vertices, s1.indices), S.Geometry(two.vertices, two.indices, transform=xf)]
+    blas = [[one], [two], [s1], [s2], [ter], pair]
+    R = S.random_rotation(rng, instances)
+    sc = rng.uniform(0.5, 2.0, instances)
+    pos = rng.uniform(-6, 6, (instances, 3))
+    tx = np.zeros((instances, 3, 4))
+    tx[:, :, :3] = R * sc[:, None, None]
+    tx[:, :, 3] = pos
+    return S.Scene(blas=blas, transforms=tx.astype(np.float32), blas_of_instance=np.arange(instances) % len(blas),
+                   custom_index=np.arange(instances, dtype=np.uint32), mask=np.full(instances, 0xFF, np.uint32),
+                   sbt_offset=np.zeros(instances, np.uint32), flags=np.zeros(instances, np.uint32),
+                   camera=(0.0, 0.0, 20.0, 60.0), width=96, height=64, meta={"name": "batch"})
+
+
+def test_batched_blas_build(ctx):
+    _say("batched BLAS build")
+    sc = _batch_scene()
+    blases_b, tlas_b = ctx.upload_scene(sc, batched=True)
+    blases_s, tlas_s = ctx.upload_scene(sc)
+    for bb, bs in zip(blases_b, blases_s):
+        ib, is_ = ctx.as_info(bb), ctx.as_info(bs)
+        assert ib.primitive_count == is_.primitive_count and ib.geometry_count == is_.geometry_count
+        assert list(ib.bounds) == list(is_.bounds)
+        assert abs(ib.sah_cost - is_.sah_cost) <= 0.05 * is_.sah_cost + 1e-9
+    # an empty BLAS in a batch
+    import pyrtx
+    e = ctx.build_blas_batch([[], [{"vertices": torch.zeros((3, 3), device="cuda"),
+                                    "indices": torch.tensor([[0, 1, 2]], dtype=torch.int32, device="cuda")}]])
+    assert ctx.as_info(e[0]).primitive_count == 0 and ctx.as_info(e[0]).slot_count == 0
+    assert ctx.as_info(e[1]).primitive_count == 1
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=4)])
+    g = _trace_closest(ctx, tlas_b, rays)
+    tuv, ids, st = O.trace(t, rays)
+    rep = compare_closest(g, tuv, ids, st)
+    _say(f"  batched build vs oracle: {rep}")
+    assert rep["robust_hits"] > 100
+    assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0 and rep["uv_bad"] == 0
+    occ = _trace_any(ctx, tlas_b, rays)
+    _, _, sta = O.trace(t, rays, any_hit=True)
+    assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
+
+
 def test_query_scene_reduced_parity(ctx):
     _say("query scene 1/200 scale")
     sc = S.query_scene(scale_tris=0.005)
