@@ -35,6 +35,13 @@ struct rtx_context_t {
     void* pt_scratch = nullptr;
     uint64_t pt_scratch_bytes = 0;
     std::vector<rtx_nrc_t*> caches;  // live caches; detached when the context is destroyed first
+    // batched renders: uploaded views, the graph of the last recorded call and the word it reads sample_offset from
+    void* views_buf = nullptr;
+    uint64_t views_bytes = 0;
+    hipStream_t capture_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_key = 0;
+    uint32_t* sample_word = nullptr;
 };
 
 struct rtx_nrc_t {
@@ -182,6 +189,10 @@ int rtx_destroy_context(rtx_context ctx) {
     if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
     (void)hipStreamSynchronize(ctx->stream);
     for (rtx_nrc n : ctx->caches) n->ctx = nullptr;  // they can still be destroyed, nothing else
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    if (ctx->sample_word) (void)hipFree(ctx->sample_word);
+    if (ctx->views_buf) (void)hipFree(ctx->views_buf);
     if (ctx->gstack) (void)hipFree(ctx->gstack);
     if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
     if (ctx->state) (void)hipFree(ctx->state);
@@ -495,7 +506,7 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
 
 static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
                         const rtx_render_params* params, bool have_out, rtx::PtArgs& a, rtx::TraceLaunch& L,
-                        uint32_t extra_paths = 0) {
+                        uint32_t extra_paths = 0, uint32_t extra_cdf = 0) {
     int r = check_tlas(ctx, tlas_address);
     if (r) return r;
     if (!sbt || !camera || !params) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt, camera, params must be set");
@@ -510,7 +521,7 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
     if (sbt->hit_record_count > 0 && !sbt->hit_records_device)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt hit records are NULL");
     const uint32_t npix = p.width * p.height;
-    const uint64_t need = rtx::pt_scratch_bytes(npix + extra_paths, p.max_depth, p.light_triangle_count);
+    const uint64_t need = rtx::pt_scratch_bytes(npix + extra_paths, p.max_depth, p.light_triangle_count + extra_cdf);
     if (need > ctx->pt_scratch_bytes) {
         if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
         ctx->pt_scratch = nullptr;
@@ -521,7 +532,7 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
         ctx->pt_scratch_bytes = need;
     }
     a = rtx::PtArgs{};
-    rtx::pt_layout(a, ctx->pt_scratch, npix + extra_paths, p.max_depth, p.light_triangle_count);
+    rtx::pt_layout(a, ctx->pt_scratch, npix + extra_paths, p.max_depth, p.light_triangle_count + extra_cdf);
     a.tlas = (const void*)(uintptr_t)tlas_address;
     a.width = p.width;
     a.height = p.height;
@@ -560,6 +571,143 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
     hipError_t e = rtx::pt_render(a, L, params->spp, params->sample_offset, film_rgba32f_device, accum_rgb_device,
                                   ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render");
+    return RTX_SUCCESS;
+}
+
+namespace {
+__global__ void set_word_kernel(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+uint64_t fnv1a(uint64_t h, const void* data, size_t n) {
+    const uint8_t* p = (const uint8_t*)data;
+    for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+    return h;
+}
+}  // namespace
+
+int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_render_view* views,
+                         uint32_t view_count, const rtx_render_params* params, float* films_device,
+                         double* accum_rgb_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!views || view_count == 0 || !params) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "no views or params");
+    if (view_count > (1u << 16)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "more than 65536 views");
+    uint64_t npix = 0;
+    std::vector<rtx::BatchView> bv(view_count);
+    for (uint32_t v = 0; v < view_count; ++v) {
+        const rtx_render_view& w = views[v];
+        if (w.width == 0 || w.height == 0) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "view %u: empty film", v);
+        if ((uint64_t)w.light_first + w.light_count > params->light_triangle_count)
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "view %u: lights out of range", v);
+        for (int k = 0; k < 3; ++k)
+            if (!(w.clip_min[k] <= w.clip_max[k]))
+                return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "view %u: empty clip box", v);
+        rtx::BatchView& b = bv[v];
+        for (int k = 0; k < 3; ++k) {
+            b.cam[k] = w.camera.position[k];
+            b.lo[k] = w.clip_min[k];
+            b.hi[k] = w.clip_max[k];
+        }
+        const double rad = (double)w.camera.y_fov_degree * (3.14159265358979323846 / 180.0);
+        b.ay = (float)std::tan(rad * 0.5);
+        const float t = b.ay * (float)w.width;
+        b.ax = t / (float)w.height;
+        b.width = w.width;
+        b.height = w.height;
+        b.pix_base = (uint32_t)npix;
+        b.light_first = w.light_first;
+        b.light_count = w.light_count;
+        b.cdf_first = w.light_first + v;
+        npix += (uint64_t)w.width * w.height;
+        if (npix > (1ull << 28)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "more than 2^28 pixels in a batch");
+    }
+    // render_setup sizes the queues for the whole batch (one CDF entry more per view); the views carry the cameras
+    rtx_render_params p = *params;
+    p.width = (uint32_t)npix;
+    p.height = 1;
+    static const rtx_camera kViewsCarryCameras = {{0.0f, 0.0f, 0.0f}, 60.0f};
+    rtx::PtArgs a;
+    rtx::TraceLaunch L;
+    int r = render_setup(ctx, tlas_address, sbt, &kViewsCarryCameras, &p, films_device || accum_rgb_device, a, L, 0,
+                         view_count - 1);
+    if (r) return r;
+    // two view slots: the recorded graph's views, and those of direct calls (which must not disturb the graph)
+    const uint64_t vbytes = view_count * sizeof(rtx::BatchView);
+    const uint64_t vslot = (vbytes + 255) & ~255ull;
+    if (2 * vslot > ctx->views_bytes) {
+        if (ctx->views_buf) (void)hipFree(ctx->views_buf);
+        ctx->views_buf = nullptr;
+        ctx->views_bytes = 0;
+        ctx->graph_key = 0;
+        if (hipMalloc(&ctx->views_buf, 2 * vslot) != hipSuccess)
+            return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "batch views");
+        ctx->views_bytes = 2 * vslot;
+    }
+    const bool graph = (params->flags & RTX_RENDER_GRAPH_BIT) != 0;
+    void* vdst = graph ? ctx->views_buf : (void*)((uint8_t*)ctx->views_buf + ctx->views_bytes / 2);
+    a.views = (const rtx::BatchView*)vdst;
+    a.nviews = view_count;
+    // key of the recorded sequence: everything the launches read, except sample_offset
+    uint64_t key = 1469598103934665603ull;
+    key = fnv1a(key, &tlas_address, sizeof tlas_address);
+    key = fnv1a(key, sbt, sizeof *sbt);
+    key = fnv1a(key, bv.data(), vbytes);
+    rtx_render_params kp = *params;
+    kp.sample_offset = 0;
+    key = fnv1a(key, &kp, sizeof kp);
+    key = fnv1a(key, &films_device, sizeof films_device);
+    key = fnv1a(key, &accum_rgb_device, sizeof accum_rgb_device);
+    key = fnv1a(key, &ctx->cfg, sizeof ctx->cfg);
+    key = fnv1a(key, &ctx->pt_scratch, sizeof ctx->pt_scratch);
+    key = fnv1a(key, &ctx->gstack, sizeof ctx->gstack);
+    key = fnv1a(key, &ctx->views_buf, sizeof ctx->views_buf);
+    hipError_t e;
+    if (!graph) {
+        if ((e = hipMemcpyAsync(vdst, bv.data(), vbytes, hipMemcpyHostToDevice, ctx->stream)))
+            return hip_fail(ctx, e, "rtx_cmd_render_batch views");
+        e = rtx::pt_render_batch(a, L, params->spp, params->sample_offset, films_device, accum_rgb_device,
+                                 ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render_batch");
+        return RTX_SUCCESS;
+    }
+    if (!ctx->sample_word) {
+        if (hipMalloc(&ctx->sample_word, 256) != hipSuccess)
+            return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "graph sample word");
+    }
+    if (!ctx->capture_stream) {
+        if ((e = hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking)))
+            return hip_fail(ctx, e, "rtx_cmd_render_batch capture stream");
+    }
+    if (!ctx->graph || ctx->graph_key != key) {
+        if ((e = hipMemcpyAsync(vdst, bv.data(), vbytes, hipMemcpyHostToDevice, ctx->stream)))
+            return hip_fail(ctx, e, "rtx_cmd_render_batch views");
+        if (ctx->graph) {
+            (void)hipGraphExecDestroy(ctx->graph);
+            ctx->graph = nullptr;
+        }
+        ctx->graph_key = 0;
+        a.sample_dev = ctx->sample_word;
+        hipGraph_t g = nullptr;
+        if ((e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal)))
+            return hip_fail(ctx, e, "rtx_cmd_render_batch capture");
+        hipError_t le = rtx::pt_render_batch(a, L, params->spp, 0u, films_device, accum_rgb_device,
+                                             ctx->capture_stream);
+        e = hipStreamEndCapture(ctx->capture_stream, &g);
+        if (le != hipSuccess || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(ctx, le != hipSuccess ? le : e, "rtx_cmd_render_batch capture");
+        }
+        e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            ctx->graph = nullptr;
+            return hip_fail(ctx, e, "rtx_cmd_render_batch instantiate");
+        }
+        ctx->graph_key = key;
+    }
+    set_word_kernel<<<1, 64, 0, ctx->stream>>>(ctx->sample_word, params->sample_offset);
+    if ((e = hipGetLastError()) || (e = hipGraphLaunch(ctx->graph, ctx->stream)))
+        return hip_fail(ctx, e, "rtx_cmd_render_batch launch");
     return RTX_SUCCESS;
 }
 
@@ -909,6 +1057,20 @@ int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* ra
     return autotune_core(
         ctx, "rtx_autotune_trace", reps,
         [&]() { return rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0xFFu, hits, occl); },
+        out_cfg, out_ms_per_launch);
+}
+
+int rtx_autotune_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt,
+                              const rtx_render_view* views, uint32_t view_count, const rtx_render_params* params,
+                              float* films_device, uint32_t reps, rtx_trace_config* out_cfg,
+                              float* out_ms_per_launch) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (!films_device || !params) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "autotuning needs films and params");
+    rtx_render_params p = *params;
+    p.flags &= ~RTX_RENDER_GRAPH_BIT;
+    return autotune_core(
+        ctx, "rtx_autotune_render_batch", reps,
+        [&]() { return rtx_cmd_render_batch(ctx, tlas_address, sbt, views, view_count, &p, films_device, nullptr); },
         out_cfg, out_ms_per_launch);
 }
 

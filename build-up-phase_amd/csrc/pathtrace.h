@@ -11,6 +11,19 @@ namespace rtx {
 
 enum PtMode : uint32_t { kPtPlain = 0, kPtCache = 2 };
 
+// One view of a batched render (configuration 4): its camera, film, light range and the box its rays are clipped
+// to (the scenes of a batch sit in disjoint boxes of one TLAS).
+struct BatchView {
+    float cam[3];
+    float ax, ay;
+    uint32_t width, height;
+    uint32_t pix_base;     // first path slot / film pixel of the view
+    uint32_t light_first;  // light triangles [light_first, light_first + light_count)
+    uint32_t light_count;
+    uint32_t cdf_first;    // its light CDF: cdf[cdf_first .. cdf_first + light_count]
+    float lo[3], hi[3];
+};
+
 struct PtArgs {
     const void* tlas;
     uint32_t width, height, npix;
@@ -47,12 +60,21 @@ struct PtArgs {
     uint32_t* cq_pid;
     float* cq_pred;
     float4* rec;        // training vertex records, [(d-1) * ntrain + (pid - npix)] x 5 float4
+    // batched render (views != nullptr)
+    const BatchView* views;
+    uint32_t nviews;
+    uint32_t* path_view;       // view of each path slot
+    const uint32_t* sample_dev;  // when set, added to every pass's sample index (graph replays)
 };
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights);
 void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights);
 hipError_t pt_render(PtArgs a, const TraceLaunch& launch, uint32_t spp, uint32_t sample_offset, float* film,
                      double* accum64, hipStream_t stream);
+// batched render: a.views / a.nviews set, a.npix = total pixels, films concatenated in view order; no host
+// synchronisation, so the whole call can be captured into a graph
+hipError_t pt_render_batch(PtArgs a, const TraceLaunch& launch, uint32_t spp, uint32_t sample_offset, float* films,
+                           double* accum64, hipStream_t stream);
 uint64_t pt_train_bytes(uint32_t train_paths, uint32_t record_depths);
 // vertices per training path that become samples
 uint32_t nrc_record_depths(uint32_t max_depth, uint32_t cache_depth, uint32_t train_suffix);

@@ -66,6 +66,46 @@ __device__ __forceinline__ void block_push3(uint32_t* c0, uint32_t* c1, uint32_t
 // counts[] slot of the training-tail query queue (past the per-depth extension / shadow counts)
 __host__ __device__ __forceinline__ uint32_t tail_counter(const PtArgs& a) { return 3u * a.max_depth + 2u; }
 
+__device__ __forceinline__ uint32_t view_of(const PtArgs& a, uint32_t pid) {
+    uint32_t lo = 0, hi = a.nviews - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (a.views[mid].pix_base <= pid) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Distance along d from o (inside the box) to the exit of the view's clip box.
+__device__ __forceinline__ float clip_exit(const BatchView& v, const float* o, const float* d) {
+    float t = INFINITY;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (d[k] != 0.0f) t = fminf(t, ((d[k] > 0.0f ? v.hi[k] : v.lo[k]) - o[k]) / d[k]);
+    return fmaxf(t, 0.0f);
+}
+
+__device__ __forceinline__ uint32_t pass_sample(const PtArgs& a, uint32_t sample) {
+    return a.sample_dev ? sample + *a.sample_dev : sample;
+}
+
+// Per-view light CDFs of a batched render (one thread per view).
+__global__ void light_cdf_batch_kernel(const BatchView* __restrict__ views, uint32_t nviews,
+                                       const float* __restrict__ lights, float* __restrict__ cdf) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nviews) return;
+    const BatchView bv = views[v];
+    float acc = 0.0f;
+    for (uint32_t i = 0; i < bv.light_count; ++i) {
+        const float* L = lights + 12 * (uint64_t)(bv.light_first + i);
+        const float e1x = L[3] - L[0], e1y = L[4] - L[1], e1z = L[5] - L[2];
+        const float e2x = L[6] - L[0], e2y = L[7] - L[1], e2z = L[8] - L[2];
+        const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+        acc += 0.5f * sqrtf(cx * cx + cy * cy + cz * cz);
+        cdf[bv.cdf_first + i] = acc;
+    }
+    cdf[bv.cdf_first + bv.light_count] = acc;
+}
+
 __global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, float* __restrict__ cdf) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     float acc = 0.0f;
@@ -81,7 +121,12 @@ __global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, f
 }
 
 __device__ __forceinline__ void path_xy(const PtArgs& a, uint32_t pid, uint32_t& x, uint32_t& y) {
-    if (pid >= a.npix) {  // training path
+    if (a.views) {  // batched render: pixel of the path's view
+        const BatchView& v = a.views[a.path_view[pid]];
+        const uint32_t local = pid - v.pix_base;
+        x = local % v.width;
+        y = local / v.width;
+    } else if (pid >= a.npix) {  // training path
         x = pid - a.npix;
         y = 0x7FFFFFFFu;
     } else {
@@ -94,8 +139,36 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) a.counts[0] = a.npaths;
     if (i >= a.npaths) return;
+    sample = pass_sample(a, sample);
     uint32_t px, py;
     uint4 j;
+    if (a.views) {  // batched render
+        const uint32_t vi = view_of(a, i);
+        a.path_view[i] = vi;
+        const BatchView& v = a.views[vi];
+        const uint32_t local = i - v.pix_base;
+        px = local % v.width;
+        py = local / v.width;
+        j = pcg4d(px, py, sample, a.seed ^ kJitterKey);
+        const float sx = ((float)px + unit_float(j.x)) / (float)v.width;
+        const float sy = ((float)py + unit_float(j.y)) / (float)v.height;
+        const float nx = sx * 2.0f - 1.0f;
+        const float ny = sy * 2.0f - 1.0f;
+        rtx_ray r;
+        r.origin[0] = v.cam[0];
+        r.origin[1] = v.cam[1];
+        r.origin[2] = v.cam[2];
+        r.tmin = 0.0f;
+        r.direction[0] = nx * v.ax;
+        r.direction[1] = -(ny * v.ay);
+        r.direction[2] = -1.0f;
+        r.tmax = clip_exit(v, r.origin, r.direction);
+        a.ext_rays[0][i] = r;
+        a.ext_pid[0][i] = i;
+        a.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        a.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     if (i >= a.npix) {  // training path: a random pixel
         const uint4 h = pcg4d(i - a.npix, sample, a.seed, kTrainKey);
         const uint32_t pix = h.x % a.npix;
@@ -132,6 +205,7 @@ template <int MODE>
 __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, uint32_t sample, int cur) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = a.counts[3 * depth];
+    sample = pass_sample(a, sample);
     bool push_ext = false, push_sh = false, push_cq = false;
     rtx_ray ext{}, sh{};
     rtx_nrc_query cq{};
@@ -197,16 +271,20 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
                 path_xy(a, pid, x, y);
                 const uint4 ra = pcg4d(x, y, sample, a.seed + 2u * depth);
                 const uint4 rb = pcg4d(x, y, sample, a.seed + 2u * depth + 1u);
-                if (reflective && a.nlights > 0) {
+                const BatchView* bv = a.views ? a.views + a.path_view[pid] : nullptr;
+                const uint32_t nl = bv ? bv->light_count : a.nlights;
+                const float* cdf = bv ? a.cdf + bv->cdf_first : a.cdf;
+                const float* lights = bv ? a.lights + 12 * (uint64_t)bv->light_first : a.lights;
+                if (reflective && nl > 0) {
                     // next-event estimation: light triangle by area, point uniform on it
-                    const float total = a.cdf[a.nlights];
+                    const float total = cdf[nl];
                     const float ut = unit_float(ra.x) * total;
-                    uint32_t lo = 0, hi = a.nlights - 1;
+                    uint32_t lo = 0, hi = nl - 1;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi) >> 1;
-                        if (a.cdf[mid] > ut) hi = mid; else lo = mid + 1;
+                        if (cdf[mid] > ut) hi = mid; else lo = mid + 1;
                     }
-                    const float* L = a.lights + 12 * (uint64_t)lo;
+                    const float* L = lights + 12 * (uint64_t)lo;
                     const float su = sqrtf(unit_float(ra.y));
                     const float b0 = 1.0f - su, b1 = unit_float(ra.z) * su, b2 = 1.0f - b0 - b1;
                     float lp[3], e1[3], e2[3];
@@ -265,7 +343,7 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
                         ext.direction[0] = dir[0];
                         ext.direction[1] = dir[1];
                         ext.direction[2] = dir[2];
-                        ext.tmax = INFINITY;
+                        ext.tmax = bv ? clip_exit(*bv, ext.origin, ext.direction) : INFINITY;
                         push_ext = true;
                     }
                 }
@@ -443,6 +521,7 @@ uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
     b += 3 * align_up(n * 16, 256);                                           // thr, rad, accum
     b += align_up((3ull * (max_depth + 2)) * 4, 256);                          // counts
     b += align_up((nlights + 1ull) * 4, 256);                                 // light cdf
+    b += align_up(n * 4, 256);                                                // path view
     b += align_up(n * sizeof(rtx_nrc_query), 256) + align_up(n * 4, 256) + align_up(n * 12, 256);  // cache queries
     return b;
 }
@@ -469,6 +548,7 @@ void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint
     a.accum = (float4*)take(n * 16);
     a.counts = (uint32_t*)take(3ull * (max_depth + 2) * 4);
     a.cdf = (float*)take((nlights + 1ull) * 4);
+    a.path_view = (uint32_t*)take(n * 4);
     a.cq = (rtx_nrc_query*)take(n * sizeof(rtx_nrc_query));
     a.cq_pid = (uint32_t*)take(n * 4);
     a.cq_pred = (float*)take(n * 12);
@@ -545,6 +625,23 @@ hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t samp
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
     if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, film);
+    return hipGetLastError();
+}
+
+hipError_t pt_render_batch(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t sample_offset, float* films,
+                           double* acc64, hipStream_t s) {
+    const unsigned gb = (a.npix + 255) / 256;
+    a.mode = kPtPlain;
+    a.npaths = a.npix;
+    a.ntrain = 0;
+    light_cdf_batch_kernel<<<(a.nviews + 63) / 64, 64, 0, s>>>(a.views, a.nviews, a.lights, a.cdf);
+    fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
+    hipError_t e;
+    for (uint32_t si = 0; si < spp; ++si) {
+        if ((e = run_pass(a, L, sample_offset + si, a.max_depth - 1, s))) return e;
+        accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
+    }
+    if (films) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, films);
     return hipGetLastError();
 }
 

@@ -98,6 +98,15 @@ class RenderParams(ctypes.Structure):
                 ("ray_offset", ctypes.c_float)]
 
 
+class RenderView(ctypes.Structure):
+    _fields_ = [("camera", Camera), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("light_first", ctypes.c_uint32), ("light_count", ctypes.c_uint32),
+                ("clip_min", ctypes.c_float * 3), ("clip_max", ctypes.c_float * 3)]
+
+
+RTX_RENDER_GRAPH_BIT = 0x1
+
+
 class BlasBuildInfo(ctypes.Structure):
     _fields_ = [("geometries", ctypes.c_void_p), ("geometry_count", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
                 ("dst_device", ctypes.c_void_p), ("dst_size", ctypes.c_uint64)]
@@ -166,6 +175,11 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_float)]),
     "rtx_get_blas_batch_build_sizes": (ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.POINTER(_U64)]),
     "rtx_cmd_build_blas_batch": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _U64, _P]),
+    "rtx_cmd_render_batch": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), _P, _U32, ctypes.POINTER(RenderParams),
+                                            _P, _P]),
+    "rtx_autotune_render_batch": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), _P, _U32,
+                                                 ctypes.POINTER(RenderParams), _P, _U32, ctypes.POINTER(TraceConfig),
+                                                 ctypes.POINTER(ctypes.c_float)]),
     "rtx_debug_treelet": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
 }
 
@@ -412,6 +426,52 @@ class Context:
                                             ctypes.byref(p), film.data_ptr(),
                                             accum64.data_ptr() if accum64 is not None else None))
         return film
+
+    @staticmethod
+    def _views(views):
+        k = len(views)
+        arr = (RenderView * k)()
+        npix = 0
+        for i, v in enumerate(views):
+            arr[i].camera = Camera((ctypes.c_float * 3)(*v["camera"][:3]), float(v["camera"][3]))
+            arr[i].width, arr[i].height = v["width"], v["height"]
+            arr[i].light_first, arr[i].light_count = v["light_first"], v["light_count"]
+            arr[i].clip_min = (ctypes.c_float * 3)(*v["clip_min"])
+            arr[i].clip_max = (ctypes.c_float * 3)(*v["clip_max"])
+            npix += v["width"] * v["height"]
+        return arr, npix
+
+    def render_batch(self, tlas: AccelerationStructure, records, views, lights, spp=1, max_depth=6, rr_depth=3,
+                     seed=1234, sample_offset=0, ray_offset=1e-4, graph=False, films=None, accum64=None):
+        """rtx_cmd_render_batch: views = list of dicts (camera, width, height, light_first, light_count, clip_min,
+        clip_max).  Returns the concatenated films [sum(w*h), 4] (view order)."""
+        torch = _torch()
+        sbt, cam, p = self._render_args(records, (0, 0, 0, 60.0), lights, 1, 1, spp, max_depth, rr_depth, seed,
+                                        sample_offset, ray_offset)
+        p.flags = RTX_RENDER_GRAPH_BIT if graph else 0
+        arr, npix = self._views(views)
+        k = len(views)
+        if films is None:
+            films = torch.empty((npix, 4), dtype=torch.float32, device=self.device)
+        self._check(self.lib.rtx_cmd_render_batch(self.handle, tlas.address, ctypes.byref(sbt),
+                                                  ctypes.cast(arr, _P), k, ctypes.byref(p), films.data_ptr(),
+                                                  accum64.data_ptr() if accum64 is not None else None))
+        return films
+
+    def autotune_render_batch(self, tlas: AccelerationStructure, records, views, lights, spp=1, max_depth=6,
+                              rr_depth=3, seed=1234, ray_offset=1e-4, reps=2):
+        """rtx_autotune_render_batch; the best configuration stays active.  Returns (config dict, ms per call)."""
+        torch = _torch()
+        sbt, cam, p = self._render_args(records, (0, 0, 0, 60.0), lights, 1, 1, spp, max_depth, rr_depth, seed,
+                                        1 << 28, ray_offset)
+        arr, npix = self._views(views)
+        films = torch.empty((npix, 4), dtype=torch.float32, device=self.device)
+        c = TraceConfig()
+        ms = ctypes.c_float()
+        self._check(self.lib.rtx_autotune_render_batch(self.handle, tlas.address, ctypes.byref(sbt),
+                                                       ctypes.cast(arr, _P), len(views), ctypes.byref(p),
+                                                       films.data_ptr(), reps, ctypes.byref(c), ctypes.byref(ms)))
+        return self.config(), ms.value
 
     def autotune_render(self, tlas: AccelerationStructure, records, camera, lights, width, height, spp=1,
                         max_depth=8, rr_depth=3, seed=1234, ray_offset=1e-4, reps=2):

@@ -434,3 +434,51 @@ def furnace_scene(albedo: float = 0.5, emission: float = 1.0, width: int = 64, h
                sbt_offset=np.zeros(1, np.uint32), flags=np.zeros(1, np.uint32), records=recs,
                camera=(0.5, 0.5, 0.9, 60.0), width=width, height=height, meta={"name": "furnace"})
     return sc, lights
+
+
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 768))
+
+
+def batch_scenes(count: int = 64, seed: int = 7, tri_lo: float = 1e3, tri_hi: float = 2e6, spacing: float = 3.0,
+                 tri_scale: float = 1.0):
+    """Uneven scenes for batched rendering, laid out as a scene atlas in one TLAS.
+
+    Scene k: a displaced icosphere (radius 0.3 at (0.5, 0.35, 0.5); triangle count log-uniform in [tri_lo, tri_hi]
+    x tri_scale, i.e. 20 f^2 with f = round(sqrt(T / 20))), a ground quad (y = 0, x and z in [-0.5, 1.5]) and one
+    area light (0.3 x 0.3 quad at y = 1.2, facing down, radiance 10); camera at (0.5, 0.55, 2.0) looking down -z
+    with a 45 deg vertical field of view; film size drawn from 256^2, 512^2, 1024x768.  Scene k's instance is
+    translated to (spacing * (k % 8), 0, spacing * (k // 8)); its clip box is the local box
+    [-0.6, 1.6] x [-0.05, 1.3] x [-0.6, 2.1] (a margin around the geometry) moved the same way.  Seeds: sizes and film sizes from `seed`, the
+    noise of icosphere k from seed * 1000 + k.
+    Returns (Scene, views: list of dicts for rtx_render_view, lights [2 * count, 12])."""
+    rng = np.random.default_rng(seed)
+    tris = np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi), count)) * tri_scale
+    res = [BATCH_RESOLUTIONS[i] for i in rng.integers(0, len(BATCH_RESOLUTIONS), count)]
+    albedo = rng.uniform(0.3, 0.8, (count, 3))
+    blas, transforms, recs, views = [], [], [], []
+    lights = np.zeros((2 * count, 12), np.float32)
+    for k in range(count):
+        f = max(1, int(round(np.sqrt(tris[k] / 20.0))))
+        sph = icosphere(f, radius=0.3, seed=seed * 1000 + k, amplitude=0.03, noise_freq=4.0, center=(0.5, 0.35, 0.5))
+        gv, gi = _quad((-0.5, 0, -0.5), (-0.5, 0, 1.5), (1.5, 0, 1.5), (1.5, 0, -0.5))      # normal +y
+        lv, li = _quad((0.35, 1.2, 0.35), (0.65, 1.2, 0.35), (0.65, 1.2, 0.65), (0.35, 1.2, 0.65))  # normal -y
+        blas.append([sph, Geometry(gv, gi), Geometry(lv, li)])
+        ox, oz = spacing * (k % 8), spacing * (k // 8)
+        transforms.append(translation(ox, 0.0, oz))
+        recs.append(hit_records([PROGRAM_LAMBERTIAN] * 3, [tuple(albedo[k]), (0.5, 0.5, 0.5), (0.0, 0.0, 0.0)],
+                                emissions=[(0, 0, 0), (0, 0, 0), (10.0, 10.0, 10.0)]))
+        off = np.array([ox, 0.0, oz], np.float32)
+        for t in range(2):
+            lights[2 * k + t, :9] = (lv[li[t]] + off).reshape(-1)
+            lights[2 * k + t, 9:] = 10.0
+        w, h = res[k]
+        views.append({"camera": (0.5 + ox, 0.55, 2.0 + oz, 45.0), "width": w, "height": h,
+                      "light_first": 2 * k, "light_count": 2,
+                      "clip_min": (-0.6 + ox, -0.05, -0.6 + oz), "clip_max": (1.6 + ox, 1.3, 2.1 + oz),
+                      "triangles": int(sph.indices.shape[0]) + 4})
+    sc = Scene(blas=blas, transforms=np.stack(transforms), blas_of_instance=np.arange(count),
+               custom_index=np.arange(count, dtype=np.uint32), mask=np.full(count, 0xFF, np.uint32),
+               sbt_offset=(3 * np.arange(count)).astype(np.uint32), flags=np.zeros(count, np.uint32),
+               records=np.concatenate(recs), camera=views[0]["camera"], width=res[0][0], height=res[0][1],
+               meta={"name": "batch", "count": count, "seed": seed})
+    return sc, views, lights

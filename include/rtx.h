@@ -318,7 +318,7 @@ typedef struct rtx_render_params {
     uint32_t rr_depth;            /* Russian roulette when continuing from vertex index >= rr_depth (3) */
     uint32_t seed;                /* PCG seed (1234) */
     uint32_t sample_offset;       /* index of this call's first sample (progressive rendering) */
-    uint32_t flags;               /* reserved, 0 */
+    uint32_t flags;               /* RTX_RENDER_*_BIT (rtx_cmd_render_batch), else 0 */
     const float* light_triangles_device; /* 12 floats per light: v0, v1, v2 (world), emitted radiance rgb */
     uint32_t light_triangle_count;
     float ray_offset;             /* secondary-ray origin offset along the normal, scene units (1e-4) */
@@ -331,6 +331,36 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
                    const rtx_render_params* params, float* film_rgba32f_device, double* accum_rgb_device);
 
 /* Rays traced by rtx_cmd_render since the context was created (camera, extension and shadow rays). */
+/* ---- batched path tracing (configuration 4 of the north star; no counterpart in the reference) ----------------
+ *
+ * Many views of one TLAS rendered by one wavefront: every stage (camera rays, extension traversal, shading, shadow
+ * traversal, gathering) is one launch over the paths of all views, whatever their resolutions.  Scenes of
+ * different content share the TLAS as a scene atlas: each scene's instances sit in their own box, and every ray
+ * of a view is clipped to that view's box, so it only ever meets its own scene. */
+typedef struct rtx_render_view {
+    rtx_camera camera;            /* world space, raygen model of main.cpp:1033-1052 */
+    uint32_t width, height;
+    uint32_t light_first;         /* the view's lights: params->light_triangles_device[light_first ..] */
+    uint32_t light_count;
+    float clip_min[3];            /* the view's scene box */
+    float clip_max[3];
+} rtx_render_view;
+
+/* params: as rtx_render_params of rtx_cmd_render with width / height unused.  films: the views' RGBA32F films
+ * concatenated in view order (sum of width x height pixels); accum (optional): FP64 rgb, same order.
+ * With RTX_RENDER_GRAPH_BIT the call's launch sequence is recorded into a hipGraph the first time and replayed by
+ * later calls with identical arguments apart from sample_offset (which the graph reads from device memory); the
+ * graph is launched on the context stream. */
+#define RTX_RENDER_GRAPH_BIT 0x1u
+int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_render_view* views,
+                         uint32_t view_count, const rtx_render_params* params, float* films_device,
+                         double* accum_rgb_device);
+/* rtx_autotune_render for batched renders (times rtx_cmd_render_batch calls without the graph). */
+int rtx_autotune_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt,
+                              const rtx_render_view* views, uint32_t view_count, const rtx_render_params* params,
+                              float* films_device, uint32_t reps, rtx_trace_config* out_cfg,
+                              float* out_ms_per_launch);
+
 /* rtx_autotune_trace for the path tracer: times `reps` rtx_cmd_render calls with `params` per candidate
  * configuration (film written, no FP64 accumulation), keeps the fastest.  Blocks until done. */
 int rtx_autotune_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
