@@ -28,6 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "build-up-phase_amd"))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s closest-hit (coherent/incoherent, 20M tris); equal-relMSE time plain vs NRC"
+DEFAULT_CFG = {"block_size": 256, "blocks_per_cu": 4, "lds_stack_depth": 12, "ray_chunk": 1, "refill_lanes": 48,
+               "refill_interval": 4, "leaf_lanes": 8}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -45,6 +47,8 @@ def main():
     ap.add_argument("--no-anyhit", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing / NRC equal-relMSE part")
     ap.add_argument("--no-autotune", action="store_true", help="use the default traversal configuration")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene part")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched part")
     ap.add_argument("--no-query", action="store_true",
                     help="skip the 20M-triangle ray-query part (profiling the path-tracing part alone)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the FP64 reference render")
@@ -65,9 +69,11 @@ def main():
 
     if args.no_query:
         ctx = pyrtx.Context(0)
-        render = render_bench(ctx, args, np, torch, pyrtx, S)
-        print(json.dumps({"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "path_tracing": render}),
-              flush=True)
+        render = None if args.no_render else render_bench(ctx, args, np, torch, pyrtx, S)
+        ctx.set_config(**DEFAULT_CFG)
+        batch = None if args.no_batch else batch_bench(ctx, args, np, torch, pyrtx, S)
+        print(json.dumps({"metric": METRIC, "value": None, "unit": "Mrays/s", "n_gpus": 1, "path_tracing": render,
+                          "batched_scenes": batch}), flush=True)
         ctx.close()
         return
 
@@ -232,6 +238,8 @@ def main():
 
     ctx.set_config(**default_cfg)
     render = None if args.no_render else render_bench(ctx, args, np, torch, pyrtx, S)
+    ctx.set_config(**default_cfg)
+    batch = None if args.no_batch else batch_bench(ctx, args, np, torch, pyrtx, S)
 
     out = {
         "metric": METRIC,
@@ -259,6 +267,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "path_tracing": render,
+        "batched_scenes": batch,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
@@ -366,6 +375,90 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
         "equal_relmse_speedup": None if ms_nrc is None else round(ms_plain / ms_nrc, 2),
         "autotuned_trace_config": tuned,
     }
+
+
+def batch_bench(ctx, args, np, torch, pyrtx, S):
+    """Configuration 4: uneven scenes built by one batched BLAS build and rendered by one wavefront replayed from a
+    HIP graph, against the same scenes built and rendered one at a time."""
+    n, spp, depth = args.batch_scenes, 16, 6
+    t0 = time.time()
+    sc, views, lights = S.batch_scenes(count=n, seed=7)
+    tris = sum(v["triangles"] for v in views)
+    npix = sum(v["width"] * v["height"] for v in views)
+    log(f"batch: {n} scenes, {tris} triangles, {npix} pixels (generated in {time.time() - t0:.1f}s)")
+    dev = ctx.device
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    lists = []
+    for geoms in sc.blas:
+        lists.append([{"vertices": torch.from_numpy(g.vertices.astype(np.float32)).to(dev),
+                       "indices": torch.from_numpy(g.indices.astype(np.uint32).view(np.int32)).to(dev),
+                       "transform": None} for g in geoms])
+    torch.cuda.synchronize()
+    ctx.build_blas_batch(lists)  # warm-up (allocations)
+    ctx.sync()
+    a, b = ev(), ev()
+    a.record()
+    blases = ctx.build_blas_batch(lists)
+    b.record()
+    torch.cuda.synchronize()
+    ms_build_batch = a.elapsed_time(b)
+    a.record()
+    singles = [ctx.build_blas(g) for g in lists]
+    b.record()
+    torch.cuda.synchronize()
+    ms_build_seq = a.elapsed_time(b)
+    del singles
+    inst = pyrtx.instance_records(sc, [x.address for x in blases])
+    tlas = ctx.build_tlas(inst)
+    rec = torch.from_numpy(np.ascontiguousarray(sc.records).view(np.uint8).reshape(-1).copy()).to(dev)
+    lt = torch.from_numpy(np.ascontiguousarray(lights, np.float32).copy()).to(dev)
+    ctx.sync()
+    log(f"batch BLAS builds: batched {ms_build_batch:.1f} ms, one by one {ms_build_seq:.1f} ms")
+    tuned = None
+    if not args.no_autotune:
+        tuned, tms = ctx.autotune_render_batch(tlas, rec, views, lt, spp=1, max_depth=depth)
+        log(f"autotuned batched render: {tuned} ({tms:.1f} ms per 1-spp batch)")
+    films = ctx.render_batch(tlas, rec, views, lt, spp=spp, max_depth=depth, sample_offset=1 << 20, graph=True)
+    ctx.sync()
+    reps = 3
+    r0 = ctx.render_ray_count()
+    a.record()
+    for k in range(reps):
+        ctx.render_batch(tlas, rec, views, lt, spp=spp, max_depth=depth, sample_offset=spp * k, graph=True,
+                         films=films)
+    b.record()
+    torch.cuda.synchronize()
+    ms_graph = a.elapsed_time(b) / reps
+    rays = (ctx.render_ray_count() - r0) // reps
+    a.record()
+    ctx.render_batch(tlas, rec, views, lt, spp=spp, max_depth=depth, films=films)
+    b.record()
+    torch.cuda.synchronize()
+    ms_direct = a.elapsed_time(b)
+    # the same views one at a time, each scene in a TLAS of its own
+    tl1 = [ctx.build_tlas(inst[k:k + 1]) for k in range(n)]
+    outs = [torch.empty((v["height"], v["width"], 4), dtype=torch.float32, device=dev) for v in views]
+    ctx.sync()
+    a.record()
+    for k, v in enumerate(views):
+        ctx.render(tl1[k], rec, v["camera"], lt[v["light_first"]:v["light_first"] + v["light_count"]], v["width"],
+                   v["height"], spp=spp, max_depth=depth, film=outs[k])
+    b.record()
+    torch.cuda.synchronize()
+    ms_seq = a.elapsed_time(b)
+    log(f"batch render {spp} spp: graph {ms_graph:.1f} ms ({rays / ms_graph / 1e3:.0f} Mrays/s), direct "
+        f"{ms_direct:.1f} ms, one view at a time {ms_seq:.1f} ms")
+    return {"workload": f"{n} scenes (seed 7): displaced icosphere + ground quad + area light each, triangles "
+                        f"log-uniform in [1e3, 2e6], films from 256^2 / 512^2 / 1024x768; {spp} spp, depth {depth}",
+            "scenes": n, "triangles": int(tris), "pixels": int(npix),
+            "build_ms": {"batched": round(ms_build_batch, 2), "one_by_one": round(ms_build_seq, 2)},
+            "render_ms": {"batched_graph": round(ms_graph, 1), "batched_direct": round(ms_direct, 1),
+                          "one_view_at_a_time": round(ms_seq, 1)},
+            "rays_per_frame": int(rays), "mrays_per_s": round(rays / ms_graph / 1e3, 1),
+            "autotuned_trace_config": tuned}
 
 
 if __name__ == "__main__":
