@@ -20,6 +20,7 @@
 #include "pathtrace.h"
 #include "rtx_internal.h"
 #include "trace.h"
+#include "util.h"
 
 struct rtx_context_t {
     int device = 0;
@@ -558,6 +559,48 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
     L = launch_shape(ctx);
     if ((r = ensure_gstack(ctx, L.grid))) return r;
     a.gstack = ctx->gstack;
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_film_to_rgba8(rtx_context ctx, const float* film_rgba32f_device, uint32_t width, uint32_t height,
+                          uint32_t flags, uint8_t* out_rgba8_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    const uint64_t npix = (uint64_t)width * height;
+    if (npix > 0 && (!film_rgba32f_device || !out_rgba8_device))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "film or output is NULL");
+    if (!aligned(film_rgba32f_device, 16) || !aligned(out_rgba8_device, 4))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "film must be 16-B and output 4-B aligned");
+    if (flags & ~(RTX_FILM_SRGB_BIT | RTX_FILM_SWAP_RB_BIT | RTX_FILM_OPAQUE_BIT))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "unknown film flags");
+    hipError_t e = rtx::film_to_rgba8(film_rgba32f_device, npix, flags, out_rgba8_device, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_film_to_rgba8");
+    return RTX_SUCCESS;
+}
+
+int rtx_get_as_compacted_size(rtx_context ctx, uint64_t as_address, uint64_t* out_bytes) {
+    if (!ctx || !out_bytes) return RTX_ERROR_INVALID_ARGUMENT;
+    int r = rtx_sync(ctx);
+    if (r) return r;
+    rtx::AsHeader h;
+    hipError_t e = hipMemcpy(&h, (const void*)(uintptr_t)as_address, sizeof h, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_as_compacted_size");
+    if (h.magic != rtx::kMagicBlas && h.magic != rtx::kMagicTlas)
+        return fail(ctx, RTX_ERROR_INVALID_AS, "not an acceleration structure");
+    *out_bytes = rtx::compacted_bytes(h);
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_copy_as(rtx_context ctx, uint64_t src_address, uint32_t mode, void* dst_device, uint64_t dst_size,
+                    uint64_t* out_address) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    if (mode != RTX_COPY_MODE_CLONE && mode != RTX_COPY_MODE_COMPACT)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "mode must be CLONE or COMPACT");
+    if (!src_address || (src_address & 255) || !dst_device || !aligned(dst_device, 256))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "source and destination must be 256-B aligned");
+    hipError_t e = rtx::copy_as((const void*)(uintptr_t)src_address, dst_device, dst_size,
+                                mode == RTX_COPY_MODE_COMPACT, &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_copy_as");
+    if (out_address) *out_address = (uint64_t)(uintptr_t)dst_device;
     return RTX_SUCCESS;
 }
 

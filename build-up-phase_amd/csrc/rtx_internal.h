@@ -172,6 +172,6 @@ __host__ __device__ inline uint64_t spread21(uint64_t v) {
     return v;
 }
 
-inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+__host__ __device__ inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 }  // namespace rtx

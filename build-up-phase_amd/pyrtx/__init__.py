@@ -39,7 +39,7 @@ RTX_RAY_FLAG_OPAQUE = 0x1
 MISS = 0xFFFFFFFF
 
 ERRORS = {-1: "OUT_OF_HOST_MEMORY", -2: "OUT_OF_DEVICE_MEMORY", -3: "INITIALIZATION_FAILED", -4: "DEVICE_LOST",
-          -1000: "INVALID_ARGUMENT", -1001: "STACK_OVERFLOW", -1002: "INVALID_AS"}
+          -1000: "INVALID_ARGUMENT", -1001: "STACK_OVERFLOW", -1002: "INVALID_AS", -1003: "FORMAT"}
 
 
 class RtxError(RuntimeError):
@@ -105,6 +105,17 @@ class RenderView(ctypes.Structure):
 
 
 RTX_RENDER_GRAPH_BIT = 0x1
+RTX_FILM_SRGB_BIT, RTX_FILM_SWAP_RB_BIT, RTX_FILM_OPAQUE_BIT = 0x1, 0x2, 0x4
+RTX_FILM_REFERENCE_BYTES = RTX_FILM_SWAP_RB_BIT
+RTX_COPY_MODE_CLONE, RTX_COPY_MODE_COMPACT = 0, 1
+RTX_ERROR_FORMAT = -1003
+
+
+class ObjMesh(ctypes.Structure):
+    _fields_ = [("vertex_count", ctypes.c_uint32), ("triangle_count", ctypes.c_uint32),
+                ("geometry_count", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("vertices", ctypes.POINTER(ctypes.c_float)), ("indices", ctypes.POINTER(ctypes.c_uint32)),
+                ("geometry_first_triangle", ctypes.POINTER(ctypes.c_uint32))]
 
 
 class BlasBuildInfo(ctypes.Structure):
@@ -180,6 +191,12 @@ SIGNATURES = {
     "rtx_autotune_render_batch": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), _P, _U32,
                                                  ctypes.POINTER(RenderParams), _P, _U32, ctypes.POINTER(TraceConfig),
                                                  ctypes.POINTER(ctypes.c_float)]),
+    "rtx_cmd_film_to_rgba8": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P]),
+    "rtx_get_as_compacted_size": (ctypes.c_int, [_P, _U64, ctypes.POINTER(_U64)]),
+    "rtx_cmd_copy_as": (ctypes.c_int, [_P, _U64, _U32, _P, _U64, ctypes.POINTER(_U64)]),
+    "rtx_obj_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ObjMesh)]),
+    "rtx_obj_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ObjMesh)]),
+    "rtx_obj_free": (None, [ctypes.POINTER(ObjMesh)]),
     "rtx_debug_treelet": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
 }
 
@@ -300,6 +317,36 @@ class Context:
                                                 scratch.numel(), ctypes.byref(addr)))
         n = sum(g["indices"].numel() // 3 for g in geoms)
         return AccelerationStructure(storage, addr.value, n, (scratch, tuple(geoms)))
+
+    def build_blas_obj(self, vertices, indices, first, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
+        """One BLAS over an OBJ mesh (load_obj output): one geometry per OBJ group, all sharing the vertex and index
+        buffers through primitive_offset."""
+        torch = _torch()
+        dv = _dev_tensor(np.ascontiguousarray(vertices, np.float32), self.device)
+        di = _dev_tensor(np.ascontiguousarray(indices, np.uint32).view(np.int32), self.device)
+        g = len(first) - 1
+        arr = (BlasGeometry * max(1, g))()
+        for k in range(g):
+            arr[k].vertex_data_device = dv.data_ptr()
+            arr[k].vertex_stride = 12
+            arr[k].max_vertex = max(0, vertices.shape[0] - 1)
+            arr[k].index_type = RTX_INDEX_TYPE_UINT32
+            arr[k].index_data_device = di.data_ptr()
+            arr[k].transform_data_device = None
+            arr[k].primitive_count = int(first[k + 1] - first[k])
+            arr[k].primitive_offset = int(first[k]) * 12
+            arr[k].first_vertex = 0
+            arr[k].transform_offset = 0
+            arr[k].flags = RTX_GEOMETRY_OPAQUE_BIT
+        sz = BuildSizes()
+        self._check(self.lib.rtx_get_blas_build_sizes(self.handle, ctypes.cast(arr, _P), g, flags, ctypes.byref(sz)))
+        storage = torch.empty(int(sz.acceleration_structure_size), dtype=torch.uint8, device=self.device)
+        scratch = torch.empty(int(sz.build_scratch_size), dtype=torch.uint8, device=self.device)
+        addr = ctypes.c_uint64()
+        self._check(self.lib.rtx_cmd_build_blas(self.handle, ctypes.cast(arr, _P), g, flags, storage.data_ptr(),
+                                                storage.numel(), scratch.data_ptr(), scratch.numel(),
+                                                ctypes.byref(addr)))
+        return AccelerationStructure(storage, addr.value, int(indices.shape[0]), (scratch, dv, di))
 
     def build_blas_batch(self, geom_lists, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
         """rtx_cmd_build_blas_batch: one build pass over several BLASes (list of geometry lists as for
@@ -549,6 +596,29 @@ class Context:
                                                 reps, ctypes.byref(c), ctypes.byref(ms)))
         return self.config(), ms.value
 
+    def film_to_rgba8(self, film, flags=RTX_FILM_REFERENCE_BYTES):
+        """rtx_cmd_film_to_rgba8: device float32 film [H, W, 4] -> device uint8 [H, W, 4]."""
+        torch = _torch()
+        h, w = film.shape[0], film.shape[1]
+        out = torch.empty((h, w, 4), dtype=torch.uint8, device=self.device)
+        self._check(self.lib.rtx_cmd_film_to_rgba8(self.handle, film.data_ptr(), w, h, flags, out.data_ptr()))
+        return out
+
+    def compacted_size(self, a: AccelerationStructure) -> int:
+        n = ctypes.c_uint64()
+        self._check(self.lib.rtx_get_as_compacted_size(self.handle, a.address, ctypes.byref(n)))
+        return n.value
+
+    def copy_as(self, a: AccelerationStructure, mode=RTX_COPY_MODE_COMPACT) -> AccelerationStructure:
+        """rtx_cmd_copy_as into new storage (compacted size for COMPACT, the source's size for CLONE)."""
+        torch = _torch()
+        size = self.compacted_size(a) if mode == RTX_COPY_MODE_COMPACT else a.storage.numel()
+        dst = torch.empty(size, dtype=torch.uint8, device=self.device)
+        addr = ctypes.c_uint64()
+        self._check(self.lib.rtx_cmd_copy_as(self.handle, a.address, mode, dst.data_ptr(), size,
+                                             ctypes.byref(addr)))
+        return AccelerationStructure(dst, addr.value, a.count, a.keepalive)
+
     # -- verification hooks ----------------------------------------------------------------------------------------
     def debug_lbvh(self, boxes, bits=30):
         """boxes: device float32 [n, 6].  Returns (sorted codes, order, left, right) as numpy arrays."""
@@ -671,6 +741,29 @@ class Nrc:
         loss = ctypes.c_float()
         self.ctx._check(self.ctx.lib.rtx_nrc_get_status(self.handle, ctypes.byref(st), ctypes.byref(loss)))
         return st.value, loss.value
+
+
+def load_obj(path=None, text=None):
+    """rtx_obj_load / rtx_obj_parse (host only, no GPU needed).  Returns (vertices float32 [V, 3], indices uint32
+    [T, 3], first uint32 [G + 1]) with geometry g = triangles first[g]:first[g + 1]."""
+    lib = load_library()
+    m = ObjMesh()
+    if path is not None:
+        code = lib.rtx_obj_load(os.fsencode(path), ctypes.byref(m))
+    else:
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        code = lib.rtx_obj_parse(data, len(data), ctypes.byref(m))
+    if code != RTX_SUCCESS:
+        raise RtxError(code, "OBJ " + (str(path) if path is not None else "text"))
+    try:
+        v = np.ctypeslib.as_array(m.vertices, (m.vertex_count * 3,)).reshape(-1, 3).copy() if m.vertex_count \
+            else np.zeros((0, 3), np.float32)
+        i = np.ctypeslib.as_array(m.indices, (m.triangle_count * 3,)).reshape(-1, 3).copy() if m.triangle_count \
+            else np.zeros((0, 3), np.uint32)
+        f = np.ctypeslib.as_array(m.geometry_first_triangle, (m.geometry_count + 1,)).copy()
+    finally:
+        lib.rtx_obj_free(ctypes.byref(m))
+    return v.astype(np.float32), i.astype(np.uint32), f.astype(np.uint32)
 
 
 INSTANCE_DTYPE = np.dtype([("xf", np.float32, (12,)), ("cm", np.uint32), ("sf", np.uint32), ("addr", np.uint64)])

@@ -44,7 +44,8 @@ typedef enum rtx_result {
     RTX_ERROR_DEVICE_LOST = -4,             /* VK_ERROR_DEVICE_LOST: a HIP call failed */
     RTX_ERROR_INVALID_ARGUMENT = -1000,     /* bad pointer, size, count or flag */
     RTX_ERROR_STACK_OVERFLOW = -1001,       /* traversal stack exceeded the built depth bound */
-    RTX_ERROR_INVALID_AS = -1002            /* address is not an acceleration structure of this library */
+    RTX_ERROR_INVALID_AS = -1002,           /* address is not an acceleration structure of this library */
+    RTX_ERROR_FORMAT = -1003                /* malformed input file (rtx_obj_*) */
 } rtx_result;
 
 typedef struct rtx_context_t* rtx_context;
@@ -170,6 +171,37 @@ typedef struct rtx_as_info {
                                       surface areas / root surface area (0 for fewer than 2 primitives) */
 } rtx_as_info;
 int rtx_get_as_info(rtx_context ctx, uint64_t as_address, rtx_as_info* out_info);
+
+/* vkCmdWriteAccelerationStructuresPropertiesKHR(COMPACTED_SIZE) + vkCmdCopyAccelerationStructureKHR (not used by
+ * the reference, which builds with PREFER_FAST_TRACE only, main.cpp:751).  The compacted size keeps the header
+ * and the used 64-B slots (plus a TLAS's instance table); it blocks until the build is done.  A copy is enqueued
+ * on the stream and reads the source header on the device.  A TLAS refers to BLASes by address: after
+ * compacting a BLAS, build the TLAS again with the new address (as in Vulkan). */
+#define RTX_COPY_MODE_CLONE 0u
+#define RTX_COPY_MODE_COMPACT 1u
+int rtx_get_as_compacted_size(rtx_context ctx, uint64_t as_address, uint64_t* out_bytes);
+int rtx_cmd_copy_as(rtx_context ctx, uint64_t src_address, uint32_t mode, void* dst_device, uint64_t dst_size,
+                    uint64_t* out_address);
+
+/* ---- OBJ meshes (host only; the reference's next step: "load an .obj file -> build the acceleration structure",
+ * vulkan-raytracing-basic/README.md:225-226) ----------------------------------------------------------------
+ * Reads `v` and `f` statements (face corners v, v/vt, v//vn, v/vt/vn; 1-based or negative relative indices;
+ * polygons fan-triangulated as (v0, vk, vk+1)); `o` / `g` start a new geometry.  All geometries share the vertex
+ * array: geometry g is triangles [geometry_first_triangle[g], geometry_first_triangle[g + 1]), i.e. an
+ * rtx_blas_geometry with primitive_offset = 12 * first and primitive_count = first[g + 1] - first[g].  Arrays are
+ * host memory owned by the mesh until rtx_obj_free.  Malformed input returns RTX_ERROR_FORMAT. */
+typedef struct rtx_obj_mesh {
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+    uint32_t geometry_count;
+    uint32_t reserved;
+    float* vertices;                     /* 3 floats per vertex */
+    uint32_t* indices;                   /* 3 per triangle */
+    uint32_t* geometry_first_triangle;   /* geometry_count + 1 entries */
+} rtx_obj_mesh;
+int rtx_obj_load(const char* path, rtx_obj_mesh* out_mesh);
+int rtx_obj_parse(const char* text, size_t length, rtx_obj_mesh* out_mesh);
+void rtx_obj_free(rtx_obj_mesh* mesh);
 
 /* ---- shader binding table ----------------------------------------------------------------------- */
 
@@ -299,6 +331,18 @@ int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg);
 int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device, uint32_t ray_count,
                        uint32_t query_mode, void* results_device, uint32_t reps, rtx_trace_config* out_cfg,
                        float* out_ms_per_launch);
+
+/* ---- film conversion -------------------------------------------------------------------------------------
+ * RGBA32F film -> 8-bit RGBA bytes: q = round(clamp(x, 0, 1) * 255) per channel (UNORM), optionally after the sRGB
+ * transfer function, with B and R swapped, or with alpha 255.  RTX_FILM_REFERENCE_BYTES gives the bytes the
+ * reference shows: its R8G8B8A8_UNORM out image is written through an R/B-swizzled view (main.cpp:952-974) and
+ * copied byte for byte to the B8G8R8A8 swapchain image (main.cpp:1357-1389). */
+#define RTX_FILM_SRGB_BIT 0x1u
+#define RTX_FILM_SWAP_RB_BIT 0x2u
+#define RTX_FILM_OPAQUE_BIT 0x4u
+#define RTX_FILM_REFERENCE_BYTES RTX_FILM_SWAP_RB_BIT
+int rtx_cmd_film_to_rgba8(rtx_context ctx, const float* film_rgba32f_device, uint32_t width, uint32_t height,
+                          uint32_t flags, uint8_t* out_rgba8_device);
 
 /* ---- path tracing: wavefront integrator (north-star extension of the raygen / closest-hit flow) ---------------
  *

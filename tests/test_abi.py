@@ -15,7 +15,7 @@ LIB = os.path.join(PKG, "librtx.so")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*|void\*)\s+(rtx_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|void\*|void)\s+(rtx_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_the_reference_flow():
