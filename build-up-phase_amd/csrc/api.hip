@@ -233,9 +233,10 @@ int rtx_get_blas_build_sizes(rtx_context ctx, const rtx_blas_geometry* geometrie
     int r = check_geometries(ctx, geometries, geometry_count, &n);
     if (r) return r;
     const bool k64 = (build_flags & RTX_BUILD_MORTON63_BIT) != 0;
-    out_sizes->acceleration_structure_size = rtx::blas_as_bytes((uint32_t)n);
+    const bool upd = (build_flags & RTX_BUILD_ALLOW_UPDATE_BIT) != 0;
+    out_sizes->acceleration_structure_size = rtx::blas_as_bytes((uint32_t)n, upd);
     out_sizes->build_scratch_size = rtx::blas_scratch_bytes((uint32_t)n, k64, geometry_count);
-    out_sizes->update_scratch_size = 0;
+    out_sizes->update_scratch_size = upd ? rtx::blas_scratch_bytes((uint32_t)n, false, geometry_count) : 0;
     return RTX_SUCCESS;
 }
 
@@ -260,7 +261,7 @@ int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uin
     const bool k64 = (build_flags & RTX_BUILD_MORTON63_BIT) != 0;
     if (!dst_device || !aligned(dst_device, 256) || !scratch_device || !aligned(scratch_device, 256))
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "dst and scratch must be non-NULL and 256-B aligned");
-    if (dst_size < rtx::blas_as_bytes((uint32_t)n))
+    if (dst_size < rtx::blas_as_bytes((uint32_t)n, (build_flags & RTX_BUILD_ALLOW_UPDATE_BIT) != 0))
         return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "dst_size %llu below the size query",
                     (unsigned long long)dst_size);
     if (scratch_size < rtx::blas_scratch_bytes((uint32_t)n, k64, geometry_count))
@@ -297,6 +298,8 @@ static int check_batch(rtx_context ctx, const rtx_blas_build_info* infos, uint32
     if (count > (1u << 20)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "more than 2^20 BLASes in one batch");
     if (flags & RTX_BUILD_MORTON63_BIT)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "batched builds use 30-bit Morton codes");
+    if (flags & RTX_BUILD_ALLOW_UPDATE_BIT)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "batched builds do not keep hierarchies for updates");
     uint64_t n = 0, g = 0;
     for (uint32_t b = 0; b < count; ++b) {
         uint64_t nb = 0;
@@ -380,6 +383,49 @@ int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, 
     // the host arrays above are read by asynchronous copies from pageable memory, which complete before return
     if (out_addresses)
         for (uint32_t b = 0; b < count; ++b) out_addresses[b] = (uint64_t)(uintptr_t)infos[b].dst_device;
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_update_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uint32_t geometry_count,
+                        uint64_t blas_address, void* scratch_device, uint64_t scratch_size) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    uint64_t n = 0;
+    int r = check_geometries(ctx, geometries, geometry_count, &n);
+    if (r) return r;
+    if (!blas_address || (blas_address & 255) || !scratch_device || !aligned(scratch_device, 256))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "BLAS and scratch must be non-NULL and 256-B aligned");
+    if ((r = rtx_sync(ctx))) return r;
+    rtx::AsHeader h;
+    hipError_t e = hipMemcpy(&h, (const void*)(uintptr_t)blas_address, sizeof h, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_update_blas header");
+    if (h.magic != rtx::kMagicBlas) return fail(ctx, RTX_ERROR_INVALID_AS, "not a BLAS");
+    if (!(h.flags & RTX_BUILD_ALLOW_UPDATE_BIT) || (n > 0 && h.update_offset == 0))
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "the BLAS was not built with RTX_BUILD_ALLOW_UPDATE_BIT");
+    if (h.prim_count != n || h.geometry_count != geometry_count)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "an update needs the build's geometry and triangle counts");
+    if (scratch_size < rtx::blas_scratch_bytes((uint32_t)n, false, geometry_count))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "scratch_size below update_scratch_size");
+    std::vector<rtx::GeomDesc> gd(geometry_count ? geometry_count : 1);
+    uint32_t base = 0;
+    for (uint32_t i = 0; i < geometry_count; ++i) {
+        const rtx_blas_geometry& x = geometries[i];
+        rtx::GeomDesc& d = gd[i];
+        d.vtx = (const uint8_t*)x.vertex_data_device;
+        d.idx = (const uint32_t*)((const uint8_t*)x.index_data_device + x.primitive_offset);
+        d.xform = x.transform_data_device ? (const float*)((const uint8_t*)x.transform_data_device + x.transform_offset)
+                                          : nullptr;
+        d.stride = x.vertex_stride;
+        d.prim_count = x.primitive_count;
+        d.first_vertex = x.first_vertex;
+        d.prim_base = base;
+        d.max_vertex = x.max_vertex;
+        d.geom_local = i;
+        d.blas = 0;
+        base += x.primitive_count;
+    }
+    e = rtx::update_blas(gd.data(), geometry_count, (uint32_t)n, h.flags, (void*)(uintptr_t)blas_address,
+                         scratch_device, &ctx->state->error_flags, ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_update_blas");
     return RTX_SUCCESS;
 }
 

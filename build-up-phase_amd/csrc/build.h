@@ -7,7 +7,7 @@
 
 namespace rtx {
 
-uint64_t blas_as_bytes(uint32_t prim_count);
+uint64_t blas_as_bytes(uint32_t prim_count, bool allow_update = false);
 uint64_t tlas_as_bytes(uint32_t instance_count);
 uint64_t blas_scratch_bytes(uint32_t prim_count, bool morton63, uint32_t geometry_count);
 uint64_t tlas_scratch_bytes(uint32_t instance_count, bool morton63);
@@ -22,6 +22,10 @@ uint64_t blas_batch_scratch_bytes(uint32_t prim_count, uint32_t geometry_count, 
 hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t geometry_count, const uint2* binfo_host,
                             AsHeader* const* dst_host, uint32_t nbatch, uint32_t prim_count, uint32_t flags,
                             void* scratch, uint32_t* err_flags, hipStream_t stream);
+// UPDATE mode for a BLAS built with RTX_BUILD_ALLOW_UPDATE_BIT (same geometry and triangle counts; scratch of
+// blas_scratch_bytes).
+hipError_t update_blas(const GeomDesc* geoms_host, uint32_t geometry_count, uint32_t prim_count, uint32_t flags,
+                       void* dst, void* scratch, uint32_t* err_flags, hipStream_t stream);
 // Enqueue a TLAS build over instance_count device-resident rtx_instance records.
 hipError_t build_tlas(const rtx_instance* instances_device, uint32_t instance_count, uint32_t flags, void* dst,
                       void* scratch, uint32_t* err_flags, hipStream_t stream);

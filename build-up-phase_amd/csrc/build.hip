@@ -828,6 +828,7 @@ __global__ void header_kernel(AsHeader* h, uint32_t type, uint32_t flags, uint32
     h->inst_table_offset = table_offset;
     h->capacity_slots = capacity_slots;
     h->sah_cost = 0.0;
+    h->update_offset = 0;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     if (n > 0) {
         const float* rb = (n == 1) ? nboxes + 0 : nboxes;  // root: internal 0, or leaf 0 when n == 1
@@ -1103,6 +1104,7 @@ __global__ __launch_bounds__(64) void batch_header_kernel(AsHeader* const* __res
     h->inst_table_offset = 0;
     h->capacity_slots = 2ull * (np ? np : 1) + 1;
     h->sah_cost = 0.0;
+    h->update_offset = 0;
     const int32_t root = np ? roots[b] : 0x7FFFFFFF;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     bool empty = root == 0x7FFFFFFF;
@@ -1216,11 +1218,19 @@ Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom, uint32_t nbatch = 
     return L;
 }
 
-uint64_t as_bytes(uint32_t n, bool tlas) {
+uint64_t as_bytes(uint32_t n, bool tlas, bool update = false) {
     uint64_t slots = 2ull * (n ? n : 1) + 1;
     uint64_t b = kHeaderBytes + slots * kSlotBytes;
     if (tlas) b = align_up(b, 256) + (uint64_t)(n ? n : 1) * sizeof(InstanceInfo);
+    if (update) b = align_up(b, 256) + (uint64_t)(n ? n : 1) * 4 * 5;  // order, left, right, parent (2n)
     return align_up(b, 256);
+}
+
+// Kept hierarchy of an updatable BLAS: order[n], left[n-1], right[n-1], parent[2n-1] (slots of 4 B).
+uint64_t update_offset(uint32_t n) { return align_up(kHeaderBytes + (2ull * (n ? n : 1) + 1) * kSlotBytes, 256); }
+
+__global__ void set_update_offset_kernel(AsHeader* h, uint64_t off) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) h->update_offset = off;
 }
 
 // Refit with treelet refinement; each further round restructures the tree the previous one left.
@@ -1366,7 +1376,7 @@ hipError_t debug_sort(void* keys, uint32_t* vals, uint32_t n, int key_bytes, uin
     return e ? e : e2;
 }
 
-uint64_t blas_as_bytes(uint32_t n) { return as_bytes(n, false); }
+uint64_t blas_as_bytes(uint32_t n, bool update) { return as_bytes(n, false, update); }
 uint64_t tlas_as_bytes(uint32_t n) { return as_bytes(n, true); }
 uint64_t blas_scratch_bytes(uint32_t n, bool k64, uint32_t ngeom) { return make_layout(n, k64 ? 8 : 4, ngeom).total; }
 uint64_t tlas_scratch_bytes(uint32_t n, bool k64) { return make_layout(n, k64 ? 8 : 4, 0).total; }
@@ -1392,6 +1402,76 @@ hipError_t build_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, ui
                                                          (float*)(sc + L.boxes), cb, nullptr);
     if ((err = lbvh_common(n, k64, 0, flags, ngeom, sc, L, (uint8_t*)dst, 0, err_flags, s))) return err;
     finish_blas_kernel<<<1, 64, 0, s>>>((AsHeader*)dst);
+    if ((flags & RTX_BUILD_ALLOW_UPDATE_BIT) && n > 0) {
+        // keep the final hierarchy (after treelet refinement) and the triangle order for updates
+        const uint64_t off = update_offset(n);
+        uint8_t* u = (uint8_t*)dst + off;
+        if ((err = hipMemcpyAsync(u, sc + L.vals, 4ull * n, hipMemcpyDeviceToDevice, s))) return err;
+        if (n > 1) {
+            if ((err = hipMemcpyAsync(u + 4ull * n, sc + L.left, 4ull * (n - 1), hipMemcpyDeviceToDevice, s)))
+                return err;
+            if ((err = hipMemcpyAsync(u + 4ull * (2 * n - 1), sc + L.right, 4ull * (n - 1), hipMemcpyDeviceToDevice,
+                                      s)))
+                return err;
+        }
+        if ((err = hipMemcpyAsync(u + 4ull * (3 * n - 2), sc + L.parent, 4ull * (2 * n - 1), hipMemcpyDeviceToDevice,
+                                  s)))
+            return err;
+        set_update_offset_kernel<<<1, 64, 0, s>>>((AsHeader*)dst, off);
+    }
+    return hipGetLastError();
+}
+
+// UPDATE mode: new triangle positions, the kept order and hierarchy, plain refit, header and collapse again.
+hipError_t update_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, uint32_t flags, void* dst,
+                       void* scratch, uint32_t* err_flags, hipStream_t s) {
+    Layout L = make_layout(n, 4, ngeom);
+    uint8_t* sc = (uint8_t*)scratch;
+    hipError_t err;
+    GeomDesc* gd = (GeomDesc*)(sc + L.geoms);
+    if (ngeom && (err = hipMemcpyAsync(gd, geoms_host, ngeom * sizeof(GeomDesc), hipMemcpyHostToDevice, s)))
+        return err;
+    uint32_t* cb = (uint32_t*)(sc + L.cbounds);
+    fill_kernel<<<1, 64, 0, s>>>(cb, 3, 0xFFFFFFFFu);
+    fill_kernel<<<1, 64, 0, s>>>(cb + 3, 3, 0u);
+    AsHeader* h = (AsHeader*)dst;
+    const uint64_t off = update_offset(n);
+    const uint8_t* u = (const uint8_t*)dst + off;
+    if (n > 0) {
+        prep_tris_kernel<<<(n + 255) / 256, 256, 0, s>>>(gd, (int)ngeom, n, (TriSlot*)(sc + L.leaves),
+                                                         (float*)(sc + L.boxes), cb, nullptr);
+        if ((err = hipMemcpyAsync(sc + L.vals, u, 4ull * n, hipMemcpyDeviceToDevice, s))) return err;
+        if (n > 1) {
+            if ((err = hipMemcpyAsync(sc + L.left, u + 4ull * n, 4ull * (n - 1), hipMemcpyDeviceToDevice, s)))
+                return err;
+            if ((err = hipMemcpyAsync(sc + L.right, u + 4ull * (2 * n - 1), 4ull * (n - 1), hipMemcpyDeviceToDevice,
+                                      s)))
+                return err;
+        }
+        if ((err = hipMemcpyAsync(sc + L.parent, u + 4ull * (3 * n - 2), 4ull * (2 * n - 1),
+                                  hipMemcpyDeviceToDevice, s)))
+            return err;
+        refit_passes(n, flags | RTX_BUILD_NO_TREELET_BIT, sc, L, (const float*)(sc + L.boxes), s);
+    }
+    uint32_t* levels = (uint32_t*)(sc + L.levels);
+    uint2* t0 = (uint2*)(sc + L.tasks0);
+    uint2* t1 = (uint2*)(sc + L.tasks1);
+    fill_kernel<<<1, 128, 0, s>>>(levels, 128, 0u);
+    header_kernel<<<1, 64, 0, s>>>(h, 0, flags, n, ngeom, (const float*)(sc + L.nboxes), (const float*)(sc + L.cost),
+                                   2ull * (n ? n : 1) + 1, 0, levels, t0);
+    if (n > 0) {
+        for (int lv = 0; lv < kMaxLevels32; ++lv) {
+            uint2* tin = (lv & 1) ? t1 : t0;
+            uint2* tout = (lv & 1) ? t0 : t1;
+            collapse_level_kernel<<<kCollapseBlocks, 256, 0, s>>>(
+                n, (const int32_t*)(sc + L.left), (const int32_t*)(sc + L.right), (const uint32_t*)(sc + L.vals),
+                (const float*)(sc + L.nboxes), (const uint4*)(sc + L.leaves), h, (uint4*)((uint8_t*)dst + kHeaderBytes),
+                levels, lv, tin, tout, nullptr, nullptr, nullptr);
+        }
+        collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, kMaxLevels32, err_flags);
+    }
+    finish_blas_kernel<<<1, 64, 0, s>>>(h);
+    set_update_offset_kernel<<<1, 64, 0, s>>>(h, off);
     return hipGetLastError();
 }
 

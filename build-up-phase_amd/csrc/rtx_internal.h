@@ -52,7 +52,8 @@ struct AsHeader {
     uint64_t inst_table_offset;
     uint64_t capacity_slots;
     double sah_cost;
-    uint8_t reserved[256 - 128];
+    uint64_t update_offset;  // BLAS built to allow updates: offset of the kept hierarchy (0 otherwise)
+    uint8_t reserved[256 - 136];
 };
 static_assert(sizeof(AsHeader) == 256, "AsHeader must be 256 B");
 

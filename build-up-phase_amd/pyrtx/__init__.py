@@ -22,6 +22,8 @@ LIB_PATH = os.path.join(PKG_DIR, "librtx.so")
 
 RTX_SUCCESS = 0
 RTX_GEOMETRY_OPAQUE_BIT = 0x1
+RTX_BUILD_ALLOW_UPDATE_BIT = 0x1
+RTX_BUILD_ALLOW_COMPACTION_BIT = 0x2
 RTX_BUILD_PREFER_FAST_TRACE_BIT = 0x4
 RTX_BUILD_PREFER_FAST_BUILD_BIT = 0x8
 RTX_BUILD_MORTON63_BIT = 0x10000
@@ -197,6 +199,7 @@ SIGNATURES = {
     "rtx_obj_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ObjMesh)]),
     "rtx_obj_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ObjMesh)]),
     "rtx_obj_free": (None, [ctypes.POINTER(ObjMesh)]),
+    "rtx_cmd_update_blas": (ctypes.c_int, [_P, _P, _U32, _U64, _P, _U64]),
     "rtx_debug_treelet": (ctypes.c_int, [_P, _P, _U32, _U32, _U32, _P, _P, _P, _P, _P]),
 }
 
@@ -317,6 +320,20 @@ class Context:
                                                 scratch.numel(), ctypes.byref(addr)))
         n = sum(g["indices"].numel() // 3 for g in geoms)
         return AccelerationStructure(storage, addr.value, n, (scratch, tuple(geoms)))
+
+    def update_blas(self, blas: AccelerationStructure, geoms):
+        """rtx_cmd_update_blas: new vertex data (same geometry / triangle counts) for a BLAS built with
+        RTX_BUILD_ALLOW_UPDATE_BIT."""
+        torch = _torch()
+        arr = self._geometry_structs(geoms)
+        sz = BuildSizes()
+        self._check(self.lib.rtx_get_blas_build_sizes(self.handle, ctypes.cast(arr, _P), len(geoms),
+                                                      RTX_BUILD_ALLOW_UPDATE_BIT, ctypes.byref(sz)))
+        scratch = torch.empty(max(256, int(sz.update_scratch_size)), dtype=torch.uint8, device=self.device)
+        self._check(self.lib.rtx_cmd_update_blas(self.handle, ctypes.cast(arr, _P), len(geoms), blas.address,
+                                                 scratch.data_ptr(), scratch.numel()))
+        blas.keepalive = (blas.keepalive, scratch, tuple(geoms))
+        return blas
 
     def build_blas_obj(self, vertices, indices, first, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
         """One BLAS over an OBJ mesh (load_obj output): one geometry per OBJ group, all sharing the vertex and index

@@ -66,6 +66,8 @@ void* rtx_get_stream(rtx_context ctx);
 
 /* VkGeometryFlagBitsKHR / VkBuildAccelerationStructureFlagBitsKHR subset */
 #define RTX_GEOMETRY_OPAQUE_BIT 0x1u                       /* main.cpp:741 */
+#define RTX_BUILD_ALLOW_UPDATE_BIT 0x1u                    /* BLAS keeps its hierarchy for rtx_cmd_update_blas */
+#define RTX_BUILD_ALLOW_COMPACTION_BIT 0x2u                /* accepted; every structure can be compacted */
 #define RTX_BUILD_PREFER_FAST_TRACE_BIT 0x4u               /* main.cpp:751, 887 */
 #define RTX_BUILD_PREFER_FAST_BUILD_BIT 0x8u
 #define RTX_BUILD_MORTON63_BIT 0x10000u                    /* 63-bit Morton keys (default: 30-bit) */
@@ -152,6 +154,15 @@ int rtx_get_blas_batch_build_sizes(rtx_context ctx, const rtx_blas_build_info* i
                                    uint32_t build_flags, uint64_t* as_sizes, uint64_t* scratch_size);
 int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, uint32_t count, uint32_t build_flags,
                              void* scratch_device, uint64_t scratch_size, uint64_t* out_addresses);
+
+/* vkCmdBuildAccelerationStructuresKHR in UPDATE mode (not used by the reference) for a BLAS built with
+ * RTX_BUILD_ALLOW_UPDATE_BIT (whose size query then includes the kept hierarchy, ~20 B per triangle): the same
+ * geometries with the same triangle counts and new vertex positions or transforms.  The binary hierarchy and
+ * triangle order of the build are kept (no re-sort, no treelet restructuring), boxes are refitted and the
+ * 8-wide nodes re-emitted in place, so traversal stays exact while quality decays with large deformations.
+ * scratch >= update_scratch_size of the size query.  Reads the structure's header (one synchronisation). */
+int rtx_cmd_update_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uint32_t geometry_count,
+                        uint64_t blas_address, void* scratch_device, uint64_t scratch_size);
 
 /* Same for a TLAS over instance_count rtx_instance records in device memory (main.cpp:912-945). */
 int rtx_cmd_build_tlas(rtx_context ctx, const rtx_instance* instances_device, uint32_t instance_count,

@@ -143,3 +143,44 @@ def test_obj_mesh_blas_matches_oracle(ctx):
     _say(f"  {rep}")
     assert rep["robust_hits"] > 500 and rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
     assert set(np.unique(g[4][g[5] != 0xFFFFFFFF]).tolist()) == {0, 1}  # both geometries are hit
+
+
+def test_blas_update_refits_in_place(ctx):
+    """A BLAS built with ALLOW_UPDATE, updated with displaced vertices (same topology), traces like the oracle on
+    the new vertices; wrong counts or a non-updatable BLAS are rejected."""
+    import pyrtx
+    from pyrtx import scenes as S
+    _say("BLAS update")
+    a = S.icosphere(12, radius=1.0, seed=1, amplitude=0.05)
+    rng = np.random.default_rng(3)
+    vb = (a.vertices * np.float32(1.3) + rng.normal(0, 0.02, a.vertices.shape).astype(np.float32)
+          + np.float32([0.2, -0.1, 0.0])).astype(np.float32)
+    b = S.Geometry(vb, a.indices)
+
+    def dev(g):
+        return [{"vertices": torch.from_numpy(g.vertices).cuda(),
+                 "indices": torch.from_numpy(g.indices.astype(np.uint32).view(np.int32)).cuda(), "transform": None}]
+
+    flags = pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT | pyrtx.RTX_BUILD_ALLOW_UPDATE_BIT
+    blas = ctx.build_blas(dev(a), flags)
+    assert blas.storage.numel() > ctx.build_blas(dev(a)).storage.numel()
+    ctx.update_blas(blas, dev(b))
+    sc = S.Scene(blas=[[b]], transforms=S.translation()[None], blas_of_instance=np.array([0]),
+                 custom_index=np.zeros(1, np.uint32), mask=np.full(1, 0xFF, np.uint32),
+                 sbt_offset=np.zeros(1, np.uint32), flags=np.zeros(1, np.uint32), camera=(0.0, 0.0, 4.0, 60.0),
+                 width=96, height=64, meta={"name": "update"})
+    tlas = ctx.build_tlas(pyrtx.instance_records(sc, [blas.address]))
+    info = ctx.as_info(blas)
+    assert np.allclose(info.bounds[:3], vb.min(0)) and np.allclose(info.bounds[3:], vb.max(0))
+    rays = np.concatenate([S.camera_rays(sc.camera, 96, 64), S.incoherent_rays(sc, 5000, seed=8)])
+    h = ctx.trace_ray_buffer(tlas, torch.from_numpy(rays).cuda())
+    ctx.sync()
+    g = pyrtx.hits_to_numpy(h)
+    tuv, ids, st = O.trace(oracle_scene(sc), rays)
+    rep = compare_closest(g, tuv, ids, st)
+    _say(f"  {rep}")
+    assert rep["robust_hits"] > 500 and rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
+    with pytest.raises(pyrtx.RtxError):
+        ctx.update_blas(blas, dev(S.icosphere(6, seed=1)))            # other triangle count
+    with pytest.raises(pyrtx.RtxError):
+        ctx.update_blas(ctx.build_blas(dev(a)), dev(b))               # built without ALLOW_UPDATE
