@@ -491,6 +491,7 @@ int rtx_cmd_trace_rays(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     q.mode = 2;
     q.n = width * height;
     q.cull_mask = p.cull_mask & 0xFFu;
+    q.ray_flags = p.ray_flags;
     // raygen aspect terms (main.cpp:1038-1039) in the GLSL operation order, tan evaluated in FP64
     const double rad = (double)camera->y_fov_degree * (3.14159265358979323846 / 180.0);
     const float ay = (float)std::tan(rad * 0.5);
@@ -521,7 +522,8 @@ int rtx_cmd_trace_rays(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
 }
 
 int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device, uint32_t ray_count,
-                             uint32_t mode, uint32_t cull_mask, rtx_hit* hits_device, uint32_t* occluded_device) {
+                             uint32_t mode, uint32_t ray_flags, uint32_t cull_mask, rtx_hit* hits_device,
+                             uint32_t* occluded_device) {
     if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
     int r = check_tlas(ctx, tlas_address);
     if (r) return r;
@@ -538,6 +540,7 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
     q.mode = mode;
     q.n = ray_count;
     q.cull_mask = cull_mask & 0xFFu;
+    q.ray_flags = ray_flags;
     q.rays = rays_device;
     q.hits = hits_device;
     q.occluded = occluded_device;
@@ -1145,7 +1148,9 @@ int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* ra
     uint32_t* occl = mode == RTX_QUERY_ANY_HIT ? (uint32_t*)results_device : nullptr;
     return autotune_core(
         ctx, "rtx_autotune_trace", reps,
-        [&]() { return rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0xFFu, hits, occl); },
+        [&]() {
+            return rtx_cmd_trace_ray_buffer(ctx, tlas_address, rays_device, ray_count, mode, 0u, 0xFFu, hits, occl);
+        },
         out_cfg, out_ms_per_launch);
 }
 

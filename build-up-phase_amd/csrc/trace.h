@@ -14,6 +14,7 @@ struct TraceRequest {
     const uint32_t* n_ptr;  // device ray count (overrides n when set)
     PathHit* path_hits;
     uint32_t cull_mask;
+    uint32_t ray_flags;  // RTX_RAY_FLAG_* (0 for the integrator's queries)
     const rtx_ray* rays;
     rtx_hit* hits;
     uint32_t* occluded;

@@ -32,6 +32,15 @@ constexpr uint32_t kEntryLeaf = 1u << 16;
 constexpr uint32_t kEntryTlas = 1u << 17;
 constexpr uint32_t kMaxSteps = 1u << 22;  // per ray; a valid BVH never needs more than ~2 steps per slot
 
+// traceRayEXT ray flags (gl_RayFlags*EXT values) and instance flags (VkGeometryInstanceFlagBitsKHR)
+constexpr uint32_t kRayTerminateOnFirstHit = RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT;
+constexpr uint32_t kRaySkipClosestHit = RTX_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER;
+constexpr uint32_t kRayCullBack = RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;
+constexpr uint32_t kRayCullFront = RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES;
+constexpr uint32_t kRayCullOpaque = RTX_RAY_FLAG_CULL_OPAQUE;
+constexpr uint32_t kRaySkipTriangles = RTX_RAY_FLAG_SKIP_TRIANGLES;
+
+
 __device__ __forceinline__ uint32_t oct_permute(uint32_t m, uint32_t oct) {
     if (oct & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
     if (oct & 2u) m = ((m & 0x33u) << 2) | ((m >> 2) & 0x33u);
@@ -61,6 +70,24 @@ __device__ __forceinline__ void setup_space(Space& s) {
     s.iy = safe_rcp(s.dy);
     s.iz = safe_rcp(s.dz);
     s.oct = (s.dx < 0.0f ? 1u : 0u) | (s.dy < 0.0f ? 2u : 0u) | (s.dz < 0.0f ? 4u : 0u);
+}
+
+// Facing culls.  A triangle is front-facing when v0, v1, v2 appear clockwise seen from the ray origin, i.e.
+// dot(d, (v1 - v0) x (v2 - v0)) > 0 in object space; an instance with TRIANGLE_FLIP_FACING makes counter-
+// clockwise front-facing, and TRIANGLE_FACING_CULL_DISABLE (the reference's instances, main.cpp:852) exempts it.
+__device__ __forceinline__ bool facing_culled(const uint4 a, const uint4 b, const uint4 c, const Space& s,
+                                              uint32_t ray_flags, uint32_t inst_flags) {
+    if (inst_flags & RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT) return false;
+    const float e1x = __uint_as_float(a.w) - __uint_as_float(a.x);
+    const float e1y = __uint_as_float(b.x) - __uint_as_float(a.y);
+    const float e1z = __uint_as_float(b.y) - __uint_as_float(a.z);
+    const float e2x = __uint_as_float(b.z) - __uint_as_float(a.x);
+    const float e2y = __uint_as_float(b.w) - __uint_as_float(a.y);
+    const float e2z = __uint_as_float(c.x) - __uint_as_float(a.z);
+    const float nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+    bool front = s.dx * nx + s.dy * ny + s.dz * nz > 0.0f;
+    if (inst_flags & RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT) front = !front;
+    return front ? (ray_flags & kRayCullFront) != 0 : (ray_flags & kRayCullBack) != 0;
 }
 
 // Quantised 8-child slab test.  Returns the child-position hit mask.
@@ -199,6 +226,7 @@ struct TraceArgs {
     uint32_t refill_lanes;     // refill a wave once fewer lanes than this are busy
     uint32_t refill_interval;  // steps between refill checks
     uint32_t leaf_lanes;       // pending leaf groups needed before a leaf step runs
+    uint32_t ray_flags;        // RTX_RAY_FLAG_* of the launch
 };
 
 enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3 };
@@ -252,8 +280,9 @@ __device__ void world_normal(const double* inv, const TriSlot* tri, float* n) {
     n[2] = (float)(nw[2] * r);
 }
 
-template <int MODE, bool STATS>
+template <int MODE, bool STATS, bool FLAGS>
 __device__ __forceinline__ void trace_body(const TraceArgs& args) {
+    const uint32_t ray_flags = FLAGS ? args.ray_flags : 0u;  // the default variant compiles the flags out
     extern __shared__ uint2 lds_stack[];
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -363,7 +392,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                     steps = 0;
                     ng_base = 0;
                     // the root as a one-child group: child position 0 sits at ordered bit (0 ^ oct)
-                    ng_bits = tlas_empty ? 0u : ((1u << s.oct) | (1u << 8));
+                    ng_bits = tlas_empty || (ray_flags & (kRaySkipTriangles | kRayCullOpaque))
+                                  ? 0u : ((1u << s.oct) | (1u << 8));
                     lg_bits = 0;
                     sp = 0;
                     hit_inst = kNoHit;
@@ -535,7 +565,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         if (STATS) ++st_tris;
                         float th;
                         const float smax = tc - s.toff;
-                        if (test_triangle(a, b, c, s, sh, smin, smax, th)) {
+                        if (test_triangle(a, b, c, s, sh, smin, smax, th) &&
+                            !(ray_flags & (kRayCullBack | kRayCullFront) &&
+                              facing_culled(a, b, c, s, ray_flags, table[cur_inst].flags))) {
                             if (MODE == kModeAny) {
                                 occl = true;
                                 done = true;
@@ -544,6 +576,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                             tc = s.toff + th;
                             hit_inst = cur_inst;
                             hit_slot = slot;
+                            if (ray_flags & kRayTerminateOnFirstHit) {
+                                done = true;
+                                break;
+                            }
                         }
                     }
                 }
@@ -648,7 +684,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         hp[5] = hit_inst;
                     } else {
                         float r = args.miss[0], g = args.miss[1], bl = args.miss[2];
-                        if (hit_inst != kNoHit) {
+                        if (hit_inst != kNoHit && (ray_flags & kRaySkipClosestHit)) {
+                            r = g = bl = 0.0f;  // the closest-hit program does not run: the payload keeps its 0
+                        } else if (hit_inst != kNoHit) {
                             const uint32_t rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
                             if (rec < args.nrec) {
                                 const rtx_hit_record hr = args.records[rec];
@@ -693,13 +731,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
 
 // Two occupancy variants of the same traversal: unconstrained registers, and at most 128 VGPRs so that four
 // waves fit on a SIMD (chosen when the launch shape asks for four or more waves per SIMD).
-template <int MODE, bool STATS>
+template <int MODE, bool STATS, bool FLAGS>
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
-    trace_body<MODE, STATS>(args);
+    trace_body<MODE, STATS, FLAGS>(args);
 }
-template <int MODE, bool STATS>
+template <int MODE, bool STATS, bool FLAGS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel_w4(TraceArgs args) {
-    trace_body<MODE, STATS>(args);
+    trace_body<MODE, STATS, FLAGS>(args);
 }
 
 __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
@@ -714,15 +752,25 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
     }
 }
 
-template <int MODE, bool STATS>
-hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
+template <int MODE, bool STATS, bool FLAGS>
+hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
     const size_t lds = (size_t)L.lds_depth * L.block_size * sizeof(uint2);
     const uint32_t waves_per_simd = (L.block_size / 64u) * L.blocks_per_cu / 4u;
     if (waves_per_simd >= 4u)
-        trace_kernel_w4<MODE, STATS><<<L.grid, L.block_size, lds, s>>>(a);
+        trace_kernel_w4<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
     else
-        trace_kernel<MODE, STATS><<<L.grid, L.block_size, lds, s>>>(a);
+        trace_kernel<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
     return hipGetLastError();
+}
+
+// ray flags that change traversal (OPAQUE / NO_OPAQUE / CULL_NO_OPAQUE do not: every triangle is opaque)
+constexpr uint32_t kActiveRayFlags =
+    kRayTerminateOnFirstHit | kRaySkipClosestHit | kRayCullBack | kRayCullFront | kRayCullOpaque | kRaySkipTriangles;
+
+template <int MODE, bool STATS>
+hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
+    if (a.ray_flags & kActiveRayFlags) return launch_variant<MODE, STATS, true>(a, L, s);
+    return launch_variant<MODE, STATS, false>(a, L, s);
 }
 
 }  // namespace
@@ -760,12 +808,13 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.refill_lanes = L.refill_lanes;
     a.refill_interval = L.refill_interval;
     a.leaf_lanes = L.leaf_lanes;
+    a.ray_flags = r.ray_flags;
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
     if (r.n == 0 && !r.n_ptr) return hipGetLastError();
     switch (r.mode) {
         case 0: return r.stats ? launch<kModeClosest, true>(a, L, s) : launch<kModeClosest, false>(a, L, s);
         case 1: return r.stats ? launch<kModeAny, true>(a, L, s) : launch<kModeAny, false>(a, L, s);
-        case 3: return launch<kModePath, false>(a, L, s);
+        case 3: return launch_variant<kModePath, false, false>(a, L, s);
         default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
     }
 }

@@ -107,6 +107,10 @@ class RenderView(ctypes.Structure):
 
 
 RTX_RENDER_GRAPH_BIT = 0x1
+RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT, RTX_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER = 0x4, 0x8
+RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES, RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES = 0x10, 0x20
+RTX_RAY_FLAG_CULL_OPAQUE, RTX_RAY_FLAG_CULL_NO_OPAQUE, RTX_RAY_FLAG_SKIP_TRIANGLES = 0x40, 0x80, 0x100
+RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT, RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT = 0x1, 0x2
 RTX_FILM_SRGB_BIT, RTX_FILM_SWAP_RB_BIT, RTX_FILM_OPAQUE_BIT = 0x1, 0x2, 0x4
 RTX_FILM_REFERENCE_BYTES = RTX_FILM_SWAP_RB_BIT
 RTX_COPY_MODE_CLONE, RTX_COPY_MODE_COMPACT = 0, 1
@@ -162,7 +166,7 @@ SIGNATURES = {
     "rtx_get_as_info": (ctypes.c_int, [_P, _U64, ctypes.POINTER(AsInfo)]),
     "rtx_cmd_trace_rays": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), ctypes.POINTER(Camera),
                                           ctypes.POINTER(TraceParams), _U32, _U32, _P]),
-    "rtx_cmd_trace_ray_buffer": (ctypes.c_int, [_P, _U64, _P, _U32, _U32, _U32, _P, _P]),
+    "rtx_cmd_trace_ray_buffer": (ctypes.c_int, [_P, _U64, _P, _U32, _U32, _U32, _U32, _P, _P]),
     "rtx_set_trace_stats": (ctypes.c_int, [_P, ctypes.c_int]),
     "rtx_get_trace_stats": (ctypes.c_int, [_P, ctypes.POINTER(TraceStats)]),
     "rtx_set_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),
@@ -421,7 +425,8 @@ class Context:
 
     # -- tracing --------------------------------------------------------------------------------------------------
     def trace_rays(self, tlas: AccelerationStructure, records, camera, width, height, miss_color=(0.0, 0.0, 0.2),
-                   tmin=0.0, tmax=100.0, cull_mask=0xFF, sbt_offset=0, sbt_stride=1, film=None):
+                   tmin=0.0, tmax=100.0, cull_mask=0xFF, sbt_offset=0, sbt_stride=1, film=None,
+                   ray_flags=RTX_RAY_FLAG_OPAQUE):
         """vkCmdTraceRaysKHR with the reference raygen (main.cpp:1033-1055) into an RGBA32F film [H, W, 4]."""
         torch = _torch()
         if isinstance(records, np.ndarray):
@@ -434,14 +439,14 @@ class Context:
         sbt = Sbt(rec.data_ptr() if nrec else None, nrec, sbt_offset, sbt_stride, 0,
                   (ctypes.c_float * 3)(*miss_color), 0.0)
         cam = Camera((ctypes.c_float * 3)(*camera[:3]), float(camera[3]))
-        params = TraceParams(RTX_RAY_FLAG_OPAQUE, cull_mask, tmin, tmax)
+        params = TraceParams(ray_flags, cull_mask, tmin, tmax)
         self._check(self.lib.rtx_cmd_trace_rays(self.handle, tlas.address, ctypes.byref(sbt), ctypes.byref(cam),
                                                 ctypes.byref(params), width, height, film.data_ptr()))
         self._keep = (rec,)
         return film
 
     def trace_ray_buffer(self, tlas: AccelerationStructure, rays, mode=RTX_QUERY_CLOSEST_HIT, cull_mask=0xFF,
-                         out=None):
+                         out=None, ray_flags=0):
         """rays: device float32 tensor [N, 8].  Returns hits (uint32 view [N, 6]: t, u, v as float bits, prim,
         geom, inst) for closest-hit, or occluded uint32 [N] for any-hit."""
         torch = _torch()
@@ -450,12 +455,12 @@ class Context:
             if out is None:
                 out = torch.empty((n, 6), dtype=torch.int32, device=self.device)
             self._check(self.lib.rtx_cmd_trace_ray_buffer(self.handle, tlas.address, rays.data_ptr(), n, mode,
-                                                          cull_mask, out.data_ptr(), None))
+                                                          ray_flags, cull_mask, out.data_ptr(), None))
         else:
             if out is None:
                 out = torch.empty((n,), dtype=torch.int32, device=self.device)
             self._check(self.lib.rtx_cmd_trace_ray_buffer(self.handle, tlas.address, rays.data_ptr(), n, mode,
-                                                          cull_mask, None, out.data_ptr()))
+                                                          ray_flags, cull_mask, None, out.data_ptr()))
         return out
 
     def _render_args(self, records, camera, lights, width, height, spp, max_depth, rr_depth, seed, sample_offset,

@@ -111,6 +111,7 @@ typedef struct rtx_instance {
 } rtx_instance;
 
 #define RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT 0x1u /* VK_GEOMETRY_INSTANCE_..., main.cpp:852 */
+#define RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT 0x2u
 #define RTX_INSTANCE_FORCE_OPAQUE_BIT 0x4u
 
 /* VkAccelerationStructureBuildSizesInfoKHR */
@@ -255,8 +256,22 @@ typedef struct rtx_camera {
 } rtx_camera;
 
 /* traceRayEXT arguments used by the raygen program (main.cpp:1047-1052). */
+/* traceRayEXT ray flags (gl_RayFlags*EXT values; the reference passes gl_RayFlagsOpaqueEXT, main.cpp:1048).
+ * Every triangle of this library is opaque (there are no any-hit programs), so OPAQUE / NO_OPAQUE change
+ * nothing, CULL_OPAQUE culls every triangle and CULL_NO_OPAQUE none.  Facing: a triangle is front-facing when
+ * v0, v1, v2 appear clockwise seen from the ray origin (dot(d, (v1 - v0) x (v2 - v0)) > 0 in object space);
+ * RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT makes counter-clockwise front-facing and
+ * RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT exempts the instance from the CULL_*_FACING flags.
+ * TERMINATE_ON_FIRST_HIT reports the first accepted hit, not necessarily the closest. */
 #define RTX_RAY_FLAG_OPAQUE 0x1u                 /* gl_RayFlagsOpaqueEXT */
+#define RTX_RAY_FLAG_NO_OPAQUE 0x2u
 #define RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT 0x4u /* gl_RayFlagsTerminateOnFirstHitEXT */
+#define RTX_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER 0x8u
+#define RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES 0x10u
+#define RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES 0x20u
+#define RTX_RAY_FLAG_CULL_OPAQUE 0x40u
+#define RTX_RAY_FLAG_CULL_NO_OPAQUE 0x80u
+#define RTX_RAY_FLAG_SKIP_TRIANGLES 0x100u
 typedef struct rtx_trace_params {
     uint32_t ray_flags;    /* RTX_RAY_FLAG_* */
     uint32_t cull_mask;    /* 8-bit, ANDed with the instance mask */
@@ -300,10 +315,10 @@ typedef struct rtx_hit {
 
 /* Trace ray_count rays.  mode RTX_QUERY_CLOSEST_HIT writes hits_device[i] (rtx_hit); RTX_QUERY_ANY_HIT writes
  * occluded_device[i] = 1 if any opaque triangle lies in [tmin, tmax], else 0.  The unused output may be NULL.
- * cull_mask is ANDed with each instance mask (0xFF in the reference). */
+ * ray_flags: RTX_RAY_FLAG_* for every ray; cull_mask is ANDed with each instance mask (0xFF in the reference). */
 int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device,
-                             uint32_t ray_count, uint32_t mode, uint32_t cull_mask, rtx_hit* hits_device,
-                             uint32_t* occluded_device);
+                             uint32_t ray_count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                             rtx_hit* hits_device, uint32_t* occluded_device);
 
 /* Per-ray traversal counters of the last rtx_cmd_trace_ray_buffer call issued with stats enabled:
  * total 64-B node fetches, triangle tests and instance entries summed over all rays. */

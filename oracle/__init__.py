@@ -39,7 +39,8 @@ def lib():
         L.oracle_blas_delta.restype = ctypes.c_double
         L.oracle_blas_delta.argtypes = [ctypes.c_int]
         L.oracle_trace.restype = None
-        L.oracle_trace.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, P, P, P, ctypes.c_int]
+        L.oracle_trace.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P, P,
+                                   P, ctypes.c_int]
         L.oracle_trace_brute.restype = None
         L.oracle_trace_brute.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_uint32, P, P, P]
         L.oracle_aspect.restype = None
@@ -137,15 +138,17 @@ def tlas_create(records: np.ndarray) -> int:
     return lib().oracle_tlas_create(len(records), _ptr(records))
 
 
-def trace(tlas: int, rays: np.ndarray, any_hit: bool = False, cull_mask: int = 0xFF, threads: int = 0):
+def trace(tlas: int, rays: np.ndarray, any_hit: bool = False, cull_mask: int = 0xFF, threads: int = 0,
+          ray_flags: int = 0):
     """rays: float32 [N,8].  Returns (t_uv float64 [N,3], ids uint32 [N,4] (prim, geom, inst, custom),
-    status int32 [N])."""
+    status int32 [N]).  ray_flags: gl_RayFlags*EXT (facing culls, CULL_OPAQUE, SKIP_TRIANGLES)."""
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
     n = rays.shape[0]
     t_uv = np.zeros((n, 3), np.float64)
     ids = np.zeros((n, 4), np.uint32)
     st = np.zeros(n, np.int32)
-    lib().oracle_trace(tlas, _ptr(rays), n, int(any_hit), cull_mask, _ptr(t_uv), _ptr(ids), _ptr(st), threads)
+    lib().oracle_trace(tlas, _ptr(rays), n, int(any_hit), cull_mask, ray_flags, _ptr(t_uv), _ptr(ids), _ptr(st),
+                       threads)
     return t_uv, ids, st
 
 

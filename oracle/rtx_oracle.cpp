@@ -270,7 +270,24 @@ struct Collector {
     std::vector<C> cands;
 };
 
-void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint32_t inst_id, Collector& col) {
+// traceRayEXT facing culls (gl_RayFlagsCull{Back,Front}FacingTrianglesEXT) with the instance flags
+// TRIANGLE_FACING_CULL_DISABLE (0x1) and TRIANGLE_FLIP_FACING (0x2): front-facing = v0, v1, v2 clockwise seen from
+// the ray origin, dot(d, (v1 - v0) x (v2 - v0)) > 0 in object space.  Returns 0 keep, 1 cull, 2 too close to
+// edge-on to tell in FP32 (an epsilon-tie).
+constexpr uint32_t kCullBack = 0x10u, kCullFront = 0x20u;
+int facing_cull(const RayFrame& f, const Tri& tr, uint32_t ray_flags, uint32_t inst_flags) {
+    if (!(ray_flags & (kCullBack | kCullFront)) || (inst_flags & 0x1u)) return 0;
+    d3 v0{tr.v[0], tr.v[1], tr.v[2]}, v1{tr.v[3], tr.v[4], tr.v[5]}, v2{tr.v[6], tr.v[7], tr.v[8]};
+    d3 n = cross(v1 - v0, v2 - v0);
+    double fd = dot(f.d, n);
+    if (std::fabs(fd) <= 1e-5 * len(n) * f.dlen) return 2;
+    bool front = fd > 0;
+    if (inst_flags & 0x2u) front = !front;
+    return (front ? (ray_flags & kCullFront) : (ray_flags & kCullBack)) ? 1 : 0;
+}
+
+void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint32_t inst_id, Collector& col,
+                uint32_t ray_flags = 0, uint32_t inst_flags = 0) {
     if (b.nodes.empty()) return;
     // boxes inflated by 4*delta so that every near-edge candidate is visited
     double infl = 4.0 * b.delta;
@@ -309,6 +326,9 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
             const Tri& tr = b.tris[b.order[n.right + k]];
             Cand c;
             if (!eval_tri(f, tr, b.delta, c)) continue;
+            const int cull = facing_cull(f, tr, ray_flags, inst_flags);
+            if (cull == 1) continue;
+            if (cull == 2) c.near = true;
             // candidates relevant to [tmin, tmax] (with uncertainty)
             if (c.t + c.tunc < tmin || c.t - c.tunc > tmax) continue;
             if (!c.inside && !c.near) continue;
@@ -333,11 +353,14 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
     }
 }
 
-void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, HitOut& out) {
+// ray_flags: gl_RayFlags*EXT; every triangle is opaque, so CULL_OPAQUE (0x40) and SKIP_TRIANGLES (0x100) leave
+// nothing to hit.
+void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, HitOut& out, uint32_t ray_flags = 0) {
     d3 o{ray[0], ray[1], ray[2]}, d{ray[4], ray[5], ray[6]};
     double tmin = ray[3], tmax = ray[7];
     Collector col;
-    for (size_t i = 0; i < tl.inst.size(); ++i) {
+    const bool skip_all = (ray_flags & (0x40u | 0x100u)) != 0;
+    for (size_t i = 0; i < tl.inst.size() && !skip_all; ++i) {
         const Inst& in = tl.inst[i];
         if (!in.valid || ((in.mask & cull) == 0)) continue;
         const Blas& b = *g_blas[in.blas];
@@ -348,7 +371,7 @@ void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, Hit
               M[8] * d.x + M[9] * d.y + M[10] * d.z};
         if (dd.x == 0 && dd.y == 0 && dd.z == 0) continue;
         RayFrame f = make_frame(oo, dd);
-        trace_blas(b, f, tmin, tmax, (uint32_t)i, col);
+        trace_blas(b, f, tmin, tmax, (uint32_t)i, col, ray_flags, in.flags);
     }
     out.n_cand = (uint32_t)col.cands.size();
     out.custom = 0;
@@ -537,8 +560,8 @@ double oracle_blas_delta(int blas) { return g_blas[blas]->delta; }
 // Trace n rays (32-B records: ox oy oz tmin dx dy dz tmax).  Outputs (any may be NULL):
 //   t_uv[3n] doubles, ids[4n] = prim, geom, inst (0xFFFFFFFF miss), custom index, status[n] (0 miss, 1 hit,
 //   2 epsilon-tie).  threads <= 0 uses hardware concurrency.
-void oracle_trace(int tlas, const float* rays, int64_t n, int any_hit, uint32_t cull_mask, double* t_uv,
-                  uint32_t* ids, int32_t* status, int threads) {
+void oracle_trace(int tlas, const float* rays, int64_t n, int any_hit, uint32_t cull_mask, uint32_t ray_flags,
+                  double* t_uv, uint32_t* ids, int32_t* status, int threads) {
     const Tlas& tl = *g_tlas[tlas];
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<int64_t> next{0};
@@ -549,7 +572,7 @@ void oracle_trace(int tlas, const float* rays, int64_t n, int any_hit, uint32_t 
             int64_t e = std::min<int64_t>(n, s + 256);
             for (int64_t i = s; i < e; ++i) {
                 HitOut h;
-                trace_one(tl, rays + 8 * i, cull_mask, any_hit, h);
+                trace_one(tl, rays + 8 * i, cull_mask, any_hit, h, ray_flags);
                 if (t_uv) {
                     t_uv[3 * i] = h.t;
                     t_uv[3 * i + 1] = h.u;

@@ -364,7 +364,7 @@ def test_error_paths(ctx):
     rc = ctx.lib.rtx_cmd_build_blas(ctx.handle, ctypes.cast(arr, ctypes.c_void_p), 1, 0, dst.data_ptr(), as_size,
                                     small.data_ptr(), small.numel(), ctypes.byref(addr))
     assert rc == -2  # RTX_ERROR_OUT_OF_DEVICE_MEMORY
-    rc = ctx.lib.rtx_cmd_trace_ray_buffer(ctx.handle, 0, None, 0, 0, 0xFF, None, None)
+    rc = ctx.lib.rtx_cmd_trace_ray_buffer(ctx.handle, 0, None, 0, 0, 0, 0xFF, None, None)
     assert rc == -1002
     with pytest.raises(pyrtx.RtxError):
         ctx.set_config(block_size=100)
@@ -390,3 +390,52 @@ def test_cpp_host_sample_matches_oracle(tmp_path):
     robust = st != 2
     # 8-bit rounding of identical values: at most one code apart (half-way cases)
     assert np.abs(img.astype(np.float64) - ref)[robust].max() <= 0.5 + 1e-3
+
+
+# ---- ray and instance flags -------------------------------------------------------------------------------------
+def test_ray_flags_match_oracle(ctx):
+    import pyrtx
+    _say("ray flags")
+    sc = small_mixed_scene(seed=13, instances=24, freq=12, cells=16)
+    # instance flags: none / facing culls disabled / flipped facing, cycling
+    sc.flags = np.array([(0, pyrtx.RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT,
+                          pyrtx.RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT)[i % 3] for i in range(24)], np.uint32)
+    blases, tlas = ctx.upload_scene(sc)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=6)])
+    dr = torch.from_numpy(rays).cuda()
+    for rf in (pyrtx.RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES, pyrtx.RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES,
+               pyrtx.RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES | pyrtx.RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES):
+        h = ctx.trace_ray_buffer(tlas, dr, ray_flags=rf)
+        ctx.sync()
+        g = pyrtx.hits_to_numpy(h)
+        tuv, ids, st = O.trace(t, rays, ray_flags=rf)
+        rep = compare_closest(g, tuv, ids, st)
+        _say(f"  flags {rf:#x}: {rep}")
+        assert rep["robust_hits"] > 200 and rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
+        occ = ctx.trace_ray_buffer(tlas, dr, pyrtx.RTX_QUERY_ANY_HIT, ray_flags=rf).cpu().numpy()
+        _, _, sta = O.trace(t, rays, any_hit=True, ray_flags=rf)
+        assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
+    # nothing to hit: every triangle is opaque
+    for rf in (pyrtx.RTX_RAY_FLAG_SKIP_TRIANGLES, pyrtx.RTX_RAY_FLAG_CULL_OPAQUE):
+        g = pyrtx.hits_to_numpy(ctx.trace_ray_buffer(tlas, dr, ray_flags=rf))
+        assert np.all(g[5] == 0xFFFFFFFF)
+    # first hit: hits exactly where something is hit, never closer than the closest
+    g = pyrtx.hits_to_numpy(ctx.trace_ray_buffer(tlas, dr, ray_flags=pyrtx.RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT))
+    tuv, ids, st = O.trace(t, rays)
+    robust = st != 2
+    assert np.array_equal((g[5] != 0xFFFFFFFF)[robust], (st == 1)[robust])
+    hit = robust & (st == 1)
+    assert np.all(g[0][hit] >= tuv[hit, 0] * (1 - 1e-5))
+
+
+def test_skip_closest_hit_program(ctx):
+    import pyrtx
+    sc = S.scene_r0()
+    blases, tlas = ctx.upload_scene(sc)
+    plain = ctx.trace_rays(tlas, sc.records, sc.camera, 300, 200).cpu().numpy()
+    skip = ctx.trace_rays(tlas, sc.records, sc.camera, 300, 200,
+                          ray_flags=pyrtx.RTX_RAY_FLAG_OPAQUE | pyrtx.RTX_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER).cpu().numpy()
+    miss = np.all(plain[..., :3] == np.float32([0, 0, 0.2]), -1)
+    assert miss.any() and (~miss).any()
+    assert np.array_equal(skip[miss], plain[miss]) and np.all(skip[~miss] == 0)
