@@ -2,11 +2,11 @@
 // plain and with a neural radiance cache (Muller et al. 2021).
 //
 // One pass traces one path per path slot.  Per path vertex (depth d):
-//   extend  persistent traversal over the extension queue (ray count read from device memory)  [trace.hip]
 //   shade   emission seen at d = 0, next-event estimation toward one light point (shadow queue), Russian
-//           roulette, cosine-weighted continuation (extension queue for d + 1); queues are compacted with one
-//           64-lane ballot + one atomic per wave
-//   shadow  any-hit traversal over the shadow queue                                               [trace.hip]
+//           roulette, cosine-weighted continuation (extension queue for d + 1); queue pushes are aggregated per
+//           256-thread block (one atomic per queue per block)
+//   trace   one persistent launch over the shadow queue of d (any hit) and the extension queue of d + 1
+//           (closest hit with path data); counts are read from device memory                [trace.hip]
 //   gather  unoccluded shadow contributions are added to their path's radiance
 // Every path owns its slot in a pass, so no accumulation needs atomics and results do not depend on queue order.
 //
@@ -201,6 +201,8 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
     a.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// `cur` selects the extension queue pair with a conditional, never as a runtime index into the kernel argument
+// (which would copy the whole argument block to per-lane scratch memory)
 template <int MODE>
 __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, uint32_t sample, int cur) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -213,13 +215,13 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
     float4 thr_new = make_float4(0, 0, 0, 0);
     uint32_t pid = 0;
     if (q < count) {
-        pid = a.ext_pid[cur][q];
+        pid = (cur ? a.ext_pid[1] : a.ext_pid[0])[q];
         const bool train = MODE == kPtCache && pid >= a.npix;
         // the vertex where this path queries the cache instead of scattering
         const bool query_here = MODE == kPtCache && ((!train && depth == a.cache_depth) ||
                                                      (train && a.tail && depth == a.train_end));
         const bool record = train && depth >= 1 && !query_here;
-        const rtx_ray r = a.ext_rays[cur][q];
+        const rtx_ray r = (cur ? a.ext_rays[1] : a.ext_rays[0])[q];
         const PathHit h = a.hits[q];
         if (h.rec != 0xFFFFFFFFu && h.rec < a.nrec) {
             const rtx_hit_record rec = a.records[h.rec];
@@ -364,8 +366,8 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
         a.sh_contrib[o[0]] = contrib;
     }
     if (push_ext) {
-        a.ext_rays[cur ^ 1][o[1]] = ext;
-        a.ext_pid[cur ^ 1][o[1]] = pid;
+        (cur ? a.ext_rays[0] : a.ext_rays[1])[o[1]] = ext;
+        (cur ? a.ext_pid[0] : a.ext_pid[1])[o[1]] = pid;
         a.thr[pid] = thr_new;
     }
     if (MODE == kPtCache && push_cq) {
@@ -567,35 +569,34 @@ static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uin
     fill_words_kernel<<<1, 64, 0, s>>>(a.counts, ncounts, 0u);
     camera_kernel<<<gb, 256, 0, s>>>(a, sample);
     count_camera_rays_kernel<<<1, 64, 0, s>>>(a.state, a.npaths);
-    int cur = 0;
     hipError_t e;
+    // vertex 0's extension rays (camera rays) alone; afterwards vertex d's shadow rays and vertex d + 1's extension
+    // rays, both produced by shade(d), go through one fused launch
+    TraceRequest t{};
+    t.tlas = a.tlas;
+    t.cull_mask = 0xFFu;
+    t.path_hits = a.hits;
+    t.sbt_offset = a.sbt_offset;
+    t.sbt_stride = a.sbt_stride;
+    t.state = a.state;
+    t.gstack = a.gstack;
+    t.mode = 3;
+    t.n_ptr = a.counts;
+    t.rays = a.ext_rays[0];
+    if ((e = trace_rays_launch(t, L, s))) return e;
+    int cur = 0;
     for (uint32_t d = 0; d <= last_depth; ++d) {
-        TraceRequest ext{};
-        ext.tlas = a.tlas;
-        ext.mode = 3;
-        ext.n_ptr = a.counts + 3 * d;
-        ext.cull_mask = 0xFFu;
-        ext.rays = a.ext_rays[cur];
-        ext.path_hits = a.hits;
-        ext.sbt_offset = a.sbt_offset;
-        ext.sbt_stride = a.sbt_stride;
-        ext.state = a.state;
-        ext.gstack = a.gstack;
-        if ((e = trace_rays_launch(ext, L, s))) return e;
         if (a.mode == kPtCache)
             shade_kernel<kPtCache><<<gb, 256, 0, s>>>(a, d, sample, cur);
         else
             shade_kernel<kPtPlain><<<gb, 256, 0, s>>>(a, d, sample, cur);
-        TraceRequest sh{};
-        sh.tlas = a.tlas;
-        sh.mode = 1;
-        sh.n_ptr = a.counts + 3 * d + 1;
-        sh.cull_mask = 0xFFu;
-        sh.rays = a.sh_rays;
-        sh.occluded = a.occl;
-        sh.state = a.state;
-        sh.gstack = a.gstack;
-        if ((e = trace_rays_launch(sh, L, s))) return e;
+        t.mode = 4;
+        t.n_ptr = a.counts + 3 * (d + 1);
+        t.rays = a.ext_rays[cur ^ 1];
+        t.n2_ptr = a.counts + 3 * d + 1;
+        t.rays2 = a.sh_rays;
+        t.occluded = a.occl;
+        if ((e = trace_rays_launch(t, L, s))) return e;
         shadow_gather_kernel<<<gb, 256, 0, s>>>(a, d);
         cur ^= 1;
     }

@@ -9,9 +9,12 @@ namespace rtx {
 
 struct TraceRequest {
     const void* tlas;
-    uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film, 3 path hits
+    uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film, 3 path hits,
+                    // 4 path hits for `rays` + occlusion for `rays2` in one launch
     uint32_t n;
     const uint32_t* n_ptr;  // device ray count (overrides n when set)
+    const rtx_ray* rays2;   // mode 4: the any-hit rays and their device count
+    const uint32_t* n2_ptr;
     PathHit* path_hits;
     uint32_t cull_mask;
     uint32_t ray_flags;  // RTX_RAY_FLAG_* (0 for the integrator's queries)

@@ -203,6 +203,8 @@ struct TraceArgs {
     const uint8_t* tlas;
     const rtx_ray* rays;
     const uint32_t* n_ptr;  // when set, the ray count is read from device memory (queues)
+    const rtx_ray* rays2;   // kModeFused: the any-hit queue and its device count
+    const uint32_t* n2_ptr;
     PathHit* path_hits;
     uint32_t n;
     uint32_t cull_mask;
@@ -229,7 +231,9 @@ struct TraceArgs {
     uint32_t ray_flags;        // RTX_RAY_FLAG_* of the launch
 };
 
-enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3 };
+// kModeFused: one launch over two queues of the wavefront integrator, rays [0, nA) of `rays` as kModePath
+// (extension rays, path hits) and rays [nA, nA + nB) of `rays2` as kModeAny (shadow rays, occlusion flags).
+enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3, kModeFused = 4 };
 
 // FP64 recomputation of (t, u, v) for the winning triangle (Moller & Trumbore 1997) in object space.
 __device__ void refine_hit(const double* inv, const float* wo, const float* wd, const TriSlot* tri, float& t,
@@ -297,7 +301,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     const uint32_t tlas_ninst = th->prim_count;
     const uint32_t D = args.lds_depth;
     const uint32_t GD = args.gdepth;
-    const uint32_t nrays = args.n_ptr ? *args.n_ptr : args.n;
+    const uint32_t nA = args.n_ptr ? *args.n_ptr : args.n;
+    const uint32_t nrays = MODE == kModeFused ? nA + *args.n2_ptr : nA;
 
     uint64_t st_nodes = 0, st_tris = 0, st_insts = 0, st_rays = 0;
 
@@ -317,6 +322,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     uint32_t sp = 0;
     uint32_t hit_inst = kNoHit, hit_slot = 0;
     bool occl = false;
+    bool shadow_ray = MODE == kModeAny;  // this lane's ray is an any-hit query
 
     // Wave-uniform pool of ray ids [pool_next, pool_end): one atomic fetches `chunk` consecutive ids (256 when the
     // launch has at least 8 rounds of work for every wave, else 64 so that small launches still spread over all
@@ -362,7 +368,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         wtmin = args.tmin;
                         tc = args.tmax;
                     } else {
-                        const float4* rp = (const float4*)(args.rays + id);
+                        if (MODE == kModeFused) shadow_ray = id >= nA;
+                        const float4* rp = (const float4*)(MODE == kModeFused && id >= nA ? args.rays2 + (id - nA)
+                                                                                          : args.rays + id);
                         const float4 r0 = rp[0], r1 = rp[1];
                         wo[0] = r0.x;
                         wo[1] = r0.y;
@@ -568,7 +576,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         if (test_triangle(a, b, c, s, sh, smin, smax, th) &&
                             !(ray_flags & (kRayCullBack | kRayCullFront) &&
                               facing_culled(a, b, c, s, ray_flags, table[cur_inst].flags))) {
-                            if (MODE == kModeAny) {
+                            if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
                                 occl = true;
                                 done = true;
                                 break;
@@ -637,8 +645,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
 
             if (done) {
                 // write the result of ray_id
-                if (MODE == kModeAny) {
-                    args.occluded[ray_id] = occl ? 1u : 0u;
+                if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
+                    args.occluded[MODE == kModeFused ? ray_id - nA : ray_id] = occl ? 1u : 0u;
                 } else {
                     float t = 0.0f, u = 0.0f, v = 0.0f;
                     uint32_t prim = 0, geom = 0;
@@ -654,7 +662,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         prim = tri->prim;
                         geom = tri->geom;
                     }
-                    if (MODE == kModePath) {
+                    if (MODE == kModePath || MODE == kModeFused) {
                         PathHit ph;
                         ph.t = t;
                         ph.u = u;
@@ -780,6 +788,8 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.tlas = (const uint8_t*)r.tlas;
     a.rays = r.rays;
     a.n_ptr = r.n_ptr;
+    a.rays2 = r.rays2;
+    a.n2_ptr = r.n2_ptr;
     a.path_hits = r.path_hits;
     a.n = r.n;
     a.cull_mask = r.cull_mask;
@@ -815,6 +825,7 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
         case 0: return r.stats ? launch<kModeClosest, true>(a, L, s) : launch<kModeClosest, false>(a, L, s);
         case 1: return r.stats ? launch<kModeAny, true>(a, L, s) : launch<kModeAny, false>(a, L, s);
         case 3: return launch_variant<kModePath, false, false>(a, L, s);
+        case 4: return launch_variant<kModeFused, false, false>(a, L, s);
         default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
     }
 }
