@@ -43,6 +43,12 @@ struct rtx_context_t {
     hipGraphExec_t graph = nullptr;
     uint64_t graph_key = 0;
     uint32_t* sample_word = nullptr;
+    // training wavefront of NRC frames: side stream, fork / join events, its ray counter and spill stack
+    hipStream_t side_stream = nullptr;
+    hipEvent_t side_fork = nullptr, side_join = nullptr;
+    rtx::DeviceState* side_state = nullptr;
+    uint2* side_gstack = nullptr;
+    uint64_t side_gstack_entries = 0;
 };
 
 struct rtx_nrc_t {
@@ -55,6 +61,7 @@ struct rtx_nrc_t {
 namespace {
 
 constexpr uint32_t kMaxPrims = (1u << 30) - 1u;
+constexpr uint32_t kSideBlocks = 1u;  // 256-thread blocks per CU of the NRC training wavefront
 
 // splitmix64 (Steele, Lea & Flood 2014): deterministic weight initialisation
 uint64_t splitmix64(uint64_t& s) {
@@ -137,6 +144,46 @@ int ensure_gstack(rtx_context ctx, uint32_t grid) {
     return RTX_SUCCESS;
 }
 
+// Side stream, events, ray counter and spill stack of the NRC training wavefront (created on first use), and its
+// launch shape: 256 threads per CU of the 128-VGPR traversal variant, which fits beside three blocks of the render
+// wavefront's on every SIMD.
+int side_chain(rtx_context ctx, const rtx::TraceLaunch& L, rtx::SideChain& side) {
+    hipError_t e = hipSuccess;
+    if (!ctx->side_stream) e = hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking);
+    if (e == hipSuccess && !ctx->side_fork) e = hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming);
+    if (e == hipSuccess && !ctx->side_join) e = hipEventCreateWithFlags(&ctx->side_join, hipEventDisableTiming);
+    if (e == hipSuccess && !ctx->side_state) {
+        e = hipMalloc(&ctx->side_state, sizeof(rtx::DeviceState));
+        if (e == hipSuccess) e = hipMemset(ctx->side_state, 0, sizeof(rtx::DeviceState));
+        if (e != hipSuccess && ctx->side_state) {
+            (void)hipFree(ctx->side_state);
+            ctx->side_state = nullptr;
+        }
+    }
+    if (e != hipSuccess) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc side stream: %s", hipGetErrorString(e));
+    rtx::TraceLaunch S = L;
+    S.blocks_per_cu = kSideBlocks * 256u / L.block_size > 0 ? kSideBlocks * 256u / L.block_size : 1u;
+    S.grid = (uint32_t)ctx->cu_count * S.blocks_per_cu;
+    S.w4 = true;
+    const uint64_t need = (uint64_t)S.grid * S.block_size * S.gdepth;
+    if (need > ctx->side_gstack_entries) {
+        if (ctx->side_gstack) (void)hipFree(ctx->side_gstack);
+        ctx->side_gstack = nullptr;
+        ctx->side_gstack_entries = 0;
+        e = hipMalloc(&ctx->side_gstack, need * sizeof(uint2));
+        if (e != hipSuccess)
+            return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc spill stack: %s", hipGetErrorString(e));
+        ctx->side_gstack_entries = need;
+    }
+    side.stream = ctx->side_stream;
+    side.fork = ctx->side_fork;
+    side.join = ctx->side_join;
+    side.state = ctx->side_state;
+    side.gstack = ctx->side_gstack;
+    side.launch = S;
+    return RTX_SUCCESS;
+}
+
 rtx::TraceLaunch launch_shape(rtx_context ctx) {
     rtx::TraceLaunch L;
     L.block_size = ctx->cfg.block_size;
@@ -148,6 +195,7 @@ rtx::TraceLaunch launch_shape(rtx_context ctx) {
     L.refill_lanes = ctx->cfg.refill_lanes;
     L.refill_interval = ctx->cfg.refill_interval;
     L.leaf_lanes = ctx->cfg.leaf_lanes;
+    L.w4 = (L.block_size / 64u) * L.blocks_per_cu / 4u >= 4u;
     return L;
 }
 
@@ -190,6 +238,12 @@ int rtx_destroy_context(rtx_context ctx) {
     if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
     (void)hipStreamSynchronize(ctx->stream);
     for (rtx_nrc n : ctx->caches) n->ctx = nullptr;  // they can still be destroyed, nothing else
+    if (ctx->side_stream) (void)hipStreamSynchronize(ctx->side_stream);
+    if (ctx->side_gstack) (void)hipFree(ctx->side_gstack);
+    if (ctx->side_state) (void)hipFree(ctx->side_state);
+    if (ctx->side_fork) (void)hipEventDestroy(ctx->side_fork);
+    if (ctx->side_join) (void)hipEventDestroy(ctx->side_join);
+    if (ctx->side_stream) (void)hipStreamDestroy(ctx->side_stream);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     if (ctx->sample_word) (void)hipFree(ctx->sample_word);
@@ -828,10 +882,17 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
         if (e) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc training buffers: %s", hipGetErrorString(e));
         nrc->train_buf_bytes = need;
     }
+    rtx::SideChain side{};
+    if ((r = side_chain(ctx, L, side))) return r;
+    // the render wavefront leaves `kSideBlocks` blocks per CU to the training wavefront
+    if (L.blocks_per_cu > side.launch.blocks_per_cu) {
+        L.blocks_per_cu -= side.launch.blocks_per_cu;
+        L.grid = (uint32_t)ctx->cu_count * L.blocks_per_cu;
+    }
     uint32_t count = 0;
-    hipError_t e = rtx::pt_render_nrc(a, L, nrc->st, train_paths, nrc->st.train_suffix, nrc->train_buf, params->spp,
-                                      params->sample_offset, film_rgba32f_device, accum_rgb_device, &count,
-                                      ctx->stream);
+    hipError_t e = rtx::pt_render_nrc(a, L, side, nrc->st, train_paths, nrc->st.train_suffix, nrc->train_buf,
+                                      params->spp, params->sample_offset, film_rgba32f_device, accum_rgb_device,
+                                      &count, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render_nrc");
     return RTX_SUCCESS;
 }
@@ -884,6 +945,7 @@ int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc)
     st.bwd = nullptr;
     if (!e) e = hipMalloc(&st.partials, (uint64_t)st.max_blocks * pbytes);
     if (!e) e = hipMalloc(&st.loss, sizeof(float) * 4);
+    if (!e) e = hipMalloc(&st.loss_part, sizeof(float) * st.max_blocks);
     if (e) {
         rtx_nrc_destroy(n);
         return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc buffers: %s", hipGetErrorString(e));
@@ -921,6 +983,7 @@ int rtx_nrc_destroy(rtx_nrc n) {
     if (st.fwd) (void)hipFree(st.fwd);
     if (st.partials) (void)hipFree(st.partials);
     if (st.loss) (void)hipFree(st.loss);
+    if (st.loss_part) (void)hipFree(st.loss_part);
     if (n->train_buf) (void)hipFree(n->train_buf);
     delete n;
     return RTX_SUCCESS;

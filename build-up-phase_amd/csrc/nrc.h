@@ -15,6 +15,7 @@ struct NrcState {
     void* bwd = nullptr;        // BF16 backward (transposed) fragments
     float* partials = nullptr;  // per-block gradients of one step
     float* loss = nullptr;      // mean loss of the last step
+    float* loss_part = nullptr; // per-block losses of one step
     uint32_t max_blocks = 0;
     uint32_t batch_size = 0;
     uint32_t steps = 0;

@@ -15,8 +15,9 @@
 // Training: one 256-thread block = 4 waves x 16 samples.  Forward activations are also written to LDS as
 // A^T[unit][64 samples] (bf16); the backward data gradient uses W^T as the A operand (same trick); the weight
 // gradient dW = G^T (out x 64) * A (64 x in) is an MFMA GEMM over the 64 samples of the block whose A operand
-// (G^T rows) and B operand (A^T rows) are 16-byte LDS reads.  Each block writes its partial gradient; a second
-// kernel sums the partials in block order (deterministic) and applies Adam.
+// (G^T rows) and B operand (A^T rows) are 16-byte LDS reads.  Each block writes its partial gradient and loss; a
+// second kernel sums the partials in block order (deterministic), applies Adam and writes the updated weights'
+// BF16 fragments.
 #include <cmath>
 
 #include "nrc.h"
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict
                                                         const float* __restrict__ params,
                                                         const rtx_nrc_sample* __restrict__ samples, uint32_t n,
                                                         uint32_t stride, uint32_t offset, float inv_n, Norm nm,
-                                                        float* __restrict__ partials, float* __restrict__ loss_sum) {
+                                                        float* __restrict__ partials, float* __restrict__ loss_part) {
     __shared__ TrainLds sm;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -338,16 +339,56 @@ __global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
     if (lane == 0) sm.red[wave] = loss;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(loss_sum, (sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3]) * inv_n);
+    if (threadIdx.x == 0) loss_part[blockIdx.x] = (sm.red[0] + sm.red[1] + sm.red[2] + sm.red[3]) * inv_n;
 }
 
-// Sum the block partials and take one Adam step, or only write the gradient.  A block covers 16 parameters with
-// 16 threads each; thread k sums partials k, k + 16, ... and a fixed LDS tree adds the 16 slices (deterministic).
+// BF16 copy of weight p into its forward fragment slot and, for layers 1..5, its backward fragment slot: the
+// inverse of the gathers in nrc_pack_kernel (biases are read in FP32 and have no slot).
+__device__ __forceinline__ void scatter_weight(int p, float w, __bf16* __restrict__ fwd, __bf16* __restrict__ bwd) {
+    int l = 5;
+    while (l > 0 && p < w_off(l)) --l;
+    const int local = p - w_off(l);
+    if (local >= out_dim(l) * in_dim(l)) return;
+    const int row = local / in_dim(l), col = local % in_dim(l);
+    const __bf16 b = (__bf16)w;
+    {  // forward: tile t = row / 16, step s = col / 32; k order 8g + j (input layer) or kperm
+        const int S = in_dim(l) / 32, t = row >> 4, r = row & 15, s = col >> 5, k = col & 31;
+        const int g = l == 0 ? k >> 3 : (k & 15) >> 2;
+        const int j = l == 0 ? k & 7 : ((k >> 4) << 2) + (k & 3);
+        fwd[((fwd_base(l) + t * S + s) * 64 + g * 16 + r) * 8 + j] = b;
+    }
+    if (l >= 1) {  // backward: tile t = col / 16, step s = row / 32, k order kperm over output rows
+        const int S = l == 5 ? 1 : 2, t = col >> 4, r = col & 15, s = row >> 5, k = row & 31;
+        const int g = (k & 15) >> 2, j = ((k >> 4) << 2) + (k & 3);
+        bwd[((bwd_base(l) + t * S + s) * 64 + g * 16 + r) * 8 + j] = b;
+    }
+}
+
+// Sum the block partials and take one Adam step (writing the BF16 fragments of the updated weights), or only
+// write the gradient.  A block covers 16 parameters with 16 threads each; thread k sums partials k, k + 16, ...
+// and a fixed LDS tree adds the 16 slices (deterministic).  The extra last block sums the per-block losses the
+// same way.
 __global__ __launch_bounds__(256) void nrc_adam_kernel(const float* __restrict__ partials, uint32_t nblk,
                                                        float* __restrict__ params, float* __restrict__ m,
                                                        float* __restrict__ v, float lr, float b1, float b2, float eps,
-                                                       float c1, float c2, float* __restrict__ grad_out) {
+                                                       float c1, float c2, float* __restrict__ grad_out,
+                                                       __bf16* __restrict__ fwd, __bf16* __restrict__ bwd,
+                                                       const float* __restrict__ loss_part,
+                                                       float* __restrict__ loss_out) {
     __shared__ float s[16][17];
+    if (blockIdx.x == gridDim.x - 1) {
+        __shared__ float ls[256];
+        float a = 0.0f;
+        for (uint32_t b = threadIdx.x; b < nblk; b += 256) a += loss_part[b];
+        ls[threadIdx.x] = a;
+        __syncthreads();
+        for (uint32_t h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) ls[threadIdx.x] += ls[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) *loss_out = ls[0];
+        return;
+    }
     const uint32_t pi = threadIdx.x & 15u, k = threadIdx.x >> 4;
     const uint32_t p = blockIdx.x * 16u + pi;
     float acc = 0.0f;
@@ -369,7 +410,9 @@ __global__ __launch_bounds__(256) void nrc_adam_kernel(const float* __restrict__
     const float vv = b2 * v[p] + (1.0f - b2) * gsum * gsum;
     m[p] = mm;
     v[p] = vv;
-    params[p] -= lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+    const float w = params[p] - lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+    params[p] = w;
+    scatter_weight((int)p, w, fwd, bwd);
 }
 
 // Rebuild the BF16 MFMA fragments from the FP32 master parameters.
@@ -407,11 +450,6 @@ __global__ __launch_bounds__(256) void nrc_pack_kernel(const float* __restrict__
     }
 }
 
-__global__ void nrc_zero_kernel(float* p, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0f;
-}
-
 }  // namespace
 
 uint64_t nrc_fragment_bytes() { return (uint64_t)(kFwdFrags + kBwdFrags) * 64 * sizeof(bf16x8); }
@@ -447,20 +485,23 @@ hipError_t nrc_step(NrcState& st, const rtx_nrc_sample* samples, uint32_t n, flo
     const uint32_t nblk = (n + 63) / 64;
     if (nblk > st.max_blocks) return hipErrorInvalidValue;
     float* loss = loss_out ? loss_out : st.loss;
-    nrc_zero_kernel<<<1, 64, 0, s>>>(loss, 1);
     nrc_train_kernel<<<nblk, 256, 0, s>>>((const bf16x8*)st.fwd, (const bf16x8*)st.bwd, st.params, samples, n,
-                                          stride, offset, 1.0f / (float)n, make_norm(st), st.partials, loss);
-    const unsigned pb = (RTX_NRC_PARAM_COUNT + 15) / 16;
+                                          stride, offset, 1.0f / (float)n, make_norm(st), st.partials,
+                                          st.loss_part);
+    // one block per 16 parameters + the loss block
+    const unsigned pb = (RTX_NRC_PARAM_COUNT + 15) / 16 + 1;
     if (grad_out) {
-        nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, 0, 0, 0, 0, 0, 0, grad_out);
+        nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, 0, 0, 0, 0, 0, 0, grad_out,
+                                           nullptr, nullptr, st.loss_part, loss);
         return hipGetLastError();
     }
     st.steps += 1;
     const double c1 = 1.0 / (1.0 - std::pow((double)st.beta1, (double)st.steps));
     const double c2 = 1.0 / (1.0 - std::pow((double)st.beta2, (double)st.steps));
     nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, st.lr, st.beta1, st.beta2, st.eps,
-                                       (float)c1, (float)c2, nullptr);
-    return nrc_pack(st, s);
+                                       (float)c1, (float)c2, nullptr, (__bf16*)st.fwd, (__bf16*)st.bwd, st.loss_part,
+                                       loss);
+    return hipGetLastError();
 }
 
 }  // namespace rtx

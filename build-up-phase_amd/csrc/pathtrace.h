@@ -28,6 +28,7 @@ struct PtArgs {
     const void* tlas;
     uint32_t width, height, npix;
     uint32_t npaths;       // path slots of this pass: npix render paths + ntrain training paths
+    uint32_t first_path;   // this wavefront traces paths [first_path, npaths); queue slots count from first_path
     uint32_t ntrain;       // training paths (slots npix .. npix + ntrain - 1; cache mode)
     uint32_t mode;         // PtMode
     uint32_t cache_depth;  // vertex at which cache-mode render paths query the cache
@@ -55,6 +56,7 @@ struct PtArgs {
     float4* rad;
     float4* accum;
     uint32_t* counts;  // per depth d: [3d] extension rays, [3d+1] shadow rays; [2] cache queries of the pass
+    uint32_t* train_counts;  // the same for the training wavefront of an NRC frame
     float* cdf;
     rtx_nrc_query* cq;  // cache queries
     uint32_t* cq_pid;
@@ -76,12 +78,24 @@ hipError_t pt_render(PtArgs a, const TraceLaunch& launch, uint32_t spp, uint32_t
 hipError_t pt_render_batch(PtArgs a, const TraceLaunch& launch, uint32_t spp, uint32_t sample_offset, float* films,
                            double* accum64, hipStream_t stream);
 uint64_t pt_train_bytes(uint32_t train_paths, uint32_t record_depths);
+
+// Resources of the training wavefront of an NRC frame, which runs on its own stream beside the render paths: the
+// persistent traversal of each wavefront needs its own ray counter (DeviceState) and spill stack.
+struct SideChain {
+    hipStream_t stream;
+    hipEvent_t fork, join;
+    DeviceState* state;
+    uint2* gstack;
+    TraceLaunch launch;  // launch shape of the training wavefront's traversals
+};
 // vertices per training path that become samples
 uint32_t nrc_record_depths(uint32_t max_depth, uint32_t cache_depth, uint32_t train_suffix);
 // one frame: spp cache-terminated passes, the first carrying train_paths training paths (self-training when
-// train_suffix > 0), then the Adam steps (synchronises once to read the sample count)
-hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& launch, NrcState& nrc, uint32_t train_paths,
-                         uint32_t train_suffix, void* train_buf, uint32_t spp, uint32_t sample_offset, float* film,
-                         double* accum64, uint32_t* host_count, hipStream_t stream);
+// train_suffix > 0) as a second wavefront on side.stream, then the Adam steps (synchronises once to read the
+// sample count)
+hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& launch, const SideChain& side, NrcState& nrc,
+                         uint32_t train_paths, uint32_t train_suffix, void* train_buf, uint32_t spp,
+                         uint32_t sample_offset, float* film, double* accum64, uint32_t* host_count,
+                         hipStream_t stream);
 
 }  // namespace rtx

@@ -135,9 +135,11 @@ __device__ __forceinline__ void path_xy(const PtArgs& a, uint32_t pid, uint32_t&
     }
 }
 
+// Camera rays of paths [first_path, npaths): path i enters the extension queue at slot i - first_path.
 __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) a.counts[0] = a.npaths;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = a.first_path + slot;
+    if (slot == 0) a.counts[0] = a.npaths - a.first_path;
     if (i >= a.npaths) return;
     sample = pass_sample(a, sample);
     uint32_t px, py;
@@ -163,8 +165,8 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
         r.direction[1] = -(ny * v.ay);
         r.direction[2] = -1.0f;
         r.tmax = clip_exit(v, r.origin, r.direction);
-        a.ext_rays[0][i] = r;
-        a.ext_pid[0][i] = i;
+        a.ext_rays[0][slot] = r;
+        a.ext_pid[0][slot] = i;
         a.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         a.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
@@ -195,8 +197,8 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
     r.direction[1] = -(ny * a.ay);
     r.direction[2] = -1.0f;
     r.tmax = INFINITY;
-    a.ext_rays[0][i] = r;
-    a.ext_pid[0][i] = i;
+    a.ext_rays[0][slot] = r;
+    a.ext_pid[0][slot] = i;
     a.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     a.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
@@ -432,7 +434,16 @@ __global__ void fill_words_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t
 }
 
 __global__ void count_camera_rays_kernel(DeviceState* st, uint32_t n) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) st->pt_rays += n;
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&st->pt_rays, (unsigned long long)n);
+}
+
+// Folds the training wavefront's ray count and error flags into the context's state and clears them.
+__global__ void merge_state_kernel(DeviceState* dst, DeviceState* src) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    atomicAdd(&dst->pt_rays, src->pt_rays);
+    atomicOr(&dst->error_flags, src->error_flags);
+    src->pt_rays = 0;
+    src->error_flags = 0;
 }
 
 // ---- training samples from vertex records: block count -> scan -> scatter (deterministic order) ----------------
@@ -521,7 +532,7 @@ uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
     b += align_up(n * sizeof(rtx_ray), 256) + align_up(n * 4, 256) + align_up(n * 16, 256);  // shadow queue
     b += align_up(n * 4, 256);                                                // occluded
     b += 3 * align_up(n * 16, 256);                                           // thr, rad, accum
-    b += align_up((3ull * (max_depth + 2)) * 4, 256);                          // counts
+    b += 2 * align_up((3ull * (max_depth + 2)) * 4, 256);                      // counts (two wavefronts)
     b += align_up((nlights + 1ull) * 4, 256);                                 // light cdf
     b += align_up(n * 4, 256);                                                // path view
     b += align_up(n * sizeof(rtx_nrc_query), 256) + align_up(n * 4, 256) + align_up(n * 12, 256);  // cache queries
@@ -549,6 +560,7 @@ void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint
     a.rad = (float4*)take(n * 16);
     a.accum = (float4*)take(n * 16);
     a.counts = (uint32_t*)take(3ull * (max_depth + 2) * 4);
+    a.train_counts = (uint32_t*)take(3ull * (max_depth + 2) * 4);
     a.cdf = (float*)take((nlights + 1ull) * 4);
     a.path_view = (uint32_t*)take(n * 4);
     a.cq = (rtx_nrc_query*)take(n * sizeof(rtx_nrc_query));
@@ -564,11 +576,12 @@ uint64_t pt_train_bytes(uint32_t train_paths, uint32_t record_depths) {
 
 // One pass: vertices 0 .. last_depth (cache queries stay queued for the caller).
 static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uint32_t last_depth, hipStream_t s) {
-    const unsigned gb = (a.npaths + 255) / 256;
+    const uint32_t n = a.npaths - a.first_path;
+    const unsigned gb = (n + 255) / 256;
     const uint64_t ncounts = 3ull * (a.max_depth + 2);
     fill_words_kernel<<<1, 64, 0, s>>>(a.counts, ncounts, 0u);
     camera_kernel<<<gb, 256, 0, s>>>(a, sample);
-    count_camera_rays_kernel<<<1, 64, 0, s>>>(a.state, a.npaths);
+    count_camera_rays_kernel<<<1, 64, 0, s>>>(a.state, n);
     hipError_t e;
     // vertex 0's extension rays (camera rays) alone; afterwards vertex d's shadow rays and vertex d + 1's extension
     // rays, both produced by shade(d), go through one fused launch
@@ -646,9 +659,9 @@ hipError_t pt_render_batch(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_
     return hipGetLastError();
 }
 
-hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, NrcState& nrc, uint32_t train_paths, uint32_t train_suffix,
-                         void* train_buf, uint32_t spp, uint32_t sample_offset, float* film, double* acc64,
-                         uint32_t* host_count, hipStream_t s) {
+hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, NrcState& nrc,
+                         uint32_t train_paths, uint32_t train_suffix, void* train_buf, uint32_t spp,
+                         uint32_t sample_offset, float* film, double* acc64, uint32_t* host_count, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
     hipError_t e;
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
@@ -656,12 +669,19 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, NrcState& nrc, uint32_t
     a.mode = kPtCache;
     a.tail = train_suffix > 0 ? 1u : 0u;
     a.train_end = a.tail ? std::min(a.max_depth - 1, a.cache_depth + train_suffix) : a.max_depth - 1;
+    a.first_path = 0;
+    a.npaths = a.npix;
     const uint32_t rec_depths = nrc_record_depths(a.max_depth, a.cache_depth, train_suffix);
     for (uint32_t si = 0; si < spp; ++si) {
         // the first sample carries the frame's training paths
         const uint32_t ntrain = si == 0 ? train_paths : 0u;
         a.ntrain = ntrain;
-        a.npaths = a.npix + ntrain;
+        if (ntrain == 0) {
+            if ((e = run_pass(a, L, sample_offset + si, a.cache_depth, s))) return e;
+            if ((e = infer_gather(a, nrc, a.counts + 2, 0, a.npix, s))) return e;
+            accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
+            continue;
+        }
         const uint64_t nrec = (uint64_t)ntrain * rec_depths;
         const uint64_t nblk = (nrec + 255) / 256;
         uint8_t* p = (uint8_t*)train_buf;
@@ -672,17 +692,39 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, NrcState& nrc, uint32_t
         uint32_t* block_counts = (uint32_t*)p;
         p += align_up(nblk * 4, 256);
         uint32_t* total = (uint32_t*)p;
-        if (ntrain) fill_words_kernel<<<1024, 256, 0, s>>>((uint32_t*)a.rec, nrec * 20, 0u);
-        const uint32_t last = ntrain ? std::max(a.cache_depth, a.train_end) : a.cache_depth;
-        if ((e = run_pass(a, L, sample_offset + si, last, s))) return e;
-        if (ntrain == 0) {
-            if ((e = infer_gather(a, nrc, a.counts + 2, 0, a.npix, s))) return e;
-            accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
-            continue;
+        // Training paths [npix, npix + ntrain) form a second wavefront on the side stream: its queues are the
+        // slots from npix of the pass's queue buffers, its counters train_counts, its traversals count rays on
+        // side.state.  Path slots, queue slots, cache-query slots (tails from npix) and records of the two
+        // wavefronts are disjoint, and every path draws its random numbers from (pixel or path index, sample,
+        // depth), so the frame's results do not depend on how the two interleave.
+        PtArgs t = a;
+        t.first_path = a.npix;
+        t.npaths = a.npix + ntrain;
+        for (int k = 0; k < 2; ++k) {
+            t.ext_rays[k] = a.ext_rays[k] + a.npix;
+            t.ext_pid[k] = a.ext_pid[k] + a.npix;
         }
-        // training tails first (self-training), then the samples and the Adam steps, then the render queries with
-        // the updated cache
-        if (a.tail && (e = infer_gather(a, nrc, a.counts + tail_counter(a), a.npix, ntrain, s))) return e;
+        t.hits = a.hits + a.npix;
+        t.sh_rays = a.sh_rays + a.npix;
+        t.sh_pid = a.sh_pid + a.npix;
+        t.sh_contrib = a.sh_contrib + a.npix;
+        t.occl = a.occl + a.npix;
+        t.counts = a.train_counts;
+        t.state = side.state;
+        t.gstack = side.gstack;
+        if ((e = hipEventRecord(side.fork, s))) return e;
+        if ((e = hipStreamWaitEvent(side.stream, side.fork, 0))) return e;
+        // render paths up to their cache vertex (a few long launches, enqueued first so that the GPU has them
+        // while the host enqueues the training wavefront's many short ones)
+        if ((e = run_pass(a, L, sample_offset + si, a.cache_depth, s))) return e;
+        fill_words_kernel<<<1024, 256, 0, side.stream>>>((uint32_t*)a.rec, nrec * 20, 0u);
+        if ((e = run_pass(t, side.launch, sample_offset + si, t.train_end, side.stream))) return e;
+        // training tails (self-training) are predicted with the cache before this frame's Adam steps
+        if (t.tail && (e = infer_gather(t, nrc, t.counts + tail_counter(t), a.npix, ntrain, side.stream))) return e;
+        merge_state_kernel<<<1, 64, 0, side.stream>>>(a.state, side.state);
+        if ((e = hipEventRecord(side.join, side.stream))) return e;
+        if ((e = hipStreamWaitEvent(s, side.join, 0))) return e;
+        // the samples and the Adam steps, then the render queries with the updated cache
         sample_count_kernel<<<(unsigned)nblk, 256, 0, s>>>(a, nrec, block_counts);
         sample_scan_kernel<<<1, 1024, 0, s>>>(block_counts, (uint32_t)nblk, total);
         sample_scatter_kernel<<<(unsigned)nblk, 256, 0, s>>>(a, nrec, block_counts, samples);

@@ -763,8 +763,7 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
 template <int MODE, bool STATS, bool FLAGS>
 hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
     const size_t lds = (size_t)L.lds_depth * L.block_size * sizeof(uint2);
-    const uint32_t waves_per_simd = (L.block_size / 64u) * L.blocks_per_cu / 4u;
-    if (waves_per_simd >= 4u)
+    if (L.w4)
         trace_kernel_w4<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
     else
         trace_kernel<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);

@@ -101,6 +101,29 @@ def test_adam_step_matches_restatement(ctx):
     nrc.close()
 
 
+def test_adam_writes_the_packed_weights(ctx):
+    """Adam writes each updated weight's BF16 fragment slots itself: inference after training must equal, bit for
+    bit, inference after re-packing the same parameters from scratch; the step's loss equals the oracle's."""
+    import pyrtx
+    _say("nrc adam fragments")
+    nrc = pyrtx.Nrc(ctx, LO, HI, seed=4, batch_size=2048)
+    train = torch.from_numpy(random_samples(2048 * 3, 12)).cuda()
+    for _ in range(2):
+        nrc.train(train)
+    q = torch.from_numpy(np.ascontiguousarray(random_samples(3001, 13)[:, :12])).cuda()
+    y_fused = nrc.infer(q).cpu().numpy()
+    p = nrc.params()
+    nrc.set_params(p)
+    y_repacked = nrc.infer(q).cpu().numpy()
+    assert np.array_equal(y_fused, y_repacked)
+    # the last batch's loss (summed per block, then over blocks) matches the oracle's loss for those samples
+    last = train[2048 * 2:].cpu().numpy()
+    _, loss = nrc.gradient(train[2048 * 2:])
+    _, _, oloss = O.nrc(p, last, np.array(LO, np.float32), np.array(HI, np.float32), bf16=True)
+    assert abs(loss - oloss) / oloss < 1e-3
+    nrc.close()
+
+
 def test_training_learns_a_smooth_field(ctx):
     import pyrtx
     _say("nrc training")
