@@ -1283,4 +1283,68 @@ int rtx_debug_sort(rtx_context ctx, void* keys_device, uint32_t* values_device, 
     return rtx_sync(ctx);
 }
 
+// Name -> entry point, as vkGetDeviceProcAddr resolves the reference's KHR functions (main.cpp:25-33, 137-145).
+void* rtx_get_proc_addr(const char* name) {
+    struct Proc {
+        const char* name;
+        void* fn;
+    };
+#define RTX_PROC(f) {#f, (void*)&f}
+    static const Proc kProcs[] = {
+        RTX_PROC(rtx_create_context),
+        RTX_PROC(rtx_destroy_context),
+        RTX_PROC(rtx_last_error_string),
+        RTX_PROC(rtx_sync),
+        RTX_PROC(rtx_get_stream),
+        RTX_PROC(rtx_get_blas_build_sizes),
+        RTX_PROC(rtx_get_tlas_build_sizes),
+        RTX_PROC(rtx_cmd_build_blas),
+        RTX_PROC(rtx_get_blas_batch_build_sizes),
+        RTX_PROC(rtx_cmd_build_blas_batch),
+        RTX_PROC(rtx_cmd_update_blas),
+        RTX_PROC(rtx_cmd_build_tlas),
+        RTX_PROC(rtx_get_as_info),
+        RTX_PROC(rtx_get_as_compacted_size),
+        RTX_PROC(rtx_cmd_copy_as),
+        RTX_PROC(rtx_obj_load),
+        RTX_PROC(rtx_obj_parse),
+        RTX_PROC(rtx_obj_free),
+        RTX_PROC(rtx_cmd_trace_rays),
+        RTX_PROC(rtx_cmd_trace_ray_buffer),
+        RTX_PROC(rtx_set_trace_stats),
+        RTX_PROC(rtx_get_trace_stats),
+        RTX_PROC(rtx_set_trace_config),
+        RTX_PROC(rtx_get_trace_config),
+        RTX_PROC(rtx_autotune_trace),
+        RTX_PROC(rtx_cmd_film_to_rgba8),
+        RTX_PROC(rtx_cmd_render),
+        RTX_PROC(rtx_cmd_render_batch),
+        RTX_PROC(rtx_autotune_render_batch),
+        RTX_PROC(rtx_autotune_render),
+        RTX_PROC(rtx_get_render_ray_count),
+        RTX_PROC(rtx_nrc_create),
+        RTX_PROC(rtx_nrc_destroy),
+        RTX_PROC(rtx_nrc_cmd_infer),
+        RTX_PROC(rtx_nrc_cmd_train),
+        RTX_PROC(rtx_nrc_get_params),
+        RTX_PROC(rtx_nrc_set_params),
+        RTX_PROC(rtx_nrc_get_status),
+        RTX_PROC(rtx_cmd_render_nrc),
+        RTX_PROC(rtx_debug_lbvh),
+        RTX_PROC(rtx_debug_treelet),
+        RTX_PROC(rtx_debug_sort),
+        RTX_PROC(rtx_nrc_debug_gradient),
+        RTX_PROC(rtx_get_proc_addr),
+        // reference calls that one librtx function replaces (with librtx's signature)
+        {"vkCmdTraceRaysKHR", (void*)&rtx_cmd_trace_rays},
+        {"vkCmdCopyAccelerationStructureKHR", (void*)&rtx_cmd_copy_as},
+        {"vkQueueWaitIdle", (void*)&rtx_sync},
+    };
+#undef RTX_PROC
+    if (!name) return nullptr;
+    for (const Proc& p : kProcs)
+        if (std::strcmp(p.name, name) == 0) return p.fn;
+    return nullptr;
+}
+
 }  // extern "C"

@@ -159,6 +159,7 @@ SIGNATURES = {
     "rtx_last_error_string": (ctypes.c_char_p, [_P]),
     "rtx_sync": (ctypes.c_int, [_P]),
     "rtx_get_stream": (_P, [_P]),
+    "rtx_get_proc_addr": (_P, [ctypes.c_char_p]),
     "rtx_get_blas_build_sizes": (ctypes.c_int, [_P, _P, _U32, _U32, ctypes.POINTER(BuildSizes)]),
     "rtx_get_tlas_build_sizes": (ctypes.c_int, [_P, _U32, _U32, ctypes.POINTER(BuildSizes)]),
     "rtx_cmd_build_blas": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _U64, _P, _U64, ctypes.POINTER(_U64)]),

@@ -531,6 +531,16 @@ int rtx_debug_sort(rtx_context ctx, void* keys_device, uint32_t* values_device, 
 int rtx_nrc_debug_gradient(rtx_nrc nrc, const rtx_nrc_sample* samples_device, uint32_t count, float* grad_device,
                            float* loss_device);
 
+/* ---- entry-point lookup ---------------------------------------------------------------------------------------- */
+
+/* Address of the entry point `name`, or NULL: every rtx_* function of this header by its own name (as
+ * vkGetDeviceProcAddr resolves the reference's KHR entry points, main.cpp:25-33, 137-145), and the reference calls
+ * that a single librtx function replaces - "vkCmdTraceRaysKHR" -> rtx_cmd_trace_rays,
+ * "vkCmdCopyAccelerationStructureKHR" -> rtx_cmd_copy_as, "vkQueueWaitIdle" -> rtx_sync - with librtx's
+ * signature, not Vulkan's.  Calls that map to several librtx functions (the build-size query and the build, which
+ * librtx splits by BLAS / TLAS) resolve to NULL under their Vulkan names. */
+void* rtx_get_proc_addr(const char* name);
+
 #ifdef __cplusplus
 }
 #endif

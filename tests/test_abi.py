@@ -32,6 +32,24 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="librtx.so not built")
+def test_proc_addr_resolves_every_entry_point():
+    lib = ctypes.CDLL(LIB)
+    get = lib.rtx_get_proc_addr
+    get.restype = ctypes.c_void_p
+    get.argtypes = [ctypes.c_char_p]
+    addr = lambda n: ctypes.cast(getattr(lib, n), ctypes.c_void_p).value
+    for n in declared_functions():
+        assert get(n.encode()) == addr(n), n
+    # reference calls with a single librtx replacement, and calls librtx splits (unresolved under Vulkan's name)
+    assert get(b"vkCmdTraceRaysKHR") == addr("rtx_cmd_trace_rays")
+    assert get(b"vkCmdCopyAccelerationStructureKHR") == addr("rtx_cmd_copy_as")
+    assert get(b"vkQueueWaitIdle") == addr("rtx_sync")
+    for n in [b"vkCmdBuildAccelerationStructuresKHR", b"vkGetAccelerationStructureBuildSizesKHR", b"rtx_nothing", b""]:
+        assert get(n) is None
+    assert get(None) is None
+
+
 def test_python_binding_covers_header():
     import pyrtx
     assert sorted(pyrtx.SIGNATURES) == declared_functions()

@@ -170,6 +170,22 @@ def test_r0_film_matches_oracle(ctx):
     assert (st == 2).sum() < 500
 
 
+def test_r0_film_matches_golden(ctx):
+    """The HIP film against the committed closed-form R0 samples (tests/golden/r0_film.npz), ties excluded."""
+    from test_golden import load_r0
+    _say("R0 film vs golden samples")
+    g, stride, rgba, tie = load_r0()
+    sc = S.scene_r0()
+    blases, tlas = ctx.upload_scene(sc)
+    film = ctx.trace_rays(tlas, sc.records, sc.camera, sc.width, sc.height, sc.miss_color,
+                          tmin=g["trace"]["tmin"], tmax=g["trace"]["tmax"])
+    ctx.sync()
+    f = film.cpu().numpy()[::stride, ::stride]
+    keep = ~tie
+    assert np.all(f[..., 3] == 0.0)
+    assert np.abs(f[keep] - rgba[keep]).max() <= 1e-5
+
+
 @pytest.mark.parametrize("z", [-2.5, 2.5])
 def test_c1_film_matches_oracle(ctx, z):
     _say(f"C1 camera z={z}")
