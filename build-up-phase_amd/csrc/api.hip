@@ -336,6 +336,8 @@ int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uin
         d.prim_base = base;
         d.max_vertex = x.max_vertex;
         d.geom_local = i;
+        d.flags = x.flags;
+        if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
         d.blas = 0;
         base += x.primitive_count;
     }
@@ -425,6 +427,8 @@ int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, 
             d.prim_base = base;
             d.max_vertex = x.max_vertex;
             d.geom_local = i;
+        d.flags = x.flags;
+        if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
             d.blas = b;
             base += x.primitive_count;
         }
@@ -474,6 +478,8 @@ int rtx_cmd_update_blas(rtx_context ctx, const rtx_blas_geometry* geometries, ui
         d.prim_base = base;
         d.max_vertex = x.max_vertex;
         d.geom_local = i;
+        d.flags = x.flags;
+        if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
         d.blas = 0;
         base += x.primitive_count;
     }
@@ -1094,7 +1100,8 @@ int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg) {
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "block_size must be 64, 128, 192 or 256");
     if (c.blocks_per_cu < 1 || c.blocks_per_cu > 32)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 32]");
-    if (c.lds_stack_depth < 1 || (uint64_t)c.lds_stack_depth * c.block_size * 8 > 64 * 1024)
+    // per lane: the stack entries (8 B each) and the world ray (24 B)
+    if (c.lds_stack_depth < 1 || ((uint64_t)c.lds_stack_depth * 8 + 24) * c.block_size > 64 * 1024)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "LDS stack above 64 KiB per block");
     ctx->cfg = c;
     return RTX_SUCCESS;

@@ -156,8 +156,8 @@ __global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restri
         for (int k = 0; k < 9; ++k) valid = valid && isfinite(t.v[k]);
         t.prim = p;
         t.geom = g.geom_local;
-        t.pad[0] = valid ? 1u : 0u;
-        t.pad[1] = t.pad[2] = t.pad[3] = t.pad[4] = 0u;
+        t.bits = (valid ? kTriValid : 0u) | ((g.flags & RTX_GEOMETRY_OPAQUE_BIT) ? 0u : kTriNonOpaque);
+        t.pad[0] = t.pad[1] = t.pad[2] = t.pad[3] = 0u;
         leaves[i] = t;
         float* b = boxes + 6 * (uint64_t)i;
         if (valid) {
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void prep_instances_kernel(const rtx_instance*
         for (int k = 0; k < 12; ++k) s.inv[k] = (float)inv[k];
         s.blas = r.blas_address;
         s.instance_id = i;
-        s.mask = info.mask;
+        s.mask = info.mask | (info.flags << 8);
         float* b = boxes + 6 * (uint64_t)i;
         if (ok) {
             const AsHeader* h = (const AsHeader*)r.blas_address;

@@ -589,6 +589,8 @@ static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uin
     t.tlas = a.tlas;
     t.cull_mask = 0xFFu;
     t.path_hits = a.hits;
+    t.records = a.records;  // any-hit programs of non-opaque geometry
+    t.nrec = a.nrec;
     t.sbt_offset = a.sbt_offset;
     t.sbt_stride = a.sbt_stride;
     t.state = a.state;

@@ -73,15 +73,18 @@ struct TriSlot {
     float v[9];  // v0, v1, v2 after the geometry transform
     uint32_t prim;
     uint32_t geom;
-    uint32_t pad[5];
+    uint32_t bits;  // kTriValid | kTriNonOpaque
+    uint32_t pad[4];
 };
+constexpr uint32_t kTriValid = 1u;
+constexpr uint32_t kTriNonOpaque = 2u;  // its geometry lacks RTX_GEOMETRY_OPAQUE_BIT
 static_assert(sizeof(TriSlot) == 64, "TriSlot must be 64 B");
 
 struct InstSlot {
     float inv[12];  // world -> object (FP32 copy)
     uint64_t blas;
     uint32_t instance_id;
-    uint32_t mask;
+    uint32_t mask;  // instance mask (bits 0-7) | instance flags << 8
 };
 static_assert(sizeof(InstSlot) == 64, "InstSlot must be 64 B");
 
@@ -108,6 +111,7 @@ struct GeomDesc {
     uint32_t max_vertex;
     uint32_t geom_local;    // geometry index within its BLAS (gl_GeometryIndexEXT)
     uint32_t blas;          // BLAS of a batched build (0 otherwise)
+    uint32_t flags;         // RTX_GEOMETRY_*_BIT
 };
 
 // Triangle record in build scratch (48 B): transformed vertices + IDs.

@@ -50,5 +50,7 @@ struct TraceLaunch {
 };
 
 hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);
+// A BLAS with non-opaque geometry was built: later launches honour opacity and any-hit programs.
+void note_nonopaque_geometry();
 
 }  // namespace rtx

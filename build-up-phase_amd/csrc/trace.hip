@@ -16,7 +16,10 @@
 //     that lowest-bit-first visits near children first (Ylitie, Karras & Laine 2017 node groups);
 //   * stack: per-lane entries (slot base, child mask, valid mask, flags) in LDS, spilling to a global
 //     per-thread area past the LDS depth; overflow beyond that raises kErrStackOverflow;
-//   * closest hit: t, u, v are recomputed in FP64 (Moller-Trumbore) for the winning triangle.
+//   * closest hit: t, u, v are recomputed in FP64 (Moller-Trumbore) for the winning triangle;
+//   * the FLAGS variant adds the traceRayEXT ray flags, triangle opacity (geometry / instance / ray) and the
+//     any-hit programs of non-opaque candidates; the host uses it only where they can change a result.
+#include <atomic>
 #include <cmath>
 
 #include "rtx_internal.h"
@@ -39,6 +42,12 @@ constexpr uint32_t kRayCullBack = RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES;
 constexpr uint32_t kRayCullFront = RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES;
 constexpr uint32_t kRayCullOpaque = RTX_RAY_FLAG_CULL_OPAQUE;
 constexpr uint32_t kRaySkipTriangles = RTX_RAY_FLAG_SKIP_TRIANGLES;
+constexpr uint32_t kRayOpaque = RTX_RAY_FLAG_OPAQUE;
+constexpr uint32_t kRayNoOpaque = RTX_RAY_FLAG_NO_OPAQUE;
+constexpr uint32_t kRayCullNoOpaque = RTX_RAY_FLAG_CULL_NO_OPAQUE;
+// FLAGS variant: the current instance register holds the instance id in bits 0-29 and the instance's
+// FORCE_OPAQUE (bit 30) / FORCE_NO_OPAQUE (bit 31) flags.
+constexpr uint32_t kInstIdMask = 0x3FFFFFFFu;
 
 
 __device__ __forceinline__ uint32_t oct_permute(uint32_t m, uint32_t oct) {
@@ -167,7 +176,8 @@ __device__ __forceinline__ void setup_shear(const Space& s, Shear& sh) {
 // Watertight test in the current (rebased object) space.  Returns true and the parametric distance s when
 // the triangle is hit with s in [smin, smax].
 __device__ __forceinline__ bool test_triangle(const uint4 a, const uint4 b, const uint4 c, const Space& s,
-                                              const Shear& sh, float smin, float smax, float& t_out) {
+                                              const Shear& sh, float smin, float smax, float& t_out, float& u_out,
+                                              float& v_out) {
     const float v0x = __uint_as_float(a.x) - s.ox, v0y = __uint_as_float(a.y) - s.oy, v0z = __uint_as_float(a.z) - s.oz;
     const float v1x = __uint_as_float(a.w) - s.ox, v1y = __uint_as_float(b.x) - s.oy, v1z = __uint_as_float(b.y) - s.oz;
     const float v2x = __uint_as_float(b.z) - s.ox, v2y = __uint_as_float(b.w) - s.oy, v2z = __uint_as_float(c.x) - s.oz;
@@ -196,6 +206,9 @@ __device__ __forceinline__ bool test_triangle(const uint4 a, const uint4 b, cons
     const float Ts = det < 0.0f ? -T : T;
     if (Ts < smin * adet || Ts > smax * adet) return false;
     t_out = Ts / adet;
+    const float rdet = 1.0f / det;
+    u_out = V * rdet;
+    v_out = W * rdet;
     return true;
 }
 
@@ -284,10 +297,46 @@ __device__ void world_normal(const double* inv, const TriSlot* tri, float* n) {
     n[2] = (float)(nw[2] * r);
 }
 
+// The any-hit program of a non-opaque candidate (its hit record: instance SBT offset + geometry index x stride +
+// ray SBT offset, as for closest hits) on the candidate's barycentrics (u, v) from the watertight test (FP32):
+// true to accept it.  Without a hit-record table, or with a record index outside it, the candidate is accepted
+// (a hit group without an any-hit shader).
+__device__ __forceinline__ bool any_hit_accepts(const TraceArgs& args, const InstanceInfo* info, uint32_t geom,
+                                                float u, float v) {
+    if (!args.records) return true;
+    const uint32_t rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
+    if (rec >= args.nrec) return true;
+    const uint32_t ah = args.records[rec].any_hit;
+    if ((ah & 255u) != RTX_ANY_HIT_CUTOUT) return true;
+    const float n = (ah >> 8) ? (float)(ah >> 8) : 1.0f;
+    const int cell = (int)floorf(u * n) + (int)floorf(v * n);
+    return (cell & 1) == 0;
+}
+
+// Opacity of a candidate (geometry bit, overridden by the instance's FORCE flags, overridden by the ray's OPAQUE /
+// NO_OPAQUE), the CULL_OPAQUE / CULL_NO_OPAQUE culls, then the any-hit program of a non-opaque candidate.  Used by
+// the FLAGS variant only; the default variant treats every triangle as opaque and runs only where that holds.
+__device__ __forceinline__ bool candidate_accepted(const TraceArgs& args, uint32_t ray_flags, uint32_t tri_bits,
+                                                   uint32_t geom, uint32_t cur_inst, const InstanceInfo* table,
+                                                   float u, float v) {
+    bool opaque = !(tri_bits & kTriNonOpaque);
+    const uint32_t force = cur_inst >> 30;
+    if (force) opaque = (force & 1u) != 0;
+    if (ray_flags & kRayOpaque) opaque = true;
+    else if (ray_flags & kRayNoOpaque) opaque = false;
+    if ((opaque && (ray_flags & kRayCullOpaque)) || (!opaque && (ray_flags & kRayCullNoOpaque))) return false;
+    if (opaque) return true;
+    return any_hit_accepts(args, table + (cur_inst & kInstIdMask), geom, u, v);
+}
+
 template <int MODE, bool STATS, bool FLAGS>
 __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     const uint32_t ray_flags = FLAGS ? args.ray_flags : 0u;  // the default variant compiles the flags out
+    // dynamic LDS: the per-lane stack (lds_depth entries per lane); the FLAGS variant adds the lane's world ray
+    // (6 floats per lane), which it reads back where it needs it (leaving a BLAS, the final FP64 refinement)
+    // instead of keeping it in registers, the default variant keeps it in wo / wd
     extern __shared__ uint2 lds_stack[];
+    float* const lds_ray = (float*)(lds_stack + args.lds_depth * blockDim.x) + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const uint64_t gtid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
@@ -301,6 +350,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     const uint32_t tlas_ninst = th->prim_count;
     const uint32_t D = args.lds_depth;
     const uint32_t GD = args.gdepth;
+    // ray flags under which no triangle can be accepted
+    const bool cull_all = (ray_flags & kRaySkipTriangles) ||
+                          ((ray_flags & kRayCullOpaque) && (ray_flags & (kRayOpaque | kRayCullNoOpaque))) ||
+                          ((ray_flags & kRayCullNoOpaque) && (ray_flags & kRayNoOpaque));
     const uint32_t nA = args.n_ptr ? *args.n_ptr : args.n;
     const uint32_t nrays = MODE == kModeFused ? nA + *args.n2_ptr : nA;
 
@@ -381,6 +434,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         wd[2] = r1.z;
                         tc = r1.w;
                     }
+                    if (FLAGS) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            lds_ray[k * blockDim.x] = wo[k];
+                            lds_ray[(3 + k) * blockDim.x] = wd[k];
+                        }
+                    }
                     s.ox = wo[0];
                     s.oy = wo[1];
                     s.oz = wo[2];
@@ -400,8 +460,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                     steps = 0;
                     ng_base = 0;
                     // the root as a one-child group: child position 0 sits at ordered bit (0 ^ oct)
-                    ng_bits = tlas_empty || (ray_flags & (kRaySkipTriangles | kRayCullOpaque))
-                                  ? 0u : ((1u << s.oct) | (1u << 8));
+                    ng_bits = tlas_empty || cull_all ? 0u : ((1u << s.oct) | (1u << 8));
                     lg_bits = 0;
                     sp = 0;
                     hit_inst = kNoHit;
@@ -432,6 +491,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         sp < D ? lds_stack[sp * blockDim.x + threadIdx.x] : args.gstack[gtid * GD + (sp - D)];
                     if ((e.y & kEntryTlas) && in_blas) {
                         in_blas = false;
+                        if (FLAGS) {
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                wo[k] = lds_ray[k * blockDim.x];
+                                wd[k] = lds_ray[(3 + k) * blockDim.x];
+                            }
+                        }
                         s.ox = wo[0];
                         s.oy = wo[1];
                         s.oz = wo[2];
@@ -484,7 +550,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                     }
                     const uint64_t blas = (uint64_t)q.x | ((uint64_t)q.y << 32);
                     const uint32_t iid = q.z, imask = q.w;
-                    if ((imask & args.cull_mask) != 0u) {
+                    if ((imask & args.cull_mask & 255u) != 0u) {
                         if (STATS) ++st_insts;
                         // push the remaining TLAS work
                         if (lg_bits & 255u) {
@@ -510,6 +576,14 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                             double M[12];
 #pragma unroll
                             for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
+                            if (FLAGS) {  // in the TLAS the current space is world space
+                                wo[0] = s.ox;
+                                wo[1] = s.oy;
+                                wo[2] = s.oz;
+                                wd[0] = s.dx;
+                                wd[1] = s.dy;
+                                wd[2] = s.dz;
+                            }
                             double od[3], dd[3];
 #pragma unroll
                             for (int r = 0; r < 3; ++r) {
@@ -548,7 +622,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                             cqs[1] = bqs.y;
                             cqs[2] = bqs.z;
                             in_blas = true;
-                            cur_inst = iid;
+                            cur_inst = FLAGS ? iid | (((imask >> 10) & 3u) << 30) : iid;  // InstSlot mask | flags << 8
                             cur_nslots = bh->slot_count;
                             ng_base = 0;
                             ng_bits = cur_nslots ? ((1u << s.oct) | (1u << 8)) : 0u;
@@ -571,18 +645,19 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         const uint4* tp = (const uint4*)(cslots + (uint64_t)slot * kSlotBytes);
                         const uint4 a = tp[0], b = tp[1], c = tp[2];
                         if (STATS) ++st_tris;
-                        float th;
+                        float th, bu, bv;
                         const float smax = tc - s.toff;
-                        if (test_triangle(a, b, c, s, sh, smin, smax, th) &&
+                        if (test_triangle(a, b, c, s, sh, smin, smax, th, bu, bv) &&
                             !(ray_flags & (kRayCullBack | kRayCullFront) &&
-                              facing_culled(a, b, c, s, ray_flags, table[cur_inst].flags))) {
+                              facing_culled(a, b, c, s, ray_flags, table[cur_inst & kInstIdMask].flags)) &&
+                            (!FLAGS || candidate_accepted(args, ray_flags, c.w, c.z, cur_inst, table, bu, bv))) {
                             if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
                                 occl = true;
                                 done = true;
                                 break;
                             }
                             tc = s.toff + th;
-                            hit_inst = cur_inst;
+                            hit_inst = cur_inst & kInstIdMask;
                             hit_slot = slot;
                             if (ray_flags & kRayTerminateOnFirstHit) {
                                 done = true;
@@ -658,6 +733,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
                         const TriSlot* tri =
                             (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes + (uint64_t)hit_slot * kSlotBytes);
+                        if (FLAGS) {
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                wo[k] = lds_ray[k * blockDim.x];
+                                wd[k] = lds_ray[(3 + k) * blockDim.x];
+                            }
+                        }
                         refine_hit(M, wo, wd, tri, t, u, v);
                         prim = tri->prim;
                         geom = tri->geom;
@@ -762,7 +844,9 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
 
 template <int MODE, bool STATS, bool FLAGS>
 hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
-    const size_t lds = (size_t)L.lds_depth * L.block_size * sizeof(uint2);
+    // the stack, and the lanes' world rays for the FLAGS variant
+    const size_t lds =
+        (size_t)L.lds_depth * L.block_size * sizeof(uint2) + (FLAGS ? (size_t)L.block_size * 6 * sizeof(float) : 0);
     if (L.w4)
         trace_kernel_w4<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
     else
@@ -770,17 +854,28 @@ hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t 
     return hipGetLastError();
 }
 
-// ray flags that change traversal (OPAQUE / NO_OPAQUE / CULL_NO_OPAQUE do not: every triangle is opaque)
-constexpr uint32_t kActiveRayFlags =
-    kRayTerminateOnFirstHit | kRaySkipClosestHit | kRayCullBack | kRayCullFront | kRayCullOpaque | kRaySkipTriangles;
+// ray flags that need the FLAGS variant
+constexpr uint32_t kActiveRayFlags = kRayTerminateOnFirstHit | kRaySkipClosestHit | kRayCullBack | kRayCullFront |
+                                     kRayCullOpaque | kRayCullNoOpaque | kRayNoOpaque | kRaySkipTriangles;
+
+// Non-opaque geometry has been built in this process (set by the BLAS builds, never cleared): from then on rays
+// without the OPAQUE flag take the FLAGS variant, which resolves opacity and runs any-hit programs.
+std::atomic<bool> g_nonopaque_geometry{false};
+
+bool needs_flags_variant(uint32_t ray_flags) {
+    return (ray_flags & kActiveRayFlags) ||
+           (!(ray_flags & kRayOpaque) && g_nonopaque_geometry.load(std::memory_order_relaxed));
+}
 
 template <int MODE, bool STATS>
 hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
-    if (a.ray_flags & kActiveRayFlags) return launch_variant<MODE, STATS, true>(a, L, s);
+    if (needs_flags_variant(a.ray_flags)) return launch_variant<MODE, STATS, true>(a, L, s);
     return launch_variant<MODE, STATS, false>(a, L, s);
 }
 
 }  // namespace
+
+void note_nonopaque_geometry() { g_nonopaque_geometry.store(true, std::memory_order_relaxed); }
 
 hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStream_t s) {
     TraceArgs a{};
@@ -823,8 +918,8 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     switch (r.mode) {
         case 0: return r.stats ? launch<kModeClosest, true>(a, L, s) : launch<kModeClosest, false>(a, L, s);
         case 1: return r.stats ? launch<kModeAny, true>(a, L, s) : launch<kModeAny, false>(a, L, s);
-        case 3: return launch_variant<kModePath, false, false>(a, L, s);
-        case 4: return launch_variant<kModeFused, false, false>(a, L, s);
+        case 3: return launch<kModePath, false>(a, L, s);
+        case 4: return launch<kModeFused, false>(a, L, s);
         default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
     }
 }

@@ -37,7 +37,7 @@ def RTX_BUILD_TREELET_ROUNDS(r: int) -> int:
 RTX_INDEX_TYPE_UINT32 = 1
 RTX_QUERY_CLOSEST_HIT = 0
 RTX_QUERY_ANY_HIT = 1
-RTX_RAY_FLAG_OPAQUE = 0x1
+RTX_RAY_FLAG_OPAQUE, RTX_RAY_FLAG_NO_OPAQUE = 0x1, 0x2
 MISS = 0xFFFFFFFF
 
 ERRORS = {-1: "OUT_OF_HOST_MEMORY", -2: "OUT_OF_DEVICE_MEMORY", -3: "INITIALIZATION_FAILED", -4: "DEVICE_LOST",
@@ -111,6 +111,9 @@ RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT, RTX_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER = 0x4,
 RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES, RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES = 0x10, 0x20
 RTX_RAY_FLAG_CULL_OPAQUE, RTX_RAY_FLAG_CULL_NO_OPAQUE, RTX_RAY_FLAG_SKIP_TRIANGLES = 0x40, 0x80, 0x100
 RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT, RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT = 0x1, 0x2
+RTX_INSTANCE_FORCE_OPAQUE_BIT, RTX_INSTANCE_FORCE_NO_OPAQUE_BIT = 0x4, 0x8
+RTX_PROGRAM_RECORD_COLOR, RTX_PROGRAM_BARYCENTRIC, RTX_PROGRAM_MATCH_BARYCENTRIC, RTX_PROGRAM_LAMBERTIAN = 0, 1, 2, 3
+RTX_ANY_HIT_ACCEPT, RTX_ANY_HIT_CUTOUT = 0, 1
 RTX_FILM_SRGB_BIT, RTX_FILM_SWAP_RB_BIT, RTX_FILM_OPAQUE_BIT = 0x1, 0x2, 0x4
 RTX_FILM_REFERENCE_BYTES = RTX_FILM_SWAP_RB_BIT
 RTX_COPY_MODE_CLONE, RTX_COPY_MODE_COMPACT = 0, 1
@@ -300,7 +303,7 @@ class Context:
             arr[i].primitive_offset = 0
             arr[i].first_vertex = 0
             arr[i].transform_offset = 0
-            arr[i].flags = RTX_GEOMETRY_OPAQUE_BIT
+            arr[i].flags = RTX_GEOMETRY_OPAQUE_BIT if g.get("opaque", True) else 0
         return arr
 
     def blas_build_sizes(self, geoms, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
@@ -693,7 +696,8 @@ class Context:
                 d = {"vertices": _dev_tensor(g.vertices.astype(np.float32), self.device),
                      "indices": _dev_tensor(g.indices.astype(np.uint32).view(np.int32), self.device),
                      "transform": None if g.transform is None else
-                     _dev_tensor(np.asarray(g.transform, np.float32).reshape(3, 4), self.device)}
+                     _dev_tensor(np.asarray(g.transform, np.float32).reshape(3, 4), self.device),
+                     "opaque": g.opaque}
                 dg.append(d)
             lists.append(dg)
         if batched:

@@ -30,6 +30,7 @@ class Geometry:
     vertices: np.ndarray            # float32 [V, 3]
     indices: np.ndarray             # uint32 [T, 3]
     transform: np.ndarray | None = None  # float32 [3, 4] row-major, applied at build time
+    opaque: bool = True             # RTX_GEOMETRY_OPAQUE_BIT; False runs the hit record's any-hit program
 
 
 @dataclass
@@ -57,18 +58,23 @@ class Scene:
         return int(sum(sum(g.indices.shape[0] for g in geoms) for geoms in self.blas))
 
 
-# rtx_hit_record (48 B): program, color[3], emission[3], match_primitive/instance/custom_index/geometry, reserved
+# rtx_hit_record (48 B): program, color[3], emission[3], match_primitive/instance/custom_index/geometry, any_hit
 HIT_RECORD_DTYPE = np.dtype([("program", np.uint32), ("color", np.float32, (3,)), ("emission", np.float32, (3,)),
-                             ("match", np.int32, (4,)), ("reserved", np.uint32)])
+                             ("match", np.int32, (4,)), ("any_hit", np.uint32)])
 assert HIT_RECORD_DTYPE.itemsize == 48
 
 PROGRAM_RECORD_COLOR, PROGRAM_BARYCENTRIC, PROGRAM_MATCH_BARYCENTRIC, PROGRAM_LAMBERTIAN = 0, 1, 2, 3
 INSTANCE_TRIANGLE_FACING_CULL_DISABLE = 0x1
+ANY_HIT_ACCEPT = 0      # rtx_hit_record.any_hit programs (RTX_ANY_HIT_*); parameter in bits 8-31
+ANY_HIT_CUTOUT = 1
 
 
-def hit_records(programs, colors, emissions=None, match=None) -> np.ndarray:
+def hit_records(programs, colors, emissions=None, match=None, any_hit=None) -> np.ndarray:
+    """any_hit: RTX_ANY_HIT_PROGRAM words (program | parameter << 8) per record, default accept."""
     n = len(programs)
     r = np.zeros(n, HIT_RECORD_DTYPE)
+    if any_hit is not None:
+        r["any_hit"] = np.asarray(any_hit, np.uint32).reshape(n)
     r["program"] = programs
     r["color"] = np.asarray(colors, np.float32).reshape(n, 3)
     if emissions is not None:

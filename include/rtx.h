@@ -65,7 +65,8 @@ void* rtx_get_stream(rtx_context ctx);
 /* ---- acceleration-structure inputs ------------------------------------------------------------ */
 
 /* VkGeometryFlagBitsKHR / VkBuildAccelerationStructureFlagBitsKHR subset */
-#define RTX_GEOMETRY_OPAQUE_BIT 0x1u                       /* main.cpp:741 */
+#define RTX_GEOMETRY_OPAQUE_BIT 0x1u                       /* main.cpp:741; without it the geometry's candidate
+                                                              hits run the record's any-hit program */
 #define RTX_BUILD_ALLOW_UPDATE_BIT 0x1u                    /* BLAS keeps its hierarchy for rtx_cmd_update_blas */
 #define RTX_BUILD_ALLOW_COMPACTION_BIT 0x2u                /* accepted; every structure can be compacted */
 #define RTX_BUILD_PREFER_FAST_TRACE_BIT 0x4u               /* main.cpp:751, 887 */
@@ -112,7 +113,8 @@ typedef struct rtx_instance {
 
 #define RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT 0x1u /* VK_GEOMETRY_INSTANCE_..., main.cpp:852 */
 #define RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT 0x2u
-#define RTX_INSTANCE_FORCE_OPAQUE_BIT 0x4u
+#define RTX_INSTANCE_FORCE_OPAQUE_BIT 0x4u                 /* the instance's triangles count as opaque */
+#define RTX_INSTANCE_FORCE_NO_OPAQUE_BIT 0x8u              /* ... as non-opaque */
 
 /* VkAccelerationStructureBuildSizesInfoKHR */
 typedef struct rtx_as_build_sizes {
@@ -225,6 +227,14 @@ void rtx_obj_free(rtx_obj_mesh* mesh);
                                             all equal the record's match_* fields, else the record color */
 #define RTX_PROGRAM_LAMBERTIAN 3u        /* path tracer: albedo = color, emission = emission */
 
+/* Compiled-in any-hit programs (rtx_hit_record.any_hit: program in bits 0-7, parameter in bits 8-31), run on the
+ * non-opaque candidate hits of the record's geometry.  (u, v) are the candidate's hitAttributeEXT barycentrics,
+ * computed in FP64 exactly as reported for a closest hit. */
+#define RTX_ANY_HIT_ACCEPT 0u  /* accept every candidate (a hit group without an any-hit shader) */
+#define RTX_ANY_HIT_CUTOUT 1u  /* ignoreIntersectionEXT() where floor(n u) + floor(n v) is odd, n = parameter
+                                  (0 counts as 1): a checkerboard of holes over the triangle */
+#define RTX_ANY_HIT_PROGRAM(p, param) ((uint32_t)(p) | ((uint32_t)(param) << 8))
+
 /* One hit-group record (main.cpp:1307-1317: handle + float color[3]); 48 B. */
 typedef struct rtx_hit_record {
     uint32_t program;
@@ -234,7 +244,7 @@ typedef struct rtx_hit_record {
     int32_t match_instance;
     int32_t match_custom_index;
     int32_t match_geometry;
-    uint32_t reserved;
+    uint32_t any_hit; /* RTX_ANY_HIT_PROGRAM(program, parameter) */
 } rtx_hit_record;
 
 /* Hit-group table + miss program.  Record index for a hit (Vulkan spec 40.3.1, main.cpp:1252-1263):
@@ -257,8 +267,10 @@ typedef struct rtx_camera {
 
 /* traceRayEXT arguments used by the raygen program (main.cpp:1047-1052). */
 /* traceRayEXT ray flags (gl_RayFlags*EXT values; the reference passes gl_RayFlagsOpaqueEXT, main.cpp:1048).
- * Every triangle of this library is opaque (there are no any-hit programs), so OPAQUE / NO_OPAQUE change
- * nothing, CULL_OPAQUE culls every triangle and CULL_NO_OPAQUE none.  Facing: a triangle is front-facing when
+ * Opacity of a candidate hit (Vulkan order): the geometry's RTX_GEOMETRY_OPAQUE_BIT, overridden by the instance's
+ * FORCE_OPAQUE / FORCE_NO_OPAQUE, overridden by the ray's OPAQUE / NO_OPAQUE.  CULL_OPAQUE / CULL_NO_OPAQUE drop
+ * candidates of that opacity.  A non-opaque candidate runs the any-hit program of its hit record (when the call
+ * has a hit-record table), which accepts it or ignores it.  Facing: a triangle is front-facing when
  * v0, v1, v2 appear clockwise seen from the ray origin (dot(d, (v1 - v0) x (v2 - v0)) > 0 in object space);
  * RTX_INSTANCE_TRIANGLE_FLIP_FACING_BIT makes counter-clockwise front-facing and
  * RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT exempts the instance from the CULL_*_FACING flags.
@@ -335,7 +347,8 @@ int rtx_get_trace_stats(rtx_context ctx, rtx_trace_stats* out_stats);
 typedef struct rtx_trace_config {
     uint32_t block_size;      /* threads per block: 64, 128, 192 or 256 */
     uint32_t blocks_per_cu;   /* persistent blocks per compute unit (1..32) */
-    uint32_t lds_stack_depth; /* stack entries held in LDS per lane before spilling to global memory */
+    uint32_t lds_stack_depth; /* stack entries held in LDS per lane before spilling to global memory; with the
+                                 lane's world ray (24 B) at most 64 KiB of LDS per block: (8 depth + 24) block */
     uint32_t ray_chunk;       /* ray ids a wave takes from the global queue per fetch: a multiple of 64 up to
                                  4096, or RTX_TRACE_RAY_CHUNK_AUTO (64, or 256 when every wave gets >= 8 chunks) */
     uint32_t refill_lanes;    /* a wave fetches new rays once fewer than this many of its 64 lanes are busy */

@@ -32,15 +32,15 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         P = ctypes.c_void_p
         L.oracle_blas_create.restype = ctypes.c_int
-        L.oracle_blas_create.argtypes = [ctypes.c_int, P, P, P, P]
+        L.oracle_blas_create.argtypes = [ctypes.c_int, P, P, P, P, P]
         L.oracle_tlas_create.restype = ctypes.c_int
         L.oracle_tlas_create.argtypes = [ctypes.c_int, P]
         L.oracle_reset.restype = None
         L.oracle_blas_delta.restype = ctypes.c_double
         L.oracle_blas_delta.argtypes = [ctypes.c_int]
         L.oracle_trace.restype = None
-        L.oracle_trace.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P, P,
-                                   P, ctypes.c_int]
+        L.oracle_trace.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P,
+                                   ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P, P, P, ctypes.c_int]
         L.oracle_trace_brute.restype = None
         L.oracle_trace_brute.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_uint32, P, P, P]
         L.oracle_aspect.restype = None
@@ -82,10 +82,12 @@ def _ptr(a: np.ndarray) -> int:
 
 
 def _geom_arrays(geoms):
-    """geoms: list of (vertices float32 [V,3], indices uint32 [T,3], transform float32 [3,4] or None)."""
+    """geoms: list of (vertices float32 [V,3], indices uint32 [T,3], transform float32 [3,4] or None[, opaque])."""
     keep = []
-    vp, ip, xp, pc = [], [], [], []
-    for v, i, x in geoms:
+    vp, ip, xp, pc, fl = [], [], [], [], []
+    for g in geoms:
+        v, i, x = g[:3]
+        fl.append(1 if (len(g) < 4 or g[3]) else 0)
         v = np.ascontiguousarray(v, dtype=np.float32)
         i = np.ascontiguousarray(i, dtype=np.uint32)
         keep += [v, i]
@@ -99,7 +101,7 @@ def _geom_arrays(geoms):
             keep.append(x)
             xp.append(_ptr(x))
     arrs = (np.array(vp, dtype=np.uint64), np.array(ip, dtype=np.uint64), np.array(pc, dtype=np.uint32),
-            np.array(xp, dtype=np.uint64))
+            np.array(xp, dtype=np.uint64), np.array(fl, dtype=np.uint32))
     return keep, arrs
 
 
@@ -108,8 +110,8 @@ def reset() -> None:
 
 
 def blas_create(geoms) -> int:
-    keep, (vp, ip, pc, xp) = _geom_arrays(geoms)
-    return lib().oracle_blas_create(len(geoms), _ptr(vp), _ptr(ip), _ptr(pc), _ptr(xp))
+    keep, (vp, ip, pc, xp, fl) = _geom_arrays(geoms)
+    return lib().oracle_blas_create(len(geoms), _ptr(vp), _ptr(ip), _ptr(pc), _ptr(xp), _ptr(fl))
 
 
 def blas_delta(h: int) -> float:
@@ -139,15 +141,20 @@ def tlas_create(records: np.ndarray) -> int:
 
 
 def trace(tlas: int, rays: np.ndarray, any_hit: bool = False, cull_mask: int = 0xFF, threads: int = 0,
-          ray_flags: int = 0):
+          ray_flags: int = 0, records=None, sbt_offset: int = 0, sbt_stride: int = 1):
     """rays: float32 [N,8].  Returns (t_uv float64 [N,3], ids uint32 [N,4] (prim, geom, inst, custom),
-    status int32 [N]).  ray_flags: gl_RayFlags*EXT (facing culls, CULL_OPAQUE, SKIP_TRIANGLES)."""
+    status int32 [N]).  ray_flags: gl_RayFlags*EXT (facing and opacity culls, OPAQUE / NO_OPAQUE, SKIP_TRIANGLES).
+    records: 48-B hit records whose any-hit programs run on non-opaque candidates (None: none run)."""
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
     n = rays.shape[0]
     t_uv = np.zeros((n, 3), np.float64)
     ids = np.zeros((n, 4), np.uint32)
     st = np.zeros(n, np.int32)
-    lib().oracle_trace(tlas, _ptr(rays), n, int(any_hit), cull_mask, ray_flags, _ptr(t_uv), _ptr(ids), _ptr(st),
+    rec = None if records is None else np.ascontiguousarray(records)
+    if rec is not None:
+        assert rec.dtype.itemsize == 48
+    lib().oracle_trace(tlas, _ptr(rays), n, int(any_hit), cull_mask, ray_flags, None if rec is None else _ptr(rec),
+                       0 if rec is None else len(rec), sbt_offset, sbt_stride, _ptr(t_uv), _ptr(ids), _ptr(st),
                        threads)
     return t_uv, ids, st
 
@@ -190,16 +197,17 @@ def shade(tlas: int, t_uv, ids, records: np.ndarray, sbt_offset: int, sbt_stride
 
 
 def trace_rays(tlas: int, records, camera, w, h, miss_color=(0.0, 0.0, 0.2), tmin=0.0, tmax=100.0, cull_mask=0xFF,
-               sbt_offset=0, sbt_stride=1, threads=0):
-    """Oracle of vkCmdTraceRaysKHR with the reference raygen/miss/chit: returns (film [H,W,4], status [H,W])."""
+               sbt_offset=0, sbt_stride=1, threads=0, ray_flags=1):
+    """Oracle of vkCmdTraceRaysKHR with the reference raygen/miss/chit (ray flags default gl_RayFlagsOpaqueEXT,
+    main.cpp:1049): returns (film [H,W,4], status [H,W])."""
     rays = raygen(camera, w, h, tmin, tmax)
-    t_uv, ids, st = trace(tlas, rays, False, cull_mask, threads)
+    t_uv, ids, st = trace(tlas, rays, False, cull_mask, threads, ray_flags, records, sbt_offset, sbt_stride)
     film = shade(tlas, t_uv, ids, records, sbt_offset, sbt_stride, miss_color)
     return film.reshape(h, w, 4), st.reshape(h, w)
 
 
 def centroids(geoms) -> np.ndarray:
-    keep, (vp, ip, pc, xp) = _geom_arrays(geoms)
+    keep, (vp, ip, pc, xp, _) = _geom_arrays(geoms)
     n = int(pc.sum())
     cen = np.zeros((n, 3), np.float32)
     lib().oracle_centroids(len(geoms), _ptr(vp), _ptr(ip), _ptr(pc), _ptr(xp), _ptr(cen))

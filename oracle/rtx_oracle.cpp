@@ -72,6 +72,7 @@ inline void xform_f32(const float* m, const float* v, float* o) {
 struct Tri {
     float v[9];
     uint32_t prim, geom;
+    bool opaque;  // its geometry has VK_GEOMETRY_OPAQUE_BIT
 };
 
 struct BNode {
@@ -286,8 +287,48 @@ int facing_cull(const RayFrame& f, const Tri& tr, uint32_t ray_flags, uint32_t i
     return (front ? (ray_flags & kCullFront) : (ray_flags & kCullBack)) ? 1 : 0;
 }
 
+// Hit-record table for any-hit programs: the any_hit word of each 48-B record (rtx_hit_record layout, offset 44)
+// and the ray's SBT offset / stride; record of a candidate = instance SBT offset + geometry x stride + offset.
+struct AnyHitTable {
+    const uint8_t* records = nullptr;
+    uint32_t nrec = 0, sbt_offset = 0, sbt_stride = 1;
+    uint32_t word(uint32_t inst_sbt, uint32_t geom) const {
+        const uint32_t r = inst_sbt + geom * sbt_stride + sbt_offset;
+        if (!records || r >= nrec) return 0;
+        uint32_t w;
+        std::memcpy(&w, records + 48 * (size_t)r + 44, 4);
+        return w;
+    }
+};
+
+// Candidate filter of traceRayEXT for non-opaque geometry.  Opacity: the geometry's OPAQUE bit, overridden by
+// the instance's FORCE_OPAQUE (0x4) / FORCE_NO_OPAQUE (0x8), overridden by the ray's OPAQUE (0x1) / NO_OPAQUE (0x2);
+// CULL_OPAQUE (0x40) / CULL_NO_OPAQUE (0x80) drop candidates of that opacity; a non-opaque candidate runs its
+// record's any-hit program.  Program 1 (cutout, parameter n = word >> 8, 0 -> 1) ignores the candidate where
+// floor(n u) + floor(n v) is odd.  Returns 0 drop, 1 keep, 2 keep as an epsilon-tie (n u or n v within 2^-16 of
+// an integer: the GPU decides on FP32 barycentrics).
+int any_hit_filter(const Tri& tr, uint32_t inst_flags, uint32_t inst_sbt, uint32_t ray_flags, double u, double v,
+                   const AnyHitTable* ah) {
+    bool opaque = tr.opaque;
+    if (inst_flags & 0x4u) opaque = true;
+    else if (inst_flags & 0x8u) opaque = false;
+    if (ray_flags & 0x1u) opaque = true;
+    else if (ray_flags & 0x2u) opaque = false;
+    if ((opaque && (ray_flags & 0x40u)) || (!opaque && (ray_flags & 0x80u))) return 0;
+    if (opaque || !ah) return 1;
+    const uint32_t w = ah->word(inst_sbt, tr.geom);
+    if ((w & 255u) != 1u) return 1;
+    const double n = (w >> 8) ? (double)(w >> 8) : 1.0;
+    const double fu = u * n, fv = v * n;
+    const double band = 0x1p-16;
+    const bool tie = std::fabs(fu - std::nearbyint(fu)) < band || std::fabs(fv - std::nearbyint(fv)) < band;
+    const bool keep = ((int64_t)std::floor(fu) + (int64_t)std::floor(fv)) % 2 == 0;
+    return tie ? 2 : (keep ? 1 : 0);
+}
+
 void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint32_t inst_id, Collector& col,
-                uint32_t ray_flags = 0, uint32_t inst_flags = 0) {
+                uint32_t ray_flags = 0, uint32_t inst_flags = 0, uint32_t inst_sbt = 0,
+                const AnyHitTable* ah = nullptr) {
     if (b.nodes.empty()) return;
     // boxes inflated by 4*delta so that every near-edge candidate is visited
     double infl = 4.0 * b.delta;
@@ -329,6 +370,11 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
             const int cull = facing_cull(f, tr, ray_flags, inst_flags);
             if (cull == 1) continue;
             if (cull == 2) c.near = true;
+            const int keep = any_hit_filter(tr, inst_flags, inst_sbt, ray_flags, c.u, c.v, ah);
+            if (keep == 0) continue;
+            if (keep == 2) {  // may be ignored on the GPU: a tie candidate that cannot win robustly
+                c.near = true;
+            }
             // candidates relevant to [tmin, tmax] (with uncertainty)
             if (c.t + c.tunc < tmin || c.t - c.tunc > tmax) continue;
             if (!c.inside && !c.near) continue;
@@ -353,13 +399,14 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
     }
 }
 
-// ray_flags: gl_RayFlags*EXT; every triangle is opaque, so CULL_OPAQUE (0x40) and SKIP_TRIANGLES (0x100) leave
-// nothing to hit.
-void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, HitOut& out, uint32_t ray_flags = 0) {
+// ray_flags: gl_RayFlags*EXT (facing and opacity culls, SKIP_TRIANGLES); ah: any-hit programs (may be NULL).
+void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, HitOut& out, uint32_t ray_flags = 0,
+               const AnyHitTable* ah = nullptr) {
     d3 o{ray[0], ray[1], ray[2]}, d{ray[4], ray[5], ray[6]};
     double tmin = ray[3], tmax = ray[7];
     Collector col;
-    const bool skip_all = (ray_flags & (0x40u | 0x100u)) != 0;
+    const bool skip_all = (ray_flags & 0x100u) || ((ray_flags & 0x40u) && (ray_flags & (0x1u | 0x80u))) ||
+                          ((ray_flags & 0x80u) && (ray_flags & 0x2u));
     for (size_t i = 0; i < tl.inst.size() && !skip_all; ++i) {
         const Inst& in = tl.inst[i];
         if (!in.valid || ((in.mask & cull) == 0)) continue;
@@ -371,7 +418,7 @@ void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, Hit
               M[8] * d.x + M[9] * d.y + M[10] * d.z};
         if (dd.x == 0 && dd.y == 0 && dd.z == 0) continue;
         RayFrame f = make_frame(oo, dd);
-        trace_blas(b, f, tmin, tmax, (uint32_t)i, col, ray_flags, in.flags);
+        trace_blas(b, f, tmin, tmax, (uint32_t)i, col, ray_flags, in.flags, in.sbt_offset, ah);
     }
     out.n_cand = (uint32_t)col.cands.size();
     out.custom = 0;
@@ -470,7 +517,7 @@ extern "C" {
 // Create a BLAS from ngeom triangle geometries.  vtx[g]: FP32 xyz vertices (tightly packed), idx[g]: uint32
 // triangles (3 per primitive), xform[g]: row-major 3x4 or NULL.
 int oracle_blas_create(int ngeom, const float* const* vtx, const uint32_t* const* idx, const uint32_t* prim_count,
-                       const float* const* xform) {
+                       const float* const* xform, const uint32_t* geom_flags) {
     Blas* b = new Blas();
     for (int g = 0; g < ngeom; ++g) {
         for (uint32_t p = 0; p < prim_count[g]; ++p) {
@@ -484,6 +531,7 @@ int oracle_blas_create(int ngeom, const float* const* vtx, const uint32_t* const
             }
             t.prim = p;
             t.geom = (uint32_t)g;
+            t.opaque = !geom_flags || (geom_flags[g] & 0x1u);
             b->tris.push_back(t);
         }
     }
@@ -561,8 +609,15 @@ double oracle_blas_delta(int blas) { return g_blas[blas]->delta; }
 //   t_uv[3n] doubles, ids[4n] = prim, geom, inst (0xFFFFFFFF miss), custom index, status[n] (0 miss, 1 hit,
 //   2 epsilon-tie).  threads <= 0 uses hardware concurrency.
 void oracle_trace(int tlas, const float* rays, int64_t n, int any_hit, uint32_t cull_mask, uint32_t ray_flags,
-                  double* t_uv, uint32_t* ids, int32_t* status, int threads) {
+                  const void* records, uint32_t nrec, uint32_t sbt_offset, uint32_t sbt_stride, double* t_uv,
+                  uint32_t* ids, int32_t* status, int threads) {
     const Tlas& tl = *g_tlas[tlas];
+    AnyHitTable ahs;
+    ahs.records = (const uint8_t*)records;
+    ahs.nrec = nrec;
+    ahs.sbt_offset = sbt_offset;
+    ahs.sbt_stride = sbt_stride;
+    const AnyHitTable* ah = records ? &ahs : nullptr;
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<int64_t> next{0};
     auto work = [&]() {
@@ -572,7 +627,7 @@ void oracle_trace(int tlas, const float* rays, int64_t n, int any_hit, uint32_t 
             int64_t e = std::min<int64_t>(n, s + 256);
             for (int64_t i = s; i < e; ++i) {
                 HitOut h;
-                trace_one(tl, rays + 8 * i, cull_mask, any_hit, h, ray_flags);
+                trace_one(tl, rays + 8 * i, cull_mask, any_hit, h, ray_flags, ah);
                 if (t_uv) {
                     t_uv[3 * i] = h.t;
                     t_uv[3 * i + 1] = h.u;
@@ -1078,7 +1133,8 @@ struct PHit {
 };
 
 // Closest opaque hit of the world ray (o, d) in [tmin, tmax] (FP64, no tie bookkeeping needed here).
-PHit pt_closest(const Tlas& tl, const double* o, const double* d, double tmin, double tmax) {
+PHit pt_closest(const Tlas& tl, const double* o, const double* d, double tmin, double tmax,
+                const AnyHitTable* ah) {
     Collector col;
     for (size_t i = 0; i < tl.inst.size(); ++i) {
         const Inst& in = tl.inst[i];
@@ -1090,7 +1146,7 @@ PHit pt_closest(const Tlas& tl, const double* o, const double* d, double tmin, d
               M[8] * d[0] + M[9] * d[1] + M[10] * d[2]};
         if (dd.x == 0 && dd.y == 0 && dd.z == 0) continue;
         RayFrame f = make_frame(oo, dd);
-        trace_blas(*g_blas[in.blas], f, tmin, tmax, (uint32_t)i, col);
+        trace_blas(*g_blas[in.blas], f, tmin, tmax, (uint32_t)i, col, 0, in.flags, in.sbt_offset, ah);
     }
     PHit h;
     if (!col.have) return h;
@@ -1146,6 +1202,9 @@ void oracle_render(int tlas, const void* records, uint32_t nrec, const float* ca
     }
     const double total = acc;
     const uint8_t* rec = (const uint8_t*)records;
+    AnyHitTable aht;  // any-hit programs of non-opaque geometry (SBT offset 0, stride 1)
+    aht.records = rec;
+    aht.nrec = nrec;
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     std::atomic<int64_t> next{0};
     const int64_t npix = (int64_t)W * H;
@@ -1163,7 +1222,7 @@ void oracle_render(int tlas, const void* records, uint32_t nrec, const float* ca
                 double d[3] = {(sx * 2 - 1) * ax, -((sy * 2 - 1) * ay), -1.0};
                 double thr[3] = {1, 1, 1}, rad[3] = {0, 0, 0};
                 for (uint32_t depth = 0; depth < max_depth; ++depth) {
-                    PHit h = pt_closest(tl, o, d, 0.0, INFINITY);
+                    PHit h = pt_closest(tl, o, d, 0.0, INFINITY, &aht);
                     if (!h.hit) break;
                     uint32_t ri = tl.inst[h.inst].sbt_offset + h.geom;
                     if (ri >= nrec) break;
@@ -1213,7 +1272,7 @@ void oracle_render(int tlas, const void* records, uint32_t nrec, const float* ca
                             double cx = (nf[0] * wi[0] + nf[1] * wi[1] + nf[2] * wi[2]) * id;
                             double cl = -(nl[0] * wi[0] + nl[1] * wi[1] + nl[2] * wi[2]) * id / nll;
                             if (cx > 0 && cl > 0) {
-                                PHit sh = pt_closest(tl, org, wi, 0.0, 1.0 - 1e-4);
+                                PHit sh = pt_closest(tl, org, wi, 0.0, 1.0 - 1e-4, &aht);
                                 if (!sh.hit) {
                                     double g = cx * cl / d2 * total / 3.14159265358979323846;
                                     for (int c = 0; c < 3; ++c) rad[c] += thr[c] * colf[c] * L[9 + c] * g;

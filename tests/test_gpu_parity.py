@@ -445,6 +445,78 @@ def test_ray_flags_match_oracle(ctx):
     assert np.all(g[0][hit] >= tuv[hit, 0] * (1 - 1e-5))
 
 
+def _cutout_mixed_scene(pyrtx):
+    """small_mixed_scene with non-opaque icospheres, instance flags cycling none / FORCE_OPAQUE / FORCE_NO_OPAQUE
+    and one hit record per instance: record colours, cutout any-hit programs of 2, 3 or 5 cells per edge."""
+    sc = small_mixed_scene(seed=17, instances=24, freq=12, cells=16)
+    sc.blas[0][0].opaque = False
+    n = len(sc.blas_of_instance)
+    sc.flags = np.array([(0, pyrtx.RTX_INSTANCE_FORCE_OPAQUE_BIT, pyrtx.RTX_INSTANCE_FORCE_NO_OPAQUE_BIT)[i % 3]
+                         for i in range(n)], np.uint32)
+    sc.sbt_offset = np.arange(n, dtype=np.uint32)
+    rng = np.random.default_rng(3)
+    cells = [(2, 3, 5)[i % 3] for i in range(n)]
+    sc.records = S.hit_records([pyrtx.RTX_PROGRAM_RECORD_COLOR] * n, rng.uniform(0.1, 0.9, (n, 3)),
+                               any_hit=[S.ANY_HIT_CUTOUT | (c << 8) for c in cells])
+    return sc
+
+
+def test_opacity_flags_match_oracle(ctx):
+    """Geometry opacity, instance FORCE flags and the ray's OPAQUE / NO_OPAQUE / CULL_*OPAQUE on ray buffers (no
+    hit-record table, so no any-hit program runs)."""
+    import pyrtx
+    _say("opacity flags")
+    sc = _cutout_mixed_scene(pyrtx)
+    blases, tlas = ctx.upload_scene(sc)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=8)])
+    dr = torch.from_numpy(rays).cuda()
+    F = pyrtx
+    for rf in (0, F.RTX_RAY_FLAG_OPAQUE, F.RTX_RAY_FLAG_NO_OPAQUE, F.RTX_RAY_FLAG_CULL_OPAQUE,
+               F.RTX_RAY_FLAG_CULL_NO_OPAQUE, F.RTX_RAY_FLAG_NO_OPAQUE | F.RTX_RAY_FLAG_CULL_OPAQUE,
+               F.RTX_RAY_FLAG_OPAQUE | F.RTX_RAY_FLAG_CULL_NO_OPAQUE):
+        g = pyrtx.hits_to_numpy(ctx.trace_ray_buffer(tlas, dr, ray_flags=rf))
+        tuv, ids, st = O.trace(t, rays, ray_flags=rf)
+        rep = compare_closest(g, tuv, ids, st)
+        _say(f"  flags {rf:#x}: {rep}")
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
+        occ = ctx.trace_ray_buffer(tlas, dr, pyrtx.RTX_QUERY_ANY_HIT, ray_flags=rf).cpu().numpy()
+        _, _, sta = O.trace(t, rays, any_hit=True, ray_flags=rf)
+        assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
+        if rf in (F.RTX_RAY_FLAG_CULL_OPAQUE, F.RTX_RAY_FLAG_CULL_NO_OPAQUE):
+            assert rep["robust_hits"] > 100  # both opacities are present
+    for rf in (F.RTX_RAY_FLAG_CULL_OPAQUE | F.RTX_RAY_FLAG_CULL_NO_OPAQUE,
+               F.RTX_RAY_FLAG_OPAQUE | F.RTX_RAY_FLAG_CULL_OPAQUE):
+        g = pyrtx.hits_to_numpy(ctx.trace_ray_buffer(tlas, dr, ray_flags=rf))
+        assert np.all(g[5] == 0xFFFFFFFF)
+
+
+def test_any_hit_programs_match_oracle(ctx):
+    """Cutout any-hit programs on the film (rtx_cmd_trace_rays with the hit-record table): colours bit-exact
+    against the oracle off epsilon-ties (edges, cell boundaries within 2^-16 of the GPU's FP32 barycentrics)."""
+    import pyrtx
+    _say("any-hit programs")
+    sc = _cutout_mixed_scene(pyrtx)
+    blases, tlas = ctx.upload_scene(sc)
+    t = oracle_scene(sc)
+    W, H = 192, 128
+    F = pyrtx
+    holes = None
+    for rf in (0, F.RTX_RAY_FLAG_NO_OPAQUE, F.RTX_RAY_FLAG_OPAQUE, F.RTX_RAY_FLAG_CULL_OPAQUE):
+        film = ctx.trace_rays(tlas, sc.records, sc.camera, W, H, sc.miss_color, ray_flags=rf).cpu().numpy()
+        of, st = O.trace_rays(t, sc.records, sc.camera, W, H, sc.miss_color, ray_flags=rf)
+        robust = st != 2
+        bad = robust & np.any(film != of, axis=-1)
+        _say(f"  flags {rf:#x}: robust {robust.mean():.4f}, mismatched robust pixels {int(bad.sum())}")
+        assert bad.sum() == 0
+        assert robust.mean() > 0.9
+        if rf == 0:
+            holes = film
+        if rf == F.RTX_RAY_FLAG_OPAQUE:
+            # the cutouts must have changed something: with OPAQUE no program runs
+            assert np.any(np.any(film != holes, axis=-1) & robust)
+
+
 def test_skip_closest_hit_program(ctx):
     import pyrtx
     sc = S.scene_r0()

@@ -138,6 +138,98 @@ def test_known_answers_axis_aligned():
     assert ids[0, 2] == 0 and ids[2, 2] == O.MISS
 
 
+def cutout_scene(front_opaque=False, inst_flags=0, n=4):
+    """A right triangle v0 (0,0,0), v1 (1,0,0), v2 (0,1,0) at z = 0, so that a ray down -z through (x, y) meets it
+    at barycentrics (u, v) = (x, y) exactly, plus an opaque backstop triangle at z = -1 (geometry 1).  Hit records:
+    geometry 0 carries the cutout any-hit program with n cells per edge."""
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    back = np.array([[-5, -5, -1], [10, -5, -1], [-5, 10, -1]], np.float32)  # one triangle: no edge over [0,1]^2
+    b = O.blas_create([(tri, np.array([[0, 1, 2]], np.uint32), None, front_opaque),
+                       (back, np.array([[0, 1, 2]], np.uint32), None, True)])
+    t = O.tlas_create(O.instance_records(S.translation()[None], [b], flags=[inst_flags]))
+    rec = S.hit_records([0, 0], [(1, 0, 0), (0, 1, 0)], any_hit=[S.ANY_HIT_CUTOUT | (n << 8), 0])
+    return t, rec
+
+
+def grid_rays(k=64):
+    # cell-centred sample points, never on a cell boundary of n = 4 (multiples of 1/4) nor on the diagonal
+    x = (np.arange(k) + 0.37) / k
+    X, Y = np.meshgrid(x, x)
+    rays = np.zeros((k * k, 8), np.float32)
+    rays[:, 0] = X.ravel()
+    rays[:, 1] = Y.ravel()
+    rays[:, 2] = 1
+    rays[:, 6] = -1
+    rays[:, 7] = 10
+    return rays
+
+
+def cutout_expected(rays, n=4, cut=True):
+    """Closed form: geometry 0 when inside the triangle and (not cut or the checker cell is kept), else 1."""
+    x, y = rays[:, 0].astype(np.float64), rays[:, 1].astype(np.float64)
+    inside = (x > 0) & (y > 0) & (x + y < 1)
+    kept = ((np.floor(n * x) + np.floor(n * y)) % 2) == 0
+    return np.where(inside & (kept | (not cut)), 0, 1)
+
+
+def test_any_hit_cutout_and_opacity_closed_form():
+    rays = grid_rays()
+    x, y = rays[:, 0].astype(np.float64), rays[:, 1].astype(np.float64)
+    inside = (x > 0) & (y > 0) & (x + y < 1)
+    away = np.abs(x + y - 1) > 1e-3  # the diagonal edge is a tie band
+    # non-opaque triangle with the cutout program: holes show the backstop
+    t, rec = cutout_scene()
+    tuv, ids, st = O.trace(t, rays, records=rec)
+    assert np.all(st[away] == 1)
+    assert np.array_equal(ids[away, 1], cutout_expected(rays)[away])
+    assert np.all(tuv[ids[:, 1] == 1, 0] == 2.0) and np.all(tuv[(ids[:, 1] == 0) & away, 0] == 1.0)
+    # (u, v) of a kept candidate are the hit point's x, y
+    k = away & (ids[:, 1] == 0)
+    assert np.abs(tuv[k, 1] - x[k]).max() < 1e-12 and np.abs(tuv[k, 2] - y[k]).max() < 1e-12
+    # without records no any-hit program runs (a hit group without an any-hit shader accepts)
+    _, ids0, _ = O.trace(t, rays)
+    assert np.array_equal(ids0[away, 1], cutout_expected(rays, cut=False)[away])
+    # ray OPAQUE overrides the geometry: no program
+    _, ids1, _ = O.trace(t, rays, records=rec, ray_flags=0x1)
+    assert np.array_equal(ids1[away, 1], cutout_expected(rays, cut=False)[away])
+    # CULL_NO_OPAQUE drops the triangle everywhere; CULL_OPAQUE drops the backstop
+    _, ids2, st2 = O.trace(t, rays, records=rec, ray_flags=0x80)
+    assert np.all(ids2[:, 1] == 1) and np.all(st2 == 1)
+    _, ids3, st3 = O.trace(t, rays, records=rec, ray_flags=0x40)
+    exp3 = cutout_expected(rays)
+    assert np.array_equal(st3[away] == 1, exp3[away] == 0)
+    # any-hit queries see only accepted candidates: a hole in front of nothing is a miss
+    _, _, st4 = O.trace(t, rays, any_hit=True, records=rec, ray_flags=0x40)
+    assert np.array_equal(st4[away] == 1, exp3[away] == 0)
+    # the instance's FORCE_OPAQUE wins over the geometry, the ray's NO_OPAQUE over the instance
+    t5, rec5 = cutout_scene(inst_flags=0x4)
+    _, ids5, _ = O.trace(t5, rays, records=rec5)
+    assert np.array_equal(ids5[away, 1], cutout_expected(rays, cut=False)[away])
+    _, ids6, _ = O.trace(t5, rays, records=rec5, ray_flags=0x2)
+    assert np.array_equal(ids6[away, 1], cutout_expected(rays)[away])
+    # an opaque geometry made non-opaque by the instance's FORCE_NO_OPAQUE runs its program
+    t7, rec7 = cutout_scene(front_opaque=True, inst_flags=0x8)
+    _, ids7, _ = O.trace(t7, rays, records=rec7)
+    assert np.array_equal(ids7[away, 1], cutout_expected(rays)[away])
+    assert inside.sum() > 1500
+
+
+def test_any_hit_cutout_boundaries_are_ties():
+    # rays exactly on the cell boundaries u = 1/4, 1/2 (x) of a n = 4 cutout are epsilon-ties, never robust
+    t, rec = cutout_scene()
+    ys = (np.arange(40) + 0.5) / 160
+    rays = np.zeros((3 * 40, 8), np.float32)
+    for i, xb in enumerate([0.25, 0.5, 0.125 + 1e-9]):
+        rays[40 * i:40 * (i + 1), 0] = xb
+        rays[40 * i:40 * (i + 1), 1] = ys
+    rays[:, 2] = 1
+    rays[:, 6] = -1
+    rays[:, 7] = 10
+    _, _, st = O.trace(t, rays, records=rec)
+    assert np.all(st[:80] == 2)
+    assert np.all(st[80:] == 1)  # mid-cell: robust
+
+
 def test_watertight_fan_and_ties():
     # 8-triangle fan around the origin: interior rays hit exactly one triangle, rays through the shared
     # vertex or along shared edges are epsilon-ties, never misses.
