@@ -332,9 +332,11 @@ __device__ __forceinline__ bool candidate_accepted(const TraceArgs& args, uint32
 template <int MODE, bool STATS, bool FLAGS>
 __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     const uint32_t ray_flags = FLAGS ? args.ray_flags : 0u;  // the default variant compiles the flags out
-    // dynamic LDS: the per-lane stack (lds_depth entries per lane); the FLAGS variant adds the lane's world ray
-    // (6 floats per lane), which it reads back where it needs it (leaving a BLAS, the final FP64 refinement)
-    // instead of keeping it in registers, the default variant keeps it in wo / wd
+    // dynamic LDS: the per-lane stack (lds_depth entries per lane); the FLAGS variant and the path-tracer modes
+    // add the lane's world ray (6 floats per lane), which they read back where they need it (leaving a BLAS, the
+    // final FP64 refinement) instead of keeping it in registers (measured: spill-free at 128 VGPRs and 1% faster
+    // path tracing); the other variants keep it in wo / wd
+    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused;
     extern __shared__ uint2 lds_stack[];
     float* const lds_ray = (float*)(lds_stack + args.lds_depth * blockDim.x) + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
@@ -434,7 +436,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         wd[2] = r1.z;
                         tc = r1.w;
                     }
-                    if (FLAGS) {
+                    if (kRayLds) {
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
                             lds_ray[k * blockDim.x] = wo[k];
@@ -491,7 +493,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         sp < D ? lds_stack[sp * blockDim.x + threadIdx.x] : args.gstack[gtid * GD + (sp - D)];
                     if ((e.y & kEntryTlas) && in_blas) {
                         in_blas = false;
-                        if (FLAGS) {
+                        if (kRayLds) {
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
                                 wo[k] = lds_ray[k * blockDim.x];
@@ -576,7 +578,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                             double M[12];
 #pragma unroll
                             for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
-                            if (FLAGS) {  // in the TLAS the current space is world space
+                            if (kRayLds) {  // in the TLAS the current space is world space
                                 wo[0] = s.ox;
                                 wo[1] = s.oy;
                                 wo[2] = s.oz;
@@ -733,7 +735,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
                         const TriSlot* tri =
                             (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes + (uint64_t)hit_slot * kSlotBytes);
-                        if (FLAGS) {
+                        if (kRayLds) {
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
                                 wo[k] = lds_ray[k * blockDim.x];
@@ -844,9 +846,10 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
 
 template <int MODE, bool STATS, bool FLAGS>
 hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
-    // the stack, and the lanes' world rays for the FLAGS variant
+    // the stack, and the lanes' world rays where the kernel keeps them in LDS
+    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused;
     const size_t lds =
-        (size_t)L.lds_depth * L.block_size * sizeof(uint2) + (FLAGS ? (size_t)L.block_size * 6 * sizeof(float) : 0);
+        (size_t)L.lds_depth * L.block_size * sizeof(uint2) + (kRayLds ? (size_t)L.block_size * 6 * sizeof(float) : 0);
     if (L.w4)
         trace_kernel_w4<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
     else
