@@ -904,9 +904,55 @@ __device__ __forceinline__ void quant_child(float clo, float chi, float p, uint3
     qhi = (uint8_t)qh;
 }
 
+// atomicAdd(addr, val), val < 16, for the active lanes of a wave whose addresses may differ (batched builds: one
+// header per task): one atomic per distinct address, lanes of one address get consecutive ranges in lane order.
+// Prefixes come from ballots of val's four bit planes, so lanes that are not active contribute nothing.  Every
+// active lane of the wave must call it.
+__device__ __forceinline__ uint32_t grouped_atomic_add(uint32_t* addr, uint32_t val) {
+    const uint32_t lane = __lane_id();
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long a = (unsigned long long)(uintptr_t)addr;
+    uint64_t todo = __ballot(1);
+    uint32_t out = 0;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const unsigned long long la = __shfl(a, leader);
+        const bool mine = a == la;
+        uint32_t prefix = 0, total = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint64_t m = __ballot(mine && ((val >> b) & 1u));
+            prefix += (uint32_t)__popcll(m & below) << b;
+            total += (uint32_t)__popcll(m) << b;
+        }
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(addr, total);
+        base = __shfl(base, leader);
+        if (mine) out = base + prefix;
+        todo &= ~__ballot(mine);
+    }
+    return out;
+}
+
+// atomicMax(addr, val) with a wave-uniform val, one atomic per distinct address of the active lanes.
+__device__ __forceinline__ void grouped_atomic_max(uint32_t* addr, uint32_t val) {
+    const unsigned long long a = (unsigned long long)(uintptr_t)addr;
+    uint64_t todo = __ballot(1);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const unsigned long long la = __shfl(a, leader);
+        const uint64_t grp = __ballot(a == la) & todo;
+        if ((int)__lane_id() == leader) atomicMax(addr, val);
+        todo &= ~grp;
+    }
+}
+
 // One collapse level.  Task = (BVH2 node code, destination slot); a negative node code is a leaf that is the
-// root itself (one-primitive structure).  Batched builds (as_ptr != nullptr) carry each task's BLAS in a parallel
-// array and write into that BLAS's structure.
+// root itself (one-primitive structure).  Batched builds (BATCHED, as_ptr) carry each task's BLAS in a parallel
+// array and write into that BLAS's structure.  Single builds keep the header pointer a kernel argument: the
+// per-node atomics on it then have a wave-uniform address, which the compiler combines into one atomic per wave
+// (a per-task pointer would send 64 times as many atomics to the same few words).
+template <bool BATCHED>
 __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const int32_t* __restrict__ left,
                                                              const int32_t* __restrict__ right,
                                                              const uint32_t* __restrict__ order,
@@ -923,9 +969,9 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
     if (count == 0) return;
     for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < count; ti += gridDim.x * blockDim.x) {
         uint2 task = tasks_in[ti];
-        const uint32_t tb = as_ptr ? tblas_in[ti] : 0u;
-        AsHeader* h = as_ptr ? as_ptr[tb] : h1;
-        uint4* slots = as_ptr ? (uint4*)((uint8_t*)h + kHeaderBytes) : slots1;
+        const uint32_t tb = BATCHED ? tblas_in[ti] : 0u;
+        AsHeader* h = BATCHED ? as_ptr[tb] : h1;
+        uint4* slots = BATCHED ? (uint4*)((uint8_t*)h + kHeaderBytes) : slots1;
         const float rlo[3] = {h->lo[0], h->lo[1], h->lo[2]};
         const float qs[3] = {h->qscale[0], h->qscale[1], h->qscale[2]};
         int32_t ch[8];
@@ -1041,7 +1087,7 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                 node.qhi[a][k] = 0;
             }
         const uint32_t nvalid = (uint32_t)__popc(used);
-        const uint32_t base = atomicAdd(&h->slot_count, nvalid);
+        const uint32_t base = BATCHED ? grouped_atomic_add(&h->slot_count, nvalid) : atomicAdd(&h->slot_count, nvalid);
         node.base = base;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -1055,7 +1101,7 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                 node.imask |= (uint8_t)(1u << k);
                 uint32_t o = atomicAdd(&level_counts[level + 1], 1u);
                 tasks_out[o] = make_uint2((uint32_t)c, slot);
-                if (as_ptr) tblas_out[o] = tb;
+                if (BATCHED) tblas_out[o] = tb;
             } else {
                 uint32_t prim = order[-1 - c];
                 const uint4* src = leaf_src + 4 * (uint64_t)prim;
@@ -1072,8 +1118,13 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
         ndst[1] = nsrc[1];
         ndst[2] = nsrc[2];
         ndst[3] = nsrc[3];
-        atomicAdd(&h->node_count, 1u);
-        atomicMax(&h->max_depth, (uint32_t)(level + 1));
+        if (BATCHED) {
+            grouped_atomic_add(&h->node_count, 1u);
+            grouped_atomic_max(&h->max_depth, (uint32_t)(level + 1));
+        } else {
+            atomicAdd(&h->node_count, 1u);
+            atomicMax(&h->max_depth, (uint32_t)(level + 1));
+        }
     }
 }
 
@@ -1294,7 +1345,7 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
         for (int lv = 0; lv < max_levels; ++lv) {
             uint2* tin = (lv & 1) ? t1 : t0;
             uint2* tout = (lv & 1) ? t0 : t1;
-            collapse_level_kernel<<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
+            collapse_level_kernel<false><<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
                                                                   (const uint4*)(scratch + L.leaves), h, slots, levels,
                                                                   lv, tin, tout, nullptr, nullptr, nullptr);
         }
@@ -1463,7 +1514,7 @@ hipError_t update_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, u
         for (int lv = 0; lv < kMaxLevels32; ++lv) {
             uint2* tin = (lv & 1) ? t1 : t0;
             uint2* tout = (lv & 1) ? t0 : t1;
-            collapse_level_kernel<<<kCollapseBlocks, 256, 0, s>>>(
+            collapse_level_kernel<false><<<kCollapseBlocks, 256, 0, s>>>(
                 n, (const int32_t*)(sc + L.left), (const int32_t*)(sc + L.right), (const uint32_t*)(sc + L.vals),
                 (const float*)(sc + L.nboxes), (const uint4*)(sc + L.leaves), h, (uint4*)((uint8_t*)dst + kHeaderBytes),
                 levels, lv, tin, tout, nullptr, nullptr, nullptr);
@@ -1549,7 +1600,7 @@ hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t ngeom, const ui
             uint2* tout = (uint2*)(sc + ((lv & 1) ? L.tasks0 : L.tasks1));
             uint32_t* bin = (uint32_t*)(sc + ((lv & 1) ? L.tblas1 : L.tblas0));
             uint32_t* bout = (uint32_t*)(sc + ((lv & 1) ? L.tblas0 : L.tblas1));
-            collapse_level_kernel<<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
+            collapse_level_kernel<true><<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
                                                                   (const uint4*)(sc + L.leaves), nullptr, nullptr,
                                                                   levels, lv, tin, tout, as_ptr, bin, bout);
         }
