@@ -114,6 +114,20 @@ int check_geometries(rtx_context ctx, const rtx_blas_geometry* g, uint32_t count
     for (uint32_t i = 0; i < count; ++i) {
         const rtx_blas_geometry& x = g[i];
         if (x.primitive_count == 0) continue;
+        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS) {
+            if (!x.vertex_data_device)
+                return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: AABB data is NULL", i);
+            if (x.vertex_stride < 24 || (x.vertex_stride & 7))
+                return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: AABB stride must be >= 24 and a multiple of 8",
+                            i);
+            if (x.primitive_offset & 7)
+                return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: AABB primitive_offset must be a multiple of 8",
+                            i);
+            n += x.primitive_count;
+            continue;
+        }
+        if (x.geometry_type != RTX_GEOMETRY_TYPE_TRIANGLES)
+            return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: unknown geometry_type %u", i, x.geometry_type);
         if (!x.vertex_data_device || !x.index_data_device)
             return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "geometry %u: vertex or index data is NULL", i);
         if (x.index_type != RTX_INDEX_TYPE_UINT32)
@@ -337,7 +351,14 @@ int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uin
         d.max_vertex = x.max_vertex;
         d.geom_local = i;
         d.flags = x.flags;
+        d.type = x.geometry_type;
+        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS) {
+            d.vtx = (const uint8_t*)x.vertex_data_device + x.primitive_offset;
+            d.idx = nullptr;
+            d.xform = nullptr;
+        }
         if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
+        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS && x.primitive_count) rtx::note_procedural_geometry();
         d.blas = 0;
         base += x.primitive_count;
     }
@@ -428,7 +449,14 @@ int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, 
             d.max_vertex = x.max_vertex;
             d.geom_local = i;
         d.flags = x.flags;
+        d.type = x.geometry_type;
+        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS) {
+            d.vtx = (const uint8_t*)x.vertex_data_device + x.primitive_offset;
+            d.idx = nullptr;
+            d.xform = nullptr;
+        }
         if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
+        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS && x.primitive_count) rtx::note_procedural_geometry();
             d.blas = b;
             base += x.primitive_count;
         }
@@ -479,7 +507,14 @@ int rtx_cmd_update_blas(rtx_context ctx, const rtx_blas_geometry* geometries, ui
         d.max_vertex = x.max_vertex;
         d.geom_local = i;
         d.flags = x.flags;
+        d.type = x.geometry_type;
+        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS) {
+            d.vtx = (const uint8_t*)x.vertex_data_device + x.primitive_offset;
+            d.idx = nullptr;
+            d.xform = nullptr;
+        }
         if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
+        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS && x.primitive_count) rtx::note_procedural_geometry();
         d.blas = 0;
         base += x.primitive_count;
     }

@@ -127,6 +127,15 @@ __global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restri
         if (prim_blas) prim_blas[i] = blas;
         TriSlot t;
         valid = true;
+        if (g.type == RTX_GEOMETRY_TYPE_AABBS) {
+            // VkAabbPositionsKHR: min xyz, max xyz; inactive when NaN or min > max
+            const float* src = (const float*)(g.vtx + (uint64_t)p * g.stride);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) t.v[k] = src[k];
+            t.v[6] = t.v[7] = t.v[8] = 0.0f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) valid = valid && isfinite(t.v[a]) && isfinite(t.v[3 + a]) && t.v[a] <= t.v[3 + a];
+        } else {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             uint32_t vi = g.idx[3 * (uint64_t)p + k];
@@ -154,17 +163,20 @@ __global__ __launch_bounds__(256) void prep_tris_kernel(const GeomDesc* __restri
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) valid = valid && isfinite(t.v[k]);
+        }
         t.prim = p;
         t.geom = g.geom_local;
-        t.bits = (valid ? kTriValid : 0u) | ((g.flags & RTX_GEOMETRY_OPAQUE_BIT) ? 0u : kTriNonOpaque);
+        t.bits = (valid ? kTriValid : 0u) | ((g.flags & RTX_GEOMETRY_OPAQUE_BIT) ? 0u : kTriNonOpaque) |
+                 (g.type == RTX_GEOMETRY_TYPE_AABBS ? kTriAabb : 0u);
         t.pad[0] = t.pad[1] = t.pad[2] = t.pad[3] = 0u;
         leaves[i] = t;
         float* b = boxes + 6 * (uint64_t)i;
         if (valid) {
+            const bool aabb = g.type == RTX_GEOMETRY_TYPE_AABBS;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                float l = fminf(t.v[a], fminf(t.v[3 + a], t.v[6 + a]));
-                float h = fmaxf(t.v[a], fmaxf(t.v[3 + a], t.v[6 + a]));
+                float l = aabb ? t.v[a] : fminf(t.v[a], fminf(t.v[3 + a], t.v[6 + a]));
+                float h = aabb ? t.v[3 + a] : fmaxf(t.v[a], fmaxf(t.v[3 + a], t.v[6 + a]));
                 b[a] = l;
                 b[3 + a] = h;
                 float s = l + h;

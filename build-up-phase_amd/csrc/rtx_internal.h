@@ -5,7 +5,7 @@
 //   [0, 256)        AsHeader
 //   [256, ...)      64-B slots.  Slot 0 is the root node.  A slot holds either
 //                     Node8    — compressed 8-wide node, children quantised to 8 bits per plane
-//                     TriSlot  — one triangle (BLAS leaf)
+//                     TriSlot  — one triangle or AABB primitive (BLAS leaf)
 //                     InstSlot — one instance reference (TLAS leaf)
 //                   The valid children of a node occupy consecutive slots starting at Node8::base, in the
 //                   node's child-position order (position k is the k-th set bit of vmask).
@@ -73,11 +73,12 @@ struct TriSlot {
     float v[9];  // v0, v1, v2 after the geometry transform
     uint32_t prim;
     uint32_t geom;
-    uint32_t bits;  // kTriValid | kTriNonOpaque
+    uint32_t bits;  // kTriValid | kTriNonOpaque | kTriAabb
     uint32_t pad[4];
 };
 constexpr uint32_t kTriValid = 1u;
 constexpr uint32_t kTriNonOpaque = 2u;  // its geometry lacks RTX_GEOMETRY_OPAQUE_BIT
+constexpr uint32_t kTriAabb = 4u;       // an AABB primitive: v[0..2] = min, v[3..5] = max, v[6..8] = 0
 static_assert(sizeof(TriSlot) == 64, "TriSlot must be 64 B");
 
 struct InstSlot {
@@ -112,6 +113,7 @@ struct GeomDesc {
     uint32_t geom_local;    // geometry index within its BLAS (gl_GeometryIndexEXT)
     uint32_t blas;          // BLAS of a batched build (0 otherwise)
     uint32_t flags;         // RTX_GEOMETRY_*_BIT
+    uint32_t type;          // RTX_GEOMETRY_TYPE_*
 };
 
 // Triangle record in build scratch (48 B): transformed vertices + IDs.

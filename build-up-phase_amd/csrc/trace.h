@@ -52,5 +52,7 @@ struct TraceLaunch {
 hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);
 // A BLAS with non-opaque geometry was built: later launches honour opacity and any-hit programs.
 void note_nonopaque_geometry();
+// A BLAS with AABB geometry was built: later launches run intersection programs.
+void note_procedural_geometry();
 
 }  // namespace rtx

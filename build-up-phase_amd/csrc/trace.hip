@@ -45,6 +45,7 @@ constexpr uint32_t kRaySkipTriangles = RTX_RAY_FLAG_SKIP_TRIANGLES;
 constexpr uint32_t kRayOpaque = RTX_RAY_FLAG_OPAQUE;
 constexpr uint32_t kRayNoOpaque = RTX_RAY_FLAG_NO_OPAQUE;
 constexpr uint32_t kRayCullNoOpaque = RTX_RAY_FLAG_CULL_NO_OPAQUE;
+constexpr uint32_t kRaySkipAabbs = RTX_RAY_FLAG_SKIP_AABBS;
 // FLAGS variant: the current instance register holds the instance id in bits 0-29 and the instance's
 // FORCE_OPAQUE (bit 30) / FORCE_NO_OPAQUE (bit 31) flags.
 constexpr uint32_t kInstIdMask = 0x3FFFFFFFu;
@@ -277,6 +278,132 @@ __device__ void refine_hit(const double* inv, const float* wo, const float* wd, 
     v = (float)vv;
 }
 
+// Intersection programs of AABB primitives (box lo = a.xyz, hi = a.w, b.x, b.y).  The first intersection with t
+// in [tmin, tmax] of the ray o + t d with the program's surface, its (u, v) and the outward normal n (object
+// space, unnormalised for boxes).  T is float for the traversal's candidate test (in the current rebased space)
+// and double for the reported hit.
+template <typename T>
+__device__ __forceinline__ bool procedural_hit(uint32_t prog, const T* lo, const T* hi, const T* o, const T* d,
+                                               T tmin, T tmax, T& t_out, T& u_out, T& v_out, T* n) {
+    if (prog == RTX_INTERSECTION_SPHERE) {
+        T c[3], oc[3];
+        T r = hi[0] - lo[0];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c[k] = (lo[k] + hi[k]) * T(0.5);
+            oc[k] = o[k] - c[k];
+            r = k ? (hi[k] - lo[k] < r ? hi[k] - lo[k] : r) : r;
+        }
+        r *= T(0.5);
+        const T A = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const T B = oc[0] * d[0] + oc[1] * d[1] + oc[2] * d[2];
+        const T C = oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2] - r * r;
+        const T disc = B * B - A * C;
+        if (!(disc >= T(0)) || !(A > T(0)) || !(r > T(0))) return false;
+        const T sq = sqrt(disc);
+        T t = (-B - sq) / A;
+        if (t < tmin) t = (-B + sq) / A;
+        if (!(t >= tmin && t <= tmax)) return false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) n[k] = (o[k] + t * d[k] - c[k]) / r;
+        t_out = t;
+        u_out = (n[0] + T(1)) * T(0.5);
+        v_out = (n[1] + T(1)) * T(0.5);
+        return true;
+    }
+    // RTX_INTERSECTION_BOX: slab entry, or the exit when the entry lies before tmin
+    T tn = -INFINITY, tf = INFINITY;
+    int an = 0, af = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (d[k] == T(0)) {
+            if (o[k] < lo[k] || o[k] > hi[k]) return false;
+            continue;
+        }
+        const T i = T(1) / d[k];
+        T t1 = (lo[k] - o[k]) * i, t2 = (hi[k] - o[k]) * i;
+        if (t1 > t2) {
+            const T x = t1;
+            t1 = t2;
+            t2 = x;
+        }
+        if (t1 > tn) {
+            tn = t1;
+            an = k;
+        }
+        if (t2 < tf) {
+            tf = t2;
+            af = k;
+        }
+    }
+    if (!(tn <= tf)) return false;
+    T t;
+    int ax;
+    if (tn >= tmin && tn <= tmax) {
+        t = tn;
+        ax = an;
+    } else if (tf >= tmin && tf <= tmax) {
+        t = tf;
+        ax = af;
+    } else {
+        return false;
+    }
+    const int a1 = ax == 2 ? 0 : ax + 1, a2 = ax == 0 ? 2 : ax - 1;
+    const T p1 = (a1 == 0 ? o[0] : (a1 == 1 ? o[1] : o[2])) + t * (a1 == 0 ? d[0] : (a1 == 1 ? d[1] : d[2]));
+    const T p2 = (a2 == 0 ? o[0] : (a2 == 1 ? o[1] : o[2])) + t * (a2 == 0 ? d[0] : (a2 == 1 ? d[1] : d[2]));
+    const T l1 = a1 == 0 ? lo[0] : (a1 == 1 ? lo[1] : lo[2]), h1 = a1 == 0 ? hi[0] : (a1 == 1 ? hi[1] : hi[2]);
+    const T l2 = a2 == 0 ? lo[0] : (a2 == 1 ? lo[1] : lo[2]), h2 = a2 == 0 ? hi[0] : (a2 == 1 ? hi[1] : hi[2]);
+    t_out = t;
+    u_out = h1 > l1 ? (p1 - l1) / (h1 - l1) : T(0);
+    v_out = h2 > l2 ? (p2 - l2) / (h2 - l2) : T(0);
+    // outward normal of the hit face: -d's sign along ax at the entry, +d's at the exit
+    const T dax = ax == 0 ? d[0] : (ax == 1 ? d[1] : d[2]);
+    const T sgn = (t == tn) ? (dax > T(0) ? T(-1) : T(1)) : (dax > T(0) ? T(1) : T(-1));
+    n[0] = ax == 0 ? sgn : T(0);
+    n[1] = ax == 1 ? sgn : T(0);
+    n[2] = ax == 2 ? sgn : T(0);
+    return true;
+}
+
+// Intersection program of an AABB primitive's hit record (bits 8-15 of its program), RTX_INTERSECTION_BOX without
+// a hit-record table.
+__device__ __forceinline__ uint32_t intersection_program(const TraceArgs& args, const InstanceInfo* info,
+                                                         uint32_t geom) {
+    if (!args.records) return RTX_INTERSECTION_BOX;
+    const uint32_t rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
+    return rec < args.nrec ? (args.records[rec].program >> 8) & 255u : RTX_INTERSECTION_BOX;
+}
+
+// FP64 (t, u, v) and world-space unit normal of an AABB primitive hit of the world ray (wo, wd) under the
+// instance transform with inverse inv; false when FP64 finds no intersection at or after tmin (then the caller
+// keeps the FP32 values).
+__device__ bool refine_procedural(const double* inv, const float* wo, const float* wd, const TriSlot* tri,
+                                  uint32_t prog, float tmin, float& t, float& u, float& v, float* nw) {
+    double o[3], d[3], lo[3], hi[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        o[r] = inv[4 * r] * wo[0] + inv[4 * r + 1] * wo[1] + inv[4 * r + 2] * wo[2] + inv[4 * r + 3];
+        d[r] = inv[4 * r] * wd[0] + inv[4 * r + 1] * wd[1] + inv[4 * r + 2] * wd[2];
+        lo[r] = tri->v[r];
+        hi[r] = tri->v[3 + r];
+    }
+    double tt, uu, vv, n[3];
+    if (!procedural_hit<double>(prog, lo, hi, o, d, (double)tmin, (double)INFINITY, tt, uu, vv, n)) return false;
+    t = (float)tt;
+    u = (float)uu;
+    v = (float)vv;
+    if (nw) {
+        double w[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) w[c] = inv[c] * n[0] + inv[4 + c] * n[1] + inv[8 + c] * n[2];
+        const double l = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        const double rl = l > 0.0 ? 1.0 / l : 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nw[c] = (float)(w[c] * rl);
+    }
+    return true;
+}
+
 // World-space geometric normal (v1 - v0) x (v2 - v0) of a triangle under the instance transform whose inverse is
 // inv: n_world = inv^T n_object (FP64, normalised, rounded to FP32).
 __device__ void world_normal(const double* inv, const TriSlot* tri, float* n) {
@@ -353,7 +480,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     const uint32_t D = args.lds_depth;
     const uint32_t GD = args.gdepth;
     // ray flags under which no triangle can be accepted
-    const bool cull_all = (ray_flags & kRaySkipTriangles) ||
+    const bool cull_all = ((ray_flags & kRaySkipTriangles) && (ray_flags & kRaySkipAabbs)) ||
                           ((ray_flags & kRayCullOpaque) && (ray_flags & (kRayOpaque | kRayCullNoOpaque))) ||
                           ((ray_flags & kRayCullNoOpaque) && (ray_flags & kRayNoOpaque));
     const uint32_t nA = args.n_ptr ? *args.n_ptr : args.n;
@@ -649,9 +776,24 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         if (STATS) ++st_tris;
                         float th, bu, bv;
                         const float smax = tc - s.toff;
-                        if (test_triangle(a, b, c, s, sh, smin, smax, th, bu, bv) &&
-                            !(ray_flags & (kRayCullBack | kRayCullFront) &&
-                              facing_culled(a, b, c, s, ray_flags, table[cur_inst & kInstIdMask].flags)) &&
+                        bool cand;
+                        if (FLAGS && (c.w & kTriAabb)) {
+                            // AABB primitive: the intersection program of its hit record
+                            const float lo[3] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)};
+                            const float hi[3] = {__uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y)};
+                            const float ro[3] = {s.ox, s.oy, s.oz}, rd[3] = {s.dx, s.dy, s.dz};
+                            float pn[3];
+                            cand = !(ray_flags & kRaySkipAabbs) &&
+                                   procedural_hit<float>(
+                                       intersection_program(args, table + (cur_inst & kInstIdMask), c.z), lo, hi, ro,
+                                       rd, smin, smax, th, bu, bv, pn);
+                        } else {
+                            cand = !(ray_flags & kRaySkipTriangles) &&
+                                   test_triangle(a, b, c, s, sh, smin, smax, th, bu, bv) &&
+                                   !(ray_flags & (kRayCullBack | kRayCullFront) &&
+                                     facing_culled(a, b, c, s, ray_flags, table[cur_inst & kInstIdMask].flags));
+                        }
+                        if (cand &&
                             (!FLAGS || candidate_accepted(args, ray_flags, c.w, c.z, cur_inst, table, bu, bv))) {
                             if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
                                 occl = true;
@@ -728,6 +870,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                     float t = 0.0f, u = 0.0f, v = 0.0f;
                     uint32_t prim = 0, geom = 0;
                     const InstanceInfo* info = nullptr;
+                    bool proc = false;  // an AABB primitive's hit: normal from its intersection program
+                    float pn[3] = {0.0f, 0.0f, 0.0f};
                     if (hit_inst != kNoHit) {
                         info = table + hit_inst;
                         double M[12];
@@ -742,7 +886,21 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                                 wd[k] = lds_ray[(3 + k) * blockDim.x];
                             }
                         }
-                        refine_hit(M, wo, wd, tri, t, u, v);
+                        if (FLAGS && (tri->bits & kTriAabb)) {
+                            // keep the traversal's FP32 t if FP64 finds no intersection (the normal then faces
+                            // the ray)
+                            proc = true;
+                            t = tc;
+                            if (!refine_procedural(M, wo, wd, tri, intersection_program(args, info, tri->geom),
+                                                   wtmin, t, u, v,
+                                                   (MODE == kModePath || MODE == kModeFused) ? pn : nullptr)) {
+                                const float l = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
+#pragma unroll
+                                for (int k = 0; k < 3; ++k) pn[k] = l > 0.0f ? -wd[k] / l : 0.0f;
+                            }
+                        } else {
+                            refine_hit(M, wo, wd, tri, t, u, v);
+                        }
                         prim = tri->prim;
                         geom = tri->geom;
                     }
@@ -762,7 +920,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                             for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
                             const TriSlot* tri = (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes +
                                                                   (uint64_t)hit_slot * kSlotBytes);
-                            world_normal(M, tri, ph.n);
+                            if (FLAGS && proc) {
+                                ph.n[0] = pn[0];
+                                ph.n[1] = pn[1];
+                                ph.n[2] = pn[2];
+                            } else {
+                                world_normal(M, tri, ph.n);
+                            }
                             ph.rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
                         }
                         args.path_hits[ray_id] = ph;
@@ -859,14 +1023,17 @@ hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t 
 
 // ray flags that need the FLAGS variant
 constexpr uint32_t kActiveRayFlags = kRayTerminateOnFirstHit | kRaySkipClosestHit | kRayCullBack | kRayCullFront |
-                                     kRayCullOpaque | kRayCullNoOpaque | kRayNoOpaque | kRaySkipTriangles;
+                                     kRayCullOpaque | kRayCullNoOpaque | kRayNoOpaque | kRaySkipTriangles |
+                                     kRaySkipAabbs;
 
 // Non-opaque geometry has been built in this process (set by the BLAS builds, never cleared): from then on rays
 // without the OPAQUE flag take the FLAGS variant, which resolves opacity and runs any-hit programs.
+// Likewise AABB geometry, whose intersection programs only the FLAGS variant runs.
 std::atomic<bool> g_nonopaque_geometry{false};
+std::atomic<bool> g_procedural_geometry{false};
 
 bool needs_flags_variant(uint32_t ray_flags) {
-    return (ray_flags & kActiveRayFlags) ||
+    return (ray_flags & kActiveRayFlags) || g_procedural_geometry.load(std::memory_order_relaxed) ||
            (!(ray_flags & kRayOpaque) && g_nonopaque_geometry.load(std::memory_order_relaxed));
 }
 
@@ -879,6 +1046,7 @@ hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
 }  // namespace
 
 void note_nonopaque_geometry() { g_nonopaque_geometry.store(true, std::memory_order_relaxed); }
+void note_procedural_geometry() { g_procedural_geometry.store(true, std::memory_order_relaxed); }
 
 hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStream_t s) {
     TraceArgs a{};

@@ -57,7 +57,7 @@ class BlasGeometry(ctypes.Structure):
                 ("index_data_device", ctypes.c_void_p), ("transform_data_device", ctypes.c_void_p),
                 ("primitive_count", ctypes.c_uint32), ("primitive_offset", ctypes.c_uint32),
                 ("first_vertex", ctypes.c_uint32), ("transform_offset", ctypes.c_uint32),
-                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32), ("geometry_type", ctypes.c_uint32)]
 
 
 class BuildSizes(ctypes.Structure):
@@ -114,6 +114,9 @@ RTX_INSTANCE_TRIANGLE_FACING_CULL_DISABLE_BIT, RTX_INSTANCE_TRIANGLE_FLIP_FACING
 RTX_INSTANCE_FORCE_OPAQUE_BIT, RTX_INSTANCE_FORCE_NO_OPAQUE_BIT = 0x4, 0x8
 RTX_PROGRAM_RECORD_COLOR, RTX_PROGRAM_BARYCENTRIC, RTX_PROGRAM_MATCH_BARYCENTRIC, RTX_PROGRAM_LAMBERTIAN = 0, 1, 2, 3
 RTX_ANY_HIT_ACCEPT, RTX_ANY_HIT_CUTOUT = 0, 1
+RTX_GEOMETRY_TYPE_TRIANGLES, RTX_GEOMETRY_TYPE_AABBS = 0, 1
+RTX_INTERSECTION_BOX, RTX_INTERSECTION_SPHERE = 0, 1
+RTX_RAY_FLAG_SKIP_AABBS = 0x200
 RTX_FILM_SRGB_BIT, RTX_FILM_SWAP_RB_BIT, RTX_FILM_OPAQUE_BIT = 0x1, 0x2, 0x4
 RTX_FILM_REFERENCE_BYTES = RTX_FILM_SWAP_RB_BIT
 RTX_COPY_MODE_CLONE, RTX_COPY_MODE_COMPACT = 0, 1
@@ -292,6 +295,14 @@ class Context:
         transform (float32 [3,4]) or None."""
         arr = (BlasGeometry * max(1, len(geoms)))()
         for i, g in enumerate(geoms):
+            if "aabbs" in g:  # procedural geometry: float32 [N, 6] boxes
+                b = g["aabbs"]
+                arr[i].vertex_data_device = b.data_ptr()
+                arr[i].vertex_stride = 24
+                arr[i].primitive_count = b.numel() // 6
+                arr[i].geometry_type = RTX_GEOMETRY_TYPE_AABBS
+                arr[i].flags = RTX_GEOMETRY_OPAQUE_BIT if g.get("opaque", True) else 0
+                continue
             v, ix, x = g["vertices"], g["indices"], g.get("transform")
             arr[i].vertex_data_device = v.data_ptr()
             arr[i].vertex_stride = 12
@@ -326,7 +337,7 @@ class Context:
         self._check(self.lib.rtx_cmd_build_blas(self.handle, ctypes.cast(arr, _P), len(geoms), flags,
                                                 storage.data_ptr(), storage.numel(), scratch.data_ptr(),
                                                 scratch.numel(), ctypes.byref(addr)))
-        n = sum(g["indices"].numel() // 3 for g in geoms)
+        n = sum((g["aabbs"].numel() // 6) if "aabbs" in g else (g["indices"].numel() // 3) for g in geoms)
         return AccelerationStructure(storage, addr.value, n, (scratch, tuple(geoms)))
 
     def update_blas(self, blas: AccelerationStructure, geoms):
@@ -690,9 +701,14 @@ class Context:
         instances.  Returns (blases, tlas)."""
         torch = _torch()
         lists = []
+        from .scenes import AabbGeometry
         for geoms in scene.blas:
             dg = []
             for g in geoms:
+                if isinstance(g, AabbGeometry):
+                    dg.append({"aabbs": _dev_tensor(np.ascontiguousarray(g.boxes, np.float32).reshape(-1, 6),
+                                                    self.device), "opaque": g.opaque})
+                    continue
                 d = {"vertices": _dev_tensor(g.vertices.astype(np.float32), self.device),
                      "indices": _dev_tensor(g.indices.astype(np.uint32).view(np.int32), self.device),
                      "transform": None if g.transform is None else

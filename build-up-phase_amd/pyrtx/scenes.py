@@ -34,6 +34,24 @@ class Geometry:
 
 
 @dataclass
+class AabbGeometry:
+    """Procedural geometry (RTX_GEOMETRY_TYPE_AABBS): one box per primitive, min xyz / max xyz; its surface comes
+    from the hit record's intersection program."""
+    boxes: np.ndarray               # float32 [N, 6]
+    opaque: bool = True
+    transform: None = None
+
+    @property
+    def indices(self) -> np.ndarray:  # no triangles
+        return np.zeros((0, 3), np.uint32)
+
+    @property
+    def vertices(self) -> np.ndarray:  # the boxes' min and max corners (for bounds)
+        b = np.asarray(self.boxes, np.float32).reshape(-1, 6)
+        return np.concatenate([b[:, :3], b[:, 3:]], 0)
+
+
+@dataclass
 class Scene:
     blas: list                      # list of list[Geometry]
     transforms: np.ndarray          # float32 [I, 3, 4] object -> world
@@ -67,6 +85,13 @@ PROGRAM_RECORD_COLOR, PROGRAM_BARYCENTRIC, PROGRAM_MATCH_BARYCENTRIC, PROGRAM_LA
 INSTANCE_TRIANGLE_FACING_CULL_DISABLE = 0x1
 ANY_HIT_ACCEPT = 0      # rtx_hit_record.any_hit programs (RTX_ANY_HIT_*); parameter in bits 8-31
 ANY_HIT_CUTOUT = 1
+INTERSECTION_BOX = 0    # intersection programs of AABB geometry: bits 8-15 of rtx_hit_record.program
+INTERSECTION_SPHERE = 1
+
+
+def hit_group(closest: int, intersection: int = INTERSECTION_BOX) -> int:
+    """rtx_hit_record.program word (RTX_HIT_GROUP)."""
+    return closest | (intersection << 8)
 
 
 def hit_records(programs, colors, emissions=None, match=None, any_hit=None) -> np.ndarray:

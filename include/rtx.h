@@ -99,8 +99,16 @@ typedef struct rtx_blas_geometry {
     uint32_t first_vertex;
     uint32_t transform_offset;        /* bytes into transform data, multiple of 16 */
     uint32_t flags;                   /* RTX_GEOMETRY_*_BIT */
-    uint32_t reserved;
+    uint32_t geometry_type;           /* RTX_GEOMETRY_TYPE_* (VkGeometryTypeKHR) */
 } rtx_blas_geometry;
+
+/* VkGeometryTypeKHR.  An AABB geometry (VkAccelerationStructureGeometryAabbsDataKHR) reads primitive p's box
+ * (VkAabbPositionsKHR: min x y z, max x y z as 6 floats) at vertex_data_device + primitive_offset + p *
+ * vertex_stride bytes (stride a multiple of 8, >= 24; index and transform fields unused); a box with a NaN or
+ * min > max is inactive.  Its candidate hits come from the intersection program of the hit record
+ * (RTX_HIT_GROUP), RTX_INTERSECTION_BOX when the call has no hit-record table. */
+#define RTX_GEOMETRY_TYPE_TRIANGLES 0u
+#define RTX_GEOMETRY_TYPE_AABBS 1u
 
 /* VkAccelerationStructureInstanceKHR, byte-for-byte (64 B; main.cpp:848-858).  transform is the
  * object-to-world row-major 3x4; blas_address is a value returned by rtx_cmd_build_blas. */
@@ -219,13 +227,24 @@ void rtx_obj_free(rtx_obj_mesh* mesh);
 
 /* ---- shader binding table ----------------------------------------------------------------------- */
 
-/* Compiled-in hit programs (the reference compiles GLSL at run time, shader_module.h:11-74). */
+/* Compiled-in hit programs (the reference compiles GLSL at run time, shader_module.h:11-74).
+ * rtx_hit_record.program = RTX_HIT_GROUP(closest-hit program, intersection program). */
 #define RTX_PROGRAM_RECORD_COLOR 0u      /* hitValue = record color */
 #define RTX_PROGRAM_BARYCENTRIC 1u       /* hitValue = (1-u-v, u, v) */
 #define RTX_PROGRAM_MATCH_BARYCENTRIC 2u /* main.cpp:1080-1091: barycentric colour when gl_PrimitiveID,
                                             gl_InstanceID, gl_InstanceCustomIndexEXT and gl_GeometryIndexEXT
                                             all equal the record's match_* fields, else the record color */
 #define RTX_PROGRAM_LAMBERTIAN 3u        /* path tracer: albedo = color, emission = emission */
+
+/* Compiled-in intersection programs for AABB geometry (bits 8-15 of rtx_hit_record.program).  Each reports the
+ * first intersection with t in [tmin, tmax] of the ray with a surface inside the primitive's box, and (u, v):
+ *   BOX     the box's surface; (u, v) = the hit point's position across the hit face, along the two other axes in
+ *           cyclic order (x -> y, z; y -> z, x; z -> x, y), each in [0, 1] from the box's min to max corner;
+ *   SPHERE  the sphere centred in the box with radius half the smallest extent; (u, v) = ((n.x + 1) / 2,
+ *           (n.y + 1) / 2) for the outward unit normal n in object space. */
+#define RTX_INTERSECTION_BOX 0u
+#define RTX_INTERSECTION_SPHERE 1u
+#define RTX_HIT_GROUP(closest, intersection) ((uint32_t)(closest) | ((uint32_t)(intersection) << 8))
 
 /* Compiled-in any-hit programs (rtx_hit_record.any_hit: program in bits 0-7, parameter in bits 8-31), run on the
  * non-opaque candidate hits of the record's geometry.  (u, v) are the candidate's hitAttributeEXT barycentrics,
@@ -284,6 +303,7 @@ typedef struct rtx_camera {
 #define RTX_RAY_FLAG_CULL_OPAQUE 0x40u
 #define RTX_RAY_FLAG_CULL_NO_OPAQUE 0x80u
 #define RTX_RAY_FLAG_SKIP_TRIANGLES 0x100u
+#define RTX_RAY_FLAG_SKIP_AABBS 0x200u          /* gl_RayFlagsSkipAABBEXT */
 typedef struct rtx_trace_params {
     uint32_t ray_flags;    /* RTX_RAY_FLAG_* */
     uint32_t cull_mask;    /* 8-bit, ANDed with the instance mask */

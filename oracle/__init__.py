@@ -82,10 +82,20 @@ def _ptr(a: np.ndarray) -> int:
 
 
 def _geom_arrays(geoms):
-    """geoms: list of (vertices float32 [V,3], indices uint32 [T,3], transform float32 [3,4] or None[, opaque])."""
+    """geoms: list of (vertices float32 [V,3], indices uint32 [T,3], transform float32 [3,4] or None[, opaque]) or
+    {"aabbs": float32 [N,6] boxes (min xyz, max xyz), "opaque": bool} for AABB geometry."""
     keep = []
     vp, ip, xp, pc, fl = [], [], [], [], []
     for g in geoms:
+        if isinstance(g, dict):
+            b = np.ascontiguousarray(g["aabbs"], dtype=np.float32).reshape(-1, 6)
+            keep.append(b)
+            vp.append(_ptr(b))
+            ip.append(0)
+            xp.append(0)
+            pc.append(b.shape[0])
+            fl.append((1 if g.get("opaque", True) else 0) | 0x100)
+            continue
         v, i, x = g[:3]
         fl.append(1 if (len(g) < 4 or g[3]) else 0)
         v = np.ascontiguousarray(v, dtype=np.float32)

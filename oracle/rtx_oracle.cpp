@@ -70,9 +70,10 @@ inline void xform_f32(const float* m, const float* v, float* o) {
 }
 
 struct Tri {
-    float v[9];
+    float v[9];  // triangle vertices, or an AABB primitive's min (v[0..2]) and max (v[3..5])
     uint32_t prim, geom;
     bool opaque;  // its geometry has VK_GEOMETRY_OPAQUE_BIT
+    bool aabb;    // an AABB primitive (VK_GEOMETRY_TYPE_AABBS_KHR)
 };
 
 struct BNode {
@@ -106,6 +107,13 @@ std::vector<Blas*> g_blas;
 std::vector<Tlas*> g_tlas;
 
 void tri_box(const Tri& t, double* lo, double* hi) {
+    if (t.aabb) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = t.v[a];
+            hi[a] = t.v[3 + a];
+        }
+        return;
+    }
     for (int a = 0; a < 3; ++a) {
         double x0 = t.v[a], x1 = t.v[3 + a], x2 = t.v[6 + a];
         lo[a] = std::min(x0, std::min(x1, x2));
@@ -249,6 +257,89 @@ bool eval_tri(const RayFrame& f, const Tri& tr, double delta, Cand& c) {
     return true;
 }
 
+// Intersection programs of AABB primitives (rtx.h RTX_INTERSECTION_*), FP64: the first intersection with t in
+// [tmin, tmax] of o + t d with the program's surface, (u, v) and the outward object-space normal.  BOX: the box
+// surface, (u, v) across the hit face along the next two axes in cyclic order; SPHERE: the sphere centred in the
+// box with radius half the smallest extent, (u, v) = ((n.x + 1) / 2, (n.y + 1) / 2).  `tie` reports decisions
+// FP32 could make differently: a grazing sphere, a hit within 1e-5 (relative) of a box face's edge, a box corner.
+bool proc_hit(uint32_t prog, const Tri& tr, const d3& o, const d3& d, double tmin, double tmax, double& t, double& u,
+              double& v, double* n, bool& tie) {
+    const double lo[3] = {tr.v[0], tr.v[1], tr.v[2]}, hi[3] = {tr.v[3], tr.v[4], tr.v[5]};
+    const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    tie = false;
+    if (prog == 1) {
+        double c[3], oc[3], r = std::min(hi[0] - lo[0], std::min(hi[1] - lo[1], hi[2] - lo[2])) * 0.5;
+        for (int k = 0; k < 3; ++k) {
+            c[k] = (lo[k] + hi[k]) * 0.5;
+            oc[k] = oo[k] - c[k];
+        }
+        const double A = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+        const double B = oc[0] * dd[0] + oc[1] * dd[1] + oc[2] * dd[2];
+        const double C = oc[0] * oc[0] + oc[1] * oc[1] + oc[2] * oc[2] - r * r;
+        const double disc = B * B - A * C;
+        if (!(A > 0) || !(r > 0)) return false;
+        if (std::fabs(disc) <= 1e-4 * A * r * r) tie = true;  // grazing
+        if (disc < 0) return false;
+        const double sq = std::sqrt(disc);
+        double tt = (-B - sq) / A;
+        if (std::fabs(tt - tmin) <= 1e-5 * std::max(std::fabs(tt), std::fabs(tmin)) + 1e-7) tie = true;  // root at tmin
+        if (tt < tmin) tt = (-B + sq) / A;
+        if (!(tt >= tmin && tt <= tmax)) return false;
+        for (int k = 0; k < 3; ++k) n[k] = (oo[k] + tt * dd[k] - c[k]) / r;
+        t = tt;
+        u = (n[0] + 1) * 0.5;
+        v = (n[1] + 1) * 0.5;
+        return true;
+    }
+    double tn = -INFINITY, tf = INFINITY;
+    int an = 0, af = 0;
+    double ts_near[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 3; ++k) {
+        if (dd[k] == 0) {
+            if (oo[k] < lo[k] || oo[k] > hi[k]) return false;
+            continue;
+        }
+        double t1 = (lo[k] - oo[k]) / dd[k], t2 = (hi[k] - oo[k]) / dd[k];
+        if (t1 > t2) std::swap(t1, t2);
+        ts_near[k] = t1;
+        if (t1 > tn) {
+            tn = t1;
+            an = k;
+        }
+        if (t2 < tf) {
+            tf = t2;
+            af = k;
+        }
+    }
+    if (!(tn <= tf)) {
+        if (tn <= tf + 1e-5 * (std::fabs(tn) + std::fabs(tf))) tie = true;
+        return false;
+    }
+    if (std::fabs(tn - tmin) <= 1e-5 * std::max(std::fabs(tn), std::fabs(tmin)) + 1e-7) tie = true;  // entry at tmin
+    int ax;
+    if (tn >= tmin && tn <= tmax) {
+        t = tn;
+        ax = an;
+    } else if (tf >= tmin && tf <= tmax) {
+        t = tf;
+        ax = af;
+    } else {
+        return false;
+    }
+    const int a1 = (ax + 1) % 3, a2 = (ax + 2) % 3;
+    const double p1 = oo[a1] + t * dd[a1], p2 = oo[a2] + t * dd[a2];
+    u = hi[a1] > lo[a1] ? (p1 - lo[a1]) / (hi[a1] - lo[a1]) : 0;
+    v = hi[a2] > lo[a2] ? (p2 - lo[a2]) / (hi[a2] - lo[a2]) : 0;
+    const double e = 1e-5;
+    if (u < e || u > 1 - e || v < e || v > 1 - e) tie = true;  // near an edge of the hit face
+    const double sgn = (t == tn) ? (dd[ax] > 0 ? -1.0 : 1.0) : (dd[ax] > 0 ? 1.0 : -1.0);
+    n[0] = ax == 0 ? sgn : 0;
+    n[1] = ax == 1 ? sgn : 0;
+    n[2] = ax == 2 ? sgn : 0;
+    (void)ts_near;
+    return true;
+}
+
 struct HitOut {
     double t, u, v;
     uint32_t prim, geom, inst, custom;
@@ -292,13 +383,16 @@ int facing_cull(const RayFrame& f, const Tri& tr, uint32_t ray_flags, uint32_t i
 struct AnyHitTable {
     const uint8_t* records = nullptr;
     uint32_t nrec = 0, sbt_offset = 0, sbt_stride = 1;
-    uint32_t word(uint32_t inst_sbt, uint32_t geom) const {
+    uint32_t field(uint32_t inst_sbt, uint32_t geom, int offset) const {
         const uint32_t r = inst_sbt + geom * sbt_stride + sbt_offset;
         if (!records || r >= nrec) return 0;
         uint32_t w;
-        std::memcpy(&w, records + 48 * (size_t)r + 44, 4);
+        std::memcpy(&w, records + 48 * (size_t)r + offset, 4);
         return w;
     }
+    uint32_t word(uint32_t inst_sbt, uint32_t geom) const { return field(inst_sbt, geom, 44); }  // any_hit
+    // intersection program of an AABB primitive: bits 8-15 of the record's program (BOX without a table)
+    uint32_t intersection(uint32_t inst_sbt, uint32_t geom) const { return (field(inst_sbt, geom, 0) >> 8) & 255u; }
 };
 
 // Candidate filter of traceRayEXT for non-opaque geometry.  Opacity: the geometry's OPAQUE bit, overridden by
@@ -366,10 +460,25 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
         for (int k = 0; k < n.count; ++k) {
             const Tri& tr = b.tris[b.order[n.right + k]];
             Cand c;
-            if (!eval_tri(f, tr, b.delta, c)) continue;
-            const int cull = facing_cull(f, tr, ray_flags, inst_flags);
-            if (cull == 1) continue;
-            if (cull == 2) c.near = true;
+            if (tr.aabb) {
+                if (ray_flags & 0x200u) continue;  // SKIP_AABBS
+                const uint32_t prog = ah ? ah->intersection(inst_sbt, tr.geom) : 0u;
+                double nn[3];
+                bool tie = false;
+                const bool hit = proc_hit(prog, tr, f.o, f.d, tmin, tmax, c.t, c.u, c.v, nn, tie);
+                if (!hit && !tie) continue;
+                c.inside = hit;
+                c.near = tie;
+                c.margin = 0;
+                c.tunc = 1e-5 * std::fabs(c.t) + 1e-7;
+                if (!hit) c.t = tmin;  // a grazing miss: a tie candidate at an unknown distance
+            } else {
+                if (ray_flags & 0x100u) continue;  // SKIP_TRIANGLES
+                if (!eval_tri(f, tr, b.delta, c)) continue;
+                const int cull = facing_cull(f, tr, ray_flags, inst_flags);
+                if (cull == 1) continue;
+                if (cull == 2) c.near = true;
+            }
             const int keep = any_hit_filter(tr, inst_flags, inst_sbt, ray_flags, c.u, c.v, ah);
             if (keep == 0) continue;
             if (keep == 2) {  // may be ignored on the GPU: a tie candidate that cannot win robustly
@@ -405,7 +514,8 @@ void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, Hit
     d3 o{ray[0], ray[1], ray[2]}, d{ray[4], ray[5], ray[6]};
     double tmin = ray[3], tmax = ray[7];
     Collector col;
-    const bool skip_all = (ray_flags & 0x100u) || ((ray_flags & 0x40u) && (ray_flags & (0x1u | 0x80u))) ||
+    const bool skip_all = ((ray_flags & 0x100u) && (ray_flags & 0x200u)) ||
+                          ((ray_flags & 0x40u) && (ray_flags & (0x1u | 0x80u))) ||
                           ((ray_flags & 0x80u) && (ray_flags & 0x2u));
     for (size_t i = 0; i < tl.inst.size() && !skip_all; ++i) {
         const Inst& in = tl.inst[i];
@@ -520,6 +630,22 @@ int oracle_blas_create(int ngeom, const float* const* vtx, const uint32_t* const
                        const float* const* xform, const uint32_t* geom_flags) {
     Blas* b = new Blas();
     for (int g = 0; g < ngeom; ++g) {
+        if (geom_flags && (geom_flags[g] & 0x100u)) {  // AABB geometry: 6 floats per primitive in vtx[g]
+            for (uint32_t p = 0; p < prim_count[g]; ++p) {
+                Tri t;
+                std::memcpy(t.v, vtx[g] + 6 * (size_t)p, 24);
+                t.v[6] = t.v[7] = t.v[8] = 0.0f;
+                bool ok = true;
+                for (int a = 0; a < 3; ++a) ok = ok && std::isfinite(t.v[a]) && std::isfinite(t.v[3 + a]) && t.v[a] <= t.v[3 + a];
+                if (!ok) continue;  // inactive
+                t.prim = p;
+                t.geom = (uint32_t)g;
+                t.opaque = (geom_flags[g] & 0x1u) != 0;
+                t.aabb = true;
+                b->tris.push_back(t);
+            }
+            continue;
+        }
         for (uint32_t p = 0; p < prim_count[g]; ++p) {
             Tri t;
             for (int k = 0; k < 3; ++k) {
@@ -532,6 +658,7 @@ int oracle_blas_create(int ngeom, const float* const* vtx, const uint32_t* const
             t.prim = p;
             t.geom = (uint32_t)g;
             t.opaque = !geom_flags || (geom_flags[g] & 0x1u);
+            t.aabb = false;
             b->tris.push_back(t);
         }
     }
@@ -1156,13 +1283,27 @@ PHit pt_closest(const Tlas& tl, const double* o, const double* d, double tmin, d
     h.geom = col.geom;
     h.prim = col.prim;
     const Tri& tr = *col.tri;
-    double e1[3], e2[3];
-    for (int a = 0; a < 3; ++a) {
-        e1[a] = (double)tr.v[3 + a] - tr.v[a];
-        e2[a] = (double)tr.v[6 + a] - tr.v[a];
-    }
-    double no[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
     const double* M = tl.inst[col.inst].minv;
+    double no[3];
+    if (tr.aabb) {  // the intersection program's normal at the hit, from the object-space ray
+        d3 oo{M[0] * o[0] + M[1] * o[1] + M[2] * o[2] + M[3], M[4] * o[0] + M[5] * o[1] + M[6] * o[2] + M[7],
+              M[8] * o[0] + M[9] * o[1] + M[10] * o[2] + M[11]};
+        d3 dd{M[0] * d[0] + M[1] * d[1] + M[2] * d[2], M[4] * d[0] + M[5] * d[1] + M[6] * d[2],
+              M[8] * d[0] + M[9] * d[1] + M[10] * d[2]};
+        double t, u, v;
+        bool tie;
+        const uint32_t prog = ah ? ah->intersection(tl.inst[col.inst].sbt_offset, tr.geom) : 0u;
+        if (!proc_hit(prog, tr, oo, dd, tmin, tmax, t, u, v, no, tie)) no[0] = no[1] = no[2] = 0;
+    } else {
+        double e1[3], e2[3];
+        for (int a = 0; a < 3; ++a) {
+            e1[a] = (double)tr.v[3 + a] - tr.v[a];
+            e2[a] = (double)tr.v[6 + a] - tr.v[a];
+        }
+        no[0] = e1[1] * e2[2] - e1[2] * e2[1];
+        no[1] = e1[2] * e2[0] - e1[0] * e2[2];
+        no[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    }
     double nw[3];
     for (int c = 0; c < 3; ++c) nw[c] = M[c] * no[0] + M[4 + c] * no[1] + M[8 + c] * no[2];
     double l = std::sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);

@@ -8,7 +8,14 @@ import oracle as O
 
 def oracle_scene(scene):
     """Build scene in the FP64 oracle.  Returns the oracle TLAS handle."""
-    handles = [O.blas_create([(g.vertices, g.indices, g.transform, g.opaque) for g in geoms]) for geoms in scene.blas]
+    from pyrtx.scenes import AabbGeometry
+
+    def geom(g):
+        if isinstance(g, AabbGeometry):
+            return {"aabbs": g.boxes, "opaque": g.opaque}
+        return (g.vertices, g.indices, g.transform, g.opaque)
+
+    handles = [O.blas_create([geom(g) for g in geoms]) for geoms in scene.blas]
     rec = O.instance_records(scene.transforms, [handles[b] for b in scene.blas_of_instance], scene.custom_index,
                              scene.mask, scene.sbt_offset, scene.flags)
     return O.tlas_create(rec)

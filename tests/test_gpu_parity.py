@@ -517,6 +517,59 @@ def test_any_hit_programs_match_oracle(ctx):
             assert np.any(np.any(film != holes, axis=-1) & robust)
 
 
+def _procedural_scene(pyrtx, seed=21, instances=12):
+    """Instances of one BLAS holding a geometry of 3000 random boxes (AABB primitives) and a terrain (triangles)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-2, 2, (3000, 3))
+    h = rng.uniform(0.03, 0.25, (3000, 3))
+    boxes = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    ter = S.terrain(12, size=4.0, height=0.3, seed=seed)
+    R = S.random_rotation(rng, instances)
+    xf = np.zeros((instances, 3, 4))
+    xf[:, :, :3] = R * rng.uniform(0.5, 1.5, instances)[:, None, None]
+    xf[:, :, 3] = rng.uniform(-6, 6, (instances, 3))
+    sc = S.Scene(blas=[[S.AabbGeometry(boxes), ter]], transforms=xf.astype(np.float32),
+                 blas_of_instance=np.zeros(instances, int), custom_index=np.arange(instances, dtype=np.uint32),
+                 mask=np.full(instances, 0xFF, np.uint32), sbt_offset=2 * np.arange(instances, dtype=np.uint32),
+                 flags=np.zeros(instances, np.uint32), camera=(0.0, 0.0, 20.0, 60.0), width=192, height=128,
+                 meta={"name": "procedural"})
+    progs = []
+    for i in range(instances):  # geometry 0 (boxes): sphere or box program; geometry 1 (terrain): triangles
+        progs += [S.hit_group((pyrtx.RTX_PROGRAM_BARYCENTRIC, pyrtx.RTX_PROGRAM_RECORD_COLOR)[i % 2],
+                              (S.INTERSECTION_SPHERE, S.INTERSECTION_BOX)[(i // 2) % 2]), pyrtx.RTX_PROGRAM_RECORD_COLOR]
+    sc.records = S.hit_records(progs, rng.uniform(0.1, 0.9, (2 * instances, 3)))
+    return sc
+
+
+def test_procedural_geometry_matches_oracle(ctx):
+    """AABB geometry: ray buffers (no hit-record table: box program) and films (sphere / box intersection programs
+    from the records) against the oracle; SKIP_AABBS / SKIP_TRIANGLES."""
+    import pyrtx
+    _say("procedural geometry")
+    sc = _procedural_scene(pyrtx)
+    blases, tlas = ctx.upload_scene(sc)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=9)])
+    dr = torch.from_numpy(rays).cuda()
+    for rf in (0, pyrtx.RTX_RAY_FLAG_SKIP_AABBS, pyrtx.RTX_RAY_FLAG_SKIP_TRIANGLES):
+        g = pyrtx.hits_to_numpy(ctx.trace_ray_buffer(tlas, dr, ray_flags=rf))
+        tuv, ids, st = O.trace(t, rays, ray_flags=rf)
+        rep = compare_closest(g, tuv, ids, st)
+        _say(f"  flags {rf:#x}: {rep}")
+        assert rep["robust_hits"] > 500
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0 and rep["uv_bad"] == 0
+        occ = ctx.trace_ray_buffer(tlas, dr, pyrtx.RTX_QUERY_ANY_HIT, ray_flags=rf).cpu().numpy()
+        _, _, sta = O.trace(t, rays, any_hit=True, ray_flags=rf)
+        assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
+    W, H = sc.width, sc.height
+    film = ctx.trace_rays(tlas, sc.records, sc.camera, W, H, sc.miss_color).cpu().numpy()
+    of, st = O.trace_rays(t, sc.records, sc.camera, W, H, sc.miss_color)
+    robust = st != 2
+    close = np.all(np.abs(film - of) <= 1e-5, axis=-1)
+    _say(f"  film: robust {robust.mean():.4f}, robust pixels off by > 1e-5: {int((robust & ~close).sum())}")
+    assert robust.mean() > 0.95 and np.all(close[robust])
+
+
 def test_skip_closest_hit_program(ctx):
     import pyrtx
     sc = S.scene_r0()

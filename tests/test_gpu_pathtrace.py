@@ -99,6 +99,27 @@ def test_cutout_sphere_matches_fp64_oracle(ctx):
     assert np.abs(g2 - g).max() > 0.1
 
 
+def test_procedural_sphere_matches_fp64_oracle(ctx):
+    """The Cornell sphere as one AABB primitive with the sphere intersection program (normals from the program)."""
+    _say("cornell with a procedural sphere vs oracle")
+    W, H, spp = 96, 54, 16
+    sc, lights = S.cornell_scene(2, width=W, height=H)
+    c, r = np.array([0.58, 0.31, 0.36]), 0.3
+    sc.blas[1] = [S.AabbGeometry(np.concatenate([c - r, c + r])[None].astype(np.float32))]
+    sc.records["program"][4] = S.hit_group(S.PROGRAM_LAMBERTIAN, S.INTERSECTION_SPHERE)
+    blases, tlas = ctx.upload_scene(sc)
+    film = ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=spp)
+    ctx.sync()
+    g = film.cpu().numpy()[..., :3].astype(np.float64)
+    t = oracle_scene(sc)
+    o = O.render(t, sc.records, sc.camera, lights, W, H, spp) / spp
+    err = relmse(g, o)
+    close = np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o))
+    _say(f"  relMSE(gpu, oracle) = {err:.3g}, pixels within 1e-3: {close.all(-1).mean():.4f}")
+    assert err < 1e-3
+    assert close.all(-1).mean() > 0.95
+
+
 def test_render_is_deterministic_and_accumulates_fp64(ctx):
     _say("determinism")
     W, H = 64, 36

@@ -230,6 +230,64 @@ def test_any_hit_cutout_boundaries_are_ties():
     assert np.all(st[80:] == 1)  # mid-cell: robust
 
 
+def _proc_tlas(boxes, opaque=True, tri=None):
+    geoms = [{"aabbs": np.asarray(boxes, np.float32), "opaque": opaque}]
+    if tri is not None:
+        geoms.append(tri)
+    b = O.blas_create(geoms)
+    return O.tlas_create(O.instance_records(S.translation()[None], [b]))
+
+
+def _down_rays(x, y, z0=10.0, tmin=0.0):
+    rays = np.zeros((len(x), 8), np.float32)
+    rays[:, 0], rays[:, 1], rays[:, 2], rays[:, 3] = x, y, z0, tmin
+    rays[:, 6] = -1
+    rays[:, 7] = 100
+    return rays
+
+
+def test_intersection_programs_closed_form():
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1.4, 1.4, 3000).astype(np.float32)
+    y = rng.uniform(-1.4, 1.4, 3000).astype(np.float32)
+    xd, yd = x.astype(np.float64), y.astype(np.float64)
+    rays = _down_rays(x, y)
+    r2 = xd ** 2 + yd ** 2
+    # sphere program: the unit sphere inscribed in [-1, 1]^3; t = 10 - sqrt(1 - x^2 - y^2), (u, v) = ((x+1)/2, (y+1)/2)
+    t = _proc_tlas([[-1, -1, -1, 1, 1, 1]])
+    rec = S.hit_records([S.hit_group(0, S.INTERSECTION_SPHERE)], [(1, 0, 0)])
+    tuv, ids, st = O.trace(t, rays, records=rec)
+    inside, outside = r2 < 1 - 1e-3, r2 > 1 + 1e-3
+    assert np.all(st[inside] == 1) and np.all(st[outside] == 0)
+    assert np.abs(tuv[inside, 0] - (10 - np.sqrt(1 - r2[inside]))).max() < 1e-12
+    assert np.abs(tuv[inside, 1] - (xd[inside] + 1) / 2).max() < 1e-12
+    assert np.abs(tuv[inside, 2] - (yd[inside] + 1) / 2).max() < 1e-12
+    # grazing rays (|r| = 1) are ties, never robust
+    g = _down_rays(np.float32([1.0, 0.0]), np.float32([0.0, -1.0]))
+    assert np.all(O.trace(t, g, records=rec)[2] == 2)
+    # from the centre the sphere is hit where the ray leaves it (t = 1), not at the entry behind the origin
+    tuv, _, st = O.trace(t, _down_rays(np.float32([0.2]), np.float32([0.1]), z0=0.0), records=rec)
+    x0, y0 = float(np.float32(0.2)), float(np.float32(0.1))
+    assert st[0] == 1 and abs(tuv[0, 0] - np.sqrt(1 - x0 * x0 - y0 * y0)) < 1e-12
+    # box program (the default without records): box [0, 2] x [0, 1] x [-1, 0], top face z = 0 at t = 10,
+    # (u, v) along x, y (cyclic after z); from inside, the bottom face
+    t = _proc_tlas([[0, 0, -1, 2, 1, 0]])
+    xb, yb = rng.uniform(0.01, 1.99, 500).astype(np.float32), rng.uniform(0.01, 0.99, 500).astype(np.float32)
+    tuv, ids, st = O.trace(t, _down_rays(xb, yb))
+    assert np.all(st == 1) and np.all(tuv[:, 0] == 10.0)
+    assert np.abs(tuv[:, 1] - xb / 2.0).max() < 1e-12 and np.abs(tuv[:, 2] - yb.astype(np.float64)).max() < 1e-12
+    tuv, _, st = O.trace(t, _down_rays(np.float32([0.5]), np.float32([0.5]), z0=-0.25))
+    assert st[0] == 1 and tuv[0, 0] == 0.75
+    assert np.all(O.trace(t, _down_rays(np.float32([2.5]), np.float32([0.5])))[2] == 0)
+    # an edge of the hit face is a tie; SKIP_AABBS removes the box, SKIP_TRIANGLES does not
+    assert O.trace(t, _down_rays(np.float32([0.0]), np.float32([0.5])))[2][0] == 2
+    assert np.all(O.trace(t, _down_rays(xb, yb), ray_flags=0x200)[2] == 0)
+    assert np.all(O.trace(t, _down_rays(xb, yb), ray_flags=0x100)[2] == 1)
+    # inactive boxes (min > max, NaN) are not primitives
+    t = _proc_tlas([[0, 0, -1, -2, 1, 0], [np.nan, 0, 0, 1, 1, 1]])
+    assert np.all(O.trace(t, _down_rays(xb, yb))[2] == 0)
+
+
 def test_watertight_fan_and_ties():
     # 8-triangle fan around the origin: interior rays hit exactly one triangle, rays through the shared
     # vertex or along shared edges are epsilon-ties, never misses.
