@@ -722,18 +722,22 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                             const float4 blo = *(const float4*)bh->lo;
                             const float4 bhi = *(const float4*)bh->hi;
                             const float4 bqs = *(const float4*)bh->qscale;
-                            const double lo3[3] = {blo.x, blo.y, blo.z}, hi3[3] = {bhi.x, bhi.y, bhi.z};
-                            double tn = -INFINITY;
+                            // entry distance of the padded BLAS box, in FP32: it only chooses how far the origin
+                            // moves (hits between tmin and te stay in the window, whose lower end becomes
+                            // tmin - te), so its rounding costs nothing but keeps the FP64 divisions off this path
+                            const float lo3[3] = {blo.x, blo.y, blo.z}, hi3[3] = {bhi.x, bhi.y, bhi.z};
+                            float tn = -INFINITY;
 #pragma unroll
                             for (int a = 0; a < 3; ++a) {
-                                const double pad = (hi3[a] - lo3[a]) * (1.0 / 64.0) + 1e-30;
-                                if (dd[a] != 0.0) {
-                                    const double r = 1.0 / dd[a];
-                                    const double t1 = (lo3[a] - pad - od[a]) * r, t2 = (hi3[a] + pad - od[a]) * r;
-                                    tn = fmax(tn, fmin(t1, t2));
+                                const float pad = (hi3[a] - lo3[a]) * (1.0f / 64.0f) + 1e-30f;
+                                const float da = (float)dd[a], oa = (float)od[a];
+                                if (fabsf(da) > 1e-30f) {
+                                    const float r = __builtin_amdgcn_rcpf(da);
+                                    const float t1 = (lo3[a] - pad - oa) * r, t2 = (hi3[a] + pad - oa) * r;
+                                    tn = fmaxf(tn, fminf(t1, t2));
                                 }
                             }
-                            const double te = fmax((double)wtmin, tn);
+                            const double te = fmax((double)wtmin, (double)tn);
                             s.ox = (float)(od[0] + te * dd[0]);
                             s.oy = (float)(od[1] + te * dd[1]);
                             s.oz = (float)(od[2] + te * dd[2]);
