@@ -282,6 +282,25 @@ __device__ void refine_hit(const double* inv, const float* wo, const float* wd, 
 // in [tmin, tmax] of the ray o + t d with the program's surface, its (u, v) and the outward normal n (object
 // space, unnormalised for boxes).  T is float for the traversal's candidate test (in the current rebased space)
 // and double for the reported hit.
+// FP32 world-space unit geometric normal of a triangle (the path tracer's shading normal; the instance inverse is
+// rounded to FP32).
+__device__ __forceinline__ void world_normal_f32(const double* inv, const TriSlot* tri, float* n) {
+    float e1[3], e2[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        e1[a] = tri->v[3 + a] - tri->v[a];
+        e2[a] = tri->v[6 + a] - tri->v[a];
+    }
+    const float no[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    float nw[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nw[c] = (float)inv[c] * no[0] + (float)inv[4 + c] * no[1] + (float)inv[8 + c] * no[2];
+    const float l = sqrtf(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
+    const float r = l > 0.0f ? 1.0f / l : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n[c] = nw[c] * r;
+}
+
 template <typename T>
 __device__ __forceinline__ bool procedural_hit(uint32_t prog, const T* lo, const T* hi, const T* o, const T* d,
                                                T tmin, T tmax, T& t_out, T& u_out, T& v_out, T* n) {
@@ -402,26 +421,6 @@ __device__ bool refine_procedural(const double* inv, const float* wo, const floa
         for (int c = 0; c < 3; ++c) nw[c] = (float)(w[c] * rl);
     }
     return true;
-}
-
-// World-space geometric normal (v1 - v0) x (v2 - v0) of a triangle under the instance transform whose inverse is
-// inv: n_world = inv^T n_object (FP64, normalised, rounded to FP32).
-__device__ void world_normal(const double* inv, const TriSlot* tri, float* n) {
-    double e1[3], e2[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        e1[a] = (double)tri->v[3 + a] - (double)tri->v[a];
-        e2[a] = (double)tri->v[6 + a] - (double)tri->v[a];
-    }
-    double no[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-    double nw[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) nw[c] = inv[c] * no[0] + inv[4 + c] * no[1] + inv[8 + c] * no[2];
-    double l = sqrt(nw[0] * nw[0] + nw[1] * nw[1] + nw[2] * nw[2]);
-    double r = l > 0.0 ? 1.0 / l : 0.0;
-    n[0] = (float)(nw[0] * r);
-    n[1] = (float)(nw[1] * r);
-    n[2] = (float)(nw[2] * r);
 }
 
 // The any-hit program of a non-opaque candidate (its hit record: instance SBT offset + geometry index x stride +
@@ -890,7 +889,18 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                                 wd[k] = lds_ray[(3 + k) * blockDim.x];
                             }
                         }
-                        if (FLAGS && (tri->bits & kTriAabb)) {
+                        if (MODE == kModePath || MODE == kModeFused) {
+                            // the path tracer shades with t and the normal only: the traversal's FP32 t (u, v
+                            // are not needed); procedural normals below
+                            t = tc;
+                            proc = FLAGS && (tri->bits & kTriAabb);
+                            if (proc && !refine_procedural(M, wo, wd, tri, intersection_program(args, info, tri->geom),
+                                                           wtmin, t, u, v, pn)) {
+                                const float l = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
+#pragma unroll
+                                for (int k = 0; k < 3; ++k) pn[k] = l > 0.0f ? -wd[k] / l : 0.0f;
+                            }
+                        } else if (FLAGS && (tri->bits & kTriAabb)) {
                             // keep the traversal's FP32 t if FP64 finds no intersection (the normal then faces
                             // the ray)
                             proc = true;
@@ -929,7 +939,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                                 ph.n[1] = pn[1];
                                 ph.n[2] = pn[2];
                             } else {
-                                world_normal(M, tri, ph.n);
+                                world_normal_f32(M, tri, ph.n);
                             }
                             ph.rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
                         }
