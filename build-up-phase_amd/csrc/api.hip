@@ -960,6 +960,7 @@ int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc)
     rtx::NrcState& st = n->st;
     st.batch_size = cfg->batch_size;
     st.max_blocks = cfg->batch_size / 64;
+    st.cu_count = (uint32_t)ctx->cu_count;
     st.lr = cfg->learning_rate;
     st.beta1 = cfg->beta1;
     st.beta2 = cfg->beta2;

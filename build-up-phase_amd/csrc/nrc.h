@@ -17,6 +17,7 @@ struct NrcState {
     float* loss = nullptr;      // mean loss of the last step
     float* loss_part = nullptr; // per-block losses of one step
     uint32_t max_blocks = 0;
+    uint32_t cu_count = 256;    // compute units of the device (persistent inference grid)
     uint32_t batch_size = 0;
     uint32_t steps = 0;
     uint32_t cache_vertex = 1;

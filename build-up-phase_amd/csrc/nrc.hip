@@ -52,28 +52,6 @@ struct Norm {
     float inv[3];
 };
 
-// Input feature f of a query (f < 96): f < 72 -> axis f/24, frequency k = (f%24)/2, sin (even) / cos (odd) of
-// 2^k pi p; 72..74 normal, 75..77 direction, 78..80 albedo, 81..95 zero.
-__device__ __forceinline__ float feature(int f, const float* pn, const rtx_nrc_query& q) {
-    if (f < 72) {
-        const int a = f / 24, r = f % 24, k = r >> 1;
-        const float p = a == 0 ? pn[0] : (a == 1 ? pn[1] : pn[2]);
-        const float x = ldexpf(p, k);
-        return (r & 1) ? cospif(x) : sinpif(x);
-    }
-    float v = 0.0f;
-    v = f == 72 ? q.normal[0] : v;
-    v = f == 73 ? q.normal[1] : v;
-    v = f == 74 ? q.normal[2] : v;
-    v = f == 75 ? q.direction[0] : v;
-    v = f == 76 ? q.direction[1] : v;
-    v = f == 77 ? q.direction[2] : v;
-    v = f == 78 ? q.albedo[0] : v;
-    v = f == 79 ? q.albedo[1] : v;
-    v = f == 80 ? q.albedo[2] : v;
-    return v;
-}
-
 __device__ __forceinline__ void load_query(const rtx_nrc_query* src, rtx_nrc_query& q) {
     const float4* p = (const float4*)src;
     const float4 a = p[0], b = p[1], c = p[2];
@@ -83,21 +61,36 @@ __device__ __forceinline__ void load_query(const rtx_nrc_query* src, rtx_nrc_que
     q.albedo[0] = c.y; q.albedo[1] = c.z; q.albedo[2] = c.w;
 }
 
-// B fragments of the input layer for the lane's sample: k = 32s + 8g + j.
+// B fragments of the input layer for the lane's sample: k = 32s + 8g + j, i.e. the lane fills the 8-feature chunk
+// c = 4s + g at step s.  Chunks 0..8 are frequency chunks (axis c / 3, frequencies 4(c % 3) .. 4(c % 3) + 3, sin
+// and cos interleaved), so each lane evaluates 4 sin/cos pairs per step with one range reduction each; chunk 9
+// holds normal, direction and albedo[0..1], chunk 10 albedo[2], chunk 11 zeros.
 __device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, int g, bool valid, bf16x8* in0,
                                        __bf16* at0, int col) {
     float pn[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) pn[a] = (q.position[a] - nm.lo[a]) * nm.inv[a];
 #pragma unroll
-    for (int s = 0; s < 3; ++s)
+    for (int s = 0; s < 3; ++s) {
+        const int c = 4 * s + g;
+        float v[8];
+        if (c < 9) {
+            const int a = c / 3, k0 = 4 * (c - 3 * a);
+            const float p = a == 0 ? pn[0] : (a == 1 ? pn[1] : pn[2]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sincospif(ldexpf(p, k0 + i), &v[2 * i], &v[2 * i + 1]);
+        } else {
+            const float c9[8] = {q.normal[0],    q.normal[1],    q.normal[2], q.direction[0],
+                                 q.direction[1], q.direction[2], q.albedo[0], q.albedo[1]};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = c == 9 ? c9[j] : (c == 10 && j == 0 ? q.albedo[2] : 0.0f);
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int f = 32 * s + 8 * g + j;
-            const float v = valid ? feature(f, pn, q) : 0.0f;
-            in0[s][j] = (__bf16)v;
-            if (at0) at0[f * 64 + col] = in0[s][j];
+            in0[s][j] = (__bf16)(valid ? v[j] : 0.0f);
+            if (at0) at0[(8 * c + j) * 64 + col] = in0[s][j];
         }
+    }
 }
 
 // One forward layer: acc[t] = W_l (tile t) * in + bias.
@@ -133,47 +126,93 @@ __device__ __forceinline__ void store_act(const bf16x8* in, __bf16* at, int g, i
         for (int j = 0; j < 8; ++j) at[kperm(s, g, j) * 64 + col] = in[s][j];
 }
 
+// Inference, LDS-staged: a persistent block copies the 46 forward fragments (46 KB) into LDS once, then each wave
+// runs two 16-query tiles per pass so that every fragment read from LDS feeds two MFMAs (weights are the A operand,
+// queries the N dimension).  Each tile sees exactly the operations of the training kernel's forward pass.
+constexpr int kInferLdsFrags = kFwdFrags * 64;  // bf16x8 entries
+
 template <int L>
-__device__ __forceinline__ void forward_all(const bf16x8* fwd, const float* params, const bf16x8* in0,
-                                            bf16x8 (*acts)[2], f32x4& out, int lane) {
-    f32x4 acc[4];
-    layer_forward<0>(fwd, params, in0, acc, lane);
-    relu_pack(acc, acts[0]);
-    layer_forward<1>(fwd, params, acts[0], acc, lane);
-    relu_pack(acc, acts[1]);
-    layer_forward<2>(fwd, params, acts[1], acc, lane);
-    relu_pack(acc, acts[2]);
-    layer_forward<3>(fwd, params, acts[2], acc, lane);
-    relu_pack(acc, acts[3]);
-    layer_forward<4>(fwd, params, acts[3], acc, lane);
-    relu_pack(acc, acts[4]);
-    f32x4 o[1];
-    layer_forward<5>(fwd, params, acts[4], o, lane);
-    out = o[0];
+__device__ __forceinline__ void layer_forward2(const bf16x8* __restrict__ frag, const float* __restrict__ params,
+                                               const bf16x8* inA, const bf16x8* inB, f32x4* accA, f32x4* accB,
+                                               int lane) {
+    constexpr int T = out_dim(L) / 16, S = in_dim(L) / 32;
+    const int g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        f32x4 cA = {0.0f, 0.0f, 0.0f, 0.0f}, cB = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const bf16x8 a = frag[(fwd_base(L) + t * S + s) * 64 + lane];
+            cA = mfma(a, inA[s], cA);
+            cB = mfma(a, inB[s], cB);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float b = params[b_off(L) + 16 * t + 4 * g + i];
+            cA[i] += b;
+            cB[i] += b;
+        }
+        accA[t] = cA;
+        accB[t] = cB;
+    }
 }
 
-__global__ __launch_bounds__(256) void nrc_infer_kernel(const bf16x8* __restrict__ fwd,
-                                                        const float* __restrict__ params,
-                                                        const rtx_nrc_query* __restrict__ queries, uint32_t n,
-                                                        const uint32_t* __restrict__ n_dev, Norm nm,
-                                                        float* __restrict__ out_rgb) {
+__global__ __launch_bounds__(256) void nrc_infer_lds_kernel(const bf16x8* __restrict__ fwd,
+                                                            const float* __restrict__ params,
+                                                            const rtx_nrc_query* __restrict__ queries, uint32_t n,
+                                                            const uint32_t* __restrict__ n_dev, Norm nm,
+                                                            float* __restrict__ out_rgb) {
+    __shared__ bf16x8 frag[kInferLdsFrags];
+    for (int i = threadIdx.x; i < kInferLdsFrags; i += blockDim.x) frag[i] = fwd[i];
+    __syncthreads();
     if (n_dev) n = min(n, *n_dev);
     const int lane = threadIdx.x & 63;
     const int c = lane & 15, g = lane >> 4;
-    const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
-    const uint64_t idx = wave * 16 + c;
-    const bool valid = idx < n;
-    rtx_nrc_query q{};
-    if (valid) load_query(queries + idx, q);
-    bf16x8 in0[3];
-    encode(q, nm, g, valid, in0, nullptr, 0);
-    bf16x8 acts[5][2];
-    f32x4 out;
-    forward_all<0>(fwd, params, in0, acts, out, lane);
-    if (valid && g == 0) {
-        out_rgb[3 * idx] = fmaxf(0.0f, out[0]);
-        out_rgb[3 * idx + 1] = fmaxf(0.0f, out[1]);
-        out_rgb[3 * idx + 2] = fmaxf(0.0f, out[2]);
+    const uint64_t pairs = ((uint64_t)n + 31) / 32;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t p = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < pairs; p += waves) {
+        // an offset the compiler cannot see through, so the fragment reads stay in the loop (hoisted, the 46
+        // fragments would take 184 VGPRs and the kernel would run at one wave per SIMD)
+        int lane_f = lane;
+        asm volatile("" : "+v"(lane_f));
+        const uint64_t iA = p * 32 + c, iB = iA + 16;
+        const bool vA = iA < n, vB = iB < n;
+        rtx_nrc_query qA{}, qB{};
+        if (vA) load_query(queries + iA, qA);
+        if (vB) load_query(queries + iB, qB);
+        bf16x8 inA[3], inB[3];
+        encode(qA, nm, g, vA, inA, nullptr, 0);
+        encode(qB, nm, g, vB, inB, nullptr, 0);
+        bf16x8 aA[2], aB[2];
+        f32x4 accA[4], accB[4];
+        layer_forward2<0>(frag, params, inA, inB, accA, accB, lane_f);
+        relu_pack(accA, aA);
+        relu_pack(accB, aB);
+        layer_forward2<1>(frag, params, aA, aB, accA, accB, lane_f);
+        relu_pack(accA, aA);
+        relu_pack(accB, aB);
+        layer_forward2<2>(frag, params, aA, aB, accA, accB, lane_f);
+        relu_pack(accA, aA);
+        relu_pack(accB, aB);
+        layer_forward2<3>(frag, params, aA, aB, accA, accB, lane_f);
+        relu_pack(accA, aA);
+        relu_pack(accB, aB);
+        layer_forward2<4>(frag, params, aA, aB, accA, accB, lane_f);
+        relu_pack(accA, aA);
+        relu_pack(accB, aB);
+        layer_forward2<5>(frag, params, aA, aB, accA, accB, lane_f);
+        if (g == 0) {
+            if (vA) {
+                out_rgb[3 * iA] = fmaxf(0.0f, accA[0][0]);
+                out_rgb[3 * iA + 1] = fmaxf(0.0f, accA[0][1]);
+                out_rgb[3 * iA + 2] = fmaxf(0.0f, accA[0][2]);
+            }
+            if (vB) {
+                out_rgb[3 * iB] = fmaxf(0.0f, accB[0][0]);
+                out_rgb[3 * iB + 1] = fmaxf(0.0f, accB[0][1]);
+                out_rgb[3 * iB + 2] = fmaxf(0.0f, accB[0][2]);
+            }
+        }
     }
 }
 
@@ -473,9 +512,12 @@ static Norm make_norm(const NrcState& st) {
 hipError_t nrc_infer(const NrcState& st, const rtx_nrc_query* q, uint32_t n, float* out, hipStream_t s,
                      const uint32_t* n_dev) {
     if (n == 0) return hipSuccess;
-    const uint64_t waves = (n + 15) / 16;
-    const unsigned blocks = (unsigned)((waves + 3) / 4);
-    nrc_infer_kernel<<<blocks, 256, 0, s>>>((const bf16x8*)st.fwd, st.params, q, n, n_dev, make_norm(st), out);
+    // persistent LDS-staged kernel: up to 3 blocks per CU (46 KB of fragments each), each wave looping over pairs
+    // of 16-query tiles
+    const uint64_t pairs = ((uint64_t)n + 31) / 32;
+    const uint64_t want = (pairs + 3) / 4;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(want, (uint64_t)st.cu_count * 3);
+    nrc_infer_lds_kernel<<<blocks, 256, 0, s>>>((const bf16x8*)st.fwd, st.params, q, n, n_dev, make_norm(st), out);
     return hipGetLastError();
 }
 
