@@ -58,9 +58,12 @@ __device__ __forceinline__ uint32_t oct_permute(uint32_t m, uint32_t oct) {
     return m;
 }
 
+// 1/d for the slab tests, |d| clamped to at least 2^-80.  v_rcp_f32 (1 ulp) on d * 2^-32 keeps the operand and
+// the result in the normal range for every |d| in [2^-80, 2^128); the node test's error margin (2^-20 relative
+// per term) covers the approximate reciprocal.
 __device__ __forceinline__ float safe_rcp(float d) {
     float a = fabsf(d) < kTinyDir ? copysignf(kTinyDir, d) : d;
-    return 1.0f / a;
+    return __builtin_amdgcn_rcpf(a * 0x1p-32f) * 0x1p-32f;
 }
 
 __device__ __forceinline__ float byte_f(uint32_t w, int b) {
@@ -487,7 +490,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
 
     uint64_t st_nodes = 0, st_tris = 0, st_insts = 0, st_rays = 0;
 
-    bool active = false, exhausted = false;
+    bool active = false, exhausted = false, finished = false;
     uint32_t ray_id = 0;
     float wo[3], wd[3], wtmin = 0.0f, tc = 0.0f;
     Space s;
@@ -513,6 +516,124 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                            : ((uint64_t)nrays >= (uint64_t)waves_total * 64u * 8u ? 256u : 64u);
     uint32_t pool_next = 0, pool_end = 0;
     for (;;) {
+        if (finished) {
+            // write the result of ray_id (lanes whose rays ended since the last refill, in one pass)
+            if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
+                args.occluded[MODE == kModeFused ? ray_id - nA : ray_id] = occl ? 1u : 0u;
+            } else {
+                float t = 0.0f, u = 0.0f, v = 0.0f;
+                uint32_t prim = 0, geom = 0;
+                const InstanceInfo* info = nullptr;
+                bool proc = false;  // an AABB primitive's hit: normal from its intersection program
+                float pn[3] = {0.0f, 0.0f, 0.0f};
+                if (hit_inst != kNoHit) {
+                    info = table + hit_inst;
+                    double M[12];
+#pragma unroll
+                    for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
+                    const TriSlot* tri =
+                        (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes + (uint64_t)hit_slot * kSlotBytes);
+                    if (kRayLds) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            wo[k] = lds_ray[k * blockDim.x];
+                            wd[k] = lds_ray[(3 + k) * blockDim.x];
+                        }
+                    }
+                    if (MODE == kModePath || MODE == kModeFused) {
+                        // the path tracer shades with t and the normal only: the traversal's FP32 t (u, v
+                        // are not needed); procedural normals below
+                        t = tc;
+                        proc = FLAGS && (tri->bits & kTriAabb);
+                        if (proc && !refine_procedural(M, wo, wd, tri, intersection_program(args, info, tri->geom),
+                                                       wtmin, t, u, v, pn)) {
+                            const float l = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) pn[k] = l > 0.0f ? -wd[k] / l : 0.0f;
+                        }
+                    } else if (FLAGS && (tri->bits & kTriAabb)) {
+                        // keep the traversal's FP32 t if FP64 finds no intersection (the normal then faces
+                        // the ray)
+                        proc = true;
+                        t = tc;
+                        if (!refine_procedural(M, wo, wd, tri, intersection_program(args, info, tri->geom),
+                                               wtmin, t, u, v,
+                                               (MODE == kModePath || MODE == kModeFused) ? pn : nullptr)) {
+                            const float l = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) pn[k] = l > 0.0f ? -wd[k] / l : 0.0f;
+                        }
+                    } else {
+                        refine_hit(M, wo, wd, tri, t, u, v);
+                    }
+                    prim = tri->prim;
+                    geom = tri->geom;
+                }
+                if (MODE == kModePath || MODE == kModeFused) {
+                    PathHit ph;
+                    ph.t = t;
+                    ph.u = u;
+                    ph.v = v;
+                    ph.inst = hit_inst;
+                    ph.prim = prim;
+                    ph.geom = geom;
+                    ph.n[0] = ph.n[1] = ph.n[2] = 0.0f;
+                    ph.rec = 0xFFFFFFFFu;
+                    if (hit_inst != kNoHit) {
+                        double M[12];
+#pragma unroll
+                        for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
+                        const TriSlot* tri = (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes +
+                                                              (uint64_t)hit_slot * kSlotBytes);
+                        if (FLAGS && proc) {
+                            ph.n[0] = pn[0];
+                            ph.n[1] = pn[1];
+                            ph.n[2] = pn[2];
+                        } else {
+                            world_normal_f32(M, tri, ph.n);
+                        }
+                        ph.rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
+                    }
+                    args.path_hits[ray_id] = ph;
+                } else if (MODE == kModeClosest) {
+                    uint32_t* hp = (uint32_t*)(args.hits + ray_id);
+                    hp[0] = __float_as_uint(t);
+                    hp[1] = __float_as_uint(u);
+                    hp[2] = __float_as_uint(v);
+                    hp[3] = prim;
+                    hp[4] = geom;
+                    hp[5] = hit_inst;
+                } else {
+                    float r = args.miss[0], g = args.miss[1], bl = args.miss[2];
+                    if (hit_inst != kNoHit && (ray_flags & kRaySkipClosestHit)) {
+                        r = g = bl = 0.0f;  // the closest-hit program does not run: the payload keeps its 0
+                    } else if (hit_inst != kNoHit) {
+                        const uint32_t rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
+                        if (rec < args.nrec) {
+                            const rtx_hit_record hr = args.records[rec];
+                            bool bary = hr.program == RTX_PROGRAM_BARYCENTRIC;
+                            if (hr.program == RTX_PROGRAM_MATCH_BARYCENTRIC)
+                                bary = (int32_t)prim == hr.match_primitive && (int32_t)hit_inst == hr.match_instance &&
+                                       (int32_t)info->custom == hr.match_custom_index &&
+                                       (int32_t)geom == hr.match_geometry;
+                            if (bary) {
+                                r = 1.0f - u - v;
+                                g = u;
+                                bl = v;
+                            } else {
+                                r = hr.color[0];
+                                g = hr.color[1];
+                                bl = hr.color[2];
+                            }
+                        } else {
+                            r = g = bl = 0.0f;
+                        }
+                    }
+                    *(float4*)(args.film + 4 * (uint64_t)ray_id) = make_float4(r, g, bl, 0.0f);
+                }
+            }
+            finished = false;
+        }
         const uint64_t need = __ballot(!active && !exhausted);
         if (need) {
             const uint32_t want = (uint32_t)__popcll(need);
@@ -866,121 +987,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
             }
 
             if (done) {
-                // write the result of ray_id
-                if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
-                    args.occluded[MODE == kModeFused ? ray_id - nA : ray_id] = occl ? 1u : 0u;
-                } else {
-                    float t = 0.0f, u = 0.0f, v = 0.0f;
-                    uint32_t prim = 0, geom = 0;
-                    const InstanceInfo* info = nullptr;
-                    bool proc = false;  // an AABB primitive's hit: normal from its intersection program
-                    float pn[3] = {0.0f, 0.0f, 0.0f};
-                    if (hit_inst != kNoHit) {
-                        info = table + hit_inst;
-                        double M[12];
-#pragma unroll
-                        for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
-                        const TriSlot* tri =
-                            (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes + (uint64_t)hit_slot * kSlotBytes);
-                        if (kRayLds) {
-#pragma unroll
-                            for (int k = 0; k < 3; ++k) {
-                                wo[k] = lds_ray[k * blockDim.x];
-                                wd[k] = lds_ray[(3 + k) * blockDim.x];
-                            }
-                        }
-                        if (MODE == kModePath || MODE == kModeFused) {
-                            // the path tracer shades with t and the normal only: the traversal's FP32 t (u, v
-                            // are not needed); procedural normals below
-                            t = tc;
-                            proc = FLAGS && (tri->bits & kTriAabb);
-                            if (proc && !refine_procedural(M, wo, wd, tri, intersection_program(args, info, tri->geom),
-                                                           wtmin, t, u, v, pn)) {
-                                const float l = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
-#pragma unroll
-                                for (int k = 0; k < 3; ++k) pn[k] = l > 0.0f ? -wd[k] / l : 0.0f;
-                            }
-                        } else if (FLAGS && (tri->bits & kTriAabb)) {
-                            // keep the traversal's FP32 t if FP64 finds no intersection (the normal then faces
-                            // the ray)
-                            proc = true;
-                            t = tc;
-                            if (!refine_procedural(M, wo, wd, tri, intersection_program(args, info, tri->geom),
-                                                   wtmin, t, u, v,
-                                                   (MODE == kModePath || MODE == kModeFused) ? pn : nullptr)) {
-                                const float l = sqrtf(wd[0] * wd[0] + wd[1] * wd[1] + wd[2] * wd[2]);
-#pragma unroll
-                                for (int k = 0; k < 3; ++k) pn[k] = l > 0.0f ? -wd[k] / l : 0.0f;
-                            }
-                        } else {
-                            refine_hit(M, wo, wd, tri, t, u, v);
-                        }
-                        prim = tri->prim;
-                        geom = tri->geom;
-                    }
-                    if (MODE == kModePath || MODE == kModeFused) {
-                        PathHit ph;
-                        ph.t = t;
-                        ph.u = u;
-                        ph.v = v;
-                        ph.inst = hit_inst;
-                        ph.prim = prim;
-                        ph.geom = geom;
-                        ph.n[0] = ph.n[1] = ph.n[2] = 0.0f;
-                        ph.rec = 0xFFFFFFFFu;
-                        if (hit_inst != kNoHit) {
-                            double M[12];
-#pragma unroll
-                            for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
-                            const TriSlot* tri = (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes +
-                                                                  (uint64_t)hit_slot * kSlotBytes);
-                            if (FLAGS && proc) {
-                                ph.n[0] = pn[0];
-                                ph.n[1] = pn[1];
-                                ph.n[2] = pn[2];
-                            } else {
-                                world_normal_f32(M, tri, ph.n);
-                            }
-                            ph.rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
-                        }
-                        args.path_hits[ray_id] = ph;
-                    } else if (MODE == kModeClosest) {
-                        uint32_t* hp = (uint32_t*)(args.hits + ray_id);
-                        hp[0] = __float_as_uint(t);
-                        hp[1] = __float_as_uint(u);
-                        hp[2] = __float_as_uint(v);
-                        hp[3] = prim;
-                        hp[4] = geom;
-                        hp[5] = hit_inst;
-                    } else {
-                        float r = args.miss[0], g = args.miss[1], bl = args.miss[2];
-                        if (hit_inst != kNoHit && (ray_flags & kRaySkipClosestHit)) {
-                            r = g = bl = 0.0f;  // the closest-hit program does not run: the payload keeps its 0
-                        } else if (hit_inst != kNoHit) {
-                            const uint32_t rec = info->sbt_offset + geom * args.sbt_stride + args.sbt_offset;
-                            if (rec < args.nrec) {
-                                const rtx_hit_record hr = args.records[rec];
-                                bool bary = hr.program == RTX_PROGRAM_BARYCENTRIC;
-                                if (hr.program == RTX_PROGRAM_MATCH_BARYCENTRIC)
-                                    bary = (int32_t)prim == hr.match_primitive && (int32_t)hit_inst == hr.match_instance &&
-                                           (int32_t)info->custom == hr.match_custom_index &&
-                                           (int32_t)geom == hr.match_geometry;
-                                if (bary) {
-                                    r = 1.0f - u - v;
-                                    g = u;
-                                    bl = v;
-                                } else {
-                                    r = hr.color[0];
-                                    g = hr.color[1];
-                                    bl = hr.color[2];
-                                }
-                            } else {
-                                r = g = bl = 0.0f;
-                            }
-                        }
-                        *(float4*)(args.film + 4 * (uint64_t)ray_id) = make_float4(r, g, bl, 0.0f);
-                    }
-                }
+                // the result is written at the next refill, together with the wave's other finished lanes
+                finished = true;
                 active = false;
                 break;
             }
