@@ -226,6 +226,65 @@ __device__ __forceinline__ float round_up_f(double x) {
     return f;
 }
 
+// World box of an instance: the union of its BLAS's second-level boxes (the root's leaf children, and the
+// children of its internal children), each decoded from the 8-bit planes with the FP32 operations the traversal
+// uses (so it encloses what they enclose) and transformed corner by corner in FP64.  For a rotated instance this
+// is much tighter than the transformed root box, and every ray that misses it skips the instance's FP64 entry.
+// A BLAS whose nodes do not check out falls back to the root box.
+__device__ void add_world_box(const double* m, const float* blo, const float* bhi, double* lo, double* hi) {
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(double)((c & 1) ? bhi[0] : blo[0]), (double)((c & 2) ? bhi[1] : blo[1]),
+                             (double)((c & 4) ? bhi[2] : blo[2])};
+        for (int a = 0; a < 3; ++a) {
+            const double w = m[4 * a] * p[0] + m[4 * a + 1] * p[1] + m[4 * a + 2] * p[2] + m[4 * a + 3];
+            lo[a] = fmin(lo[a], w);
+            hi[a] = fmax(hi[a], w);
+        }
+    }
+}
+
+__device__ void decode_child_box(const Node8* nd, int k, const float* rlo, const float* qs, float* blo, float* bhi) {
+    for (int a = 0; a < 3; ++a) {
+        const float p = fmaf((float)nd->pq[a], qs[a], rlo[a]);
+        const float sc = __uint_as_float((uint32_t)nd->e[a] << 23);
+        blo[a] = fmaf((float)nd->qlo[a][k], sc, p);
+        bhi[a] = fmaf((float)nd->qhi[a][k], sc, p);
+    }
+}
+
+__device__ void instance_world_box(const AsHeader* h, const double* m, double* lo, double* hi) {
+    const uint8_t* slots = (const uint8_t*)h + kHeaderBytes;
+    const Node8* root = (const Node8*)slots;
+    const float rlo[3] = {h->lo[0], h->lo[1], h->lo[2]}, qs[3] = {h->qscale[0], h->qscale[1], h->qscale[2]};
+    const uint32_t nslots = h->slot_count;
+    bool ok = root->vmask != 0 && root->base < nslots;
+    for (int k = 0; k < 8 && ok; ++k) {
+        if (!((root->vmask >> k) & 1u) || !((root->imask >> k) & 1u)) continue;
+        const uint32_t cs = root->base + (uint32_t)__popc(root->vmask & ((1u << k) - 1u));
+        ok = cs < nslots && ((const Node8*)(slots + (uint64_t)cs * kSlotBytes))->vmask != 0;
+    }
+    if (!ok) {
+        add_world_box(m, h->lo, h->hi, lo, hi);
+        return;
+    }
+    for (int k = 0; k < 8; ++k) {
+        if (!((root->vmask >> k) & 1u)) continue;
+        float blo[3], bhi[3];
+        if ((root->imask >> k) & 1u) {
+            const uint32_t cs = root->base + (uint32_t)__popc(root->vmask & ((1u << k) - 1u));
+            const Node8* c = (const Node8*)(slots + (uint64_t)cs * kSlotBytes);
+            for (int j = 0; j < 8; ++j) {
+                if (!((c->vmask >> j) & 1u)) continue;
+                decode_child_box(c, j, rlo, qs, blo, bhi);
+                add_world_box(m, blo, bhi, lo, hi);
+            }
+        } else {
+            decode_child_box(root, k, rlo, qs, blo, bhi);
+            add_world_box(m, blo, bhi, lo, hi);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void prep_instances_kernel(const rtx_instance* __restrict__ inst, uint32_t n,
                                                              InstSlot* __restrict__ leaves,
                                                              InstanceInfo* __restrict__ table,
@@ -262,15 +321,7 @@ __global__ __launch_bounds__(256) void prep_instances_kernel(const rtx_instance*
             ok = h->magic == kMagicBlas && h->prim_count > 0 && h->slot_count > 0;
             if (ok) {
                 double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-                for (int c = 0; c < 8; ++c) {
-                    double p[3] = {(double)((c & 1) ? h->hi[0] : h->lo[0]), (double)((c & 2) ? h->hi[1] : h->lo[1]),
-                                   (double)((c & 4) ? h->hi[2] : h->lo[2])};
-                    for (int a = 0; a < 3; ++a) {
-                        double w = m[4 * a] * p[0] + m[4 * a + 1] * p[1] + m[4 * a + 2] * p[2] + m[4 * a + 3];
-                        lo[a] = fmin(lo[a], w);
-                        hi[a] = fmax(hi[a], w);
-                    }
-                }
+                instance_world_box(h, m, lo, hi);
                 double ext = 0;
                 for (int a = 0; a < 3; ++a) ext = fmax(ext, fmax(fabs(lo[a]), fabs(hi[a])) + (hi[a] - lo[a]));
                 double pad = ext * 0x1p-20;
