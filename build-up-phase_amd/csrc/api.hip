@@ -178,7 +178,7 @@ int side_chain(rtx_context ctx, const rtx::TraceLaunch& L, rtx::SideChain& side)
     rtx::TraceLaunch S = L;
     S.blocks_per_cu = kSideBlocks * 256u / L.block_size > 0 ? kSideBlocks * 256u / L.block_size : 1u;
     S.grid = (uint32_t)ctx->cu_count * S.blocks_per_cu;
-    S.w4 = true;
+    S.waves = 4;
     const uint64_t need = (uint64_t)S.grid * S.block_size * S.gdepth;
     if (need > ctx->side_gstack_entries) {
         if (ctx->side_gstack) (void)hipFree(ctx->side_gstack);
@@ -209,7 +209,8 @@ rtx::TraceLaunch launch_shape(rtx_context ctx) {
     L.refill_lanes = ctx->cfg.refill_lanes;
     L.refill_interval = ctx->cfg.refill_interval;
     L.leaf_lanes = ctx->cfg.leaf_lanes;
-    L.w4 = (L.block_size / 64u) * L.blocks_per_cu / 4u >= 4u;
+    const uint32_t waves = (L.block_size / 64u) * L.blocks_per_cu / 4u;  // per SIMD
+    L.waves = waves >= 5u ? 5u : (waves == 4u ? 4u : 0u);
     return L;
 }
 
@@ -1187,7 +1188,8 @@ static int autotune_core(rtx_context ctx, const char* who, uint32_t reps, Launch
         ms = el / (float)reps;
         return rtx_sync(ctx);
     };
-    static const uint32_t shapes[][2] = {{256, 2}, {256, 3}, {256, 4}, {128, 6}, {128, 8}, {64, 12}, {64, 16}};
+    static const uint32_t shapes[][2] = {{256, 2}, {256, 3}, {256, 4}, {128, 6},  {128, 8},
+                                         {64, 12}, {64, 16}, {256, 5}, {128, 10}, {64, 20}};
     static const uint32_t refills[][2] = {{40, 16}, {32, 8}, {40, 4}, {48, 4}, {48, 2}, {56, 2}, {56, 4}, {60, 1}};
     static const uint32_t leafs[] = {1, 4, 8, 16, 24, 32, 48};
     static const uint32_t chunks[] = {RTX_TRACE_RAY_CHUNK_AUTO, 64, 128, 256, 512};

@@ -46,7 +46,7 @@ struct TraceLaunch {
     uint32_t refill_lanes;     // refill a wave once fewer lanes than this are busy (1..64)
     uint32_t refill_interval;  // traversal steps between refill checks (1..64)
     uint32_t leaf_lanes;       // pending leaf groups in a wave before the leaf step runs (1..64)
-    bool w4;                   // the 128-VGPR kernel variant (4 waves per SIMD)
+    uint32_t waves;            // VGPR-bounded kernel variant: 0 (unbounded), 4 (128 VGPRs) or 5 (96 VGPRs) waves per SIMD
 };
 
 hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);

@@ -1007,14 +1007,19 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     }
 }
 
-// Two occupancy variants of the same traversal: unconstrained registers, and at most 128 VGPRs so that four
-// waves fit on a SIMD (chosen when the launch shape asks for four or more waves per SIMD).
+// Three occupancy variants of the same traversal: unconstrained registers, at most 128 VGPRs (four waves per
+// SIMD) and at most 96 VGPRs (five waves per SIMD), chosen by the waves per SIMD that the launch shape asks for.
 template <int MODE, bool STATS, bool FLAGS>
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs args) {
     trace_body<MODE, STATS, FLAGS>(args);
 }
 template <int MODE, bool STATS, bool FLAGS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel_w4(TraceArgs args) {
+    trace_body<MODE, STATS, FLAGS>(args);
+}
+
+template <int MODE, bool STATS, bool FLAGS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void trace_kernel_w5(TraceArgs args) {
     trace_body<MODE, STATS, FLAGS>(args);
 }
 
@@ -1036,7 +1041,9 @@ hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t 
     constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused;
     const size_t lds =
         (size_t)L.lds_depth * L.block_size * sizeof(uint2) + (kRayLds ? (size_t)L.block_size * 6 * sizeof(float) : 0);
-    if (L.w4)
+    if (L.waves >= 5)
+        trace_kernel_w5<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
+    else if (L.waves == 4)
         trace_kernel_w4<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
     else
         trace_kernel<MODE, STATS, FLAGS><<<L.grid, L.block_size, lds, s>>>(a);
