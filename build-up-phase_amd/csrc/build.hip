@@ -871,6 +871,76 @@ __global__ __launch_bounds__(64) void refit_refine_kernel(uint32_t n, int32_t* l
     }
 }
 
+// ---- costs of the 8-wide collapse ------------------------------------------------------------------------------
+//
+// The SAH-optimal collapse of the binary tree into 8-wide nodes (Ylitie, Karras & Laine 2017, section 4), for
+// this layout's leaves (one primitive per child slot).  wc[8x + j - 1] = C(x, j), the least cost of the subtree of
+// x spread over at most j child slots of its parent:
+//   leaf:      C(x, j) = c_leaf * A(x)
+//   internal:  D(x, j) = min over k in [1, j-1] of C(l, k) + C(r, j - k)        (x dissolved into j slots)
+//              C(x, 1) = A(x) + D(x, 8)                                         (x becomes a wide node)
+//              C(x, j) = min(C(x, j - 1), D(x, j)),  j >= 2
+// computed bottom-up (second arrival merges, as the refit).  The collapse then takes, for each wide node x, the
+// children that D(x, 8) picks.
+__device__ __forceinline__ void wide_table(uint32_t n, int32_t c, const float* nboxes, const float* wc, float c_leaf,
+                                           float* t) {
+    if (c < 0) {
+        float b[6];
+        const uint64_t bi = (uint64_t)n - 1 + (uint64_t)(-1 - c);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) b[k] = load_coherent(nboxes + 6 * bi + k);
+        const float a = c_leaf * box_area(b);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = a;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = load_coherent(wc + 8 * (uint64_t)c + j);
+    }
+}
+
+// D(x, j) from the children's tables: the least cost and the left child's slot count k.
+__device__ __forceinline__ float wide_split(const float* tl, const float* tr, int j, int& best_k) {
+    float best = INFINITY;
+    best_k = 1;
+    for (int k = 1; k < j; ++k) {
+        const float c = tl[k - 1] + tr[j - k - 1];
+        if (c < best) {
+            best = c;
+            best_k = k;
+        }
+    }
+    return best;
+}
+
+__global__ __launch_bounds__(256) void wide_cost_kernel(uint32_t n, const int32_t* __restrict__ left,
+                                                        const int32_t* __restrict__ right,
+                                                        const int32_t* __restrict__ parent, const float* nboxes,
+                                                        float* wc, uint32_t* arrivals, float c_leaf) {
+    const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= n) return;
+    int32_t node = load_coherent_i32(parent + (uint64_t)n - 1 + pos);
+    while (node >= 0) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const uint32_t old =
+            __hip_atomic_fetch_add(&arrivals[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0u) return;  // the sibling's thread merges
+        float tl[8], tr[8], b[6];
+        wide_table(n, load_coherent_i32(left + node), nboxes, wc, c_leaf, tl);
+        wide_table(n, load_coherent_i32(right + node), nboxes, wc, c_leaf, tr);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) b[k] = load_coherent(nboxes + 6 * (uint64_t)node + k);
+        float c[8];
+        int k;
+        c[0] = box_area(b) + wide_split(tl, tr, 8, k);
+        for (int j = 2; j <= 8; ++j) c[j - 1] = fminf(c[j - 2], wide_split(tl, tr, j, k));
+#pragma unroll
+        for (int j = 0; j < 8; ++j) store_coherent(wc + 8 * (uint64_t)node + j, c[j]);
+        node = load_coherent_i32(parent + node);
+    }
+}
+
 // Header setup after refit: root box and the 16-bit origin quantisation step.
 __global__ void header_kernel(AsHeader* h, uint32_t type, uint32_t flags, uint32_t n, uint32_t ngeom,
                               const float* nboxes, const float* cost, uint64_t capacity_slots, uint64_t table_offset,
@@ -1027,7 +1097,8 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                                                              uint2* __restrict__ tasks_out,
                                                              AsHeader* const* __restrict__ as_ptr,
                                                              const uint32_t* __restrict__ tblas_in,
-                                                             uint32_t* __restrict__ tblas_out) {
+                                                             uint32_t* __restrict__ tblas_out,
+                                                             const float* __restrict__ wc, float c_leaf) {
     const uint32_t count = level_counts[level];
     if (count == 0) return;
     for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < count; ti += gridDim.x * blockDim.x) {
@@ -1042,13 +1113,47 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
         if ((int32_t)task.x < 0) {
             ch[0] = (int32_t)task.x;
             cnt = 1;
+        } else if (wc) {
+            // the children D(task, 8) picks: a stack of (subtree, slots) pairs whose slots add up to 8
+            int32_t sn[8];
+            int si[8];
+            int sp = 0;
+            sn[sp] = (int32_t)task.x;
+            si[sp++] = 8;
+            bool root = true;  // the task node itself is always split
+            cnt = 0;
+            while (sp > 0) {
+                --sp;
+                const int32_t x = sn[sp];
+                const int j = si[sp];
+                if (x < 0 || (j == 1 && !root)) {
+                    ch[cnt++] = x;
+                    continue;
+                }
+                float tl[8], tr[8];
+                const int32_t l = left[x], r = right[x];
+                wide_table(n, l, nboxes, wc, c_leaf, tl);
+                wide_table(n, r, nboxes, wc, c_leaf, tr);
+                int k;
+                const float d = wide_split(tl, tr, j, k);
+                if (!root && !(d < wc[8 * (uint64_t)x + j - 2])) {
+                    sn[sp] = x;  // fewer slots cost no more
+                    si[sp++] = j - 1;
+                } else {
+                    sn[sp] = l;
+                    si[sp++] = k;
+                    sn[sp] = r;
+                    si[sp++] = j - k;
+                }
+                root = false;
+            }
         } else {
             ch[0] = left[task.x];
             ch[1] = right[task.x];
             cnt = 2;
         }
-        // open the internal child with the largest surface area until 8 children
-        while (cnt < 8) {
+        // greedy (no cost table): open the internal child with the largest surface area until 8 children
+        while (!wc && cnt < 8) {
             int best = -1;
             float best_a = -1.0f;
 #pragma unroll
@@ -1284,8 +1389,8 @@ __global__ void fill_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
 
 // ---- scratch layout --------------------------------------------------------------------------------------------
 struct Layout {
-    uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, cost, cnt, hgt, tasks0, tasks1,
-        levels, cbounds, misc, geoms, total;
+    uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, cost, cnt, hgt, wcost, tasks0,
+        tasks1, levels, cbounds, misc, geoms, total;
     // batched builds only
     uint64_t pblas = 0, nblas = 0, roots = 0, asptr = 0, binfo = 0, tblas0 = 0, tblas1 = 0, bcb = 0;
 };
@@ -1312,6 +1417,7 @@ Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom, uint32_t nbatch = 
     L.cost = take(nn * 4);
     L.cnt = take(nn * 4);
     L.hgt = take(nn * 4);
+    L.wcost = take(nn * 32);
     L.tasks0 = take(nn * 8);
     L.tasks1 = take(nn * 8);
     L.levels = take(128 * 4);
@@ -1347,9 +1453,15 @@ __global__ void set_update_offset_kernel(AsHeader* h, uint64_t off) {
     if (threadIdx.x == 0 && blockIdx.x == 0) h->update_offset = off;
 }
 
-// Refit with treelet refinement; each further round restructures the tree the previous one left.
+// Leaf cost of the wide collapse, relative to a node visit (cost 1): a triangle test (~80 VALU operations
+// against the node test's ~250) or an instance entry (FP64 ray transform and a BLAS root visit).
+constexpr float kLeafCostTri = 0.3f;
+constexpr float kLeafCostInstance = 1.0f;
+
+// Refit with treelet refinement; each further round restructures the tree the previous one left.  Then the costs
+// of the 8-wide collapse (wcost) over the final tree.
 void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L, const float* prim_boxes,
-                  hipStream_t s, const uint32_t* node_blas = nullptr) {
+                  hipStream_t s, const uint32_t* node_blas = nullptr, float c_leaf = kLeafCostTri) {
     const int rounds = treelet_rounds(flags, n);
     uint32_t* arrivals = (uint32_t*)(scratch + L.arrivals);
     for (int round = 0; round < (rounds ? rounds : 1); ++round) {
@@ -1359,6 +1471,13 @@ void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L,
             (const uint32_t*)(scratch + L.vals), prim_boxes, (float*)(scratch + L.nboxes), (float*)(scratch + L.cost),
             (uint32_t*)(scratch + L.cnt), (uint32_t*)(scratch + L.hgt), arrivals,
             rounds ? kTreeletMinPrims[round] : 0xFFFFFFFFu, node_blas);
+    }
+    if (n > 1) {
+        fill_kernel<<<256, 256, 0, s>>>(arrivals, n, 0u);
+        wide_cost_kernel<<<(n + 255) / 256, 256, 0, s>>>(
+            n, (const int32_t*)(scratch + L.left), (const int32_t*)(scratch + L.right),
+            (const int32_t*)(scratch + L.parent), (const float*)(scratch + L.nboxes), (float*)(scratch + L.wcost),
+            arrivals, c_leaf);
     }
 }
 
@@ -1398,7 +1517,7 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
         } else {
             fill_kernel<<<1, 64, 0, s>>>((uint32_t*)parent, 1, 0xFFFFFFFFu);
         }
-        refit_passes(n, flags, scratch, L, boxes, s);
+        refit_passes(n, flags, scratch, L, boxes, s, nullptr, type == 1 ? kLeafCostInstance : kLeafCostTri);
     }
     fill_kernel<<<1, 128, 0, s>>>(levels, 128, 0u);
     // root box: internal node 0, or the single leaf (stored at index n-1 == 0)
@@ -1410,7 +1529,9 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
             uint2* tout = (lv & 1) ? t0 : t1;
             collapse_level_kernel<false><<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
                                                                   (const uint4*)(scratch + L.leaves), h, slots, levels,
-                                                                  lv, tin, tout, nullptr, nullptr, nullptr);
+                                                                  lv, tin, tout, nullptr, nullptr, nullptr,
+                                                                  n > 1 ? (const float*)(scratch + L.wcost) : nullptr,
+                                                                  type == 1 ? kLeafCostInstance : kLeafCostTri);
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, max_levels, err_flags);
     }
@@ -1580,7 +1701,8 @@ hipError_t update_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, u
             collapse_level_kernel<false><<<kCollapseBlocks, 256, 0, s>>>(
                 n, (const int32_t*)(sc + L.left), (const int32_t*)(sc + L.right), (const uint32_t*)(sc + L.vals),
                 (const float*)(sc + L.nboxes), (const uint4*)(sc + L.leaves), h, (uint4*)((uint8_t*)dst + kHeaderBytes),
-                levels, lv, tin, tout, nullptr, nullptr, nullptr);
+                levels, lv, tin, tout, nullptr, nullptr, nullptr, n > 1 ? (const float*)(sc + L.wcost) : nullptr,
+                kLeafCostTri);
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, kMaxLevels32, err_flags);
     }
@@ -1665,7 +1787,9 @@ hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t ngeom, const ui
             uint32_t* bout = (uint32_t*)(sc + ((lv & 1) ? L.tblas0 : L.tblas1));
             collapse_level_kernel<true><<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
                                                                   (const uint4*)(sc + L.leaves), nullptr, nullptr,
-                                                                  levels, lv, tin, tout, as_ptr, bin, bout);
+                                                                  levels, lv, tin, tout, as_ptr, bin, bout,
+                                                                  n > 1 ? (const float*)(sc + L.wcost) : nullptr,
+                                                                  kLeafCostTri);
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(nullptr, levels, kMaxLevels32, err_flags);
     }
