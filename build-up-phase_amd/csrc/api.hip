@@ -61,7 +61,12 @@ struct rtx_nrc_t {
 namespace {
 
 constexpr uint32_t kMaxPrims = (1u << 30) - 1u;
-constexpr uint32_t kSideBlocks = 1u;  // 256-thread blocks per CU of the NRC training wavefront
+// NRC training wavefront: the render wavefront leaves kSideReserve 256-thread blocks per CU free for it, and its
+// grid has kSideBlocks 256-thread blocks per CU: the blocks beyond the reserve start as render blocks retire (the
+// traversal hands out rays from an atomic counter, so late blocks only take what is left), giving the training
+// chain's latency-bound launches more waves once the render wavefront is done.
+constexpr uint32_t kSideReserve = 1u;
+constexpr uint32_t kSideBlocks = 2u;
 
 // splitmix64 (Steele, Lea & Flood 2014): deterministic weight initialisation
 uint64_t splitmix64(uint64_t& s) {
@@ -926,9 +931,10 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     }
     rtx::SideChain side{};
     if ((r = side_chain(ctx, L, side))) return r;
-    // the render wavefront leaves `kSideBlocks` blocks per CU to the training wavefront
-    if (L.blocks_per_cu > side.launch.blocks_per_cu) {
-        L.blocks_per_cu -= side.launch.blocks_per_cu;
+    // the render wavefront leaves kSideReserve 256-thread blocks per CU to the training wavefront
+    const uint32_t reserve = kSideReserve * 256u / L.block_size > 0 ? kSideReserve * 256u / L.block_size : 1u;
+    if (L.blocks_per_cu > reserve) {
+        L.blocks_per_cu -= reserve;
         L.grid = (uint32_t)ctx->cu_count * L.blocks_per_cu;
     }
     uint32_t count = 0;
