@@ -53,6 +53,9 @@ def main():
                     help="skip the 20M-triangle ray-query part (profiling the path-tracing part alone)")
     ap.add_argument("--ref-spp", type=int, default=4096, help="samples of the FP64 reference render")
     ap.add_argument("--nrc-frames", type=int, default=160, help="NRC frames rendered (one sample each)")
+    ap.add_argument("--nrc-kernels", action="store_true",
+                    help="only the NRC network kernels on random inputs (inference of 2,073,600 queries, training "
+                         "steps of 16,384 samples; for profiling)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (reference kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -66,6 +69,12 @@ def main():
 
     import pyrtx
     from pyrtx import scenes as S
+
+    if args.nrc_kernels:
+        ctx = pyrtx.Context(0)
+        print(json.dumps({"nrc_kernels": nrc_kernel_bench(ctx, torch, pyrtx)}), flush=True)
+        ctx.close()
+        return
 
     if args.no_query:
         ctx = pyrtx.Context(0)
@@ -271,6 +280,26 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def nrc_kernel_bench(ctx, torch, pyrtx):
+    """The NRC network's kernels alone on random inputs: inference of one 1920x1080 frame of cache queries and
+    training on 65,536 samples (4 Adam steps of 16,384), three rounds; HIP-event times of the last round."""
+    nrc = pyrtx.Nrc(ctx, (0, 0, 0), (1, 1, 1), seed=1, batch_size=16384)
+    g = torch.Generator(device=ctx.device).manual_seed(1)
+    q = torch.rand((2073600, 12), device=ctx.device, generator=g)
+    smp = torch.rand((65536, 16), device=ctx.device, generator=g)
+    out = {}
+    for _ in range(3):
+        for name, fn in (("infer_2073600_ms", lambda: nrc.infer(q)), ("train_4x16384_ms", lambda: nrc.train(smp))):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            out[name] = round(a.elapsed_time(b), 4)
+    nrc.close()
+    return out
 
 
 def measured_traffic():
