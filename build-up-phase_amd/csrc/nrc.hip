@@ -43,6 +43,11 @@ static_assert(b_off(5) + 16 == (int)RTX_NRC_PARAM_COUNT, "parameter layout");
 
 __device__ __forceinline__ int kperm(int s, int g, int j) { return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4)); }
 
+// LDS rows of the training kernel: [row][64 samples] BF16, the 8 16-byte chunks of a row permuted by the row's
+// low 3 bits (chunk c stored at c ^ (row & 7)), so that the weight-gradient operand reads (16 rows x one chunk)
+// spread over the banks instead of all landing in the same 16 bytes of every 128-byte row.
+__device__ __forceinline__ int lds_idx(int row, int col) { return row * 64 + (col ^ ((row & 7) << 3)); }
+
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -88,7 +93,7 @@ __device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, i
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             in0[s][j] = (__bf16)(valid ? v[j] : 0.0f);
-            if (at0) at0[(8 * c + j) * 64 + col] = in0[s][j];
+            if (at0) at0[lds_idx(8 * c + j, col)] = in0[s][j];
         }
     }
 }
@@ -123,7 +128,7 @@ __device__ __forceinline__ void store_act(const bf16x8* in, __bf16* at, int g, i
 #pragma unroll
     for (int s = 0; s < 2; ++s)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) at[kperm(s, g, j) * 64 + col] = in[s][j];
+        for (int j = 0; j < 8; ++j) at[lds_idx(kperm(s, g, j), col)] = in[s][j];
 }
 
 // Inference, LDS-staged: a persistent block copies the 46 forward fragments (46 KB) into LDS once, then each wave
@@ -236,19 +241,26 @@ __device__ __forceinline__ void weight_grad(TrainLds& sm, const __bf16* at, floa
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const bf16x8 a = *(const bf16x8*)(sm.gt + (16 * t + r) * 64 + 32 * s + 8 * g);
-            const bf16x8 b = *(const bf16x8*)(at + (16 * u + r) * 64 + 32 * s + 8 * g);
+            const bf16x8 a = *(const bf16x8*)(sm.gt + lds_idx(16 * t + r, 32 * s + 8 * g));
+            const bf16x8 b = *(const bf16x8*)(at + lds_idx(16 * u + r, 32 * s + 8 * g));
             c = mfma(a, b, c);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) part[w_off(L) + (16 * t + 4 * g + i) * in_dim(L) + 16 * u + r] = c[i];
     }
-    // bias gradient: sum over the 64 samples of G^T rows
-    const int tid = threadIdx.x;
-    if (tid < out_dim(L)) {
+    // bias gradient: sum over the 64 samples of G^T rows; 4 lanes per row (16 samples each), then a shuffle sum
+    const int row = threadIdx.x >> 2, q = threadIdx.x & 3;
+    if (row < out_dim(L)) {
         float sum = 0.0f;
-        for (int b = 0; b < 64; ++b) sum += (float)sm.gt[tid * 64 + b];
-        part[b_off(L) + tid] = sum;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bf16x8 v = *(const bf16x8*)(sm.gt + lds_idx(row, 16 * q + 8 * h));
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sum += (float)v[j];
+        }
+        sum += __shfl_xor(sum, 1);
+        sum += __shfl_xor(sum, 2);
+        if (q == 0) part[b_off(L) + row] = sum;
     }
 }
 
@@ -276,7 +288,7 @@ __device__ __forceinline__ void store_grad(const f32x4* gv, int tiles, __bf16* g
     // C-layout gradient -> LDS G^T and -> B fragments (kperm order) for the next backward step
     for (int t = 0; t < tiles; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) gt[(16 * t + 4 * g + i) * 64 + col] = (__bf16)gv[t][i];
+        for (int i = 0; i < 4; ++i) gt[lds_idx(16 * t + 4 * g + i, col)] = (__bf16)gv[t][i];
     for (int s = 0; s < 2; ++s)
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
