@@ -98,23 +98,6 @@ __device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, i
     }
 }
 
-// One forward layer: acc[t] = W_l (tile t) * in + bias.
-template <int L>
-__device__ __forceinline__ void layer_forward(const bf16x8* __restrict__ fwd, const float* __restrict__ params,
-                                              const bf16x8* in, f32x4* acc, int lane) {
-    constexpr int T = out_dim(L) / 16, S = in_dim(L) / 32;
-    const int g = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < S; ++s) c = mfma(fwd[(fwd_base(L) + t * S + s) * 64 + lane], in[s], c);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) c[i] += params[b_off(L) + 16 * t + 4 * g + i];
-        acc[t] = c;
-    }
-}
-
 // ReLU and repack 4 output tiles into the 2 B fragments of the next layer (k order kperm).
 __device__ __forceinline__ void relu_pack(const f32x4* acc, bf16x8* out) {
 #pragma unroll
@@ -264,16 +247,52 @@ __device__ __forceinline__ void weight_grad(TrainLds& sm, const __bf16* at, floa
     }
 }
 
-// Backward data gradient through layer L (1..5): G_{L-1} = (W_L^T G_L) masked by the ReLU of layer L-1.
+// The training kernel keeps a layer's fragments (and forward biases) in registers, loaded one layer ahead so that
+// their L2 latency overlaps the previous layer's work: acc[t] = W_L (tile t) * in + bias.
 template <int L>
-__device__ __forceinline__ void back_data(const bf16x8* __restrict__ bwd, const bf16x8* gin, const bf16x8* act,
-                                          f32x4* gout, int lane) {
+__device__ __forceinline__ void load_fwd_layer(const bf16x8* __restrict__ fwd, const float* __restrict__ params,
+                                               int lane, bf16x8* f, float* b) {
+    constexpr int T = out_dim(L) / 16, S = in_dim(L) / 32;
+    const int g = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < T * S; ++i) f[i] = fwd[(fwd_base(L) + i) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[4 * t + i] = params[b_off(L) + 16 * t + 4 * g + i];
+}
+
+template <int L>
+__device__ __forceinline__ void layer_forward_regs(const bf16x8* f, const float* b, const bf16x8* in, f32x4* acc) {
+    constexpr int T = out_dim(L) / 16, S = in_dim(L) / 32;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < S; ++s) c = mfma(f[t * S + s], in[s], c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] += b[4 * t + i];
+        acc[t] = c;
+    }
+}
+
+template <int L>
+__device__ __forceinline__ void load_bwd_layer(const bf16x8* __restrict__ bwd, int lane, bf16x8* f) {
+    constexpr int S = (L == 5) ? 1 : 2;
+#pragma unroll
+    for (int i = 0; i < 4 * S; ++i) f[i] = bwd[(bwd_base(L) + i) * 64 + lane];
+}
+
+// Backward data gradient through layer L (1..5): G_{L-1} = (W_L^T G_L) masked by the ReLU of layer L-1; f holds
+// the layer's backward fragments.
+template <int L>
+__device__ __forceinline__ void back_data(const bf16x8* f, const bf16x8* gin, const bf16x8* act, f32x4* gout) {
     constexpr int S = (L == 5) ? 1 : 2;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < S; ++s) c = mfma(bwd[(bwd_base(L) + t * S + s) * 64 + lane], gin[s], c);
+        for (int s = 0; s < S; ++s) c = mfma(f[t * S + s], gin[s], c);
         gout[t] = c;
     }
     // ReLU mask: element (t, i) sits at fragment s = t / 2, j = (t % 2) * 4 + i of the layer input
@@ -310,6 +329,9 @@ __global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict
     const uint64_t idx = (uint64_t)blockIdx.x * 64 + col;
     const bool valid = idx < n;
     float* part = partials + (uint64_t)blockIdx.x * RTX_NRC_PARAM_COUNT;
+    bf16x8 fa[12], fb[8];  // fragments of the current and the next layer
+    float ba[16], bb[16];
+    load_fwd_layer<0>(fwd, params, lane, fa, ba);
 
     rtx_nrc_sample smp{};
     if (valid) {
@@ -324,23 +346,29 @@ __global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict
     encode(smp.query, nm, g, valid, in0, sm.at0, col);
     bf16x8 acts[5][2];
     f32x4 acc[4];
-    layer_forward<0>(fwd, params, in0, acc, lane);
+    load_fwd_layer<1>(fwd, params, lane, fb, bb);
+    layer_forward_regs<0>(fa, ba, in0, acc);
     relu_pack(acc, acts[0]);
     store_act(acts[0], sm.at[0], g, col);
-    layer_forward<1>(fwd, params, acts[0], acc, lane);
+    load_fwd_layer<2>(fwd, params, lane, fa, ba);
+    layer_forward_regs<1>(fb, bb, acts[0], acc);
     relu_pack(acc, acts[1]);
     store_act(acts[1], sm.at[1], g, col);
-    layer_forward<2>(fwd, params, acts[1], acc, lane);
+    load_fwd_layer<3>(fwd, params, lane, fb, bb);
+    layer_forward_regs<2>(fa, ba, acts[1], acc);
     relu_pack(acc, acts[2]);
     store_act(acts[2], sm.at[2], g, col);
-    layer_forward<3>(fwd, params, acts[2], acc, lane);
+    load_fwd_layer<4>(fwd, params, lane, fa, ba);
+    layer_forward_regs<3>(fb, bb, acts[2], acc);
     relu_pack(acc, acts[3]);
     store_act(acts[3], sm.at[3], g, col);
-    layer_forward<4>(fwd, params, acts[3], acc, lane);
+    load_fwd_layer<5>(fwd, params, lane, fb, bb);
+    layer_forward_regs<4>(fa, ba, acts[3], acc);
     relu_pack(acc, acts[4]);
     store_act(acts[4], sm.at[4], g, col);
+    load_bwd_layer<5>(bwd, lane, fa);
     f32x4 out[1];
-    layer_forward<5>(fwd, params, acts[4], out, lane);
+    layer_forward_regs<5>(fb, bb, acts[4], out);
 
     // relative L2 loss on RGB (rows 0..2 of the output tile, held by k-group 0)
     f32x4 gy[1] = {{0.0f, 0.0f, 0.0f, 0.0f}};
@@ -357,29 +385,33 @@ __global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict
     bf16x8 gfrag[2];
     store_grad(gy, 1, sm.gt, g, col, gfrag);
     __syncthreads();
+    load_bwd_layer<4>(bwd, lane, fb);
     weight_grad<5>(sm, sm.at[4], part, wave, lane);
     f32x4 gv[4];
-    back_data<5>(bwd, gfrag, acts[4], gv, lane);
+    back_data<5>(fa, gfrag, acts[4], gv);
     __syncthreads();
     store_grad(gv, 4, sm.gt, g, col, gfrag);
     __syncthreads();
+    load_bwd_layer<3>(bwd, lane, fa);
     weight_grad<4>(sm, sm.at[3], part, wave, lane);
-    back_data<4>(bwd, gfrag, acts[3], gv, lane);
+    back_data<4>(fb, gfrag, acts[3], gv);
     __syncthreads();
     store_grad(gv, 4, sm.gt, g, col, gfrag);
     __syncthreads();
+    load_bwd_layer<2>(bwd, lane, fb);
     weight_grad<3>(sm, sm.at[2], part, wave, lane);
-    back_data<3>(bwd, gfrag, acts[2], gv, lane);
+    back_data<3>(fa, gfrag, acts[2], gv);
     __syncthreads();
     store_grad(gv, 4, sm.gt, g, col, gfrag);
     __syncthreads();
+    load_bwd_layer<1>(bwd, lane, fa);
     weight_grad<2>(sm, sm.at[1], part, wave, lane);
-    back_data<2>(bwd, gfrag, acts[1], gv, lane);
+    back_data<2>(fb, gfrag, acts[1], gv);
     __syncthreads();
     store_grad(gv, 4, sm.gt, g, col, gfrag);
     __syncthreads();
     weight_grad<1>(sm, sm.at[0], part, wave, lane);
-    back_data<1>(bwd, gfrag, acts[0], gv, lane);
+    back_data<1>(fa, gfrag, acts[0], gv);
     __syncthreads();
     store_grad(gv, 4, sm.gt, g, col, gfrag);
     __syncthreads();
