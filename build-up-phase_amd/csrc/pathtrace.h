@@ -69,6 +69,9 @@ struct PtArgs {
     const uint32_t* sample_dev;  // when set, added to every pass's sample index (graph replays)
 };
 
+// NRC frames trace their training paths in one launch (trace mode 5) when true (default), else as a second
+// wavefront of per-vertex launches; both give the same records and samples.
+extern bool g_train_megakernel;
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights);
 void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights);
 hipError_t pt_render(PtArgs a, const TraceLaunch& launch, uint32_t spp, uint32_t sample_offset, float* film,

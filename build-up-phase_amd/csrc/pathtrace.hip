@@ -24,13 +24,12 @@
 #include "nrc.h"
 #include "pathtrace.h"
 #include "rtx_internal.h"
+#include "pt_shade.h"
 #include "sampling.h"
 
 namespace rtx {
 
 namespace {
-
-constexpr uint32_t kTrainKey = 0x5EEDu;
 
 // Block-wide compaction of up to 3 queues: one atomic per queue per 256-thread block.  Every thread of the block
 // must call it.  Returns each thread's slot (or 0xFFFFFFFF) in o[q] and the block totals in tot[q].
@@ -66,28 +65,6 @@ __device__ __forceinline__ void block_push3(uint32_t* c0, uint32_t* c1, uint32_t
 // counts[] slot of the training-tail query queue (past the per-depth extension / shadow counts)
 __host__ __device__ __forceinline__ uint32_t tail_counter(const PtArgs& a) { return 3u * a.max_depth + 2u; }
 
-__device__ __forceinline__ uint32_t view_of(const PtArgs& a, uint32_t pid) {
-    uint32_t lo = 0, hi = a.nviews - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (a.views[mid].pix_base <= pid) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Distance along d from o (inside the box) to the exit of the view's clip box.
-__device__ __forceinline__ float clip_exit(const BatchView& v, const float* o, const float* d) {
-    float t = INFINITY;
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        if (d[k] != 0.0f) t = fminf(t, ((d[k] > 0.0f ? v.hi[k] : v.lo[k]) - o[k]) / d[k]);
-    return fmaxf(t, 0.0f);
-}
-
-__device__ __forceinline__ uint32_t pass_sample(const PtArgs& a, uint32_t sample) {
-    return a.sample_dev ? sample + *a.sample_dev : sample;
-}
-
 // Per-view light CDFs of a batched render (one thread per view).
 __global__ void light_cdf_batch_kernel(const BatchView* __restrict__ views, uint32_t nviews,
                                        const float* __restrict__ lights, float* __restrict__ cdf) {
@@ -120,21 +97,6 @@ __global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, f
     cdf[n] = acc;
 }
 
-__device__ __forceinline__ void path_xy(const PtArgs& a, uint32_t pid, uint32_t& x, uint32_t& y) {
-    if (a.views) {  // batched render: pixel of the path's view
-        const BatchView& v = a.views[a.path_view[pid]];
-        const uint32_t local = pid - v.pix_base;
-        x = local % v.width;
-        y = local / v.width;
-    } else if (pid >= a.npix) {  // training path
-        x = pid - a.npix;
-        y = 0x7FFFFFFFu;
-    } else {
-        x = pid % a.width;
-        y = pid / a.width;
-    }
-}
-
 // Camera rays of paths [first_path, npaths): path i enters the extension queue at slot i - first_path.
 __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -142,9 +104,9 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
     if (slot == 0) a.counts[0] = a.npaths - a.first_path;
     if (i >= a.npaths) return;
     sample = pass_sample(a, sample);
-    uint32_t px, py;
-    uint4 j;
     if (a.views) {  // batched render
+        uint32_t px, py;
+        uint4 j;
         const uint32_t vi = view_of(a, i);
         a.path_view[i] = vi;
         const BatchView& v = a.views[vi];
@@ -171,32 +133,8 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
         a.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    if (i >= a.npix) {  // training path: a random pixel
-        const uint4 h = pcg4d(i - a.npix, sample, a.seed, kTrainKey);
-        const uint32_t pix = h.x % a.npix;
-        px = pix % a.width;
-        py = pix / a.width;
-        j = h;
-        j.x = h.y;
-        j.y = h.z;
-    } else {
-        px = i % a.width;
-        py = i / a.width;
-        j = pcg4d(px, py, sample, a.seed ^ kJitterKey);
-    }
-    const float sx = ((float)px + unit_float(j.x)) / (float)a.width;
-    const float sy = ((float)py + unit_float(j.y)) / (float)a.height;
-    const float nx = sx * 2.0f - 1.0f;
-    const float ny = sy * 2.0f - 1.0f;
     rtx_ray r;
-    r.origin[0] = a.cam[0];
-    r.origin[1] = a.cam[1];
-    r.origin[2] = a.cam[2];
-    r.tmin = 0.0f;
-    r.direction[0] = nx * a.ax;
-    r.direction[1] = -(ny * a.ay);
-    r.direction[2] = -1.0f;
-    r.tmax = INFINITY;
+    pt_camera_ray(a, i, sample, r);
     a.ext_rays[0][slot] = r;
     a.ext_pid[0][slot] = i;
     a.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -210,150 +148,19 @@ __global__ __launch_bounds__(256) void shade_kernel(PtArgs a, uint32_t depth, ui
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = a.counts[3 * depth];
     sample = pass_sample(a, sample);
-    bool push_ext = false, push_sh = false, push_cq = false;
-    rtx_ray ext{}, sh{};
-    rtx_nrc_query cq{};
-    float4 contrib = make_float4(0, 0, 0, 0);
-    float4 thr_new = make_float4(0, 0, 0, 0);
+    ShadeOut so;
     uint32_t pid = 0;
     if (q < count) {
         pid = (cur ? a.ext_pid[1] : a.ext_pid[0])[q];
-        const bool train = MODE == kPtCache && pid >= a.npix;
-        // the vertex where this path queries the cache instead of scattering
-        const bool query_here = MODE == kPtCache && ((!train && depth == a.cache_depth) ||
-                                                     (train && a.tail && depth == a.train_end));
-        const bool record = train && depth >= 1 && !query_here;
         const rtx_ray r = (cur ? a.ext_rays[1] : a.ext_rays[0])[q];
         const PathHit h = a.hits[q];
-        if (h.rec != 0xFFFFFFFFu && h.rec < a.nrec) {
-            const rtx_hit_record rec = a.records[h.rec];
-            const float alb[3] = {rec.color[0], rec.color[1], rec.color[2]};
-            const bool lambert = rec.program == RTX_PROGRAM_LAMBERTIAN;
-            const float d[3] = {r.direction[0], r.direction[1], r.direction[2]};
-            const float dn = h.n[0] * d[0] + h.n[1] * d[1] + h.n[2] * d[2];
-            const bool front = dn < 0.0f;
-            const float4 thr = a.thr[pid];
-            if (depth == 0 && front && lambert) {
-                float4 rd = a.rad[pid];
-                rd.x += thr.x * rec.emission[0];
-                rd.y += thr.y * rec.emission[1];
-                rd.z += thr.z * rec.emission[2];
-                a.rad[pid] = rd;
-            }
-            const float nf[3] = {front ? h.n[0] : -h.n[0], front ? h.n[1] : -h.n[1], front ? h.n[2] : -h.n[2]};
-            float P[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) P[k] = fmaf(h.t, d[k], r.origin[k]);
-            float org[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) org[k] = fmaf(a.ray_offset, nf[k], P[k]);
-            const bool reflective = alb[0] > 0.0f || alb[1] > 0.0f || alb[2] > 0.0f;
-            if (query_here || record) {
-                const float il = 1.0f / sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    cq.position[k] = P[k];
-                    cq.normal[k] = nf[k];
-                    cq.direction[k] = -d[k] * il;
-                    cq.albedo[k] = alb[k];
-                }
-            }
-            if (record) {
-                // record the vertex before its own next-event contribution is gathered
-                const float4 rd = a.rad[pid];
-                float4* rp = a.rec + ((uint64_t)(depth - 1) * a.ntrain + (pid - a.npix)) * 5;
-                rp[0] = make_float4(cq.position[0], cq.position[1], cq.position[2], cq.normal[0]);
-                rp[1] = make_float4(cq.normal[1], cq.normal[2], cq.direction[0], cq.direction[1]);
-                rp[2] = make_float4(cq.direction[2], cq.albedo[0], cq.albedo[1], cq.albedo[2]);
-                rp[3] = make_float4(thr.x, thr.y, thr.z, rd.x);
-                rp[4] = make_float4(rd.y, rd.z, 1.0f, 0.0f);
-            }
-            if (query_here) {
-                push_cq = reflective;
-            } else {
-                uint32_t x, y;
-                path_xy(a, pid, x, y);
-                const uint4 ra = pcg4d(x, y, sample, a.seed + 2u * depth);
-                const uint4 rb = pcg4d(x, y, sample, a.seed + 2u * depth + 1u);
-                const BatchView* bv = a.views ? a.views + a.path_view[pid] : nullptr;
-                const uint32_t nl = bv ? bv->light_count : a.nlights;
-                const float* cdf = bv ? a.cdf + bv->cdf_first : a.cdf;
-                const float* lights = bv ? a.lights + 12 * (uint64_t)bv->light_first : a.lights;
-                if (reflective && nl > 0) {
-                    // next-event estimation: light triangle by area, point uniform on it
-                    const float total = cdf[nl];
-                    const float ut = unit_float(ra.x) * total;
-                    uint32_t lo = 0, hi = nl - 1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (cdf[mid] > ut) hi = mid; else lo = mid + 1;
-                    }
-                    const float* L = lights + 12 * (uint64_t)lo;
-                    const float su = sqrtf(unit_float(ra.y));
-                    const float b0 = 1.0f - su, b1 = unit_float(ra.z) * su, b2 = 1.0f - b0 - b1;
-                    float lp[3], e1[3], e2[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        lp[k] = b0 * L[k] + b1 * L[3 + k] + b2 * L[6 + k];
-                        e1[k] = L[3 + k] - L[k];
-                        e2[k] = L[6 + k] - L[k];
-                    }
-                    const float nl[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
-                                         e1[0] * e2[1] - e1[1] * e2[0]};
-                    const float nll = sqrtf(nl[0] * nl[0] + nl[1] * nl[1] + nl[2] * nl[2]);
-                    const float wi[3] = {lp[0] - org[0], lp[1] - org[1], lp[2] - org[2]};
-                    const float d2 = wi[0] * wi[0] + wi[1] * wi[1] + wi[2] * wi[2];
-                    if (nll > 0.0f && d2 > 0.0f) {
-                        const float inv_d = 1.0f / sqrtf(d2);
-                        const float cx = (nf[0] * wi[0] + nf[1] * wi[1] + nf[2] * wi[2]) * inv_d;
-                        const float cl = -(nl[0] * wi[0] + nl[1] * wi[1] + nl[2] * wi[2]) * inv_d / nll;
-                        if (cx > 0.0f && cl > 0.0f) {
-                            const float g = cx * cl / d2 * total * 0.31830988618379067154f;
-                            contrib.x = thr.x * alb[0] * L[9] * g;
-                            contrib.y = thr.y * alb[1] * L[10] * g;
-                            contrib.z = thr.z * alb[2] * L[11] * g;
-                            push_sh = contrib.x > 0.0f || contrib.y > 0.0f || contrib.z > 0.0f;
-                            sh.origin[0] = org[0];
-                            sh.origin[1] = org[1];
-                            sh.origin[2] = org[2];
-                            sh.tmin = 0.0f;
-                            sh.direction[0] = wi[0];
-                            sh.direction[1] = wi[1];
-                            sh.direction[2] = wi[2];
-                            sh.tmax = 1.0f - 1e-4f;
-                        }
-                    }
-                }
-                if (reflective && depth + 1 < a.max_depth && (!train || !a.tail || depth < a.train_end)) {
-                    thr_new = make_float4(thr.x * alb[0], thr.y * alb[1], thr.z * alb[2], 0.0f);
-                    bool alive = true;
-                    if (depth >= a.rr_depth) {
-                        const float p = fminf(0.95f, fmaxf(thr_new.x, fmaxf(thr_new.y, thr_new.z)));
-                        if (unit_float(ra.w) >= p) {
-                            alive = false;
-                        } else {
-                            thr_new.x /= p;
-                            thr_new.y /= p;
-                            thr_new.z /= p;
-                        }
-                    }
-                    if (alive) {
-                        float dir[3];
-                        cosine_dir(nf, unit_float(rb.x), unit_float(rb.y), dir);
-                        ext.origin[0] = org[0];
-                        ext.origin[1] = org[1];
-                        ext.origin[2] = org[2];
-                        ext.tmin = 0.0f;
-                        ext.direction[0] = dir[0];
-                        ext.direction[1] = dir[1];
-                        ext.direction[2] = dir[2];
-                        ext.tmax = bv ? clip_exit(*bv, ext.origin, ext.direction) : INFINITY;
-                        push_ext = true;
-                    }
-                }
-            }
-        }
+        shade_vertex<MODE>(a, pid, r, h, depth, sample, so);
     }
+    const bool push_sh = so.push_sh, push_ext = so.push_ext, push_cq = so.push_cq;
+    const rtx_ray& ext = so.ext;
+    const rtx_ray& sh = so.sh;
+    const rtx_nrc_query& cq = so.cq;
+    const float4 contrib = so.contrib, thr_new = so.thr_new;
     const bool pr[3] = {push_sh, push_ext, push_cq};
     uint32_t o[3], tot[3];
     // render paths queue their cache queries at cache_depth (slots from 0), training tails at train_end (slots from
@@ -523,6 +330,8 @@ __global__ __launch_bounds__(256) void sample_scatter_kernel(PtArgs a, uint64_t 
 }
 
 }  // namespace
+
+bool g_train_megakernel = true;
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
     uint64_t n = npix ? npix : 1;
@@ -720,7 +529,25 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         // while the host enqueues the training wavefront's many short ones)
         if ((e = run_pass(a, L, sample_offset + si, a.cache_depth, s))) return e;
         fill_words_kernel<<<1024, 256, 0, side.stream>>>((uint32_t*)a.rec, nrec * 20, 0u);
-        if ((e = run_pass(t, side.launch, sample_offset + si, t.train_end, side.stream))) return e;
+        if (!t.tail && g_train_megakernel && trace_train_supported()) {
+            // the whole training wavefront in one traversal launch (each lane shades and continues its own path)
+            TraceRequest tr{};
+            tr.tlas = t.tlas;
+            tr.mode = 5;
+            tr.n = ntrain;
+            tr.cull_mask = 0xFFu;
+            tr.records = t.records;
+            tr.nrec = t.nrec;
+            tr.sbt_offset = t.sbt_offset;
+            tr.sbt_stride = t.sbt_stride;
+            tr.state = t.state;
+            tr.gstack = t.gstack;
+            tr.pt = &t;
+            tr.pt_sample = sample_offset + si;
+            if ((e = trace_rays_launch(tr, side.launch, side.stream))) return e;
+        } else if ((e = run_pass(t, side.launch, sample_offset + si, t.train_end, side.stream))) {
+            return e;
+        }
         // training tails (self-training) are predicted with the cache before this frame's Adam steps
         if (t.tail && (e = infer_gather(t, nrc, t.counts + tail_counter(t), a.npix, ntrain, side.stream))) return e;
         merge_state_kernel<<<1, 64, 0, side.stream>>>(a.state, side.state);

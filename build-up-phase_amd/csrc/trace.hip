@@ -22,6 +22,7 @@
 #include <atomic>
 #include <cmath>
 
+#include "pt_shade.h"
 #include "rtx_internal.h"
 #include "trace.h"
 
@@ -250,11 +251,18 @@ struct TraceArgs {
     uint32_t refill_interval;  // steps between refill checks
     uint32_t leaf_lanes;       // pending leaf groups needed before a leaf step runs
     uint32_t ray_flags;        // RTX_RAY_FLAG_* of the launch
+    // kModeTrain: the path tracer's state (training paths [npix, npix + n)) and the pass's sample index
+    PtArgs pt;
+    uint32_t pt_sample;
 };
 
 // kModeFused: one launch over two queues of the wavefront integrator, rays [0, nA) of `rays` as kModePath
 // (extension rays, path hits) and rays [nA, nA + nB) of `rays2` as kModeAny (shadow rays, occlusion flags).
-enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3, kModeFused = 4 };
+// kModeTrain: the NRC frame's training paths whole in one launch.  A lane takes a path index from the ray pool,
+// traces its camera ray, and whenever one of its rays ends (the wave handles its lanes' ended rays together at its
+// next refill) shades the vertex (pathtrace.hip's per-vertex step) and continues the same path with the
+// next-event shadow ray and then the continuation ray, until the path ends; then it takes a new path.
+enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3, kModeFused = 4, kModeTrain = 5 };
 
 // FP64 recomputation of (t, u, v) for the winning triangle (Moller & Trumbore 1997) in object space.
 __device__ void refine_hit(const double* inv, const float* wo, const float* wd, const TriSlot* tri, float& t,
@@ -469,7 +477,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     // add the lane's world ray (6 floats per lane), which they read back where they need it (leaving a BLAS, the
     // final FP64 refinement) instead of keeping it in registers (measured: spill-free at 128 VGPRs and 1% faster
     // path tracing); the other variants keep it in wo / wd
-    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused;
+    constexpr bool kTrain = MODE == kModeTrain;
+    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused || kTrain;
     extern __shared__ uint2 lds_stack[];
     float* const lds_ray = (float*)(lds_stack + args.lds_depth * blockDim.x) + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63u;
@@ -511,6 +520,56 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     uint32_t hit_inst = kNoHit, hit_slot = 0;
     bool occl = false;
     bool shadow_ray = MODE == kModeAny;  // this lane's ray is an any-hit query
+    uint32_t pdepth = 0;       // kModeTrain: vertex of the path that the current ray leaves
+    bool ext_pending = false;  // kModeTrain: the continuation ray waits behind the shadow ray
+    uint32_t pt_rays = 0;      // kModeTrain: rays traced (path-tracer ray count)
+    const uint32_t pt_sample = kTrain ? pass_sample(args.pt, args.pt_sample) : 0u;
+
+    // a new ray (wo, wd, wtmin, tc set): world space at the TLAS root
+    auto begin_ray = [&]() {
+        if (kRayLds) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                lds_ray[k * blockDim.x] = wo[k];
+                lds_ray[(3 + k) * blockDim.x] = wd[k];
+            }
+        }
+        s.ox = wo[0];
+        s.oy = wo[1];
+        s.oz = wo[2];
+        s.dx = wd[0];
+        s.dy = wd[1];
+        s.dz = wd[2];
+        s.toff = 0.0f;
+        setup_space(s);
+        cslots = tslots;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            crlo[a] = trlo[a];
+            cqs[a] = tqs[a];
+        }
+        in_blas = false;
+        cur_nslots = tlas_nslots;
+        steps = 0;
+        ng_base = 0;
+        // the root as a one-child group: child position 0 sits at ordered bit (0 ^ oct)
+        ng_bits = tlas_empty || cull_all ? 0u : ((1u << s.oct) | (1u << 8));
+        lg_bits = 0;
+        sp = 0;
+        hit_inst = kNoHit;
+        occl = false;
+        if (STATS) ++st_rays;
+    };
+    auto set_ray = [&](const rtx_ray& r) {
+        wo[0] = r.origin[0];
+        wo[1] = r.origin[1];
+        wo[2] = r.origin[2];
+        wtmin = r.tmin;
+        wd[0] = r.direction[0];
+        wd[1] = r.direction[1];
+        wd[2] = r.direction[2];
+        tc = r.tmax;
+    };
 
     // Wave-uniform pool of ray ids [pool_next, pool_end): one atomic fetches `chunk` consecutive ids (256 when the
     // launch has at least 8 rounds of work for every wave, else 64 so that small launches still spread over all
@@ -520,7 +579,90 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                            : ((uint64_t)nrays >= (uint64_t)waves_total * 64u * 8u ? 256u : 64u);
     uint32_t pool_next = 0, pool_end = 0;
     for (;;) {
-        if (finished) {
+        if (kTrain && finished) {
+            // continue the lane's path: gather a shadow ray's contribution, or shade the vertex the continuation
+            // ray reached; then trace the next ray of the path, or let the lane take a new path.  The pending
+            // continuation ray and the shadow contribution wait in the path's slot (ray_id) of the queue buffers.
+            const uint32_t pid = args.pt.npix + ray_id;
+            bool cont = false;
+            if (shadow_ray) {
+                if (!occl) {
+                    const float4 c = args.pt.sh_contrib[ray_id];
+                    float4 r = args.pt.rad[pid];
+                    r.x += c.x;
+                    r.y += c.y;
+                    r.z += c.z;
+                    args.pt.rad[pid] = r;
+                }
+                shadow_ray = false;
+                if (ext_pending) {
+                    set_ray(args.pt.ext_rays[0][ray_id]);
+                    ++pdepth;
+                    cont = true;
+                }
+            } else {
+                PathHit ph;
+                ph.t = 0.0f;
+                ph.u = 0.0f;
+                ph.v = 0.0f;
+                ph.inst = hit_inst;
+                ph.prim = 0;
+                ph.geom = 0;
+                ph.n[0] = ph.n[1] = ph.n[2] = 0.0f;
+                ph.rec = 0xFFFFFFFFu;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    wo[k] = lds_ray[k * blockDim.x];
+                    wd[k] = lds_ray[(3 + k) * blockDim.x];
+                }
+                if (hit_inst != kNoHit) {
+                    const InstanceInfo* info = table + hit_inst;
+                    double M[12];
+#pragma unroll
+                    for (int kk = 0; kk < 12; ++kk) M[kk] = info->inv[kk];
+                    const TriSlot* tri =
+                        (const TriSlot*)((const uint8_t*)info->blas + kHeaderBytes + (uint64_t)hit_slot * kSlotBytes);
+                    ph.t = tc;
+                    ph.prim = tri->prim;
+                    ph.geom = tri->geom;
+                    world_normal_f32(M, tri, ph.n);
+                    ph.rec = info->sbt_offset + ph.geom * args.sbt_stride + args.sbt_offset;
+                }
+                rtx_ray cr;
+                cr.origin[0] = wo[0];
+                cr.origin[1] = wo[1];
+                cr.origin[2] = wo[2];
+                cr.tmin = wtmin;
+                cr.direction[0] = wd[0];
+                cr.direction[1] = wd[1];
+                cr.direction[2] = wd[2];
+                cr.tmax = tc;
+                ShadeOut so;
+                shade_vertex<kPtCache>(args.pt, pid, cr, ph, pdepth, pt_sample, so);
+                pt_rays += (so.push_sh ? 1u : 0u) + (so.push_ext ? 1u : 0u);
+                if (so.push_ext) {
+                    args.pt.thr[pid] = so.thr_new;
+                    args.pt.ext_rays[0][ray_id] = so.ext;
+                }
+                ext_pending = so.push_ext;
+                if (so.push_sh) {
+                    args.pt.sh_contrib[ray_id] = so.contrib;
+                    set_ray(so.sh);
+                    shadow_ray = true;
+                    cont = true;
+                } else if (so.push_ext) {
+                    set_ray(so.ext);
+                    ++pdepth;
+                    cont = true;
+                }
+            }
+            if (cont) {
+                begin_ray();
+                active = true;
+            }
+            finished = false;
+        }
+        if (!kTrain && finished) {
             // write the result of ray_id (lanes whose rays ended since the last refill, in one pass)
             if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
                 args.occluded[MODE == kModeFused ? ray_id - nA : ray_id] = occl ? 1u : 0u;
@@ -673,6 +815,17 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         wd[2] = -1.0f;
                         wtmin = args.tmin;
                         tc = args.tmax;
+                    } else if (kTrain) {
+                        const uint32_t pid = args.pt.npix + id;
+                        rtx_ray r;
+                        pt_camera_ray(args.pt, pid, pt_sample, r);
+                        args.pt.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+                        args.pt.rad[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        pdepth = 0;
+                        shadow_ray = false;
+                        ext_pending = false;
+                        ++pt_rays;
+                        set_ray(r);
                     } else {
                         if (MODE == kModeFused) shadow_ray = id >= nA;
                         const float4* rp = (const float4*)(MODE == kModeFused && id >= nA ? args.rays2 + (id - nA)
@@ -687,38 +840,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         wd[2] = r1.z;
                         tc = r1.w;
                     }
-                    if (kRayLds) {
-#pragma unroll
-                        for (int k = 0; k < 3; ++k) {
-                            lds_ray[k * blockDim.x] = wo[k];
-                            lds_ray[(3 + k) * blockDim.x] = wd[k];
-                        }
-                    }
-                    s.ox = wo[0];
-                    s.oy = wo[1];
-                    s.oz = wo[2];
-                    s.dx = wd[0];
-                    s.dy = wd[1];
-                    s.dz = wd[2];
-                    s.toff = 0.0f;
-                    setup_space(s);
-                    cslots = tslots;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        crlo[a] = trlo[a];
-                        cqs[a] = tqs[a];
-                    }
-                    in_blas = false;
-                    cur_nslots = tlas_nslots;
-                    steps = 0;
-                    ng_base = 0;
-                    // the root as a one-child group: child position 0 sits at ordered bit (0 ^ oct)
-                    ng_bits = tlas_empty || cull_all ? 0u : ((1u << s.oct) | (1u << 8));
-                    lg_bits = 0;
-                    sp = 0;
-                    hit_inst = kNoHit;
-                    occl = false;
-                    if (STATS) ++st_rays;
+                    begin_ray();
                 } else {
                     exhausted = true;
                 }
@@ -726,6 +848,14 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
         }
         if (!__any(active)) break;
         const bool wave_exhausted = __any(exhausted);
+        // kModeTrain: back to the refill point (where ended rays continue their paths) once a quarter of the lanes
+        // that start this round (64 - refill_lanes of 64) have ended their rays, at least one
+        uint32_t train_floor = 0;
+        if (kTrain) {
+            const uint32_t n0 = (uint32_t)__popcll(__ballot(active));
+            const uint32_t q = n0 * (64u - args.refill_lanes) / 64u;
+            train_floor = n0 - (q > 0u ? q : 1u);
+        }
 
         int iter = 0;
         while (active) {
@@ -928,7 +1058,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         }
                         if (cand &&
                             (!FLAGS || candidate_accepted(args, ray_flags, c.w, c.z, cur_inst, table, bu, bv))) {
-                            if (MODE == kModeAny || (MODE == kModeFused && shadow_ray)) {
+                            if (MODE == kModeAny || ((MODE == kModeFused || kTrain) && shadow_ray)) {
                                 occl = true;
                                 done = true;
                                 break;
@@ -1003,11 +1133,14 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
             }
             if (++iter >= (int)args.refill_interval) {
                 iter = 0;
-                if (!wave_exhausted && __popcll(__ballot(true)) < args.refill_lanes) break;
+                if (kTrain ? (uint32_t)__popcll(__ballot(true)) <= train_floor
+                           : (!wave_exhausted && __popcll(__ballot(true)) < args.refill_lanes))
+                    break;
             }
         }
     }
 
+    if (kTrain && pt_rays) atomicAdd(&args.state->pt_rays, (unsigned long long)pt_rays);
     if (STATS) {
         atomicAdd(&args.state->stat_nodes, (unsigned long long)st_nodes);
         atomicAdd(&args.state->stat_tris, (unsigned long long)st_tris);
@@ -1047,7 +1180,7 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
 template <int MODE, bool STATS, bool FLAGS>
 hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
     // the stack, and the lanes' world rays where the kernel keeps them in LDS
-    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused;
+    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused || MODE == kModeTrain;
     const size_t lds =
         (size_t)L.lds_depth * L.block_size * sizeof(uint2) + (kRayLds ? (size_t)L.block_size * 6 * sizeof(float) : 0);
     if (L.waves >= 5)
@@ -1083,6 +1216,7 @@ hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
 
 }  // namespace
 
+bool trace_train_supported() { return !needs_flags_variant(0u); }
 void note_nonopaque_geometry() { g_nonopaque_geometry.store(true, std::memory_order_relaxed); }
 void note_procedural_geometry() { g_procedural_geometry.store(true, std::memory_order_relaxed); }
 
@@ -1122,6 +1256,12 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.refill_interval = L.refill_interval;
     a.leaf_lanes = L.leaf_lanes;
     a.ray_flags = r.ray_flags;
+    if (r.mode == 5) {
+        // training paths: opaque triangle geometry and no ray flags only (the default traversal variant)
+        if (!r.pt || needs_flags_variant(0u)) return hipErrorInvalidValue;
+        a.pt = *r.pt;
+        a.pt_sample = r.pt_sample;
+    }
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
     if (r.n == 0 && !r.n_ptr) return hipGetLastError();
     switch (r.mode) {
@@ -1129,6 +1269,7 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
         case 1: return r.stats ? launch<kModeAny, true>(a, L, s) : launch<kModeAny, false>(a, L, s);
         case 3: return launch<kModePath, false>(a, L, s);
         case 4: return launch<kModeFused, false>(a, L, s);
+        case 5: return launch_variant<kModeTrain, false, false>(a, L, s);
         default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
     }
 }
