@@ -362,6 +362,34 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
     log(f"plain relMSE {err_plain:.5f} vs {64 * chunks}-spp reference")
 
     lo, hi = S.scene_bounds(sc)
+    # The NRC frame's launches (a render wavefront to the cache vertex beside the training paths' whole-path
+    # launch) want their own launch shape: a few shapes timed on a scratch cache (untimed setup, like the other
+    # autotuning); the plain render keeps its own.
+    nrc_cfg = ctx.config()
+    if not args.no_autotune:
+        base = ctx.config()
+        best = None
+        scratch_nrc = pyrtx.Nrc(ctx, lo, hi, seed=2, batch_size=16384, cache_vertex=1)
+        for bs, bpc in ((256, 4), (256, 5), (128, 8), (128, 10)):
+            for leaf in (1, 8):
+                cfg = dict(base, block_size=bs, blocks_per_cu=bpc, leaf_lanes=leaf)
+                ctx.set_config(**cfg)
+                ts = []
+                for k in range(3):
+                    a0, b0 = ev(), ev()
+                    a0.record()
+                    ctx.render_nrc(tlas, rec, sc.camera, lt, scratch_nrc, W, H, spp=1, train_paths=65536,
+                                   sample_offset=(1 << 22) + k)
+                    b0.record()
+                    torch.cuda.synchronize()
+                    ts.append(a0.elapsed_time(b0))
+                t = sorted(ts)[1]
+                if best is None or t < best[0]:
+                    best = (t, cfg)
+        scratch_nrc.close()
+        nrc_cfg = best[1]
+        log(f"autotuned NRC frame: {nrc_cfg} ({best[0]:.2f} ms per frame)")
+    ctx.set_config(**nrc_cfg)
     nrc = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=16384, cache_vertex=1)
     accn = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
     errs = torch.empty(args.nrc_frames, dtype=torch.float32, device=dev)
@@ -396,7 +424,7 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
                   "mrays_per_s": round(n_rays / ms_plain / 1e3, 1), "relmse": round(err_plain, 6)},
         "reference_spp": 64 * chunks,
         "nrc": {"frames": args.nrc_frames, "train_paths_per_frame": 65536, "batch_size": 16384,
-                "cache_vertex": 1, "frame_ms": round(float(fms.mean()), 3),
+                "cache_vertex": 1, "frame_ms": round(float(fms.mean()), 3), "autotuned_trace_config": nrc_cfg,
                 "relmse_final": round(float(e[-1]), 6), "adam_steps": int(steps),
                 "frames_to_plain_relmse": None if f is None else f + 1,
                 "ms_to_plain_relmse": None if ms_nrc is None else round(ms_nrc, 2),

@@ -332,6 +332,7 @@ __global__ __launch_bounds__(256) void sample_scatter_kernel(PtArgs a, uint64_t 
 }  // namespace
 
 bool g_train_megakernel = true;
+bool g_path_megakernel = true;
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
     uint64_t n = npix ? npix : 1;
@@ -445,8 +446,27 @@ hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t samp
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
     fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
     hipError_t e;
+    const bool whole_paths = g_path_megakernel && trace_train_supported();
     for (uint32_t si = 0; si < spp; ++si) {
-        if ((e = run_pass(a, L, sample_offset + si, a.max_depth - 1, s))) return e;
+        if (whole_paths) {
+            // one launch per sample: every lane traces, shades and continues its own pixel paths
+            TraceRequest tr{};
+            tr.tlas = a.tlas;
+            tr.mode = 6;
+            tr.n = a.npix;
+            tr.cull_mask = 0xFFu;
+            tr.records = a.records;
+            tr.nrec = a.nrec;
+            tr.sbt_offset = a.sbt_offset;
+            tr.sbt_stride = a.sbt_stride;
+            tr.state = a.state;
+            tr.gstack = a.gstack;
+            tr.pt = &a;
+            tr.pt_sample = sample_offset + si;
+            if ((e = trace_rays_launch(tr, L, s))) return e;
+        } else if ((e = run_pass(a, L, sample_offset + si, a.max_depth - 1, s))) {
+            return e;
+        }
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
     if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, film);

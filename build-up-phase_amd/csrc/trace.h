@@ -13,7 +13,8 @@ struct TraceRequest {
     const void* tlas;
     uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film, 3 path hits,
                     // 4 path hits for `rays` + occlusion for `rays2` in one launch,
-                    // 5 whole training paths of an NRC frame (n paths, pt / pt_sample)
+                    // 5 whole training paths of an NRC frame (n paths, pt / pt_sample),
+                    // 6 whole pixel paths of one sample of a plain single-view render
     uint32_t n;
     const uint32_t* n_ptr;  // device ray count (overrides n when set)
     const rtx_ray* rays2;   // mode 4: the any-hit rays and their device count
@@ -55,7 +56,7 @@ struct TraceLaunch {
 };
 
 hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);
-// Mode 5 (whole training paths in one launch) runs the default traversal variant: false once non-opaque or AABB
+// Modes 5 and 6 (whole paths in one launch) run the default traversal variant: false once non-opaque or AABB
 // geometry has been built in this process.
 bool trace_train_supported();
 // A BLAS with non-opaque geometry was built: later launches honour opacity and any-hit programs.

@@ -251,18 +251,29 @@ struct TraceArgs {
     uint32_t refill_interval;  // steps between refill checks
     uint32_t leaf_lanes;       // pending leaf groups needed before a leaf step runs
     uint32_t ray_flags;        // RTX_RAY_FLAG_* of the launch
-    // kModeTrain: the path tracer's state (training paths [npix, npix + n)) and the pass's sample index
+    // kModeTrain / kModePixelPaths: the path tracer's state, the first path id (npix for training paths, 0 for
+    // pixel paths) and the pass's sample index
     PtArgs pt;
+    uint32_t pt_first;
     uint32_t pt_sample;
 };
 
 // kModeFused: one launch over two queues of the wavefront integrator, rays [0, nA) of `rays` as kModePath
 // (extension rays, path hits) and rays [nA, nA + nB) of `rays2` as kModeAny (shadow rays, occlusion flags).
-// kModeTrain: the NRC frame's training paths whole in one launch.  A lane takes a path index from the ray pool,
-// traces its camera ray, and whenever one of its rays ends (the wave handles its lanes' ended rays together at its
-// next refill) shades the vertex (pathtrace.hip's per-vertex step) and continues the same path with the
-// next-event shadow ray and then the continuation ray, until the path ends; then it takes a new path.
-enum Mode { kModeClosest = 0, kModeAny = 1, kModeRaygen = 2, kModePath = 3, kModeFused = 4, kModeTrain = 5 };
+// kModeTrain / kModePixelPaths: whole paths in one launch - the NRC frame's training paths, or one sample of a
+// plain render's pixel paths.  A lane takes a path index from the ray pool, traces its camera ray, and whenever
+// one of its rays ends (the wave handles its lanes' ended rays together at its next refill) shades the vertex
+// (pathtrace.hip's per-vertex step) and continues the same path with the next-event shadow ray and then the
+// continuation ray, until the path ends; then it takes a new path.
+enum Mode {
+    kModeClosest = 0,
+    kModeAny = 1,
+    kModeRaygen = 2,
+    kModePath = 3,
+    kModeFused = 4,
+    kModeTrain = 5,
+    kModePixelPaths = 6
+};
 
 // FP64 recomputation of (t, u, v) for the winning triangle (Moller & Trumbore 1997) in object space.
 __device__ void refine_hit(const double* inv, const float* wo, const float* wd, const TriSlot* tri, float& t,
@@ -477,7 +488,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     // add the lane's world ray (6 floats per lane), which they read back where they need it (leaving a BLAS, the
     // final FP64 refinement) instead of keeping it in registers (measured: spill-free at 128 VGPRs and 1% faster
     // path tracing); the other variants keep it in wo / wd
-    constexpr bool kTrain = MODE == kModeTrain;
+    constexpr bool kTrain = MODE == kModeTrain || MODE == kModePixelPaths;  // whole paths in this launch
+    constexpr int kShadeMode = MODE == kModeTrain ? (int)kPtCache : (int)kPtPlain;
     constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused || kTrain;
     extern __shared__ uint2 lds_stack[];
     float* const lds_ray = (float*)(lds_stack + args.lds_depth * blockDim.x) + threadIdx.x;
@@ -583,7 +595,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
             // continue the lane's path: gather a shadow ray's contribution, or shade the vertex the continuation
             // ray reached; then trace the next ray of the path, or let the lane take a new path.  The pending
             // continuation ray and the shadow contribution wait in the path's slot (ray_id) of the queue buffers.
-            const uint32_t pid = args.pt.npix + ray_id;
+            const uint32_t pid = args.pt_first + ray_id;
             bool cont = false;
             if (shadow_ray) {
                 if (!occl) {
@@ -638,7 +650,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                 cr.direction[2] = wd[2];
                 cr.tmax = tc;
                 ShadeOut so;
-                shade_vertex<kPtCache>(args.pt, pid, cr, ph, pdepth, pt_sample, so);
+                shade_vertex<kShadeMode>(args.pt, pid, cr, ph, pdepth, pt_sample, so);
                 pt_rays += (so.push_sh ? 1u : 0u) + (so.push_ext ? 1u : 0u);
                 if (so.push_ext) {
                     args.pt.thr[pid] = so.thr_new;
@@ -816,7 +828,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                         wtmin = args.tmin;
                         tc = args.tmax;
                     } else if (kTrain) {
-                        const uint32_t pid = args.pt.npix + id;
+                        const uint32_t pid = args.pt_first + id;
                         rtx_ray r;
                         pt_camera_ray(args.pt, pid, pt_sample, r);
                         args.pt.thr[pid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -1063,9 +1075,16 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                                 done = true;
                                 break;
                             }
-                            tc = s.toff + th;
-                            hit_inst = cur_inst & kInstIdMask;
-                            hit_slot = slot;
+                            // exact ties in t go to the smaller (instance, slot), so the closest hit does not
+                            // depend on the order in which a lane meets its candidates (which the wave's leaf
+                            // scheduling can change)
+                            const float tn = s.toff + th;
+                            const uint32_t ci = cur_inst & kInstIdMask;
+                            if (!(tn == tc && (ci > hit_inst || (ci == hit_inst && slot > hit_slot)))) {
+                                tc = tn;
+                                hit_inst = ci;
+                                hit_slot = slot;
+                            }
                             if (ray_flags & kRayTerminateOnFirstHit) {
                                 done = true;
                                 break;
@@ -1180,7 +1199,8 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
 template <int MODE, bool STATS, bool FLAGS>
 hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
     // the stack, and the lanes' world rays where the kernel keeps them in LDS
-    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused || MODE == kModeTrain;
+    constexpr bool kRayLds =
+        FLAGS || MODE == kModePath || MODE == kModeFused || MODE == kModeTrain || MODE == kModePixelPaths;
     const size_t lds =
         (size_t)L.lds_depth * L.block_size * sizeof(uint2) + (kRayLds ? (size_t)L.block_size * 6 * sizeof(float) : 0);
     if (L.waves >= 5)
@@ -1256,10 +1276,11 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.refill_interval = L.refill_interval;
     a.leaf_lanes = L.leaf_lanes;
     a.ray_flags = r.ray_flags;
-    if (r.mode == 5) {
-        // training paths: opaque triangle geometry and no ray flags only (the default traversal variant)
+    if (r.mode == 5 || r.mode == 6) {
+        // whole paths: opaque triangle geometry and no ray flags only (the default traversal variant)
         if (!r.pt || needs_flags_variant(0u)) return hipErrorInvalidValue;
         a.pt = *r.pt;
+        a.pt_first = r.mode == 5 ? r.pt->npix : 0u;
         a.pt_sample = r.pt_sample;
     }
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
@@ -1270,6 +1291,7 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
         case 3: return launch<kModePath, false>(a, L, s);
         case 4: return launch<kModeFused, false>(a, L, s);
         case 5: return launch_variant<kModeTrain, false, false>(a, L, s);
+        case 6: return launch_variant<kModePixelPaths, false, false>(a, L, s);
         default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
     }
 }
