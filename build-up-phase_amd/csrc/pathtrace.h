@@ -72,8 +72,8 @@ struct PtArgs {
 // NRC frames trace their training paths in one launch (trace mode 5) when true (default), else as a second
 // wavefront of per-vertex launches; both give the same records and samples.
 extern bool g_train_megakernel;
-// Plain single-view renders trace each sample's pixel paths in one launch (trace mode 6) when true, else as a
-// wavefront of per-vertex launches; both give the same film.
+// Plain renders (single view or batched) trace each sample's pixel paths in one launch (trace mode 6) when true,
+// else as a wavefront of per-vertex launches; both give the same film.
 extern bool g_path_megakernel;
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights);
 void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights);

@@ -104,35 +104,6 @@ __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) 
     if (slot == 0) a.counts[0] = a.npaths - a.first_path;
     if (i >= a.npaths) return;
     sample = pass_sample(a, sample);
-    if (a.views) {  // batched render
-        uint32_t px, py;
-        uint4 j;
-        const uint32_t vi = view_of(a, i);
-        a.path_view[i] = vi;
-        const BatchView& v = a.views[vi];
-        const uint32_t local = i - v.pix_base;
-        px = local % v.width;
-        py = local / v.width;
-        j = pcg4d(px, py, sample, a.seed ^ kJitterKey);
-        const float sx = ((float)px + unit_float(j.x)) / (float)v.width;
-        const float sy = ((float)py + unit_float(j.y)) / (float)v.height;
-        const float nx = sx * 2.0f - 1.0f;
-        const float ny = sy * 2.0f - 1.0f;
-        rtx_ray r;
-        r.origin[0] = v.cam[0];
-        r.origin[1] = v.cam[1];
-        r.origin[2] = v.cam[2];
-        r.tmin = 0.0f;
-        r.direction[0] = nx * v.ax;
-        r.direction[1] = -(ny * v.ay);
-        r.direction[2] = -1.0f;
-        r.tmax = clip_exit(v, r.origin, r.direction);
-        a.ext_rays[0][slot] = r;
-        a.ext_pid[0][slot] = i;
-        a.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        a.rad[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        return;
-    }
     rtx_ray r;
     pt_camera_ray(a, i, sample, r);
     a.ext_rays[0][slot] = r;
@@ -437,6 +408,26 @@ static hipError_t infer_gather(PtArgs& a, const NrcState& nrc, const uint32_t* c
     return hipGetLastError();
 }
 
+// One sample of a plain render: every pixel path whole in one traversal launch (each lane traces, shades and
+// continues its own paths), or the per-vertex wavefront where that launch is not available.
+static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample, hipStream_t s) {
+    if (!g_path_megakernel || !trace_train_supported()) return run_pass(a, L, sample, a.max_depth - 1, s);
+    TraceRequest tr{};
+    tr.tlas = a.tlas;
+    tr.mode = 6;
+    tr.n = a.npix;
+    tr.cull_mask = 0xFFu;
+    tr.records = a.records;
+    tr.nrec = a.nrec;
+    tr.sbt_offset = a.sbt_offset;
+    tr.sbt_stride = a.sbt_stride;
+    tr.state = a.state;
+    tr.gstack = a.gstack;
+    tr.pt = &a;
+    tr.pt_sample = sample;
+    return trace_rays_launch(tr, L, s);
+}
+
 hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t sample_offset, float* film,
                      double* acc64, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
@@ -446,27 +437,8 @@ hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t samp
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
     fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
     hipError_t e;
-    const bool whole_paths = g_path_megakernel && trace_train_supported();
     for (uint32_t si = 0; si < spp; ++si) {
-        if (whole_paths) {
-            // one launch per sample: every lane traces, shades and continues its own pixel paths
-            TraceRequest tr{};
-            tr.tlas = a.tlas;
-            tr.mode = 6;
-            tr.n = a.npix;
-            tr.cull_mask = 0xFFu;
-            tr.records = a.records;
-            tr.nrec = a.nrec;
-            tr.sbt_offset = a.sbt_offset;
-            tr.sbt_stride = a.sbt_stride;
-            tr.state = a.state;
-            tr.gstack = a.gstack;
-            tr.pt = &a;
-            tr.pt_sample = sample_offset + si;
-            if ((e = trace_rays_launch(tr, L, s))) return e;
-        } else if ((e = run_pass(a, L, sample_offset + si, a.max_depth - 1, s))) {
-            return e;
-        }
+        if ((e = render_sample(a, L, sample_offset + si, s))) return e;
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
     if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, film);
@@ -483,7 +455,7 @@ hipError_t pt_render_batch(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_
     fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
     hipError_t e;
     for (uint32_t si = 0; si < spp; ++si) {
-        if ((e = run_pass(a, L, sample_offset + si, a.max_depth - 1, s))) return e;
+        if ((e = render_sample(a, L, sample_offset + si, s))) return e;
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
     if (films) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, films);

@@ -49,10 +49,33 @@ __device__ __forceinline__ void path_xy(const PtArgs& a, uint32_t pid, uint32_t&
     }
 }
 
-// Camera ray of path i (single view): pixel paths jitter within their pixel, training paths start at a random pixel.
+// Camera ray of path i: pixel paths jitter within their pixel (of their view, in a batched render: the path's
+// view is looked up and stored), training paths start at a random pixel.
 __device__ __forceinline__ void pt_camera_ray(const PtArgs& a, uint32_t i, uint32_t sample, rtx_ray& r) {
     uint32_t px, py;
     uint4 j;
+    if (a.views) {  // batched render
+        const uint32_t vi = view_of(a, i);
+        a.path_view[i] = vi;
+        const BatchView& v = a.views[vi];
+        const uint32_t local = i - v.pix_base;
+        px = local % v.width;
+        py = local / v.width;
+        j = pcg4d(px, py, sample, a.seed ^ kJitterKey);
+        const float sx = ((float)px + unit_float(j.x)) / (float)v.width;
+        const float sy = ((float)py + unit_float(j.y)) / (float)v.height;
+        const float nx = sx * 2.0f - 1.0f;
+        const float ny = sy * 2.0f - 1.0f;
+        r.origin[0] = v.cam[0];
+        r.origin[1] = v.cam[1];
+        r.origin[2] = v.cam[2];
+        r.tmin = 0.0f;
+        r.direction[0] = nx * v.ax;
+        r.direction[1] = -(ny * v.ay);
+        r.direction[2] = -1.0f;
+        r.tmax = clip_exit(v, r.origin, r.direction);
+        return;
+    }
     if (i >= a.npix) {  // training path: a random pixel
         const uint4 h = pcg4d(i - a.npix, sample, a.seed, kTrainKey);
         const uint32_t pix = h.x % a.npix;

@@ -14,7 +14,7 @@ struct TraceRequest {
     uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film, 3 path hits,
                     // 4 path hits for `rays` + occlusion for `rays2` in one launch,
                     // 5 whole training paths of an NRC frame (n paths, pt / pt_sample),
-                    // 6 whole pixel paths of one sample of a plain single-view render
+                    // 6 whole pixel paths of one sample of a plain render
     uint32_t n;
     const uint32_t* n_ptr;  // device ray count (overrides n when set)
     const rtx_ray* rays2;   // mode 4: the any-hit rays and their device count
