@@ -371,8 +371,9 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
         best = None
         scratch_nrc = pyrtx.Nrc(ctx, lo, hi, seed=2, batch_size=16384, cache_vertex=1)
         for bs, bpc in ((256, 4), (256, 5), (128, 8), (128, 10)):
-            for leaf in (1, 8):
-                cfg = dict(base, block_size=bs, blocks_per_cu=bpc, leaf_lanes=leaf)
+            for leaf, refill, interval, chunk in ((1, 48, 4, 128), (8, 48, 4, 128), (1, 32, 8, 1), (8, 32, 8, 1)):
+                cfg = dict(base, block_size=bs, blocks_per_cu=bpc, leaf_lanes=leaf, refill_lanes=refill,
+                           refill_interval=interval, ray_chunk=chunk)
                 ctx.set_config(**cfg)
                 ts = []
                 for k in range(3):

@@ -428,6 +428,27 @@ static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample
     return trace_rays_launch(tr, L, s);
 }
 
+// The pixel paths of one NRC sample up to their cache queries (queued from a.counts[2]): whole paths in one
+// traversal launch, or the per-vertex wavefront where that launch is not available.
+static hipError_t render_cache_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample, hipStream_t s) {
+    if (!g_path_megakernel || !trace_train_supported()) return run_pass(a, L, sample, a.cache_depth, s);
+    fill_words_kernel<<<1, 64, 0, s>>>(a.counts, 3ull * (a.max_depth + 2), 0u);
+    TraceRequest tr{};
+    tr.tlas = a.tlas;
+    tr.mode = 7;
+    tr.n = a.npix;
+    tr.cull_mask = 0xFFu;
+    tr.records = a.records;
+    tr.nrec = a.nrec;
+    tr.sbt_offset = a.sbt_offset;
+    tr.sbt_stride = a.sbt_stride;
+    tr.state = a.state;
+    tr.gstack = a.gstack;
+    tr.pt = &a;
+    tr.pt_sample = sample;
+    return trace_rays_launch(tr, L, s);
+}
+
 hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t sample_offset, float* film,
                      double* acc64, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
@@ -480,7 +501,7 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         const uint32_t ntrain = si == 0 ? train_paths : 0u;
         a.ntrain = ntrain;
         if (ntrain == 0) {
-            if ((e = run_pass(a, L, sample_offset + si, a.cache_depth, s))) return e;
+            if ((e = render_cache_sample(a, L, sample_offset + si, s))) return e;
             if ((e = infer_gather(a, nrc, a.counts + 2, 0, a.npix, s))) return e;
             accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
             continue;
@@ -519,7 +540,7 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         if ((e = hipStreamWaitEvent(side.stream, side.fork, 0))) return e;
         // render paths up to their cache vertex (a few long launches, enqueued first so that the GPU has them
         // while the host enqueues the training wavefront's many short ones)
-        if ((e = run_pass(a, L, sample_offset + si, a.cache_depth, s))) return e;
+        if ((e = render_cache_sample(a, L, sample_offset + si, s))) return e;
         fill_words_kernel<<<1024, 256, 0, side.stream>>>((uint32_t*)a.rec, nrec * 20, 0u);
         if (!t.tail && g_train_megakernel && trace_train_supported()) {
             // the whole training wavefront in one traversal launch (each lane shades and continues its own path)

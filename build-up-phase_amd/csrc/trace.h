@@ -14,7 +14,8 @@ struct TraceRequest {
     uint32_t mode;  // 0 closest (ray buffer), 1 any (ray buffer), 2 raygen into a film, 3 path hits,
                     // 4 path hits for `rays` + occlusion for `rays2` in one launch,
                     // 5 whole training paths of an NRC frame (n paths, pt / pt_sample),
-                    // 6 whole pixel paths of one sample of a plain render
+                    // 6 whole pixel paths of one sample of a plain render,
+                    // 7 whole pixel paths of an NRC frame up to their cache query
     uint32_t n;
     const uint32_t* n_ptr;  // device ray count (overrides n when set)
     const rtx_ray* rays2;   // mode 4: the any-hit rays and their device count
@@ -56,7 +57,7 @@ struct TraceLaunch {
 };
 
 hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);
-// Modes 5 and 6 (whole paths in one launch) run the default traversal variant: false once non-opaque or AABB
+// Modes 5-7 (whole paths in one launch) run the default traversal variant: false once non-opaque or AABB
 // geometry has been built in this process.
 bool trace_train_supported();
 // A BLAS with non-opaque geometry was built: later launches honour opacity and any-hit programs.

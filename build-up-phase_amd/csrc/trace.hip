@@ -272,7 +272,8 @@ enum Mode {
     kModePath = 3,
     kModeFused = 4,
     kModeTrain = 5,
-    kModePixelPaths = 6
+    kModePixelPaths = 6,
+    kModePixelCache = 7  // the pixel paths of an NRC frame: whole paths up to their cache query
 };
 
 // FP64 recomputation of (t, u, v) for the winning triangle (Moller & Trumbore 1997) in object space.
@@ -488,8 +489,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
     // add the lane's world ray (6 floats per lane), which they read back where they need it (leaving a BLAS, the
     // final FP64 refinement) instead of keeping it in registers (measured: spill-free at 128 VGPRs and 1% faster
     // path tracing); the other variants keep it in wo / wd
-    constexpr bool kTrain = MODE == kModeTrain || MODE == kModePixelPaths;  // whole paths in this launch
-    constexpr int kShadeMode = MODE == kModeTrain ? (int)kPtCache : (int)kPtPlain;
+    constexpr bool kTrain = MODE == kModeTrain || MODE == kModePixelPaths || MODE == kModePixelCache;  // whole paths
+    constexpr int kShadeMode = MODE == kModePixelPaths ? (int)kPtPlain : (int)kPtCache;
     constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused || kTrain;
     extern __shared__ uint2 lds_stack[];
     float* const lds_ray = (float*)(lds_stack + args.lds_depth * blockDim.x) + threadIdx.x;
@@ -657,6 +658,21 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args) {
                     args.pt.ext_rays[0][ray_id] = so.ext;
                 }
                 ext_pending = so.push_ext;
+                if (MODE == kModePixelCache) {
+                    // the path ends in a cache query: one slot per query, one atomic per wave
+                    const uint64_t qm = __ballot(so.push_cq);
+                    if (qm) {
+                        const uint32_t leader = (uint32_t)(__ffsll((long long)qm) - 1);
+                        uint32_t base = 0;
+                        if (lane == leader) base = atomicAdd(&args.pt.counts[2], (uint32_t)__popcll(qm));
+                        base = __shfl(base, (int)leader);
+                        if (so.push_cq) {
+                            const uint32_t slot = base + (uint32_t)__popcll(qm & lt_mask);
+                            args.pt.cq[slot] = so.cq;
+                            args.pt.cq_pid[slot] = pid;
+                        }
+                    }
+                }
                 if (so.push_sh) {
                     args.pt.sh_contrib[ray_id] = so.contrib;
                     set_ray(so.sh);
@@ -1199,8 +1215,8 @@ __global__ void reset_counter_kernel(DeviceState* st, int clear_stats) {
 template <int MODE, bool STATS, bool FLAGS>
 hipError_t launch_variant(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
     // the stack, and the lanes' world rays where the kernel keeps them in LDS
-    constexpr bool kRayLds =
-        FLAGS || MODE == kModePath || MODE == kModeFused || MODE == kModeTrain || MODE == kModePixelPaths;
+    constexpr bool kRayLds = FLAGS || MODE == kModePath || MODE == kModeFused || MODE == kModeTrain ||
+                             MODE == kModePixelPaths || MODE == kModePixelCache;
     const size_t lds =
         (size_t)L.lds_depth * L.block_size * sizeof(uint2) + (kRayLds ? (size_t)L.block_size * 6 * sizeof(float) : 0);
     if (L.waves >= 5)
@@ -1276,7 +1292,7 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.refill_interval = L.refill_interval;
     a.leaf_lanes = L.leaf_lanes;
     a.ray_flags = r.ray_flags;
-    if (r.mode == 5 || r.mode == 6) {
+    if (r.mode >= 5 && r.mode <= 7) {
         // whole paths: opaque triangle geometry and no ray flags only (the default traversal variant)
         if (!r.pt || needs_flags_variant(0u)) return hipErrorInvalidValue;
         a.pt = *r.pt;
@@ -1292,6 +1308,7 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
         case 4: return launch<kModeFused, false>(a, L, s);
         case 5: return launch_variant<kModeTrain, false, false>(a, L, s);
         case 6: return launch_variant<kModePixelPaths, false, false>(a, L, s);
+        case 7: return launch_variant<kModePixelCache, false, false>(a, L, s);
         default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
     }
 }
