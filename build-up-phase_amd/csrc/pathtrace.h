@@ -69,12 +69,6 @@ struct PtArgs {
     const uint32_t* sample_dev;  // when set, added to every pass's sample index (graph replays)
 };
 
-// NRC frames trace their training paths in one launch (trace mode 5) when true (default), else as a second
-// wavefront of per-vertex launches; both give the same records and samples.
-extern bool g_train_megakernel;
-// Plain renders (single view or batched) trace each sample's pixel paths in one launch (trace mode 6) when true,
-// else as a wavefront of per-vertex launches; both give the same film.
-extern bool g_path_megakernel;
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights);
 void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights);
 hipError_t pt_render(PtArgs a, const TraceLaunch& launch, uint32_t spp, uint32_t sample_offset, float* film,

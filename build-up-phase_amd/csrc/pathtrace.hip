@@ -19,6 +19,7 @@
 // vertex) / throughput, i.e. the radiance the path carried from that vertex toward the previous one.  Samples are
 // compacted with a deterministic block scan.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 
 #include "nrc.h"
@@ -302,8 +303,12 @@ __global__ __launch_bounds__(256) void sample_scatter_kernel(PtArgs a, uint64_t 
 
 }  // namespace
 
-bool g_train_megakernel = true;
-bool g_path_megakernel = true;
+// Whole-path launches (trace modes 5-7), unless RTX_PATH_WAVEFRONT=1 asks for the per-vertex wavefront (tests
+// compare the two), and only where the default traversal variant serves the process's geometry.
+static bool whole_paths() {
+    const char* e = std::getenv("RTX_PATH_WAVEFRONT");
+    return !(e && e[0] == '1') && trace_train_supported();
+}
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
     uint64_t n = npix ? npix : 1;
@@ -411,7 +416,7 @@ static hipError_t infer_gather(PtArgs& a, const NrcState& nrc, const uint32_t* c
 // One sample of a plain render: every pixel path whole in one traversal launch (each lane traces, shades and
 // continues its own paths), or the per-vertex wavefront where that launch is not available.
 static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample, hipStream_t s) {
-    if (!g_path_megakernel || !trace_train_supported()) return run_pass(a, L, sample, a.max_depth - 1, s);
+    if (!whole_paths()) return run_pass(a, L, sample, a.max_depth - 1, s);
     TraceRequest tr{};
     tr.tlas = a.tlas;
     tr.mode = 6;
@@ -431,7 +436,7 @@ static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample
 // The pixel paths of one NRC sample up to their cache queries (queued from a.counts[2]): whole paths in one
 // traversal launch, or the per-vertex wavefront where that launch is not available.
 static hipError_t render_cache_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample, hipStream_t s) {
-    if (!g_path_megakernel || !trace_train_supported()) return run_pass(a, L, sample, a.cache_depth, s);
+    if (!whole_paths()) return run_pass(a, L, sample, a.cache_depth, s);
     fill_words_kernel<<<1, 64, 0, s>>>(a.counts, 3ull * (a.max_depth + 2), 0u);
     TraceRequest tr{};
     tr.tlas = a.tlas;
@@ -542,7 +547,7 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         // while the host enqueues the training wavefront's many short ones)
         if ((e = render_cache_sample(a, L, sample_offset + si, s))) return e;
         fill_words_kernel<<<1024, 256, 0, side.stream>>>((uint32_t*)a.rec, nrec * 20, 0u);
-        if (!t.tail && g_train_megakernel && trace_train_supported()) {
+        if (!t.tail && whole_paths()) {
             // the whole training wavefront in one traversal launch (each lane shades and continues its own path)
             TraceRequest tr{};
             tr.tlas = t.tlas;

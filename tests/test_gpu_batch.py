@@ -78,3 +78,19 @@ def test_batch_graph_replays_match_direct_calls(ctx):
         ctx.sync()
         assert torch.isfinite(x).all()
     assert not torch.equal(direct[0], direct[16])
+
+
+def test_batched_whole_paths_match_wavefront(ctx, monkeypatch):
+    """A batched render traces whole paths in one launch per sample; RTX_PATH_WAVEFRONT=1 runs the per-vertex
+    wavefront instead: bit-identical films."""
+    _say("batched render: whole paths vs wavefront")
+    sc, views, lights = _small_batch(4)
+    blases, tlas = ctx.upload_scene(sc, batched=True)
+    monkeypatch.delenv("RTX_PATH_WAVEFRONT", raising=False)
+    whole = ctx.render_batch(tlas, sc.records, views, lights, spp=2, max_depth=6).clone()
+    monkeypatch.setenv("RTX_PATH_WAVEFRONT", "1")
+    wave = ctx.render_batch(tlas, sc.records, views, lights, spp=2, max_depth=6).clone()
+    monkeypatch.delenv("RTX_PATH_WAVEFRONT")
+    ctx.sync()
+    assert float(whole[:, :3].mean()) > 0.01
+    assert torch.equal(whole, wave)

@@ -461,6 +461,7 @@ def _cutout_mixed_scene(pyrtx):
     return sc
 
 
+@pytest.mark.process_flags
 def test_opacity_flags_match_oracle(ctx):
     """Geometry opacity, instance FORCE flags and the ray's OPAQUE / NO_OPAQUE / CULL_*OPAQUE on ray buffers (no
     hit-record table, so no any-hit program runs)."""
@@ -491,6 +492,7 @@ def test_opacity_flags_match_oracle(ctx):
         assert np.all(g[5] == 0xFFFFFFFF)
 
 
+@pytest.mark.process_flags
 def test_any_hit_programs_match_oracle(ctx):
     """Cutout any-hit programs on the film (rtx_cmd_trace_rays with the hit-record table): colours bit-exact
     against the oracle off epsilon-ties (edges, cell boundaries within 2^-16 of the GPU's FP32 barycentrics)."""
@@ -541,6 +543,7 @@ def _procedural_scene(pyrtx, seed=21, instances=12):
     return sc
 
 
+@pytest.mark.process_flags
 def test_procedural_geometry_matches_oracle(ctx):
     """AABB geometry: ray buffers (no hit-record table: box program) and films (sphere / box intersection programs
     from the records) against the oracle; SKIP_AABBS / SKIP_TRIANGLES."""

@@ -73,6 +73,7 @@ def test_cornell_matches_fp64_oracle(ctx):
     assert close.all(-1).mean() > 0.95
 
 
+@pytest.mark.process_flags
 def test_cutout_sphere_matches_fp64_oracle(ctx):
     """The Cornell sphere as non-opaque geometry with a cutout any-hit program (8 cells per edge): extension and
     shadow rays pass through its holes on the GPU as in the oracle."""
@@ -99,6 +100,7 @@ def test_cutout_sphere_matches_fp64_oracle(ctx):
     assert np.abs(g2 - g).max() > 0.1
 
 
+@pytest.mark.process_flags
 def test_procedural_sphere_matches_fp64_oracle(ctx):
     """The Cornell sphere as one AABB primitive with the sphere intersection program (normals from the program)."""
     _say("cornell with a procedural sphere vs oracle")
@@ -137,3 +139,38 @@ def test_render_is_deterministic_and_accumulates_fp64(ctx):
     assert not np.array_equal(f1, f3)
     n = ctx.render_ray_count()
     assert n > 3 * W * H * 4
+
+
+def test_whole_path_launches_match_wavefront(ctx, monkeypatch):
+    """Plain renders (trace mode 6) and NRC frames (modes 5 and 7) trace whole paths in one launch; with
+    RTX_PATH_WAVEFRONT=1 the same calls run the per-vertex wavefront.  Same random numbers, traversal and shading
+    code, and order of radiance additions: the films must be bit-identical."""
+    import pyrtx
+    _say("whole-path launches vs per-vertex wavefront")
+    W, H = 160, 120
+    sc, lights = S.cornell_scene(5, width=W, height=H)
+    blases, tlas = ctx.upload_scene(sc)
+
+    def frames():
+        plain = ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=4, sample_offset=7).clone()
+        lo, hi = S.scene_bounds(sc)
+        nrc = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=4096, cache_vertex=1)
+        acc = torch.zeros((H, W, 3), dtype=torch.float64, device=ctx.device)
+        for fr in range(3):
+            cached = ctx.render_nrc(tlas, sc.records, sc.camera, lights, nrc, W, H, spp=1, train_paths=8192,
+                                    sample_offset=fr, accum64=acc).clone()
+        ctx.sync()
+        steps, loss = nrc.status()
+        nrc.close()
+        return plain.cpu().numpy(), cached.cpu().numpy(), acc.cpu().numpy(), steps, loss
+
+    monkeypatch.delenv("RTX_PATH_WAVEFRONT", raising=False)
+    whole = frames()
+    monkeypatch.setenv("RTX_PATH_WAVEFRONT", "1")
+    wave = frames()
+    monkeypatch.delenv("RTX_PATH_WAVEFRONT")
+    assert whole[0][..., :3].mean() > 0.01 and whole[1][..., :3].mean() > 0.01
+    assert np.array_equal(whole[0], wave[0])
+    assert np.array_equal(whole[1], wave[1])
+    assert np.array_equal(whole[2], wave[2])
+    assert whole[3] == wave[3] and whole[4] == wave[4]
