@@ -68,7 +68,7 @@ __device__ __forceinline__ void load_query(const rtx_nrc_query* src, rtx_nrc_que
 
 // B fragments of the input layer for the lane's sample: k = 32s + 8g + j, i.e. the lane fills the 8-feature chunk
 // c = 4s + g at step s.  Chunks 0..8 are frequency chunks (axis c / 3, frequencies 4(c % 3) .. 4(c % 3) + 3, sin
-// and cos interleaved), so each lane evaluates 4 sin/cos pairs per step with one range reduction each; chunk 9
+// and cos interleaved), so each lane evaluates one sin/cos pair and three angle doublings per step; chunk 9
 // holds normal, direction and albedo[0..1], chunk 10 albedo[2], chunk 11 zeros.
 __device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, int g, bool valid, bf16x8* in0,
                                        __bf16* at0, int col) {
@@ -82,8 +82,22 @@ __device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, i
         if (c < 9) {
             const int a = c / 3, k0 = 4 * (c - 3 * a);
             const float p = a == 0 ? pn[0] : (a == 1 ? pn[1] : pn[2]);
+            // sin / cos of 2^k0 pi p, then three angle doublings: sin 2x = 2 sin x cos x, cos 2x = (cos x - sin x)
+            // (cos x + sin x); each doubling at most doubles the absolute error (a few FP32 ulps after three, far
+            // below the BF16 rounding of the features)
+            float sn, cs;
+            sincospif(ldexpf(p, k0), &sn, &cs);
+            v[0] = sn;
+            v[1] = cs;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) sincospif(ldexpf(p, k0 + i), &v[2 * i], &v[2 * i + 1]);
+            for (int i = 1; i < 4; ++i) {
+                const float s2 = 2.0f * sn * cs;
+                const float c2 = (cs - sn) * (cs + sn);
+                sn = s2;
+                cs = c2;
+                v[2 * i] = sn;
+                v[2 * i + 1] = cs;
+            }
         } else {
             const float c9[8] = {q.normal[0],    q.normal[1],    q.normal[2], q.direction[0],
                                  q.direction[1], q.direction[2], q.albedo[0], q.albedo[1]};
