@@ -18,7 +18,11 @@
 //     per-thread area past the LDS depth; overflow beyond that raises kErrStackOverflow;
 //   * closest hit: t, u, v are recomputed in FP64 (Moller-Trumbore) for the winning triangle;
 //   * the FLAGS variant adds the traceRayEXT ray flags, triangle opacity (geometry / instance / ray) and the
-//     any-hit programs of non-opaque candidates; the host uses it only where they can change a result.
+//     any-hit programs of non-opaque candidates; the host uses it only where they can change a result;
+//   * a lane's ended ray is finished at the wave's next refill point, together with the wave's other ended rays
+//     (result writes and their FP64 refinement run with many lanes instead of one or two); in the whole-path
+//     modes (training paths, pixel paths) finishing means shading the vertex and continuing the lane's path
+//     with its next ray, so one launch traces whole paths.
 #include <atomic>
 #include <cmath>
 
