@@ -568,24 +568,25 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         }
         // training tails (self-training) are predicted with the cache before this frame's Adam steps
         if (t.tail && (e = infer_gather(t, nrc, t.counts + tail_counter(t), a.npix, ntrain, side.stream))) return e;
-        merge_state_kernel<<<1, 64, 0, side.stream>>>(a.state, side.state);
-        if ((e = hipEventRecord(side.join, side.stream))) return e;
-        if ((e = hipStreamWaitEvent(s, side.join, 0))) return e;
-        // the samples and the Adam steps, then the render queries with the updated cache
-        sample_count_kernel<<<(unsigned)nblk, 256, 0, s>>>(a, nrec, block_counts);
-        sample_scan_kernel<<<1, 1024, 0, s>>>(block_counts, (uint32_t)nblk, total);
-        sample_scatter_kernel<<<(unsigned)nblk, 256, 0, s>>>(a, nrec, block_counts, samples);
-        if ((e = hipMemcpyAsync(host_count, total, 4, hipMemcpyDeviceToHost, s))) return e;
-        if ((e = hipStreamSynchronize(s))) return e;
+        // the samples and the Adam steps follow the training paths on the side stream, beside the render paths'
+        // launch if that is still running; then the render queries with the updated cache
+        sample_count_kernel<<<(unsigned)nblk, 256, 0, side.stream>>>(a, nrec, block_counts);
+        sample_scan_kernel<<<1, 1024, 0, side.stream>>>(block_counts, (uint32_t)nblk, total);
+        sample_scatter_kernel<<<(unsigned)nblk, 256, 0, side.stream>>>(a, nrec, block_counts, samples);
+        if ((e = hipMemcpyAsync(host_count, total, 4, hipMemcpyDeviceToHost, side.stream))) return e;
+        if ((e = hipStreamSynchronize(side.stream))) return e;
         const uint32_t count = *host_count;
         if (count > 0) {
             // interleaved batches: batch b takes samples b, b + nb, b + 2 nb, ... (all depths in every batch)
             const uint32_t nb = (count + nrc.batch_size - 1) / nrc.batch_size;
             for (uint32_t b = 0; b < nb; ++b) {
                 const uint32_t nbatch = (count - b + nb - 1) / nb;
-                if ((e = nrc_step(nrc, samples, nbatch, nullptr, nullptr, s, nb, b))) return e;
+                if ((e = nrc_step(nrc, samples, nbatch, nullptr, nullptr, side.stream, nb, b))) return e;
             }
         }
+        merge_state_kernel<<<1, 64, 0, side.stream>>>(a.state, side.state);
+        if ((e = hipEventRecord(side.join, side.stream))) return e;
+        if ((e = hipStreamWaitEvent(s, side.join, 0))) return e;
         if ((e = infer_gather(a, nrc, a.counts + 2, 0, a.npix, s))) return e;
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
