@@ -61,10 +61,9 @@ struct rtx_nrc_t {
 namespace {
 
 constexpr uint32_t kMaxPrims = (1u << 30) - 1u;
-// NRC training wavefront: the render wavefront leaves kSideReserve 256-thread blocks per CU free for it, and its
-// grid has kSideBlocks 256-thread blocks per CU: the blocks beyond the reserve start as render blocks retire (the
-// traversal hands out rays from an atomic counter, so late blocks only take what is left), giving the training
-// chain's latency-bound launches more waves once the render wavefront is done.
+// NRC training paths: the pixel-path launch leaves kSideReserve 256-thread blocks per CU free for them, and their
+// launch has kSideBlocks 256-thread blocks per CU: the blocks beyond the reserve start as pixel-path blocks retire
+// (the traversal hands out paths from an atomic counter, so late blocks only take what is left).
 constexpr uint32_t kSideReserve = 1u;
 constexpr uint32_t kSideBlocks = 2u;
 
@@ -931,7 +930,7 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     }
     rtx::SideChain side{};
     if ((r = side_chain(ctx, L, side))) return r;
-    // the render wavefront leaves kSideReserve 256-thread blocks per CU to the training wavefront
+    // the pixel paths leave kSideReserve 256-thread blocks per CU to the training paths
     const uint32_t reserve = kSideReserve * 256u / L.block_size > 0 ? kSideReserve * 256u / L.block_size : 1u;
     if (L.blocks_per_cu > reserve) {
         L.blocks_per_cu -= reserve;
