@@ -28,9 +28,16 @@ struct NrcState {
 
 uint64_t nrc_fragment_bytes();
 hipError_t nrc_pack(NrcState& st, hipStream_t stream);
+// Optional epilogue of an inference: instead of writing the predictions, add throughput x prediction to the
+// radiance of each query's path (query i belongs to path pid[i]; one query per path).
+struct InferGather {
+    const uint32_t* pid = nullptr;
+    const float4* thr = nullptr;
+    float4* rad = nullptr;
+};
 // radiance predictions for n queries (n_dev, when set, overrides n with a device count bounded by n)
 hipError_t nrc_infer(const NrcState& st, const rtx_nrc_query* queries, uint32_t n, float* out_rgb, hipStream_t stream,
-                     const uint32_t* n_dev = nullptr);
+                     const uint32_t* n_dev = nullptr, const InferGather* gather = nullptr);
 // one Adam step over n samples, taking sample (i * stride + offset) for i < n; with grad_out: gradient only
 hipError_t nrc_step(NrcState& st, const rtx_nrc_sample* samples, uint32_t n, float* grad_out, float* loss_out,
                     hipStream_t stream, uint32_t stride = 1, uint32_t offset = 0);

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void nrc_infer_lds_kernel(const bf16x8* __rest
                                                             const float* __restrict__ params,
                                                             const rtx_nrc_query* __restrict__ queries, uint32_t n,
                                                             const uint32_t* __restrict__ n_dev, Norm nm,
-                                                            float* __restrict__ out_rgb) {
+                                                            float* __restrict__ out_rgb, InferGather gt) {
     __shared__ bf16x8 frag[kInferLdsFrags];
     for (int i = threadIdx.x; i < kInferLdsFrags; i += blockDim.x) frag[i] = fwd[i];
     __syncthreads();
@@ -204,15 +204,25 @@ __global__ __launch_bounds__(256) void nrc_infer_lds_kernel(const bf16x8* __rest
         relu_pack(accB, aB);
         layer_forward2<5>(frag, params, aA, aB, accA, accB, lane_f);
         if (g == 0) {
-            if (vA) {
-                out_rgb[3 * iA] = fmaxf(0.0f, accA[0][0]);
-                out_rgb[3 * iA + 1] = fmaxf(0.0f, accA[0][1]);
-                out_rgb[3 * iA + 2] = fmaxf(0.0f, accA[0][2]);
-            }
-            if (vB) {
-                out_rgb[3 * iB] = fmaxf(0.0f, accB[0][0]);
-                out_rgb[3 * iB + 1] = fmaxf(0.0f, accB[0][1]);
-                out_rgb[3 * iB + 2] = fmaxf(0.0f, accB[0][2]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint64_t i = h ? iB : iA;
+                if (!(h ? vB : vA)) continue;
+                const f32x4& acc = h ? accB[0] : accA[0];
+                const float y0 = fmaxf(0.0f, acc[0]), y1 = fmaxf(0.0f, acc[1]), y2 = fmaxf(0.0f, acc[2]);
+                if (gt.rad) {
+                    const uint32_t pid = gt.pid[i];
+                    const float4 t = gt.thr[pid];
+                    float4 r = gt.rad[pid];
+                    r.x += t.x * y0;
+                    r.y += t.y * y1;
+                    r.z += t.z * y2;
+                    gt.rad[pid] = r;
+                } else {
+                    out_rgb[3 * i] = y0;
+                    out_rgb[3 * i + 1] = y1;
+                    out_rgb[3 * i + 2] = y2;
+                }
             }
         }
     }
@@ -568,14 +578,15 @@ static Norm make_norm(const NrcState& st) {
 }
 
 hipError_t nrc_infer(const NrcState& st, const rtx_nrc_query* q, uint32_t n, float* out, hipStream_t s,
-                     const uint32_t* n_dev) {
+                     const uint32_t* n_dev, const InferGather* gather) {
     if (n == 0) return hipSuccess;
     // persistent LDS-staged kernel: up to 3 blocks per CU (46 KB of fragments each), each wave looping over pairs
     // of 16-query tiles
     const uint64_t pairs = ((uint64_t)n + 31) / 32;
     const uint64_t want = (pairs + 3) / 4;
     const unsigned blocks = (unsigned)std::min<uint64_t>(want, (uint64_t)st.cu_count * 3);
-    nrc_infer_lds_kernel<<<blocks, 256, 0, s>>>((const bf16x8*)st.fwd, st.params, q, n, n_dev, make_norm(st), out);
+    nrc_infer_lds_kernel<<<blocks, 256, 0, s>>>((const bf16x8*)st.fwd, st.params, q, n, n_dev, make_norm(st), out,
+                                                 gather ? *gather : InferGather{});
     return hipGetLastError();
 }
 

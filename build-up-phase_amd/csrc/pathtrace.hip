@@ -171,19 +171,6 @@ __global__ __launch_bounds__(256) void shadow_gather_kernel(PtArgs a, uint32_t d
     a.rad[pid] = r;
 }
 
-// Adds throughput x prediction for the queries [base, base + *count) of the query buffer.
-__global__ __launch_bounds__(256) void cache_gather_kernel(PtArgs a, const uint32_t* count, uint32_t base) {
-    const uint32_t q = base + blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= base + *count) return;
-    const uint32_t pid = a.cq_pid[q];
-    const float4 t = a.thr[pid];
-    float4 r = a.rad[pid];
-    r.x += t.x * a.cq_pred[3 * (uint64_t)q];
-    r.y += t.y * a.cq_pred[3 * (uint64_t)q + 1];
-    r.z += t.z * a.cq_pred[3 * (uint64_t)q + 2];
-    a.rad[pid] = r;
-}
-
 __global__ __launch_bounds__(256) void accumulate_kernel(PtArgs a, double* __restrict__ acc64) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.npix) return;
@@ -404,13 +391,14 @@ static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uin
     return hipGetLastError();
 }
 
-// Cache predictions for one query queue, added to the paths' radiance.
+// Cache predictions for one query queue, added to the paths' radiance by the inference kernel itself.
 static hipError_t infer_gather(PtArgs& a, const NrcState& nrc, const uint32_t* count, uint32_t base, uint32_t cap,
                                hipStream_t s) {
-    hipError_t e = nrc_infer(nrc, a.cq + base, cap, a.cq_pred + 3ull * base, s, count);
-    if (e) return e;
-    cache_gather_kernel<<<(cap + 255) / 256, 256, 0, s>>>(a, count, base);
-    return hipGetLastError();
+    InferGather g;
+    g.pid = a.cq_pid + base;
+    g.thr = a.thr;
+    g.rad = a.rad;
+    return nrc_infer(nrc, a.cq + base, cap, a.cq_pred + 3ull * base, s, count, &g);
 }
 
 // One sample of a plain render: every pixel path whole in one traversal launch (each lane traces, shades and
