@@ -220,6 +220,7 @@ def main():
                 "launch_ms": {"coherent": round(ms_coh, 3), "incoherent": round(ms_inc, 3)}}
 
     cpu = None
+    parity = {}
     if args.cpu_sample > 0:
         import oracle as O
         O.reset()
@@ -234,16 +235,37 @@ def main():
         inc_s = inc[::step][: args.cpu_sample]
         O.trace(ot, coh_s[:4096], threads=cores)  # warm-up
         tc = time.perf_counter()
-        O.trace(ot, coh_s, threads=cores)
-        O.trace(ot, inc_s, threads=cores)
+        o_coh = O.trace(ot, coh_s, threads=cores)
+        o_inc = O.trace(ot, inc_s, threads=cores)
         dt = time.perf_counter() - tc
+        # the same rays on one thread, every 16th of the sample
+        coh_1, inc_1 = coh_s[::16], inc_s[::16]
+        t1 = time.perf_counter()
+        O.trace(ot, coh_1, threads=1)
+        O.trace(ot, inc_1, threads=1)
+        dt1 = time.perf_counter() - t1
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = {"value": round((len(coh_s) + len(inc_s)) / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores,
                "kind": "port",
+               "value_1core": round((len(coh_1) + len(inc_1)) / dt1 / 1e6, 4),
+               "host_cpus": os.cpu_count(), "cpu_model": model,
                "sample": f"{len(coh_s)} coherent + {len(inc_s)} incoherent closest-hit rays (every {step}th ray of "
                          f"each set) traced by the FP64 oracle (scalar C++, BVH2, std::thread x {cores}) in "
-                         f"{dt:.1f}s"}
+                         f"{dt:.1f}s; value_1core: every 16th of them on one thread in {dt1:.1f}s"}
         O.reset()
-        log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads")
+        log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads, {cpu['value_1core']} on one")
+        # full-size parity on the same sample: the GPU's hits from the timed configurations vs the FP64 oracle
+        for name, r, o in (("coherent", rc, o_coh), ("incoherent", ri, o_inc)):
+            trace_cfg(f"closest/{name}", r)
+            ctx.sync()
+            idx = torch.arange(0, W * H, step, device=dev)[: args.cpu_sample]
+            parity[name] = hit_parity(hits[idx].cpu().numpy(), *o)
+        log(f"parity vs the FP64 oracle: {json.dumps(parity)}")
 
     ctx.set_config(**default_cfg)
     render = None if args.no_render else render_bench(ctx, args, np, torch, pyrtx, S)
@@ -275,11 +297,31 @@ def main():
                   "nodes": int(sum(i.node_count for i in info)), "max_depth": int(tinfo.max_depth)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity_vs_oracle": parity or None,
         "path_tracing": render,
         "batched_scenes": batch,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def hit_parity(h, o_tuv, o_ids, o_st, rel_tol=1e-5):
+    """Closest-hit parity of GPU hit records h (int32 [N, 6]: t, u, v bits, prim, geom, inst) against the FP64
+    oracle's (t_uv, ids, status) for the same rays: IDs and hit/miss on every ray the oracle calls robust
+    (status 0 miss, 1 hit; 2 = epsilon-tie, excluded and counted), t relative and u, v absolute errors."""
+    import numpy as np
+    f = np.ascontiguousarray(h[:, :3]).view(np.float32).astype(np.float64)
+    ids = np.ascontiguousarray(h[:, 3:]).view(np.uint32)
+    hit, miss = o_st == 1, o_st == 0
+    id_bad = hit & np.any(ids != o_ids[:, :3], axis=1)
+    ok = hit & ~id_bad
+    t_err = np.abs(f[ok, 0] - o_tuv[ok, 0]) / np.maximum(np.abs(o_tuv[ok, 0]), 1e-30)
+    uv_err = np.abs(f[ok, 1:] - o_tuv[ok, 1:]).max(axis=1) if ok.any() else np.zeros(0)
+    return {"rays": int(len(o_st)), "ties": int((o_st == 2).sum()), "robust_hits": int(hit.sum()),
+            "id_mismatch": int(id_bad.sum()), "miss_mismatch": int((miss & (ids[:, 2] != 0xFFFFFFFF)).sum()),
+            "t_over_tol": int((t_err > rel_tol).sum()), "uv_over_tol": int((uv_err > rel_tol).sum()),
+            "max_t_rel": float(t_err.max()) if t_err.size else 0.0,
+            "max_uv_abs": float(uv_err.max()) if uv_err.size else 0.0}
 
 
 def nrc_kernel_bench(ctx, torch, pyrtx):

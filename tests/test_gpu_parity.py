@@ -329,6 +329,39 @@ def test_query_scene_reduced_parity(ctx):
         assert rep["t_bad"] == 0 and rep["uv_bad"] == 0
 
 
+def test_query_scene_full_size_sample(ctx):
+    """The 19,994,340-triangle scene at full size: both 16,777,216-ray sets traced in full-size launches
+    (closest and any hit), every 256th ray compared with the FP64 oracle on the same scene."""
+    import pyrtx
+    _say("query scene full size (20M triangles, 2 x 16M rays)")
+    sc = S.query_scene()
+    assert sc.unique_triangle_count == 19_994_340
+    blases, tlas = ctx.upload_scene(sc)
+    ctx.sync()
+    t = oracle_scene(sc)
+    W = H = 4096
+    for name, rays in (("coherent", S.camera_rays(sc.camera, W, H)),
+                       ("incoherent", S.incoherent_rays(sc, W * H, seed=1234))):
+        r = torch.from_numpy(rays).cuda()
+        h = ctx.trace_ray_buffer(tlas, r, pyrtx.RTX_QUERY_CLOSEST_HIT)
+        o = ctx.trace_ray_buffer(tlas, r, pyrtx.RTX_QUERY_ANY_HIT)
+        ctx.sync()
+        idx = torch.arange(0, W * H, 256, device=r.device)
+        g = pyrtx.hits_to_numpy(h[idx])
+        occ = o[idx].cpu().numpy().astype(np.uint32)
+        sample = rays[::256]
+        tuv, ids, st = O.trace(t, sample)
+        rep = compare_closest(g, tuv, ids, st)
+        _say(f"  {name}: {rep}")
+        assert rep["robust_hits"] > 1000
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
+        assert rep["t_bad"] == 0 and rep["uv_bad"] == 0
+        assert rep["ties"] < 0.01 * len(sample)
+        _, _, sta = O.trace(t, sample, any_hit=True)
+        assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
+        del r, h, o
+
+
 def test_cull_mask_and_instance_mask(ctx):
     _say("cull mask")
     sc = small_mixed_scene(seed=5, instances=12, freq=8, cells=8)
