@@ -28,6 +28,10 @@ namespace rtx {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ constexpr int in_dim(int l) { return l == 0 ? 96 : 64; }
@@ -112,12 +116,22 @@ __device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, i
     }
 }
 
-// ReLU and repack 4 output tiles into the 2 B fragments of the next layer (k order kperm).
+// ReLU and repack 4 output tiles into the 2 B fragments of the next layer (k order kperm).  The values are
+// rounded to BF16 first and the ReLU is taken on the BF16 bit patterns as 16-bit integers (packed integer max with
+// 0): negative values, -0 included, have the sign bit set and become +0, the others are unchanged, which is
+// bit for bit the rounding of max(0, x).
 __device__ __forceinline__ void relu_pack(const f32x4* acc, bf16x8* out) {
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < 2; ++s) {
+        u32x4 w;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) out[s][j] = (__bf16)fmaxf(0.0f, acc[2 * s + (j >> 2)][j & 3]);
+        for (int q = 0; q < 4; ++q) {
+            const f32x2 f = {acc[2 * s + (q >> 1)][2 * (q & 1)], acc[2 * s + (q >> 1)][2 * (q & 1) + 1]};
+            const i16x2 h = __builtin_bit_cast(i16x2, __builtin_convertvector(f, bf16x2));
+            w[q] = __builtin_bit_cast(unsigned, __builtin_elementwise_max(h, i16x2{0, 0}));
+        }
+        out[s] = __builtin_bit_cast(bf16x8, w);
+    }
 }
 
 // Store the activations held in B fragments (kperm order) into LDS A^T[unit][64 samples].
@@ -141,18 +155,15 @@ __device__ __forceinline__ void layer_forward2(const bf16x8* __restrict__ frag, 
     const int g = lane >> 4;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        f32x4 cA = {0.0f, 0.0f, 0.0f, 0.0f}, cB = {0.0f, 0.0f, 0.0f, 0.0f};
+        // the bias is the accumulator's initial value (no separate adds after the products)
+        f32x4 cA, cB;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) cA[i] = cB[i] = params[b_off(L) + 16 * t + 4 * g + i];
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const bf16x8 a = frag[(fwd_base(L) + t * S + s) * 64 + lane];
             cA = mfma(a, inA[s], cA);
             cB = mfma(a, inB[s], cB);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float b = params[b_off(L) + 16 * t + 4 * g + i];
-            cA[i] += b;
-            cB[i] += b;
         }
         accA[t] = cA;
         accB[t] = cB;
@@ -291,11 +302,12 @@ __device__ __forceinline__ void layer_forward_regs(const bf16x8* f, const float*
     constexpr int T = out_dim(L) / 16, S = in_dim(L) / 32;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        // the bias is the accumulator's initial value, as in inference
+        f32x4 c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) c[i] = b[4 * t + i];
 #pragma unroll
         for (int s = 0; s < S; ++s) c = mfma(f[t * S + s], in[s], c);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) c[i] += b[4 * t + i];
         acc[t] = c;
     }
 }
