@@ -28,8 +28,9 @@ struct NrcState {
 
 uint64_t nrc_fragment_bytes();
 hipError_t nrc_pack(NrcState& st, hipStream_t stream);
-// Optional epilogue of an inference: instead of writing the predictions, add throughput x prediction to the
-// radiance of each query's path (query i belongs to path pid[i]; one query per path).
+// Optional epilogue of an inference: instead of writing the predictions (out_rgb may then be null), add
+// throughput x prediction to the radiance of each query's path (query i belongs to path pid[i]; one query per
+// path).
 struct InferGather {
     const uint32_t* pid = nullptr;
     const float4* thr = nullptr;

@@ -60,7 +60,6 @@ struct PtArgs {
     float* cdf;
     rtx_nrc_query* cq;  // cache queries
     uint32_t* cq_pid;
-    float* cq_pred;
     float4* rec;        // training vertex records, [(d-1) * ntrain + (pid - npix)] x 5 float4
     // batched render (views != nullptr)
     const BatchView* views;

@@ -308,7 +308,7 @@ uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
     b += 2 * align_up((3ull * (max_depth + 2)) * 4, 256);                      // counts (two wavefronts)
     b += align_up((nlights + 1ull) * 4, 256);                                 // light cdf
     b += align_up(n * 4, 256);                                                // path view
-    b += align_up(n * sizeof(rtx_nrc_query), 256) + align_up(n * 4, 256) + align_up(n * 12, 256);  // cache queries
+    b += align_up(n * sizeof(rtx_nrc_query), 256) + align_up(n * 4, 256);  // cache queries
     return b;
 }
 
@@ -338,7 +338,6 @@ void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint
     a.path_view = (uint32_t*)take(n * 4);
     a.cq = (rtx_nrc_query*)take(n * sizeof(rtx_nrc_query));
     a.cq_pid = (uint32_t*)take(n * 4);
-    a.cq_pred = (float*)take(n * 12);
 }
 
 uint64_t pt_train_bytes(uint32_t train_paths, uint32_t record_depths) {
@@ -398,7 +397,7 @@ static hipError_t infer_gather(PtArgs& a, const NrcState& nrc, const uint32_t* c
     g.pid = a.cq_pid + base;
     g.thr = a.thr;
     g.rad = a.rad;
-    return nrc_infer(nrc, a.cq + base, cap, a.cq_pred + 3ull * base, s, count, &g);
+    return nrc_infer(nrc, a.cq + base, cap, nullptr, s, count, &g);
 }
 
 // One sample of a plain render: every pixel path whole in one traversal launch (each lane traces, shades and
