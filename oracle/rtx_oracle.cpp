@@ -358,6 +358,7 @@ struct Collector {
     struct C {
         double t, tunc;
         bool inside, near;
+        uint32_t inst, geom, prim;
     };
     std::vector<C> cands;
 };
@@ -487,7 +488,7 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
             // candidates relevant to [tmin, tmax] (with uncertainty)
             if (c.t + c.tunc < tmin || c.t - c.tunc > tmax) continue;
             if (!c.inside && !c.near) continue;
-            col.cands.push_back({c.t, c.tunc, c.inside, c.near});
+            col.cands.push_back({c.t, c.tunc, c.inside, c.near, inst_id, tr.geom, tr.prim});
             if (c.inside && c.t >= tmin && c.t <= tmax) {
                 bool better = !col.have || c.t < col.t ||
                               (c.t == col.t && (inst_id < col.inst || (inst_id == col.inst &&
@@ -510,10 +511,11 @@ void trace_blas(const Blas& b, const RayFrame& f, double tmin, double tmax, uint
 
 // ray_flags: gl_RayFlags*EXT (facing and opacity culls, SKIP_TRIANGLES); ah: any-hit programs (may be NULL).
 void trace_one(const Tlas& tl, const float* ray, uint32_t cull, int any_hit, HitOut& out, uint32_t ray_flags = 0,
-               const AnyHitTable* ah = nullptr) {
+               const AnyHitTable* ah = nullptr, Collector* keep = nullptr) {
     d3 o{ray[0], ray[1], ray[2]}, d{ray[4], ray[5], ray[6]};
     double tmin = ray[3], tmax = ray[7];
-    Collector col;
+    Collector local;
+    Collector& col = keep ? *keep : local;
     const bool skip_all = ((ray_flags & 0x100u) && (ray_flags & 0x200u)) ||
                           ((ray_flags & 0x40u) && (ray_flags & (0x1u | 0x80u))) ||
                           ((ray_flags & 0x80u) && (ray_flags & 0x2u));
@@ -767,6 +769,46 @@ void oracle_trace(int tlas, const float* rays, int64_t n, int any_hit, uint32_t 
                     ids[4 * i + 3] = h.custom;
                 }
                 if (status) status[i] = h.status;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int k = 1; k < threads; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
+
+// Candidate sets of epsilon-tie rays: every triangle the FP64 test finds inside, or within its FP32 uncertainty
+// of an edge, with t in [tmin, tmax] up to its t-uncertainty - the triangles an FP32 traversal may report.  For ray
+// i: count[i] candidates (all of them, also beyond max_c), the first min(count, max_c) in ascending t with
+// ids[3 (i max_c + k) ..] = inst, geom, prim; t / tunc[i max_c + k]; flags = 1 inside | 2 near an edge.
+void oracle_trace_candidates(int tlas, const float* rays, int64_t n, uint32_t cull_mask, int max_c, uint32_t* count,
+                             uint32_t* ids, double* t, double* tunc, uint32_t* flags, int threads) {
+    const Tlas& tl = *g_tlas[tlas];
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (;;) {
+            int64_t s = next.fetch_add(64);
+            if (s >= n) break;
+            int64_t e = std::min<int64_t>(n, s + 64);
+            for (int64_t i = s; i < e; ++i) {
+                HitOut h;
+                Collector col;
+                trace_one(tl, rays + 8 * i, cull_mask, 0, h, 0u, nullptr, &col);
+                std::sort(col.cands.begin(), col.cands.end(),
+                          [](const Collector::C& a, const Collector::C& b) { return a.t < b.t; });
+                count[i] = (uint32_t)col.cands.size();
+                for (int k = 0; k < max_c && k < (int)col.cands.size(); ++k) {
+                    const Collector::C& c = col.cands[k];
+                    const int64_t j = i * max_c + k;
+                    ids[3 * j] = c.inst;
+                    ids[3 * j + 1] = c.geom;
+                    ids[3 * j + 2] = c.prim;
+                    t[j] = c.t;
+                    tunc[j] = c.tunc;
+                    flags[j] = (c.inside ? 1u : 0u) | (c.near ? 2u : 0u);
+                }
             }
         }
     };
@@ -1476,13 +1518,30 @@ float bf16r(float x) {
     return r;
 }
 
-double sinpi_d(double x) {
+// sin(pi x) and cos(pi x) with the argument reduced exactly to the nearest multiple of 1/2 (x = n/2 + f,
+// |f| <= 1/4), so that multiples of 1/2 give exactly 0 and +-1, as the GPU's sincospif does.
+void sincospi_d(double x, double* s, double* c) {
     double r = std::fmod(x, 2.0);
-    return std::sin(3.14159265358979323846 * r);
+    if (r < 0.0) r += 2.0;
+    const double n = std::nearbyint(2.0 * r);
+    const double f = r - 0.5 * n;  // exact
+    const double sf = std::sin(3.14159265358979323846 * f), cf = std::cos(3.14159265358979323846 * f);
+    switch ((int)n & 3) {
+        case 0: *s = sf; *c = cf; break;
+        case 1: *s = cf; *c = -sf; break;
+        case 2: *s = -sf; *c = -cf; break;
+        default: *s = -cf; *c = sf; break;
+    }
+}
+double sinpi_d(double x) {
+    double s, c;
+    sincospi_d(x, &s, &c);
+    return s;
 }
 double cospi_d(double x) {
-    double r = std::fmod(x, 2.0);
-    return std::cos(3.14159265358979323846 * r);
+    double s, c;
+    sincospi_d(x, &s, &c);
+    return c;
 }
 
 // features of one query (12 floats: position, normal, direction, albedo)
