@@ -8,9 +8,11 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -30,8 +32,9 @@ struct rtx_context_t {
     uint2* gstack = nullptr;
     uint64_t gstack_entries = 0;
     int cu_count = 256;
-    rtx_trace_config cfg{256, 4, 12, RTX_TRACE_RAY_CHUNK_AUTO, 48, 4, 8};
-    uint32_t gdepth = 64;
+    rtx_trace_config cfg{256, 4, 12, RTX_TRACE_RAY_CHUNK_AUTO, 48, 4, 8, 0};
+    uint32_t last_stack = 0;   // stack entries per lane of the last traversal launch (error message)
+    uint32_t render_form = RTX_RENDER_FORM_NONE;
     bool stats = false;
     void* pt_scratch = nullptr;
     uint64_t pt_scratch_bytes = 0;
@@ -150,8 +153,125 @@ int check_geometries(rtx_context ctx, const rtx_blas_geometry* g, uint32_t count
     return RTX_SUCCESS;
 }
 
-int ensure_gstack(rtx_context ctx, uint32_t grid) {
-    uint64_t need = (uint64_t)grid * ctx->cfg.block_size * ctx->gdepth;
+// ---- TLAS properties the host needs before a launch -------------------------------------------------------------
+// The traversal variant (and whether renders can use whole-path launches) follows the traced TLAS's content bits,
+// and the stack size its depth.  Both live in the TLAS header on the device.  Every TLAS build (and AS copy)
+// enqueues a 16-byte read-back of the header words into pinned host memory behind the build and records an event; a
+// launch on that address reads them once the event has completed, waiting for it only if the build is still running
+// (the launch would wait for it on the stream anyway).  Slots are process-wide (a TLAS built by one context can be
+// traced by another) and reused least recently first.
+constexpr uint32_t kPropSlots = 256;
+constexpr uint32_t kPropWords = 4;  // AsHeader words 8..11: max_depth, max_total_depth, build_error, content
+static_assert(offsetof(rtx::AsHeader, max_depth) == 32 && offsetof(rtx::AsHeader, content) == 44, "header words");
+
+struct PropSlot {
+    uint64_t address = 0;
+    uint32_t* host = nullptr;   // kPropWords pinned words
+    hipEvent_t ready = nullptr;
+    int device = -1;
+    uint64_t used = 0;
+};
+std::mutex g_prop_mu;
+PropSlot g_prop[kPropSlots];
+uint32_t* g_prop_host = nullptr;
+uint64_t g_prop_clock = 0;
+
+struct TlasProps {
+    uint32_t content;  // kContent* bits
+    uint32_t depth;    // BVH8 levels, TLAS + deepest BLAS (0: unknown)
+};
+// what a launch assumes when the properties cannot be read without breaking a graph capture: every program kind and
+// the default stack
+constexpr TlasProps kUnknownProps = {rtx::kContentNonOpaque | rtx::kContentAabb, 0};
+
+bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// After enqueuing a build or copy of the structure at `address` on the context stream.
+void note_as_written(rtx_context ctx, uint64_t address) {
+    std::lock_guard<std::mutex> lock(g_prop_mu);
+    PropSlot* slot = nullptr;
+    for (PropSlot& p : g_prop)
+        if (p.address == address) slot = &p;
+    if (stream_capturing(ctx->stream)) {  // no read-back inside a capture: forget the address instead
+        if (slot) slot->address = 0;
+        return;
+    }
+    if (!g_prop_host) {
+        if (hipHostMalloc((void**)&g_prop_host, kPropSlots * kPropWords * sizeof(uint32_t), hipHostMallocDefault) !=
+            hipSuccess) {
+            g_prop_host = nullptr;
+            if (slot) slot->address = 0;
+            return;
+        }
+        for (uint32_t i = 0; i < kPropSlots; ++i) g_prop[i].host = g_prop_host + i * kPropWords;
+    }
+    if (!slot) {
+        slot = &g_prop[0];
+        for (PropSlot& p : g_prop)
+            if (p.used < slot->used) slot = &p;
+    }
+    // a read-back still in flight into this slot finishes before the slot is reused
+    if (slot->ready) (void)hipEventSynchronize(slot->ready);
+    if (slot->ready && slot->device != ctx->device) {
+        (void)hipEventDestroy(slot->ready);
+        slot->ready = nullptr;
+    }
+    if (!slot->ready && hipEventCreateWithFlags(&slot->ready, hipEventDisableTiming) != hipSuccess) {
+        slot->ready = nullptr;
+        slot->address = 0;
+        return;
+    }
+    slot->device = ctx->device;
+    if (hipMemcpyAsync(slot->host, (const uint8_t*)(uintptr_t)address + 32, kPropWords * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipEventRecord(slot->ready, ctx->stream) != hipSuccess) {
+        slot->address = 0;
+        return;
+    }
+    slot->address = address;
+    slot->used = ++g_prop_clock;
+}
+
+// Properties of the TLAS at `address` for a launch on the context stream.
+TlasProps tlas_props(rtx_context ctx, uint64_t address) {
+    const bool capturing = stream_capturing(ctx->stream);
+    {
+        std::lock_guard<std::mutex> lock(g_prop_mu);
+        for (PropSlot& p : g_prop) {
+            if (p.address != address) continue;
+            hipError_t e = hipEventQuery(p.ready);
+            if (e == hipErrorNotReady) {
+                if (capturing) return kUnknownProps;
+                e = hipEventSynchronize(p.ready);
+            }
+            if (e != hipSuccess) break;
+            p.used = ++g_prop_clock;
+            return TlasProps{p.host[3], p.host[1]};
+        }
+    }
+    // not recorded (built inside a capture, or the slot was reused): read the header after the context's work
+    if (capturing) return kUnknownProps;
+    uint32_t w[kPropWords];
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        hipMemcpy(w, (const uint8_t*)(uintptr_t)address + 32, sizeof w, hipMemcpyDeviceToHost) != hipSuccess)
+        return kUnknownProps;
+    return TlasProps{w[3], w[1]};
+}
+
+// Traversal stack entries per lane: 2 per BVH8 level of the deepest TLAS -> BLAS path + 4 (each level's node step
+// pushes at most the rest of its node group and one leaf group; entering a BLAS pushes the TLAS's pending groups),
+// or the configured limit; 76 when the depth is unknown.
+uint32_t stack_entries(rtx_context ctx, uint32_t depth) {
+    if (ctx->cfg.stack_limit) return ctx->cfg.stack_limit;
+    return depth ? 2u * depth + 4u : 76u;
+}
+
+int ensure_gstack(rtx_context ctx, const rtx::TraceLaunch& L) {
+    ctx->last_stack = L.lds_depth + L.gdepth;
+    uint64_t need = (uint64_t)L.grid * L.block_size * L.gdepth;
     if (need <= ctx->gstack_entries) return RTX_SUCCESS;
     if (ctx->gstack) (void)hipFree(ctx->gstack);
     ctx->gstack = nullptr;
@@ -202,13 +322,14 @@ int side_chain(rtx_context ctx, const rtx::TraceLaunch& L, rtx::SideChain& side)
     return RTX_SUCCESS;
 }
 
-rtx::TraceLaunch launch_shape(rtx_context ctx) {
+rtx::TraceLaunch launch_shape(rtx_context ctx, uint32_t depth) {
     rtx::TraceLaunch L;
     L.block_size = ctx->cfg.block_size;
     L.grid = (uint32_t)ctx->cu_count * ctx->cfg.blocks_per_cu;
     L.blocks_per_cu = ctx->cfg.blocks_per_cu;
-    L.lds_depth = ctx->cfg.lds_stack_depth;
-    L.gdepth = ctx->gdepth;
+    const uint32_t total = stack_entries(ctx, depth);
+    L.lds_depth = std::min(ctx->cfg.lds_stack_depth, total);
+    L.gdepth = total - L.lds_depth;
     L.ray_chunk = ctx->cfg.ray_chunk;
     L.refill_lanes = ctx->cfg.refill_lanes;
     L.refill_interval = ctx->cfg.refill_interval;
@@ -289,8 +410,8 @@ int rtx_sync(rtx_context ctx) {
         uint32_t zero = 0;
         (void)hipMemcpy(&ctx->state->error_flags, &zero, 4, hipMemcpyHostToDevice);
         if (flags & rtx::kErrStackOverflow)
-            return fail(ctx, RTX_ERROR_STACK_OVERFLOW, "traversal stack overflow (depth > %u)",
-                        ctx->cfg.lds_stack_depth + ctx->gdepth);
+            return fail(ctx, RTX_ERROR_STACK_OVERFLOW, "traversal stack overflow (more than %u entries per lane)",
+                        ctx->last_stack);
         if (flags & rtx::kErrSortTimeout) return fail(ctx, RTX_ERROR_DEVICE_LOST, "radix sort look-back timed out");
         if (flags & rtx::kErrBuildDepth)
             return fail(ctx, RTX_ERROR_INVALID_AS, "BVH deeper than the builder's collapse level bound");
@@ -362,8 +483,6 @@ int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uin
             d.idx = nullptr;
             d.xform = nullptr;
         }
-        if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
-        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS && x.primitive_count) rtx::note_procedural_geometry();
         d.blas = 0;
         base += x.primitive_count;
     }
@@ -460,8 +579,6 @@ int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, 
             d.idx = nullptr;
             d.xform = nullptr;
         }
-        if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
-        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS && x.primitive_count) rtx::note_procedural_geometry();
             d.blas = b;
             base += x.primitive_count;
         }
@@ -494,6 +611,13 @@ int rtx_cmd_update_blas(rtx_context ctx, const rtx_blas_geometry* geometries, ui
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "the BLAS was not built with RTX_BUILD_ALLOW_UPDATE_BIT");
     if (h.prim_count != n || h.geometry_count != geometry_count)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "an update needs the build's geometry and triangle counts");
+    uint32_t content = 0;
+    for (uint32_t i = 0; i < geometry_count; ++i)
+        if (geometries[i].primitive_count)
+            content |= ((geometries[i].flags & RTX_GEOMETRY_OPAQUE_BIT) ? 0u : rtx::kContentNonOpaque) |
+                       (geometries[i].geometry_type == RTX_GEOMETRY_TYPE_AABBS ? rtx::kContentAabb : 0u);
+    if (content != h.content)  // TLASes built over this BLAS chose their traversal from these bits
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "an update keeps the build's geometry types and opacity flags");
     if (scratch_size < rtx::blas_scratch_bytes((uint32_t)n, false, geometry_count))
         return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "scratch_size below update_scratch_size");
     std::vector<rtx::GeomDesc> gd(geometry_count ? geometry_count : 1);
@@ -518,8 +642,6 @@ int rtx_cmd_update_blas(rtx_context ctx, const rtx_blas_geometry* geometries, ui
             d.idx = nullptr;
             d.xform = nullptr;
         }
-        if (!(x.flags & RTX_GEOMETRY_OPAQUE_BIT) && x.primitive_count) rtx::note_nonopaque_geometry();
-        if (x.geometry_type == RTX_GEOMETRY_TYPE_AABBS && x.primitive_count) rtx::note_procedural_geometry();
         d.blas = 0;
         base += x.primitive_count;
     }
@@ -546,6 +668,7 @@ int rtx_cmd_build_tlas(rtx_context ctx, const rtx_instance* instances_device, ui
     hipError_t e = rtx::build_tlas(instances_device, instance_count, build_flags, dst_device, scratch_device,
                                    &ctx->state->error_flags, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_build_tlas");
+    note_as_written(ctx, (uint64_t)(uintptr_t)dst_device);
     if (out_address) *out_address = (uint64_t)(uintptr_t)dst_device;
     return RTX_SUCCESS;
 }
@@ -571,6 +694,8 @@ int rtx_get_as_info(rtx_context ctx, uint64_t as_address, rtx_as_info* out_info)
         out_info->bounds[3 + a] = h.hi[a];
     }
     out_info->sah_cost = h.sah_cost;
+    out_info->content_flags = h.content;
+    out_info->reserved = 0;
     return RTX_SUCCESS;
 }
 
@@ -613,8 +738,10 @@ int rtx_cmd_trace_rays(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     q.film = film_rgba32f_device;
     q.state = ctx->state;
     q.stats = ctx->stats;
-    rtx::TraceLaunch L = launch_shape(ctx);
-    if ((r = ensure_gstack(ctx, L.grid))) return r;
+    const TlasProps tp = tlas_props(ctx, tlas_address);
+    q.content = tp.content;
+    rtx::TraceLaunch L = launch_shape(ctx, tp.depth);
+    if ((r = ensure_gstack(ctx, L))) return r;
     q.gstack = ctx->gstack;
     hipError_t e = rtx::trace_rays_launch(q, L, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_trace_rays");
@@ -646,8 +773,10 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
     q.occluded = occluded_device;
     q.state = ctx->state;
     q.stats = ctx->stats;
-    rtx::TraceLaunch L = launch_shape(ctx);
-    if ((r = ensure_gstack(ctx, L.grid))) return r;
+    const TlasProps tp = tlas_props(ctx, tlas_address);
+    q.content = tp.content;
+    rtx::TraceLaunch L = launch_shape(ctx, tp.depth);
+    if ((r = ensure_gstack(ctx, L))) return r;
     q.gstack = ctx->gstack;
     hipError_t e = rtx::trace_rays_launch(q, L, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_trace_ray_buffer");
@@ -705,9 +834,12 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
     a.lights = p.light_triangles_device;
     a.nlights = p.light_triangle_count;
     a.state = ctx->state;
-    L = launch_shape(ctx);
-    if ((r = ensure_gstack(ctx, L.grid))) return r;
+    const TlasProps tp = tlas_props(ctx, tlas_address);
+    a.content = tp.content;
+    L = launch_shape(ctx, tp.depth);
+    if ((r = ensure_gstack(ctx, L))) return r;
     a.gstack = ctx->gstack;
+    ctx->render_form = rtx::pt_whole_paths(a) ? RTX_RENDER_FORM_WHOLE_PATH : RTX_RENDER_FORM_WAVEFRONT;
     return RTX_SUCCESS;
 }
 
@@ -749,6 +881,7 @@ int rtx_cmd_copy_as(rtx_context ctx, uint64_t src_address, uint32_t mode, void* 
     hipError_t e = rtx::copy_as((const void*)(uintptr_t)src_address, dst_device, dst_size,
                                 mode == RTX_COPY_MODE_COMPACT, &ctx->state->error_flags, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_copy_as");
+    note_as_written(ctx, (uint64_t)(uintptr_t)dst_device);
     if (out_address) *out_address = (uint64_t)(uintptr_t)dst_device;
     return RTX_SUCCESS;
 }
@@ -850,6 +983,10 @@ int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* 
     key = fnv1a(key, &films_device, sizeof films_device);
     key = fnv1a(key, &accum_rgb_device, sizeof accum_rgb_device);
     key = fnv1a(key, &ctx->cfg, sizeof ctx->cfg);
+    // the launch shape and traversal variant the recorded launches bake in (TLAS content, stack, integrator form)
+    key = fnv1a(key, &L, sizeof L);
+    key = fnv1a(key, &a.content, sizeof a.content);
+    key = fnv1a(key, &ctx->render_form, sizeof ctx->render_form);
     key = fnv1a(key, &ctx->pt_scratch, sizeof ctx->pt_scratch);
     key = fnv1a(key, &ctx->gstack, sizeof ctx->gstack);
     key = fnv1a(key, &ctx->views_buf, sizeof ctx->views_buf);
@@ -952,6 +1089,12 @@ int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays) {
     e = hipMemcpy(&st, ctx->state, sizeof(st), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_get_render_ray_count copy");
     *out_rays = st.pt_rays;
+    return RTX_SUCCESS;
+}
+
+int rtx_get_render_form(rtx_context ctx, uint32_t* out_form) {
+    if (!ctx || !out_form) return RTX_ERROR_INVALID_ARGUMENT;
+    *out_form = ctx->render_form;
     return RTX_SUCCESS;
 }
 
@@ -1130,6 +1273,8 @@ int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg) {
     if (cfg->refill_lanes) c.refill_lanes = cfg->refill_lanes;
     if (cfg->refill_interval) c.refill_interval = cfg->refill_interval;
     if (cfg->leaf_lanes) c.leaf_lanes = cfg->leaf_lanes;
+    if (cfg->stack_limit) c.stack_limit = cfg->stack_limit == RTX_TRACE_STACK_LIMIT_DERIVED ? 0u : cfg->stack_limit;
+    if (c.stack_limit > 4096) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "stack_limit above 4096 entries");
     if (c.leaf_lanes < 1 || c.leaf_lanes > 64)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "leaf_lanes must be in [1, 64]");
     if (c.ray_chunk != RTX_TRACE_RAY_CHUNK_AUTO && (c.ray_chunk < 64 || c.ray_chunk > 4096 || (c.ray_chunk & 63)))
@@ -1372,6 +1517,7 @@ void* rtx_get_proc_addr(const char* name) {
         RTX_PROC(rtx_autotune_render_batch),
         RTX_PROC(rtx_autotune_render),
         RTX_PROC(rtx_get_render_ray_count),
+        RTX_PROC(rtx_get_render_form),
         RTX_PROC(rtx_nrc_create),
         RTX_PROC(rtx_nrc_destroy),
         RTX_PROC(rtx_nrc_cmd_infer),

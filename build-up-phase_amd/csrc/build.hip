@@ -290,9 +290,11 @@ __global__ __launch_bounds__(256) void prep_instances_kernel(const rtx_instance*
                                                              InstanceInfo* __restrict__ table,
                                                              float* __restrict__ boxes,
                                                              uint32_t* __restrict__ cbounds,
-                                                             uint32_t* __restrict__ max_blas_depth) {
+                                                             uint32_t* __restrict__ max_blas_depth,
+                                                             uint32_t* __restrict__ content_out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = false;
+    uint32_t content = 0;
     float cen[3] = {0, 0, 0};
     if (i < n) {
         rtx_instance r = inst[i];
@@ -332,6 +334,7 @@ __global__ __launch_bounds__(256) void prep_instances_kernel(const rtx_instance*
                     cen[a] = s2 * 0.5f;
                 }
                 atomicMax(max_blas_depth, h->max_depth);
+                content = h->content | ((info.flags & RTX_INSTANCE_FORCE_NO_OPAQUE_BIT) ? kContentNonOpaque : 0u);
                 valid = true;
             }
         }
@@ -344,6 +347,10 @@ __global__ __launch_bounds__(256) void prep_instances_kernel(const rtx_instance*
         leaves[i] = s;
         table[i] = info;
     }
+    // one atomic per wave for the content bits
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) content |= __shfl_xor(content, off);
+    if ((threadIdx.x & 63u) == 0 && content) atomicOr(content_out, content);
     reduce_bounds(cen, valid, cbounds);
 }
 
@@ -957,6 +964,7 @@ __global__ void header_kernel(AsHeader* h, uint32_t type, uint32_t flags, uint32
     h->max_depth = 0;
     h->max_total_depth = 0;
     h->build_error = 0;
+    h->content = 0;
     h->slots_offset = kHeaderBytes;
     h->inst_table_offset = table_offset;
     h->capacity_slots = capacity_slots;
@@ -1319,6 +1327,7 @@ __global__ __launch_bounds__(64) void batch_header_kernel(AsHeader* const* __res
     h->max_depth = 0;
     h->max_total_depth = 0;
     h->build_error = 0;
+    h->content = 0;
     h->slots_offset = kHeaderBytes;
     h->inst_table_offset = 0;
     h->capacity_slots = 2ull * (np ? np : 1) + 1;
@@ -1376,7 +1385,23 @@ __global__ void collapse_check_kernel(AsHeader* h, const uint32_t* level_counts,
 }
 
 __global__ void finish_tlas_kernel(AsHeader* h, const uint32_t* max_blas_depth) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) h->max_total_depth = h->max_depth + *max_blas_depth;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        h->max_total_depth = h->max_depth + max_blas_depth[0];
+        h->content = max_blas_depth[1];
+    }
+}
+
+// BLAS content bits from its geometries (one thread per geometry; the header's word was zeroed by the header
+// kernel): into `single`, or into as_ptr[geometry's BLAS] for batched builds.
+__global__ void content_kernel(const GeomDesc* __restrict__ geoms, uint32_t ngeom, AsHeader* single,
+                               AsHeader* const* __restrict__ as_ptr) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngeom) return;
+    const GeomDesc d = geoms[g];
+    if (d.prim_count == 0) return;
+    const uint32_t c = ((d.flags & RTX_GEOMETRY_OPAQUE_BIT) ? 0u : kContentNonOpaque) |
+                       (d.type == RTX_GEOMETRY_TYPE_AABBS ? kContentAabb : 0u);
+    if (c) atomicOr(&(single ? single : as_ptr[d.blas])->content, c);
 }
 __global__ void finish_blas_kernel(AsHeader* h) {
     if (threadIdx.x == 0 && blockIdx.x == 0) h->max_total_depth = h->max_depth;
@@ -1636,6 +1661,7 @@ hipError_t build_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, ui
         prep_tris_kernel<<<(n + 255) / 256, 256, 0, s>>>(gd, (int)ngeom, n, (TriSlot*)(sc + L.leaves),
                                                          (float*)(sc + L.boxes), cb, nullptr);
     if ((err = lbvh_common(n, k64, 0, flags, ngeom, sc, L, (uint8_t*)dst, 0, err_flags, s))) return err;
+    if (ngeom) content_kernel<<<(ngeom + 255) / 256, 256, 0, s>>>(gd, ngeom, (AsHeader*)dst, nullptr);
     finish_blas_kernel<<<1, 64, 0, s>>>((AsHeader*)dst);
     if ((flags & RTX_BUILD_ALLOW_UPDATE_BIT) && n > 0) {
         // keep the final hierarchy (after treelet refinement) and the triangle order for updates
@@ -1706,6 +1732,7 @@ hipError_t update_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, u
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, kMaxLevels32, err_flags);
     }
+    if (ngeom) content_kernel<<<(ngeom + 255) / 256, 256, 0, s>>>(gd, ngeom, h, nullptr);
     finish_blas_kernel<<<1, 64, 0, s>>>(h);
     set_update_offset_kernel<<<1, 64, 0, s>>>(h, off);
     return hipGetLastError();
@@ -1721,11 +1748,11 @@ hipError_t build_tlas(const rtx_instance* inst, uint32_t n, uint32_t flags, void
     uint32_t* maxd = (uint32_t*)(sc + L.misc);
     fill_kernel<<<1, 64, 0, s>>>(cb, 3, 0xFFFFFFFFu);
     fill_kernel<<<1, 64, 0, s>>>(cb + 3, 3, 0u);
-    fill_kernel<<<1, 64, 0, s>>>(maxd, 1, 0u);
+    fill_kernel<<<1, 64, 0, s>>>(maxd, 2, 0u);  // deepest BLAS, content bits
     if (n > 0)
         prep_instances_kernel<<<(n + 255) / 256, 256, 0, s>>>(inst, n, (InstSlot*)(sc + L.leaves),
                                                               (InstanceInfo*)((uint8_t*)dst + table_offset),
-                                                              (float*)(sc + L.boxes), cb, maxd);
+                                                              (float*)(sc + L.boxes), cb, maxd, maxd + 1);
     hipError_t err;
     if ((err = lbvh_common(n, k64, 1, flags, 0, sc, L, (uint8_t*)dst, table_offset, err_flags, s))) return err;
     finish_tlas_kernel<<<1, 64, 0, s>>>((AsHeader*)dst, maxd);
@@ -1793,6 +1820,7 @@ hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t ngeom, const ui
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(nullptr, levels, kMaxLevels32, err_flags);
     }
+    if (ngeom) content_kernel<<<(ngeom + 255) / 256, 256, 0, s>>>(gd, ngeom, nullptr, as_ptr);
     batch_finish_kernel<<<(nbatch + 63) / 64, 64, 0, s>>>(as_ptr, nbatch);
     return hipGetLastError();
 }

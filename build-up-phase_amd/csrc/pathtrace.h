@@ -66,7 +66,13 @@ struct PtArgs {
     uint32_t nviews;
     uint32_t* path_view;       // view of each path slot
     const uint32_t* sample_dev;  // when set, added to every pass's sample index (graph replays)
+    uint32_t content;            // AsHeader::content of the TLAS (traversal variant, whole-path availability)
 };
+
+// Whether the render's samples run as whole-path launches (trace modes 6 / 7, and 5 for full-length training paths):
+// for a TLAS whose content the default traversal variant serves, unless RTX_PATH_WAVEFRONT=1 asks for the per-vertex
+// wavefront (both give bit-identical films).
+bool pt_whole_paths(const PtArgs& a);
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights);
 void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights);

@@ -290,11 +290,9 @@ __global__ __launch_bounds__(256) void sample_scatter_kernel(PtArgs a, uint64_t 
 
 }  // namespace
 
-// Whole-path launches (trace modes 5-7), unless RTX_PATH_WAVEFRONT=1 asks for the per-vertex wavefront (tests
-// compare the two), and only where the default traversal variant serves the process's geometry.
-static bool whole_paths() {
+bool pt_whole_paths(const PtArgs& a) {
     const char* e = std::getenv("RTX_PATH_WAVEFRONT");
-    return !(e && e[0] == '1') && trace_train_supported();
+    return !(e && e[0] == '1') && !trace_needs_flags(0u, a.content);
 }
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
@@ -367,6 +365,7 @@ static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uin
     t.sbt_stride = a.sbt_stride;
     t.state = a.state;
     t.gstack = a.gstack;
+    t.content = a.content;
     t.mode = 3;
     t.n_ptr = a.counts;
     t.rays = a.ext_rays[0];
@@ -403,7 +402,7 @@ static hipError_t infer_gather(PtArgs& a, const NrcState& nrc, const uint32_t* c
 // One sample of a plain render: every pixel path whole in one traversal launch (each lane traces, shades and
 // continues its own paths), or the per-vertex wavefront where that launch is not available.
 static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample, hipStream_t s) {
-    if (!whole_paths()) return run_pass(a, L, sample, a.max_depth - 1, s);
+    if (!pt_whole_paths(a)) return run_pass(a, L, sample, a.max_depth - 1, s);
     TraceRequest tr{};
     tr.tlas = a.tlas;
     tr.mode = 6;
@@ -417,13 +416,14 @@ static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample
     tr.gstack = a.gstack;
     tr.pt = &a;
     tr.pt_sample = sample;
+    tr.content = a.content;
     return trace_rays_launch(tr, L, s);
 }
 
 // The pixel paths of one NRC sample up to their cache queries (queued from a.counts[2]): whole paths in one
 // traversal launch, or the per-vertex wavefront where that launch is not available.
 static hipError_t render_cache_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample, hipStream_t s) {
-    if (!whole_paths()) return run_pass(a, L, sample, a.cache_depth, s);
+    if (!pt_whole_paths(a)) return run_pass(a, L, sample, a.cache_depth, s);
     fill_words_kernel<<<1, 64, 0, s>>>(a.counts, 3ull * (a.max_depth + 2), 0u);
     TraceRequest tr{};
     tr.tlas = a.tlas;
@@ -438,6 +438,7 @@ static hipError_t render_cache_sample(PtArgs& a, const TraceLaunch& L, uint32_t 
     tr.gstack = a.gstack;
     tr.pt = &a;
     tr.pt_sample = sample;
+    tr.content = a.content;
     return trace_rays_launch(tr, L, s);
 }
 
@@ -534,7 +535,7 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         // while the host enqueues the training wavefront's many short ones)
         if ((e = render_cache_sample(a, L, sample_offset + si, s))) return e;
         fill_words_kernel<<<1024, 256, 0, side.stream>>>((uint32_t*)a.rec, nrec * 20, 0u);
-        if (!t.tail && whole_paths()) {
+        if (!t.tail && pt_whole_paths(t)) {
             // the whole training wavefront in one traversal launch (each lane shades and continues its own path)
             TraceRequest tr{};
             tr.tlas = t.tlas;
@@ -549,6 +550,7 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
             tr.gstack = t.gstack;
             tr.pt = &t;
             tr.pt_sample = sample_offset + si;
+            tr.content = t.content;
             if ((e = trace_rays_launch(tr, side.launch, side.stream))) return e;
         } else if ((e = run_pass(t, side.launch, sample_offset + si, t.train_end, side.stream))) {
             return e;

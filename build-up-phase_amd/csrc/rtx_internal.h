@@ -44,7 +44,7 @@ struct AsHeader {
     uint32_t max_depth;        // BVH8 levels (root = 1)
     uint32_t max_total_depth;  // TLAS: own depth + deepest referenced BLAS
     uint32_t build_error;      // non-zero when a build step hit a device-side limit
-    uint32_t pad0;
+    uint32_t content;          // kContent* bits: what rays through this structure can meet
     float lo[4];      // root box
     float hi[4];
     float qscale[4];  // step of the 16-bit node origins
@@ -56,6 +56,11 @@ struct AsHeader {
     uint8_t reserved[256 - 136];
 };
 static_assert(sizeof(AsHeader) == 256, "AsHeader must be 256 B");
+// AsHeader::content.  A BLAS: the union over its geometries with primitives; a TLAS: the union over the BLASes its
+// valid instances reference, plus kContentNonOpaque when an instance has FORCE_NO_OPAQUE.  Traversal of a
+// structure with neither bit runs the opaque-triangle variant.
+constexpr uint32_t kContentNonOpaque = RTX_AS_CONTENT_NON_OPAQUE_BIT;  // geometry without RTX_GEOMETRY_OPAQUE_BIT
+constexpr uint32_t kContentAabb = RTX_AS_CONTENT_AABBS_BIT;            // AABB primitives (intersection programs)
 
 struct Node8 {
     uint16_t pq[3];

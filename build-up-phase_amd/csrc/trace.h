@@ -41,6 +41,7 @@ struct TraceRequest {
     bool stats;
     const PtArgs* pt;  // mode 5: the path tracer's arguments
     uint32_t pt_sample;
+    uint32_t content;  // AsHeader::content of the TLAS (kContent* bits): selects the traversal variant
 };
 
 struct TraceLaunch {
@@ -57,12 +58,9 @@ struct TraceLaunch {
 };
 
 hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch, hipStream_t stream);
-// Modes 5-7 (whole paths in one launch) run the default traversal variant: false once non-opaque or AABB
-// geometry has been built in this process.
-bool trace_train_supported();
-// A BLAS with non-opaque geometry was built: later launches honour opacity and any-hit programs.
-void note_nonopaque_geometry();
-// A BLAS with AABB geometry was built: later launches run intersection programs.
-void note_procedural_geometry();
+// Whether rays with these flags through a TLAS of this content need the FLAGS traversal variant (opacity, any-hit and
+// intersection programs, ray-flag culls); modes 5-7 (whole paths in one launch) exist only in the default variant, so
+// they serve a TLAS for which trace_needs_flags(0, content) is false.
+bool trace_needs_flags(uint32_t ray_flags, uint32_t content);
 
 }  // namespace rtx

@@ -23,7 +23,6 @@
 //     (result writes and their FP64 refinement run with many lanes instead of one or two); in the whole-path
 //     modes (training paths, pixel paths) finishing means shading the vertex and continuing the lane's path
 //     with its next ray, so one launch traces whole paths.
-#include <atomic>
 #include <cmath>
 
 #include "pt_shade.h"
@@ -1237,28 +1236,21 @@ constexpr uint32_t kActiveRayFlags = kRayTerminateOnFirstHit | kRaySkipClosestHi
                                      kRayCullOpaque | kRayCullNoOpaque | kRayNoOpaque | kRaySkipTriangles |
                                      kRaySkipAabbs;
 
-// Non-opaque geometry has been built in this process (set by the BLAS builds, never cleared): from then on rays
-// without the OPAQUE flag take the FLAGS variant, which resolves opacity and runs any-hit programs.
-// Likewise AABB geometry, whose intersection programs only the FLAGS variant runs.
-std::atomic<bool> g_nonopaque_geometry{false};
-std::atomic<bool> g_procedural_geometry{false};
-
-bool needs_flags_variant(uint32_t ray_flags) {
-    return (ray_flags & kActiveRayFlags) || g_procedural_geometry.load(std::memory_order_relaxed) ||
-           (!(ray_flags & kRayOpaque) && g_nonopaque_geometry.load(std::memory_order_relaxed));
-}
-
 template <int MODE, bool STATS>
-hipError_t launch(const TraceArgs& a, const TraceLaunch& L, hipStream_t s) {
-    if (needs_flags_variant(a.ray_flags)) return launch_variant<MODE, STATS, true>(a, L, s);
+hipError_t launch(const TraceArgs& a, const TraceLaunch& L, uint32_t content, hipStream_t s) {
+    if (trace_needs_flags(a.ray_flags, content)) return launch_variant<MODE, STATS, true>(a, L, s);
     return launch_variant<MODE, STATS, false>(a, L, s);
 }
 
 }  // namespace
 
-bool trace_train_supported() { return !needs_flags_variant(0u); }
-void note_nonopaque_geometry() { g_nonopaque_geometry.store(true, std::memory_order_relaxed); }
-void note_procedural_geometry() { g_procedural_geometry.store(true, std::memory_order_relaxed); }
+// The FLAGS variant resolves opacity and runs any-hit programs (needed for non-opaque content unless the ray is
+// OPAQUE), runs intersection programs (AABB content) and applies the ray-flag culls; the default variant treats
+// every primitive as an opaque triangle.
+bool trace_needs_flags(uint32_t ray_flags, uint32_t content) {
+    return (ray_flags & kActiveRayFlags) || (content & kContentAabb) ||
+           (!(ray_flags & kRayOpaque) && (content & kContentNonOpaque));
+}
 
 hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStream_t s) {
     TraceArgs a{};
@@ -1298,7 +1290,7 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     a.ray_flags = r.ray_flags;
     if (r.mode >= 5 && r.mode <= 7) {
         // whole paths: opaque triangle geometry and no ray flags only (the default traversal variant)
-        if (!r.pt || needs_flags_variant(0u)) return hipErrorInvalidValue;
+        if (!r.pt || trace_needs_flags(0u, r.content)) return hipErrorInvalidValue;
         a.pt = *r.pt;
         a.pt_first = r.mode == 5 ? r.pt->npix : 0u;
         a.pt_sample = r.pt_sample;
@@ -1306,14 +1298,16 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
     if (r.n == 0 && !r.n_ptr) return hipGetLastError();
     switch (r.mode) {
-        case 0: return r.stats ? launch<kModeClosest, true>(a, L, s) : launch<kModeClosest, false>(a, L, s);
-        case 1: return r.stats ? launch<kModeAny, true>(a, L, s) : launch<kModeAny, false>(a, L, s);
-        case 3: return launch<kModePath, false>(a, L, s);
-        case 4: return launch<kModeFused, false>(a, L, s);
+        case 0:
+            return r.stats ? launch<kModeClosest, true>(a, L, r.content, s) : launch<kModeClosest, false>(a, L, r.content, s);
+        case 1: return r.stats ? launch<kModeAny, true>(a, L, r.content, s) : launch<kModeAny, false>(a, L, r.content, s);
+        case 3: return launch<kModePath, false>(a, L, r.content, s);
+        case 4: return launch<kModeFused, false>(a, L, r.content, s);
         case 5: return launch_variant<kModeTrain, false, false>(a, L, s);
         case 6: return launch_variant<kModePixelPaths, false, false>(a, L, s);
         case 7: return launch_variant<kModePixelCache, false, false>(a, L, s);
-        default: return r.stats ? launch<kModeRaygen, true>(a, L, s) : launch<kModeRaygen, false>(a, L, s);
+        default:
+            return r.stats ? launch<kModeRaygen, true>(a, L, r.content, s) : launch<kModeRaygen, false>(a, L, r.content, s);
     }
 }
 

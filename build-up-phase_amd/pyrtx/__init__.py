@@ -68,7 +68,12 @@ class BuildSizes(ctypes.Structure):
 class AsInfo(ctypes.Structure):
     _fields_ = [("type", ctypes.c_uint32), ("primitive_count", ctypes.c_uint32), ("node_count", ctypes.c_uint32),
                 ("slot_count", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("geometry_count", ctypes.c_uint32),
-                ("bounds", ctypes.c_float * 6), ("sah_cost", ctypes.c_double)]
+                ("bounds", ctypes.c_float * 6), ("sah_cost", ctypes.c_double), ("content_flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+RTX_AS_CONTENT_NON_OPAQUE_BIT, RTX_AS_CONTENT_AABBS_BIT = 0x1, 0x2
+RTX_RENDER_FORM_NONE, RTX_RENDER_FORM_WHOLE_PATH, RTX_RENDER_FORM_WAVEFRONT = 0, 1, 2
 
 
 class Sbt(ctypes.Structure):
@@ -149,10 +154,11 @@ class TraceConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32),
                 ("lds_stack_depth", ctypes.c_uint32), ("ray_chunk", ctypes.c_uint32),
                 ("refill_lanes", ctypes.c_uint32), ("refill_interval", ctypes.c_uint32),
-                ("leaf_lanes", ctypes.c_uint32)]
+                ("leaf_lanes", ctypes.c_uint32), ("stack_limit", ctypes.c_uint32)]
 
 
 RTX_TRACE_RAY_CHUNK_AUTO = 1
+RTX_TRACE_STACK_LIMIT_DERIVED = 0xFFFFFFFF
 
 
 # symbol -> (restype, argtypes); the exported surface of include/rtx.h
@@ -181,6 +187,7 @@ SIGNATURES = {
     "rtx_cmd_render": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), ctypes.POINTER(Camera),
                                       ctypes.POINTER(RenderParams), _P, _P]),
     "rtx_get_render_ray_count": (ctypes.c_int, [_P, ctypes.POINTER(_U64)]),
+    "rtx_get_render_form": (ctypes.c_int, [_P, ctypes.POINTER(_U32)]),
     "rtx_nrc_create": (ctypes.c_int, [_P, ctypes.POINTER(NrcConfig), ctypes.POINTER(_P)]),
     "rtx_nrc_destroy": (ctypes.c_int, [_P]),
     "rtx_nrc_cmd_infer": (ctypes.c_int, [_P, _P, _U32, _P]),
@@ -412,9 +419,10 @@ class Context:
             out.append(AccelerationStructure(storages[b], addrs[b], n, (scratch, tuple(g))))
         return out
 
-    def build_tlas(self, records, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT, scratch=None) -> AccelerationStructure:
+    def build_tlas(self, records, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT, scratch=None,
+                   into: AccelerationStructure | None = None) -> AccelerationStructure:
         """records: numpy structured array of 64-B instance records (oracle-compatible layout) or a device uint8
-        tensor of n*64 bytes."""
+        tensor of n*64 bytes.  into: an existing TLAS whose storage the build reuses (same address)."""
         torch = _torch()
         if isinstance(records, np.ndarray):
             raw = np.ascontiguousarray(records).view(np.uint8).reshape(-1)
@@ -424,7 +432,10 @@ class Context:
         n = inst.numel() // 64
         sz = BuildSizes()
         self._check(self.lib.rtx_get_tlas_build_sizes(self.handle, n, flags, ctypes.byref(sz)))
-        storage = torch.empty(int(sz.acceleration_structure_size), dtype=torch.uint8, device=self.device)
+        if into is not None and into.storage.numel() >= sz.acceleration_structure_size:
+            storage = into.storage
+        else:
+            storage = torch.empty(int(sz.acceleration_structure_size), dtype=torch.uint8, device=self.device)
         if scratch is None or scratch.numel() < sz.build_scratch_size:
             scratch = torch.empty(int(sz.build_scratch_size), dtype=torch.uint8, device=self.device)
         addr = ctypes.c_uint64()
@@ -599,6 +610,12 @@ class Context:
         self._check(self.lib.rtx_get_render_ray_count(self.handle, ctypes.byref(n)))
         return n.value
 
+    def render_form(self) -> int:
+        """RTX_RENDER_FORM_* of the last render call."""
+        f = ctypes.c_uint32()
+        self._check(self.lib.rtx_get_render_form(self.handle, ctypes.byref(f)))
+        return f.value
+
     def set_stats(self, enable: bool):
         self._check(self.lib.rtx_set_trace_stats(self.handle, int(enable)))
 
@@ -609,9 +626,9 @@ class Context:
                 "instance_entries": s.instance_entries}
 
     def set_config(self, block_size=0, blocks_per_cu=0, lds_stack_depth=0, ray_chunk=0, refill_lanes=0,
-                   refill_interval=0, leaf_lanes=0):
+                   refill_interval=0, leaf_lanes=0, stack_limit=0):
         c = TraceConfig(block_size, blocks_per_cu, lds_stack_depth, ray_chunk, refill_lanes, refill_interval,
-                        leaf_lanes)
+                        leaf_lanes, stack_limit)
         self._check(self.lib.rtx_set_trace_config(self.handle, ctypes.byref(c)))
 
     def config(self) -> dict:
@@ -619,7 +636,7 @@ class Context:
         self._check(self.lib.rtx_get_trace_config(self.handle, ctypes.byref(c)))
         return {"block_size": c.block_size, "blocks_per_cu": c.blocks_per_cu, "lds_stack_depth": c.lds_stack_depth,
                 "ray_chunk": c.ray_chunk, "refill_lanes": c.refill_lanes, "refill_interval": c.refill_interval,
-                "leaf_lanes": c.leaf_lanes}
+                "leaf_lanes": c.leaf_lanes, "stack_limit": c.stack_limit}
 
     def autotune(self, tlas: AccelerationStructure, rays, mode=RTX_QUERY_CLOSEST_HIT, reps=2):
         """rtx_autotune_trace on a device ray tensor [N, 8] float32; the best configuration stays active.

@@ -191,7 +191,15 @@ typedef struct rtx_as_info {
     float bounds[6];               /* lo xyz, hi xyz */
     double sah_cost;               /* SAH cost of the binary hierarchy before collapse: sum of internal-node
                                       surface areas / root surface area (0 for fewer than 2 primitives) */
+    uint32_t content_flags;        /* RTX_AS_CONTENT_* bits */
+    uint32_t reserved;
 } rtx_as_info;
+/* What rays through a structure can meet, recorded at build time.  A BLAS: its geometries with primitives; a TLAS:
+ * the BLASes its instances reference, and NON_OPAQUE as well when an instance has RTX_INSTANCE_FORCE_NO_OPAQUE_BIT.
+ * Traces and renders of a TLAS with neither bit run the opaque-triangle traversal (and whole-path renders); the
+ * others resolve opacity, any-hit and intersection programs.  The choice is made per call from the traced TLAS. */
+#define RTX_AS_CONTENT_NON_OPAQUE_BIT 0x1u  /* geometry built without RTX_GEOMETRY_OPAQUE_BIT */
+#define RTX_AS_CONTENT_AABBS_BIT 0x2u       /* AABB geometry */
 int rtx_get_as_info(rtx_context ctx, uint64_t as_address, rtx_as_info* out_info);
 
 /* vkCmdWriteAccelerationStructuresPropertiesKHR(COMPACTED_SIZE) + vkCmdCopyAccelerationStructureKHR (not used by
@@ -375,7 +383,13 @@ typedef struct rtx_trace_config {
     uint32_t refill_interval; /* traversal steps between two refill checks (1..64) */
     uint32_t leaf_lanes;      /* postponed leaves: a wave runs its leaf step (BLAS entry, triangle tests) once this
                                  many lanes hold a pending leaf group or no lane can traverse (1..64; 1 = at once) */
+    uint32_t stack_limit;     /* traversal stack entries per lane, LDS and global spill together: 0 (default) sizes the
+                                 stack from the traced TLAS's depth (2 entries per BVH level + 4, enough for every
+                                 ray); with a smaller value a ray that needs more ends there (reporting the closest
+                                 hit found so far, or a miss) and rtx_sync reports RTX_ERROR_STACK_OVERFLOW.  Not an autotuner knob; rtx_set_trace_config keeps it
+                                 when the field is 0, so RTX_TRACE_STACK_LIMIT_DERIVED restores the default. */
 } rtx_trace_config;
+#define RTX_TRACE_STACK_LIMIT_DERIVED 0xFFFFFFFFu
 #define RTX_TRACE_RAY_CHUNK_AUTO 1u
 int rtx_set_trace_config(rtx_context ctx, const rtx_trace_config* cfg);
 int rtx_get_trace_config(rtx_context ctx, rtx_trace_config* out_cfg);
@@ -472,6 +486,15 @@ int rtx_autotune_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
 
 int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays);
 
+/* Form of the integrator the last rtx_cmd_render / _batch / _nrc call of the context enqueued (diagnostic; both forms
+ * give bit-identical films): WHOLE_PATH = each sample's paths traced, shaded and continued in one traversal launch;
+ * WAVEFRONT = per-vertex launches (shade, fused shadow + extension trace, gather), used for TLASes whose content
+ * needs opacity or intersection programs, or when the environment variable RTX_PATH_WAVEFRONT=1 asks for it. */
+#define RTX_RENDER_FORM_NONE 0u
+#define RTX_RENDER_FORM_WHOLE_PATH 1u
+#define RTX_RENDER_FORM_WAVEFRONT 2u
+int rtx_get_render_form(rtx_context ctx, uint32_t* out_form);
+
 /* ---- neural radiance cache (Muller et al. 2021) --------------------------------------------------------------
  *
  * MLP: 96 inputs (81 used) -> 5 hidden layers of 64 (ReLU) -> 16 outputs (RGB in the first 3), BF16 weights and
@@ -531,8 +554,9 @@ int rtx_nrc_get_status(rtx_nrc nrc, uint32_t* steps, float* last_loss);
  * passes also carries train_paths training paths from random pixels (sample index params->sample_offset) through
  * the same queues: each gives one training sample per vertex after the first (target: the path's radiance from
  * that vertex on), either at full length or, with cfg->train_suffix > 0, ending in a cache query of their own
- * (self-training).  The predictions come from the cache before this frame's update; then one Adam step per
- * batch_size samples.  film / accum as in rtx_cmd_render. */
+ * (self-training; those tail queries are answered by the cache as it was before this frame).  Order within the
+ * frame: the training paths' samples take one Adam step per batch_size samples (interleaved batches), and then the
+ * pixel paths' cache queries of every pass are answered by the updated cache.  film / accum as in rtx_cmd_render. */
 int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
                        const rtx_render_params* params, rtx_nrc nrc, uint32_t train_paths,
                        float* film_rgba32f_device, double* accum_rgb_device);

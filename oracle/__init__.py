@@ -41,6 +41,9 @@ def lib():
         L.oracle_trace.restype = None
         L.oracle_trace.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, P,
                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P, P, P, ctypes.c_int]
+        L.oracle_trace_candidates.restype = None
+        L.oracle_trace_candidates.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int, P, P, P,
+                                              P, P, ctypes.c_int]
         L.oracle_trace_brute.restype = None
         L.oracle_trace_brute.argtypes = [ctypes.c_int, P, ctypes.c_int64, ctypes.c_uint32, P, P, P]
         L.oracle_aspect.restype = None
@@ -167,6 +170,65 @@ def trace(tlas: int, rays: np.ndarray, any_hit: bool = False, cull_mask: int = 0
                        0 if rec is None else len(rec), sbt_offset, sbt_stride, _ptr(t_uv), _ptr(ids), _ptr(st),
                        threads)
     return t_uv, ids, st
+
+
+def trace_candidates(tlas: int, rays: np.ndarray, cull_mask: int = 0xFF, max_c: int = 16, threads: int = 0):
+    """Candidate sets of (epsilon-tie) rays: every triangle an FP32 traversal may report for each ray (FP64 inside,
+    or within its FP32 uncertainty of an edge; t in the ray's window up to its uncertainty), ascending t.  Returns
+    dict(count [n], ids [n, max_c, 3] (inst, geom, prim), t, tunc [n, max_c], inside, near [n, max_c] bool)."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    count = np.zeros(n, np.uint32)
+    ids = np.zeros((n, max_c, 3), np.uint32)
+    t = np.zeros((n, max_c), np.float64)
+    tunc = np.zeros((n, max_c), np.float64)
+    fl = np.zeros((n, max_c), np.uint32)
+    lib().oracle_trace_candidates(tlas, _ptr(rays), n, cull_mask, max_c, _ptr(count), _ptr(ids), _ptr(t), _ptr(tunc),
+                                  _ptr(fl), threads)
+    return {"count": count, "ids": ids, "t": t, "tunc": tunc, "inside": (fl & 1) != 0, "near": (fl & 2) != 0,
+            "max_c": max_c}
+
+
+def check_ties(rays: np.ndarray, cand: dict, gpu_inst, gpu_geom, gpu_prim, gpu_t, occluded=None) -> dict:
+    """What an FP32 traversal may answer on rays whose FP64 answer is not robust (epsilon-ties), given their candidate
+    sets (trace_candidates):
+      - closest hit: a hit must be a candidate that can be the closest one within the uncertainties (t - tunc at
+        most the smallest t + tunc of the FP64-inside candidates in the window) with its t within tunc + 1e-5 t of
+        the candidate's; a miss is allowed only when no candidate is robustly inside (inside, not near an edge, t
+        clear of tmin / tmax by its uncertainty);
+      - any hit (occluded given): 1 needs some candidate, 0 needs no robustly inside one.
+    Returns counts: rays, hit_not_candidate, miss_not_allowed, t_bad, unchecked (candidate list truncated),
+    occluded_bad."""
+    rays = np.asarray(rays, np.float32).reshape(-1, 8)
+    n = rays.shape[0]
+    mc = cand["max_c"]
+    cnt = cand["count"].astype(np.int64)
+    valid = np.arange(mc)[None, :] < np.minimum(cnt, mc)[:, None]
+    tmin = rays[:, 3].astype(np.float64)[:, None]
+    tmax = rays[:, 7].astype(np.float64)[:, None]
+    t, tu, ins, near = cand["t"], cand["tunc"], cand["inside"] & valid, cand["near"] & valid
+    in_win = ins & (t >= tmin) & (t <= tmax)
+    tstar = np.where(in_win, t + tu, np.inf).min(1)
+    admissible = valid & (t - tu <= tstar[:, None])
+    robust = ins & ~near & (t - tu >= tmin) & (t + tu <= tmax)
+    must_hit = robust.any(1)
+    truncated = cnt > mc
+    out = {"rays": int(n), "unchecked": int(truncated.sum())}
+    if occluded is not None:
+        occ = np.asarray(occluded).astype(bool)
+        bad = (occ & (cnt == 0)) | (~occ & must_hit)
+        out["occluded_bad"] = int((bad & ~truncated).sum())
+        return out
+    miss = np.asarray(gpu_inst, np.uint32) == 0xFFFFFFFF
+    same = admissible & (cand["ids"][:, :, 0] == np.asarray(gpu_inst, np.uint32)[:, None]) & \
+        (cand["ids"][:, :, 1] == np.asarray(gpu_geom, np.uint32)[:, None]) & \
+        (cand["ids"][:, :, 2] == np.asarray(gpu_prim, np.uint32)[:, None])
+    tg = np.asarray(gpu_t, np.float64)[:, None]
+    t_ok = same & (np.abs(tg - t) <= tu + 1e-5 * np.abs(t))
+    out["miss_not_allowed"] = int((miss & must_hit & ~truncated).sum())
+    out["hit_not_candidate"] = int((~miss & ~same.any(1) & ~truncated).sum())
+    out["t_bad"] = int((~miss & same.any(1) & ~t_ok.any(1)).sum())
+    return out
 
 
 def trace_brute(tlas: int, rays: np.ndarray, cull_mask: int = 0xFF):

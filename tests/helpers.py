@@ -21,8 +21,10 @@ def oracle_scene(scene):
     return O.tlas_create(rec)
 
 
-def compare_closest(gpu_hits, o_tuv, o_ids, o_st, rel_tol=1e-5):
-    """gpu_hits: (t, u, v, prim, geom, inst) numpy arrays.  Returns a dict of mismatch counts on robust rays."""
+def compare_closest(gpu_hits, o_tuv, o_ids, o_st, rel_tol=1e-5, tlas=None, rays=None):
+    """gpu_hits: (t, u, v, prim, geom, inst) numpy arrays.  Returns a dict of mismatch counts on robust rays; with
+    the oracle TLAS and the rays, the epsilon-tie rays are checked against their candidate sets too (tie_* keys:
+    a hit outside the set, a miss where a candidate is robustly inside, a t off its candidate's)."""
     t, u, v, prim, geom, inst = gpu_hits
     robust_hit = o_st == 1
     robust_miss = o_st == 0
@@ -32,7 +34,14 @@ def compare_closest(gpu_hits, o_tuv, o_ids, o_st, rel_tol=1e-5):
     t_err = np.abs(t[ok].astype(np.float64) - o_tuv[ok, 0]) / np.maximum(np.abs(o_tuv[ok, 0]), 1e-30)
     u_err = np.abs(u[ok].astype(np.float64) - o_tuv[ok, 1])
     v_err = np.abs(v[ok].astype(np.float64) - o_tuv[ok, 2])
+    tie_rep = {}
+    if tlas is not None and (o_st == 2).any():
+        tie = o_st == 2
+        r = np.asarray(rays, np.float32).reshape(-1, 8)[tie]
+        c = O.trace_candidates(tlas, r)
+        tie_rep = {"tie_" + k: v for k, v in O.check_ties(r, c, inst[tie], geom[tie], prim[tie], t[tie]).items()}
     return {
+        **tie_rep,
         "robust_hits": int(robust_hit.sum()),
         "robust_misses": int(robust_miss.sum()),
         "ties": int((o_st == 2).sum()),
@@ -43,6 +52,12 @@ def compare_closest(gpu_hits, o_tuv, o_ids, o_st, rel_tol=1e-5):
         "max_t_rel": float(t_err.max()) if t_err.size else 0.0,
         "max_uv_abs": float(max(u_err.max(), v_err.max())) if u_err.size else 0.0,
     }
+
+
+def assert_ties_ok(rep):
+    """No epsilon-tie ray answered outside its candidate set (when compare_closest checked them)."""
+    for k in ("tie_hit_not_candidate", "tie_miss_not_allowed", "tie_t_bad"):
+        assert rep.get(k, 0) == 0, rep
 
 
 def small_mixed_scene(seed=7, instances=24, freq=12, cells=16):

@@ -55,13 +55,14 @@ def test_gradient_matches_oracle(ctx):
     _, og, oloss = O.nrc(p, smp, np.array(LO, np.float32), np.array(HI, np.float32), bf16=True)
     rel = np.linalg.norm(g - og) / np.linalg.norm(og)
     _say(f"  |g - g_oracle| / |g_oracle| = {rel:.3g}, loss {loss:.6f} vs {oloss:.6f}")
-    assert rel < 1e-2
-    assert abs(loss - oloss) / oloss < 1e-3
+    assert rel < 1e-3  # measured ~2e-5
+    assert abs(loss - oloss) / oloss < 1e-4
     # per layer
     offs = [0, 6208, 10368, 14528, 18688, 22848, 23888]
     for l in range(6):
         a, b = g[offs[l]:offs[l + 1]], og[offs[l]:offs[l + 1]]
-        assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(b) + 1e-9, l
+        _say(f"  layer {l}: {np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30):.3g}")
+        assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b) + 1e-9, l
     nrc.close()
 
 
@@ -78,8 +79,71 @@ def test_inference_matches_oracle(ctx):
     err = np.abs(y - oy).max()
     scale = np.abs(oy).max()
     _say(f"  max |y - y_oracle| = {err:.3g} (scale {scale:.3g})")
-    assert err <= 1e-3 * max(scale, 1.0)
+    assert err <= 1e-4 * max(scale, 1.0)
     nrc.close()
+
+
+def test_inference_encoding_bit_exact(ctx):
+    """Every input feature of the GPU encoding, read through selector weights (a network that copies feature f to
+    its outputs: out0 = max(0, f), out1 = max(0, -f)), equals the oracle's (closed-form-pinned) value exactly, at
+    positions whose normalised coordinates are 0, 1/4, 1/2 and 1."""
+    import pyrtx
+    from test_oracle_pins import _selector
+    _say("nrc encoding, feature by feature")
+    lo = np.array([-2.0, -2.0, -2.0], np.float32)
+    hi = np.array([2.0, 2.0, 2.0], np.float32)
+    smp = np.zeros((12, 16), np.float32)
+    for a in range(3):
+        for i, x in enumerate((-2.0, -1.0, 0.0, 2.0)):
+            smp[4 * a + i, a] = x
+    smp[:, 3:12] = np.arange(1, 10, dtype=np.float32) * np.float32(0.0625) - np.float32(0.3)
+    q = torch.from_numpy(np.ascontiguousarray(smp[:, :12])).cuda()
+    nrc = pyrtx.Nrc(ctx, tuple(lo), tuple(hi), seed=1, batch_size=64)
+    try:
+        for f in range(96):
+            p = _selector(f)
+            nrc.set_params(p)
+            y = nrc.infer(q).cpu().numpy()
+            oy, _, _ = O.nrc(p, smp, lo, hi, bf16=True, want_grad=False)
+            assert np.array_equal(y[:, 0], np.maximum(oy[:, 0], 0.0).astype(np.float32)), f
+            assert np.array_equal(y[:, 1], np.maximum(oy[:, 1], 0.0).astype(np.float32)), f
+    finally:
+        nrc.close()
+
+
+def test_render_nrc_frame_answers_queries_with_the_updated_cache(ctx):
+    """Order within an NRC frame: the training paths' Adam steps come first, then the pixel paths' cache queries
+    are answered with the updated parameters.  The frame's film equals a frame without training rendered with the
+    parameters after the update (same sample index), and differs from one with the parameters before it."""
+    import pyrtx
+    from pyrtx import scenes as S
+    _say("nrc frame order")
+    W, H = 64, 48
+    sc, lights = S.cornell_scene(3, width=W, height=H)
+    blases, tlas = ctx.upload_scene(sc)
+    lo, hi = S.scene_bounds(sc)
+    nrc = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=4096, cache_vertex=1)
+    other = pyrtx.Nrc(ctx, lo, hi, seed=2, batch_size=4096, cache_vertex=1)
+    try:
+        for fr in range(2):
+            ctx.render_nrc(tlas, sc.records, sc.camera, lights, nrc, W, H, spp=1, train_paths=8192, sample_offset=fr)
+        before = nrc.params()
+        f1 = ctx.render_nrc(tlas, sc.records, sc.camera, lights, nrc, W, H, spp=1, train_paths=8192,
+                            sample_offset=5).clone()
+        after = nrc.params()
+        assert not np.array_equal(before, after)
+        other.set_params(after)
+        f_after = ctx.render_nrc(tlas, sc.records, sc.camera, lights, other, W, H, spp=1, train_paths=0,
+                                 sample_offset=5).clone()
+        other.set_params(before)
+        f_before = ctx.render_nrc(tlas, sc.records, sc.camera, lights, other, W, H, spp=1, train_paths=0,
+                                  sample_offset=5).clone()
+        ctx.sync()
+    finally:
+        nrc.close()
+        other.close()
+    assert torch.equal(f1, f_after)
+    assert not torch.equal(f1, f_before)
 
 
 def test_adam_step_matches_restatement(ctx):

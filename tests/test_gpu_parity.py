@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import compare_closest, oracle_scene, small_mixed_scene
+from helpers import assert_ties_ok, compare_closest, oracle_scene, small_mixed_scene
 import pyrtx
 from pyrtx import scenes as S
 
@@ -222,12 +222,13 @@ def test_mixed_scene_closest_and_any_hit(ctx, seed):
     for name, rays in (("coherent", coh), ("incoherent", inc)):
         g = _trace_closest(ctx, tlas, rays)
         tuv, ids, st = O.trace(t, rays)
-        rep = compare_closest(g, tuv, ids, st)
+        rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
         _say(f"  {name}: {rep}")
         assert rep["robust_hits"] > 100
         assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
         assert rep["t_bad"] == 0 and rep["uv_bad"] == 0
         assert rep["ties"] < 0.01 * len(rays)
+        assert_ties_ok(rep)
         occ = _trace_any(ctx, tlas, rays)
         _, _, sta = O.trace(t, rays, any_hit=True)
         assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
@@ -323,10 +324,11 @@ def test_query_scene_reduced_parity(ctx):
     for name, rays in (("coherent", coh), ("incoherent", inc)):
         g = _trace_closest(ctx, tlas, rays)
         tuv, ids, st = O.trace(t, rays)
-        rep = compare_closest(g, tuv, ids, st)
+        rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
         _say(f"  {name}: {rep}")
         assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
         assert rep["t_bad"] == 0 and rep["uv_bad"] == 0
+        assert_ties_ok(rep)
 
 
 def test_query_scene_full_size_sample(ctx):
@@ -351,12 +353,13 @@ def test_query_scene_full_size_sample(ctx):
         occ = o[idx].cpu().numpy().astype(np.uint32)
         sample = rays[::256]
         tuv, ids, st = O.trace(t, sample)
-        rep = compare_closest(g, tuv, ids, st)
+        rep = compare_closest(g, tuv, ids, st, tlas=t, rays=sample)
         _say(f"  {name}: {rep}")
         assert rep["robust_hits"] > 1000
         assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0
         assert rep["t_bad"] == 0 and rep["uv_bad"] == 0
         assert rep["ties"] < 0.01 * len(sample)
+        assert_ties_ok(rep)
         _, _, sta = O.trace(t, sample, any_hit=True)
         assert np.all(occ[sta == 1] == 1) and np.all(occ[sta == 0] == 0)
         del r, h, o
@@ -494,7 +497,6 @@ def _cutout_mixed_scene(pyrtx):
     return sc
 
 
-@pytest.mark.process_flags
 def test_opacity_flags_match_oracle(ctx):
     """Geometry opacity, instance FORCE flags and the ray's OPAQUE / NO_OPAQUE / CULL_*OPAQUE on ray buffers (no
     hit-record table, so no any-hit program runs)."""
@@ -525,7 +527,6 @@ def test_opacity_flags_match_oracle(ctx):
         assert np.all(g[5] == 0xFFFFFFFF)
 
 
-@pytest.mark.process_flags
 def test_any_hit_programs_match_oracle(ctx):
     """Cutout any-hit programs on the film (rtx_cmd_trace_rays with the hit-record table): colours bit-exact
     against the oracle off epsilon-ties (edges, cell boundaries within 2^-16 of the GPU's FP32 barycentrics)."""
@@ -576,7 +577,6 @@ def _procedural_scene(pyrtx, seed=21, instances=12):
     return sc
 
 
-@pytest.mark.process_flags
 def test_procedural_geometry_matches_oracle(ctx):
     """AABB geometry: ray buffers (no hit-record table: box program) and films (sphere / box intersection programs
     from the records) against the oracle; SKIP_AABBS / SKIP_TRIANGLES."""

@@ -56,24 +56,33 @@ def test_furnace_closed_form(ctx, rr_depth):
     assert np.all(f[..., 3] == 1.0)
 
 
-def test_cornell_matches_fp64_oracle(ctx):
-    _say("cornell vs oracle")
+def _check_vs_oracle(g, o, max_relmse, min_close):
+    """The GPU film against the oracle's estimator fed the same random numbers: relMSE and the fraction of pixels
+    within 1e-3 (relative above 1).  Bounds are about ten times what is measured, so a regression that touches a
+    few pixels fails."""
+    err = relmse(g, o)
+    close = (np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o))).all(-1).mean()
+    _say(f"  relMSE(gpu, oracle) = {err:.3g}, pixels within 1e-3: {close:.6f} ({g.shape[0] * g.shape[1]} pixels)")
+    assert err < max_relmse
+    assert close >= min_close
+
+
+@pytest.mark.parametrize("level", [4, 8])
+def test_cornell_matches_fp64_oracle(ctx, level):
+    """Configuration 2's scene (level 8: the 1.31M-triangle displaced sphere) at a film the oracle renders in
+    seconds."""
+    _say(f"cornell (sphere level {level}) vs oracle")
     W, H, spp = 96, 54, 16
-    sc, lights = S.cornell_scene(4, width=W, height=H)
+    sc, lights = S.cornell_scene(level, width=W, height=H)
     blases, tlas = ctx.upload_scene(sc)
     film = ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=spp)
     ctx.sync()
     g = film.cpu().numpy()[..., :3].astype(np.float64)
     t = oracle_scene(sc)
     o = O.render(t, sc.records, sc.camera, lights, W, H, spp) / spp
-    err = relmse(g, o)
-    close = np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o))
-    _say(f"  relMSE(gpu, oracle) = {err:.3g}, pixels within 1e-3: {close.all(-1).mean():.4f}")
-    assert err < 1e-3
-    assert close.all(-1).mean() > 0.95
+    _check_vs_oracle(g, o, 1e-8, 0.9999)
 
 
-@pytest.mark.process_flags
 def test_cutout_sphere_matches_fp64_oracle(ctx):
     """The Cornell sphere as non-opaque geometry with a cutout any-hit program (8 cells per edge): extension and
     shadow rays pass through its holes on the GPU as in the oracle."""
@@ -88,11 +97,9 @@ def test_cutout_sphere_matches_fp64_oracle(ctx):
     g = film.cpu().numpy()[..., :3].astype(np.float64)
     t = oracle_scene(sc)
     o = O.render(t, sc.records, sc.camera, lights, W, H, spp) / spp
-    err = relmse(g, o)
-    close = np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o))
-    _say(f"  relMSE(gpu, oracle) = {err:.3g}, pixels within 1e-3: {close.all(-1).mean():.4f}")
-    assert err < 1e-3
-    assert close.all(-1).mean() > 0.95
+    # cutout cell boundaries: the GPU's any-hit program decides on FP32 barycentrics, the oracle's on FP64, so
+    # paths that graze a boundary can differ
+    _check_vs_oracle(g, o, 2e-5, 0.999)
     # the holes matter: the opaque sphere renders differently
     sc.blas[1][0].opaque = True
     blases2, tlas2 = ctx.upload_scene(sc)
@@ -100,7 +107,6 @@ def test_cutout_sphere_matches_fp64_oracle(ctx):
     assert np.abs(g2 - g).max() > 0.1
 
 
-@pytest.mark.process_flags
 def test_procedural_sphere_matches_fp64_oracle(ctx):
     """The Cornell sphere as one AABB primitive with the sphere intersection program (normals from the program)."""
     _say("cornell with a procedural sphere vs oracle")
@@ -115,11 +121,7 @@ def test_procedural_sphere_matches_fp64_oracle(ctx):
     g = film.cpu().numpy()[..., :3].astype(np.float64)
     t = oracle_scene(sc)
     o = O.render(t, sc.records, sc.camera, lights, W, H, spp) / spp
-    err = relmse(g, o)
-    close = np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o))
-    _say(f"  relMSE(gpu, oracle) = {err:.3g}, pixels within 1e-3: {close.all(-1).mean():.4f}")
-    assert err < 1e-3
-    assert close.all(-1).mean() > 0.95
+    _check_vs_oracle(g, o, 1e-8, 0.9999)
 
 
 def test_render_is_deterministic_and_accumulates_fp64(ctx):
