@@ -385,3 +385,99 @@ def adam(params, m, v, grad, lr, b1, b2, eps, step):
     """In-place Adam step on float32 arrays."""
     lib().oracle_adam(_ptr(params), _ptr(m), _ptr(v), _ptr(np.ascontiguousarray(grad, np.float64)), params.size,
                       lr, b1, b2, eps, step)
+
+
+# ---- CPU baseline (libcpubase.so): the reference's ray tracing on CPU cores, FP32, timed by bench.py ----------------
+_CB_PATH = os.path.join(_HERE, "libcpubase.so")
+_cb = None
+
+
+def cpubase():
+    global _cb
+    if _cb is None:
+        if not os.path.exists(_CB_PATH):
+            build()
+        L = ctypes.CDLL(_CB_PATH)
+        P, I, U32, I64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_int64
+        L.cb_reset.restype = None
+        L.cb_blas_create.restype = I
+        L.cb_blas_create.argtypes = [I, P, P, P, P]
+        L.cb_blas_create_many.restype = None
+        L.cb_blas_create_many.argtypes = [I, P, P, P, P, P, P]
+        L.cb_tlas_create.restype = I
+        L.cb_tlas_create.argtypes = [I, P]
+        L.cb_trace.restype = None
+        L.cb_trace.argtypes = [I, P, I64, I, P, P, I]
+        L.cb_trace_rays.restype = None
+        L.cb_trace_rays.argtypes = [I, P, U32, U32, U32, P, P, U32, U32, ctypes.c_float, ctypes.c_float, P, I]
+        L.cb_render.restype = ctypes.c_uint64
+        L.cb_render.argtypes = [I, P, U32, P, U32, U32, U32, U32, U32, U32, U32, P, U32, ctypes.c_float, P, I]
+        _cb = L
+    return _cb
+
+
+class CpuBaseline:
+    """A scene in the CPU baseline (FP32 binned-SAH BVH2 per BLAS + one over the instances): the reference's ray
+    tracing as a CPU driver would run it.  BLASes are built in parallel, one thread each."""
+
+    def __init__(self, scene):
+        L = cpubase()
+        L.cb_reset()
+        keep, vp, ip, pc, xp, first = [], [], [], [], [], [0]
+        for geoms in scene.blas:
+            for g in geoms:
+                v = np.ascontiguousarray(g.vertices, np.float32)
+                i = np.ascontiguousarray(g.indices, np.uint32)
+                keep += [v, i]
+                vp.append(v.ctypes.data)
+                ip.append(i.ctypes.data)
+                pc.append(i.shape[0])
+                if g.transform is None:
+                    xp.append(0)
+                else:
+                    x = np.ascontiguousarray(g.transform, np.float32).reshape(3, 4)
+                    keep.append(x)
+                    xp.append(x.ctypes.data)
+            first.append(len(vp))
+        arr = [np.array(a, t) for a, t in ((first, np.int32), (vp, np.uint64), (ip, np.uint64), (pc, np.uint32),
+                                           (xp, np.uint64))]
+        handles = np.zeros(len(scene.blas), np.int32)
+        L.cb_blas_create_many(len(scene.blas), *[a.ctypes.data for a in arr], handles.ctypes.data)
+        rec = instance_records(scene.transforms, [int(handles[b]) for b in scene.blas_of_instance],
+                               scene.custom_index, scene.mask, scene.sbt_offset, scene.flags)
+        self.tlas = L.cb_tlas_create(len(rec), rec.ctypes.data)
+        self.L = L
+
+    def trace(self, rays, any_hit=False, threads=0):
+        """Closest hit: uint32 [n, 6] (t, u, v float bits, prim, geom, inst); any hit: uint32 [n] occluded."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        if any_hit:
+            occ = np.zeros(n, np.uint32)
+            self.L.cb_trace(self.tlas, rays.ctypes.data, n, 1, None, occ.ctypes.data, threads)
+            return occ
+        hits = np.zeros((n, 6), np.uint32)
+        self.L.cb_trace(self.tlas, rays.ctypes.data, n, 0, hits.ctypes.data, None, threads)
+        return hits
+
+    def trace_rays(self, records, camera, w, h, miss_color=(0.0, 0.0, 0.2), tmin=0.0, tmax=100.0, sbt_offset=0,
+                   sbt_stride=1, threads=0):
+        rec = np.ascontiguousarray(records)
+        film = np.zeros((h, w, 4), np.float32)
+        miss = np.asarray(miss_color, np.float32)
+        cam = np.asarray(camera, np.float32).reshape(4)
+        self.L.cb_trace_rays(self.tlas, rec.ctypes.data, len(rec), sbt_offset, sbt_stride, miss.ctypes.data,
+                             cam.ctypes.data, w, h, tmin, tmax, film.ctypes.data, threads)
+        return film
+
+    def render(self, records, camera, lights, w, h, spp, sample_offset=0, max_depth=8, rr_depth=3, seed=1234,
+               ray_offset=1e-4, threads=0):
+        """Radiance sums [h, w, 3] (float32) of spp samples per pixel, and the rays traced."""
+        rec = np.ascontiguousarray(records)
+        cam = np.asarray(camera, np.float32).reshape(4)
+        lt = np.ascontiguousarray(lights, np.float32).reshape(-1, 12)
+        acc = np.zeros((h, w, 3), np.float32)
+        n = self.L.cb_render(self.tlas, rec.ctypes.data, len(rec), cam.ctypes.data, w, h, spp, sample_offset,
+                             max_depth, rr_depth, seed, lt.ctypes.data, lt.shape[0], ray_offset, acc.ctypes.data,
+                             threads)
+        return acc, int(n)
