@@ -43,7 +43,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=float, default=1.0, help="triangle-count scale of the scene (1.0 = 20M)")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20, help="rays per set for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 22,
+                    help="rays per set for the CPU baseline, every (W*H / n)th ray (0: skip the CPU baseline)")
+    ap.add_argument("--parity-sample", type=int, default=1 << 20,
+                    help="rays per set checked against the FP64 oracle (0: skip)")
     ap.add_argument("--no-anyhit", action="store_true")
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing / NRC equal-relMSE part")
     ap.add_argument("--no-autotune", action="store_true", help="use the default traversal configuration")
@@ -214,63 +217,45 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if traffic is None else round((traffic["coherent"] + traffic["incoherent"]) / 2),
-                "traffic_bytes_per_launch": traffic,
+                "traffic_bytes_per_launch": None if traffic is None else
+                {k: traffic[k] for k in ("coherent", "incoherent")},
+                "traffic_source": None if traffic is None else traffic["source"],
                 "kernel": "trace_kernel<closest> (persistent BVH8 traversal)",
                 "bytes_per_ray": {"coherent": round(b_coh, 1), "incoherent": round(b_inc, 1)},
                 "launch_ms": {"coherent": round(ms_coh, 3), "incoherent": round(ms_inc, 3)}}
 
     cpu = None
     parity = {}
+    cores = min(os.cpu_count() or 1, 16)
     if args.cpu_sample > 0:
+        cpu = cpu_baseline_rays(args, sc, inc, W, H, cores, np, S)
+    if args.parity_sample > 0:
         import oracle as O
         O.reset()
         tb = time.time()
         handles = [O.blas_create([(g.vertices, g.indices, g.transform) for g in geoms]) for geoms in sc.blas]
         ot = O.tlas_create(O.instance_records(sc.transforms, [handles[b] for b in sc.blas_of_instance],
                                               sc.custom_index, sc.mask, sc.sbt_offset, sc.flags))
-        log(f"oracle BVH built in {time.time() - tb:.1f}s; timing CPU sample")
-        cores = min(os.cpu_count() or 1, 16)
-        step = max(1, (W * H) // args.cpu_sample)
-        coh_s = S.camera_rays(sc.camera, W, H)[::step][: args.cpu_sample]
-        inc_s = inc[::step][: args.cpu_sample]
-        O.trace(ot, coh_s[:4096], threads=cores)  # warm-up
-        tc = time.perf_counter()
-        o_coh = O.trace(ot, coh_s, threads=cores)
-        o_inc = O.trace(ot, inc_s, threads=cores)
-        dt = time.perf_counter() - tc
-        # the same rays on one thread, every 16th of the sample
-        coh_1, inc_1 = coh_s[::16], inc_s[::16]
-        t1 = time.perf_counter()
-        O.trace(ot, coh_1, threads=1)
-        O.trace(ot, inc_1, threads=1)
-        dt1 = time.perf_counter() - t1
-        model = ""
-        try:
-            with open("/proc/cpuinfo") as f:
-                model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
-        except OSError:
-            pass
-        cpu = {"value": round((len(coh_s) + len(inc_s)) / dt / 1e6, 4), "unit": "Mrays/s", "cores": cores,
-               "kind": "port",
-               "value_1core": round((len(coh_1) + len(inc_1)) / dt1 / 1e6, 4),
-               "host_cpus": os.cpu_count(), "cpu_model": model,
-               "sample": f"{len(coh_s)} coherent + {len(inc_s)} incoherent closest-hit rays (every {step}th ray of "
-                         f"each set) traced by the FP64 oracle (scalar C++, BVH2, std::thread x {cores}) in "
-                         f"{dt:.1f}s; value_1core: every 16th of them on one thread in {dt1:.1f}s"}
-        O.reset()
-        log(f"cpu baseline {cpu['value']} Mrays/s on {cores} threads, {cpu['value_1core']} on one")
-        # full-size parity on the same sample: the GPU's hits from the timed configurations vs the FP64 oracle
-        for name, r, o in (("coherent", rc, o_coh), ("incoherent", ri, o_inc)):
+        log(f"oracle BVH built in {time.time() - tb:.1f}s; full-size parity sample")
+        # full-size parity: the GPU's hits from the timed configurations vs the FP64 oracle on every (W*H / n)th
+        # ray of each set; epsilon-tie rays are checked against their candidate sets
+        step = max(1, (W * H) // args.parity_sample)
+        for name, r, rays in (("coherent", rc, S.camera_rays(sc.camera, W, H)[::step]),
+                              ("incoherent", ri, inc[::step])):
+            rays = rays[: args.parity_sample]
+            o = O.trace(ot, rays, threads=cores)
             trace_cfg(f"closest/{name}", r)
             ctx.sync()
-            idx = torch.arange(0, W * H, step, device=dev)[: args.cpu_sample]
-            parity[name] = hit_parity(hits[idx].cpu().numpy(), *o)
+            idx = torch.arange(0, W * H, step, device=dev)[: args.parity_sample]
+            parity[name] = hit_parity(hits[idx].cpu().numpy(), *o, tlas=ot, rays=rays)
+        O.reset()
         log(f"parity vs the FP64 oracle: {json.dumps(parity)}")
 
     ctx.set_config(**default_cfg)
     render = None if args.no_render else render_bench(ctx, args, np, torch, pyrtx, S)
     ctx.set_config(**default_cfg)
     batch = None if args.no_batch else batch_bench(ctx, args, np, torch, pyrtx, S)
+    nrc_roof = None if args.no_render else nrc_roofline(nrc_kernel_bench(ctx, torch, pyrtx))
 
     out = {
         "metric": METRIC,
@@ -296,6 +281,7 @@ def main():
         "build": {"blas_tlas_ms_incl_upload": round(build_s * 1e3, 1),
                   "nodes": int(sum(i.node_count for i in info)), "max_depth": int(tinfo.max_depth)},
         "roofline": roofline,
+        "roofline_nrc": nrc_roof,
         "cpu_baseline": cpu,
         "parity_vs_oracle": parity or None,
         "path_tracing": render,
@@ -305,10 +291,12 @@ def main():
     ctx.close()
 
 
-def hit_parity(h, o_tuv, o_ids, o_st, rel_tol=1e-5):
+def hit_parity(h, o_tuv, o_ids, o_st, rel_tol=1e-5, tlas=None, rays=None):
     """Closest-hit parity of GPU hit records h (int32 [N, 6]: t, u, v bits, prim, geom, inst) against the FP64
     oracle's (t_uv, ids, status) for the same rays: IDs and hit/miss on every ray the oracle calls robust
-    (status 0 miss, 1 hit; 2 = epsilon-tie, excluded and counted), t relative and u, v absolute errors."""
+    (status 0 miss, 1 hit), t relative and u, v absolute errors; epsilon-tie rays (status 2) against their candidate
+    sets when the oracle TLAS and the rays are given (a hit outside the set, a miss where a candidate is robustly
+    inside, a t off its candidate's)."""
     import numpy as np
     f = np.ascontiguousarray(h[:, :3]).view(np.float32).astype(np.float64)
     ids = np.ascontiguousarray(h[:, 3:]).view(np.uint32)
@@ -317,11 +305,65 @@ def hit_parity(h, o_tuv, o_ids, o_st, rel_tol=1e-5):
     ok = hit & ~id_bad
     t_err = np.abs(f[ok, 0] - o_tuv[ok, 0]) / np.maximum(np.abs(o_tuv[ok, 0]), 1e-30)
     uv_err = np.abs(f[ok, 1:] - o_tuv[ok, 1:]).max(axis=1) if ok.any() else np.zeros(0)
-    return {"rays": int(len(o_st)), "ties": int((o_st == 2).sum()), "robust_hits": int(hit.sum()),
-            "id_mismatch": int(id_bad.sum()), "miss_mismatch": int((miss & (ids[:, 2] != 0xFFFFFFFF)).sum()),
-            "t_over_tol": int((t_err > rel_tol).sum()), "uv_over_tol": int((uv_err > rel_tol).sum()),
-            "max_t_rel": float(t_err.max()) if t_err.size else 0.0,
-            "max_uv_abs": float(uv_err.max()) if uv_err.size else 0.0}
+    out = {"rays": int(len(o_st)), "ties": int((o_st == 2).sum()), "robust_hits": int(hit.sum()),
+           "id_mismatch": int(id_bad.sum()), "miss_mismatch": int((miss & (ids[:, 2] != 0xFFFFFFFF)).sum()),
+           "t_over_tol": int((t_err > rel_tol).sum()), "uv_over_tol": int((uv_err > rel_tol).sum()),
+           "max_t_rel": float(t_err.max()) if t_err.size else 0.0,
+           "max_uv_abs": float(uv_err.max()) if uv_err.size else 0.0}
+    tie = o_st == 2
+    if tlas is not None and tie.any():
+        import oracle as O
+        r = np.asarray(rays, np.float32).reshape(-1, 8)[tie]
+        c = O.trace_candidates(tlas, r)
+        tr = O.check_ties(r, c, ids[tie, 2], ids[tie, 1], ids[tie, 0], f[tie, 0])
+        out["ties_checked"] = {k: tr[k] for k in ("hit_not_candidate", "miss_not_allowed", "t_bad", "unchecked")}
+    return out
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            return next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        return ""
+
+
+def cpu_baseline_rays(args, sc, inc, W, H, cores, np, S):
+    """The reference's ray tracing on the host cores (oracle/cpu_baseline.cpp: FP32 raygen / closest-hit / miss over
+    binned-SAH BVHs with the watertight test, std::thread x cores) on the same scene and a bounded sample of the same
+    rays: median of 5 timed runs after a warm-up; value_1core: every 16th ray of the sample on one thread."""
+    import oracle as O
+    tb = time.time()
+    cb = O.CpuBaseline(sc)
+    build_s = time.time() - tb
+    step = max(1, (W * H) // args.cpu_sample)
+    coh_s = S.camera_rays(sc.camera, W, H)[::step][: args.cpu_sample]
+    inc_s = inc[::step][: args.cpu_sample]
+    cb.trace(coh_s[:65536], threads=cores)
+    runs = {"coherent": [], "incoherent": []}
+    for _ in range(5):
+        for name, r in (("coherent", coh_s), ("incoherent", inc_s)):
+            t0 = time.perf_counter()
+            cb.trace(r, threads=cores)
+            runs[name].append(time.perf_counter() - t0)
+    med = {k: float(np.median(v)) for k, v in runs.items()}
+    t1 = time.perf_counter()
+    cb.trace(coh_s[::16], threads=1)
+    cb.trace(inc_s[::16], threads=1)
+    dt1 = time.perf_counter() - t1
+    n = len(coh_s) + len(inc_s)
+    out = {"value": round(n / (med["coherent"] + med["incoherent"]) / 1e6, 3), "unit": "Mrays/s", "cores": cores,
+           "kind": "port",
+           "closest_hit_mrays": {k: round(len(coh_s) / v / 1e6, 3) for k, v in med.items()},
+           "value_1core": round((len(coh_s[::16]) + len(inc_s[::16])) / dt1 / 1e6, 4),
+           "build_s": round(build_s, 2), "host_cpus": os.cpu_count(), "cpu_model": _cpu_model(),
+           "sample": f"{len(coh_s)} coherent + {len(inc_s)} incoherent closest-hit rays (every {step}th ray of each "
+                     f"set) on the 20M-triangle scene, traced by oracle/cpu_baseline.cpp (FP32 raygen / chit / miss of "
+                     f"main.cpp:1033-1091, binned-SAH BVH2, watertight test, -O3 x86-64-v3, std::thread x {cores}); "
+                     f"median of 5 runs; value_1core: every 16th of those rays on one thread"}
+    log(f"cpu baseline {out['value']} Mrays/s on {cores} threads ({out['closest_hit_mrays']}), "
+        f"{out['value_1core']} on one; CPU BVH build {build_s:.1f}s")
+    return out
 
 
 def nrc_kernel_bench(ctx, torch, pyrtx):
@@ -344,6 +386,27 @@ def nrc_kernel_bench(ctx, torch, pyrtx):
     return out
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+NRC_FLOP_PER_QUERY = 47104    # forward: 2 x sum(fan_in x fan_out) over the 6 layers
+NRC_FLOP_PER_SAMPLE = 141312  # forward + data gradient + weight gradient
+
+
+def nrc_roofline(k):
+    """MFMA roofline of the cache network's kernels from nrc_kernel_bench's HIP-event times: inference of 2,073,600
+    queries in one launch, and 4 training steps of 16,384 samples (the training kernel plus the Adam kernel that sums
+    the block partials and updates the weights, per step)."""
+    inf_tf = 2073600 * NRC_FLOP_PER_QUERY / (k["infer_2073600_ms"] * 1e-3) / 1e12
+    tr_tf = 65536 * NRC_FLOP_PER_SAMPLE / (k["train_4x16384_ms"] * 1e-3) / 1e12
+    return {"infer": {"bound": "mfma", "achieved": round(inf_tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": round(inf_tf / MFMA_BF16_PEAK_TFLOPS, 4),
+                      "kernel": "nrc_infer_lds_kernel", "units": "2,073,600 queries x 47,104 FLOP",
+                      "launch_ms": k["infer_2073600_ms"]},
+            "train": {"bound": "mfma", "achieved": round(tr_tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS,
+                      "unit": "TFLOP/s", "frac": round(tr_tf / MFMA_BF16_PEAK_TFLOPS, 4),
+                      "kernel": "nrc_train_kernel + nrc_adam_kernel", "units": "4 x 16,384 samples x 141,312 FLOP",
+                      "ms": k["train_4x16384_ms"]}}
+
+
 def measured_traffic():
     """HBM bytes per traversal launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-calibrated) as recorded in the
     committed profile of this kernel and workload; None when the profile is absent."""
@@ -354,7 +417,12 @@ def measured_traffic():
     except OSError:
         return None
     m = re.search(r"bench-traffic-bytes-per-launch: coherent (\d+) incoherent (\d+)", txt)
-    return {"coherent": int(m.group(1)), "incoherent": int(m.group(2))} if m else None
+    if not m:
+        return None
+    c = re.search(r"measured at commit ([0-9a-f]{7,40})", txt)
+    return {"coherent": int(m.group(1)), "incoherent": int(m.group(2)),
+            "source": "profiles/trace_kernel_hbm_traffic_20M_query_scene.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
+                      "copy-calibrated" + (f", measured at commit {c.group(1)})" if c else ")")}
 
 
 def render_bench(ctx, args, np, torch, pyrtx, S):
@@ -402,6 +470,30 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
     ref = (acc / (64 * chunks)).float()
     err_plain = float(relmse(film[..., :3], ref))
     log(f"plain relMSE {err_plain:.5f} vs {64 * chunks}-spp reference")
+
+    cpu_pt = None
+    if args.cpu_sample > 0:
+        # the same estimator on the host cores (oracle/cpu_baseline.cpp, FP32): 1-spp renders of the full film,
+        # median of 3; the 64-spp time is 64 of them
+        import oracle as O
+        cores = min(os.cpu_count() or 1, 16)
+        tb = time.time()
+        cb = O.CpuBaseline(sc)
+        cb_build = time.time() - tb
+        cb.render(sc.records, sc.camera, lights, 192, 108, 1, threads=cores)
+        ts, nr = [], 0
+        for k in range(3):
+            t0 = time.perf_counter()
+            _, nr = cb.render(sc.records, sc.camera, lights, W, H, 1, sample_offset=k, threads=cores)
+            ts.append(time.perf_counter() - t0)
+        tcpu = float(np.median(ts))
+        cpu_pt = {"ms_per_spp": round(tcpu * 1e3, 1), "mrays_per_s": round(nr / tcpu / 1e6, 2),
+                  "ms_64spp": round(64 * tcpu * 1e3, 0), "cores": cores, "build_s": round(cb_build, 2),
+                  "sample": f"{W}x{H} at 1 spp (median of 3 renders; 64 spp = 64 x that), oracle/cpu_baseline.cpp "
+                            f"FP32 estimator, std::thread x {cores}"}
+        del cb
+        log(f"cpu path tracing: {cpu_pt['ms_per_spp']} ms per spp ({cpu_pt['mrays_per_s']} Mrays/s on {cores} "
+            f"threads), 64 spp ~{cpu_pt['ms_64spp']:.0f} ms")
 
     lo, hi = S.scene_bounds(sc)
     # The NRC frame's launches (a render wavefront to the cache vertex beside the training paths' whole-path
@@ -474,6 +566,7 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
                 "wall_s": round(wall_nrc, 2)},
         "equal_relmse_speedup": None if ms_nrc is None else round(ms_plain / ms_nrc, 2),
         "autotuned_trace_config": tuned,
+        "cpu_baseline": cpu_pt,
     }
 
 

@@ -45,6 +45,8 @@ struct rtx_context_t {
     hipStream_t capture_stream = nullptr;
     hipGraphExec_t graph = nullptr;
     uint64_t graph_key = 0;
+    hipGraphExec_t nrc_graph = nullptr;  // the last recorded NRC frame (rtx_cmd_render_nrc with the graph bit)
+    uint64_t nrc_graph_key = 0;
     uint32_t* sample_word = nullptr;
     // training wavefront of NRC frames: side stream, fork / join events, its ray counter and spill stack
     hipStream_t side_stream = nullptr;
@@ -385,6 +387,7 @@ int rtx_destroy_context(rtx_context ctx) {
     if (ctx->side_join) (void)hipEventDestroy(ctx->side_join);
     if (ctx->side_stream) (void)hipStreamDestroy(ctx->side_stream);
     if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+    if (ctx->nrc_graph) (void)hipGraphExecDestroy(ctx->nrc_graph);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
     if (ctx->sample_word) (void)hipFree(ctx->sample_word);
     if (ctx->views_buf) (void)hipFree(ctx->views_buf);
@@ -909,6 +912,51 @@ uint64_t fnv1a(uint64_t h, const void* data, size_t n) {
     for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
     return h;
 }
+
+// The word recorded launches read sample_offset from, and the private stream calls are captured on.
+int graph_resources(rtx_context ctx) {
+    if (!ctx->sample_word && hipMalloc(&ctx->sample_word, 256) != hipSuccess) {
+        ctx->sample_word = nullptr;
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "graph sample word");
+    }
+    hipError_t e;
+    if (!ctx->capture_stream && (e = hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking)))
+        return hip_fail(ctx, e, "capture stream");
+    return RTX_SUCCESS;
+}
+
+// Record record(stream) on the capture stream into *exec (replacing the previous graph).
+extern "C++" template <typename Record>
+int capture_graph(rtx_context ctx, hipGraphExec_t* exec, Record record, const char* who) {
+    if (*exec) {
+        (void)hipGraphExecDestroy(*exec);
+        *exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e) return hip_fail(ctx, e, who);
+    const hipError_t le = record(ctx->capture_stream);
+    e = hipStreamEndCapture(ctx->capture_stream, &g);
+    if (le != hipSuccess || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(ctx, le != hipSuccess ? le : e, who);
+    }
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        return hip_fail(ctx, e, who);
+    }
+    return RTX_SUCCESS;
+}
+
+// Replay: the sample offset into the word the recorded launches read, then the graph, on the context stream.
+int replay_graph(rtx_context ctx, hipGraphExec_t exec, uint32_t sample_offset, const char* who) {
+    set_word_kernel<<<1, 64, 0, ctx->stream>>>(ctx->sample_word, sample_offset);
+    hipError_t e;
+    if ((e = hipGetLastError()) || (e = hipGraphLaunch(exec, ctx->stream))) return hip_fail(ctx, e, who);
+    return RTX_SUCCESS;
+}
 }  // namespace
 
 int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_render_view* views,
@@ -999,45 +1047,19 @@ int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* 
         if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render_batch");
         return RTX_SUCCESS;
     }
-    if (!ctx->sample_word) {
-        if (hipMalloc(&ctx->sample_word, 256) != hipSuccess)
-            return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "graph sample word");
-    }
-    if (!ctx->capture_stream) {
-        if ((e = hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking)))
-            return hip_fail(ctx, e, "rtx_cmd_render_batch capture stream");
-    }
+    if ((r = graph_resources(ctx))) return r;
     if (!ctx->graph || ctx->graph_key != key) {
         if ((e = hipMemcpyAsync(vdst, bv.data(), vbytes, hipMemcpyHostToDevice, ctx->stream)))
             return hip_fail(ctx, e, "rtx_cmd_render_batch views");
-        if (ctx->graph) {
-            (void)hipGraphExecDestroy(ctx->graph);
-            ctx->graph = nullptr;
-        }
         ctx->graph_key = 0;
         a.sample_dev = ctx->sample_word;
-        hipGraph_t g = nullptr;
-        if ((e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal)))
-            return hip_fail(ctx, e, "rtx_cmd_render_batch capture");
-        hipError_t le = rtx::pt_render_batch(a, L, params->spp, 0u, films_device, accum_rgb_device,
-                                             ctx->capture_stream);
-        e = hipStreamEndCapture(ctx->capture_stream, &g);
-        if (le != hipSuccess || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return hip_fail(ctx, le != hipSuccess ? le : e, "rtx_cmd_render_batch capture");
-        }
-        e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            ctx->graph = nullptr;
-            return hip_fail(ctx, e, "rtx_cmd_render_batch instantiate");
-        }
+        if ((r = capture_graph(ctx, &ctx->graph, [&](hipStream_t cs) {
+                 return rtx::pt_render_batch(a, L, params->spp, 0u, films_device, accum_rgb_device, cs);
+             }, "rtx_cmd_render_batch capture")))
+            return r;
         ctx->graph_key = key;
     }
-    set_word_kernel<<<1, 64, 0, ctx->stream>>>(ctx->sample_word, params->sample_offset);
-    if ((e = hipGetLastError()) || (e = hipGraphLaunch(ctx->graph, ctx->stream)))
-        return hip_fail(ctx, e, "rtx_cmd_render_batch launch");
-    return RTX_SUCCESS;
+    return replay_graph(ctx, ctx->graph, params->sample_offset, "rtx_cmd_render_batch launch");
 }
 
 int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
@@ -1073,12 +1095,47 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
         L.blocks_per_cu -= reserve;
         L.grid = (uint32_t)ctx->cu_count * L.blocks_per_cu;
     }
-    uint32_t count = 0;
-    hipError_t e = rtx::pt_render_nrc(a, L, side, nrc->st, train_paths, nrc->st.train_suffix, nrc->train_buf,
-                                      params->spp, params->sample_offset, film_rgba32f_device, accum_rgb_device,
-                                      &count, ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render_nrc");
-    return RTX_SUCCESS;
+    if (!(params->flags & RTX_RENDER_GRAPH_BIT)) {
+        hipError_t e = rtx::pt_render_nrc(a, L, side, nrc->st, train_paths, nrc->st.train_suffix, nrc->train_buf,
+                                          params->spp, params->sample_offset, film_rgba32f_device, accum_rgb_device,
+                                          ctx->stream);
+        if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render_nrc");
+        return RTX_SUCCESS;
+    }
+    // the frame recorded once into a graph (both streams: the training wavefront's side stream joins the capture
+    // through the fork / join events) and replayed; the cache's parameters, moments and step counter live in device
+    // memory, so replays train on; sample_offset is read from the context's sample word
+    uint64_t key = 1469598103934665603ull;
+    key = fnv1a(key, &tlas_address, sizeof tlas_address);
+    key = fnv1a(key, sbt, sizeof *sbt);
+    key = fnv1a(key, camera, sizeof *camera);
+    rtx_render_params kp = *params;
+    kp.sample_offset = 0;
+    key = fnv1a(key, &kp, sizeof kp);
+    key = fnv1a(key, &nrc, sizeof nrc);
+    key = fnv1a(key, &nrc->train_buf, sizeof nrc->train_buf);
+    key = fnv1a(key, &train_paths, sizeof train_paths);
+    key = fnv1a(key, &film_rgba32f_device, sizeof film_rgba32f_device);
+    key = fnv1a(key, &accum_rgb_device, sizeof accum_rgb_device);
+    key = fnv1a(key, &L, sizeof L);
+    key = fnv1a(key, &side.launch, sizeof side.launch);
+    key = fnv1a(key, &side.gstack, sizeof side.gstack);
+    key = fnv1a(key, &a.content, sizeof a.content);
+    key = fnv1a(key, &ctx->render_form, sizeof ctx->render_form);
+    key = fnv1a(key, &ctx->pt_scratch, sizeof ctx->pt_scratch);
+    key = fnv1a(key, &ctx->gstack, sizeof ctx->gstack);
+    if ((r = graph_resources(ctx))) return r;
+    if (!ctx->nrc_graph || ctx->nrc_graph_key != key) {
+        ctx->nrc_graph_key = 0;
+        a.sample_dev = ctx->sample_word;
+        if ((r = capture_graph(ctx, &ctx->nrc_graph, [&](hipStream_t cs) {
+                 return rtx::pt_render_nrc(a, L, side, nrc->st, train_paths, nrc->st.train_suffix, nrc->train_buf,
+                                           params->spp, 0u, film_rgba32f_device, accum_rgb_device, cs);
+             }, "rtx_cmd_render_nrc capture")))
+            return r;
+        ctx->nrc_graph_key = key;
+    }
+    return replay_graph(ctx, ctx->nrc_graph, params->sample_offset, "rtx_cmd_render_nrc launch");
 }
 
 int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays) {
@@ -1137,6 +1194,7 @@ int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc)
     if (!e) e = hipMalloc(&st.partials, (uint64_t)st.max_blocks * pbytes);
     if (!e) e = hipMalloc(&st.loss, sizeof(float) * 4);
     if (!e) e = hipMalloc(&st.loss_part, sizeof(float) * st.max_blocks);
+    if (!e) e = hipMalloc(&st.steps_dev, 256);
     if (e) {
         rtx_nrc_destroy(n);
         return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc buffers: %s", hipGetErrorString(e));
@@ -1149,6 +1207,7 @@ int rtx_nrc_create(rtx_context ctx, const rtx_nrc_config* cfg, rtx_nrc* out_nrc)
     if (!e) e = hipMemcpy(st.m, zeros.data(), pbytes, hipMemcpyHostToDevice);
     if (!e) e = hipMemcpy(st.v, zeros.data(), pbytes, hipMemcpyHostToDevice);
     if (!e) e = hipMemcpy(st.loss, zeros.data(), sizeof(float) * 4, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(st.steps_dev, zeros.data(), sizeof(uint32_t), hipMemcpyHostToDevice);
     if (!e) e = rtx::nrc_pack(st, ctx->stream);
     if (!e) e = hipStreamSynchronize(ctx->stream);
     if (e) {
@@ -1175,6 +1234,7 @@ int rtx_nrc_destroy(rtx_nrc n) {
     if (st.partials) (void)hipFree(st.partials);
     if (st.loss) (void)hipFree(st.loss);
     if (st.loss_part) (void)hipFree(st.loss_part);
+    if (st.steps_dev) (void)hipFree(st.steps_dev);
     if (n->train_buf) (void)hipFree(n->train_buf);
     delete n;
     return RTX_SUCCESS;
@@ -1236,9 +1296,11 @@ int rtx_nrc_get_status(rtx_nrc n, uint32_t* steps, float* last_loss) {
     if (!n || !n->ctx) return RTX_ERROR_INVALID_ARGUMENT;  // NULL, or its context was destroyed
     hipError_t e = hipStreamSynchronize(n->ctx->stream);
     float l = 0.0f;
+    uint32_t k = 0;
     if (!e) e = hipMemcpy(&l, n->st.loss, sizeof(float), hipMemcpyDeviceToHost);
+    if (!e) e = hipMemcpy(&k, n->st.steps_dev, sizeof(uint32_t), hipMemcpyDeviceToHost);
     if (e) return hip_fail(n->ctx, e, "rtx_nrc_get_status");
-    if (steps) *steps = n->st.steps;
+    if (steps) *steps = k;
     if (last_loss) *last_loss = l;
     return RTX_SUCCESS;
 }

@@ -16,10 +16,10 @@ struct NrcState {
     float* partials = nullptr;  // per-block gradients of one step
     float* loss = nullptr;      // mean loss of the last step
     float* loss_part = nullptr; // per-block losses of one step
+    uint32_t* steps_dev = nullptr;  // Adam steps taken (device word: the bias corrections read it)
     uint32_t max_blocks = 0;
     uint32_t cu_count = 256;    // compute units of the device (persistent inference grid)
     uint32_t batch_size = 0;
-    uint32_t steps = 0;
     uint32_t cache_vertex = 1;
     uint32_t train_suffix = 0;  // training-path vertices past the cache vertex before the self-training query
     float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f;
@@ -39,8 +39,14 @@ struct InferGather {
 // radiance predictions for n queries (n_dev, when set, overrides n with a device count bounded by n)
 hipError_t nrc_infer(const NrcState& st, const rtx_nrc_query* queries, uint32_t n, float* out_rgb, hipStream_t stream,
                      const uint32_t* n_dev = nullptr, const InferGather* gather = nullptr);
-// one Adam step over n samples, taking sample (i * stride + offset) for i < n; with grad_out: gradient only
+// one Adam step over n samples (sample i of the batch is samples[i]); with grad_out: gradient only, no update
 hipError_t nrc_step(NrcState& st, const rtx_nrc_sample* samples, uint32_t n, float* grad_out, float* loss_out,
-                    hipStream_t stream, uint32_t stride = 1, uint32_t offset = 0);
+                    hipStream_t stream);
+// Adam steps over a device-resident sample count: batch b of ceil(count / batch_size) interleaved batches takes
+// samples b, b + nb, b + 2 nb, ...; steps b >= nb return at once.  The enqueued work does not depend on the count,
+// so a frame needs no host round trip (and can be captured).  Launches max_batches steps, then advances the step
+// counter by nb.
+hipError_t nrc_steps_device(NrcState& st, const rtx_nrc_sample* samples, const uint32_t* count_dev,
+                            uint32_t max_batches, hipStream_t stream);
 
 }  // namespace rtx

@@ -61,6 +61,32 @@ struct Norm {
     float inv[3];
 };
 
+// Which samples one training step takes.  Host-sized steps: samples [0, n).  Device-sized steps (count_dev set):
+// batch b of nb = ceil(count / batch_size) interleaved batches, samples b, b + nb, b + 2 nb, ... (n of them); a
+// step with b >= nb does nothing.
+struct StepShape {
+    const uint32_t* count_dev;
+    uint32_t n;
+    uint32_t batch;
+    uint32_t batch_size;
+};
+
+__device__ __forceinline__ bool step_samples(const StepShape& sh, uint32_t& n, uint32_t& stride, uint32_t& offset) {
+    if (!sh.count_dev) {
+        n = sh.n;
+        stride = 1;
+        offset = 0;
+        return n > 0;
+    }
+    const uint32_t count = *sh.count_dev;
+    const uint32_t nb = (count + sh.batch_size - 1) / sh.batch_size;
+    if (sh.batch >= nb) return false;
+    n = (count - sh.batch + nb - 1) / nb;
+    stride = nb;
+    offset = sh.batch;
+    return true;
+}
+
 __device__ __forceinline__ void load_query(const rtx_nrc_query* src, rtx_nrc_query& q) {
     const float4* p = (const float4*)src;
     const float4 a = p[0], b = p[1], c = p[2];
@@ -354,10 +380,14 @@ __device__ __forceinline__ void store_grad(const f32x4* gv, int tiles, __bf16* g
 
 __global__ __launch_bounds__(256) void nrc_train_kernel(const bf16x8* __restrict__ fwd, const bf16x8* __restrict__ bwd,
                                                         const float* __restrict__ params,
-                                                        const rtx_nrc_sample* __restrict__ samples, uint32_t n,
-                                                        uint32_t stride, uint32_t offset, float inv_n, Norm nm,
-                                                        float* __restrict__ partials, float* __restrict__ loss_part) {
+                                                        const rtx_nrc_sample* __restrict__ samples, StepShape shape,
+                                                        Norm nm, float* __restrict__ partials,
+                                                        float* __restrict__ loss_part) {
     __shared__ TrainLds sm;
+    uint32_t n, stride, offset;
+    // a step that does not run, and blocks past the step's samples, leave at once (the whole block alike)
+    if (!step_samples(shape, n, stride, offset) || blockIdx.x * 64u >= n) return;
+    const float inv_n = 1.0f / (float)n;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
@@ -487,14 +517,17 @@ __device__ __forceinline__ void scatter_weight(int p, float w, __bf16* __restric
 // write the gradient.  A block covers 16 parameters with 16 threads each; thread k sums partials k, k + 16, ...
 // and a fixed LDS tree adds the 16 slices (deterministic).  The extra last block sums the per-block losses the
 // same way.
-__global__ __launch_bounds__(256) void nrc_adam_kernel(const float* __restrict__ partials, uint32_t nblk,
+__global__ __launch_bounds__(256) void nrc_adam_kernel(const float* __restrict__ partials, StepShape shape,
                                                        float* __restrict__ params, float* __restrict__ m,
                                                        float* __restrict__ v, float lr, float b1, float b2, float eps,
-                                                       float c1, float c2, float* __restrict__ grad_out,
+                                                       const uint32_t* __restrict__ steps, float* __restrict__ grad_out,
                                                        __bf16* __restrict__ fwd, __bf16* __restrict__ bwd,
                                                        const float* __restrict__ loss_part,
                                                        float* __restrict__ loss_out) {
     __shared__ float s[16][17];
+    uint32_t n, stride, offset;
+    if (!step_samples(shape, n, stride, offset)) return;
+    const uint32_t nblk = (n + 63) / 64;
     if (blockIdx.x == gridDim.x - 1) {
         __shared__ float ls[256];
         float a = 0.0f;
@@ -525,6 +558,10 @@ __global__ __launch_bounds__(256) void nrc_adam_kernel(const float* __restrict__
         grad_out[p] = gsum;
         return;
     }
+    // bias corrections of step t = steps taken + this batch's index + 1 (FP64, rounded)
+    const double t = (double)(*steps + shape.batch + 1u);
+    const float c1 = (float)(1.0 / (1.0 - pow((double)b1, t)));
+    const float c2 = (float)(1.0 / (1.0 - pow((double)b2, t)));
     const float mm = b1 * m[p] + (1.0f - b1) * gsum;
     const float vv = b2 * v[p] + (1.0f - b2) * gsum * gsum;
     m[p] = mm;
@@ -602,28 +639,49 @@ hipError_t nrc_infer(const NrcState& st, const rtx_nrc_query* q, uint32_t n, flo
     return hipGetLastError();
 }
 
-hipError_t nrc_step(NrcState& st, const rtx_nrc_sample* samples, uint32_t n, float* grad_out, float* loss_out,
-                    hipStream_t s, uint32_t stride, uint32_t offset) {
-    if (n == 0) return hipSuccess;
-    const uint32_t nblk = (n + 63) / 64;
-    if (nblk > st.max_blocks) return hipErrorInvalidValue;
+// Adds the steps just taken to the step counter: one (host-sized step) or ceil(count / batch_size).
+__global__ void nrc_advance_kernel(uint32_t* steps, const uint32_t* count_dev, uint32_t batch_size) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *steps += count_dev ? (*count_dev + batch_size - 1) / batch_size : 1u;
+}
+
+// One training step (gradient kernel + Adam kernel) of the given shape; the grid covers the largest batch.
+static hipError_t launch_step(NrcState& st, const rtx_nrc_sample* samples, const StepShape& shape, uint32_t nblk,
+                              float* grad_out, float* loss_out, hipStream_t s) {
     float* loss = loss_out ? loss_out : st.loss;
-    nrc_train_kernel<<<nblk, 256, 0, s>>>((const bf16x8*)st.fwd, (const bf16x8*)st.bwd, st.params, samples, n,
-                                          stride, offset, 1.0f / (float)n, make_norm(st), st.partials,
-                                          st.loss_part);
+    nrc_train_kernel<<<nblk, 256, 0, s>>>((const bf16x8*)st.fwd, (const bf16x8*)st.bwd, st.params, samples, shape,
+                                          make_norm(st), st.partials, st.loss_part);
     // one block per 16 parameters + the loss block
     const unsigned pb = (RTX_NRC_PARAM_COUNT + 15) / 16 + 1;
     if (grad_out) {
-        nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, 0, 0, 0, 0, 0, 0, grad_out,
-                                           nullptr, nullptr, st.loss_part, loss);
+        nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, shape, st.params, st.m, st.v, 0, 0, 0, 0, st.steps_dev,
+                                           grad_out, nullptr, nullptr, st.loss_part, loss);
         return hipGetLastError();
     }
-    st.steps += 1;
-    const double c1 = 1.0 / (1.0 - std::pow((double)st.beta1, (double)st.steps));
-    const double c2 = 1.0 / (1.0 - std::pow((double)st.beta2, (double)st.steps));
-    nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, nblk, st.params, st.m, st.v, st.lr, st.beta1, st.beta2, st.eps,
-                                       (float)c1, (float)c2, nullptr, (__bf16*)st.fwd, (__bf16*)st.bwd, st.loss_part,
-                                       loss);
+    nrc_adam_kernel<<<pb, 256, 0, s>>>(st.partials, shape, st.params, st.m, st.v, st.lr, st.beta1, st.beta2, st.eps,
+                                       st.steps_dev, nullptr, (__bf16*)st.fwd, (__bf16*)st.bwd, st.loss_part, loss);
+    return hipGetLastError();
+}
+
+hipError_t nrc_step(NrcState& st, const rtx_nrc_sample* samples, uint32_t n, float* grad_out, float* loss_out,
+                    hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t nblk = (n + 63) / 64;
+    if (nblk > st.max_blocks) return hipErrorInvalidValue;
+    hipError_t e = launch_step(st, samples, StepShape{nullptr, n, 0u, st.batch_size}, nblk, grad_out, loss_out, s);
+    if (e || grad_out) return e;
+    nrc_advance_kernel<<<1, 64, 0, s>>>(st.steps_dev, nullptr, st.batch_size);
+    return hipGetLastError();
+}
+
+hipError_t nrc_steps_device(NrcState& st, const rtx_nrc_sample* samples, const uint32_t* count_dev,
+                            uint32_t max_batches, hipStream_t s) {
+    for (uint32_t b = 0; b < max_batches; ++b) {
+        hipError_t e = launch_step(st, samples, StepShape{count_dev, 0u, b, st.batch_size}, st.max_blocks, nullptr,
+                                   nullptr, s);
+        if (e) return e;
+    }
+    nrc_advance_kernel<<<1, 64, 0, s>>>(st.steps_dev, count_dev, st.batch_size);
     return hipGetLastError();
 }
 

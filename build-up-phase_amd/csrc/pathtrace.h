@@ -96,11 +96,10 @@ struct SideChain {
 // vertices per training path that become samples
 uint32_t nrc_record_depths(uint32_t max_depth, uint32_t cache_depth, uint32_t train_suffix);
 // one frame: spp cache-terminated passes, the first carrying train_paths training paths (self-training when
-// train_suffix > 0) as a second wavefront on side.stream, then the Adam steps (synchronises once to read the
-// sample count)
+// train_suffix > 0) as a second wavefront on side.stream, then the Adam steps over the device-resident sample count
+// (no host synchronisation: the frame can be captured into a graph)
 hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& launch, const SideChain& side, NrcState& nrc,
                          uint32_t train_paths, uint32_t train_suffix, void* train_buf, uint32_t spp,
-                         uint32_t sample_offset, float* film, double* accum64, uint32_t* host_count,
-                         hipStream_t stream);
+                         uint32_t sample_offset, float* film, double* accum64, hipStream_t stream);
 
 }  // namespace rtx

@@ -478,7 +478,7 @@ hipError_t pt_render_batch(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_
 
 hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, NrcState& nrc,
                          uint32_t train_paths, uint32_t train_suffix, void* train_buf, uint32_t spp,
-                         uint32_t sample_offset, float* film, double* acc64, uint32_t* host_count, hipStream_t s) {
+                         uint32_t sample_offset, float* film, double* acc64, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
     hipError_t e;
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
@@ -562,17 +562,11 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         sample_count_kernel<<<(unsigned)nblk, 256, 0, side.stream>>>(a, nrec, block_counts);
         sample_scan_kernel<<<1, 1024, 0, side.stream>>>(block_counts, (uint32_t)nblk, total);
         sample_scatter_kernel<<<(unsigned)nblk, 256, 0, side.stream>>>(a, nrec, block_counts, samples);
-        if ((e = hipMemcpyAsync(host_count, total, 4, hipMemcpyDeviceToHost, side.stream))) return e;
-        if ((e = hipStreamSynchronize(side.stream))) return e;
-        const uint32_t count = *host_count;
-        if (count > 0) {
-            // interleaved batches: batch b takes samples b, b + nb, b + 2 nb, ... (all depths in every batch)
-            const uint32_t nb = (count + nrc.batch_size - 1) / nrc.batch_size;
-            for (uint32_t b = 0; b < nb; ++b) {
-                const uint32_t nbatch = (count - b + nb - 1) / nb;
-                if ((e = nrc_step(nrc, samples, nbatch, nullptr, nullptr, side.stream, nb, b))) return e;
-            }
-        }
+        // interleaved batches over the device-resident sample count (batch b takes samples b, b + nb, b + 2 nb, ...:
+        // all depths in every batch): as many steps as the largest count needs, the surplus ones return at once, so
+        // the frame has no host round trip
+        const uint32_t max_batches = (uint32_t)((nrec + nrc.batch_size - 1) / nrc.batch_size);
+        if ((e = nrc_steps_device(nrc, samples, total, max_batches, side.stream))) return e;
         merge_state_kernel<<<1, 64, 0, side.stream>>>(a.state, side.state);
         if ((e = hipEventRecord(side.join, side.stream))) return e;
         if ((e = hipStreamWaitEvent(s, side.join, 0))) return e;

@@ -585,8 +585,9 @@ class Context:
 
     def render_nrc(self, tlas: AccelerationStructure, records, camera, lights, nrc: "Nrc", width, height, spp=1,
                    train_paths=65536, max_depth=8, rr_depth=3, seed=1234, sample_offset=0, ray_offset=1e-4,
-                   film=None, accum64=None):
-        """rtx_cmd_render_nrc: one training pass of train_paths paths + spp cache-terminated samples per pixel."""
+                   film=None, accum64=None, graph=False):
+        """rtx_cmd_render_nrc: one training pass of train_paths paths + spp cache-terminated samples per pixel
+        (graph: record the frame into a HIP graph once and replay it)."""
         torch = _torch()
         rec = _dev_tensor(np.ascontiguousarray(records).view(np.uint8).reshape(-1), self.device) \
             if isinstance(records, np.ndarray) else records
@@ -597,8 +598,8 @@ class Context:
         nrec = rec.numel() // 48
         sbt = Sbt(rec.data_ptr(), nrec, 0, 1, 0, (ctypes.c_float * 3)(0, 0, 0), 0.0)
         cam = Camera((ctypes.c_float * 3)(*camera[:3]), float(camera[3]))
-        p = RenderParams(width, height, spp, max_depth, rr_depth, seed, sample_offset, 0, lt.data_ptr(),
-                         lt.shape[0], ray_offset)
+        p = RenderParams(width, height, spp, max_depth, rr_depth, seed, sample_offset,
+                         RTX_RENDER_GRAPH_BIT if graph else 0, lt.data_ptr(), lt.shape[0], ray_offset)
         self._check(self.lib.rtx_cmd_render_nrc(self.handle, tlas.address, ctypes.byref(sbt), ctypes.byref(cam),
                                                 ctypes.byref(p), nrc.handle, train_paths, film.data_ptr(),
                                                 accum64.data_ptr() if accum64 is not None else None))

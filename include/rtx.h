@@ -435,7 +435,7 @@ typedef struct rtx_render_params {
     uint32_t rr_depth;            /* Russian roulette when continuing from vertex index >= rr_depth (3) */
     uint32_t seed;                /* PCG seed (1234) */
     uint32_t sample_offset;       /* index of this call's first sample (progressive rendering) */
-    uint32_t flags;               /* RTX_RENDER_*_BIT (rtx_cmd_render_batch), else 0 */
+    uint32_t flags;               /* RTX_RENDER_*_BIT (rtx_cmd_render_batch, rtx_cmd_render_nrc), else 0 */
     const float* light_triangles_device; /* 12 floats per light: v0, v1, v2 (world), emitted radiance rgb */
     uint32_t light_triangle_count;
     float ray_offset;             /* secondary-ray origin offset along the normal, scene units (1e-4) */
@@ -467,7 +467,8 @@ typedef struct rtx_render_view {
  * concatenated in view order (sum of width x height pixels); accum (optional): FP64 rgb, same order.
  * With RTX_RENDER_GRAPH_BIT the call's launch sequence is recorded into a hipGraph the first time and replayed by
  * later calls with identical arguments apart from sample_offset (which the graph reads from device memory); the
- * graph is launched on the context stream. */
+ * graph is launched on the context stream.  A call with other arguments, a TLAS of other content (re)built at the
+ * same address, or another traversal configuration records a new graph.  rtx_cmd_render_nrc takes the bit too. */
 #define RTX_RENDER_GRAPH_BIT 0x1u
 int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_render_view* views,
                          uint32_t view_count, const rtx_render_params* params, float* films_device,
@@ -549,7 +550,9 @@ int rtx_nrc_set_params(rtx_nrc nrc, const float* params_host);
 /* Adam steps taken so far and the mean loss of the last step */
 int rtx_nrc_get_status(rtx_nrc nrc, uint32_t* steps, float* last_loss);
 
-/* One frame of the cache-accelerated integrator.  Every pixel traces params->spp paths that stop at vertex
+/* One frame of the cache-accelerated integrator (no host synchronisation: the Adam steps read the frame's sample count
+ * from device memory; with RTX_RENDER_GRAPH_BIT in params->flags the frame is recorded into a hipGraph once and
+ * replayed, as for rtx_cmd_render_batch).  Every pixel traces params->spp paths that stop at vertex
  * cfg->cache_vertex (instead of scattering there) and add throughput x the cache's prediction.  The first of these
  * passes also carries train_paths training paths from random pixels (sample index params->sample_offset) through
  * the same queues: each gives one training sample per vertex after the first (target: the path's radiance from

@@ -55,3 +55,50 @@ def test_hit_parity_counts_planted_errors():
     ref = compare_closest(_split(hb), tuv, o_ids, st)
     assert (ref["id_mismatch"], ref["miss_mismatch"], ref["t_bad"], ref["uv_bad"]) == (
         rep["id_mismatch"], rep["miss_mismatch"], rep["t_over_tol"], rep["uv_over_tol"])
+
+
+def test_tie_check_counts_planted_errors():
+    """Epsilon-tie rays are checked against their candidate sets: the oracle's own answers pass, and a hit outside
+    the set, a miss on a ray with a robustly inside candidate and a wrong t are each caught (by bench.hit_parity
+    too)."""
+    import oracle as O
+    from helpers import oracle_scene, small_mixed_scene
+    from pyrtx import scenes as S
+    O.reset()
+    sc = small_mixed_scene(seed=3, instances=24, freq=14, cells=20)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 160, 120), S.incoherent_rays(sc, 40000, seed=3)])
+    tuv, ids, st = O.trace(t, rays)
+    tie = st == 2
+    assert tie.sum() > 20
+    r = rays[tie]
+    c = O.trace_candidates(t, r)
+    inst, geom, prim, tt = ids[tie, 2].copy(), ids[tie, 1].copy(), ids[tie, 0].copy(), tuv[tie, 0].copy()
+    ok = O.check_ties(r, c, inst, geom, prim, tt)
+    assert ok["hit_not_candidate"] == 0 and ok["miss_not_allowed"] == 0 and ok["t_bad"] == 0, ok
+    hit = inst != 0xFFFFFFFF
+    k = int(np.nonzero(hit)[0][0])
+    bad = prim.copy()
+    bad[k] += 100000
+    assert O.check_ties(r, c, inst, geom, bad, tt)["hit_not_candidate"] == 1
+    bt = tt.copy()
+    bt[k] *= 1.01
+    assert O.check_ties(r, c, inst, geom, prim, bt)["t_bad"] == 1
+    # a ray whose candidate set has a robustly inside member must not miss
+    valid = np.arange(c["max_c"])[None, :] < np.minimum(c["count"], c["max_c"])[:, None]
+    robust = (c["inside"] & ~c["near"] & valid & (c["t"] - c["tunc"] >= r[:, 3:4]) &
+              (c["t"] + c["tunc"] <= r[:, 7:8])).any(1)
+    if robust.any():
+        j = int(np.nonzero(robust)[0][0])
+        mi = inst.copy()
+        mi[j] = 0xFFFFFFFF
+        assert O.check_ties(r, c, mi, geom, prim, tt)["miss_not_allowed"] == 1
+    # the same through the bench's full-size parity count
+    h = _records(tuv, ids[:, :3].copy())
+    h[:, 5] = ids[:, 2].view(np.int32)
+    rep = bench.hit_parity(h, tuv, ids, st, tlas=t, rays=rays)
+    assert rep["ties_checked"]["hit_not_candidate"] == 0 and rep["ties_checked"]["miss_not_allowed"] == 0
+    h2 = h.copy()
+    h2[np.nonzero(tie)[0][k], 3] += 100000
+    assert bench.hit_parity(h2, tuv, ids, st, tlas=t, rays=rays)["ties_checked"]["hit_not_candidate"] == 1
+    O.reset()

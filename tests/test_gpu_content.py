@@ -82,11 +82,11 @@ def test_opaque_scene_renders_whole_paths_after_a_cutout_scene(ctx, monkeypatch)
     monkeypatch.delenv("RTX_PATH_WAVEFRONT", raising=False)
     W, H = 64, 48
     cut, lights = _cutout_cornell(3, W, H)
-    _, t_cut = ctx.upload_scene(cut)
+    b_cut, t_cut = ctx.upload_scene(cut)  # the BLASes must outlive every trace of the TLAS
     f_cut = ctx.render(t_cut, cut.records, cut.camera, lights, W, H, spp=2).clone()
     assert ctx.render_form() == pyrtx.RTX_RENDER_FORM_WAVEFRONT
     opaque, lights2 = S.cornell_scene(3, width=W, height=H)
-    _, t_op = ctx.upload_scene(opaque)
+    b_op, t_op = ctx.upload_scene(opaque)
     f_whole = ctx.render(t_op, opaque.records, opaque.camera, lights2, W, H, spp=2).clone()
     assert ctx.render_form() == pyrtx.RTX_RENDER_FORM_WHOLE_PATH
     monkeypatch.setenv("RTX_PATH_WAVEFRONT", "1")

@@ -146,6 +146,45 @@ def test_render_nrc_frame_answers_queries_with_the_updated_cache(ctx):
     assert not torch.equal(f1, f_before)
 
 
+def test_render_nrc_graph_replays_match_direct_frames(ctx):
+    """NRC frames recorded into a HIP graph (both streams, no host synchronisation inside the frame) and replayed:
+    film, parameters and step count equal direct frames from an identical cache, frame after frame, with other GPU
+    work between the replays."""
+    import pyrtx
+    from pyrtx import scenes as S
+    _say("nrc frame graph replays")
+    W, H = 80, 60
+    sc, lights = S.cornell_scene(3, width=W, height=H)
+    blases, tlas = ctx.upload_scene(sc)
+    rec = torch.from_numpy(np.ascontiguousarray(sc.records).view(np.uint8).reshape(-1).copy()).cuda()
+    lt = torch.from_numpy(np.ascontiguousarray(lights, np.float32).copy()).cuda()
+    lo, hi = S.scene_bounds(sc)
+    direct = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=2048, cache_vertex=1)
+    graph = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=2048, cache_vertex=1)
+    fd = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    fg = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    try:
+        for fr in range(4):
+            ctx.render_nrc(tlas, rec, sc.camera, lt, direct, W, H, spp=2, train_paths=4096, sample_offset=fr,
+                           film=fd)
+            ctx.render_nrc(tlas, rec, sc.camera, lt, graph, W, H, spp=2, train_paths=4096, sample_offset=fr,
+                           film=fg, graph=True)
+            ctx.sync()
+            assert torch.equal(fd, fg), fr
+            assert np.array_equal(direct.params(), graph.params()), fr
+            assert direct.status() == graph.status(), fr
+            x = torch.randn(1 << 20, device="cuda").cumsum(0)  # unrelated work between replays
+            ctx.render(tlas, rec, sc.camera, lt, W, H, spp=1, sample_offset=100 + fr)
+            ctx.sync()
+            assert torch.isfinite(x).all()
+        steps, _ = graph.status()
+        assert steps >= 4
+        assert float(fg[..., :3].mean()) > 0.01
+    finally:
+        direct.close()
+        graph.close()
+
+
 def test_adam_step_matches_restatement(ctx):
     import pyrtx
     _say("nrc adam")
