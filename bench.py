@@ -610,10 +610,27 @@ def batch_bench(ctx, args, np, torch, pyrtx, S):
     lt = torch.from_numpy(np.ascontiguousarray(lights, np.float32).copy()).to(dev)
     ctx.sync()
     log(f"batch BLAS builds: batched {ms_build_batch:.1f} ms, one by one {ms_build_seq:.1f} ms")
-    tuned = None
+    tuned, width, width_ms = None, 8, {}
     if not args.no_autotune:
-        tuned, tms = ctx.autotune_render_batch(tlas, rec, views, lt, spp=1, max_depth=depth)
-        log(f"autotuned batched render: {tuned} ({tms:.1f} ms per 1-spp batch)")
+        # autotuner: BVH width (a build-time choice: the batch rebuilt at each width) x the traversal launch
+        # configuration (rtx_autotune_render_batch on each build); the fastest 1-spp batch render wins
+        best = None
+        base_cfg = ctx.config()
+        keep = [blases, tlas]  # every width's structures stay alive (inst below refers to the width-8 BLASes)
+        for w in (4, 6, 8):
+            flags = pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT | (pyrtx.RTX_BUILD_MAX_WIDTH(w) if w < 8 else 0)
+            bw = ctx.build_blas_batch(lists, flags) if w < 8 else blases
+            tw = ctx.build_tlas(pyrtx.instance_records(sc, [x.address for x in bw]), flags) if w < 8 else tlas
+            ctx.set_config(**base_cfg)
+            cfg, tms = ctx.autotune_render_batch(tw, rec, views, lt, spp=1, max_depth=depth)
+            keep += [bw, tw]
+            width_ms[w] = round(tms, 3)
+            log(f"autotuned batched render at width {w}: {cfg} ({tms:.2f} ms per 1-spp batch)")
+            if best is None or tms < best[0]:
+                best = (tms, w, cfg, bw, tw)
+        _, width, tuned, blases, tlas = best
+        ctx.set_config(**tuned)
+        log(f"batched render: width {width}")
     films = ctx.render_batch(tlas, rec, views, lt, spp=spp, max_depth=depth, sample_offset=1 << 20, graph=True)
     ctx.sync()
     reps = 3
@@ -651,7 +668,7 @@ def batch_bench(ctx, args, np, torch, pyrtx, S):
             "render_ms": {"batched_graph": round(ms_graph, 1), "batched_direct": round(ms_direct, 1),
                           "one_view_at_a_time": round(ms_seq, 1)},
             "rays_per_frame": int(rays), "mrays_per_s": round(rays / ms_graph / 1e3, 1),
-            "autotuned_trace_config": tuned}
+            "autotuned_trace_config": tuned, "autotuned_bvh_width": width, "ms_per_1spp_by_width": width_ms or None}
 
 
 if __name__ == "__main__":

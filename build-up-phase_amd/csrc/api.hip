@@ -369,6 +369,7 @@ int rtx_create_context(int device, void* hip_stream, rtx_context* out_ctx) {
     e = hipMalloc(&ctx->state, sizeof(rtx::DeviceState));
     if (e == hipSuccess) e = hipMemset(ctx->state, 0, sizeof(rtx::DeviceState));
     if (e != hipSuccess) {
+        if (ctx->state) (void)hipFree(ctx->state);
         delete ctx;
         return RTX_ERROR_INITIALIZATION_FAILED;
     }
@@ -409,9 +410,9 @@ int rtx_sync(rtx_context ctx) {
     uint32_t flags = 0;
     e = hipMemcpy(&flags, &ctx->state->error_flags, 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_sync flags");
+    uint32_t zero = 0;
+    if (flags) (void)hipMemcpy(&ctx->state->error_flags, &zero, 4, hipMemcpyHostToDevice);
     if (flags) {
-        uint32_t zero = 0;
-        (void)hipMemcpy(&ctx->state->error_flags, &zero, 4, hipMemcpyHostToDevice);
         if (flags & rtx::kErrStackOverflow)
             return fail(ctx, RTX_ERROR_STACK_OVERFLOW, "traversal stack overflow (more than %u entries per lane)",
                         ctx->last_stack);

@@ -885,10 +885,11 @@ __global__ __launch_bounds__(64) void refit_refine_kernel(uint32_t n, int32_t* l
 // x spread over at most j child slots of its parent:
 //   leaf:      C(x, j) = c_leaf * A(x)
 //   internal:  D(x, j) = min over k in [1, j-1] of C(l, k) + C(r, j - k)        (x dissolved into j slots)
-//              C(x, 1) = A(x) + D(x, 8)                                         (x becomes a wide node)
+//              C(x, 1) = A(x) + D(x, W)                                         (x becomes a wide node)
 //              C(x, j) = min(C(x, j - 1), D(x, j)),  j >= 2
 // computed bottom-up (second arrival merges, as the refit).  The collapse then takes, for each wide node x, the
-// children that D(x, 8) picks.
+// children that D(x, W) picks.  W is the build's node width (8, or 2..7 from RTX_BUILD_MAX_WIDTH): narrower trees
+// use the same 64-B node with fewer children.
 __device__ __forceinline__ void wide_table(uint32_t n, int32_t c, const float* nboxes, const float* wc, float c_leaf,
                                            float* t) {
     if (c < 0) {
@@ -922,7 +923,7 @@ __device__ __forceinline__ float wide_split(const float* tl, const float* tr, in
 __global__ __launch_bounds__(256) void wide_cost_kernel(uint32_t n, const int32_t* __restrict__ left,
                                                         const int32_t* __restrict__ right,
                                                         const int32_t* __restrict__ parent, const float* nboxes,
-                                                        float* wc, uint32_t* arrivals, float c_leaf) {
+                                                        float* wc, uint32_t* arrivals, float c_leaf, int width) {
     const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= n) return;
     int32_t node = load_coherent_i32(parent + (uint64_t)n - 1 + pos);
@@ -940,7 +941,7 @@ __global__ __launch_bounds__(256) void wide_cost_kernel(uint32_t n, const int32_
         for (int k = 0; k < 6; ++k) b[k] = load_coherent(nboxes + 6 * (uint64_t)node + k);
         float c[8];
         int k;
-        c[0] = box_area(b) + wide_split(tl, tr, 8, k);
+        c[0] = box_area(b) + wide_split(tl, tr, width, k);
         for (int j = 2; j <= 8; ++j) c[j - 1] = fminf(c[j - 2], wide_split(tl, tr, j, k));
 #pragma unroll
         for (int j = 0; j < 8; ++j) store_coherent(wc + 8 * (uint64_t)node + j, c[j]);
@@ -1106,7 +1107,7 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                                                              AsHeader* const* __restrict__ as_ptr,
                                                              const uint32_t* __restrict__ tblas_in,
                                                              uint32_t* __restrict__ tblas_out,
-                                                             const float* __restrict__ wc, float c_leaf) {
+                                                             const float* __restrict__ wc, float c_leaf, int width) {
     const uint32_t count = level_counts[level];
     if (count == 0) return;
     for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < count; ti += gridDim.x * blockDim.x) {
@@ -1122,12 +1123,12 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
             ch[0] = (int32_t)task.x;
             cnt = 1;
         } else if (wc) {
-            // the children D(task, 8) picks: a stack of (subtree, slots) pairs whose slots add up to 8
+            // the children D(task, W) picks: a stack of (subtree, slots) pairs whose slots add up to W
             int32_t sn[8];
             int si[8];
             int sp = 0;
             sn[sp] = (int32_t)task.x;
-            si[sp++] = 8;
+            si[sp++] = width;
             bool root = true;  // the task node itself is always split
             cnt = 0;
             while (sp > 0) {
@@ -1160,8 +1161,8 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
             ch[1] = right[task.x];
             cnt = 2;
         }
-        // greedy (no cost table): open the internal child with the largest surface area until 8 children
-        while (!wc && cnt < 8) {
+        // greedy (no cost table): open the internal child with the largest surface area until W children
+        while (!wc && cnt < width) {
             int best = -1;
             float best_a = -1.0f;
 #pragma unroll
@@ -1485,6 +1486,12 @@ constexpr float kLeafCostInstance = 1.0f;
 
 // Refit with treelet refinement; each further round restructures the tree the previous one left.  Then the costs
 // of the 8-wide collapse (wcost) over the final tree.
+// Children per wide node: RTX_BUILD_MAX_WIDTH(w) caps it at w (2..7), 0 = 8.
+int node_width(uint32_t flags) {
+    const uint32_t w = (flags >> RTX_BUILD_MAX_WIDTH_SHIFT) & 7u;
+    return w == 0u ? 8 : (w < 2u ? 2 : (int)w);
+}
+
 void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L, const float* prim_boxes,
                   hipStream_t s, const uint32_t* node_blas = nullptr, float c_leaf = kLeafCostTri) {
     const int rounds = treelet_rounds(flags, n);
@@ -1502,7 +1509,7 @@ void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L,
         wide_cost_kernel<<<(n + 255) / 256, 256, 0, s>>>(
             n, (const int32_t*)(scratch + L.left), (const int32_t*)(scratch + L.right),
             (const int32_t*)(scratch + L.parent), (const float*)(scratch + L.nboxes), (float*)(scratch + L.wcost),
-            arrivals, c_leaf);
+            arrivals, c_leaf, node_width(flags));
     }
 }
 
@@ -1556,7 +1563,8 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
                                                                   (const uint4*)(scratch + L.leaves), h, slots, levels,
                                                                   lv, tin, tout, nullptr, nullptr, nullptr,
                                                                   n > 1 ? (const float*)(scratch + L.wcost) : nullptr,
-                                                                  type == 1 ? kLeafCostInstance : kLeafCostTri);
+                                                                  type == 1 ? kLeafCostInstance : kLeafCostTri,
+                                                                  node_width(flags));
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, max_levels, err_flags);
     }
@@ -1728,7 +1736,7 @@ hipError_t update_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, u
                 n, (const int32_t*)(sc + L.left), (const int32_t*)(sc + L.right), (const uint32_t*)(sc + L.vals),
                 (const float*)(sc + L.nboxes), (const uint4*)(sc + L.leaves), h, (uint4*)((uint8_t*)dst + kHeaderBytes),
                 levels, lv, tin, tout, nullptr, nullptr, nullptr, n > 1 ? (const float*)(sc + L.wcost) : nullptr,
-                kLeafCostTri);
+                kLeafCostTri, node_width(flags));
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, kMaxLevels32, err_flags);
     }
@@ -1816,7 +1824,7 @@ hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t ngeom, const ui
                                                                   (const uint4*)(sc + L.leaves), nullptr, nullptr,
                                                                   levels, lv, tin, tout, as_ptr, bin, bout,
                                                                   n > 1 ? (const float*)(sc + L.wcost) : nullptr,
-                                                                  kLeafCostTri);
+                                                                  kLeafCostTri, node_width(flags));
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(nullptr, levels, kMaxLevels32, err_flags);
     }

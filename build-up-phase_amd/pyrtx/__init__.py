@@ -34,6 +34,14 @@ RTX_BUILD_TREELET_ROUNDS_SHIFT = 18
 def RTX_BUILD_TREELET_ROUNDS(r: int) -> int:
     return (r & 3) << RTX_BUILD_TREELET_ROUNDS_SHIFT
 
+
+RTX_BUILD_MAX_WIDTH_SHIFT = 20
+
+
+def RTX_BUILD_MAX_WIDTH(w: int) -> int:
+    """Children per BVH node (2..8; 8 is the default)."""
+    return (w & 7) << RTX_BUILD_MAX_WIDTH_SHIFT
+
 RTX_INDEX_TYPE_UINT32 = 1
 RTX_QUERY_CLOSEST_HIT = 0
 RTX_QUERY_ANY_HIT = 1

@@ -77,6 +77,10 @@ void* rtx_get_stream(rtx_context ctx);
  * least 7 * 2^r primitives. */
 #define RTX_BUILD_TREELET_ROUNDS_SHIFT 18u
 #define RTX_BUILD_TREELET_ROUNDS(r) (((uint32_t)(r) & 3u) << RTX_BUILD_TREELET_ROUNDS_SHIFT)
+/* Children per BVH node in bits 20-22: 0 = 8 (default), 2..7 caps the SAH-optimal wide collapse at that many (the
+ * same 64-B node, fewer children: shallower per-node work, deeper trees).  A TLAS and its BLASes may differ. */
+#define RTX_BUILD_MAX_WIDTH_SHIFT 20u
+#define RTX_BUILD_MAX_WIDTH(w) (((uint32_t)(w) & 7u) << RTX_BUILD_MAX_WIDTH_SHIFT)
 
 #define RTX_INDEX_TYPE_UINT32 1u                           /* VK_INDEX_TYPE_UINT32, main.cpp:736 */
 

@@ -616,3 +616,45 @@ def test_skip_closest_hit_program(ctx):
     miss = np.all(plain[..., :3] == np.float32([0, 0, 0.2]), -1)
     assert miss.any() and (~miss).any()
     assert np.array_equal(skip[miss], plain[miss]) and np.all(skip[~miss] == 0)
+
+
+def _children_per_node(storage):
+    """Walk a built structure from its root (64-B node slots after the 256-B header; internal-child mask byte 9,
+    present-child mask byte 10, first-child slot at bytes 12-15): the number of children of every node."""
+    raw = storage.cpu().numpy()
+    slots = raw[256:].reshape(-1, 64)
+    counts, todo = [], [0]
+    while todo:
+        s = todo.pop()
+        imask, vmask = int(slots[s, 9]), int(slots[s, 10])
+        base = int(slots[s, 12:16].view(np.uint32)[0])
+        counts.append(bin(vmask).count("1"))
+        for k in range(8):
+            if vmask >> k & 1 and imask >> k & 1:
+                todo.append(base + bin(vmask & ((1 << k) - 1)).count("1"))
+    return np.array(counts)
+
+
+def test_node_width_cap(ctx):
+    """RTX_BUILD_MAX_WIDTH(w) caps the SAH wide collapse at w children per node (same node format): every node has
+    at most w children, the trees get deeper, and traces stay exact against the oracle."""
+    import pyrtx
+    _say("node width cap")
+    sc = small_mixed_scene(seed=8, instances=20, freq=14, cells=20)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=8)])
+    tuv, ids, st = O.trace(t, rays)
+    depth = {}
+    for w in (2, 4, 8):
+        flags = pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT | (pyrtx.RTX_BUILD_MAX_WIDTH(w) if w < 8 else 0)
+        blases, tlas = ctx.upload_scene(sc, flags=flags)
+        for b in blases:
+            c = _children_per_node(b.storage)
+            assert c.max() <= w and c.min() >= 1
+        depth[w] = ctx.as_info(tlas).max_depth
+        g = _trace_closest(ctx, tlas, rays)
+        rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
+        _say(f"  width {w}: depth {depth[w]}, children per BLAS node {c.mean():.2f}, {rep['robust_hits']} hits")
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0
+        assert_ties_ok(rep)
+    assert depth[2] > depth[4] > depth[8]
