@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "build.h"
+#include "noalloc.h"
 #include "nrc.h"
 #include "pathtrace.h"
 #include "rtx_internal.h"
@@ -368,6 +369,8 @@ int rtx_create_context(int device, void* hip_stream, rtx_context* out_ctx) {
         ctx->cu_count = cus;
     e = hipMalloc(&ctx->state, sizeof(rtx::DeviceState));
     if (e == hipSuccess) e = hipMemset(ctx->state, 0, sizeof(rtx::DeviceState));
+    // the device's scratch array of rtx_cmd_trace_ray_buffer_noalloc (looked up once per device)
+    if (e == hipSuccess) e = rtx::noalloc_setup(device, nullptr);
     if (e != hipSuccess) {
         if (ctx->state) (void)hipFree(ctx->state);
         delete ctx;
@@ -412,6 +415,15 @@ int rtx_sync(rtx_context ctx) {
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_sync flags");
     uint32_t zero = 0;
     if (flags) (void)hipMemcpy(&ctx->state->error_flags, &zero, 4, hipMemcpyHostToDevice);
+    // flags raised by rtx_cmd_trace_ray_buffer_noalloc launches on this device (their state lives in the device's
+    // scratch array)
+    if (rtx::DeviceState* ns = rtx::noalloc_state(ctx->device)) {
+        uint32_t nf = 0;
+        if (hipMemcpy(&nf, &ns->error_flags, 4, hipMemcpyDeviceToHost) == hipSuccess && nf) {
+            (void)hipMemcpy(&ns->error_flags, &zero, 4, hipMemcpyHostToDevice);
+            flags |= nf;
+        }
+    }
     if (flags) {
         if (flags & rtx::kErrStackOverflow)
             return fail(ctx, RTX_ERROR_STACK_OVERFLOW, "traversal stack overflow (more than %u entries per lane)",
@@ -784,6 +796,47 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
     q.gstack = ctx->gstack;
     hipError_t e = rtx::trace_rays_launch(q, L, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_trace_ray_buffer");
+    return RTX_SUCCESS;
+}
+
+int rtx_cmd_trace_ray_buffer_noalloc(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device,
+                                     uint32_t ray_count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                                     rtx_hit* hits_device, uint32_t* occluded_device) {
+    if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
+    int r = check_tlas(ctx, tlas_address);
+    if (r) return r;
+    if (mode != RTX_QUERY_CLOSEST_HIT && mode != RTX_QUERY_ANY_HIT)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "mode must be RTX_QUERY_CLOSEST_HIT or RTX_QUERY_ANY_HIT");
+    if (ray_count > 0 && !rays_device) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "rays is NULL");
+    if (ray_count > 0 && mode == RTX_QUERY_CLOSEST_HIT && !hits_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "hits is NULL");
+    if (ray_count > 0 && mode == RTX_QUERY_ANY_HIT && !occluded_device)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "occluded is NULL");
+    if (!aligned(rays_device, 16)) return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "rays must be 16-B aligned");
+    rtx::TraceRequest q{};
+    q.tlas = (const void*)(uintptr_t)tlas_address;
+    q.mode = mode;
+    q.n = ray_count;
+    q.cull_mask = cull_mask & 0xFFu;
+    q.ray_flags = ray_flags;
+    q.rays = rays_device;
+    q.hits = hits_device;
+    q.occluded = occluded_device;
+    q.stats = false;  // the traversal counters live in the context's state, which this variant does not use
+    const TlasProps tp = tlas_props(ctx, tlas_address);
+    q.content = tp.content;
+    const rtx::TraceLaunch L = launch_shape(ctx, tp.depth);
+    ctx->last_stack = L.lds_depth + L.gdepth;
+    uint64_t need = 0;
+    const hipError_t e = rtx::noalloc_trace(ctx->device, q, L, ctx->stream, &need);
+    if (e == hipErrorNotInitialized)
+        return fail(ctx, RTX_ERROR_INITIALIZATION_FAILED, "no scratch array on device %d", ctx->device);
+    if (e == hipErrorOutOfMemory)
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY,
+                    "the launch needs %llu bytes of scratch (%u stack entries per lane in the global spill area), the "
+                    "no-alloc entry point has %llu: use fewer blocks per CU, more LDS stack entries or a shallower TLAS",
+                    (unsigned long long)need, L.gdepth, (unsigned long long)rtx::noalloc_scratch_bytes());
+    if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_trace_ray_buffer_noalloc");
     return RTX_SUCCESS;
 }
 
@@ -1569,6 +1622,7 @@ void* rtx_get_proc_addr(const char* name) {
         RTX_PROC(rtx_obj_free),
         RTX_PROC(rtx_cmd_trace_rays),
         RTX_PROC(rtx_cmd_trace_ray_buffer),
+        RTX_PROC(rtx_cmd_trace_ray_buffer_noalloc),
         RTX_PROC(rtx_set_trace_stats),
         RTX_PROC(rtx_get_trace_stats),
         RTX_PROC(rtx_set_trace_config),

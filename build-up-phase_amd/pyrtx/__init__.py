@@ -188,6 +188,7 @@ SIGNATURES = {
     "rtx_cmd_trace_rays": (ctypes.c_int, [_P, _U64, ctypes.POINTER(Sbt), ctypes.POINTER(Camera),
                                           ctypes.POINTER(TraceParams), _U32, _U32, _P]),
     "rtx_cmd_trace_ray_buffer": (ctypes.c_int, [_P, _U64, _P, _U32, _U32, _U32, _U32, _P, _P]),
+    "rtx_cmd_trace_ray_buffer_noalloc": (ctypes.c_int, [_P, _U64, _P, _U32, _U32, _U32, _U32, _P, _P]),
     "rtx_set_trace_stats": (ctypes.c_int, [_P, ctypes.c_int]),
     "rtx_get_trace_stats": (ctypes.c_int, [_P, ctypes.POINTER(TraceStats)]),
     "rtx_set_trace_config": (ctypes.c_int, [_P, ctypes.POINTER(TraceConfig)]),

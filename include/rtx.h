@@ -364,6 +364,19 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
                              uint32_t ray_count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
                              rtx_hit* hits_device, uint32_t* occluded_device);
 
+/* rtx_cmd_trace_ray_buffer that allocates nothing.  No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc), where
+ * rtx_cmd_trace_ray_buffer allocates its traversal-stack spill area on first use of a launch shape.  The ray counter,
+ * error flags and stack spill live in a 48 MiB array that the library keeps on each device until the process ends:
+ * enough for the default launch shape (4 blocks of 256 per CU) on a TLAS whose depth with its deepest BLAS is at most
+ * 16 BVH levels; a launch needing more returns RTX_ERROR_OUT_OF_DEVICE_MEMORY (with the byte count in
+ * rtx_last_error_string).  Every call on one GPU shares that array: calls that overlap on one GPU (several streams
+ * or contexts) race.  Results equal rtx_cmd_trace_ray_buffer's; traversal statistics are not collected; errors are
+ * reported by rtx_sync. */
+int rtx_cmd_trace_ray_buffer_noalloc(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device,
+                                     uint32_t ray_count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
+                                     rtx_hit* hits_device, uint32_t* occluded_device);
+
 /* Per-ray traversal counters of the last rtx_cmd_trace_ray_buffer call issued with stats enabled:
  * total 64-B node fetches, triangle tests and instance entries summed over all rays. */
 typedef struct rtx_trace_stats {
