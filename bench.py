@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--no-render", action="store_true", help="skip the path-tracing / NRC equal-relMSE part")
     ap.add_argument("--no-autotune", action="store_true", help="use the default traversal configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-scene part")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the ray queries on the dense path-tracing scene (every ray enters a BLAS)")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched part")
     ap.add_argument("--no-query", action="store_true",
                     help="skip the 20M-triangle ray-query part (profiling the path-tracing part alone)")
@@ -210,6 +212,31 @@ def main():
             any_res[name] = W * H * 3 / a.elapsed_time(b) / 1e3
         log(f"any-hit: {any_res}")
 
+    # tuned shapes of 5 waves per SIMD run the 96-VGPR kernel variant, which spills a few registers to scratch: time
+    # each such shape against the same configuration at 4 waves per SIMD (the spill-free 128-VGPR variant)
+    variant_check = {}
+    for key, cfg in tuned.items():
+        if (cfg["block_size"] // 64) * cfg["blocks_per_cu"] // 4 < 5:
+            continue
+        r = rc if key.endswith("coherent") and not key.endswith("incoherent") else ri
+        mode = pyrtx.RTX_QUERY_CLOSEST_HIT if key.startswith("closest") else pyrtx.RTX_QUERY_ANY_HIT
+        res = {}
+        for tag, c in (("w5", cfg), ("w4", dict(cfg, blocks_per_cu=16 // (cfg["block_size"] // 64)))):
+            ctx.set_config(**c)
+            trace(r, mode)
+            ctx.sync()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(3):
+                trace(r, mode)
+            b.record()
+            torch.cuda.synchronize()
+            res[f"{tag}_ms"] = round(a.elapsed_time(b) / 3, 3)
+            res[f"{tag}_blocks_per_cu"] = c["blocks_per_cu"]
+        variant_check[key] = res
+    if variant_check:
+        log(f"96- vs 128-VGPR variant at the tuned shapes: {json.dumps(variant_check)}")
+
     # roofline of the traversal kernel (both launches of a step)
     b_coh, b_inc = bytes_per_ray(stats["coherent"]), bytes_per_ray(stats["incoherent"])
     achieved = (b_coh * W * H + b_inc * W * H) / ((ms_coh + ms_inc) * 1e-3) / 1e9
@@ -251,6 +278,10 @@ def main():
         O.reset()
         log(f"parity vs the FP64 oracle: {json.dumps(parity)}")
 
+    del rc, ri, hits, occl
+    torch.cuda.empty_cache()
+    ctx.set_config(**default_cfg)
+    dense = None if args.no_dense else dense_query_bench(ctx, args, np, torch, pyrtx, S, cores)
     ctx.set_config(**default_cfg)
     render = None if args.no_render else render_bench(ctx, args, np, torch, pyrtx, S)
     ctx.set_config(**default_cfg)
@@ -278,6 +309,8 @@ def main():
         "any_hit_mrays": {k: round(v, 1) for k, v in any_res.items()},
         "traversal_counters_per_ray": stats,
         "autotuned_trace_config": tuned or None,
+        "variant_96_vs_128_vgpr": variant_check or None,
+        "dense_scene_queries": dense,
         "build": {"blas_tlas_ms_incl_upload": round(build_s * 1e3, 1),
                   "nodes": int(sum(i.node_count for i in info)), "max_depth": int(tinfo.max_depth)},
         "roofline": roofline,
@@ -423,6 +456,81 @@ def measured_traffic():
     return {"coherent": int(m.group(1)), "incoherent": int(m.group(2)),
             "source": "profiles/trace_kernel_hbm_traffic_20M_query_scene.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
                       "copy-calibrated" + (f", measured at commit {c.group(1)})" if c else ")")}
+
+
+def dense_query_bench(ctx, args, np, torch, pyrtx, S, cores):
+    """Closest- and any-hit ray queries where every ray traverses a BLAS: the closed Cornell box with the displaced
+    level-8 icosphere (1,310,744 triangles), 4096 x 4096 camera rays from the scene's camera and 16,777,216
+    incoherent rays (origins uniform in the box, directions uniform on the sphere).  Each set and query type is
+    autotuned, timed over 5 launches, counted (node fetches, triangle tests per ray: HBM roofline of the launch) and
+    checked against the FP64 oracle on every 64th ray (epsilon ties against their candidate sets)."""
+    W = H = 4096
+    sc, _ = S.cornell_scene(8, width=W, height=H)
+    dev = ctx.device
+    blases, tlas = ctx.upload_scene(sc)
+    ctx.sync()
+    sets = {"coherent": S.camera_rays(sc.camera, W, H), "incoherent": S.incoherent_rays(sc, W * H, seed=77)}
+    hits = torch.empty((W * H, 6), dtype=torch.int32, device=dev)
+    occl = torch.empty((W * H,), dtype=torch.int32, device=dev)
+    default_cfg = ctx.config()
+    out = {"workload": "closed Cornell box + displaced level-8 icosphere (1,310,744 triangles); 16,777,216 camera "
+                       "rays (4096 x 4096) and 16,777,216 incoherent rays per launch",
+           "closest_hit_mrays": {}, "any_hit_mrays": {}, "traversal_counters_per_ray": {}, "roofline": {},
+           "autotuned_trace_config": {}, "parity_vs_oracle": {}}
+    ot = None
+    if args.parity_sample > 0:
+        import oracle as O
+        O.reset()
+        handles = [O.blas_create([(g.vertices, g.indices, g.transform) for g in geoms]) for geoms in sc.blas]
+        ot = O.tlas_create(O.instance_records(sc.transforms, [handles[b] for b in sc.blas_of_instance],
+                                              sc.custom_index, sc.mask, sc.sbt_offset, sc.flags))
+    for name, rays in sets.items():
+        r = torch.from_numpy(rays).to(dev)
+        for qn, mode, res in (("closest", pyrtx.RTX_QUERY_CLOSEST_HIT, hits), ("any", pyrtx.RTX_QUERY_ANY_HIT, occl)):
+            ctx.set_config(**default_cfg)
+            if not args.no_autotune:
+                cfg, _ = ctx.autotune(tlas, r, mode, reps=2)
+                out["autotuned_trace_config"][f"{qn}/{name}"] = cfg
+            if qn == "closest":
+                ctx.set_stats(True)
+                ctx.trace_ray_buffer(tlas, r, mode, 0xFF, res)
+                ctx.sync()
+                st = ctx.stats()
+                ctx.set_stats(False)
+                n = max(1, st["rays"])
+                cnt = {k: st[k] / n for k in ("node_fetches", "triangle_tests", "instance_entries")}
+                cnt["hit_fraction"] = float((hits[:, 5].cpu().numpy().view(np.uint32) != 0xFFFFFFFF).mean())
+                out["traversal_counters_per_ray"][name] = cnt
+            ctx.trace_ray_buffer(tlas, r, mode, 0xFF, res)
+            ctx.sync()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(5):
+                ctx.trace_ray_buffer(tlas, r, mode, 0xFF, res)
+            b.record()
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / 5
+            out[f"{qn}_hit_mrays"][name] = round(W * H / ms / 1e3, 1)
+            if qn == "closest":
+                c = out["traversal_counters_per_ray"][name]
+                bpr = 32 + 24 + 64 * c["node_fetches"] + 48 * c["triangle_tests"] + 64 * c["instance_entries"]
+                gbs = bpr * W * H / (ms * 1e-3) / 1e9
+                out["roofline"][name] = {"bound": "hbm", "bytes_per_ray": round(bpr, 1), "launch_ms": round(ms, 3),
+                                         "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
+                if ot is not None:
+                    sample = rays[::64]
+                    o = O.trace(ot, sample, threads=cores)
+                    out["parity_vs_oracle"][name] = hit_parity(hits[::64].cpu().numpy(), *o, tlas=ot, rays=sample)
+        del r
+        log(f"dense scene {name}: closest {out['closest_hit_mrays'][name]} / any {out['any_hit_mrays'][name]} "
+            f"Mrays/s, counters {out['traversal_counters_per_ray'][name]}, parity {out['parity_vs_oracle'].get(name)}")
+    if ot is not None:
+        O.reset()
+    ctx.sync()
+    del hits, occl, blases, tlas
+    torch.cuda.empty_cache()
+    return out
 
 
 def render_bench(ctx, args, np, torch, pyrtx, S):

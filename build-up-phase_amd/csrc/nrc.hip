@@ -145,7 +145,8 @@ __device__ __forceinline__ void encode(const rtx_nrc_query& q, const Norm& nm, i
 // ReLU and repack 4 output tiles into the 2 B fragments of the next layer (k order kperm).  The values are
 // rounded to BF16 first and the ReLU is taken on the BF16 bit patterns as 16-bit integers (packed integer max with
 // 0): negative values, -0 included, have the sign bit set and become +0, the others are unchanged, which is
-// bit for bit the rounding of max(0, x).
+// bit for bit the rounding of max(0, x) for every x but NaN.  A NaN with the sign bit clear stays NaN (fmaxf(0, x)
+// would give 0), so a network whose activations diverged shows NaN in its outputs and loss instead of hiding it.
 __device__ __forceinline__ void relu_pack(const f32x4* acc, bf16x8* out) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
