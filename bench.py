@@ -452,7 +452,7 @@ def measured_traffic():
     m = re.search(r"bench-traffic-bytes-per-launch: coherent (\d+) incoherent (\d+)", txt)
     if not m:
         return None
-    c = re.search(r"measured at commit ([0-9a-f]{7,40})", txt)
+    c = re.search(r"measured at commit ([0-9a-f]{7,40})", txt, re.I)
     return {"coherent": int(m.group(1)), "incoherent": int(m.group(2)),
             "source": "profiles/trace_kernel_hbm_traffic_20M_query_scene.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
                       "copy-calibrated" + (f", measured at commit {c.group(1)})" if c else ")")}
