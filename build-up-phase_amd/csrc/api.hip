@@ -37,6 +37,7 @@ struct rtx_context_t {
     uint32_t last_stack = 0;   // stack entries per lane of the last traversal launch (error message)
     uint32_t render_form = RTX_RENDER_FORM_NONE;
     bool stats = false;
+    bool used_noalloc = false;  // rtx_sync also reads the device's no-alloc error flags
     void* pt_scratch = nullptr;
     uint64_t pt_scratch_bytes = 0;
     std::vector<rtx_nrc_t*> caches;  // live caches; detached when the context is destroyed first
@@ -417,7 +418,7 @@ int rtx_sync(rtx_context ctx) {
     if (flags) (void)hipMemcpy(&ctx->state->error_flags, &zero, 4, hipMemcpyHostToDevice);
     // flags raised by rtx_cmd_trace_ray_buffer_noalloc launches on this device (their state lives in the device's
     // scratch array)
-    if (rtx::DeviceState* ns = rtx::noalloc_state(ctx->device)) {
+    if (rtx::DeviceState* ns = ctx->used_noalloc ? rtx::noalloc_state(ctx->device) : nullptr) {
         uint32_t nf = 0;
         if (hipMemcpy(&nf, &ns->error_flags, 4, hipMemcpyDeviceToHost) == hipSuccess && nf) {
             (void)hipMemcpy(&ns->error_flags, &zero, 4, hipMemcpyHostToDevice);
@@ -828,6 +829,7 @@ int rtx_cmd_trace_ray_buffer_noalloc(rtx_context ctx, uint64_t tlas_address, con
     const rtx::TraceLaunch L = launch_shape(ctx, tp.depth);
     ctx->last_stack = L.lds_depth + L.gdepth;
     uint64_t need = 0;
+    ctx->used_noalloc = true;
     const hipError_t e = rtx::noalloc_trace(ctx->device, q, L, ctx->stream, &need);
     if (e == hipErrorNotInitialized)
         return fail(ctx, RTX_ERROR_INITIALIZATION_FAILED, "no scratch array on device %d", ctx->device);
