@@ -237,6 +237,21 @@ def main():
     if variant_check:
         log(f"96- vs 128-VGPR variant at the tuned shapes: {json.dumps(variant_check)}")
 
+    # the box's own streaming rate: a 1 GiB device-to-device copy (read + write), best of 3, as context for the
+    # published peak
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    copy_ms = []
+    for _ in range(3):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        torch.cuda.synchronize()
+        copy_ms.append(a.elapsed_time(b))
+    copy_gbs = 2 * src.numel() * 4 / (min(copy_ms) * 1e-3) / 1e9
+    del src, dst
+
     # roofline of the traversal kernel (both launches of a step)
     b_coh, b_inc = bytes_per_ray(stats["coherent"]), bytes_per_ray(stats["incoherent"])
     achieved = (b_coh * W * H + b_inc * W * H) / ((ms_coh + ms_inc) * 1e-3) / 1e9
@@ -249,7 +264,8 @@ def main():
                 "traffic_source": None if traffic is None else traffic["source"],
                 "kernel": "trace_kernel<closest> (persistent BVH8 traversal)",
                 "bytes_per_ray": {"coherent": round(b_coh, 1), "incoherent": round(b_inc, 1)},
-                "launch_ms": {"coherent": round(ms_coh, 3), "incoherent": round(ms_inc, 3)}}
+                "launch_ms": {"coherent": round(ms_coh, 3), "incoherent": round(ms_inc, 3)},
+                "measured_copy_gbs": round(copy_gbs, 1)}
 
     cpu = None
     parity = {}

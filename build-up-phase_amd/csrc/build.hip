@@ -780,14 +780,16 @@ __device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old
     new_h = T.lh[0];
 }
 
-// Treelet refinement rounds: default kTreeletRounds, RTX_BUILD_TREELET_ROUNDS_* overrides, NO_TREELET disables;
-// round r restructures nodes with at least kTreeletMinPrims[r] primitives.
+// Treelet refinement rounds: default kTreeletRounds, none under PREFER_FAST_BUILD, RTX_BUILD_TREELET_ROUNDS_*
+// overrides both, NO_TREELET disables; round r restructures nodes with at least kTreeletMinPrims[r] primitives.
+// (Each round costs ~17 ms on 20M triangles and buys 0.5-1.5% of closest-hit speed on the query scenes.)
 constexpr int kTreeletRounds = 2;
 constexpr uint32_t kTreeletMinPrims[3] = {7, 14, 28};
 int treelet_rounds(uint32_t flags, uint32_t n) {
     if ((flags & RTX_BUILD_NO_TREELET_BIT) || n < (uint32_t)kTreeletLeaves) return 0;
     const uint32_t r = (flags >> RTX_BUILD_TREELET_ROUNDS_SHIFT) & 3u;
-    return r ? (int)r : kTreeletRounds;
+    if (r) return (int)r;
+    return (flags & RTX_BUILD_PREFER_FAST_BUILD_BIT) ? 0 : kTreeletRounds;
 }
 
 // One wave per block.  min_prims = 0xFFFFFFFF: plain refit.

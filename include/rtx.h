@@ -70,11 +70,12 @@ void* rtx_get_stream(rtx_context ctx);
 #define RTX_BUILD_ALLOW_UPDATE_BIT 0x1u                    /* BLAS keeps its hierarchy for rtx_cmd_update_blas */
 #define RTX_BUILD_ALLOW_COMPACTION_BIT 0x2u                /* accepted; every structure can be compacted */
 #define RTX_BUILD_PREFER_FAST_TRACE_BIT 0x4u               /* main.cpp:751, 887 */
-#define RTX_BUILD_PREFER_FAST_BUILD_BIT 0x8u
+#define RTX_BUILD_PREFER_FAST_BUILD_BIT 0x8u               /* no treelet refinement unless
+                                                              RTX_BUILD_TREELET_ROUNDS asks for it */
 #define RTX_BUILD_MORTON63_BIT 0x10000u                    /* 63-bit Morton keys (default: 30-bit) */
 #define RTX_BUILD_NO_TREELET_BIT 0x20000u                  /* skip SAH treelet refinement */
-/* Treelet refinement rounds in bits 18-19 (0 = default 2, 1..3 rounds); round r restructures nodes with at
- * least 7 * 2^r primitives. */
+/* Treelet refinement rounds in bits 18-19 (0 = default: 2, or none with PREFER_FAST_BUILD; 1..3 rounds); round r
+ * restructures nodes with at least 7 * 2^r primitives. */
 #define RTX_BUILD_TREELET_ROUNDS_SHIFT 18u
 #define RTX_BUILD_TREELET_ROUNDS(r) (((uint32_t)(r) & 3u) << RTX_BUILD_TREELET_ROUNDS_SHIFT)
 /* Children per BVH node in bits 20-22: 0 = 8 (default), 2..7 caps the SAH-optimal wide collapse at that many (the

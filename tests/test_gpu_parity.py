@@ -658,3 +658,32 @@ def test_node_width_cap(ctx):
         assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0
         assert_ties_ok(rep)
     assert depth[2] > depth[4] > depth[8]
+
+
+def test_prefer_fast_build_skips_treelets(ctx):
+    """PREFER_FAST_BUILD builds without treelet refinement (each BLAS reports the SAH cost of a NO_TREELET build),
+    RTX_BUILD_TREELET_ROUNDS still asks for rounds under it (the cost of a PREFER_FAST_TRACE build, lower), and
+    every variant traces exactly against the oracle."""
+    import pyrtx
+    _say("PREFER_FAST_BUILD and treelet rounds")
+    sc = small_mixed_scene(seed=9, instances=12, freq=24, cells=24)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 96, 72), S.incoherent_rays(sc, 12000, seed=9)])
+    tuv, ids, st = O.trace(t, rays)
+    fast_trace = pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT
+    fast_build = pyrtx.RTX_BUILD_PREFER_FAST_BUILD_BIT
+    cost = {}
+    for name, flags in (("fast_build", fast_build), ("no_treelet", fast_trace | pyrtx.RTX_BUILD_NO_TREELET_BIT),
+                        ("fast_build_2_rounds", fast_build | pyrtx.RTX_BUILD_TREELET_ROUNDS(2)),
+                        ("fast_trace", fast_trace)):
+        blases, tlas = ctx.upload_scene(sc, flags=flags)
+        ctx.sync()
+        cost[name] = [ctx.as_info(b).sah_cost for b in blases]
+        g = _trace_closest(ctx, tlas, rays)
+        rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0, name
+        assert_ties_ok(rep)
+    _say(f"  SAH cost per BLAS: fast build {cost['fast_build']}, fast trace {cost['fast_trace']}")
+    assert cost["fast_build"] == cost["no_treelet"]
+    assert cost["fast_build_2_rounds"] == cost["fast_trace"]
+    assert all(a < b for a, b in zip(cost["fast_trace"], cost["fast_build"]))
