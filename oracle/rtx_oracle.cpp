@@ -657,6 +657,12 @@ int oracle_blas_create(int ngeom, const float* const* vtx, const uint32_t* const
                 else
                     std::memcpy(&t.v[3 * k], v, 12);
             }
+            // inactive triangle (Vulkan spec "Inactive Primitives and Instances": a NaN X component makes a
+            // triangle inactive; other NaN / Inf components are undefined, and the build treats them the same way):
+            // never hit, but it keeps its primitive index, so the triangles after it keep theirs
+            bool finite = true;
+            for (int k = 0; k < 9; ++k) finite = finite && std::isfinite(t.v[k]);
+            if (!finite) continue;
             t.prim = p;
             t.geom = (uint32_t)g;
             t.opaque = !geom_flags || (geom_flags[g] & 0x1u);

@@ -687,3 +687,84 @@ def test_prefer_fast_build_skips_treelets(ctx):
     assert cost["fast_build"] == cost["no_treelet"]
     assert cost["fast_build_2_rounds"] == cost["fast_trace"]
     assert all(a < b for a, b in zip(cost["fast_trace"], cost["fast_build"]))
+
+
+def test_inactive_and_degenerate_geometry_and_rays(ctx):
+    """Triangles with a NaN or infinite vertex component (inactive: never hit, primitive indices kept), a zero-area
+    triangle, and degenerate rays (empty and single-point [tmin, tmax], origins on the plane, rays in the plane,
+    axis-parallel rays) give the oracle's results, closest and any hit, through a transformed instance."""
+    import pyrtx
+    from test_oracle import degenerate_geometry
+    _say("inactive / degenerate triangles and rays")
+    v, idx = degenerate_geometry()
+    dead = v.copy()
+    dead[:, 0] = np.nan  # a BLAS whose triangles are all inactive, instanced beside the first
+    rng = np.random.default_rng(21)
+    xf = np.stack([S.translation(1.0, -2.0, 3.0), S.translation(1.0, 0.5, 3.0)]).astype(np.float32)
+    sc = S.Scene(blas=[[S.Geometry(v, idx)], [S.Geometry(dead, idx)]], transforms=xf, blas_of_instance=np.array([0, 1]),
+                 custom_index=np.zeros(2, np.uint32), mask=np.full(2, 0xFF, np.uint32),
+                 sbt_offset=np.zeros(2, np.uint32), flags=np.zeros(2, np.uint32), meta={"name": "degenerate"})
+    n = 4000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0] = rng.uniform(-2, 16, n) + 1.0
+    rays[:, 1] = rng.uniform(-1, 5.5, n) - 2.0
+    rays[:, 2] = 3.0 + rng.choice([1.0, -1.0, 0.0], n)
+    rays[:, 6] = np.where(rays[:, 2] > 3.0, -1.0, 1.0)
+    rays[:, 3] = rng.choice([0.0, 0.5, 1.0, 2.0], n)
+    rays[:, 7] = rng.choice([0.5, 1.0, 10.0, np.inf], n)
+    k = n // 4  # a quarter of the rays travel in the triangles' plane along +x or +y
+    rays[:k, 2] = 3.0
+    rays[:k, 4:7] = [[1, 0, 0], [0, 1, 0]][0]
+    rays[k // 2:k, 4:7] = [0, 1, 0]
+    # and 1,000 ordinary rays straight down onto the boxes of the two active triangles (world x in [1, 3] and
+    # [13, 15], y in [-2, 0]; each triangle fills half its box)
+    aim = np.zeros((1000, 8), np.float32)
+    aim[:, 0] = rng.uniform(1.0, 3.0, 1000) + 12.0 * rng.integers(0, 2, 1000)
+    aim[:, 1] = rng.uniform(-2.0, 0.0, 1000)
+    aim[:, 2] = 5.0
+    aim[:, 6] = -1.0
+    aim[:, 7] = 10.0
+    rays = np.concatenate([rays, aim])
+    t = oracle_scene(sc)
+    tuv, ids, st = O.trace(t, rays)
+    occl_o = O.trace(t, rays, any_hit=True)[2]
+    blases, tlas = ctx.upload_scene(sc)
+    assert ctx.as_info(blases[0]).primitive_count == 5 and ctx.as_info(blases[1]).primitive_count == 5
+    g = _trace_closest(ctx, tlas, rays)
+    rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
+    _say(f"  {rep}")
+    assert rep["robust_hits"] > 400
+    assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0
+    assert_ties_ok(rep)
+    hit = g[5] != 0xFFFFFFFF
+    assert set(int(p) for p in g[3][hit]) <= {0, 4}  # the inactive and degenerate triangles are never hit
+    assert set(int(i) for i in g[5][hit]) <= {0}  # nor anything of the all-inactive BLAS
+    r = torch.from_numpy(rays).cuda()
+    occ = ctx.trace_ray_buffer(tlas, r, pyrtx.RTX_QUERY_ANY_HIT).cpu().numpy()
+    ctx.sync()
+    robust = occl_o != 2
+    assert np.array_equal(occ[robust] != 0, occl_o[robust] == 1)
+
+
+def test_scene_far_from_the_origin(ctx):
+    """The mixed scene moved 10^5 units away from the world origin, with the camera and the incoherent rays moved
+    along: the TLAS is built and traversed in FP32 world coordinates of size 10^5, the BLASes in their own rebased
+    frames, and the results still match the FP64 oracle (ties checked)."""
+    _say("scene far from the origin")
+    sc = small_mixed_scene(seed=11, instances=16, freq=16, cells=16)
+    off = np.array([1.0e5, -3.0e4, 2.0e5], np.float32)
+    sc.transforms = sc.transforms.copy()
+    sc.transforms[:, :, 3] += off
+    cam = list(sc.camera)
+    cam[:3] = [c + o for c, o in zip(cam[:3], off)]
+    sc.camera = tuple(cam)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=11)])
+    tuv, ids, st = O.trace(t, rays)
+    blases, tlas = ctx.upload_scene(sc)
+    g = _trace_closest(ctx, tlas, rays)
+    rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
+    _say(f"  {rep}")
+    assert rep["robust_hits"] > 2000
+    assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0
+    assert_ties_ok(rep)

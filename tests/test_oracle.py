@@ -138,6 +138,43 @@ def test_known_answers_axis_aligned():
     assert ids[0, 2] == 0 and ids[2, 2] == O.MISS
 
 
+def degenerate_geometry():
+    """Five triangles side by side at z = 0 along x (triangle p covers x in [3p, 3p + 2], y in [0, 2]): 0 and 4 are
+    ordinary, 1 has a NaN X component (inactive under Vulkan's rule), 2 an infinite Y component, 3 is collinear
+    (zero area)."""
+    v = []
+    for p in range(5):
+        x0 = 3.0 * p
+        v += [[x0, 0, 0], [x0 + 2, 0, 0], [x0, 2, 0]]
+    v = np.array(v, np.float32)
+    v[3 * 1 + 1, 0] = np.nan
+    v[3 * 2 + 2, 1] = np.inf
+    v[3 * 3 + 2] = [9.0 + 1.0, 0.0, 0.0]  # on the edge v0 v1: collinear
+    return v, np.arange(15, dtype=np.uint32).reshape(5, 3)
+
+
+def test_inactive_and_degenerate_triangles_known_answers():
+    """Triangles with a NaN or infinite vertex component are inactive (never hit) and keep their primitive index;
+    a zero-area triangle is never hit; rays with an empty [tmin, tmax] miss; axis-parallel rays hit exactly."""
+    v, idx = degenerate_geometry()
+    b = O.blas_create([(v, idx, None)])
+    t = O.tlas_create(O.instance_records(S.translation()[None], [b]))
+    # straight down -z through (3p + 0.5, 0.5): hits triangle p at t = 1 where it is active and not degenerate
+    rays = np.array([[3 * p + 0.5, 0.5, 1, 0, 0, 0, -1, 10] for p in range(5)] +
+                    [[10.0, 0.0, 1, 0, 0, 0, -1, 10],        # onto the degenerate triangle's segment
+                     [0.5, 0.5, 1, 2.0, 0, 0, -1, 1.5],      # tmin > tmax
+                     [0.5, 0.5, 1, 1.0, 0, 0, -1, 1.0],      # tmin = tmax = the hit distance
+                     [0.5, 0.5, 0.0, 0, 0, 0, 1, 10],        # origin on the triangle's plane, going up: t = 0
+                     [-1.0, 0.5, 0.0, 0, 1, 0, 0, 100]],     # in the triangle's plane (parallel): no hit
+                    np.float32)
+    tuv, ids, st = O.trace(t, rays)
+    # status 1 robust hit, 0 robust miss, 2 epsilon-tie: a hit exactly at tmin or tmax (rays 7 and 8) is a tie,
+    # which an FP32 traversal may report either way
+    assert list(st) == [1, 0, 0, 0, 1, 0, 0, 2, 2, 0], list(st)
+    assert ids[0, 0] == 0 and ids[4, 0] == 4  # primitive indices keep the inactive triangles' positions
+    assert tuv[0, 0] == 1.0 and tuv[4, 0] == 1.0 and tuv[7, 0] == 1.0 and tuv[8, 0] == 0.0
+
+
 def cutout_scene(front_opaque=False, inst_flags=0, n=4):
     """A right triangle v0 (0,0,0), v1 (1,0,0), v2 (0,1,0) at z = 0, so that a ray down -z through (x, y) meets it
     at barycentrics (u, v) = (x, y) exactly, plus an opaque backstop triangle at z = -1 (geometry 1).  Hit records:
