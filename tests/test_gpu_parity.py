@@ -768,3 +768,31 @@ def test_scene_far_from_the_origin(ctx):
     assert rep["robust_hits"] > 2000
     assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0
     assert_ties_ok(rep)
+
+
+def test_inactive_instances_keep_instance_ids(ctx):
+    """Instances with a null BLAS reference (Vulkan's inactive instances) or a singular transform are never hit
+    and keep their place in the instance numbering: the other instances report the oracle's instance IDs, the
+    oracle scene holding mask-0 instances (never hit) in the same places."""
+    import pyrtx
+    _say("inactive instances")
+    sc = small_mixed_scene(seed=12, instances=16, freq=14, cells=16)
+    blases, _ = ctx.upload_scene(sc)
+    rec = pyrtx.instance_records(sc, [b.address for b in blases])
+    rec["addr"][3] = 0  # null reference
+    rec["xf"][9] = 0.0  # singular transform
+    tlas = ctx.build_tlas(rec)
+    masked = S.Scene(**{**sc.__dict__, "mask": sc.mask.copy()})
+    masked.mask[[3, 9]] = 0
+    t = oracle_scene(masked)
+    rays = np.concatenate([S.camera_rays(sc.camera, 128, 96), S.incoherent_rays(sc, 20000, seed=12)])
+    tuv, ids, st = O.trace(t, rays)
+    g = _trace_closest(ctx, tlas, rays)
+    rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
+    _say(f"  {rep}")
+    assert rep["robust_hits"] > 1000
+    assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0
+    assert_ties_ok(rep)
+    hit = g[5] != 0xFFFFFFFF
+    assert not np.isin(g[5][hit], [3, 9]).any()
+    assert len(set(int(i) for i in g[5][hit])) >= 8
