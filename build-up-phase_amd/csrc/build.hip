@@ -586,14 +586,18 @@ __device__ __forceinline__ float coherent_area(const float* nboxes, uint64_t i) 
     return box_area(b);
 }
 
-// Whole-wave treelet restructuring below internal node R (uniform arguments).  Returns R's cost and height
-// afterwards through new_cost / new_h.
-__device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old_cost, uint32_t old_h,
-                                 int32_t* left, int32_t* right, int32_t* parent, float* nboxes, float* cost,
-                                 uint32_t* cnt, uint32_t* hgt, float& new_cost, uint32_t& new_h) {
-    const uint32_t lane = threadIdx.x;
-    __syncthreads();
-    // formation (every lane computes the same thing; the loads are wave-uniform)
+// Treelet formation for root R, by one lane: open the largest-area internal treelet leaf until there are 7
+// leaves.  Writes the leaves and the 6 internal nodes (R first, in opening order) to the lane's staging row;
+// false when fewer than 7 primitives lie below R (nothing to restructure).  The lanes of a wave form their
+// treelets at the same time: their roots were completed in the same step of the climb, so none is an ancestor
+// of another and their subtrees - everything a formation reads and a restructuring writes - are disjoint.
+struct TreeletForm {
+    int32_t leaf[kTreeletLeaves];
+    int32_t pool[kTreeletLeaves - 1];
+};
+
+__device__ bool treelet_form(int32_t R, const int32_t* left, const int32_t* right, const float* nboxes,
+                             TreeletForm& out) {
     int32_t lf[kTreeletLeaves];
     float la[kTreeletLeaves];
     int32_t pool[kTreeletLeaves - 1];
@@ -612,11 +616,7 @@ __device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old
                 ba = la[j];
                 best = j;
             }
-        if (best < 0) {  // fewer than 7 primitives below R: nothing to restructure
-            new_cost = old_cost;
-            new_h = old_h;
-            return;
-        }
+        if (best < 0) return false;
         int32_t c = 0;
 #pragma unroll
         for (int j = 0; j < k; ++j)
@@ -634,11 +634,31 @@ __device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old
         lf[k] = r;
         la[k] = ar;
     }
-    if (lane == 0) {
 #pragma unroll
-        for (int j = 0; j < kTreeletLeaves; ++j) T.leaf[j] = lf[j];
+    for (int j = 0; j < kTreeletLeaves; ++j) out.leaf[j] = lf[j];
 #pragma unroll
-        for (int j = 0; j < kTreeletLeaves - 1; ++j) T.pool[j] = pool[j];
+    for (int j = 0; j < kTreeletLeaves - 1; ++j) out.pool[j] = pool[j];
+    return true;
+}
+
+// Whole-wave treelet restructuring below internal node R (uniform arguments) from its formation, which lane
+// `owner` holds in `f` (formed = true).  Returns R's cost and height afterwards through new_cost / new_h.
+__device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old_cost, uint32_t old_h,
+                                 uint32_t owner, bool formed, const TreeletForm& f, int32_t* left, int32_t* right,
+                                 int32_t* parent, float* nboxes, float* cost, uint32_t* cnt, uint32_t* hgt,
+                                 float& new_cost, uint32_t& new_h) {
+    const uint32_t lane = threadIdx.x;
+    __syncthreads();
+    if (!formed) {  // fewer than 7 primitives below R: nothing to restructure
+        new_cost = old_cost;
+        new_h = old_h;
+        return;
+    }
+    if (lane == owner) {
+#pragma unroll
+        for (int j = 0; j < kTreeletLeaves; ++j) T.leaf[j] = f.leaf[j];
+#pragma unroll
+        for (int j = 0; j < kTreeletLeaves - 1; ++j) T.pool[j] = f.pool[j];
     }
     __syncthreads();
     if (lane < (uint32_t)kTreeletLeaves) {
@@ -853,7 +873,13 @@ __global__ __launch_bounds__(64) void refit_refine_kernel(uint32_t n, int32_t* l
             }
         }
         // batched builds never restructure across BLASes
-        uint64_t need = __ballot(own && ccnt >= min_prims && (!node_blas || node_blas[node] != kMixedBlas));
+        const bool root = own && ccnt >= min_prims && (!node_blas || node_blas[node] != kMixedBlas);
+        uint64_t need = __ballot(root);
+        // every lane with a treelet root forms its treelet at once (the dependent child / box loads overlap), then
+        // the wave restructures them one after another
+        TreeletForm form;
+        const bool formed = root && treelet_form(node, left, right, nboxes, form);
+        const uint64_t formed_mask = __ballot(formed);
         while (need) {
             const int L = __ffsll((unsigned long long)need) - 1;
             need &= need - 1;
@@ -862,7 +888,8 @@ __global__ __launch_bounds__(64) void refit_refine_kernel(uint32_t n, int32_t* l
             const uint32_t rh = (uint32_t)__shfl((int)chgt, L);
             float nc;
             uint32_t nh;
-            treelet_optimize(T, n, R, rc, rh, left, right, parent, nboxes, cost, cnt, hgt, nc, nh);
+            treelet_optimize(T, n, R, rc, rh, (uint32_t)L, ((formed_mask >> L) & 1ull) != 0ull, form, left, right,
+                             parent, nboxes, cost, cnt, hgt, nc, nh);
             if ((int)lane == L) {
                 ccost = nc;
                 chgt = nh;
