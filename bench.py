@@ -436,6 +436,8 @@ def nrc_kernel_bench(ctx, torch, pyrtx):
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+NRC_SEEDS = (1, 2, 3, 4, 5)  # cache initialisations of the equal-relMSE comparison (median reported)
+NRC_WARMUP = 2               # first frames of an NRC run that are timed but not accumulated
 NRC_FLOP_PER_QUERY = 47104    # forward: 2 x sum(fan_in x fan_out) over the 6 layers
 NRC_FLOP_PER_SAMPLE = 141312  # forward + data gradient + weight gradient
 
@@ -649,33 +651,48 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
         nrc_cfg = best[1]
         log(f"autotuned NRC frame: {nrc_cfg} ({best[0]:.2f} ms per frame)")
     ctx.set_config(**nrc_cfg)
-    nrc = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=16384, cache_vertex=1)
+    # Equal-relMSE time over several cache initialisations: the first NRC_WARMUP frames (untrained cache) are
+    # rendered, trained on and timed but not accumulated, then frames are averaged with equal weights.  Averaging
+    # the untrained frames too leaves a bias that depends on the initialisation (2.1x - 4.1x over 8 seeds, median
+    # 2.5x, against 3.4x - 3.8x with two warm-up frames).
     accn = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
-    errs = torch.empty(args.nrc_frames, dtype=torch.float32, device=dev)
-    evs = [(ev(), ev()) for _ in range(args.nrc_frames)]
-    torch.cuda.synchronize()
-    tw = time.perf_counter()
-    for fr in range(args.nrc_frames):
-        evs[fr][0].record()
-        ctx.render_nrc(tlas, rec, sc.camera, lt, nrc, W, H, spp=1, train_paths=65536, sample_offset=fr,
-                       accum64=accn)
-        evs[fr][1].record()
-        errs[fr] = relmse((accn / (fr + 1)).float(), ref)
-    torch.cuda.synchronize()
-    wall_nrc = time.perf_counter() - tw
-    fms = np.array([x.elapsed_time(y) for x, y in evs])
-    e = errs.cpu().numpy()
-    cum = np.cumsum(fms)
-    hit = np.nonzero(e <= err_plain)[0]
-    if len(hit):
-        f = int(hit[0])
-        ms_nrc = float(cum[f])
-    else:
-        f, ms_nrc = None, None
-    steps, loss = nrc.status()
-    nrc.close()
-    log(f"NRC: frame {fms.mean():.2f} ms, relMSE after {args.nrc_frames} frames {e[-1]:.5f}; reaches plain relMSE "
-        f"at frame {None if f is None else f + 1} ({ms_nrc} ms)")
+    film_nrc = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    per_seed = []
+    for si, seed in enumerate(NRC_SEEDS):
+        frames = args.nrc_frames if si == 0 else min(args.nrc_frames, 64)
+        nrc = pyrtx.Nrc(ctx, lo, hi, seed=seed, batch_size=16384, cache_vertex=1)
+        accn.zero_()
+        errs = torch.full((frames,), float("inf"), dtype=torch.float32, device=dev)
+        evs = [(ev(), ev()) for _ in range(frames)]
+        torch.cuda.synchronize()
+        tw = time.perf_counter()
+        for fr in range(frames):
+            acc_fr = accn if fr >= NRC_WARMUP else None
+            evs[fr][0].record()
+            ctx.render_nrc(tlas, rec, sc.camera, lt, nrc, W, H, spp=1, train_paths=65536, sample_offset=fr,
+                           film=film_nrc, accum64=acc_fr)
+            evs[fr][1].record()
+            if acc_fr is not None:
+                errs[fr] = relmse((accn / (fr + 1 - NRC_WARMUP)).float(), ref)
+        torch.cuda.synchronize()
+        wall_nrc = time.perf_counter() - tw
+        fms = np.array([x.elapsed_time(y) for x, y in evs])
+        e = errs.cpu().numpy()
+        cum = np.cumsum(fms)
+        hit = np.nonzero(e <= err_plain)[0]
+        f = int(hit[0]) if len(hit) else None
+        ms_nrc = float(cum[f]) if f is not None else None
+        steps, loss = nrc.status()
+        nrc.close()
+        per_seed.append({"seed": seed, "frames": frames, "frame_ms": round(float(fms.mean()), 3),
+                         "frames_to_plain_relmse": None if f is None else f + 1,
+                         "ms_to_plain_relmse": None if ms_nrc is None else round(ms_nrc, 2),
+                         "speedup": None if ms_nrc is None else round(ms_plain / ms_nrc, 2),
+                         "relmse_final": round(float(e[-1]), 6), "adam_steps": int(steps), "wall_s": round(wall_nrc, 2)})
+        log(f"NRC seed {seed}: frame {fms.mean():.2f} ms, relMSE after {frames} frames {e[-1]:.5f}; reaches plain "
+            f"relMSE at frame {None if f is None else f + 1} ({ms_nrc} ms)")
+    sp = np.array([r["speedup"] or 0.0 for r in per_seed])
+    first = per_seed[0]
     return {
         "workload": f"closed Cornell box + displaced level-8 icosphere ({sc.unique_triangle_count} triangles), "
                     f"{W}x{H}, depth 8, RR after 3, seed 1234",
@@ -683,12 +700,13 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
                   "mrays_per_s": round(n_rays / ms_plain / 1e3, 1), "relmse": round(err_plain, 6)},
         "reference_spp": 64 * chunks,
         "nrc": {"frames": args.nrc_frames, "train_paths_per_frame": 65536, "batch_size": 16384,
-                "cache_vertex": 1, "frame_ms": round(float(fms.mean()), 3), "autotuned_trace_config": nrc_cfg,
-                "relmse_final": round(float(e[-1]), 6), "adam_steps": int(steps),
-                "frames_to_plain_relmse": None if f is None else f + 1,
-                "ms_to_plain_relmse": None if ms_nrc is None else round(ms_nrc, 2),
-                "wall_s": round(wall_nrc, 2)},
-        "equal_relmse_speedup": None if ms_nrc is None else round(ms_plain / ms_nrc, 2),
+                "cache_vertex": 1, "frame_ms": first["frame_ms"], "autotuned_trace_config": nrc_cfg,
+                "warmup_frames_not_accumulated": NRC_WARMUP,
+                "relmse_final": first["relmse_final"], "adam_steps": first["adam_steps"],
+                "frames_to_plain_relmse": first["frames_to_plain_relmse"],
+                "ms_to_plain_relmse": first["ms_to_plain_relmse"], "per_seed": per_seed},
+        "equal_relmse_speedup": round(float(np.median(sp)), 2),
+        "equal_relmse_speedup_range": [round(float(sp.min()), 2), round(float(sp.max()), 2)],
         "autotuned_trace_config": tuned,
         "cpu_baseline": cpu_pt,
     }
