@@ -796,3 +796,31 @@ def test_inactive_instances_keep_instance_ids(ctx):
     hit = g[5] != 0xFFFFFFFF
     assert not np.isin(g[5][hit], [3, 9]).any()
     assert len(set(int(i) for i in g[5][hit])) >= 8
+
+
+def test_morton63_builds_trace_exactly(ctx):
+    """Structures built with 63-bit Morton keys trace like the oracle, their reported SAH cost is within a few
+    percent of a 30-bit build's (the same LBVH method on a finer grid), and a batched build, which keys BLAS id and
+    a 30-bit code into one word, rejects the flag."""
+    import pyrtx
+    _say("63-bit Morton builds")
+    sc = small_mixed_scene(seed=13, instances=14, freq=20, cells=20)
+    t = oracle_scene(sc)
+    rays = np.concatenate([S.camera_rays(sc.camera, 96, 72), S.incoherent_rays(sc, 15000, seed=13)])
+    tuv, ids, st = O.trace(t, rays)
+    cost = {}
+    for name, flags in (("30", pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT),
+                        ("63", pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT | pyrtx.RTX_BUILD_MORTON63_BIT)):
+        blases, tlas = ctx.upload_scene(sc, flags=flags)
+        ctx.sync()
+        cost[name] = [ctx.as_info(b).sah_cost for b in blases]
+        g = _trace_closest(ctx, tlas, rays)
+        rep = compare_closest(g, tuv, ids, st, tlas=t, rays=rays)
+        assert rep["id_mismatch"] == 0 and rep["miss_mismatch"] == 0 and rep["t_bad"] == 0, name
+        assert_ties_ok(rep)
+    _say(f"  SAH cost per BLAS: 30-bit {cost['30']}, 63-bit {cost['63']}")
+    for a, b in zip(cost["30"], cost["63"]):
+        assert abs(a - b) <= 0.05 * a
+    with pytest.raises(pyrtx.RtxError) as e:
+        ctx.upload_scene(sc, flags=pyrtx.RTX_BUILD_MORTON63_BIT, batched=True)
+    assert e.value.code == -1000  # RTX_ERROR_INVALID_ARGUMENT
