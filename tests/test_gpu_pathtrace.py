@@ -177,3 +177,85 @@ def test_whole_path_launches_match_wavefront(ctx, monkeypatch):
     assert np.array_equal(whole[1], wave[1])
     assert np.array_equal(whole[2], wave[2])
     assert whole[3] == wave[3] and whole[4] == wave[4]
+
+
+def _export_obj(sc, obj_path, mat_path):
+    """The scene's geometries (identity instances, records in geometry order) as one OBJ group each, and its
+    camera and materials in the host program's text format."""
+    verts, lines, base = [], [], 0
+    for geoms in sc.blas:
+        for g in geoms:
+            verts.append(g.vertices.astype(np.float32))
+            lines.append(f"g part{len(lines)}\n" + "".join(
+                f"f {a + base + 1} {b + base + 1} {c + base + 1}\n" for a, b, c in g.indices.tolist()))
+            base += g.vertices.shape[0]
+    with open(obj_path, "w") as f:
+        for v in np.concatenate(verts):
+            f.write("v %.9g %.9g %.9g\n" % tuple(float(x) for x in v))
+        f.write("".join(lines))
+    with open(mat_path, "w") as f:
+        f.write("camera %.9g %.9g %.9g %.9g\n" % tuple(float(x) for x in sc.camera))
+        for r in sc.records:
+            f.write(" ".join("%.9g" % float(x) for x in list(r["color"]) + list(r["emission"])) + "\n")
+
+
+def test_cpp_path_tracer_host_program(ctx, tmp_path):
+    """host/path_tracer.cpp, the path tracer and the radiance-cache renderer driven from C++ through the C ABI, on
+    the configuration 2 scene exported as OBJ: its plain and NRC films equal, bit for bit, what the same calls give
+    through the Python binding, and its plain film matches the FP64 oracle's estimator."""
+    import os
+    import subprocess
+    import pyrtx
+    from conftest import PKG
+    exe = os.path.join(PKG, "host", "path_tracer")
+    assert os.path.exists(exe), "build the sample first (make -C build-up-phase_amd)"
+    _say("C++ path tracer host program")
+    W, H, spp, frames = 96, 54, 8, 6
+    sc, _ = S.cornell_scene(5, width=W, height=H)
+    obj, mat = tmp_path / "scene.obj", tmp_path / "materials.txt"
+    _export_obj(sc, obj, mat)
+    r = subprocess.run([exe, str(obj), str(mat), str(tmp_path / "out"), str(W), str(H), str(spp), str(frames)],
+                       check=True, timeout=300, capture_output=True, text=True)
+    _say("  " + r.stdout.strip().replace("\n", "; "))
+    plain_c = np.fromfile(tmp_path / "out_plain.f32", np.float32).reshape(H, W, 4)
+    nrc_c = np.fromfile(tmp_path / "out_nrc.f32", np.float32).reshape(H, W, 4)
+    # the same calls through the Python binding
+    v, idx, first = pyrtx.load_obj(path=str(obj))
+    blas = ctx.build_blas_obj(v, idx, first)
+    inst = np.zeros(1, pyrtx.INSTANCE_DTYPE)
+    inst["xf"][0] = S.translation().reshape(12)
+    inst["cm"][0] = 0xFF << 24
+    inst["addr"][0] = blas.address
+    tlas = ctx.build_tlas(inst)
+    lights = []
+    for g in range(len(first) - 1):
+        e = sc.records[g]["emission"]
+        if (e > 0).any():
+            for t in range(first[g], first[g + 1]):
+                lights.append(np.concatenate([v[idx[t]].reshape(9), e]).astype(np.float32))
+    lights = np.array(lights, np.float32)
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda")
+    ctx.render(tlas, sc.records, sc.camera, lights, W, H, spp=spp, accum64=acc)
+    plain_p = (acc / spp).float().cpu().numpy()
+    nrc = pyrtx.Nrc(ctx, v.min(0), v.max(0), seed=1, batch_size=16384, cache_vertex=1)
+    try:
+        acc.zero_()
+        for fr in range(frames):
+            ctx.render_nrc(tlas, sc.records, sc.camera, lights, nrc, W, H, spp=1, train_paths=65536,
+                           sample_offset=fr, accum64=acc if fr >= 2 else None)
+        ctx.sync()
+        nrc_p = (acc / (frames - 2)).float().cpu().numpy()
+    finally:
+        nrc.close()
+    assert np.array_equal(plain_c[..., :3], plain_p) and np.all(plain_c[..., 3] == 1.0)
+    assert np.array_equal(nrc_c[..., :3], nrc_p)
+    # and the plain film against the FP64 oracle's estimator on the same OBJ geometry
+    one = S.Scene(blas=[[S.Geometry(v, idx[first[g]:first[g + 1]]) for g in range(len(first) - 1)]],
+                  transforms=S.translation()[None].astype(np.float32), blas_of_instance=np.array([0]),
+                  custom_index=np.zeros(1, np.uint32), mask=np.full(1, 0xFF, np.uint32),
+                  sbt_offset=np.zeros(1, np.uint32), flags=np.zeros(1, np.uint32), records=sc.records,
+                  camera=sc.camera, width=W, height=H)
+    t = oracle_scene(one)
+    o = O.render(t, sc.records, sc.camera, lights, W, H, spp) / spp
+    # measured 4.0e-7: one pixel of 5,184 follows a different path (FP32 vs FP64 near an edge)
+    _check_vs_oracle(plain_c[..., :3].astype(np.float64), o, 4e-6, 0.999)
