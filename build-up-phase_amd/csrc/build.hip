@@ -550,14 +550,14 @@ __device__ __forceinline__ void store_coherent_i32(int32_t* p, int32_t v) { stor
 constexpr int kTreeletLeaves = 7;
 constexpr int kTreeletSubsets = 127;
 
-// Subsets of the 7 treelet leaves ordered by size, then value; kSizeOffset[k] = first subset of size k.
+// Subsets of the 7 treelet leaves ordered by size, then value (size k starts at entry 0, 7, 28, 63, 98, 119, 126
+// for k = 1..7).
 __constant__ uint8_t kSubsetsBySize[127] = {
     1, 2, 4, 8, 16, 32, 64, 3, 5, 6, 9, 10, 12, 17, 18, 20, 24, 33, 34, 36, 40, 48, 65, 66, 68, 72, 80, 96, 7, 11,
     13, 14, 19, 21, 22, 25, 26, 28, 35, 37, 38, 41, 42, 44, 49, 50, 52, 56, 67, 69, 70, 73, 74, 76, 81, 82, 84, 88,
     97, 98, 100, 104, 112, 15, 23, 27, 29, 30, 39, 43, 45, 46, 51, 53, 54, 57, 58, 60, 71, 75, 77, 78, 83, 85, 86,
     89, 90, 92, 99, 101, 102, 105, 106, 108, 113, 114, 116, 120, 31, 47, 55, 59, 61, 62, 79, 87, 91, 93, 94, 103,
     107, 109, 110, 115, 117, 118, 121, 122, 124, 63, 95, 111, 119, 123, 125, 126, 127};
-__constant__ uint8_t kSizeOffset[9] = {0, 0, 7, 28, 63, 98, 119, 126, 127};
 
 struct TreeletLds {
     float area[128];
@@ -641,6 +641,61 @@ __device__ bool treelet_form(int32_t R, const int32_t* left, const int32_t* righ
     return true;
 }
 
+// DP step for the treelet-leaf subsets of size K.  A subset S's splits {P, S\P} with P containing S's lowest
+// leaf are P = low | Q for the proper subsets Q of S\low (2^(K-1) - 1 of them).  G consecutive lanes (G a power
+// of two, G x C(7, K) <= 64) share a subset, lane r taking the Q of index r, r + G, ... (the index's bits placed
+// on S\low's leaves); the group then takes the minimum (cost, P), with (inf, 0) while no finite split was seen -
+// the same result as one lane scanning every split, whatever the order.
+template <int K, int G>
+__device__ __forceinline__ void treelet_dp_size(TreeletLds& T, uint32_t lane) {
+    constexpr uint32_t kOffset[kTreeletLeaves] = {0, 0, 7, 28, 63, 98, 119};
+    constexpr uint32_t kCount[kTreeletLeaves] = {0, 7, 21, 35, 35, 21, 7};
+    constexpr uint32_t kSplits = (1u << (K - 1)) - 1u;
+    constexpr uint32_t kPerLane = (kSplits + G - 1) / G;
+    static_assert((G & (G - 1)) == 0 && G * kCount[K] <= 64u, "one wave per subset size");
+    const uint32_t sub = lane / G, r = lane % G;
+    uint32_t S = 0;
+    float best = INFINITY;
+    uint32_t bp = 0;
+    if (sub < kCount[K]) {
+        S = kSubsetsBySize[kOffset[K] + sub];
+        const uint32_t low = S & (0u - S), rest = S ^ low;
+#pragma unroll
+        for (uint32_t t = 0; t < kPerLane; ++t) {
+            const uint32_t i = r + t * G;
+            if (i < kSplits) {
+                uint32_t q = 0, m = rest;
+#pragma unroll
+                for (int b = 0; b < K - 1; ++b) {
+                    const uint32_t bit = m & (0u - m);
+                    if (i & (1u << b)) q |= bit;
+                    m ^= bit;
+                }
+                const uint32_t P = low | q;
+                const float c = T.cost[P] + T.cost[S ^ P];
+                if (c < best || (c == best && P < bp)) {
+                    best = c;
+                    bp = P;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        const float oc = __shfl_xor(best, off);
+        const uint32_t op = (uint32_t)__shfl_xor((int)bp, off);
+        if (oc < best || (oc == best && op < bp)) {
+            best = oc;
+            bp = op;
+        }
+    }
+    if (sub < kCount[K] && r == 0) {
+        T.cost[S] = T.area[S] + best;
+        T.part[S] = bp;
+    }
+    __syncthreads();
+}
+
 // Whole-wave treelet restructuring below internal node R (uniform arguments) from its formation, which lane
 // `owner` holds in `f` (formed = true).  Returns R's cost and height afterwards through new_cost / new_h.
 __device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old_cost, uint32_t old_h,
@@ -690,27 +745,12 @@ __device__ void treelet_optimize(TreeletLds& T, uint32_t n, int32_t R, float old
         }
     }
     __syncthreads();
-    // subsets of size 2..6: one lane per subset
-    for (int k = 2; k < kTreeletLeaves; ++k) {
-        const uint32_t o = kSizeOffset[k], m = kSizeOffset[k + 1] - o;
-        if (lane < m) {
-            const uint32_t S = kSubsetsBySize[o + lane];
-            const uint32_t low = S & (0u - S);
-            float best = INFINITY;
-            uint32_t bp = 0;
-            for (uint32_t P = (S - 1) & S; P; P = (P - 1) & S) {
-                if (!(P & low)) continue;
-                const float c = T.cost[P] + T.cost[S ^ P];
-                if (c < best || (c == best && P < bp)) {
-                    best = c;
-                    bp = P;
-                }
-            }
-            T.cost[S] = T.area[S] + best;
-            T.part[S] = bp;
-        }
-        __syncthreads();
-    }
+    // subsets of size 2..6, each size after the smaller ones
+    treelet_dp_size<2, 1>(T, lane);
+    treelet_dp_size<3, 1>(T, lane);
+    treelet_dp_size<4, 1>(T, lane);
+    treelet_dp_size<5, 2>(T, lane);
+    treelet_dp_size<6, 8>(T, lane);
     // all 7 leaves: the 63 splits containing leaf 0, one per lane, then a (cost, split) minimum over the wave
     float c7 = INFINITY;
     uint32_t p7 = 0xFFFFFFFFu;
