@@ -56,6 +56,10 @@ struct rtx_context_t {
     rtx::DeviceState* side_state = nullptr;
     uint2* side_gstack = nullptr;
     uint64_t side_gstack_entries = 0;
+    // set once a call has been recorded by a capture of the context stream: the graph may hold the context's
+    // buffers, so buffers replaced by larger ones are kept (retired) until the context is destroyed
+    bool recorded = false;
+    std::vector<void*> retired;
 };
 
 struct rtx_nrc_t {
@@ -172,6 +176,8 @@ struct PropSlot {
     uint64_t address = 0;
     uint32_t* host = nullptr;   // kPropWords pinned words
     hipEvent_t ready = nullptr;
+    hipStream_t stream = nullptr;  // the stream the read-back was enqueued on
+    bool done = false;             // the read-back is known to have completed (host words valid)
     int device = -1;
     uint64_t used = 0;
 };
@@ -191,6 +197,32 @@ constexpr TlasProps kUnknownProps = {rtx::kContentNonOpaque | rtx::kContentAabb,
 bool stream_capturing(hipStream_t s) {
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
     return hipStreamIsCapturing(s, &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// Grow a context-owned device buffer to `need` bytes (no-op when it is large enough).  A call being recorded into
+// a graph cannot allocate (allocations are not stream work): the buffer must have been sized by an earlier direct
+// call of the same shape.  A replaced buffer is freed, or retired when a recorded graph may still use it.
+int grow_buffer(rtx_context ctx, void** buf, uint64_t* have, uint64_t need, const char* what) {
+    if (need <= *have) return RTX_SUCCESS;
+    if (stream_capturing(ctx->stream))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY,
+                    "%s must grow to %llu bytes while the call is being recorded: make the same call once before "
+                    "recording it", what, (unsigned long long)need);
+    if (*buf) {
+        if (ctx->recorded)
+            ctx->retired.push_back(*buf);
+        else
+            (void)hipFree(*buf);
+    }
+    *buf = nullptr;
+    *have = 0;
+    hipError_t e = hipMalloc(buf, need);
+    if (e != hipSuccess) {
+        *buf = nullptr;
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "%s: %s", what, hipGetErrorString(e));
+    }
+    *have = need;
+    return RTX_SUCCESS;
 }
 
 // After enqueuing a build or copy of the structure at `address` on the context stream.
@@ -236,7 +268,16 @@ void note_as_written(rtx_context ctx, uint64_t address) {
         return;
     }
     slot->address = address;
+    slot->stream = ctx->stream;
+    slot->done = false;
     slot->used = ++g_prop_clock;
+}
+
+// After the context stream has drained: its read-backs are complete.
+void note_stream_idle(rtx_context ctx) {
+    std::lock_guard<std::mutex> lock(g_prop_mu);
+    for (PropSlot& p : g_prop)
+        if (p.address && p.device == ctx->device && p.stream == ctx->stream) p.done = true;
 }
 
 // Properties of the TLAS at `address` for a launch on the context stream.
@@ -246,12 +287,13 @@ TlasProps tlas_props(rtx_context ctx, uint64_t address) {
         std::lock_guard<std::mutex> lock(g_prop_mu);
         for (PropSlot& p : g_prop) {
             if (p.address != address) continue;
-            hipError_t e = hipEventQuery(p.ready);
-            if (e == hipErrorNotReady) {
+            if (!p.done) {
+                // inside a capture, querying the event would invalidate the capture: the properties are known only
+                // if the read-back was seen complete before (rtx_sync or an earlier launch)
                 if (capturing) return kUnknownProps;
-                e = hipEventSynchronize(p.ready);
+                if (hipEventSynchronize(p.ready) != hipSuccess) break;
+                p.done = true;
             }
-            if (e != hipSuccess) break;
             p.used = ++g_prop_clock;
             return TlasProps{p.host[3], p.host[1]};
         }
@@ -273,17 +315,24 @@ uint32_t stack_entries(rtx_context ctx, uint32_t depth) {
     return depth ? 2u * depth + 4u : 76u;
 }
 
+// Spill entries per lane that a launch with this LDS depth needs when the TLAS depth is unknown (a call recorded
+// into a graph cannot read it).  Spill areas are sized for at least that, so that recording a call never has to
+// grow them.
+uint32_t unknown_depth_gdepth(rtx_context ctx, const rtx::TraceLaunch& L) {
+    const uint32_t total = stack_entries(ctx, 0);
+    return total - std::min(L.lds_depth, total);
+}
+
+// Every traversal and render call passes here, so this is also where a recording of the context stream is noted.
 int ensure_gstack(rtx_context ctx, const rtx::TraceLaunch& L) {
+    if (stream_capturing(ctx->stream)) ctx->recorded = true;
     ctx->last_stack = L.lds_depth + L.gdepth;
-    uint64_t need = (uint64_t)L.grid * L.block_size * L.gdepth;
-    if (need <= ctx->gstack_entries) return RTX_SUCCESS;
-    if (ctx->gstack) (void)hipFree(ctx->gstack);
-    ctx->gstack = nullptr;
-    ctx->gstack_entries = 0;
-    hipError_t e = hipMalloc(&ctx->gstack, need * sizeof(uint2));
-    if (e != hipSuccess) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "stack spill area: %s", hipGetErrorString(e));
-    ctx->gstack_entries = need;
-    return RTX_SUCCESS;
+    uint64_t bytes = ctx->gstack_entries * sizeof(uint2);
+    const uint64_t per_lane = std::max(L.gdepth, unknown_depth_gdepth(ctx, L));
+    const int r = grow_buffer(ctx, (void**)&ctx->gstack, &bytes, (uint64_t)L.grid * L.block_size * per_lane *
+                                                                      sizeof(uint2), "stack spill area");
+    ctx->gstack_entries = bytes / sizeof(uint2);
+    return r;
 }
 
 // Side stream, events, ray counter and spill stack of the NRC training wavefront (created on first use), and its
@@ -291,6 +340,9 @@ int ensure_gstack(rtx_context ctx, const rtx::TraceLaunch& L) {
 // wavefront's on every SIMD.
 int side_chain(rtx_context ctx, const rtx::TraceLaunch& L, rtx::SideChain& side) {
     hipError_t e = hipSuccess;
+    if ((!ctx->side_stream || !ctx->side_fork || !ctx->side_join || !ctx->side_state) && stream_capturing(ctx->stream))
+        return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY,
+                    "the NRC side stream is created by the first NRC frame: make one before recording frames");
     if (!ctx->side_stream) e = hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking);
     if (e == hipSuccess && !ctx->side_fork) e = hipEventCreateWithFlags(&ctx->side_fork, hipEventDisableTiming);
     if (e == hipSuccess && !ctx->side_join) e = hipEventCreateWithFlags(&ctx->side_join, hipEventDisableTiming);
@@ -307,16 +359,12 @@ int side_chain(rtx_context ctx, const rtx::TraceLaunch& L, rtx::SideChain& side)
     S.blocks_per_cu = kSideBlocks * 256u / L.block_size > 0 ? kSideBlocks * 256u / L.block_size : 1u;
     S.grid = (uint32_t)ctx->cu_count * S.blocks_per_cu;
     S.waves = 4;
-    const uint64_t need = (uint64_t)S.grid * S.block_size * S.gdepth;
-    if (need > ctx->side_gstack_entries) {
-        if (ctx->side_gstack) (void)hipFree(ctx->side_gstack);
-        ctx->side_gstack = nullptr;
-        ctx->side_gstack_entries = 0;
-        e = hipMalloc(&ctx->side_gstack, need * sizeof(uint2));
-        if (e != hipSuccess)
-            return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc spill stack: %s", hipGetErrorString(e));
-        ctx->side_gstack_entries = need;
-    }
+    uint64_t bytes = ctx->side_gstack_entries * sizeof(uint2);
+    const uint64_t per_lane = std::max(S.gdepth, unknown_depth_gdepth(ctx, S));
+    const int r = grow_buffer(ctx, (void**)&ctx->side_gstack, &bytes,
+                              (uint64_t)S.grid * S.block_size * per_lane * sizeof(uint2), "nrc spill stack");
+    ctx->side_gstack_entries = bytes / sizeof(uint2);
+    if (r) return r;
     side.stream = ctx->side_stream;
     side.fork = ctx->side_fork;
     side.join = ctx->side_join;
@@ -398,6 +446,7 @@ int rtx_destroy_context(rtx_context ctx) {
     if (ctx->views_buf) (void)hipFree(ctx->views_buf);
     if (ctx->gstack) (void)hipFree(ctx->gstack);
     if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
+    for (void* p : ctx->retired) (void)hipFree(p);
     if (ctx->state) (void)hipFree(ctx->state);
     delete ctx;
     return RTX_SUCCESS;
@@ -411,6 +460,7 @@ int rtx_sync(rtx_context ctx) {
     if (!ctx) return RTX_ERROR_INVALID_ARGUMENT;
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_sync");
+    note_stream_idle(ctx);
     uint32_t flags = 0;
     e = hipMemcpy(&flags, &ctx->state->error_flags, 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_sync flags");
@@ -619,6 +669,8 @@ int rtx_cmd_update_blas(rtx_context ctx, const rtx_blas_geometry* geometries, ui
     if (r) return r;
     if (!blas_address || (blas_address & 255) || !scratch_device || !aligned(scratch_device, 256))
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "BLAS and scratch must be non-NULL and 256-B aligned");
+    if (stream_capturing(ctx->stream))  // the synchronisation below would invalidate the caller's capture
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "an update reads the structure's header and cannot be recorded");
     if ((r = rtx_sync(ctx))) return r;
     rtx::AsHeader h;
     hipError_t e = hipMemcpy(&h, (const void*)(uintptr_t)blas_address, sizeof h, hipMemcpyDeviceToHost);
@@ -859,16 +911,9 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
     if (sbt->hit_record_count > 0 && !sbt->hit_records_device)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "sbt hit records are NULL");
     const uint32_t npix = p.width * p.height;
+    if (stream_capturing(ctx->stream)) ctx->recorded = true;
     const uint64_t need = rtx::pt_scratch_bytes(npix + extra_paths, p.max_depth, p.light_triangle_count + extra_cdf);
-    if (need > ctx->pt_scratch_bytes) {
-        if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
-        ctx->pt_scratch = nullptr;
-        ctx->pt_scratch_bytes = 0;
-        hipError_t e = hipMalloc(&ctx->pt_scratch, need);
-        if (e != hipSuccess) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "path tracer queues: %s",
-                                         hipGetErrorString(e));
-        ctx->pt_scratch_bytes = need;
-    }
+    if ((r = grow_buffer(ctx, &ctx->pt_scratch, &ctx->pt_scratch_bytes, need, "path tracer queues"))) return r;
     a = rtx::PtArgs{};
     rtx::pt_layout(a, ctx->pt_scratch, npix + extra_paths, p.max_depth, p.light_triangle_count + extra_cdf);
     a.tlas = (const void*)(uintptr_t)tlas_address;
@@ -1064,15 +1109,11 @@ int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* 
     const uint64_t vbytes = view_count * sizeof(rtx::BatchView);
     const uint64_t vslot = (vbytes + 255) & ~255ull;
     if (2 * vslot > ctx->views_bytes) {
-        if (ctx->views_buf) (void)hipFree(ctx->views_buf);
-        ctx->views_buf = nullptr;
-        ctx->views_bytes = 0;
         ctx->graph_key = 0;
-        if (hipMalloc(&ctx->views_buf, 2 * vslot) != hipSuccess)
-            return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "batch views");
-        ctx->views_bytes = 2 * vslot;
+        if ((r = grow_buffer(ctx, &ctx->views_buf, &ctx->views_bytes, 2 * vslot, "batch views"))) return r;
     }
-    const bool graph = (params->flags & RTX_RENDER_GRAPH_BIT) != 0;
+    // a call that is itself being recorded (a capture of the context stream) is recorded as a direct call
+    const bool graph = (params->flags & RTX_RENDER_GRAPH_BIT) != 0 && !stream_capturing(ctx->stream);
     void* vdst = graph ? ctx->views_buf : (void*)((uint8_t*)ctx->views_buf + ctx->views_bytes / 2);
     a.views = (const rtx::BatchView*)vdst;
     a.nviews = view_count;
@@ -1096,7 +1137,7 @@ int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* 
     key = fnv1a(key, &ctx->views_buf, sizeof ctx->views_buf);
     hipError_t e;
     if (!graph) {
-        if ((e = hipMemcpyAsync(vdst, bv.data(), vbytes, hipMemcpyHostToDevice, ctx->stream)))
+        if ((e = rtx::upload_host_array(vdst, bv.data(), vbytes, ctx->stream)))
             return hip_fail(ctx, e, "rtx_cmd_render_batch views");
         e = rtx::pt_render_batch(a, L, params->spp, params->sample_offset, films_device, accum_rgb_device,
                                  ctx->stream);
@@ -1135,14 +1176,7 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     a.cache_depth = nrc->st.cache_vertex;
     const uint32_t rec_depths = rtx::nrc_record_depths(params->max_depth, a.cache_depth, nrc->st.train_suffix);
     const uint64_t need = rtx::pt_train_bytes(train_paths, rec_depths);
-    if (need > nrc->train_buf_bytes) {
-        if (nrc->train_buf) (void)hipFree(nrc->train_buf);
-        nrc->train_buf = nullptr;
-        nrc->train_buf_bytes = 0;
-        hipError_t e = hipMalloc(&nrc->train_buf, need);
-        if (e) return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "nrc training buffers: %s", hipGetErrorString(e));
-        nrc->train_buf_bytes = need;
-    }
+    if ((r = grow_buffer(ctx, &nrc->train_buf, &nrc->train_buf_bytes, need, "nrc training buffers"))) return r;
     rtx::SideChain side{};
     if ((r = side_chain(ctx, L, side))) return r;
     // the pixel paths leave kSideReserve 256-thread blocks per CU to the training paths
@@ -1151,7 +1185,7 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
         L.blocks_per_cu -= reserve;
         L.grid = (uint32_t)ctx->cu_count * L.blocks_per_cu;
     }
-    if (!(params->flags & RTX_RENDER_GRAPH_BIT)) {
+    if (!(params->flags & RTX_RENDER_GRAPH_BIT) || stream_capturing(ctx->stream)) {
         hipError_t e = rtx::pt_render_nrc(a, L, side, nrc->st, train_paths, nrc->st.train_suffix, nrc->train_buf,
                                           params->spp, params->sample_offset, film_rgba32f_device, accum_rgb_device,
                                           ctx->stream);

@@ -7,6 +7,13 @@
 
 namespace rtx {
 
+// Host -> device copy of a small host array (build descriptors, batch views) enqueued on `stream`.  Outside a
+// stream capture it is hipMemcpyAsync.  While the stream is being captured into a graph, the bytes travel as the
+// arguments of copy launches, which the graph keeps, so a replay does not read host memory that the recording
+// call has since released; up to kArgUploadMax bytes (multiples of 4 B), hipErrorNotSupported above.
+constexpr uint64_t kArgUploadMax = 64 * 1024;
+hipError_t upload_host_array(void* dst_device, const void* src_host, uint64_t bytes, hipStream_t stream);
+
 uint64_t blas_as_bytes(uint32_t prim_count, bool allow_update = false);
 uint64_t tlas_as_bytes(uint32_t instance_count);
 uint64_t blas_scratch_bytes(uint32_t prim_count, bool morton63, uint32_t geometry_count);

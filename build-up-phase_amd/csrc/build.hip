@@ -1482,6 +1482,15 @@ __global__ void fill_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
+// One piece of a host array carried by value as a launch argument (a recorded launch keeps its arguments).
+constexpr uint32_t kArgBlobWords = 256;
+struct ArgBlob {
+    uint32_t w[kArgBlobWords];
+};
+__global__ void arg_blob_kernel(ArgBlob b, uint32_t* __restrict__ dst, uint32_t words) {
+    if (threadIdx.x < words) dst[threadIdx.x] = b.w[threadIdx.x];
+}
+
 // ---- scratch layout --------------------------------------------------------------------------------------------
 struct Layout {
     uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, cost, cnt, hgt, wcost, tasks0,
@@ -1713,6 +1722,24 @@ hipError_t debug_sort(void* keys, uint32_t* vals, uint32_t n, int key_bytes, uin
     return e ? e : e2;
 }
 
+hipError_t upload_host_array(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(s, &st);
+    if (e != hipSuccess) return e;
+    if (st == hipStreamCaptureStatusNone) return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    if (bytes > kArgUploadMax || (bytes & 3) || ((uintptr_t)dst & 3)) return hipErrorNotSupported;
+    const uint8_t* in = (const uint8_t*)src;
+    for (uint64_t at = 0; at < bytes; at += 4ull * kArgBlobWords) {
+        const uint32_t words = (uint32_t)std::min<uint64_t>(kArgBlobWords, (bytes - at) / 4);
+        ArgBlob b{};
+        std::memcpy(b.w, in + at, 4ull * words);
+        arg_blob_kernel<<<1, kArgBlobWords, 0, s>>>(b, (uint32_t*)((uint8_t*)dst + at), words);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 uint64_t blas_as_bytes(uint32_t n, bool update) { return as_bytes(n, false, update); }
 uint64_t tlas_as_bytes(uint32_t n) { return as_bytes(n, true); }
 uint64_t blas_scratch_bytes(uint32_t n, bool k64, uint32_t ngeom) { return make_layout(n, k64 ? 8 : 4, ngeom).total; }
@@ -1729,7 +1756,7 @@ hipError_t build_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, ui
     hipError_t err;
     GeomDesc* gd = (GeomDesc*)(sc + L.geoms);
     if (ngeom) {
-        if ((err = hipMemcpyAsync(gd, geoms_host, ngeom * sizeof(GeomDesc), hipMemcpyHostToDevice, s))) return err;
+        if ((err = upload_host_array(gd, geoms_host, ngeom * sizeof(GeomDesc), s))) return err;
     }
     uint32_t* cb = (uint32_t*)(sc + L.cbounds);
     fill_kernel<<<1, 64, 0, s>>>(cb, 3, 0xFFFFFFFFu);
@@ -1767,7 +1794,7 @@ hipError_t update_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, u
     uint8_t* sc = (uint8_t*)scratch;
     hipError_t err;
     GeomDesc* gd = (GeomDesc*)(sc + L.geoms);
-    if (ngeom && (err = hipMemcpyAsync(gd, geoms_host, ngeom * sizeof(GeomDesc), hipMemcpyHostToDevice, s)))
+    if (ngeom && (err = upload_host_array(gd, geoms_host, ngeom * sizeof(GeomDesc), s)))
         return err;
     uint32_t* cb = (uint32_t*)(sc + L.cbounds);
     fill_kernel<<<1, 64, 0, s>>>(cb, 3, 0xFFFFFFFFu);
@@ -1848,10 +1875,10 @@ hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t ngeom, const ui
     GeomDesc* gd = (GeomDesc*)(sc + L.geoms);
     AsHeader** as_ptr = (AsHeader**)(sc + L.asptr);
     uint2* binfo = (uint2*)(sc + L.binfo);
-    if (ngeom && (err = hipMemcpyAsync(gd, geoms_host, ngeom * sizeof(GeomDesc), hipMemcpyHostToDevice, s)))
+    if (ngeom && (err = upload_host_array(gd, geoms_host, ngeom * sizeof(GeomDesc), s)))
         return err;
-    if ((err = hipMemcpyAsync(as_ptr, dst_host, nbatch * sizeof(AsHeader*), hipMemcpyHostToDevice, s))) return err;
-    if ((err = hipMemcpyAsync(binfo, binfo_host, nbatch * sizeof(uint2), hipMemcpyHostToDevice, s))) return err;
+    if ((err = upload_host_array(as_ptr, dst_host, nbatch * sizeof(AsHeader*), s))) return err;
+    if ((err = upload_host_array(binfo, binfo_host, nbatch * sizeof(uint2), s))) return err;
     uint32_t* cb = (uint32_t*)(sc + L.bcb);
     uint32_t* pblas = (uint32_t*)(sc + L.pblas);
     uint32_t* nblas = (uint32_t*)(sc + L.nblas);

@@ -133,6 +133,8 @@ RTX_RAY_FLAG_SKIP_AABBS = 0x200
 RTX_FILM_SRGB_BIT, RTX_FILM_SWAP_RB_BIT, RTX_FILM_OPAQUE_BIT = 0x1, 0x2, 0x4
 RTX_FILM_REFERENCE_BYTES = RTX_FILM_SWAP_RB_BIT
 RTX_COPY_MODE_CLONE, RTX_COPY_MODE_COMPACT = 0, 1
+RTX_ERROR_OUT_OF_DEVICE_MEMORY = -2
+RTX_ERROR_INVALID_ARGUMENT = -1000
 RTX_ERROR_FORMAT = -1003
 
 
@@ -400,9 +402,10 @@ class Context:
                                                 ctypes.byref(addr)))
         return AccelerationStructure(storage, addr.value, int(indices.shape[0]), (scratch, dv, di))
 
-    def build_blas_batch(self, geom_lists, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT):
+    def build_blas_batch(self, geom_lists, flags=RTX_BUILD_PREFER_FAST_TRACE_BIT, into=None, scratch=None):
         """rtx_cmd_build_blas_batch: one build pass over several BLASes (list of geometry lists as for
-        build_blas).  Returns one AccelerationStructure per list."""
+        build_blas).  Returns one AccelerationStructure per list.  into: existing BLASes (one per list) whose
+        storage the build reuses (same addresses); scratch: a device uint8 tensor to build in."""
         torch = _torch()
         k = len(geom_lists)
         arrs = [self._geometry_structs(g) for g in geom_lists]
@@ -414,11 +417,17 @@ class Context:
         scratch_size = ctypes.c_uint64()
         self._check(self.lib.rtx_get_blas_batch_build_sizes(self.handle, ctypes.cast(infos, _P), k, flags,
                                                             ctypes.cast(as_sizes, _P), ctypes.byref(scratch_size)))
-        storages = [torch.empty(int(as_sizes[b]), dtype=torch.uint8, device=self.device) for b in range(k)]
+        if into is not None:
+            if len(into) != k or any(a.storage.numel() < as_sizes[b] for b, a in enumerate(into)):
+                raise RtxError(RTX_ERROR_INVALID_ARGUMENT, "into: one BLAS per list, each at least its size query")
+            storages = [a.storage for a in into]
+        else:
+            storages = [torch.empty(int(as_sizes[b]), dtype=torch.uint8, device=self.device) for b in range(k)]
         for b in range(k):
             infos[b].dst_device = storages[b].data_ptr()
             infos[b].dst_size = storages[b].numel()
-        scratch = torch.empty(int(scratch_size.value), dtype=torch.uint8, device=self.device)
+        if scratch is None or scratch.numel() < scratch_size.value:
+            scratch = torch.empty(int(scratch_size.value), dtype=torch.uint8, device=self.device)
         addrs = (ctypes.c_uint64 * k)()
         self._check(self.lib.rtx_cmd_build_blas_batch(self.handle, ctypes.cast(infos, _P), k, flags,
                                                       scratch.data_ptr(), scratch.numel(), ctypes.cast(addrs, _P)))

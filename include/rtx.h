@@ -21,6 +21,18 @@
  *  - Return values follow VkResult: 0 = success, negative = error.  No C++ exception crosses the ABI.
  *  - An rtx_context is externally synchronised (like a VkQueue).  All rtx_cmd_* calls are ordered on the
  *    context's HIP stream and return before the GPU work finishes; rtx_sync waits for it.
+ *  - Recording (the command-buffer analogue, main.cpp:789-812 and 1340-1356): rtx_cmd_build_blas,
+ *    rtx_cmd_build_blas_batch, rtx_cmd_build_tlas, rtx_cmd_copy_as, rtx_cmd_trace_rays, rtx_cmd_trace_ray_buffer(
+ *    _noalloc), rtx_cmd_render(_batch, _nrc) and rtx_cmd_film_to_rgba8 may be recorded by a HIP stream capture of
+ *    the context's stream (hipStreamBeginCapture ... hipStreamEndCapture, or torch.cuda.graph on that stream);
+ *    replaying the graph redoes the recorded work on the device data as it is at replay time.  Host arrays passed
+ *    to a recorded call (build descriptors, batch views: up to 64 KiB each) are copied into the graph.  A
+ *    recorded call cannot allocate: make each call once outside the capture first, so the context's internal
+ *    buffers have their size (a call that would grow one fails with RTX_ERROR_OUT_OF_DEVICE_MEMORY).  A TLAS
+ *    built before the capture is traced with the traversal its content allows; one built inside it is traced
+ *    with the general (any-hit / intersection capable) traversal and the wavefront integrator, which give the
+ *    same results.  RTX_RENDER_GRAPH_BIT is ignored inside a capture.  rtx_cmd_update_blas and the rtx_get_*
+ *    queries synchronise and cannot be recorded.
  *  - Only gfx950 code objects are built.
  */
 #ifndef RTX_H_
