@@ -803,6 +803,7 @@ def batch_bench(ctx, args, np, torch, pyrtx, S):
     ms_seq = a.elapsed_time(b)
     log(f"batch render {spp} spp: graph {ms_graph:.1f} ms ({rays / ms_graph / 1e3:.0f} Mrays/s), direct "
         f"{ms_direct:.1f} ms, one view at a time {ms_seq:.1f} ms")
+    recorded = recorded_frame_bench(ctx, lists, sc, rec, views, lt, spp, depth, width, tuned, np, torch, pyrtx)
     return {"workload": f"{n} scenes (seed 7): displaced icosphere + ground quad + area light each, triangles "
                         f"log-uniform in [1e3, 2e6], films from 256^2 / 512^2 / 1024x768; {spp} spp, depth {depth}",
             "scenes": n, "triangles": int(tris), "pixels": int(npix),
@@ -810,7 +811,62 @@ def batch_bench(ctx, args, np, torch, pyrtx, S):
             "render_ms": {"batched_graph": round(ms_graph, 1), "batched_direct": round(ms_direct, 1),
                           "one_view_at_a_time": round(ms_seq, 1)},
             "rays_per_frame": int(rays), "mrays_per_s": round(rays / ms_graph / 1e3, 1),
-            "autotuned_trace_config": tuned, "autotuned_bvh_width": width, "ms_per_1spp_by_width": width_ms or None}
+            "autotuned_trace_config": tuned, "autotuned_bvh_width": width, "ms_per_1spp_by_width": width_ms or None,
+            "recorded_build_and_render": recorded}
+
+
+def recorded_frame_bench(ctx, lists, sc, rec, views, lt, spp, depth, width, tuned, np, torch, pyrtx):
+    """The batch's whole frame - batched BLAS build, TLAS build, batched render - recorded by a capture of a
+    context's stream into one HIP graph (the command-buffer form of the reference's recorded build and trace) and
+    replayed; each replay rebuilds both levels from the device data.  A TLAS built inside the recording cannot be
+    read back on the host, so its render takes the general traversal and the wavefront integrator.  The replays
+    are checked bit-identical to the same calls made directly."""
+    s = torch.cuda.Stream()
+    c = pyrtx.Context(0, stream=s)
+    try:
+        if tuned:
+            c.set_config(**tuned)
+        flags = pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT | (pyrtx.RTX_BUILD_MAX_WIDTH(width) if width < 8 else 0)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(s):
+            blases = c.build_blas_batch(lists, flags)
+            inst = torch.from_numpy(pyrtx.instance_records(sc, [x.address for x in blases]).view(np.uint8)).to(
+                c.device)
+            tlas = c.build_tlas(inst, flags)
+            films = torch.empty((sum(v["width"] * v["height"] for v in views), 4), dtype=torch.float32,
+                                device=c.device)
+            # the direct calls the recording must reproduce (in place, as the recording will rebuild)
+            c.build_blas_batch(lists, flags, into=blases, scratch=blases[0].keepalive[0])
+            c.build_tlas(inst, flags, into=tlas, scratch=tlas.keepalive[0])
+            c.render_batch(tlas, rec, views, lt, spp=spp, max_depth=depth, films=films)
+            c.sync()
+            direct = films.clone()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+            c.build_blas_batch(lists, flags, into=blases, scratch=blases[0].keepalive[0])
+            c.build_tlas(inst, flags, into=tlas, scratch=tlas.keepalive[0])
+            c.render_batch(tlas, rec, views, lt, spp=spp, max_depth=depth, films=films)
+        form = c.render_form()
+        g.replay()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(films, direct))
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 3
+        with torch.cuda.stream(s):
+            a.record()
+            for _ in range(reps):
+                g.replay()
+            b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        c.sync()
+        log(f"recorded frame (batched build + TLAS build + {spp}-spp batched render, one graph): {ms:.1f} ms, "
+            f"replay equals direct calls: {same}")
+        del g
+        return {"ms_per_frame": round(ms, 1), "replay_equals_direct": same,
+                "render_form": "wavefront" if form == pyrtx.RTX_RENDER_FORM_WAVEFRONT else "whole-path"}
+    finally:
+        c.close()
 
 
 if __name__ == "__main__":
