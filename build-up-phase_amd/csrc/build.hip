@@ -992,7 +992,8 @@ __device__ __forceinline__ float wide_split(const float* tl, const float* tr, in
 __global__ __launch_bounds__(256) void wide_cost_kernel(uint32_t n, const int32_t* __restrict__ left,
                                                         const int32_t* __restrict__ right,
                                                         const int32_t* __restrict__ parent, const float* nboxes,
-                                                        float* wc, uint32_t* arrivals, float c_leaf, int width) {
+                                                        float* wc, uint32_t* __restrict__ wdec, uint32_t* arrivals,
+                                                        float c_leaf, int width) {
     const uint32_t pos = blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= n) return;
     int32_t node = load_coherent_i32(parent + (uint64_t)n - 1 + pos);
@@ -1011,9 +1012,15 @@ __global__ __launch_bounds__(256) void wide_cost_kernel(uint32_t n, const int32_
         float c[8];
         int k;
         c[0] = box_area(b) + wide_split(tl, tr, width, k);
-        for (int j = 2; j <= 8; ++j) c[j - 1] = fminf(c[j - 2], wide_split(tl, tr, j, k));
+        uint32_t dec = 0;
+        for (int j = 2; j <= 8; ++j) {
+            const float d = wide_split(tl, tr, j, k);
+            dec |= ((uint32_t)k | (d < c[j - 2] ? 8u : 0u)) << (4 * (j - 2));
+            c[j - 1] = fminf(c[j - 2], d);
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) store_coherent(wc + 8 * (uint64_t)node + j, c[j]);
+        wdec[node] = dec;  // read by later launches only
         node = load_coherent_i32(parent + node);
     }
 }
@@ -1176,7 +1183,7 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                                                              AsHeader* const* __restrict__ as_ptr,
                                                              const uint32_t* __restrict__ tblas_in,
                                                              uint32_t* __restrict__ tblas_out,
-                                                             const float* __restrict__ wc, float c_leaf, int width) {
+                                                             const uint32_t* __restrict__ wdec, int width) {
     const uint32_t count = level_counts[level];
     if (count == 0) return;
     for (uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x; ti < count; ti += gridDim.x * blockDim.x) {
@@ -1191,8 +1198,9 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
         if ((int32_t)task.x < 0) {
             ch[0] = (int32_t)task.x;
             cnt = 1;
-        } else if (wc) {
-            // the children D(task, W) picks: a stack of (subtree, slots) pairs whose slots add up to W
+        } else if (wdec) {
+            // the children D(task, W) picks: a stack of (subtree, slots) pairs whose slots add up to W; each visited
+            // node's split and "fewer slots" choices for every slot count come from the cost pass (wdec)
             int32_t sn[8];
             int si[8];
             int sp = 0;
@@ -1208,13 +1216,10 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                     ch[cnt++] = x;
                     continue;
                 }
-                float tl[8], tr[8];
                 const int32_t l = left[x], r = right[x];
-                wide_table(n, l, nboxes, wc, c_leaf, tl);
-                wide_table(n, r, nboxes, wc, c_leaf, tr);
-                int k;
-                const float d = wide_split(tl, tr, j, k);
-                if (!root && !(d < wc[8 * (uint64_t)x + j - 2])) {
+                const uint32_t dj = wdec[x] >> (4 * (j - 2));
+                const int k = (int)(dj & 7u);
+                if (!root && !(dj & 8u)) {
                     sn[sp] = x;  // fewer slots cost no more
                     si[sp++] = j - 1;
                 } else {
@@ -1231,7 +1236,7 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
             cnt = 2;
         }
         // greedy (no cost table): open the internal child with the largest surface area until W children
-        while (!wc && cnt < width) {
+        while (!wdec && cnt < width) {
             int best = -1;
             float best_a = -1.0f;
 #pragma unroll
@@ -1311,40 +1316,32 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                 if (j == bj) pos_of[j] = bp;
             used |= 1u << bp;
         }
-        // node frame
-        Node8 node;
+        // node frame; the 64-B node is assembled in registers (child bytes placed by shifts: a per-child byte
+        // index into a local array would put the node in private memory)
         float p[3];
         uint32_t e[3];
+        uint16_t pq[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            uint16_t pq;
-            make_frame(nlo[a], nhi[a], rlo[a], qs[a], pq, p[a], e[a]);
-            node.pq[a] = pq;
-            node.e[a] = (uint8_t)e[a];
-        }
-        node.imask = 0;
-        node.vmask = (uint8_t)used;
-        node.pad = 0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                node.qlo[a][k] = 0;
-                node.qhi[a][k] = 0;
-            }
+        for (int a = 0; a < 3; ++a) make_frame(nlo[a], nhi[a], rlo[a], qs[a], pq[a], p[a], e[a]);
+        uint64_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+        uint32_t imask = 0;
         const uint32_t nvalid = (uint32_t)__popc(used);
         const uint32_t base = BATCHED ? grouped_atomic_add(&h->slot_count, nvalid) : atomicAdd(&h->slot_count, nvalid);
-        node.base = base;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             if (!cv[j]) continue;
             int k = pos_of[j];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) quant_child(cb[j][a], cb[j][3 + a], p[a], e[a], node.qlo[a][k], node.qhi[a][k]);
+            for (int a = 0; a < 3; ++a) {
+                uint8_t ql, qh;
+                quant_child(cb[j][a], cb[j][3 + a], p[a], e[a], ql, qh);
+                qlo[a] |= (uint64_t)ql << (8 * k);
+                qhi[a] |= (uint64_t)qh << (8 * k);
+            }
             uint32_t slot = base + (uint32_t)__popc(used & ((1u << k) - 1u));
             int32_t c = ch[j];
             if (c >= 0) {
-                node.imask |= (uint8_t)(1u << k);
+                imask |= 1u << k;
                 uint32_t o = atomicAdd(&level_counts[level + 1], 1u);
                 tasks_out[o] = make_uint2((uint32_t)c, slot);
                 if (BATCHED) tblas_out[o] = tb;
@@ -1358,12 +1355,14 @@ __global__ __launch_bounds__(256) void collapse_level_kernel(uint32_t n, const i
                 dst[3] = src[3];
             }
         }
-        const uint4* nsrc = (const uint4*)&node;
+        // Node8 layout: pq[3] (bytes 0-5), e[3] (6-8), imask (9), vmask (10), pad (11), base (12-15),
+        // qlo[3][8] (16-39), qhi[3][8] (40-63)
         uint4* ndst = slots + 4 * (uint64_t)task.y;
-        ndst[0] = nsrc[0];
-        ndst[1] = nsrc[1];
-        ndst[2] = nsrc[2];
-        ndst[3] = nsrc[3];
+        ndst[0] = make_uint4((uint32_t)pq[0] | (uint32_t)pq[1] << 16, (uint32_t)pq[2] | e[0] << 16 | e[1] << 24,
+                             e[2] | imask << 8 | used << 16, base);
+        ndst[1] = make_uint4((uint32_t)qlo[0], (uint32_t)(qlo[0] >> 32), (uint32_t)qlo[1], (uint32_t)(qlo[1] >> 32));
+        ndst[2] = make_uint4((uint32_t)qlo[2], (uint32_t)(qlo[2] >> 32), (uint32_t)qhi[0], (uint32_t)(qhi[0] >> 32));
+        ndst[3] = make_uint4((uint32_t)qhi[1], (uint32_t)(qhi[1] >> 32), (uint32_t)qhi[2], (uint32_t)(qhi[2] >> 32));
         if (BATCHED) {
             grouped_atomic_add(&h->node_count, 1u);
             grouped_atomic_max(&h->max_depth, (uint32_t)(level + 1));
@@ -1493,8 +1492,8 @@ __global__ void arg_blob_kernel(ArgBlob b, uint32_t* __restrict__ dst, uint32_t 
 
 // ---- scratch layout --------------------------------------------------------------------------------------------
 struct Layout {
-    uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, cost, cnt, hgt, wcost, tasks0,
-        tasks1, levels, cbounds, misc, geoms, total;
+    uint64_t leaves, boxes, keys, vals, sort, left, right, parent, nboxes, arrivals, cost, cnt, hgt, wcost, wdec,
+        tasks0, tasks1, levels, cbounds, misc, geoms, total;
     // batched builds only
     uint64_t pblas = 0, nblas = 0, roots = 0, asptr = 0, binfo = 0, tblas0 = 0, tblas1 = 0, bcb = 0;
 };
@@ -1522,6 +1521,7 @@ Layout make_layout(uint32_t n, int key_bytes, uint32_t ngeom, uint32_t nbatch = 
     L.cnt = take(nn * 4);
     L.hgt = take(nn * 4);
     L.wcost = take(nn * 32);
+    L.wdec = take(nn * 4);
     L.tasks0 = take(nn * 8);
     L.tasks1 = take(nn * 8);
     L.levels = take(128 * 4);
@@ -1587,7 +1587,7 @@ void refit_passes(uint32_t n, uint32_t flags, uint8_t* scratch, const Layout& L,
         wide_cost_kernel<<<(n + 255) / 256, 256, 0, s>>>(
             n, (const int32_t*)(scratch + L.left), (const int32_t*)(scratch + L.right),
             (const int32_t*)(scratch + L.parent), (const float*)(scratch + L.nboxes), (float*)(scratch + L.wcost),
-            arrivals, c_leaf, node_width(flags));
+            (uint32_t*)(scratch + L.wdec), arrivals, c_leaf, node_width(flags));
     }
 }
 
@@ -1640,8 +1640,7 @@ hipError_t lbvh_common(uint32_t n, bool k64, uint32_t type, uint32_t flags, uint
             collapse_level_kernel<false><<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
                                                                   (const uint4*)(scratch + L.leaves), h, slots, levels,
                                                                   lv, tin, tout, nullptr, nullptr, nullptr,
-                                                                  n > 1 ? (const float*)(scratch + L.wcost) : nullptr,
-                                                                  type == 1 ? kLeafCostInstance : kLeafCostTri,
+                                                                  n > 1 ? (const uint32_t*)(scratch + L.wdec) : nullptr,
                                                                   node_width(flags));
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, max_levels, err_flags);
@@ -1831,8 +1830,8 @@ hipError_t update_blas(const GeomDesc* geoms_host, uint32_t ngeom, uint32_t n, u
             collapse_level_kernel<false><<<kCollapseBlocks, 256, 0, s>>>(
                 n, (const int32_t*)(sc + L.left), (const int32_t*)(sc + L.right), (const uint32_t*)(sc + L.vals),
                 (const float*)(sc + L.nboxes), (const uint4*)(sc + L.leaves), h, (uint4*)((uint8_t*)dst + kHeaderBytes),
-                levels, lv, tin, tout, nullptr, nullptr, nullptr, n > 1 ? (const float*)(sc + L.wcost) : nullptr,
-                kLeafCostTri, node_width(flags));
+                levels, lv, tin, tout, nullptr, nullptr, nullptr, n > 1 ? (const uint32_t*)(sc + L.wdec) : nullptr,
+                node_width(flags));
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(h, levels, kMaxLevels32, err_flags);
     }
@@ -1919,8 +1918,8 @@ hipError_t build_blas_batch(const GeomDesc* geoms_host, uint32_t ngeom, const ui
             collapse_level_kernel<true><<<kCollapseBlocks, 256, 0, s>>>(n, left, right, vals, nboxes,
                                                                   (const uint4*)(sc + L.leaves), nullptr, nullptr,
                                                                   levels, lv, tin, tout, as_ptr, bin, bout,
-                                                                  n > 1 ? (const float*)(sc + L.wcost) : nullptr,
-                                                                  kLeafCostTri, node_width(flags));
+                                                                  n > 1 ? (const uint32_t*)(sc + L.wdec) : nullptr,
+                                                                  node_width(flags));
         }
         collapse_check_kernel<<<1, 64, 0, s>>>(nullptr, levels, kMaxLevels32, err_flags);
     }
