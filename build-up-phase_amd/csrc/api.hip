@@ -225,6 +225,13 @@ int grow_buffer(rtx_context ctx, void** buf, uint64_t* have, uint64_t need, cons
     return RTX_SUCCESS;
 }
 
+// A host array that a call being recorded must carry inside the graph (rtx::upload_host_array's limit).
+int check_recordable(rtx_context ctx, uint64_t bytes, const char* what) {
+    if (bytes <= rtx::kArgUploadMax || !stream_capturing(ctx->stream)) return RTX_SUCCESS;
+    return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "a recorded call carries at most %llu bytes of %s (%llu given)",
+                (unsigned long long)rtx::kArgUploadMax, what, (unsigned long long)bytes);
+}
+
 // After enqueuing a build or copy of the structure at `address` on the context stream.
 void note_as_written(rtx_context ctx, uint64_t address) {
     std::lock_guard<std::mutex> lock(g_prop_mu);
@@ -528,6 +535,8 @@ int rtx_cmd_build_blas(rtx_context ctx, const rtx_blas_geometry* geometries, uin
     if (scratch_size < rtx::blas_scratch_bytes((uint32_t)n, k64, geometry_count))
         return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "scratch_size %llu below the size query",
                     (unsigned long long)scratch_size);
+    if ((r = check_recordable(ctx, (uint64_t)geometry_count * sizeof(rtx::GeomDesc), "geometry descriptors")))
+        return r;
     std::vector<rtx::GeomDesc> gd(geometry_count ? geometry_count : 1);
     uint32_t base = 0;
     for (uint32_t i = 0; i < geometry_count; ++i) {
@@ -613,6 +622,9 @@ int rtx_cmd_build_blas_batch(rtx_context ctx, const rtx_blas_build_info* infos, 
     if (scratch_size < rtx::blas_batch_scratch_bytes((uint32_t)n, ng, count))
         return fail(ctx, RTX_ERROR_OUT_OF_DEVICE_MEMORY, "scratch_size %llu below the size query",
                     (unsigned long long)scratch_size);
+    if ((r = check_recordable(ctx, (uint64_t)ng * sizeof(rtx::GeomDesc), "geometry descriptors")) ||
+        (r = check_recordable(ctx, (uint64_t)count * sizeof(void*), "BLAS addresses")))
+        return r;
     std::vector<rtx::GeomDesc> gd(ng ? ng : 1);
     std::vector<uint2> binfo(count);
     std::vector<rtx::AsHeader*> dst(count);
@@ -1107,6 +1119,7 @@ int rtx_cmd_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* 
     if (r) return r;
     // two view slots: the recorded graph's views, and those of direct calls (which must not disturb the graph)
     const uint64_t vbytes = view_count * sizeof(rtx::BatchView);
+    if ((r = check_recordable(ctx, vbytes, "batch views"))) return r;
     const uint64_t vslot = (vbytes + 255) & ~255ull;
     if (2 * vslot > ctx->views_bytes) {
         ctx->graph_key = 0;

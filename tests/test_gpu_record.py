@@ -157,7 +157,8 @@ def test_recording_refuses_what_it_cannot_record(rec):
     _say("recording refusals")
     sc, lights = S.cornell_scene(3, width=32, height=32)
     with torch.cuda.stream(s):
-        walls = c.build_blas(_dev_geoms(sc.blas[0]))
+        wall_geoms = _dev_geoms(sc.blas[0])
+        walls = c.build_blas(wall_geoms)
         geoms = _dev_geoms(sc.blas[1])
         blas = c.build_blas(geoms, pyrtx.RTX_BUILD_PREFER_FAST_TRACE_BIT | pyrtx.RTX_BUILD_ALLOW_UPDATE_BIT)
         tlas = c.build_tlas(_dev(pyrtx.instance_records(sc, [walls.address, blas.address])))
@@ -179,10 +180,15 @@ def test_recording_refuses_what_it_cannot_record(rec):
             c.update_blas(blas, geoms)
         except pyrtx.RtxError as e:
             errs["update"] = (e.code, str(e))
+        try:  # 1,200 geometry descriptors (77 KB) exceed what a recorded build carries
+            c.build_blas(wall_geoms * 300, scratch=big.view(torch.uint8))
+        except pyrtx.RtxError as e:
+            errs["descriptors"] = (e.code, str(e))
     # the recorded render ran on a TLAS whose content was read back before the capture: whole-path launches
     assert c.render_form() == pyrtx.RTX_RENDER_FORM_WHOLE_PATH
     assert errs["grow"][0] == pyrtx.RTX_ERROR_OUT_OF_DEVICE_MEMORY and "before recording" in errs["grow"][1]
     assert errs["update"][0] == pyrtx.RTX_ERROR_INVALID_ARGUMENT and "cannot be recorded" in errs["update"][1]
+    assert errs["descriptors"][0] == pyrtx.RTX_ERROR_INVALID_ARGUMENT and "at most 65536 bytes" in errs["descriptors"][1]
     with torch.cuda.stream(s):
         ref = c.render(tlas, recs, sc.camera, lt, 32, 32, spp=1).clone()
         c.sync()
