@@ -198,3 +198,53 @@ def test_recording_refuses_what_it_cannot_record(rec):
     c.sync()
     assert torch.equal(small, ref)
     del g
+
+
+def test_recorded_nrc_frames_train_on(rec):
+    """NRC frames recorded by a capture of the context stream (the training wavefront's side stream joins it
+    through the frame's fork / join events): each replay trains the cache again, and film, parameters and step
+    count equal direct frames on an identical cache, replay after replay."""
+    import pyrtx
+    from pyrtx import scenes as S
+    c, s = rec
+    _say("recorded NRC frames")
+    W, H = 80, 60
+    sc, lights = S.cornell_scene(3, width=W, height=H)
+    lo, hi = S.scene_bounds(sc)
+    with torch.cuda.stream(s):
+        blases, tlas = c.upload_scene(sc)
+        recs = _dev(sc.records)
+        lt = torch.from_numpy(np.ascontiguousarray(lights, np.float32).reshape(-1, 12)).cuda()
+        direct = pyrtx.Nrc(c, lo, hi, seed=2, batch_size=2048, cache_vertex=1)
+        graph = pyrtx.Nrc(c, lo, hi, seed=2, batch_size=2048, cache_vertex=1)
+        fd = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        fg = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        for n in (direct, graph):  # a first frame on each cache outside the capture sizes its buffers
+            c.render_nrc(tlas, recs, sc.camera, lt, n, W, H, spp=2, train_paths=4096, sample_offset=0,
+                         film=fd if n is direct else fg)
+        c.sync()
+    try:
+        assert torch.equal(fd, fg)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+            c.render_nrc(tlas, recs, sc.camera, lt, graph, W, H, spp=2, train_paths=4096, sample_offset=1, film=fg)
+        for fr in range(3):
+            with torch.cuda.stream(s):
+                c.render_nrc(tlas, recs, sc.camera, lt, direct, W, H, spp=2, train_paths=4096, sample_offset=1,
+                             film=fd)
+                c.sync()
+            fg.fill_(-1.0)
+            g.replay()
+            torch.cuda.synchronize()
+            c.sync()
+            assert torch.equal(fd, fg), fr
+            assert np.array_equal(direct.params(), graph.params()), fr
+            assert direct.status() == graph.status(), fr
+            x = torch.randn(1 << 20, device="cuda").cumsum(0)  # unrelated work between replays
+            torch.cuda.synchronize()
+            assert torch.isfinite(x).all()
+        assert graph.status()[0] >= 4 and float(fg[..., :3].mean()) > 0.01
+        del g
+    finally:
+        direct.close()
+        graph.close()
