@@ -830,8 +830,8 @@ def recorded_frame_bench(ctx, lists, sc, rec, views, lt, spp, depth, width, tune
     """The batch's whole frame - batched BLAS build, TLAS build, batched render - recorded by a capture of a
     context's stream into one HIP graph (the command-buffer form of the reference's recorded build and trace) and
     replayed; each replay rebuilds both levels from the device data.  A TLAS built inside the recording cannot be
-    read back on the host, so its render takes the general traversal and the wavefront integrator.  The replays
-    are checked bit-identical to the same calls made directly."""
+    read back on the host, so the render is recorded in both integrator forms, each guarded by the content bits the
+    device reads from the TLAS at replay.  The replays are checked bit-identical to the same calls made directly."""
     s = torch.cuda.Stream()
     c = pyrtx.Context(0, stream=s)
     try:
@@ -875,7 +875,8 @@ def recorded_frame_bench(ctx, lists, sc, rec, views, lt, spp, depth, width, tune
             f"replay equals direct calls: {same}")
         del g
         return {"ms_per_frame": round(ms, 1), "replay_equals_direct": same,
-                "render_form": "wavefront" if form == pyrtx.RTX_RENDER_FORM_WAVEFRONT else "whole-path"}
+                "render_form": {pyrtx.RTX_RENDER_FORM_WHOLE_PATH: "whole-path", pyrtx.RTX_RENDER_FORM_WAVEFRONT:
+                                "wavefront", pyrtx.RTX_RENDER_FORM_DEVICE_CHOSEN: "device-chosen"}.get(form, str(form))}
     finally:
         c.close()
 

@@ -189,10 +189,11 @@ uint64_t g_prop_clock = 0;
 struct TlasProps {
     uint32_t content;  // kContent* bits
     uint32_t depth;    // BVH8 levels, TLAS + deepest BLAS (0: unknown)
+    bool known;
 };
 // what a launch assumes when the properties cannot be read without breaking a graph capture: every program kind and
-// the default stack
-constexpr TlasProps kUnknownProps = {rtx::kContentNonOpaque | rtx::kContentAabb, 0};
+// the default stack (recorded launches then let the device choose the traversal variant where they can)
+constexpr TlasProps kUnknownProps = {rtx::kContentNonOpaque | rtx::kContentAabb, 0, false};
 
 bool stream_capturing(hipStream_t s) {
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
@@ -302,7 +303,7 @@ TlasProps tlas_props(rtx_context ctx, uint64_t address) {
                 p.done = true;
             }
             p.used = ++g_prop_clock;
-            return TlasProps{p.host[3], p.host[1]};
+            return TlasProps{p.host[3], p.host[1], true};
         }
     }
     // not recorded (built inside a capture, or the slot was reused): read the header after the context's work
@@ -311,7 +312,7 @@ TlasProps tlas_props(rtx_context ctx, uint64_t address) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess ||
         hipMemcpy(w, (const uint8_t*)(uintptr_t)address + 32, sizeof w, hipMemcpyDeviceToHost) != hipSuccess)
         return kUnknownProps;
-    return TlasProps{w[3], w[1]};
+    return TlasProps{w[3], w[1], true};
 }
 
 // Traversal stack entries per lane: 2 per BVH8 level of the deepest TLAS -> BLAS path + 4 (each level's node step
@@ -821,6 +822,7 @@ int rtx_cmd_trace_rays(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     q.stats = ctx->stats;
     const TlasProps tp = tlas_props(ctx, tlas_address);
     q.content = tp.content;
+    q.guard = tp.known ? 0u : 3u;  // recorded before the content could be read back: the device picks the variant
     rtx::TraceLaunch L = launch_shape(ctx, tp.depth);
     if ((r = ensure_gstack(ctx, L))) return r;
     q.gstack = ctx->gstack;
@@ -856,6 +858,7 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
     q.stats = ctx->stats;
     const TlasProps tp = tlas_props(ctx, tlas_address);
     q.content = tp.content;
+    q.guard = tp.known ? 0u : 3u;  // recorded before the content could be read back: the device picks the variant
     rtx::TraceLaunch L = launch_shape(ctx, tp.depth);
     if ((r = ensure_gstack(ctx, L))) return r;
     q.gstack = ctx->gstack;
@@ -890,6 +893,7 @@ int rtx_cmd_trace_ray_buffer_noalloc(rtx_context ctx, uint64_t tlas_address, con
     q.stats = false;  // the traversal counters live in the context's state, which this variant does not use
     const TlasProps tp = tlas_props(ctx, tlas_address);
     q.content = tp.content;
+    q.guard = tp.known ? 0u : 3u;
     const rtx::TraceLaunch L = launch_shape(ctx, tp.depth);
     ctx->last_stack = L.lds_depth + L.gdepth;
     uint64_t need = 0;
@@ -952,10 +956,12 @@ static int render_setup(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* s
     a.state = ctx->state;
     const TlasProps tp = tlas_props(ctx, tlas_address);
     a.content = tp.content;
+    a.dual = tp.known ? 0u : 1u;  // recorded before the content could be read back: both forms, the device picks
     L = launch_shape(ctx, tp.depth);
     if ((r = ensure_gstack(ctx, L))) return r;
     a.gstack = ctx->gstack;
-    ctx->render_form = rtx::pt_whole_paths(a) ? RTX_RENDER_FORM_WHOLE_PATH : RTX_RENDER_FORM_WAVEFRONT;
+    ctx->render_form = rtx::pt_dual_forms(a) ? RTX_RENDER_FORM_DEVICE_CHOSEN
+                       : rtx::pt_whole_paths(a) ? RTX_RENDER_FORM_WHOLE_PATH : RTX_RENDER_FORM_WAVEFRONT;
     return RTX_SUCCESS;
 }
 

@@ -67,12 +67,19 @@ struct PtArgs {
     uint32_t* path_view;       // view of each path slot
     const uint32_t* sample_dev;  // when set, added to every pass's sample index (graph replays)
     uint32_t content;            // AsHeader::content of the TLAS (traversal variant, whole-path availability)
+    // Calls recorded before the TLAS's content can be read back: dual = 1 records a plain sample both as the
+    // whole-path launch (guarded to run only for opaque-triangle content) and as the wavefront (guard = 2: its
+    // camera stage runs only for other content, so its later stages find empty queues), and the device runs one.
+    uint32_t dual;
+    uint32_t guard;
 };
 
 // Whether the render's samples run as whole-path launches (trace modes 6 / 7, and 5 for full-length training paths):
 // for a TLAS whose content the default traversal variant serves, unless RTX_PATH_WAVEFRONT=1 asks for the per-vertex
 // wavefront (both give bit-identical films).
 bool pt_whole_paths(const PtArgs& a);
+// Whether a plain render records both forms (PtArgs::dual, unless RTX_PATH_WAVEFRONT=1 forces the wavefront).
+bool pt_dual_forms(const PtArgs& a);
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights);
 void pt_layout(PtArgs& a, void* scratch, uint32_t npix, uint32_t max_depth, uint32_t nlights);

@@ -100,6 +100,7 @@ __global__ void light_cdf_kernel(const float* __restrict__ lights, uint32_t n, f
 
 // Camera rays of paths [first_path, npaths): path i enters the extension queue at slot i - first_path.
 __global__ __launch_bounds__(256) void camera_kernel(PtArgs a, uint32_t sample) {
+    if (a.guard && !guard_passes(a.tlas, 0u, a.guard)) return;  // the other form runs this sample
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i = a.first_path + slot;
     if (slot == 0) a.counts[0] = a.npaths - a.first_path;
@@ -199,8 +200,9 @@ __global__ void fill_words_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t
         p[i] = v;
 }
 
-__global__ void count_camera_rays_kernel(DeviceState* st, uint32_t n) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(&st->pt_rays, (unsigned long long)n);
+__global__ void count_camera_rays_kernel(DeviceState* st, uint32_t n, const void* tlas, uint32_t guard) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && (!guard || guard_passes(tlas, 0u, guard)))
+        atomicAdd(&st->pt_rays, (unsigned long long)n);
 }
 
 // Folds the training wavefront's ray count and error flags into the context's state and clears them.
@@ -290,10 +292,14 @@ __global__ __launch_bounds__(256) void sample_scatter_kernel(PtArgs a, uint64_t 
 
 }  // namespace
 
-bool pt_whole_paths(const PtArgs& a) {
+static bool wavefront_forced() {
     const char* e = std::getenv("RTX_PATH_WAVEFRONT");
-    return !(e && e[0] == '1') && !trace_needs_flags(0u, a.content);
+    return e && e[0] == '1';
 }
+
+bool pt_whole_paths(const PtArgs& a) { return !wavefront_forced() && !trace_needs_flags(0u, a.content); }
+
+bool pt_dual_forms(const PtArgs& a) { return a.dual && !wavefront_forced(); }
 
 uint64_t pt_scratch_bytes(uint32_t npix, uint32_t max_depth, uint32_t nlights) {
     uint64_t n = npix ? npix : 1;
@@ -351,7 +357,7 @@ static hipError_t run_pass(PtArgs& a, const TraceLaunch& L, uint32_t sample, uin
     const uint64_t ncounts = 3ull * (a.max_depth + 2);
     fill_words_kernel<<<1, 64, 0, s>>>(a.counts, ncounts, 0u);
     camera_kernel<<<gb, 256, 0, s>>>(a, sample);
-    count_camera_rays_kernel<<<1, 64, 0, s>>>(a.state, n);
+    count_camera_rays_kernel<<<1, 64, 0, s>>>(a.state, n, a.tlas, a.guard);
     hipError_t e;
     // vertex 0's extension rays (camera rays) alone; afterwards vertex d's shadow rays and vertex d + 1's extension
     // rays, both produced by shade(d), go through one fused launch
@@ -402,7 +408,8 @@ static hipError_t infer_gather(PtArgs& a, const NrcState& nrc, const uint32_t* c
 // One sample of a plain render: every pixel path whole in one traversal launch (each lane traces, shades and
 // continues its own paths), or the per-vertex wavefront where that launch is not available.
 static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample, hipStream_t s) {
-    if (!pt_whole_paths(a)) return run_pass(a, L, sample, a.max_depth - 1, s);
+    const bool dual = pt_dual_forms(a);
+    if (!dual && !pt_whole_paths(a)) return run_pass(a, L, sample, a.max_depth - 1, s);
     TraceRequest tr{};
     tr.tlas = a.tlas;
     tr.mode = 6;
@@ -416,8 +423,13 @@ static hipError_t render_sample(PtArgs& a, const TraceLaunch& L, uint32_t sample
     tr.gstack = a.gstack;
     tr.pt = &a;
     tr.pt_sample = sample;
-    tr.content = a.content;
-    return trace_rays_launch(tr, L, s);
+    tr.content = dual ? 0u : a.content;
+    tr.guard = dual ? 1u : 0u;
+    hipError_t e = trace_rays_launch(tr, L, s);
+    if (e || !dual) return e;
+    PtArgs w = a;
+    w.guard = 2;
+    return run_pass(w, L, sample, a.max_depth - 1, s);
 }
 
 // The pixel paths of one NRC sample up to their cache queries (queued from a.counts[2]): whole paths in one
@@ -481,6 +493,7 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
                          uint32_t sample_offset, float* film, double* acc64, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
     hipError_t e;
+    a.dual = 0;  // NRC frames take one form (the wavefront where the content is not known)
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
     fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
     a.mode = kPtCache;

@@ -42,6 +42,10 @@ struct TraceRequest {
     const PtArgs* pt;  // mode 5: the path tracer's arguments
     uint32_t pt_sample;
     uint32_t content;  // AsHeader::content of the TLAS (kContent* bits): selects the traversal variant
+    // Device-side variant choice, for calls recorded into a graph before the TLAS's content can be read back:
+    // 1 = the launch runs only if the TLAS, read on the device, needs no FLAGS traversal for ray_flags; 2 = only if
+    // it does; 3 (ray buffers and raygen) = both variants are launched, each guarded, so the device runs one.
+    uint32_t guard = 0;
 };
 
 struct TraceLaunch {
@@ -62,5 +66,20 @@ hipError_t trace_rays_launch(const TraceRequest& req, const TraceLaunch& launch,
 // intersection programs, ray-flag culls); modes 5-7 (whole paths in one launch) exist only in the default variant, so
 // they serve a TLAS for which trace_needs_flags(0, content) is false.
 bool trace_needs_flags(uint32_t ray_flags, uint32_t content);
+
+// Ray flags that need the FLAGS traversal variant, and the variant choice itself (host and device).
+constexpr uint32_t kFlagsVariantRayFlags =
+    RTX_RAY_FLAG_TERMINATE_ON_FIRST_HIT | RTX_RAY_FLAG_SKIP_CLOSEST_HIT_SHADER |
+    RTX_RAY_FLAG_CULL_BACK_FACING_TRIANGLES | RTX_RAY_FLAG_CULL_FRONT_FACING_TRIANGLES | RTX_RAY_FLAG_CULL_OPAQUE |
+    RTX_RAY_FLAG_CULL_NO_OPAQUE | RTX_RAY_FLAG_NO_OPAQUE | RTX_RAY_FLAG_SKIP_TRIANGLES | RTX_RAY_FLAG_SKIP_AABBS;
+__host__ __device__ __forceinline__ bool needs_flags_variant(uint32_t ray_flags, uint32_t content) {
+    return (ray_flags & kFlagsVariantRayFlags) || (content & kContentAabb) ||
+           (!(ray_flags & RTX_RAY_FLAG_OPAQUE) && (content & kContentNonOpaque));
+}
+// The choice on the device, from the TLAS header: whether a launch guarded with `guard` (1 / 2 as
+// TraceRequest::guard) is the one to run.
+__device__ __forceinline__ bool guard_passes(const void* tlas, uint32_t ray_flags, uint32_t guard) {
+    return needs_flags_variant(ray_flags, ((const AsHeader*)tlas)->content) == (guard == 2u);
+}
 
 }  // namespace rtx

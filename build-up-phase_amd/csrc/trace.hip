@@ -1247,10 +1247,17 @@ hipError_t launch(const TraceArgs& a, const TraceLaunch& L, uint32_t content, hi
 // The FLAGS variant resolves opacity and runs any-hit programs (needed for non-opaque content unless the ray is
 // OPAQUE), runs intersection programs (AABB content) and applies the ray-flag culls; the default variant treats
 // every primitive as an opaque triangle.
-bool trace_needs_flags(uint32_t ray_flags, uint32_t content) {
-    return (ray_flags & kActiveRayFlags) || (content & kContentAabb) ||
-           (!(ray_flags & kRayOpaque) && (content & kContentNonOpaque));
+static_assert(kFlagsVariantRayFlags == kActiveRayFlags, "one set of ray flags selects the FLAGS variant");
+bool trace_needs_flags(uint32_t ray_flags, uint32_t content) { return needs_flags_variant(ray_flags, content); }
+
+namespace {
+// A guarded launch that the device does not choose takes no ray: its rays come from the ray counter, which is set
+// past every ray id (no wrap-around: fetches add at most a few chunks per wave).
+constexpr uint32_t kGuardSkip = 0xC0000000u;
+__global__ void guard_counter_kernel(const void* tlas, uint32_t ray_flags, uint32_t guard, DeviceState* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !guard_passes(tlas, ray_flags, guard)) st->ray_counter = kGuardSkip;
 }
+}  // namespace
 
 hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStream_t s) {
     TraceArgs a{};
@@ -1295,8 +1302,31 @@ hipError_t trace_rays_launch(const TraceRequest& r, const TraceLaunch& L, hipStr
         a.pt_first = r.mode == 5 ? r.pt->npix : 0u;
         a.pt_sample = r.pt_sample;
     }
+    if (r.n == 0 && !r.n_ptr) {
+        reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
+        return hipGetLastError();
+    }
+    if (r.guard == 3u && r.mode <= 2 && !(r.ray_flags & kActiveRayFlags)) {
+        // both variants, each guarded by the TLAS content the device reads: one of them takes every ray
+        hipError_t e;
+        for (uint32_t g = 1; g <= 2; ++g) {
+            reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats && g == 1 ? 1 : 0);
+            guard_counter_kernel<<<1, 64, 0, s>>>(r.tlas, r.ray_flags, g, r.state);
+            const uint32_t c = g == 1 ? 0u : (kContentAabb | kContentNonOpaque);
+            switch (r.mode) {
+                case 0:
+                    e = r.stats ? launch<kModeClosest, true>(a, L, c, s) : launch<kModeClosest, false>(a, L, c, s);
+                    break;
+                case 1: e = r.stats ? launch<kModeAny, true>(a, L, c, s) : launch<kModeAny, false>(a, L, c, s); break;
+                default:
+                    e = r.stats ? launch<kModeRaygen, true>(a, L, c, s) : launch<kModeRaygen, false>(a, L, c, s);
+            }
+            if (e) return e;
+        }
+        return hipSuccess;
+    }
     reset_counter_kernel<<<1, 64, 0, s>>>(r.state, r.stats ? 1 : 0);
-    if (r.n == 0 && !r.n_ptr) return hipGetLastError();
+    if (r.guard == 1u || r.guard == 2u) guard_counter_kernel<<<1, 64, 0, s>>>(r.tlas, r.ray_flags, r.guard, r.state);
     switch (r.mode) {
         case 0:
             return r.stats ? launch<kModeClosest, true>(a, L, r.content, s) : launch<kModeClosest, false>(a, L, r.content, s);

@@ -81,7 +81,7 @@ class AsInfo(ctypes.Structure):
 
 
 RTX_AS_CONTENT_NON_OPAQUE_BIT, RTX_AS_CONTENT_AABBS_BIT = 0x1, 0x2
-RTX_RENDER_FORM_NONE, RTX_RENDER_FORM_WHOLE_PATH, RTX_RENDER_FORM_WAVEFRONT = 0, 1, 2
+RTX_RENDER_FORM_NONE, RTX_RENDER_FORM_WHOLE_PATH, RTX_RENDER_FORM_WAVEFRONT, RTX_RENDER_FORM_DEVICE_CHOSEN = 0, 1, 2, 3
 
 
 class Sbt(ctypes.Structure):

@@ -29,10 +29,11 @@
  *    to a recorded call (build descriptors, batch views: up to 64 KiB each) are copied into the graph.  A
  *    recorded call cannot allocate: make each call once outside the capture first, so the context's internal
  *    buffers have their size (a call that would grow one fails with RTX_ERROR_OUT_OF_DEVICE_MEMORY).  A TLAS
- *    built before the capture is traced with the traversal its content allows; one built inside it is traced
- *    with the general (any-hit / intersection capable) traversal and the wavefront integrator, which give the
- *    same results.  RTX_RENDER_GRAPH_BIT is ignored inside a capture.  rtx_cmd_update_blas and the rtx_get_*
- *    queries synchronise and cannot be recorded.
+ *    built and synchronised before the capture is traced with the traversal its content allows; for one built
+ *    inside it, traces and plain / batched renders record both traversal variants (and both integrator forms),
+ *    each guarded by the content the device reads at replay, and NRC frames take the general traversal and the
+ *    wavefront; all give the same results.  RTX_RENDER_GRAPH_BIT is ignored inside a capture.
+ *    rtx_cmd_update_blas and the rtx_get_* queries synchronise and cannot be recorded.
  *  - Only gfx950 code objects are built.
  */
 #ifndef RTX_H_
@@ -520,10 +521,14 @@ int rtx_get_render_ray_count(rtx_context ctx, uint64_t* out_rays);
 /* Form of the integrator the last rtx_cmd_render / _batch / _nrc call of the context enqueued (diagnostic; both forms
  * give bit-identical films): WHOLE_PATH = each sample's paths traced, shaded and continued in one traversal launch;
  * WAVEFRONT = per-vertex launches (shade, fused shadow + extension trace, gather), used for TLASes whose content
- * needs opacity or intersection programs, or when the environment variable RTX_PATH_WAVEFRONT=1 asks for it. */
+ * needs opacity or intersection programs, or when the environment variable RTX_PATH_WAVEFRONT=1 asks for it;
+ * DEVICE_CHOSEN = a plain or batched render recorded by a stream capture before the TLAS's content could be read
+ * back (e.g. a TLAS built in the same recording): both forms are recorded, each guarded by the content bits the
+ * device reads from the TLAS header at replay, so that one of them runs (NRC frames take the wavefront there). */
 #define RTX_RENDER_FORM_NONE 0u
 #define RTX_RENDER_FORM_WHOLE_PATH 1u
 #define RTX_RENDER_FORM_WAVEFRONT 2u
+#define RTX_RENDER_FORM_DEVICE_CHOSEN 3u
 int rtx_get_render_form(rtx_context ctx, uint32_t* out_form);
 
 /* ---- neural radiance cache (Muller et al. 2021) --------------------------------------------------------------

@@ -56,8 +56,8 @@ def _device_scene(count=5, seed=3):
 
 def test_recorded_batched_build_tlas_and_render(rec):
     """One graph holds the batched BLAS build, the TLAS build and the batched render.  Its replays equal the direct
-    calls, also after the vertex data changed (a replay rebuilds from the data as it is then), and the TLAS built
-    inside the recording is traced with the general traversal and the wavefront integrator."""
+    calls, also after the vertex data changed (a replay rebuilds from the data as it is then); the render of the
+    TLAS built inside the recording is recorded in both integrator forms and the device picks one."""
     import pyrtx
     c, s = rec
     _say("recorded batched build + TLAS build + batched render")
@@ -78,7 +78,7 @@ def test_recorded_batched_build_tlas_and_render(rec):
         c.build_blas_batch(lists, into=blases, scratch=bscratch)
         c.build_tlas(inst, into=tlas, scratch=tscratch)
         c.render_batch(tlas, records, views, lt, spp=4, max_depth=6, films=films)
-    assert c.render_form() == pyrtx.RTX_RENDER_FORM_WAVEFRONT
+    assert c.render_form() == pyrtx.RTX_RENDER_FORM_DEVICE_CHOSEN
     for rep in range(3):
         films.fill_(-1.0)
         g.replay()
@@ -248,3 +248,59 @@ def test_recorded_nrc_frames_train_on(rec):
     finally:
         direct.close()
         graph.close()
+
+
+def test_device_picks_the_traversal_of_a_tlas_built_in_the_recording(rec):
+    """A TLAS rebuilt inside the recording from an instance buffer that the caller switches between replays -
+    opaque BLASes, then a cutout (any-hit) BLAS, then opaque again: each replay's render and closest-hit queries
+    equal the direct calls on that content, so the guarded launches follow the content the device reads."""
+    import pyrtx
+    from pyrtx import scenes as S
+    c, s = rec
+    _say("device-chosen traversal for a TLAS built in the recording")
+    W, H = 48, 32
+    opaque, lights = S.cornell_scene(3, width=W, height=H)
+    cut, _ = S.cornell_scene(3, width=W, height=H)
+    cut.blas[1][0].opaque = False
+    cut.records["any_hit"][4] = S.ANY_HIT_CUTOUT | (8 << 8)
+    rays = torch.from_numpy(S.incoherent_rays(opaque, 1 << 14, seed=5)).cuda()
+    with torch.cuda.stream(s):
+        recs = _dev(cut.records)  # the cutout program only runs on non-opaque geometry
+        lt = torch.from_numpy(np.ascontiguousarray(lights, np.float32).reshape(-1, 12)).cuda()
+        b_op = [c.build_blas(_dev_geoms(g)) for g in opaque.blas]
+        b_cut = [c.build_blas(_dev_geoms(g)) for g in cut.blas]
+        inst = {k: _dev(pyrtx.instance_records(sc, [b.address for b in bl]))
+                for k, sc, bl in (("opaque", opaque, b_op), ("cutout", cut, b_cut))}
+        ref = {}
+        for k in ("opaque", "cutout"):
+            t = c.build_tlas(inst[k])
+            c.sync()
+            film = c.render(t, recs, opaque.camera, lt, W, H, spp=2).clone()
+            hits = c.trace_ray_buffer(t, rays, pyrtx.RTX_QUERY_CLOSEST_HIT, 0xFF).clone()
+            c.sync()
+            ref[k] = (film, hits, c.render_form())
+        assert ref["opaque"][2] == pyrtx.RTX_RENDER_FORM_WHOLE_PATH
+        assert ref["cutout"][2] == pyrtx.RTX_RENDER_FORM_WAVEFRONT
+        assert not torch.equal(ref["opaque"][0], ref["cutout"][0])
+        buf = inst["opaque"].clone()
+        tlas = c.build_tlas(buf)
+        film = torch.empty_like(ref["opaque"][0])
+        hits = torch.empty_like(ref["opaque"][1])
+        c.sync()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
+        c.build_tlas(buf, into=tlas, scratch=tlas.keepalive[0])
+        c.render(tlas, recs, opaque.camera, lt, W, H, spp=2, film=film)
+        c.trace_ray_buffer(tlas, rays, pyrtx.RTX_QUERY_CLOSEST_HIT, 0xFF, hits)
+    assert c.render_form() == pyrtx.RTX_RENDER_FORM_DEVICE_CHOSEN
+    for k in ("opaque", "cutout", "opaque", "cutout"):
+        buf.copy_(inst[k])
+        film.fill_(-1.0)
+        hits.fill_(7)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        c.sync()
+        assert torch.equal(film, ref[k][0]), k
+        assert torch.equal(hits, ref[k][1]), k
+    del g
