@@ -1,7 +1,8 @@
-// path_tracer.cpp — the north star's path-tracing configurations on librtx from C++: an OBJ scene with one
-// Lambertian material per OBJ group, rendered by the plain path tracer (rtx_cmd_render) and by the neural radiance
-// cache renderer (rtx_cmd_render_nrc), the same flow the reference's render loop (main.cpp:1322-1423) drives
-// with vkCmdTraceRaysKHR.
+// path_tracer.cpp — path tracing on librtx from C++: an OBJ scene with one Lambertian material per OBJ group,
+// rendered by the plain path tracer (rtx_cmd_render), by the same frame recorded into a HIP graph (BLAS build, TLAS
+// build and render captured from the context's stream and replayed) and by the neural radiance cache renderer
+// (rtx_cmd_render_nrc), the same flow the reference's render loop (main.cpp:1322-1423) drives with
+// vkCmdTraceRaysKHR and its recorded command buffers.
 //
 // Usage: path_tracer scene.obj materials.txt out_prefix [width height spp nrc_frames]
 //   materials.txt: "camera x y z fov", then one line per OBJ group, "r g b er eg eb": albedo and emitted radiance.
@@ -16,6 +17,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -106,8 +108,14 @@ int main(int argc, char** argv) {
         return 2;
     }
 
+    // a stream of the program's own (default priority): a graph cannot be captured from the null stream
+    hipStream_t stream;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+        std::fprintf(stderr, "no GPU\n");
+        return 1;
+    }
     rtx_context ctx;
-    if (rtx_create_context(0, nullptr, &ctx) != RTX_SUCCESS) {
+    if (rtx_create_context(0, stream, &ctx) != RTX_SUCCESS) {
         std::fprintf(stderr, "no GPU\n");
         return 1;
     }
@@ -182,7 +190,6 @@ int main(int argc, char** argv) {
                                       blas_mem, bs.acceleration_structure_size, blas_scratch, bs.build_scratch_size,
                                       &blas_address));
     CHECK_RTX(ctx, rtx_sync(ctx));
-    CHECK_HIP(hipFree(blas_scratch));
 
     // ---- TLAS: one identity instance (createTLAS, main.cpp:833-949) ----
     rtx_instance inst{};
@@ -199,8 +206,6 @@ int main(int argc, char** argv) {
                                       ts.acceleration_structure_size, tlas_scratch, ts.build_scratch_size,
                                       &tlas_address));
     CHECK_RTX(ctx, rtx_sync(ctx));
-    CHECK_HIP(hipFree(tlas_scratch));
-    CHECK_HIP(hipFree(d_inst));
 
     // ---- hit records (black environment), lights, films ----
     rtx_hit_record* d_records = upload(records.data(), records.size());
@@ -239,6 +244,42 @@ int main(int argc, char** argv) {
     std::printf("plain: %u spp at %ux%u in %.2f ms (%u triangles, %zu lights)\n", SPP, W, H, ms, mesh.triangle_count,
                 lights.size() / 12);
     write_film(prefix + "_plain", resolve(d_accum, W, H, SPP), W, H);
+    std::vector<float> film_direct((size_t)W * H * 4), film_recorded((size_t)W * H * 4);
+    CHECK_HIP(hipMemcpy(film_direct.data(), d_film, film_direct.size() * 4, hipMemcpyDeviceToHost));
+
+    // ---- the same frame recorded: BLAS build, TLAS build and render captured from the context's stream into a
+    //      HIP graph (the command-buffer form of main.cpp:789-812, 930-942, 1340-1356) and replayed; every replay
+    //      rebuilds both structures from the device data, and the render of a TLAS built in the same recording is
+    //      recorded in both integrator forms, of which the device runs the one its content allows ----
+    float* d_film_rec = (float*)device_alloc((uint64_t)W * H * 16);
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    CHECK_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    CHECK_RTX(ctx, rtx_cmd_build_blas(ctx, geoms.data(), mesh.geometry_count, RTX_BUILD_PREFER_FAST_TRACE_BIT,
+                                      blas_mem, bs.acceleration_structure_size, blas_scratch, bs.build_scratch_size,
+                                      &blas_address));
+    CHECK_RTX(ctx, rtx_cmd_build_tlas(ctx, d_inst, 1, RTX_BUILD_PREFER_FAST_TRACE_BIT, tlas_mem,
+                                      ts.acceleration_structure_size, tlas_scratch, ts.build_scratch_size,
+                                      &tlas_address));
+    CHECK_RTX(ctx, rtx_cmd_render(ctx, tlas_address, &sbt, &camera, &p, d_film_rec, nullptr));
+    CHECK_HIP(hipStreamEndCapture(stream, &graph));
+    CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    CHECK_HIP(hipGraphLaunch(exec, stream));  // first replay (uploads the graph)
+    CHECK_RTX(ctx, rtx_sync(ctx));
+    const uint32_t kReplays = 3;
+    CHECK_HIP(hipEventRecord(e0, stream));
+    for (uint32_t k = 0; k < kReplays; ++k) CHECK_HIP(hipGraphLaunch(exec, stream));
+    CHECK_HIP(hipEventRecord(e1, stream));
+    CHECK_RTX(ctx, rtx_sync(ctx));
+    CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    CHECK_HIP(hipMemcpy(film_recorded.data(), d_film_rec, film_recorded.size() * 4, hipMemcpyDeviceToHost));
+    const bool same = std::memcmp(film_direct.data(), film_recorded.data(), film_direct.size() * 4) == 0;
+    std::printf("recorded: BLAS + TLAS build + %u-spp render replayed in %.2f ms per frame; film equals the direct "
+                "call's: %s\n", SPP, ms / (float)kReplays, same ? "yes" : "no");
+    CHECK_HIP(hipGraphExecDestroy(exec));
+    CHECK_HIP(hipGraphDestroy(graph));
+    CHECK_HIP(hipFree(d_film_rec));
+    if (!same) return 1;
 
     // ---- neural radiance cache: one sample per frame, 65,536 training paths per frame ----
     rtx_nrc_config c{};
@@ -275,10 +316,14 @@ int main(int argc, char** argv) {
     CHECK_HIP(hipFree(d_lights));
     CHECK_HIP(hipFree(d_records));
     CHECK_HIP(hipFree(tlas_mem));
+    CHECK_HIP(hipFree(tlas_scratch));
+    CHECK_HIP(hipFree(d_inst));
     CHECK_HIP(hipFree(blas_mem));
+    CHECK_HIP(hipFree(blas_scratch));
     CHECK_HIP(hipFree(d_vertices));
     CHECK_HIP(hipFree(d_indices));
     rtx_obj_free(&mesh);
     CHECK_RTX(ctx, rtx_destroy_context(ctx));
+    CHECK_HIP(hipStreamDestroy(stream));
     return 0;
 }

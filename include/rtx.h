@@ -448,7 +448,7 @@ int rtx_autotune_trace(rtx_context ctx, uint64_t tlas_address, const rtx_ray* ra
 int rtx_cmd_film_to_rgba8(rtx_context ctx, const float* film_rgba32f_device, uint32_t width, uint32_t height,
                           uint32_t flags, uint8_t* out_rgba8_device);
 
-/* ---- path tracing: wavefront integrator (north-star extension of the raygen / closest-hit flow) ---------------
+/* ---- path tracing: wavefront integrator (an extension of the raygen / closest-hit flow) ----------------------
  *
  * Camera rays follow the reference raygen model (main.cpp:1033-1052) with the pixel position jittered inside the
  * pixel.  Surfaces are Lambertian (hit records with RTX_PROGRAM_LAMBERTIAN: albedo = color, emitted radiance =
@@ -479,7 +479,7 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
                    const rtx_render_params* params, float* film_rgba32f_device, double* accum_rgb_device);
 
 /* Rays traced by rtx_cmd_render since the context was created (camera, extension and shadow rays). */
-/* ---- batched path tracing (configuration 4 of the north star; no counterpart in the reference) ----------------
+/* ---- batched path tracing (many uneven scenes in one wavefront; no counterpart in the reference) --------------
  *
  * Many views of one TLAS rendered by one wavefront: every stage (camera rays, extension traversal, shading, shadow
  * traversal, gathering) is one launch over the paths of all views, whatever their resolutions.  Scenes of

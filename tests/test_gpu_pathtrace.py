@@ -202,7 +202,8 @@ def _export_obj(sc, obj_path, mat_path):
 def test_cpp_path_tracer_host_program(ctx, tmp_path):
     """host/path_tracer.cpp, the path tracer and the radiance-cache renderer driven from C++ through the C ABI, on
     the configuration 2 scene exported as OBJ: its plain and NRC films equal, bit for bit, what the same calls give
-    through the Python binding, and its plain film matches the FP64 oracle's estimator."""
+    through the Python binding, its frame recorded into a HIP graph by a stream capture (BLAS build, TLAS build,
+    render) replays to the direct film, and its plain film matches the FP64 oracle's estimator."""
     import os
     import subprocess
     import pyrtx
@@ -217,6 +218,8 @@ def test_cpp_path_tracer_host_program(ctx, tmp_path):
     r = subprocess.run([exe, str(obj), str(mat), str(tmp_path / "out"), str(W), str(H), str(spp), str(frames)],
                        check=True, timeout=300, capture_output=True, text=True)
     _say("  " + r.stdout.strip().replace("\n", "; "))
+    # the frame recorded into a HIP graph (BLAS + TLAS build + render) replays to the direct call's film
+    assert "film equals the direct call's: yes" in r.stdout
     plain_c = np.fromfile(tmp_path / "out_plain.f32", np.float32).reshape(H, W, 4)
     nrc_c = np.fromfile(tmp_path / "out_nrc.f32", np.float32).reshape(H, W, 4)
     # the same calls through the Python binding

@@ -1,7 +1,7 @@
 """CPU pins for the oracle's neural-radiance-cache MLP, its Adam update and pcg4d.
 
 None of these three has a counterpart in the reference (its arithmetic lives in the Vulkan driver and it has no
-tests); they come from the north-star specification: a frequency-encoded MLP (position 3 x 12 frequencies x
+tests); they come from the radiance-cache specification this library implements: a frequency-encoded MLP (position 3 x 12 frequencies x
 sin / cos, normal, direction, albedo; 5 hidden layers of 64, ReLU, RGB output; BF16 weights and activations with
 FP32 accumulation, FP32 biases; relative L2 loss), Adam (Kingma & Ba 2015) and pcg4d (Jarzynski & Olano 2020,
 JCGT 9(3)).  Each function is checked against something it shares no code with:
