@@ -724,7 +724,7 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
 
 
 def batch_bench(ctx, args, np, torch, pyrtx, S):
-    """Configuration 4: uneven scenes built by one batched BLAS build and rendered by one wavefront replayed from a
+    """The batched-scenes workload: uneven scenes built by one batched BLAS build and rendered by one wavefront replayed from a
     HIP graph, against the same scenes built and rendered one at a time."""
     n, spp, depth = args.batch_scenes, 16, 6
     t0 = time.time()

@@ -11,7 +11,7 @@ namespace rtx {
 
 enum PtMode : uint32_t { kPtPlain = 0, kPtCache = 2 };
 
-// One view of a batched render (configuration 4): its camera, film, light range and the box its rays are clipped
+// One view of a batched render (many uneven scenes in one wavefront): its camera, film, light range and the box its rays are clipped
 // to (the scenes of a batch sit in disjoint boxes of one TLAS).
 struct BatchView {
     float cam[3];

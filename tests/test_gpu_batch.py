@@ -101,7 +101,7 @@ def _relmse(x, ref):
 
 
 def test_batched_views_match_fp64_oracle(ctx):
-    """Configuration 4's scene kind at oracle-affordable films: 8 uneven scenes (seed 1: 2,991 to 1,372,489
+    """The batched-scenes workload's scene kind at oracle-affordable films: 8 uneven scenes (seed 1: 2,991 to 1,372,489
     sphere triangles) in one batched build and one batched render (films 256^2 / 512^2 / 1024x768 scaled by 1/8),
     each view against the FP64 oracle's estimator on its scene alone with the same camera, lights and random
     numbers (the clip box keeps a batched view's rays inside its scene)."""
@@ -145,12 +145,12 @@ def test_batched_views_match_fp64_oracle(ctx):
 
 
 def test_full_64_scene_batch(ctx):
-    """The bench's configuration 4 batch itself (64 scenes, seed 7: 15.5M triangles, 23.3M pixels) for one sample:
+    """The bench's 64-scene batch itself (64 scenes, seed 7: 15.5M triangles, 23.3M pixels) for one sample:
     every view finite, non-negative and lit, and the views of the largest and the smallest scene equal to rendering
     those scenes alone."""
     import pyrtx
     from pyrtx import scenes as S
-    _say("64-scene batch (configuration 4), one sample")
+    _say("64-scene batch, one sample")
     sc, views, lights = S.batch_scenes(count=64, seed=7)
     blases, tlas = ctx.upload_scene(sc, batched=True)
     films = ctx.render_batch(tlas, sc.records, views, lights, spp=1, max_depth=6)

@@ -330,15 +330,15 @@ def test_cache_outliving_its_context(ctx):
     n2.close()
 
 
-def test_render_nrc_configuration5_at_1080p(ctx):
-    """Configuration 5 at its own size: the displaced level-8 icosphere Cornell box at 1920x1080, 65,536 training
+def test_render_nrc_at_1080p_with_65536_training_paths(ctx):
+    """The radiance-cache workload at its own size: the displaced level-8 icosphere Cornell box at 1920x1080, 65,536 training
     paths per frame, batches of 16,384, paths ending in the cache at the second bounce.  Over 32 frames the
     accumulated image approaches a 256-spp plain reference (error after 32 frames below half that after 8), beats
     plain path tracing at the same samples per pixel, and its mean radiance stays within 5% of the reference's
     (the cache is biased while it trains)."""
     import pyrtx
     from pyrtx import scenes as S
-    _say("render_nrc, configuration 5 at 1920x1080")
+    _say("render_nrc, the radiance-cache workload at 1920x1080")
     W, H, frames = 1920, 1080, 32
     sc, lights = S.cornell_scene(8, width=W, height=H)
     blases, tlas = ctx.upload_scene(sc)

@@ -70,7 +70,7 @@ def _check_vs_oracle(g, o, max_relmse, min_close):
 # measured: level 4 relMSE 2.8e-10, all pixels within 1e-3; level 8 (1.31M triangles) 4.6e-9, 1 pixel of 5,184 off
 @pytest.mark.parametrize("level,max_relmse,min_close", [(4, 3e-9, 0.9999), (8, 5e-8, 0.999)])
 def test_cornell_matches_fp64_oracle(ctx, level, max_relmse, min_close):
-    """Configuration 2's scene (level 8: the 1.31M-triangle displaced sphere) at a film the oracle renders in
+    """The 1080p path-tracing scene (level 8: the 1.31M-triangle displaced sphere) at a film the oracle renders in
     seconds."""
     _say(f"cornell (sphere level {level}) vs oracle")
     W, H, spp = 96, 54, 16
@@ -201,7 +201,7 @@ def _export_obj(sc, obj_path, mat_path):
 
 def test_cpp_path_tracer_host_program(ctx, tmp_path):
     """host/path_tracer.cpp, the path tracer and the radiance-cache renderer driven from C++ through the C ABI, on
-    the configuration 2 scene exported as OBJ: its plain and NRC films equal, bit for bit, what the same calls give
+    the path-tracing scene (Cornell box + displaced sphere) exported as OBJ: its plain and NRC films equal, bit for bit, what the same calls give
     through the Python binding, its frame recorded into a HIP graph by a stream capture (BLAS build, TLAS build,
     render) replays to the direct film, and its plain film matches the FP64 oracle's estimator."""
     import os
