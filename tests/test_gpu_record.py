@@ -77,7 +77,8 @@ def test_recorded_batched_build_tlas_and_render(rec):
     with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
         c.build_blas_batch(lists, into=blases, scratch=bscratch)
         c.build_tlas(inst, into=tlas, scratch=tscratch)
-        c.render_batch(tlas, records, views, lt, spp=4, max_depth=6, films=films)
+        # RTX_RENDER_GRAPH_BIT inside a caller's capture: recorded as a direct call
+        c.render_batch(tlas, records, views, lt, spp=4, max_depth=6, films=films, graph=True)
     assert c.render_form() == pyrtx.RTX_RENDER_FORM_DEVICE_CHOSEN
     for rep in range(3):
         films.fill_(-1.0)
@@ -110,8 +111,9 @@ def test_recorded_batched_build_tlas_and_render(rec):
 
 
 def test_recorded_traces_on_a_prebuilt_tlas(rec):
-    """Ray-buffer queries and the reference raygen recorded on a TLAS built before the capture replay to the
-    direct calls' hits and film."""
+    """Ray-buffer queries and the reference raygen recorded on a TLAS built before the capture, a compacting copy of
+    that TLAS traced in the same recording, and the film's conversion to display bytes replay to the direct calls'
+    hits, film and bytes."""
     import pyrtx
     from pyrtx import scenes as S
     c, s = rec
@@ -127,22 +129,39 @@ def test_recorded_traces_on_a_prebuilt_tlas(rec):
         c.trace_ray_buffer(tlas, rays, pyrtx.RTX_QUERY_ANY_HIT, 0xFF, d_occ)
         d_film = c.trace_rays(tlas, recs, sc.camera, 96, 64).clone()
         c.sync()
+        d_rgba = c.film_to_rgba8(d_film).clone()
+        size = c.compacted_size(tlas)  # a query: made before the capture
+        dst = torch.empty(size, dtype=torch.uint8, device="cuda")
+        c.sync()
         hits = torch.empty_like(d_hits)
+        hits2 = torch.empty_like(d_hits)
         occ = torch.empty_like(d_occ)
         film = torch.empty_like(d_film)
+        rgba = torch.empty_like(d_rgba)
+    copy_t = pyrtx.AccelerationStructure(dst, dst.data_ptr(), tlas.count, (tlas,))
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g, stream=s, capture_error_mode="thread_local"):
         c.trace_ray_buffer(tlas, rays, pyrtx.RTX_QUERY_CLOSEST_HIT, 0xFF, hits)
         c.trace_ray_buffer(tlas, rays, pyrtx.RTX_QUERY_ANY_HIT, 0xFF, occ)
         c.trace_rays(tlas, recs, sc.camera, 96, 64, film=film)
+        c._check(c.lib.rtx_cmd_copy_as(c.handle, tlas.address, pyrtx.RTX_COPY_MODE_COMPACT, dst.data_ptr(), size,
+                                       None))
+        c.trace_ray_buffer(copy_t, rays, pyrtx.RTX_QUERY_CLOSEST_HIT, 0xFF, hits2)
+        c._check(c.lib.rtx_cmd_film_to_rgba8(c.handle, film.data_ptr(), 96, 64, pyrtx.RTX_FILM_REFERENCE_BYTES,
+                                             rgba.data_ptr()))
     for rep in range(3):
         hits.fill_(7)
+        hits2.fill_(7)
         occ.fill_(7)
         film.fill_(-1.0)
+        rgba.fill_(3)
+        dst.fill_(0)
+        torch.cuda.synchronize()
         g.replay()
         torch.cuda.synchronize()
         c.sync()
         assert torch.equal(hits, d_hits) and torch.equal(occ, d_occ) and torch.equal(film, d_film), rep
+        assert torch.equal(hits2, d_hits) and torch.equal(rgba, d_rgba), rep
     assert int((d_hits[:, 5] != -1).sum()) > 100
     del g
 
