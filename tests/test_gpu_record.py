@@ -111,9 +111,9 @@ def test_recorded_batched_build_tlas_and_render(rec):
 
 
 def test_recorded_traces_on_a_prebuilt_tlas(rec):
-    """Ray-buffer queries and the reference raygen recorded on a TLAS built before the capture, a compacting copy of
-    that TLAS traced in the same recording, and the film's conversion to display bytes replay to the direct calls'
-    hits, film and bytes."""
+    """Ray-buffer queries (also through the allocation-free entry point) and the reference raygen recorded on a TLAS
+    built before the capture, a compacting copy of that TLAS traced in the same recording, and the film's
+    conversion to display bytes replay to the direct calls' hits, film and bytes."""
     import pyrtx
     from pyrtx import scenes as S
     c, s = rec
@@ -135,6 +135,7 @@ def test_recorded_traces_on_a_prebuilt_tlas(rec):
         c.sync()
         hits = torch.empty_like(d_hits)
         hits2 = torch.empty_like(d_hits)
+        hits3 = torch.empty_like(d_hits)
         occ = torch.empty_like(d_occ)
         film = torch.empty_like(d_film)
         rgba = torch.empty_like(d_rgba)
@@ -147,11 +148,14 @@ def test_recorded_traces_on_a_prebuilt_tlas(rec):
         c._check(c.lib.rtx_cmd_copy_as(c.handle, tlas.address, pyrtx.RTX_COPY_MODE_COMPACT, dst.data_ptr(), size,
                                        None))
         c.trace_ray_buffer(copy_t, rays, pyrtx.RTX_QUERY_CLOSEST_HIT, 0xFF, hits2)
+        c._check(c.lib.rtx_cmd_trace_ray_buffer_noalloc(c.handle, tlas.address, rays.data_ptr(), rays.shape[0],
+                                                        pyrtx.RTX_QUERY_CLOSEST_HIT, 0, 0xFF, hits3.data_ptr(), None))
         c._check(c.lib.rtx_cmd_film_to_rgba8(c.handle, film.data_ptr(), 96, 64, pyrtx.RTX_FILM_REFERENCE_BYTES,
                                              rgba.data_ptr()))
     for rep in range(3):
         hits.fill_(7)
         hits2.fill_(7)
+        hits3.fill_(7)
         occ.fill_(7)
         film.fill_(-1.0)
         rgba.fill_(3)
@@ -161,7 +165,7 @@ def test_recorded_traces_on_a_prebuilt_tlas(rec):
         torch.cuda.synchronize()
         c.sync()
         assert torch.equal(hits, d_hits) and torch.equal(occ, d_occ) and torch.equal(film, d_film), rep
-        assert torch.equal(hits2, d_hits) and torch.equal(rgba, d_rgba), rep
+        assert torch.equal(hits2, d_hits) and torch.equal(hits3, d_hits) and torch.equal(rgba, d_rgba), rep
     assert int((d_hits[:, 5] != -1).sum()) > 100
     del g
 
