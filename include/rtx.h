@@ -386,7 +386,8 @@ int rtx_cmd_trace_ray_buffer(rtx_context ctx, uint64_t tlas_address, const rtx_r
  * 16 BVH levels; a launch needing more returns RTX_ERROR_OUT_OF_DEVICE_MEMORY (with the byte count in
  * rtx_last_error_string).  Every call on one GPU shares that array: calls that overlap on one GPU (several streams
  * or contexts) race.  Results equal rtx_cmd_trace_ray_buffer's; traversal statistics are not collected; errors are
- * reported by rtx_sync. */
+ * reported by rtx_sync.  Recorded into a graph, it needs a TLAS whose depth the host read back before the capture:
+ * for a TLAS built in the same recording the stack of an unknown depth (76 entries) exceeds the array. */
 int rtx_cmd_trace_ray_buffer_noalloc(rtx_context ctx, uint64_t tlas_address, const rtx_ray* rays_device,
                                      uint32_t ray_count, uint32_t mode, uint32_t ray_flags, uint32_t cull_mask,
                                      rtx_hit* hits_device, uint32_t* occluded_device);
