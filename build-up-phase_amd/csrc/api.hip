@@ -1190,6 +1190,10 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L,
                          train_paths);
     if (r) return r;
+    if (a.dual) {  // NRC frames record one form: the wavefront where the TLAS content is not known
+        a.dual = 0;
+        ctx->render_form = RTX_RENDER_FORM_WAVEFRONT;
+    }
     if (params->max_depth <= nrc->st.cache_vertex)
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "max_depth must exceed the cache vertex");
     a.cache_depth = nrc->st.cache_vertex;
