@@ -40,6 +40,8 @@ struct rtx_context_t {
     bool used_noalloc = false;  // rtx_sync also reads the device's no-alloc error flags
     void* pt_scratch = nullptr;
     uint64_t pt_scratch_bytes = 0;
+    void* ms_buf = nullptr;  // plain renders: per-path state of several samples per launch
+    uint64_t ms_bytes = 0;
     std::vector<rtx_nrc_t*> caches;  // live caches; detached when the context is destroyed first
     // batched renders: uploaded views, the graph of the last recorded call and the word it reads sample_offset from
     void* views_buf = nullptr;
@@ -76,6 +78,9 @@ constexpr uint32_t kMaxPrims = (1u << 30) - 1u;
 // launch has kSideBlocks 256-thread blocks per CU: the blocks beyond the reserve start as pixel-path blocks retire
 // (the traversal hands out paths from an atomic counter, so late blocks only take what is left).
 constexpr uint32_t kSideReserve = 1u;
+// Samples per whole-path launch of a plain render (8: the tail of a launch's last, longest paths is paid once per
+// 8 samples; the per-path state takes 84 B per pixel and sample, 1.4 GB at 1920x1080).
+constexpr uint32_t kSampleViews = 8u;
 constexpr uint32_t kSideBlocks = 2u;
 
 // splitmix64 (Steele, Lea & Flood 2014): deterministic weight initialisation
@@ -454,6 +459,7 @@ int rtx_destroy_context(rtx_context ctx) {
     if (ctx->views_buf) (void)hipFree(ctx->views_buf);
     if (ctx->gstack) (void)hipFree(ctx->gstack);
     if (ctx->pt_scratch) (void)hipFree(ctx->pt_scratch);
+    if (ctx->ms_buf) (void)hipFree(ctx->ms_buf);
     for (void* p : ctx->retired) (void)hipFree(p);
     if (ctx->state) (void)hipFree(ctx->state);
     delete ctx;
@@ -1015,6 +1021,15 @@ int rtx_cmd_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, c
     rtx::TraceLaunch L;
     int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L);
     if (r) return r;
+    // whole-path renders of several samples run up to kSampleViews samples per launch (one launch tail per group)
+    const uint32_t S = std::min<uint32_t>(params->spp, kSampleViews);
+    if (S > 1 && rtx::pt_whole_paths(a) && !rtx::pt_dual_forms(a)) {
+        if ((r = grow_buffer(ctx, &ctx->ms_buf, &ctx->ms_bytes, rtx::pt_sample_views_bytes(a.npix, S),
+                             "multi-sample path state")))
+            return r;
+        a.ms_buf = ctx->ms_buf;
+        a.ms_max = S;
+    }
     hipError_t e = rtx::pt_render(a, L, params->spp, params->sample_offset, film_rgba32f_device, accum_rgb_device,
                                   ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "rtx_cmd_render");

@@ -11,8 +11,10 @@ namespace rtx {
 
 enum PtMode : uint32_t { kPtPlain = 0, kPtCache = 2 };
 
-// One view of a batched render (many uneven scenes in one wavefront): its camera, film, light range and the box its rays are clipped
-// to (the scenes of a batch sit in disjoint boxes of one TLAS).
+// One view of a batched render (many uneven scenes in one wavefront): its camera, film, light range, the box its
+// rays are clipped to (the scenes of a batch sit in disjoint boxes of one TLAS) and the offset added to the pass's
+// sample index for its paths (a plain render runs several samples in one launch as views of one camera, each with
+// its own offset and an unbounded box).
 struct BatchView {
     float cam[3];
     float ax, ay;
@@ -22,6 +24,7 @@ struct BatchView {
     uint32_t light_count;
     uint32_t cdf_first;    // its light CDF: cdf[cdf_first .. cdf_first + light_count]
     float lo[3], hi[3];
+    uint32_t sample_offset;
 };
 
 struct PtArgs {
@@ -72,12 +75,19 @@ struct PtArgs {
     // camera stage runs only for other content, so its later stages find empty queues), and the device runs one.
     uint32_t dual;
     uint32_t guard;
+    // Plain renders of several samples: each whole-path launch runs up to ms_max samples as views of the camera,
+    // their per-path state in ms_buf (pt_sample_views_bytes), so that a launch's tail of long paths is paid once per
+    // ms_max samples instead of once per sample.
+    void* ms_buf;
+    uint32_t ms_max;
 };
 
 // Whether the render's samples run as whole-path launches (trace modes 6 / 7, and 5 for full-length training paths):
 // for a TLAS whose content the default traversal variant serves, unless RTX_PATH_WAVEFRONT=1 asks for the per-vertex
 // wavefront (both give bit-identical films).
 bool pt_whole_paths(const PtArgs& a);
+// Bytes of PtArgs::ms_buf for a film of npix pixels and up to `samples` samples per launch.
+uint64_t pt_sample_views_bytes(uint32_t npix, uint32_t samples);
 // Whether a plain render records both forms (PtArgs::dual, unless RTX_PATH_WAVEFRONT=1 forces the wavefront).
 bool pt_dual_forms(const PtArgs& a);
 

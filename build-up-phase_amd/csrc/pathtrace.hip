@@ -188,6 +188,59 @@ __global__ __launch_bounds__(256) void accumulate_kernel(PtArgs a, double* __res
     }
 }
 
+// The views of a multi-sample plain launch: view k = the render's camera and film, pixel paths from k * npix, the
+// pass's sample index + k, every light, and an unbounded clip box (the exit distance is +inf, as for the plain
+// render's rays).
+__global__ void sample_views_kernel(PtArgs a, uint32_t nviews, BatchView* __restrict__ views) {
+    const uint32_t k = threadIdx.x;
+    if (blockIdx.x != 0 || k >= nviews) return;
+    BatchView v;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        v.cam[c] = a.cam[c];
+        v.lo[c] = -INFINITY;
+        v.hi[c] = INFINITY;
+    }
+    v.ax = a.ax;
+    v.ay = a.ay;
+    v.width = a.width;
+    v.height = a.height;
+    v.pix_base = k * a.npix;
+    v.light_first = 0;
+    v.light_count = a.nlights;
+    v.cdf_first = 0;
+    v.sample_offset = k;
+    views[k] = v;
+}
+
+// accumulate_kernel for `nviews` samples at once: pixel i adds the radiance of its path in each view, in view
+// (= sample) order, the same additions in the same order as one accumulate_kernel per sample.
+__global__ __launch_bounds__(256) void accumulate_samples_kernel(PtArgs a, const float4* __restrict__ rad,
+                                                                 uint32_t nviews, double* __restrict__ acc64) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.npix) return;
+    float4 s = a.accum[i];
+    double d[3] = {0.0, 0.0, 0.0};
+    if (acc64) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) d[c] = acc64[3 * (uint64_t)i + c];
+    }
+    for (uint32_t k = 0; k < nviews; ++k) {
+        const float4 r = rad[(uint64_t)k * a.npix + i];
+        s.x += r.x;
+        s.y += r.y;
+        s.z += r.z;
+        d[0] += (double)r.x;
+        d[1] += (double)r.y;
+        d[2] += (double)r.z;
+    }
+    a.accum[i] = s;
+    if (acc64) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc64[3 * (uint64_t)i + c] = d[c];
+    }
+}
+
 __global__ __launch_bounds__(256) void resolve_kernel(PtArgs a, float inv_spp, float* __restrict__ film) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.npix) return;
@@ -454,6 +507,14 @@ static hipError_t render_cache_sample(PtArgs& a, const TraceLaunch& L, uint32_t 
     return trace_rays_launch(tr, L, s);
 }
 
+// Per path slot of a multi-sample launch, every array the whole-path traversal indexes by path: throughput and
+// radiance, the pending continuation ray and shadow contribution (which wait in the path's slot), and its view.
+uint64_t pt_sample_views_bytes(uint32_t npix, uint32_t samples) {
+    const uint64_t n = (uint64_t)npix * samples;
+    return align_up(samples * sizeof(BatchView), 256) + 3 * align_up(n * 16, 256) + align_up(n * sizeof(rtx_ray), 256) +
+           align_up(n * 4, 256);
+}
+
 hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t sample_offset, float* film,
                      double* acc64, hipStream_t s) {
     const unsigned gb = (a.npix + 255) / 256;
@@ -463,6 +524,36 @@ hipError_t pt_render(PtArgs a, const TraceLaunch& L, uint32_t spp, uint32_t samp
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
     fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
     hipError_t e;
+    const uint32_t S = std::min(spp, a.ms_max);
+    if (a.ms_buf && !a.views && S > 1 && pt_whole_paths(a) && !pt_dual_forms(a)) {
+        // several samples per whole-path launch, as views of the camera; their per-path state in ms_buf
+        uint8_t* p = (uint8_t*)a.ms_buf;
+        BatchView* views = (BatchView*)p;
+        p += align_up(S * sizeof(BatchView), 256);
+        const uint64_t n = (uint64_t)a.npix * S;
+        PtArgs m = a;
+        m.thr = (float4*)p;
+        p += align_up(n * 16, 256);
+        m.rad = (float4*)p;
+        p += align_up(n * 16, 256);
+        m.sh_contrib = (float4*)p;
+        p += align_up(n * 16, 256);
+        m.ext_rays[0] = (rtx_ray*)p;
+        p += align_up(n * sizeof(rtx_ray), 256);
+        m.path_view = (uint32_t*)p;
+        m.views = views;
+        sample_views_kernel<<<1, 64, 0, s>>>(a, S, views);
+        for (uint32_t si = 0; si < spp; si += S) {
+            const uint32_t k = std::min(S, spp - si);
+            m.nviews = k;
+            m.npix = a.npix * k;
+            m.npaths = m.npix;
+            if ((e = render_sample(m, L, sample_offset + si, s))) return e;
+            accumulate_samples_kernel<<<gb, 256, 0, s>>>(a, m.rad, k, acc64);
+        }
+        if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, 1.0f / (float)spp, film);
+        return hipGetLastError();
+    }
     for (uint32_t si = 0; si < spp; ++si) {
         if ((e = render_sample(a, L, sample_offset + si, s))) return e;
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);

@@ -61,7 +61,7 @@ __device__ __forceinline__ void pt_camera_ray(const PtArgs& a, uint32_t i, uint3
         const uint32_t local = i - v.pix_base;
         px = local % v.width;
         py = local / v.width;
-        j = pcg4d(px, py, sample, a.seed ^ kJitterKey);
+        j = pcg4d(px, py, sample + v.sample_offset, a.seed ^ kJitterKey);
         const float sx = ((float)px + unit_float(j.x)) / (float)v.width;
         const float sy = ((float)py + unit_float(j.y)) / (float)v.height;
         const float nx = sx * 2.0f - 1.0f;
@@ -172,9 +172,10 @@ __device__ __forceinline__ void shade_vertex(const PtArgs& a, uint32_t pid, cons
         } else {
             uint32_t x, y;
             path_xy(a, pid, x, y);
-            const uint4 ra = pcg4d(x, y, sample, a.seed + 2u * depth);
-            const uint4 rb = pcg4d(x, y, sample, a.seed + 2u * depth + 1u);
             const BatchView* bv = a.views ? a.views + a.path_view[pid] : nullptr;
+            const uint32_t smp = bv ? sample + bv->sample_offset : sample;
+            const uint4 ra = pcg4d(x, y, smp, a.seed + 2u * depth);
+            const uint4 rb = pcg4d(x, y, smp, a.seed + 2u * depth + 1u);
             const uint32_t nl = bv ? bv->light_count : a.nlights;
             const float* cdf = bv ? a.cdf + bv->cdf_first : a.cdf;
             const float* lights = bv ? a.lights + 12 * (uint64_t)bv->light_first : a.lights;
