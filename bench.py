@@ -428,20 +428,24 @@ def cpu_baseline_rays(args, sc, inc, W, H, cores, np, S):
 
 def nrc_kernel_bench(ctx, torch, pyrtx):
     """The NRC network's kernels alone on random inputs: inference of one 1920x1080 frame of cache queries and
-    training on 65,536 samples (4 Adam steps of 16,384), three rounds; HIP-event times of the last round."""
+    training on 65,536 samples (4 Adam steps of 16,384).  A warm-up round, then 10 back-to-back calls of each between
+    two HIP events, so that the host's per-call enqueue overlaps the previous call's kernels; the time per call."""
     nrc = pyrtx.Nrc(ctx, (0, 0, 0), (1, 1, 1), seed=1, batch_size=16384)
     g = torch.Generator(device=ctx.device).manual_seed(1)
     q = torch.rand((2073600, 12), device=ctx.device, generator=g)
     smp = torch.rand((65536, 16), device=ctx.device, generator=g)
     out = {}
-    for _ in range(3):
-        for name, fn in (("infer_2073600_ms", lambda: nrc.infer(q)), ("train_4x16384_ms", lambda: nrc.train(smp))):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
+    reps = 10
+    for name, fn in (("infer_2073600_ms", lambda: nrc.infer(q)), ("train_4x16384_ms", lambda: nrc.train(smp))):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
             fn()
-            b.record()
-            torch.cuda.synchronize()
-            out[name] = round(a.elapsed_time(b), 4)
+        b.record()
+        torch.cuda.synchronize()
+        out[name] = round(a.elapsed_time(b) / reps, 4)
     nrc.close()
     return out
 
