@@ -452,7 +452,7 @@ def nrc_kernel_bench(ctx, torch, pyrtx):
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 NRC_SEEDS = (1, 2, 3, 4, 5)  # cache initialisations of the equal-relMSE comparison (median reported)
-NRC_WARMUP = 2               # first frames of an NRC run that are timed but not accumulated
+NRC_WARMUP = 2               # first frames of an NRC run: timed, training only (spp 0), nothing accumulated
 NRC_FLOP_PER_QUERY = 47104    # forward: 2 x sum(fan_in x fan_out) over the 6 layers
 NRC_FLOP_PER_SAMPLE = 141312  # forward + data gradient + weight gradient
 
@@ -585,8 +585,10 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
 
     tuned = None
     if not args.no_autotune:
-        tuned, tms = ctx.autotune_render(tlas, rec, sc.camera, lt, W, H, spp=1)
-        log(f"autotuned path tracing: {tuned} ({tms:.2f} ms per 1-spp render)")
+        # tuned on renders of 4 samples, which run as one whole-path launch of 4 views of the camera like the
+        # timed render's groups of 16 (1-sample renders favour shapes that are ~3% slower on those launches)
+        tuned, tms = ctx.autotune_render(tlas, rec, sc.camera, lt, W, H, spp=4)
+        log(f"autotuned path tracing: {tuned} ({tms:.2f} ms per 4-spp render)")
     ctx.render(tlas, rec, sc.camera, lt, W, H, spp=2, sample_offset=1 << 24)  # warm-up
     ctx.sync()
     r0 = ctx.render_ray_count()
@@ -666,10 +668,12 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
         nrc_cfg = best[1]
         log(f"autotuned NRC frame: {nrc_cfg} ({best[0]:.2f} ms per frame)")
     ctx.set_config(**nrc_cfg)
-    # Equal-relMSE time over several cache initialisations: the first NRC_WARMUP frames (untrained cache) are
-    # rendered, trained on and timed but not accumulated, then frames are averaged with equal weights.  Averaging
-    # the untrained frames too leaves a bias that depends on the initialisation (2.1x - 4.1x over 8 seeds, median
-    # 2.5x, against 3.4x - 3.8x with two warm-up frames).
+    # Equal-relMSE time over several cache initialisations: the first NRC_WARMUP frames (untrained cache) only
+    # train (spp 0: their training paths and Adam steps, exactly as in a rendered frame, no pixel paths) and are
+    # timed, then rendered frames are averaged with equal weights.  Averaging frames of the untrained cache leaves a
+    # bias that depends on the initialisation (2.1x - 4.1x over 8 seeds, median 2.5x, against 3.4x - 3.8x with two
+    # warm-up frames, at the 324-ms plain render of that time); a warm-up frame that renders costs 2.9 ms instead of
+    # 1.8 ms and its image would be discarded.
     accn = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
     film_nrc = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
     per_seed = []
@@ -684,8 +688,8 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
         for fr in range(frames):
             acc_fr = accn if fr >= NRC_WARMUP else None
             evs[fr][0].record()
-            ctx.render_nrc(tlas, rec, sc.camera, lt, nrc, W, H, spp=1, train_paths=65536, sample_offset=fr,
-                           film=film_nrc, accum64=acc_fr)
+            ctx.render_nrc(tlas, rec, sc.camera, lt, nrc, W, H, spp=1 if acc_fr is not None else 0,
+                           train_paths=65536, sample_offset=fr, film=film_nrc, accum64=acc_fr)
             evs[fr][1].record()
             if acc_fr is not None:
                 errs[fr] = relmse((accn / (fr + 1 - NRC_WARMUP)).float(), ref)
@@ -699,12 +703,14 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
         ms_nrc = float(cum[f]) if f is not None else None
         steps, loss = nrc.status()
         nrc.close()
-        per_seed.append({"seed": seed, "frames": frames, "frame_ms": round(float(fms.mean()), 3),
+        per_seed.append({"seed": seed, "frames": frames, "frame_ms": round(float(fms[NRC_WARMUP:].mean()), 3),
+                         "warmup_frame_ms": round(float(fms[:NRC_WARMUP].mean()), 3),
                          "frames_to_plain_relmse": None if f is None else f + 1,
                          "ms_to_plain_relmse": None if ms_nrc is None else round(ms_nrc, 2),
                          "speedup": None if ms_nrc is None else round(ms_plain / ms_nrc, 2),
                          "relmse_final": round(float(e[-1]), 6), "adam_steps": int(steps), "wall_s": round(wall_nrc, 2)})
-        log(f"NRC seed {seed}: frame {fms.mean():.2f} ms, relMSE after {frames} frames {e[-1]:.5f}; reaches plain "
+        log(f"NRC seed {seed}: frame {fms[NRC_WARMUP:].mean():.2f} ms (training-only warm-up frames "
+            f"{fms[:NRC_WARMUP].mean():.2f} ms), relMSE after {frames} frames {e[-1]:.5f}; reaches plain "
             f"relMSE at frame {None if f is None else f + 1} ({ms_nrc} ms)")
     sp = np.array([r["speedup"] or 0.0 for r in per_seed])
     first = per_seed[0]
@@ -716,7 +722,8 @@ def render_bench(ctx, args, np, torch, pyrtx, S):
         "reference_spp": 64 * chunks,
         "nrc": {"frames": args.nrc_frames, "train_paths_per_frame": 65536, "batch_size": 16384,
                 "cache_vertex": 1, "frame_ms": first["frame_ms"], "autotuned_trace_config": nrc_cfg,
-                "warmup_frames_not_accumulated": NRC_WARMUP,
+                "warmup_frames_not_accumulated": NRC_WARMUP, "warmup_frames_train_only": True,
+                "warmup_frame_ms": first["warmup_frame_ms"],
                 "relmse_final": first["relmse_final"], "adam_steps": first["adam_steps"],
                 "frames_to_plain_relmse": first["frames_to_plain_relmse"],
                 "ms_to_plain_relmse": first["ms_to_plain_relmse"], "per_seed": per_seed},

@@ -1205,8 +1205,11 @@ int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sb
     rtx::TraceLaunch L;
     if (params && (uint64_t)params->width * params->height + train_paths > (1ull << 28))
         return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "pixels + train_paths above 2^28");
-    int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device, a, L,
-                         train_paths);
+    const bool train_only = params && params->spp == 0;
+    if (train_only && train_paths == 0)
+        return fail(ctx, RTX_ERROR_INVALID_ARGUMENT, "a frame needs spp > 0 or train_paths > 0");
+    int r = render_setup(ctx, tlas_address, sbt, camera, params, film_rgba32f_device || accum_rgb_device || train_only,
+                         a, L, train_paths);
     if (r) return r;
     if (a.dual) {  // NRC frames record one form: the wavefront where the TLAS content is not known
         a.dual = 0;

@@ -585,15 +585,18 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
     const unsigned gb = (a.npix + 255) / 256;
     hipError_t e;
     a.dual = 0;  // NRC frames take one form (the wavefront where the content is not known)
+    // spp == 0: a frame that only trains (its training paths and Adam steps, exactly as in a rendered frame; no
+    // pixel paths, film and accumulator untouched)
+    const bool train_only = spp == 0;
     if (a.nlights) light_cdf_kernel<<<1, 64, 0, s>>>(a.lights, a.nlights, a.cdf);
-    fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
+    if (!train_only) fill_words_kernel<<<gb, 256, 0, s>>>((uint32_t*)a.accum, 4ull * a.npix, 0u);
     a.mode = kPtCache;
     a.tail = train_suffix > 0 ? 1u : 0u;
     a.train_end = a.tail ? std::min(a.max_depth - 1, a.cache_depth + train_suffix) : a.max_depth - 1;
     a.first_path = 0;
     a.npaths = a.npix;
     const uint32_t rec_depths = nrc_record_depths(a.max_depth, a.cache_depth, train_suffix);
-    for (uint32_t si = 0; si < spp; ++si) {
+    for (uint32_t si = 0; si < (train_only ? 1u : spp); ++si) {
         // the first sample carries the frame's training paths
         const uint32_t ntrain = si == 0 ? train_paths : 0u;
         a.ntrain = ntrain;
@@ -637,7 +640,7 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         if ((e = hipStreamWaitEvent(side.stream, side.fork, 0))) return e;
         // render paths up to their cache vertex (a few long launches, enqueued first so that the GPU has them
         // while the host enqueues the training wavefront's many short ones)
-        if ((e = render_cache_sample(a, L, sample_offset + si, s))) return e;
+        if (!train_only && (e = render_cache_sample(a, L, sample_offset + si, s))) return e;
         fill_words_kernel<<<1024, 256, 0, side.stream>>>((uint32_t*)a.rec, nrec * 20, 0u);
         if (!t.tail && pt_whole_paths(t)) {
             // the whole training wavefront in one traversal launch (each lane shades and continues its own path)
@@ -674,10 +677,11 @@ hipError_t pt_render_nrc(PtArgs a, const TraceLaunch& L, const SideChain& side, 
         merge_state_kernel<<<1, 64, 0, side.stream>>>(a.state, side.state);
         if ((e = hipEventRecord(side.join, side.stream))) return e;
         if ((e = hipStreamWaitEvent(s, side.join, 0))) return e;
+        if (train_only) break;
         if ((e = infer_gather(a, nrc, a.counts + 2, 0, a.npix, s))) return e;
         accumulate_kernel<<<gb, 256, 0, s>>>(a, acc64);
     }
-    if (film) resolve_kernel<<<gb, 256, 0, s>>>(a, spp ? 1.0f / (float)spp : 0.0f, film);
+    if (film && !train_only) resolve_kernel<<<gb, 256, 0, s>>>(a, 1.0f / (float)spp, film);
     return hipGetLastError();
 }
 

@@ -102,7 +102,7 @@ int main(int argc, char** argv) {
     const uint32_t H = argc > 5 ? (uint32_t)std::atoi(argv[5]) : 1080u;
     const uint32_t SPP = argc > 6 ? (uint32_t)std::atoi(argv[6]) : 64u;
     const uint32_t FRAMES = argc > 7 ? (uint32_t)std::atoi(argv[7]) : 64u;
-    const uint32_t kWarmup = 2;  // NRC frames rendered and trained on but not averaged (the cache starts untrained)
+    const uint32_t kWarmup = 2;  // first NRC frames: training only, nothing averaged (the cache starts untrained)
     if (W == 0 || H == 0 || SPP == 0 || FRAMES <= kWarmup) {
         std::fprintf(stderr, "width, height and spp must be positive and nrc_frames above %u\n", kWarmup);
         return 2;
@@ -297,10 +297,11 @@ int main(int argc, char** argv) {
     rtx_nrc nrc;
     CHECK_RTX(ctx, rtx_nrc_create(ctx, &c, &nrc));
     CHECK_HIP(hipMemset(d_accum, 0, (size_t)W * H * 24));
-    p.spp = 1;
     CHECK_HIP(hipEventRecord(e0, (hipStream_t)rtx_get_stream(ctx)));
     for (uint32_t frame = 0; frame < FRAMES; ++frame) {
         p.sample_offset = frame;
+        // the first frames only train the fresh cache (no pixel paths); the others render one sample each
+        p.spp = frame >= kWarmup ? 1 : 0;
         CHECK_RTX(ctx, rtx_cmd_render_nrc(ctx, tlas_address, &sbt, &camera, &p, nrc, 65536, d_film,
                                           frame >= kWarmup ? d_accum : nullptr));
     }

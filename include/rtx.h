@@ -512,7 +512,9 @@ int rtx_autotune_render_batch(rtx_context ctx, uint64_t tlas_address, const rtx_
                               float* out_ms_per_launch);
 
 /* rtx_autotune_trace for the path tracer: times `reps` rtx_cmd_render calls with `params` per candidate
- * configuration (film written, no FP64 accumulation), keeps the fastest.  Blocks until done. */
+ * configuration (film written, no FP64 accumulation), keeps the fastest.  Blocks until done.  Tune with the
+ * sample count of the renders to come, or with a few samples where that is many: a render of several samples
+ * traces up to 16 of them in one launch, whose best shape can differ from that of a one-sample launch. */
 int rtx_autotune_render(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
                         const rtx_render_params* params, float* film_rgba32f_device, uint32_t reps,
                         rtx_trace_config* out_cfg, float* out_ms_per_launch);
@@ -595,7 +597,10 @@ int rtx_nrc_get_status(rtx_nrc nrc, uint32_t* steps, float* last_loss);
  * that vertex on), either at full length or, with cfg->train_suffix > 0, ending in a cache query of their own
  * (self-training; those tail queries are answered by the cache as it was before this frame).  Order within the
  * frame: the training paths' samples take one Adam step per batch_size samples (interleaved batches), and then the
- * pixel paths' cache queries of every pass are answered by the updated cache.  film / accum as in rtx_cmd_render. */
+ * pixel paths' cache queries of every pass are answered by the updated cache.  film / accum as in rtx_cmd_render.
+ * params->spp == 0 makes a frame that only trains: its training paths and Adam steps exactly as in a rendered frame
+ * (the cache ends bit-identical), no pixel paths; film and accum are not touched and may be NULL (train_paths must
+ * then be > 0).  It is for the first frames of a run, whose images an accumulating caller would discard. */
 int rtx_cmd_render_nrc(rtx_context ctx, uint64_t tlas_address, const rtx_sbt* sbt, const rtx_camera* camera,
                        const rtx_render_params* params, rtx_nrc nrc, uint32_t train_paths,
                        float* film_rgba32f_device, double* accum_rgb_device);

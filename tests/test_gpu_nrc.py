@@ -146,6 +146,52 @@ def test_render_nrc_frame_answers_queries_with_the_updated_cache(ctx):
     assert not torch.equal(f1, f_before)
 
 
+@pytest.mark.parametrize("wavefront", [False, True])
+def test_train_only_frames_train_like_rendered_frames(ctx, monkeypatch, wavefront):
+    """spp = 0 makes an NRC frame that only trains: the same training paths and Adam steps as a rendered frame of
+    the same sample index, so the cache ends bit-identical, while the film it is given stays untouched.  Rendered
+    frames after the training-only ones then equal those after rendered ones."""
+    import pyrtx
+    from pyrtx import scenes as S
+    _say(f"nrc train-only frames (wavefront {wavefront})")
+    if wavefront:
+        monkeypatch.setenv("RTX_PATH_WAVEFRONT", "1")
+    else:
+        monkeypatch.delenv("RTX_PATH_WAVEFRONT", raising=False)
+    W, H = 64, 48
+    sc, lights = S.cornell_scene(3, width=W, height=H)
+    blases, tlas = ctx.upload_scene(sc)
+    lo, hi = S.scene_bounds(sc)
+    rendered = pyrtx.Nrc(ctx, lo, hi, seed=4, batch_size=4096, cache_vertex=1)
+    trained = pyrtx.Nrc(ctx, lo, hi, seed=4, batch_size=4096, cache_vertex=1)
+    try:
+        sentinel = torch.full((H, W, 4), 7.0, dtype=torch.float32, device="cuda")
+        for fr in range(2):
+            ctx.render_nrc(tlas, sc.records, sc.camera, lights, rendered, W, H, spp=1, train_paths=8192,
+                           sample_offset=fr)
+            ctx.render_nrc(tlas, sc.records, sc.camera, lights, trained, W, H, spp=0, train_paths=8192,
+                           sample_offset=fr, film=sentinel)
+        assert ctx.render_form() == (pyrtx.RTX_RENDER_FORM_WAVEFRONT if wavefront else
+                                     pyrtx.RTX_RENDER_FORM_WHOLE_PATH)
+        ctx.sync()
+        assert rendered.status()[0] > 0
+        assert rendered.status() == trained.status()
+        assert np.array_equal(rendered.params(), trained.params())
+        assert bool((sentinel == 7.0).all())
+        f_r = ctx.render_nrc(tlas, sc.records, sc.camera, lights, rendered, W, H, spp=1, train_paths=8192,
+                             sample_offset=2).clone()
+        f_t = ctx.render_nrc(tlas, sc.records, sc.camera, lights, trained, W, H, spp=1, train_paths=8192,
+                             sample_offset=2).clone()
+        ctx.sync()
+        assert torch.equal(f_r, f_t)
+        assert np.array_equal(rendered.params(), trained.params())
+        with pytest.raises(pyrtx.RtxError):
+            ctx.render_nrc(tlas, sc.records, sc.camera, lights, trained, W, H, spp=0, train_paths=0)
+    finally:
+        rendered.close()
+        trained.close()
+
+
 def test_render_nrc_graph_replays_match_direct_frames(ctx):
     """NRC frames recorded into a HIP graph (both streams, no host synchronisation inside the frame) and replayed:
     film, parameters and step count equal direct frames from an identical cache, frame after frame, with other GPU

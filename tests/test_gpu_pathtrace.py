@@ -179,6 +179,31 @@ def test_whole_path_launches_match_wavefront(ctx, monkeypatch):
     assert whole[3] == wave[3] and whole[4] == wave[4]
 
 
+@pytest.mark.parametrize("width,height,spp,offset", [
+    (96, 64, 19, 5),        # one group of 16 samples, then one of 3
+    (2048, 1100, 16, 2),    # 2,252,800 pixels: at most 2^25 path slots per launch = groups of 14 and 2
+])
+def test_sample_groups_match_wavefront(ctx, monkeypatch, width, height, spp, offset):
+    """A plain render of several samples runs them as groups of views of the camera in one whole-path launch each
+    (at most 16 samples and 2^25 paths per launch) and adds the samples to the film in sample order: the film
+    equals, bit for bit, the per-vertex wavefront's, which renders one sample after the other."""
+    import pyrtx
+    _say(f"sample groups {width}x{height} spp {spp}")
+    sc, lights = S.cornell_scene(3, width=width, height=height)
+    blases, tlas = ctx.upload_scene(sc)
+    monkeypatch.delenv("RTX_PATH_WAVEFRONT", raising=False)
+    whole = ctx.render(tlas, sc.records, sc.camera, lights, width, height, spp=spp, max_depth=6,
+                       sample_offset=offset).cpu().numpy()
+    assert ctx.render_form() == pyrtx.RTX_RENDER_FORM_WHOLE_PATH
+    monkeypatch.setenv("RTX_PATH_WAVEFRONT", "1")
+    wave = ctx.render(tlas, sc.records, sc.camera, lights, width, height, spp=spp, max_depth=6,
+                      sample_offset=offset).cpu().numpy()
+    assert ctx.render_form() == pyrtx.RTX_RENDER_FORM_WAVEFRONT
+    monkeypatch.delenv("RTX_PATH_WAVEFRONT")
+    assert whole[..., :3].mean() > 0.01
+    assert np.array_equal(whole, wave)
+
+
 def _export_obj(sc, obj_path, mat_path):
     """The scene's geometries (identity instances, records in geometry order) as one OBJ group each, and its
     camera and materials in the host program's text format."""
@@ -203,7 +228,8 @@ def test_cpp_path_tracer_host_program(ctx, tmp_path):
     """host/path_tracer.cpp, the path tracer and the radiance-cache renderer driven from C++ through the C ABI, on
     the path-tracing scene (Cornell box + displaced sphere) exported as OBJ: its plain and NRC films equal, bit for bit, what the same calls give
     through the Python binding, its frame recorded into a HIP graph by a stream capture (BLAS build, TLAS build,
-    render) replays to the direct film, and its plain film matches the FP64 oracle's estimator."""
+    render) replays to the direct film, and its plain film matches the FP64 oracle's estimator.  Its two NRC warm-up
+    frames only train (spp 0) where the Python side renders them: the accumulated films must still be equal."""
     import os
     import subprocess
     import pyrtx
