@@ -195,7 +195,7 @@ def test_train_only_frames_train_like_rendered_frames(ctx, monkeypatch, wavefron
 def test_render_nrc_graph_replays_match_direct_frames(ctx):
     """NRC frames recorded into a HIP graph (both streams, no host synchronisation inside the frame) and replayed:
     film, parameters and step count equal direct frames from an identical cache, frame after frame, with other GPU
-    work between the replays."""
+    work between the replays; the first two frames only train (spp 0, a graph of their own)."""
     import pyrtx
     from pyrtx import scenes as S
     _say("nrc frame graph replays")
@@ -207,13 +207,14 @@ def test_render_nrc_graph_replays_match_direct_frames(ctx):
     lo, hi = S.scene_bounds(sc)
     direct = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=2048, cache_vertex=1)
     graph = pyrtx.Nrc(ctx, lo, hi, seed=1, batch_size=2048, cache_vertex=1)
-    fd = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
-    fg = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    fd = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
+    fg = torch.zeros((H, W, 4), dtype=torch.float32, device="cuda")
     try:
-        for fr in range(4):
-            ctx.render_nrc(tlas, rec, sc.camera, lt, direct, W, H, spp=2, train_paths=4096, sample_offset=fr,
+        for fr in range(6):
+            spp = 0 if fr < 2 else 2
+            ctx.render_nrc(tlas, rec, sc.camera, lt, direct, W, H, spp=spp, train_paths=4096, sample_offset=fr,
                            film=fd)
-            ctx.render_nrc(tlas, rec, sc.camera, lt, graph, W, H, spp=2, train_paths=4096, sample_offset=fr,
+            ctx.render_nrc(tlas, rec, sc.camera, lt, graph, W, H, spp=spp, train_paths=4096, sample_offset=fr,
                            film=fg, graph=True)
             ctx.sync()
             assert torch.equal(fd, fg), fr
@@ -223,8 +224,10 @@ def test_render_nrc_graph_replays_match_direct_frames(ctx):
             ctx.render(tlas, rec, sc.camera, lt, W, H, spp=1, sample_offset=100 + fr)
             ctx.sync()
             assert torch.isfinite(x).all()
+            if fr < 2:
+                assert not bool(fg.any()), fr  # training-only frames leave the film alone
         steps, _ = graph.status()
-        assert steps >= 4
+        assert steps >= 6
         assert float(fg[..., :3].mean()) > 0.01
     finally:
         direct.close()
