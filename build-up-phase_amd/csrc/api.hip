@@ -78,12 +78,12 @@ constexpr uint32_t kMaxPrims = (1u << 30) - 1u;
 // launch has kSideBlocks 256-thread blocks per CU: the blocks beyond the reserve start as pixel-path blocks retire
 // (the traversal hands out paths from an atomic counter, so late blocks only take what is left).
 constexpr uint32_t kSideReserve = 1u;
+constexpr uint32_t kSideBlocks = 2u;
 // Samples per whole-path launch of a plain render: up to 16, and at most 2^25 paths per launch (the per-path state
 // takes 84 B per pixel and sample: 2.8 GB at 1920x1080 x 16).  At 1080p, 64 spp: 4 / 8 / 16 samples per launch
 // 303 / 289 / 282 ms against 324 ms for one.
 constexpr uint32_t kSampleViews = 16u;
 constexpr uint64_t kSampleViewPaths = 1ull << 25;
-constexpr uint32_t kSideBlocks = 2u;
 
 // splitmix64 (Steele, Lea & Flood 2014): deterministic weight initialisation
 uint64_t splitmix64(uint64_t& s) {
